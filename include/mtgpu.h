@@ -1,0 +1,139 @@
+/*
+ * mtgpu -- MI355X (gfx950) intra rate-distortion candidate evaluation for maroontree.
+ *
+ * C ABI of the HIP library libmtgpu.so.  Plain pointers and sizes only; every pointer
+ * named *_d is device memory, `stream` is a hipStream_t (NULL = the default stream).
+ *
+ * Conventions of the reference encoder (src/quant.rs:95-99, src/rate.rs:93-95):
+ *   residual / reconstruction blocks are row-major [y*W + x];
+ *   levels and targets are x-major ("transposed") [x*H + y].
+ * Transform sizes and types use the AV1 TxSize / TxType numbering (MT_TX_*, MT_TXTP_*).
+ *
+ * Return codes: 0 ok, negative MT_ERR_* on failure (no call aborts the process);
+ * mt_last_error() describes the last failure of the calling thread.
+ */
+#ifndef MTGPU_H
+#define MTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MT_OK = 0,
+    MT_ERR_ARG = -1,          /* bad argument (size, pointer, count) */
+    MT_ERR_UNSUPPORTED = -2,  /* (size, type) has no reference forward transform */
+    MT_ERR_DEVICE = -3,       /* HIP runtime error */
+    MT_ERR_NOT_INIT = -4,     /* mt_init not called on this device */
+};
+
+/* AV1 TxSize */
+enum {
+    MT_TX_4X4 = 0, MT_TX_8X8 = 1, MT_TX_16X16 = 2, MT_TX_32X32 = 3, MT_TX_64X64 = 4,
+    MT_TX_4X8 = 5, MT_TX_8X4 = 6, MT_TX_8X16 = 7, MT_TX_16X8 = 8, MT_TX_16X32 = 9,
+    MT_TX_32X16 = 10, MT_TX_32X64 = 11, MT_TX_64X32 = 12, MT_TX_4X16 = 13, MT_TX_16X4 = 14,
+    MT_TX_8X32 = 15, MT_TX_32X8 = 16, MT_TX_16X64 = 17, MT_TX_64X16 = 18
+};
+/* AV1 TxType (the ones the reference encoder evaluates) */
+enum {
+    MT_TXTP_DCT_DCT = 0, MT_TXTP_ADST_DCT = 1, MT_TXTP_DCT_ADST = 2, MT_TXTP_ADST_ADST = 3,
+    MT_TXTP_IDTX = 9, MT_TXTP_V_DCT = 10, MT_TXTP_H_DCT = 11
+};
+
+/* Quantizer state, field for field the reference's `Quant` (src/quant.rs:327-345),
+ * plus the bit depth it was built for. */
+typedef struct {
+    int32_t dc, ac;                          /* dequant steps (dc_q / ac_q) */
+    int32_t q_mult_dc, q_mult_ac;            /* round(65536 / step) */
+    int32_t rmin, rmax, cmin, cmax, cf_max;  /* inverse-transform clips (itx_clips) */
+    int32_t qm_level;                        /* 15 = flat */
+    int32_t qm_chroma;                       /* 0 luma, 1 chroma */
+    int32_t qidx;                            /* AC-side qindex */
+    int32_t bd;                              /* 8, 10 or 12 */
+} mt_quant;
+
+/* One block of the batched transform chain: 8 bytes in HBM. */
+typedef struct {
+    uint32_t offset;   /* element offset of the block's int16 residual in the pool */
+    uint8_t tx_size;   /* AV1 TxSize */
+    uint8_t tx_type;   /* AV1 TxType */
+    uint8_t qidx;      /* base_q_idx of Quant::new_with_qm */
+    uint8_t bd_qm;     /* bits 0-3: QM level (15 = flat); bits 4-5: 0 8-bit, 1 10-bit, 2 12-bit */
+} mt_txblk;
+
+/* Per-block result of the chain: 16 bytes in HBM. */
+typedef struct {
+    int64_t sse;       /* sse_recon(P, inverse, clamp(P + residual)), P = 1 << (bd - 1); -1 unsupported */
+    uint32_t nnz;      /* nonzero levels */
+    uint32_t hash;     /* order-independent checksum of levels, targets and reconstruction */
+} mt_txres;
+
+/* ---- setup ------------------------------------------------------------------------ */
+/* Uploads the quantizer tables and the 64-point basis to `device` and makes it current.
+ * Replaces the reference's per-encode dispatch construction (EncodingContext::new,
+ * src/encoding_context.rs:99). */
+int mt_init(int device);
+const char* mt_last_error(void);
+const char* mt_version(void);
+
+/* ---- quantizer state (host) --------------------------------------------------------- */
+/* Quant::new_with_qm (src/quant.rs:354) */
+int mt_quant_new(int base_q_idx, int bd, int qm_level, mt_quant* out);
+/* Quant::new_chroma_with_delta_qm (src/quant.rs:394) */
+int mt_quant_new_chroma(int base_q_idx, int dc_delta, int ac_delta, int bd, int qm_level,
+                        mt_quant* out);
+/* qm_level_law (src/quant.rs:137) */
+int mt_qm_level_law(int base_q_idx, int sub);
+/* width/height of an AV1 TxSize; coded coefficient count (64-point sizes code 32) */
+int mt_tx_width(int tx_size);
+int mt_tx_height(int tx_size);
+int mt_tx_ncoef(int tx_size);
+/* 1 when the reference has a forward transform for (tx_size, tx_type) */
+int mt_tx_supported(int tx_size, int tx_type);
+
+/* ---- per-kernel entry points (the dispatch tables, batched) ------------------------ */
+/* Forward transform + quantization (+QM) of n blocks of one (size, type):
+ * DctDispatch::*_t (src/dct.rs:2624-2950).  resid_d: n * W*H int32 row-major;
+ * quants_d: n mt_quant; levels_d / targets_d: n * ncoef x-major. */
+int mt_fwd_txfm(int tx_size, int tx_type, int n, const int32_t* resid_d,
+                const mt_quant* quants_d, int32_t* levels_d, float* targets_d, void* stream);
+/* Dequantization + inverse transform of n blocks of one (size, type):
+ * IdctDispatch::* (src/idct.rs:91-728) and idct_dequant_64x64 (src/idct.rs:2677).
+ * levels_d: n * ncoef x-major; out_d: n * W*H int32 row-major. */
+int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* levels_d,
+                const mt_quant* quants_d, int32_t* out_d, void* stream);
+
+/* ---- batched candidate chain ------------------------------------------------------- */
+/* fwd -> quant(+QM) -> dequant -> inverse -> SSE for n blocks of mixed size/type/q/bd.
+ * pool_d holds the int16 residuals.  res_d: n results.  levels_d / targets_d / recon_d
+ * are optional (NULL) full outputs laid out like pool_d: block i's levels, targets
+ * (ncoef, x-major) and reconstructed residual (W*H, int32) start at blocks[i].offset.
+ * Blocks of unsupported (size, type) get sse = -1.  Uses a library-owned workspace on
+ * the device (allocated on first use, grown as needed). */
+int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
+                int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream);
+
+/* Per-kernel timing of mt_tx_chain (no counterpart in the reference; measurement hook).
+ * Between begin and end every chain launch is bracketed by HIP events on its stream;
+ * end synchronizes them and returns, per slot, the summed milliseconds and launch count.
+ * Slot 0 = bucketing (3 small kernels), slots 1..13 = the size kernels in the order
+ * 4x4, 4x8, 8x4, 4x16, 16x4, 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 64x64. */
+int mt_profile_begin(void);
+int mt_profile_end(double* kernel_ms, int* launches, int max_slots);
+
+/* Synthetic input of the batched transform microbench (host memory):
+ * n blocks, TX size uniform over the 13 reference sizes (or all 19 when all_sizes),
+ * type uniform over the size's reference types, residual discrete Laplace(12) clipped to
+ * +-255 (8-bit) / +-1023 (10-bit, half the blocks), qindex uniform 0-255, QM level flat or
+ * qm_level_law(q, 0).  blocks: n entries; pool: pool_elems int16 (mt_gen_pool_size). */
+uint64_t mt_gen_pool_size(uint64_t seed, int n, int all_sizes);
+int mt_gen_tx_blocks(uint64_t seed, int n, int all_sizes, mt_txblk* blocks, int16_t* pool,
+                     uint64_t pool_elems);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTGPU_H */

@@ -1,0 +1,12 @@
+"""maroontree_amd: MI355X (gfx950) intra rate-distortion candidate evaluation for maroontree.
+
+The compute path is the HIP library ``libmtgpu.so`` (C ABI in ``include/mtgpu.h``); this
+package loads it with ctypes and passes device pointers of torch tensors through.  There is
+no CPU fallback: every entry point raises when the library or a GPU is missing.
+"""
+from ._lib import (  # noqa: F401
+    MtQuant, MtTxBlk, MtTxRes, MtError, lib, lib_path, init, quant_new, quant_new_chroma,
+    qm_level_law, tx_width, tx_height, tx_ncoef, tx_supported, fwd_txfm, inv_txfm, tx_chain,
+    gen_tx_blocks, profile_begin, profile_end, PROFILE_SLOTS, TX_SIZES_REF, TX_SIZES_ALL, TX_TYPES_REF, TX_NAMES, TXTP_NAMES,
+)
+from .dispatch import DctDispatch, IdctDispatch  # noqa: F401

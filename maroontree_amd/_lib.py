@@ -1,0 +1,214 @@
+"""ctypes binding of libmtgpu.so (include/mtgpu.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(PKG, "libmtgpu.so")
+
+# AV1 TxSize numbering; the 13 sizes the reference has a forward transform for.
+TX_NAMES = ["4x4", "8x8", "16x16", "32x32", "64x64", "4x8", "8x4", "8x16", "16x8", "16x32",
+            "32x16", "32x64", "64x32", "4x16", "16x4", "8x32", "32x8", "16x64", "64x16"]
+TX_SIZES_REF = [0, 5, 6, 13, 14, 1, 7, 8, 2, 9, 10, 3, 4]
+TX_SIZES_ALL = list(range(19))
+TXTP_NAMES = {0: "DCT_DCT", 1: "ADST_DCT", 2: "DCT_ADST", 3: "ADST_ADST", 9: "IDTX",
+              10: "V_DCT", 11: "H_DCT"}
+TX_TYPES_REF = [0, 1, 2, 3, 9, 10, 11]
+
+
+class MtError(RuntimeError):
+    pass
+
+
+class MtQuant(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "dc", "ac", "q_mult_dc", "q_mult_ac", "rmin", "rmax", "cmin", "cmax", "cf_max",
+        "qm_level", "qm_chroma", "qidx", "bd")]
+
+
+class MtTxBlk(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint32), ("tx_size", ctypes.c_uint8),
+                ("tx_type", ctypes.c_uint8), ("qidx", ctypes.c_uint8), ("bd_qm", ctypes.c_uint8)]
+
+
+class MtTxRes(ctypes.Structure):
+    _fields_ = [("sse", ctypes.c_int64), ("nnz", ctypes.c_uint32), ("hash", ctypes.c_uint32)]
+
+
+BLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("tx_type", "u1"), ("qidx", "u1"),
+                      ("bd_qm", "u1")])
+RES_DTYPE = np.dtype([("sse", "<i8"), ("nnz", "<u4"), ("hash", "<u4")])
+QUANT_DTYPE = np.dtype([(n, "<i4") for n, _ in MtQuant._fields_])
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises MtError when libmtgpu.so has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise MtError(f"libmtgpu.so not built ({lib_path}); run __graft_entry__.build()")
+        L = ctypes.CDLL(lib_path)
+        P, vp, i32, u64 = ctypes.POINTER, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+        L.mt_init.argtypes = [i32]
+        L.mt_last_error.restype = ctypes.c_char_p
+        L.mt_version.restype = ctypes.c_char_p
+        L.mt_quant_new.argtypes = [i32, i32, i32, P(MtQuant)]
+        L.mt_quant_new_chroma.argtypes = [i32, i32, i32, i32, i32, P(MtQuant)]
+        L.mt_qm_level_law.argtypes = [i32, i32]
+        L.mt_tx_width.argtypes = L.mt_tx_height.argtypes = L.mt_tx_ncoef.argtypes = [i32]
+        L.mt_tx_supported.argtypes = [i32, i32]
+        L.mt_fwd_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp]
+        L.mt_inv_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp]
+        L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+        L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
+        L.mt_gen_pool_size.argtypes = [u64, i32, i32]
+        L.mt_gen_pool_size.restype = u64
+        L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MtError(f"mtgpu error {rc}: {lib().mt_last_error().decode()}")
+
+
+_inited = set()
+
+
+def init(device=0):
+    """mt_init on `device` (idempotent); requires a GPU."""
+    if device not in _inited:
+        _check(lib().mt_init(device))
+        _inited.add(device)
+
+
+def quant_new(qidx, bd=8, qm_level=15):
+    q = MtQuant()
+    _check(lib().mt_quant_new(qidx, bd, qm_level, ctypes.byref(q)))
+    return q
+
+
+def quant_new_chroma(qidx, dc_delta, ac_delta, bd, qm_level):
+    q = MtQuant()
+    _check(lib().mt_quant_new_chroma(qidx, dc_delta, ac_delta, bd, qm_level, ctypes.byref(q)))
+    return q
+
+
+def quant_to_np(qs):
+    """List of MtQuant -> structured numpy array (QUANT_DTYPE)."""
+    arr = np.zeros(len(qs), QUANT_DTYPE)
+    for i, q in enumerate(qs):
+        arr[i] = tuple(getattr(q, n) for n, _ in MtQuant._fields_)
+    return arr
+
+
+def qm_level_law(qidx, sub=0):
+    return lib().mt_qm_level_law(qidx, sub)
+
+
+def tx_width(s):
+    return lib().mt_tx_width(s)
+
+
+def tx_height(s):
+    return lib().mt_tx_height(s)
+
+
+def tx_ncoef(s):
+    return lib().mt_tx_ncoef(s)
+
+
+def tx_supported(s, t):
+    return bool(lib().mt_tx_supported(s, t))
+
+
+def _dev_ptr(t):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise MtError("expected a torch tensor on the GPU")
+    if not t.is_contiguous():
+        raise MtError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _quants_tensor(quants, device):
+    import torch
+    arr = quants if isinstance(quants, np.ndarray) else quant_to_np(quants)
+    return torch.from_numpy(arr.view(np.int32).reshape(-1).copy()).to(device)
+
+
+def fwd_txfm(tx_size, tx_type, resid, quants, stream=None):
+    """Forward transform + quant of n blocks: resid int32 (n, W*H) cuda tensor, quants a list
+    of MtQuant (or QUANT_DTYPE array).  Returns (levels int32 (n, ncoef), targets f32)."""
+    import torch
+    n = resid.shape[0]
+    nc = tx_ncoef(tx_size)
+    dev = resid.device
+    init(dev.index or 0)
+    qt = _quants_tensor(quants, dev)
+    lv = torch.empty((n, nc), dtype=torch.int32, device=dev)
+    tg = torch.empty((n, nc), dtype=torch.float32, device=dev)
+    _check(lib().mt_fwd_txfm(tx_size, tx_type, n, _dev_ptr(resid.contiguous()), _dev_ptr(qt),
+                             _dev_ptr(lv), _dev_ptr(tg), _stream_ptr(stream)))
+    return lv, tg
+
+
+def inv_txfm(tx_size, tx_type, levels, quants, stream=None):
+    """Dequant + inverse of n blocks: levels int32 (n, ncoef) -> residual int32 (n, W*H)."""
+    import torch
+    n = levels.shape[0]
+    dev = levels.device
+    init(dev.index or 0)
+    qt = _quants_tensor(quants, dev)
+    out = torch.empty((n, tx_width(tx_size) * tx_height(tx_size)), dtype=torch.int32, device=dev)
+    _check(lib().mt_inv_txfm(tx_size, tx_type, n, _dev_ptr(levels.contiguous()), _dev_ptr(qt),
+                             _dev_ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=None):
+    """Run the batched chain.  blocks: uint8 cuda tensor of n*8 bytes (BLK_DTYPE records),
+    pool: int16 cuda tensor, res: uint8 cuda tensor of n*16 bytes (RES_DTYPE)."""
+    n = blocks.numel() // BLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    opt = lambda t: _dev_ptr(t) if t is not None else None  # noqa: E731
+    _check(lib().mt_tx_chain(_dev_ptr(blocks), n, _dev_ptr(pool), _dev_ptr(res), opt(levels),
+                             opt(targets), opt(recon), _stream_ptr(stream)))
+
+
+PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",
+                 "16x32", "32x16", "32x32", "64x64"]
+
+
+def profile_begin():
+    _check(lib().mt_profile_begin())
+
+
+def profile_end():
+    """{slot name: (total ms, launches)} of the chain launches since profile_begin()."""
+    n = len(PROFILE_SLOTS)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int * n)()
+    _check(lib().mt_profile_end(ms, cnt, n))
+    return {PROFILE_SLOTS[i]: (ms[i], cnt[i]) for i in range(n)}
+
+
+def gen_tx_blocks(seed, n, all_sizes=False):
+    """Synthetic microbench input on the host: (blocks structured array, int16 pool)."""
+    L = lib()
+    total = L.mt_gen_pool_size(seed, n, int(all_sizes))
+    blocks = np.zeros(n, BLK_DTYPE)
+    pool = np.zeros(max(total, 1), np.int16)
+    _check(L.mt_gen_tx_blocks(seed, n, int(all_sizes), blocks.ctypes.data_as(ctypes.c_void_p),
+                              pool.ctypes.data_as(ctypes.c_void_p), total))
+    return blocks, pool

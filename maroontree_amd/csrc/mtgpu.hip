@@ -1,0 +1,1353 @@
+// mtgpu: batched intra candidate transform chain on MI355X (gfx950).
+//
+// Per (TX size) one kernel template walks tiles of B same-type blocks through LDS:
+//   load int16 residual -> forward pass 1 -> forward pass 2 + quant (+QM) + dequant
+//   -> inverse row pass -> inverse column pass -> SSE / nnz / checksum.
+// Each 1-D transform is one lane's register array; LDS (rows padded to W+1 words) is the
+// transpose between passes.  Integer VALU only: the stage-wise rounding of the reference
+// encoder (src/dct.rs, src/idct.rs) must stay bit-exact, so nothing here uses MFMA.
+// 64x64 has its own kernel: its forward transform is the reference's f64 separable basis.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/mtgpu.h"
+#include "av1_tables.h"
+#include "mt_tx1d.h"
+
+using namespace mt;
+
+// ------------------------------------------------------------------------- error state
+static thread_local std::string g_err;
+static int set_err(int code, const char* what) {
+    g_err = what;
+    return code;
+}
+#define HIP_TRY(expr)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            g_err = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+            return MT_ERR_DEVICE;                                                   \
+        }                                                                           \
+    } while (0)
+
+// ------------------------------------------------------------------------- device tables
+__constant__ uint16_t c_dcq[3][256];
+__constant__ uint16_t c_acq[3][256];
+__constant__ int32_t c_fadst4[16];
+__constant__ int32_t c_fadst8[64];
+__constant__ int32_t c_fadst16[256];
+__device__ uint8_t g_iqm[MT_QM_LEVELS][2][MT_QM_USED_SIZE];
+__device__ double g_dct64_basis[32 * 64];
+
+enum TxKind { KD = 0, KA = 1, KI = 2 };  // DCT, ADST, identity
+
+// Dense class index of the 13 reference sizes (-1: none) and of the 7 reference types.
+__host__ __device__ inline int size_slot(int s) {
+    switch (s) {
+        case MT_TX_4X4: return 0;
+        case MT_TX_4X8: return 1;
+        case MT_TX_8X4: return 2;
+        case MT_TX_4X16: return 3;
+        case MT_TX_16X4: return 4;
+        case MT_TX_8X8: return 5;
+        case MT_TX_8X16: return 6;
+        case MT_TX_16X8: return 7;
+        case MT_TX_16X16: return 8;
+        case MT_TX_16X32: return 9;
+        case MT_TX_32X16: return 10;
+        case MT_TX_32X32: return 11;
+        case MT_TX_64X64: return 12;
+        default: return -1;
+    }
+}
+static const int kSlotSize[13] = {MT_TX_4X4,   MT_TX_4X8,   MT_TX_8X4,   MT_TX_4X16, MT_TX_16X4,
+                                  MT_TX_8X8,   MT_TX_8X16,  MT_TX_16X8,  MT_TX_16X16,
+                                  MT_TX_16X32, MT_TX_32X16, MT_TX_32X32, MT_TX_64X64};
+__host__ __device__ inline int type_slot(int t) {
+    switch (t) {
+        case MT_TXTP_DCT_DCT: return 0;
+        case MT_TXTP_ADST_DCT: return 1;
+        case MT_TXTP_DCT_ADST: return 2;
+        case MT_TXTP_ADST_ADST: return 3;
+        case MT_TXTP_IDTX: return 4;
+        case MT_TXTP_V_DCT: return 5;
+        case MT_TXTP_H_DCT: return 6;
+        default: return -1;
+    }
+}
+__host__ __device__ inline int type_of_slot(int t) {
+    return t == 0 ? MT_TXTP_DCT_DCT : t == 1 ? MT_TXTP_ADST_DCT : t == 2 ? MT_TXTP_DCT_ADST
+         : t == 3 ? MT_TXTP_ADST_ADST : t == 4 ? MT_TXTP_IDTX : t == 5 ? MT_TXTP_V_DCT
+         : MT_TXTP_H_DCT;
+}
+static const int kSlotType[7] = {MT_TXTP_DCT_DCT, MT_TXTP_ADST_DCT, MT_TXTP_DCT_ADST,
+                                 MT_TXTP_ADST_ADST, MT_TXTP_IDTX, MT_TXTP_V_DCT, MT_TXTP_H_DCT};
+
+// The reference's forward coverage (src/dct.rs:2416-2470, :2624-2950).
+__host__ __device__ inline bool tx_supported(int s, int t) {
+    const bool hyb = t == MT_TXTP_DCT_DCT || t == MT_TXTP_ADST_ADST || t == MT_TXTP_ADST_DCT ||
+                     t == MT_TXTP_DCT_ADST;
+    const bool all7 = hyb || t == MT_TXTP_V_DCT || t == MT_TXTP_H_DCT || t == MT_TXTP_IDTX;
+    switch (s) {
+        case MT_TX_4X4: case MT_TX_8X8: case MT_TX_8X16: case MT_TX_16X8: return all7;
+        case MT_TX_4X8: case MT_TX_8X4: return hyb;
+        case MT_TX_16X16: return hyb || t == MT_TXTP_IDTX;
+        case MT_TX_4X16: case MT_TX_16X4: case MT_TX_16X32: case MT_TX_32X16:
+        case MT_TX_32X32: case MT_TX_64X64:
+            return t == MT_TXTP_DCT_DCT;
+        default: return false;
+    }
+}
+
+static const int kTxW[19] = {4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 4, 16, 8, 32, 16, 64};
+static const int kTxH[19] = {4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 4, 32, 8, 64, 16};
+
+// Offset of a size's inverse QM row inside one 3344-entry level table (src/quant.rs:59).
+__host__ __device__ constexpr int qm_offset(int w, int h) {
+    return (w == 4 && h == 4) ? 0 : (w == 8 && h == 8) ? 16 : (w == 16 && h == 16) ? 80
+         : (w == 32 && h == 32) ? 336 : (w == 4 && h == 8) ? 1360 : (w == 8 && h == 4) ? 1392
+         : (w == 8 && h == 16) ? 1424 : (w == 16 && h == 8) ? 1552 : (w == 16 && h == 32) ? 1680
+         : (w == 32 && h == 16) ? 2192 : (w == 4 && h == 16) ? 2704 : (w == 16 && h == 4) ? 2768
+         : (w == 8 && h == 32) ? 2832 : (w == 32 && h == 8) ? 3088 : -1;
+}
+
+// ------------------------------------------------------------------------- per-size rules
+// Everything that differs between the reference's per-size forward/inverse functions,
+// as compile-time properties of (W, H).
+template <int W, int H> struct TxRules {
+    static constexpr int N = W * H;
+    // forward: multiplier ratio 8/sqrt(W*H) in Q16 for the 2-D types (src/dct.rs:3167-3169)
+    static constexpr int32_t ratio = (W == 4 && H == 4) ? 131072
+                                   : ((W == 4 && H == 8) || (W == 8 && H == 4)) ? 92682
+                                   : ((W == 16 && H == 32) || (W == 32 && H == 16)) ? 23170 : 0;
+    // forward: B = 6 headroom shift of the 32-point sizes (src/dct.rs:705, :3134, :3342)
+    static constexpr int pre_shift = (W == 32 || H == 32) ? 6 : 0;
+    // forward: row transform first (dct8x4 src/dct.rs:3253, dct32x16 src/dct.rs:3342)
+    static constexpr bool row_first = (W == 8 && H == 4) || (W == 32 && H == 16);
+    // forward post-scale: 0 none, 1 x 1/sqrt2, 2 x 1/2, 3 (x + 32) >> 6, 4 the 32x32 i64 form
+    static constexpr int post = ((W == 8 && H == 16) || (W == 16 && H == 8)) ? 1
+                              : (W == 16 && H == 16) ? 2
+                              : ((W == 16 && H == 32) || (W == 32 && H == 16)) ? 3
+                              : (W == 32 && H == 32) ? 4 : 0;
+    // inverse: rect2 prescale for 2:1 shapes, inter-pass shift, dequant shift
+    static constexpr bool rect2 = (W == 2 * H) || (H == 2 * W);
+    static constexpr int mid = (N <= 32) ? 0 : ((W == 16 && H == 16) || (W == 32 && H == 32)) ? 2 : 1;
+    static constexpr int dq_shift = (N >= 512) ? 1 : 0;
+    static constexpr int qmoff = qm_offset(W, H);
+    // tile geometry: B blocks per 256-thread workgroup
+    static constexpr int MINWH = W < H ? W : H;
+    static constexpr int B0 = 256 / MINWH;
+    static constexpr int B1 = 4096 / N;
+    static constexpr int B = B0 > B1 ? B0 : B1;
+    static constexpr int RS = W + 1;      // padded LDS row stride
+    static constexpr int BS = H * RS;     // LDS words per block
+};
+
+// ------------------------------------------------------------------------- per-block params
+struct BlkParam {
+    int32_t dc, ac;          // dequant steps
+    int32_t qdc, qac;        // q multipliers, un-ratioed
+    int32_t rmin, rmax, cmin, cmax, cf_max;
+    int32_t qm_level, qm_chroma, bd;
+};
+
+__device__ inline void param_from_desc(const mt_txblk& d, BlkParam& p) {
+    const int qm = d.bd_qm & 15;
+    const int bdi = (d.bd_qm >> 4) & 3;
+    const int bd = bdi == 0 ? 8 : bdi == 1 ? 10 : 12;
+    const int q = d.qidx;
+    const int dci = q - 8 < 0 ? 0 : q - 8;  // luma DC delta -8 (src/quant.rs:350)
+    p.dc = c_dcq[bdi][dci];
+    p.ac = c_acq[bdi][q];
+    // round(65536 / step) of the reference's f64 (src/quant.rs:362): no step divides
+    // 131072 oddly, so the integer round-half-up below is the same value.
+    p.qdc = (131072 + p.dc) / (2 * p.dc);
+    p.qac = (131072 + p.ac) / (2 * p.ac);
+    if (bd <= 8) {
+        p.rmin = p.cmin = -32768;
+        p.rmax = p.cmax = 32767;
+        p.cf_max = 32767;
+    } else {
+        const int32_t rmax = (1 << (bd + 7)) - 1, cmax = (1 << (bd + 5)) - 1;
+        p.rmin = ~rmax;
+        p.rmax = rmax;
+        p.cmin = ~cmax;
+        p.cmax = cmax;
+        p.cf_max = rmax;
+    }
+    p.qm_level = qm;
+    p.qm_chroma = 0;
+    p.bd = bd;
+}
+
+__device__ inline void param_from_quant(const mt_quant& q, BlkParam& p) {
+    p.dc = q.dc;
+    p.ac = q.ac;
+    p.qdc = q.q_mult_dc;
+    p.qac = q.q_mult_ac;
+    p.rmin = q.rmin;
+    p.rmax = q.rmax;
+    p.cmin = q.cmin;
+    p.cmax = q.cmax;
+    p.cf_max = q.cf_max;
+    p.qm_level = q.qm_level;
+    p.qm_chroma = q.qm_chroma;
+    p.bd = q.bd;
+}
+
+// ------------------------------------------------------------------------- scalar helpers
+// quant_q16 (src/dct.rs:70): half-step dead zone, round to nearest.
+__device__ inline int32_t quant_q16(int32_t c, int32_t m) {
+    const int64_t prod = (int64_t)c * (int64_t)m;
+    const uint64_t mag = prod < 0 ? (uint64_t)0 - (uint64_t)prod : (uint64_t)prod;
+    if (mag < 32768) return 0;
+    const int32_t l = (int32_t)((mag + 32768) >> 16);
+    return prod >= 0 ? l : -l;
+}
+// quant_fullstep levels (src/dct.rs:373): full-step dead zone.
+__device__ inline int32_t quant_full(int32_t c, int32_t m) {
+    const int64_t prod = (int64_t)c * (int64_t)m;
+    const uint64_t mag = prod < 0 ? (uint64_t)0 - (uint64_t)prod : (uint64_t)prod;
+    if (mag < 65536) return 0;
+    const int32_t l = (int32_t)((mag + 32768) >> 16);
+    return prod >= 0 ? l : -l;
+}
+// Target of a Q16 level: f32(coeff) * f32(m) * 2^-16, each product rounded (src/dct.rs:1204;
+// the x86 selection computes the 1-D family the same way, src/avx/dct.rs:184).
+__device__ inline float q16_target(int32_t c, int32_t m) {
+    return __fmul_rn(__fmul_rn((float)c, (float)m), 1.0f / 65536.0f);
+}
+// apply_qmatrix on one coefficient (src/avx/qmatrix.rs:34-66): scale the target by the
+// forward weight (1024 + iwt/2) / iwt / 32, then round half away from zero.
+__device__ inline int32_t qm_apply(float& t, int iwt) {
+    const int w = (1024 + iwt / 2) / iwt;
+    t = __fmul_rn(t, (float)w * (1.0f / 32.0f));
+    const float tr = truncf(t);
+    const float fr = fabsf(__fsub_rn(t, tr));
+    const float step = fr >= 0.5f ? copysignf(1.0f, t) : 0.0f;
+    return (int32_t)__fadd_rn(tr, step);
+}
+// Dequantize one level (src/idct.rs:1576-1590): 24-bit mask, dq shift, cf_max clamp.
+__device__ inline int32_t dequant(int32_t lvl, int32_t step, int dq_shift, int32_t cf_max) {
+    if (lvl == 0) return 0;
+    const uint32_t a = lvl < 0 ? 0u - (uint32_t)lvl : (uint32_t)lvl;
+    int32_t mag = (int32_t)(((a * (uint32_t)step) & 0xffffffu) >> dq_shift);
+    const int32_t lim = cf_max + (lvl < 0 ? 1 : 0);
+    mag = mag < lim ? mag : lim;
+    return lvl < 0 ? -mag : mag;
+}
+__device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+__device__ inline uint32_t coef_hash(int rc, int32_t lvl, float t) {
+    return mix32((uint32_t)rc * 0x9E3779B1u ^ (uint32_t)lvl) +
+           mix32(((uint32_t)rc * 0x85EBCA77u + 0x165667B1u) ^ __float_as_uint(t));
+}
+__device__ inline uint32_t pix_hash(int j, int32_t r) {
+    return mix32(((uint32_t)j * 0xC2B2AE3Du + 0x27D4EB2Fu) ^ (uint32_t)r);
+}
+
+// One forward 1-D transform of length L on a register array.
+template <int L> __device__ inline void fwd_line(bool adst, int32_t* v) {
+    if (adst) {
+        if constexpr (L == 4) fadst<4>(c_fadst4, v);
+        else if constexpr (L == 8) fadst<8>(c_fadst8, v);
+        else if constexpr (L == 16) fadst<16>(c_fadst16, v);
+    } else {
+        FDct<L>::run(v);
+    }
+}
+
+// One inverse 1-D transform of length L on a register array.
+template <int L> __device__ inline void inv_line(int kind, int32_t* v, Clip c) {
+    if (kind == KI) {
+#pragma unroll
+        for (int i = 0; i < L; i++) v[i] = L == 4 ? iid4(v[i]) : L == 8 ? iid8(v[i]) : iid16(v[i]);
+    } else if (kind == KA) {
+        if constexpr (L == 4) iadst4(v);
+        else if constexpr (L == 8) iadst8(v, c);
+        else if constexpr (L == 16) iadst16(v, c);
+    } else {
+        if constexpr (L == 4) idct4<1>(v, c);
+        else if constexpr (L == 8) idct8<1>(v, c);
+        else if constexpr (L == 16) idct16<1>(v, c);
+        else if constexpr (L == 32) idct32<1>(v, c);
+    }
+}
+
+// ------------------------------------------------------------------------- tile kernel
+enum { MODE_CHAIN = 0, MODE_FWD = 1, MODE_INV = 2 };
+
+struct KArgs {
+    // chain mode
+    const mt_txblk* blocks;
+    const uint32_t* order;       // bucketed block indices
+    const uint32_t* cls_start;   // [13*7] first index of each class in `order`
+    const uint32_t* cls_count;   // [13*7]
+    const int16_t* pool16;
+    mt_txres* res;
+    // parity modes (contiguous per block) and optional chain outputs (pool layout)
+    const int32_t* in32;
+    const mt_quant* quants;
+    int32_t* levels;
+    float* targets;
+    int32_t* recon;
+    int n;      // parity modes: block count
+    int type;   // parity modes: tx type
+};
+
+template <int W, int H, int MODE>
+__global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx) {
+    using R = TxRules<W, H>;
+    constexpr int N = R::N, B = R::B, RS = R::RS, BS = R::BS;
+    constexpr int NT = 256;
+
+    __shared__ int32_t work[B * BS];
+    __shared__ int16_t in16[MODE == MODE_CHAIN ? B * N : 1];
+    __shared__ BlkParam prm[B];
+    __shared__ uint32_t gidx[B];      // global block index (chain) or block number (parity)
+    __shared__ uint32_t goff[B];      // element offset of the block in the pool / arrays
+    __shared__ unsigned long long s_sse[B];
+    __shared__ uint32_t s_nnz[B], s_hash[B];
+    __shared__ int s_nb, s_type;
+
+    const int tid = threadIdx.x;
+
+    // virtual tile space: chain mode walks the 7 type classes of this size
+    int total_tiles = 0;
+    uint32_t cnt[7], st[7];
+    if (MODE == MODE_CHAIN) {
+#pragma unroll
+        for (int t = 0; t < 7; t++) {
+            cnt[t] = a.cls_count[size_slot_idx * 7 + t];
+            st[t] = a.cls_start[size_slot_idx * 7 + t];
+            total_tiles += (int)((cnt[t] + B - 1) / B);
+        }
+    } else {
+        total_tiles = (a.n + B - 1) / B;
+    }
+
+    for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+        // ---- tile setup: which blocks, their parameters
+        if (tid == 0) {
+            if (MODE == MODE_CHAIN) {
+                int rem = tile, t = 0;
+                for (; t < 7; t++) {
+                    const int nt = (int)((cnt[t] + B - 1) / B);
+                    if (rem < nt) break;
+                    rem -= nt;
+                }
+                const uint32_t first = (uint32_t)rem * B;
+                s_type = type_of_slot(t);
+                s_nb = (int)min((uint32_t)B, cnt[t] - first);
+                gidx[0] = st[t] + first;  // position in `order` of the first block
+            } else {
+                s_type = a.type;
+                s_nb = min(B, a.n - tile * B);
+                gidx[0] = (uint32_t)tile * B;
+            }
+        }
+        __syncthreads();
+        const int nb = s_nb;
+        const int type = s_type;
+        const uint32_t base = gidx[0];
+        __syncthreads();
+        for (int b = tid; b < B; b += NT) {
+            if (b < nb) {
+                if (MODE == MODE_CHAIN) {
+                    const uint32_t g = a.order[base + b];
+                    const mt_txblk d = a.blocks[g];
+                    gidx[b] = g;
+                    goff[b] = d.offset;
+                    param_from_desc(d, prm[b]);
+                } else {
+                    const uint32_t g = base + b;
+                    gidx[b] = g;
+                    goff[b] = g;
+                    param_from_quant(a.quants[g], prm[b]);
+                }
+            }
+            s_sse[b] = 0;
+            s_nnz[b] = 0;
+            s_hash[b] = 0;
+        }
+        __syncthreads();
+
+        const bool is_idtx = type == MT_TXTP_IDTX;
+        const bool is_1d = type == MT_TXTP_V_DCT || type == MT_TXTP_H_DCT;
+        const bool col_adst = type == MT_TXTP_ADST_ADST || type == MT_TXTP_ADST_DCT;
+        const bool row_adst = type == MT_TXTP_ADST_ADST || type == MT_TXTP_DCT_ADST;
+        // the identity family is quantized and dequantized flat (src/quant.rs:300)
+        const bool use_qm_type = !(is_idtx || is_1d);
+
+        // Quantize one coefficient (fx, fy) of block b, record it, and leave its dequantized
+        // (and rect2-prescaled) value for the inverse in work[b][fy][fx].
+        auto emit_coef = [&](int b, int fx, int fy, int32_t lvl, float tgt) {
+            const int rc = fx * H + fy;
+            const BlkParam& p = prm[b];
+            if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
+                const size_t o = (MODE == MODE_FWD ? (size_t)gidx[b] * N : (size_t)goff[b]) + rc;
+                a.levels[o] = lvl;
+                a.targets[o] = tgt;
+            }
+            if (MODE == MODE_CHAIN) {
+                atomicAdd(&s_hash[b], coef_hash(rc, lvl, tgt));
+                if (lvl) atomicAdd(&s_nnz[b], 1u);
+            }
+            if (MODE != MODE_FWD) {
+                int32_t step = rc == 0 ? p.dc : p.ac;
+                if (use_qm_type && p.qm_level < 15)
+                    step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
+                int32_t c = dequant(lvl, step, R::dq_shift, p.cf_max);
+                if (R::rect2) c = (c * 181 + 128) >> 8;
+                work[b * BS + fy * RS + fx] = c;
+            }
+        };
+        // Level and target of a 2-D-type coefficient (store_quant_target + apply_qmatrix).
+        auto quant2d = [&](int b, int fx, int fy, int32_t coeff) {
+            const BlkParam& p = prm[b];
+            const int rc = fx * H + fy;
+            int32_t m = rc == 0 ? p.qdc : p.qac;
+            if (R::ratio) m = mulq16(m, R::ratio);
+            int32_t lvl = quant_q16(coeff, m);
+            float tgt = q16_target(coeff, m);
+            if (p.qm_level < 15) lvl = qm_apply(tgt, g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc]);
+            emit_coef(b, fx, fy, lvl, tgt);
+        };
+
+        if (MODE != MODE_INV) {
+            // ---- load the residual into LDS (row-major, padded rows)
+            for (int i = tid; i < nb * N; i += NT) {
+                const int b = i / N, p = i % N, y = p / W, x = p % W;
+                int32_t v;
+                if (MODE == MODE_CHAIN) {
+                    const int16_t s = a.pool16[(size_t)goff[b] + p];
+                    in16[b * N + p] = s;
+                    v = s;
+                } else {
+                    v = a.in32[(size_t)gidx[b] * N + p];
+                }
+                work[b * BS + y * RS + x] = v;
+            }
+            __syncthreads();
+
+            if (is_idtx) {
+                // identity_quant_t_direct (src/dct.rs:1586): division by the dequant step.
+                for (int i = tid; i < nb * N; i += NT) {
+                    const int b = i / N, p = i % N, y = p / W, x = p % W;
+                    const BlkParam& pp = prm[b];
+                    const int rc = y + x * H;
+                    const int32_t qd = rc == 0 ? pp.dc : pp.ac;
+                    const int32_t num = work[b * BS + y * RS + x] * 8;
+                    const float tgt = __fdiv_rn((float)num, (float)qd);
+                    const int32_t am = num < 0 ? -num : num;
+                    int32_t lvl = 0;
+                    if (am >= qd) {
+                        const int32_t l = (am + qd / 2) / qd;
+                        lvl = num < 0 ? -l : l;
+                    }
+                    emit_coef(b, x, y, lvl, tgt);
+                }
+            } else if (is_1d) {
+                // f1ddct*: DCT along one axis, identity along the other, per-length scale,
+                // full-step quantizer, un-ratioed multipliers (src/dct.rs:328, :391, :1044).
+                const bool vertical = type == MT_TXTP_V_DCT;
+                if (vertical) {
+                    for (int line = tid; line < nb * W; line += NT) {
+                        const int b = line / W, x = line % W;
+                        int32_t v[H];
+#pragma unroll
+                        for (int y = 0; y < H; y++) v[y] = work[b * BS + y * RS + x];
+                        if constexpr (H <= 16) FDct<H>::run(v);
+                        const BlkParam& p = prm[b];
+#pragma unroll
+                        for (int y = 0; y < H; y++) {
+                            int32_t f;
+                            if (W == 4 && H == 4) f = mulq16(v[y] * 2, kSqrt2Q16);
+                            else if (H == 8) f = v[y] * 2;
+                            else f = mulq16(v[y], kSqrt2Q16);
+                            const int rc = y + x * H;
+                            const int32_t m = rc == 0 ? p.qdc : p.qac;
+                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m));
+                        }
+                    }
+                } else {
+                    for (int line = tid; line < nb * H; line += NT) {
+                        const int b = line / H, y = line % H;
+                        int32_t v[W];
+#pragma unroll
+                        for (int x = 0; x < W; x++) v[x] = work[b * BS + y * RS + x];
+                        if constexpr (W <= 16) FDct<W>::run(v);
+                        const BlkParam& p = prm[b];
+#pragma unroll
+                        for (int x = 0; x < W; x++) {
+                            int32_t f;
+                            if (W == 4 && H == 4) f = mulq16(v[x] * 2, kSqrt2Q16);
+                            else if (W == 8) f = v[x] * 2;
+                            else f = mulq16(v[x], kSqrt2Q16);
+                            const int rc = y + x * H;
+                            const int32_t m = rc == 0 ? p.qdc : p.qac;
+                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m));
+                        }
+                    }
+                }
+            } else if (!R::row_first) {
+                // column pass (length H), then row pass (length W) + quantization
+                for (int line = tid; line < nb * W; line += NT) {
+                    const int b = line / W, x = line % W;
+                    int32_t v[H];
+#pragma unroll
+                    for (int y = 0; y < H; y++)
+                        v[y] = (int32_t)((uint32_t)work[b * BS + y * RS + x] << R::pre_shift);
+                    fwd_line<H>(col_adst, v);
+#pragma unroll
+                    for (int y = 0; y < H; y++) work[b * BS + y * RS + x] = v[y];
+                }
+                __syncthreads();
+                for (int line = tid; line < nb * H; line += NT) {
+                    const int b = line / H, fy = line % H;
+                    int32_t v[W];
+#pragma unroll
+                    for (int x = 0; x < W; x++) v[x] = work[b * BS + fy * RS + x];
+                    fwd_line<W>(row_adst, v);
+#pragma unroll
+                    for (int fx = 0; fx < W; fx++) {
+                        int32_t c = v[fx];
+                        if (R::post == 1) c = mulq16(c, 46341);
+                        else if (R::post == 2) c = mulq16(c, 32768);
+                        else if (R::post == 3) c = (c + 32) >> 6;
+                        else if (R::post == 4) c = (int32_t)(((int64_t)c * 16384 + (1ll << 21)) >> 22);
+                        quant2d(b, fx, fy, c);
+                    }
+                }
+            } else {
+                // row pass (length W), then column pass (length H) + quantization
+                for (int line = tid; line < nb * H; line += NT) {
+                    const int b = line / H, y = line % H;
+                    int32_t v[W];
+#pragma unroll
+                    for (int x = 0; x < W; x++)
+                        v[x] = (int32_t)((uint32_t)work[b * BS + y * RS + x] << R::pre_shift);
+                    fwd_line<W>(row_adst, v);
+#pragma unroll
+                    for (int x = 0; x < W; x++) work[b * BS + y * RS + x] = v[x];
+                }
+                __syncthreads();
+                for (int line = tid; line < nb * W; line += NT) {
+                    const int b = line / W, fx = line % W;
+                    int32_t v[H];
+#pragma unroll
+                    for (int y = 0; y < H; y++) v[y] = work[b * BS + y * RS + fx];
+                    fwd_line<H>(col_adst, v);
+#pragma unroll
+                    for (int fy = 0; fy < H; fy++) {
+                        int32_t c = v[fy];
+                        if (R::post == 3) c = (c + 32) >> 6;
+                        quant2d(b, fx, fy, c);
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
+            // ---- inverse-only: dequantize the given levels into LDS
+            const bool use_qm = use_qm_type;
+            for (int i = tid; i < nb * N; i += NT) {
+                const int b = i / N, rc = i % N, fx = rc / H, fy = rc % H;
+                const BlkParam& p = prm[b];
+                const int32_t lvl = a.in32[(size_t)gidx[b] * N + rc];
+                int32_t step = rc == 0 ? p.dc : p.ac;
+                if (use_qm && p.qm_level < 15)
+                    step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
+                int32_t c = dequant(lvl, step, R::dq_shift, p.cf_max);
+                if (R::rect2) c = (c * 181 + 128) >> 8;
+                work[b * BS + fy * RS + fx] = c;
+            }
+            __syncthreads();
+        }
+
+        if (MODE != MODE_FWD) {
+            // inverse kinds: ADST_DCT is vertical ADST (column pass)
+            int rowk = KD, colk = KD;
+            if (type == MT_TXTP_ADST_ADST) { rowk = KA; colk = KA; }
+            else if (type == MT_TXTP_ADST_DCT) { rowk = KD; colk = KA; }
+            else if (type == MT_TXTP_DCT_ADST) { rowk = KA; colk = KD; }
+            else if (type == MT_TXTP_V_DCT) { rowk = KI; colk = KD; }
+            else if (type == MT_TXTP_H_DCT) { rowk = KD; colk = KI; }
+            else if (type == MT_TXTP_IDTX) { rowk = KI; colk = KI; }
+
+            // ---- inverse row pass (length W) with the row clip, inter-pass shift + clip
+            for (int line = tid; line < nb * H; line += NT) {
+                const int b = line / H, y = line % H;
+                const BlkParam& p = prm[b];
+                int32_t v[W];
+#pragma unroll
+                for (int x = 0; x < W; x++) v[x] = work[b * BS + y * RS + x];
+                inv_line<W>(rowk, v, Clip{p.rmin, p.rmax});
+                const Clip cc{p.cmin, p.cmax};
+#pragma unroll
+                for (int x = 0; x < W; x++) {
+                    int32_t t = v[x];
+                    if (R::mid) t = (t + (1 << (R::mid - 1))) >> R::mid;
+                    work[b * BS + y * RS + x] = cc(t);
+                }
+            }
+            __syncthreads();
+            // ---- inverse column pass (length H), final (t + 8) >> 4, distortion
+            for (int line = tid; line < nb * W; line += NT) {
+                const int b = line / W, x = line % W;
+                const BlkParam& p = prm[b];
+                int32_t v[H];
+#pragma unroll
+                for (int y = 0; y < H; y++) v[y] = work[b * BS + y * RS + x];
+                inv_line<H>(colk, v, Clip{p.cmin, p.cmax});
+                uint32_t sse = 0, hsh = 0;
+                const int P = 1 << (p.bd - 1), maxv = (1 << p.bd) - 1;
+#pragma unroll
+                for (int y = 0; y < H; y++) {
+                    const int32_t r = (v[y] + 8) >> 4;
+                    const int j = y * W + x;
+                    if (MODE == MODE_INV) {
+                        a.recon[(size_t)gidx[b] * N + j] = r;
+                    } else {
+                        if (a.recon) a.recon[(size_t)goff[b] + j] = r;
+                        const int32_t s = in16[b * N + j];
+                        const int32_t src = min(max(P + s, 0), maxv);
+                        const int32_t rec = min(max(P + r, 0), maxv);
+                        const int32_t d = src - rec;
+                        sse += (uint32_t)(d * d);
+                        hsh += pix_hash(j, r);
+                    }
+                }
+                if (MODE == MODE_CHAIN) {
+                    atomicAdd(&s_sse[b], (unsigned long long)sse);
+                    atomicAdd(&s_hash[b], hsh);
+                }
+            }
+        }
+        __syncthreads();
+        if (MODE == MODE_CHAIN) {
+            for (int b = tid; b < nb; b += NT) {
+                mt_txres r;
+                r.sse = (int64_t)s_sse[b];
+                r.nnz = s_nnz[b];
+                r.hash = s_hash[b];
+                a.res[gidx[b]] = r;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------- 64x64
+// TX_64X64: the reference codes the low 32x32 of a 64-point DCT.  Forward = f64 separable
+// basis (dct64x64_coeffs, src/dct.rs:4155: vertical pass accumulated over y, then an
+// ordered sum over x, round half away from zero of 8x the sum); quantizer with the 32x32
+// QM; inverse = idct_dequant_64x64 (src/idct.rs:2677).  One 256-thread workgroup per block.
+template <int MODE>
+__global__ void __launch_bounds__(256) tx64_kernel(KArgs a) {
+    constexpr int RS = 65;
+    __shared__ int32_t work[64 * RS];
+    __shared__ double vt[32 * 64];
+    __shared__ int16_t in16[MODE == MODE_CHAIN ? 4096 : 1];
+    __shared__ BlkParam prm;
+    __shared__ uint32_t s_g, s_off;
+    __shared__ unsigned long long s_sse;
+    __shared__ uint32_t s_nnz, s_hash;
+    __shared__ int s_n;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        if (MODE == MODE_CHAIN) {
+            const int sl = 12 * 7;  // 64x64, DCT_DCT
+            s_n = (int)a.cls_count[sl];
+        } else {
+            s_n = a.n;
+        }
+    }
+    __syncthreads();
+    const int nblocks = s_n;
+    const uint32_t chain_start = MODE == MODE_CHAIN ? a.cls_start[12 * 7] : 0;
+    for (int blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        if (tid == 0) {
+            if (MODE == MODE_CHAIN) {
+                const uint32_t g = a.order[chain_start + blk];
+                const mt_txblk d = a.blocks[g];
+                s_g = g;
+                s_off = d.offset;
+                param_from_desc(d, prm);
+            } else {
+                s_g = blk;
+                s_off = blk;
+                param_from_quant(a.quants[blk], prm);
+            }
+            s_sse = 0;
+            s_nnz = 0;
+            s_hash = 0;
+        }
+        __syncthreads();
+        const uint32_t g = s_g, off = s_off;
+        for (int i = tid; i < 64 * RS; i += 256) work[i] = 0;
+        __syncthreads();
+        if (MODE != MODE_INV) {
+            for (int p = tid; p < 4096; p += 256) {
+                int32_t v;
+                if (MODE == MODE_CHAIN) {
+                    const int16_t s = a.pool16[(size_t)off + p];
+                    in16[p] = s;
+                    v = s;
+                } else {
+                    v = a.in32[(size_t)g * 4096 + p];
+                }
+                work[(p >> 6) * RS + (p & 63)] = v;
+            }
+            __syncthreads();
+            // vertical pass: vt[v][x] = sum_y basis[v][y] * in[y][x], y ascending
+            {
+                const int x = tid & 63;
+                for (int v = tid >> 6; v < 32; v += 4) {
+                    double acc = 0.0;
+                    const double* bv = g_dct64_basis + v * 64;
+                    for (int y = 0; y < 64; y++)
+                        acc = __dadd_rn(acc, __dmul_rn(bv[y], (double)work[y * RS + x]));
+                    vt[v * 64 + x] = acc;
+                }
+            }
+            __syncthreads();
+            // horizontal pass + quantization of coefficient (u, v), rc = u*32 + v
+            for (int i = tid; i < 1024; i += 256) {
+                const int u = i & 31, v = i >> 5;
+                const double* bu = g_dct64_basis + u * 64;
+                double s = -0.0;
+                for (int x = 0; x < 64; x++) s = __dadd_rn(s, __dmul_rn(vt[v * 64 + x], bu[x]));
+                const int32_t coeff = (int32_t)round(__dmul_rn(s, 8.0));
+                const int rc = u * 32 + v;
+                const int32_t m = rc == 0 ? prm.qdc : prm.qac;
+                int32_t lvl = quant_q16(coeff, m);
+                float tgt = q16_target(coeff, m);
+                const bool qm = prm.qm_level < 15;
+                int iwt = 32;
+                if (qm) {
+                    iwt = g_iqm[prm.qm_level][prm.qm_chroma][336 + rc];
+                    lvl = qm_apply(tgt, iwt);
+                }
+                if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
+                    const size_t o = (MODE == MODE_FWD ? (size_t)g * 1024 : (size_t)off) + rc;
+                    a.levels[o] = lvl;
+                    a.targets[o] = tgt;
+                }
+                if (MODE == MODE_CHAIN) {
+                    int32_t step = rc == 0 ? prm.dc : prm.ac;
+                    if (qm) step = (step * iwt + 16) >> 5;
+                    atomicAdd(&s_hash, coef_hash(rc, lvl, tgt));
+                    if (lvl) atomicAdd(&s_nnz, 1u);
+                    work[v * RS + u] = dequant(lvl, step, 2, prm.cf_max);
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int rc = tid; rc < 1024; rc += 256) {
+                const int u = rc >> 5, v = rc & 31;
+                int32_t step = rc == 0 ? prm.dc : prm.ac;
+                if (prm.qm_level < 15)
+                    step = (step * (int32_t)g_iqm[prm.qm_level][prm.qm_chroma][336 + rc] + 16) >> 5;
+                work[v * RS + u] = dequant(a.in32[(size_t)g * 1024 + rc], step, 2, prm.cf_max);
+            }
+            __syncthreads();
+        }
+        if (MODE != MODE_FWD) {
+            // row pass over the 32 coded rows, inter-pass (t + 2) >> 2 clip
+            if (tid < 32) {
+                const int y = tid;
+                int32_t v[64];
+#pragma unroll
+                for (int x = 0; x < 64; x++) v[x] = x < 32 ? work[y * RS + x] : 0;
+                idct64<1>(v, Clip{prm.rmin, prm.rmax});
+                const Clip cc{prm.cmin, prm.cmax};
+#pragma unroll
+                for (int x = 0; x < 64; x++) work[y * RS + x] = cc((v[x] + 2) >> 2);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int x = tid;
+                int32_t v[64];
+#pragma unroll
+                for (int y = 0; y < 64; y++) v[y] = y < 32 ? work[y * RS + x] : 0;
+                idct64<1>(v, Clip{prm.cmin, prm.cmax});
+                uint32_t sse = 0, hsh = 0;
+                const int P = 1 << (prm.bd - 1), maxv = (1 << prm.bd) - 1;
+#pragma unroll
+                for (int y = 0; y < 64; y++) {
+                    const int32_t r = (v[y] + 8) >> 4;
+                    const int j = y * 64 + x;
+                    if (MODE == MODE_INV) {
+                        a.recon[(size_t)g * 4096 + j] = r;
+                    } else {
+                        if (a.recon) a.recon[(size_t)off + j] = r;
+                        const int32_t s = in16[j];
+                        const int32_t src = min(max(P + s, 0), maxv);
+                        const int32_t rec = min(max(P + r, 0), maxv);
+                        const int32_t d = src - rec;
+                        sse += (uint32_t)(d * d);
+                        hsh += pix_hash(j, r);
+                    }
+                }
+                if (MODE == MODE_CHAIN) {
+                    atomicAdd(&s_sse, (unsigned long long)sse);
+                    atomicAdd(&s_hash, hsh);
+                }
+            }
+        }
+        __syncthreads();
+        if (MODE == MODE_CHAIN && tid == 0) {
+            mt_txres r;
+            r.sse = (int64_t)s_sse;
+            r.nnz = s_nnz;
+            r.hash = s_hash;
+            a.res[g] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------- bucketing
+// Counting sort of the block list by (size slot, type slot): 13 x 7 classes.  Blocks of an
+// unsupported class are answered directly (sse = -1).
+constexpr int kClasses = 13 * 7;
+
+__global__ void __launch_bounds__(256) bucket_count_kernel(const mt_txblk* blocks, int n,
+                                                          uint32_t* counts, mt_txres* res) {
+    __shared__ uint32_t h[kClasses];
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const mt_txblk d = blocks[i];
+        const int ss = size_slot(d.tx_size), ts = type_slot(d.tx_type);
+        if (ss >= 0 && ts >= 0 && tx_supported(d.tx_size, d.tx_type)) {
+            atomicAdd(&h[ss * 7 + ts], 1u);
+        } else {
+            mt_txres r;
+            r.sse = -1;
+            r.nnz = 0;
+            r.hash = 0;
+            res[i] = r;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+        if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Exclusive scan of the class counts into starts and scatter cursors (one workgroup).
+__global__ void bucket_scan_kernel(const uint32_t* counts, uint32_t* starts, uint32_t* cursor) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < kClasses; i++) {
+            starts[i] = acc;
+            cursor[i] = acc;
+            acc += counts[i];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(const mt_txblk* blocks, int n,
+                                                            uint32_t* cursor, uint32_t* order) {
+    // Per-workgroup LDS histogram, one global reservation per class, then placement.
+    __shared__ uint32_t h[kClasses], basepos[kClasses];
+    const int chunk_begin = blockIdx.x * 4096;
+    const int chunk_end = min(n, chunk_begin + 4096);
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
+        const mt_txblk d = blocks[i];
+        const int ss = size_slot(d.tx_size), ts = type_slot(d.tx_type);
+        if (ss >= 0 && ts >= 0 && tx_supported(d.tx_size, d.tx_type)) atomicAdd(&h[ss * 7 + ts], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) {
+        basepos[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0;
+        h[i] = 0;
+    }
+    __syncthreads();
+    for (int i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
+        const mt_txblk d = blocks[i];
+        const int ss = size_slot(d.tx_size), ts = type_slot(d.tx_type);
+        if (ss >= 0 && ts >= 0 && tx_supported(d.tx_size, d.tx_type)) {
+            const int c = ss * 7 + ts;
+            order[basepos[c] + atomicAdd(&h[c], 1u)] = (uint32_t)i;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------- host side
+namespace {
+
+struct DeviceState {
+    bool init = false;
+    int cus = 256;
+    // chain workspace
+    uint32_t* counts = nullptr;  // [kClasses]
+    uint32_t* starts = nullptr;  // [kClasses]
+    uint32_t* cursor = nullptr;  // [kClasses]
+    uint32_t* order = nullptr;   // [cap]
+    size_t order_cap = 0;
+};
+std::mutex g_mu;
+DeviceState g_dev[64];
+
+// Optional per-launch timing: while enabled, every launch of mt_tx_chain is bracketed by
+// a pair of HIP events on the caller's stream; mt_profile_end sums them per kernel slot.
+struct ProfState {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> recs;  // (slot, events)
+    std::vector<hipEvent_t> pool;
+};
+ProfState g_prof;
+
+hipEvent_t prof_event() {
+    if (!g_prof.pool.empty()) {
+        hipEvent_t e = g_prof.pool.back();
+        g_prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ProfScope {
+    int slot;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(int slot_, hipStream_t s_) : slot(slot_), s(s_) {
+        if (!g_prof.on) return;
+        a = prof_event();
+        b = prof_event();
+        if (a) (void)hipEventRecord(a, s);
+    }
+    ~ProfScope() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, s);
+        g_prof.recs.push_back({slot, {a, b}});
+    }
+};
+
+int current_device(int* dev) {
+    HIP_TRY(hipGetDevice(dev));
+    if (*dev < 0 || *dev >= 64) return set_err(MT_ERR_ARG, "device index out of range");
+    if (!g_dev[*dev].init) return set_err(MT_ERR_NOT_INIT, "mt_init was not called for this device");
+    return MT_OK;
+}
+
+int ensure_order(DeviceState& st, size_t n) {
+    if (st.order_cap >= n && st.order) return MT_OK;
+    if (st.order) HIP_TRY(hipFree(st.order));
+    st.order = nullptr;
+    size_t cap = std::max<size_t>(n, 1 << 16);
+    HIP_TRY(hipMalloc(&st.order, cap * sizeof(uint32_t)));
+    st.order_cap = cap;
+    return MT_OK;
+}
+
+int grid_for(int tiles, int cus) {
+    const int cap = cus * 8;
+    return std::max(1, std::min(tiles, cap));
+}
+
+template <int W, int H, int MODE> int launch_tiles(const KArgs& a, int slot, int tiles_hint,
+                                                  int cus, hipStream_t s) {
+    const int grid = grid_for(tiles_hint, cus);
+    hipLaunchKernelGGL((tx_tile_kernel<W, H, MODE>), dim3(grid), dim3(256), 0, s, a, slot);
+    HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+
+template <int MODE> int launch_size(int tx_size, const KArgs& a, int tiles_hint, int cus,
+                                    hipStream_t s) {
+    switch (tx_size) {
+        case MT_TX_4X4: return launch_tiles<4, 4, MODE>(a, 0, tiles_hint, cus, s);
+        case MT_TX_4X8: return launch_tiles<4, 8, MODE>(a, 1, tiles_hint, cus, s);
+        case MT_TX_8X4: return launch_tiles<8, 4, MODE>(a, 2, tiles_hint, cus, s);
+        case MT_TX_4X16: return launch_tiles<4, 16, MODE>(a, 3, tiles_hint, cus, s);
+        case MT_TX_16X4: return launch_tiles<16, 4, MODE>(a, 4, tiles_hint, cus, s);
+        case MT_TX_8X8: return launch_tiles<8, 8, MODE>(a, 5, tiles_hint, cus, s);
+        case MT_TX_8X16: return launch_tiles<8, 16, MODE>(a, 6, tiles_hint, cus, s);
+        case MT_TX_16X8: return launch_tiles<16, 8, MODE>(a, 7, tiles_hint, cus, s);
+        case MT_TX_16X16: return launch_tiles<16, 16, MODE>(a, 8, tiles_hint, cus, s);
+        case MT_TX_16X32: return launch_tiles<16, 32, MODE>(a, 9, tiles_hint, cus, s);
+        case MT_TX_32X16: return launch_tiles<32, 16, MODE>(a, 10, tiles_hint, cus, s);
+        case MT_TX_32X32: return launch_tiles<32, 32, MODE>(a, 11, tiles_hint, cus, s);
+        case MT_TX_64X64: {
+            const int grid = grid_for(tiles_hint, cus);
+            hipLaunchKernelGGL((tx64_kernel<MODE>), dim3(grid), dim3(256), 0, s, a);
+            HIP_TRY(hipGetLastError());
+            return MT_OK;
+        }
+        default: return set_err(MT_ERR_UNSUPPORTED, "unsupported transform size");
+    }
+}
+
+int blocks_per_tile(int tx_size) {
+    const int w = kTxW[tx_size], h = kTxH[tx_size];
+    if (w == 64) return 1;
+    const int n = w * h, mn = std::min(w, h);
+    return std::max(256 / mn, 4096 / n);
+}
+
+// round(65536 / step) as the reference computes it (f64, round half away from zero).
+int32_t q_mult(int32_t step) { return (int32_t)std::round(65536.0 / (double)step); }
+
+void itx_clips(int bd, mt_quant* q) {  // src/quant.rs:243
+    if (bd <= 8) {
+        q->rmin = q->cmin = -32768;
+        q->rmax = q->cmax = 32767;
+        q->cf_max = 32767;
+    } else {
+        const int32_t rmax = (1 << (bd + 7)) - 1, cmax = (1 << (bd + 5)) - 1;
+        q->rmin = ~rmax;
+        q->rmax = rmax;
+        q->cmin = ~cmax;
+        q->cmax = cmax;
+        q->cf_max = rmax;
+    }
+}
+
+int bd_index(int bd) { return bd == 10 ? 1 : bd == 12 ? 2 : 0; }
+
+}  // namespace
+
+extern "C" const char* mt_last_error(void) { return g_err.c_str(); }
+extern "C" const char* mt_version(void) { return "mtgpu 0.1 (gfx950)"; }
+
+extern "C" int mt_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (device < 0 || device >= 64) return set_err(MT_ERR_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    DeviceState& st = g_dev[device];
+    if (st.init) return MT_OK;
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_dcq), MT_DC_QLOOKUP, sizeof(MT_DC_QLOOKUP)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_acq), MT_AC_QLOOKUP, sizeof(MT_AC_QLOOKUP)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst4), MT_FADST4_Q12, sizeof(MT_FADST4_Q12)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst8), MT_FADST8_Q12, sizeof(MT_FADST8_Q12)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst16), MT_FADST16_Q12, sizeof(MT_FADST16_Q12)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_iqm), MT_IQM, sizeof(MT_IQM)));
+    // dct64_basis (src/dct.rs:4132): built on the host with the C library's cos, as the
+    // reference builds it, so every f64 weight is the same.
+    std::vector<double> basis(32 * 64);
+    for (int k = 0; k < 32; k++) {
+        const double a = k == 0 ? std::sqrt(1.0 / 64.0) : std::sqrt(2.0 / 64.0);
+        for (int i = 0; i < 64; i++)
+            basis[k * 64 + i] = a * std::cos(M_PI * (double)(2 * i + 1) * (double)k / 128.0);
+    }
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_dct64_basis), basis.data(), basis.size() * sizeof(double)));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    st.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&st.cursor, kClasses * sizeof(uint32_t)));
+    HIP_TRY(hipDeviceSynchronize());
+    st.init = true;
+    return MT_OK;
+}
+
+extern "C" int mt_quant_new(int base_q_idx, int bd, int qm_level, mt_quant* out) {
+    if (!out || base_q_idx < 0 || base_q_idx > 255 || (bd != 8 && bd != 10 && bd != 12))
+        return set_err(MT_ERR_ARG, "mt_quant_new: bad argument");
+    memset(out, 0, sizeof(*out));
+    itx_clips(bd, out);
+    const int dc_idx = std::min(255, std::max(0, base_q_idx - 8));  // luma DC delta -8
+    out->dc = MT_DC_QLOOKUP[bd_index(bd)][dc_idx];
+    out->ac = MT_AC_QLOOKUP[bd_index(bd)][base_q_idx];
+    out->q_mult_dc = q_mult(out->dc);
+    out->q_mult_ac = q_mult(out->ac);
+    out->qm_level = std::min(std::max(qm_level, 0), 15);
+    out->qm_chroma = 0;
+    out->qidx = base_q_idx;
+    out->bd = bd;
+    return MT_OK;
+}
+
+extern "C" int mt_quant_new_chroma(int base_q_idx, int dc_delta, int ac_delta, int bd,
+                                   int qm_level, mt_quant* out) {
+    if (!out || base_q_idx < 0 || base_q_idx > 255 || (bd != 8 && bd != 10 && bd != 12))
+        return set_err(MT_ERR_ARG, "mt_quant_new_chroma: bad argument");
+    memset(out, 0, sizeof(*out));
+    const int dc_idx = std::min(255, std::max(0, base_q_idx + dc_delta));
+    const int ac_idx = std::min(255, std::max(0, base_q_idx + ac_delta));
+    itx_clips(bd, out);
+    out->dc = MT_DC_QLOOKUP[bd_index(bd)][dc_idx];
+    out->ac = MT_AC_QLOOKUP[bd_index(bd)][ac_idx];
+    out->q_mult_dc = q_mult(out->dc);
+    out->q_mult_ac = q_mult(out->ac);
+    out->qm_level = std::min(std::max(qm_level, 0), 15);
+    out->qm_chroma = 1;
+    out->qidx = ac_idx;
+    out->bd = bd;
+    return MT_OK;
+}
+
+// FastRound (src/util.rs:35-59): a one-ulp nudge, then the 1.5 * 2^23 magic round.
+static float fast_round(float x) {
+    const float magic = (float)((1u << 23) + (1u << 22));
+    uint32_t bits;
+    memcpy(&bits, &x, 4);
+    bits += 1;
+    float y;
+    memcpy(&y, &bits, 4);
+    volatile float s = y + magic;
+    return s - magic;
+}
+
+extern "C" int mt_qm_level_law(int base_q_idx, int sub) {
+    // qm_level_law (src/quant.rs:137-154)
+    const bool subsampled = sub != 0;
+    const float hi = 48.0f, lo = 128.0f;
+    const float qi = (float)base_q_idx;
+    if (qi <= 20.0f) return subsampled ? 8 : 15;
+    if (qi <= hi) {
+        const float top = subsampled ? 8.0f : 15.0f;
+        volatile float num = (8.0f - top) * (qi - 20.0f);
+        volatile float v = top + num / (hi - 20.0f);
+        return (int)(uint8_t)fast_round(v);
+    }
+    if (qi >= lo) return 0;
+    volatile float num = 8.0f * (lo - qi);
+    volatile float v = num / (lo - hi);
+    return (int)(uint8_t)fast_round(v);
+}
+
+extern "C" int mt_tx_width(int s) { return (s >= 0 && s < 19) ? kTxW[s] : 0; }
+extern "C" int mt_tx_height(int s) { return (s >= 0 && s < 19) ? kTxH[s] : 0; }
+extern "C" int mt_tx_ncoef(int s) {
+    if (s < 0 || s >= 19) return 0;
+    return std::min(kTxW[s], 32) * std::min(kTxH[s], 32);
+}
+extern "C" int mt_tx_supported(int s, int t) { return (s >= 0 && s < 19 && tx_supported(s, t)) ? 1 : 0; }
+
+static int parity_launch(int mode, int tx_size, int tx_type, int n, const int32_t* in,
+                         const mt_quant* quants, int32_t* levels, float* targets, int32_t* out,
+                         void* stream) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!in || !quants))) return set_err(MT_ERR_ARG, "bad pointer or count");
+    if (tx_size < 0 || tx_size >= 19 || !tx_supported(tx_size, tx_type))
+        return set_err(MT_ERR_UNSUPPORTED, "no reference transform for this (size, type)");
+    if (n == 0) return MT_OK;
+    KArgs a{};
+    a.in32 = in;
+    a.quants = quants;
+    a.levels = levels;
+    a.targets = targets;
+    a.recon = out;
+    a.n = n;
+    a.type = tx_type;
+    const int tiles = (n + blocks_per_tile(tx_size) - 1) / blocks_per_tile(tx_size);
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == MODE_FWD) return launch_size<MODE_FWD>(tx_size, a, tiles, g_dev[dev].cus, s);
+    return launch_size<MODE_INV>(tx_size, a, tiles, g_dev[dev].cus, s);
+}
+
+extern "C" int mt_fwd_txfm(int tx_size, int tx_type, int n, const int32_t* resid_d,
+                           const mt_quant* quants_d, int32_t* levels_d, float* targets_d,
+                           void* stream) {
+    if (n > 0 && (!levels_d || !targets_d)) return set_err(MT_ERR_ARG, "null output");
+    return parity_launch(MODE_FWD, tx_size, tx_type, n, resid_d, quants_d, levels_d, targets_d,
+                         nullptr, stream);
+}
+
+extern "C" int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* levels_d,
+                           const mt_quant* quants_d, int32_t* out_d, void* stream) {
+    if (n > 0 && !out_d) return set_err(MT_ERR_ARG, "null output");
+    return parity_launch(MODE_INV, tx_size, tx_type, n, levels_d, quants_d, nullptr, nullptr,
+                         out_d, stream);
+}
+
+extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
+                           int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!blocks_d || !pool_d || !res_d)))
+        return set_err(MT_ERR_ARG, "mt_tx_chain: bad pointer or count");
+    if ((levels_d == nullptr) != (targets_d == nullptr))
+        return set_err(MT_ERR_ARG, "mt_tx_chain: levels and targets go together");
+    if (n == 0) return MT_OK;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState& st = g_dev[dev];
+    rc = ensure_order(st, (size_t)n);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    {
+    ProfScope ps(0, s);
+    // cleared by a kernel, not hipMemsetAsync, so the sequence stays graph-capturable
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, st.counts, kClasses);
+    HIP_TRY(hipGetLastError());
+    const int cgrid = std::min((n + 255) / 256, st.cus * 4);
+    hipLaunchKernelGGL(bucket_count_kernel, dim3(cgrid), dim3(256), 0, s, blocks_d, n, st.counts, res_d);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(64), 0, s, st.counts, st.starts, st.cursor);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n,
+                       st.cursor, st.order);
+    HIP_TRY(hipGetLastError());
+    }
+    KArgs a{};
+    a.blocks = blocks_d;
+    a.order = st.order;
+    a.cls_start = st.starts;
+    a.cls_count = st.counts;
+    a.pool16 = pool_d;
+    a.res = res_d;
+    a.levels = levels_d;
+    a.targets = targets_d;
+    a.recon = recon_d;
+    // The class sizes live on the device; each launch is persistent over its own tiles, so
+    // the grid only needs an upper bound: n blocks can never need more tiles than this.
+    for (int slot = 0; slot < 13; slot++) {
+        const int sz = kSlotSize[slot];
+        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
+        ProfScope ps(1 + slot, s);
+        rc = launch_size<MODE_CHAIN>(sz, a, tiles, st.cus, s);
+        if (rc) return rc;
+    }
+    return MT_OK;
+}
+
+extern "C" int mt_profile_begin(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.on = true;
+    return MT_OK;
+}
+
+extern "C" int mt_profile_end(double* kernel_ms, int* launches, int max_slots) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.on = false;
+    for (int i = 0; i < max_slots; i++) {
+        if (kernel_ms) kernel_ms[i] = 0.0;
+        if (launches) launches[i] = 0;
+    }
+    int rc = MT_OK;
+    for (auto& r : g_prof.recs) {
+        hipEvent_t a = r.second.first, b = r.second.second;
+        float ms = 0.0f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+            rc = set_err(MT_ERR_DEVICE, "profiling event failed");
+        if (r.first < max_slots) {
+            if (kernel_ms) kernel_ms[r.first] += ms;
+            if (launches) launches[r.first] += 1;
+        }
+        g_prof.pool.push_back(a);
+        g_prof.pool.push_back(b);
+    }
+    g_prof.recs.clear();
+    return rc;
+}
+
+// ------------------------------------------------------------------------- generator
+namespace {
+inline uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double unit(uint64_t& x) { return (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0); }
+
+const int kAllSizes[19] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+
+struct GenBlock {
+    int size, type, qidx, bd, qm;
+};
+
+GenBlock gen_block_header(uint64_t seed, int i, int all_sizes) {
+    uint64_t st = seed ^ (0xA0761D6478BD642Full * (uint64_t)(i + 1));
+    splitmix64(st);
+    GenBlock g;
+    if (all_sizes) {
+        g.size = kAllSizes[splitmix64(st) % 19];
+    } else {
+        g.size = kSlotSize[splitmix64(st) % 13];
+    }
+    int types[7], nt = 0;
+    for (int t = 0; t < 7; t++)
+        if (tx_supported(g.size, kSlotType[t])) types[nt++] = kSlotType[t];
+    if (nt == 0) types[nt++] = MT_TXTP_DCT_DCT;  // non-reference sizes: DCT_DCT request
+    g.type = types[splitmix64(st) % nt];
+    g.qidx = (int)(splitmix64(st) & 255);
+    g.bd = (splitmix64(st) & 1) ? 10 : 8;
+    g.qm = (splitmix64(st) & 1) ? mt_qm_level_law(g.qidx, 0) : 15;
+    return g;
+}
+}  // namespace
+
+extern "C" uint64_t mt_gen_pool_size(uint64_t seed, int n, int all_sizes) {
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        const GenBlock g = gen_block_header(seed, i, all_sizes);
+        total += (uint64_t)kTxW[g.size] * kTxH[g.size];
+    }
+    return total;
+}
+
+extern "C" int mt_gen_tx_blocks(uint64_t seed, int n, int all_sizes, mt_txblk* blocks, int16_t* pool,
+                                uint64_t pool_elems) {
+    if (n < 0 || !blocks || !pool) return set_err(MT_ERR_ARG, "mt_gen_tx_blocks: bad argument");
+    std::vector<GenBlock> hdr((size_t)n);
+    std::vector<uint64_t> off((size_t)n);
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        hdr[i] = gen_block_header(seed, i, all_sizes);
+        off[i] = total;
+        total += (uint64_t)kTxW[hdr[i].size] * kTxH[hdr[i].size];
+    }
+    if (total > pool_elems || total > 0xFFFFFFFFull) return set_err(MT_ERR_ARG, "pool too small");
+    // discrete Laplace(b = 12): difference of two geometric(p = 1 - e^(-1/12)) draws,
+    // floor(-12 ln U) each; clipped to the bit depth's residual range.
+    const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; k++) {
+        th.emplace_back([&, k]() {
+            for (int i = k; i < n; i += nth) {
+                const GenBlock& g = hdr[i];
+                uint64_t st = seed * 0xD1B54A32D192ED03ull + (uint64_t)i * 0x9E3779B97F4A7C15ull + 7;
+                const int npx = kTxW[g.size] * kTxH[g.size];
+                const int lim = g.bd == 8 ? 255 : 1023;
+                for (int p = 0; p < npx; p++) {
+                    const double u1 = 1.0 - unit(st), u2 = 1.0 - unit(st);
+                    const int g1 = (int)std::floor(-12.0 * std::log(u1));
+                    const int g2 = (int)std::floor(-12.0 * std::log(u2));
+                    int v = g1 - g2;
+                    v = v < -lim ? -lim : (v > lim ? lim : v);
+                    pool[off[i] + p] = (int16_t)v;
+                }
+                mt_txblk b;
+                b.offset = (uint32_t)off[i];
+                b.tx_size = (uint8_t)g.size;
+                b.tx_type = (uint8_t)g.type;
+                b.qidx = (uint8_t)g.qidx;
+                b.bd_qm = (uint8_t)((g.qm & 15) | ((g.bd == 10 ? 1 : g.bd == 12 ? 2 : 0) << 4));
+                blocks[i] = b;
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+    return MT_OK;
+}

@@ -1,0 +1,102 @@
+/*
+ * CPU oracle for the maroontree intra candidate-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a scalar restatement of the reference encoder's
+ * kernels, written from the Rust source text, used as the checker for the HIP library.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (maroontree_amd / libmtgpu) never links or calls it.
+ *
+ * Conventions (reference src/quant.rs:95-99, src/rate.rs:93-95):
+ *   residual / prediction / reconstruction blocks are row-major  [y*W + x]
+ *   levels and targets are x-major ("transposed")                [x*H + y]
+ * Sizes and transform types use the AV1 TxSize / TxType numbering.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Quantizer state, field for field the reference's `Quant` (src/quant.rs:327-345). */
+typedef struct {
+    int32_t dc, ac;               /* dequant steps */
+    int32_t q_mult_dc, q_mult_ac; /* round(65536 / step), f64 */
+    int32_t rmin, rmax, cmin, cmax, cf_max; /* itx clips */
+    int32_t qm_level;             /* 15 = flat */
+    int32_t qm_chroma;            /* 0 luma, 1 chroma */
+    int32_t qidx;
+    int32_t bd;
+} orc_quant;
+
+/* Quant::new_with_qm (src/quant.rs:354) */
+void orc_quant_new(int base_q_idx, int bd, int qm_level, orc_quant* out);
+/* Quant::new_chroma_with_delta_qm (src/quant.rs:394) */
+void orc_quant_new_chroma(int base_q_idx, int dc_delta, int ac_delta, int bd, int qm_level,
+                          orc_quant* out);
+/* qm_level_law (src/quant.rs:137) */
+int orc_qm_level_law(int base_q_idx, int sub);
+/* qm_weight (src/quant.rs:92); 32 when flat */
+int orc_qm_weight(int level, int chroma, int rc, int w, int h);
+/* Dct::dequant_step (src/quant.rs:281) */
+int orc_dequant_step(const orc_quant* q, int rc, int w, int h);
+/* FastRound::fast_round (src/util.rs:35) */
+float orc_fast_round(float x);
+
+/* TX size helpers: width/height of an AV1 TxSize, and whether the reference has a
+ * forward transform for (size, type).  Returns 0/1. */
+int orc_tx_w(int tx_size);
+int orc_tx_h(int tx_size);
+int orc_tx_supported(int tx_size, int tx_type);
+/* number of coded coefficients (64x64 codes its low 32x32) */
+int orc_tx_ncoef(int tx_size);
+
+/* Forward transform + quantization + QM, i.e. the reference's DctDispatch::*_t methods
+ * (src/dct.rs:2624-2950) on the x86 AVX2 selection.  resid: W*H row-major.
+ * levels/targets: ncoef, x-major.  Returns 0, or -1 for an unsupported (size, type). */
+int orc_fwd_txfm(int tx_size, int tx_type, const int32_t* resid, const orc_quant* q,
+                 int32_t* levels, float* targets);
+/* Dequant + inverse transform, the reference's IdctDispatch fields plus idct_dequant_64x64
+ * (src/idct.rs:168-728, :1338, :2677).  levels: ncoef x-major.  out: W*H row-major. */
+int orc_inv_txfm(int tx_size, int tx_type, const int32_t* levels, const orc_quant* q,
+                 int32_t* out);
+
+/* sse_recon_scalar (src/rd_sse.rs:1262): sum over a w x h block of
+ * (src - clamp(pred + resid, 0, maxv))^2; pred/resid packed w-wide, src strided. */
+int64_t orc_sse_recon(const int32_t* pred, const int32_t* resid, const uint16_t* src,
+                      int stride, int w, int h, int maxv);
+
+/* apply_qmatrix in the selected x86 form (src/avx/qmatrix.rs:34): exact round half away
+ * from zero; scalar_fastround=1 uses the scalar FastRound form (src/dct.rs:111). */
+void orc_apply_qmatrix(int32_t* levels, float* targets, const uint8_t* iwt, int n,
+                       int scalar_fastround);
+
+/* 1-D primitives, exported for closed-form and brute-force tests. */
+void orc_dct1d(int n, int32_t* buf);          /* dct1d_{4,8,16,32}_i32 (src/dct.rs:171-633) */
+void orc_fadst1d(int n, const int32_t* in, int32_t* out); /* fwd_adst{4,8,16}_1d */
+void orc_inv_dct1d(int n, int32_t* c, int mn, int mx);    /* inv_dct{4..64}_1d, stride 1 */
+void orc_inv_adst1d(int n, int32_t* c, int mn, int mx);   /* inv_adst{4,8,16}_1d */
+int32_t orc_mul_q16(int32_t data, int32_t coeff);
+int32_t orc_quant_q16(int32_t data, int32_t coeff);
+
+/* The batched transform/quant microbench chain for one block:
+ * resid (int16 row-major W*H) -> fwd+quant(+QM) -> dequant -> inverse -> SSE, with the
+ * SSE convention sse_recon(pred = P, resid_inv, src = clamp(P + resid)), P = 1 << (bd-1).
+ * Any of levels/targets/recon may be NULL.  Returns the SSE, or -1 if unsupported. */
+int64_t orc_chain_block(int tx_size, int tx_type, const int16_t* resid, const orc_quant* q,
+                        int32_t* levels, float* targets, int32_t* recon);
+
+/* Whole batch, threads over blocks (nthreads <= 0: one thread).  desc/offsets as in the
+ * library's microbench layout; sse_out[i] = SSE of block i, -1 if unsupported. */
+int orc_chain_batch(int n, const uint8_t* tx_size, const uint8_t* tx_type, const uint8_t* qidx,
+                    const uint8_t* bd, const uint8_t* qm_level, const uint64_t* offset,
+                    const int16_t* pool, int64_t* sse_out, uint32_t* hash_out, int nthreads);
+
+/* Order-independent per-block checksum over levels, targets and reconstructed residual,
+ * the same mix the library computes on the device. */
+uint32_t orc_block_hash(int ncoef, const int32_t* levels, const float* targets, int npx,
+                        const int32_t* recon);
+
+#ifdef __cplusplus
+}
+#endif

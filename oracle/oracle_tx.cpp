@@ -1,0 +1,596 @@
+// CPU oracle: forward transform + quantization, quantizer state, QM.
+//
+// TEST INFRASTRUCTURE ONLY -- the checker for the HIP library, never part of the product.
+// A scalar restatement of the reference encoder (Rust) written from its source text.  Each
+// function cites the reference file:line it follows.  Integer semantics follow a Rust
+// release build (wrapping i32, arithmetic shifts): the recipe compiles with -fwrapv and
+// -ffp-contract=off so float expressions round exactly where the Rust ones do.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+
+#include "oracle.h"
+#include "../maroontree_amd/csrc/av1_tables.h"
+
+namespace {
+
+// ---------------------------------------------------------------- AV1 size/type tables
+// AV1 TxSize numbering.
+const int kTxW[19] = {4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 4, 16, 8, 32, 16, 64};
+const int kTxH[19] = {4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 4, 32, 8, 64, 16};
+enum { DCT_DCT = 0, ADST_DCT = 1, DCT_ADST = 2, ADST_ADST = 3, IDTX = 9, V_DCT = 10, H_DCT = 11 };
+enum {
+    TX_4X4 = 0, TX_8X8 = 1, TX_16X16 = 2, TX_32X32 = 3, TX_64X64 = 4, TX_4X8 = 5, TX_8X4 = 6,
+    TX_8X16 = 7, TX_16X8 = 8, TX_16X32 = 9, TX_32X16 = 10, TX_4X16 = 13, TX_16X4 = 14
+};
+
+// ---------------------------------------------------------------- constants (src/dct.rs:48-54)
+const int32_t WC4_0 = 35468, WC4_1 = 85627;
+const int32_t WC8[4] = {33410, 39410, 58981, 167963};
+const int32_t SQRT2_Q16 = 92682;
+// src/dct.rs:231-238
+const int32_t WC16[8] = {32927, 34242, 37155, 42390, 51653, 69513, 112882, 334309};
+// src/dct.rs:589
+const int32_t WC32[16] = {32808, 33127, 33780, 34802, 36248, 38203, 40796, 44224,
+                          48794, 55008, 63738, 76640, 97266, 134859, 223321, 667812};
+// src/dct.rs:3167-3169
+const int32_t RATIO_4X4_Q16 = 131072, RATIO_4X8_Q16 = 92682, RATIO_16X32_Q16 = 23170;
+
+}  // namespace
+
+// ---------------------------------------------------------------- primitives
+// mul_q16 (src/dct.rs:60)
+extern "C" int32_t orc_mul_q16(int32_t data, int32_t coeff) {
+    return (int32_t)(((int64_t)data * (int64_t)coeff + 32768) >> 16);
+}
+
+// quant_q16 (src/dct.rs:70): half-step dead zone, round to nearest.
+extern "C" int32_t orc_quant_q16(int32_t data, int32_t coeff) {
+    int64_t prod = (int64_t)data * (int64_t)coeff;
+    uint64_t mag = prod < 0 ? (uint64_t)(-(prod + 1)) + 1 : (uint64_t)prod;
+    if (mag < 32768) return 0;
+    int32_t lvl = (int32_t)((mag + 32768) >> 16);
+    return prod >= 0 ? lvl : -lvl;
+}
+
+namespace {
+
+inline int32_t mul_q16(int32_t d, int32_t c) { return orc_mul_q16(d, c); }
+
+// fmla_sqrt2 (src/dct.rs:158)
+inline int32_t fmla_sqrt2(int32_t a, int32_t b) { return mul_q16(a, SQRT2_Q16) + b; }
+
+// store_quant_target_scalar (src/dct.rs:1194)
+inline void store_quant_target(int32_t* cf, float* tf, int idx, int32_t coeff, int32_t mdc,
+                               int32_t mac) {
+    int32_t m = idx == 0 ? mdc : mac;
+    cf[idx] = orc_quant_q16(coeff, m);
+    tf[idx] = (float)coeff * (float)m * (1.0f / 65536.0f);
+}
+
+// quant_fullstep (src/dct.rs:373).  Levels as the scalar code; the target follows the x86
+// AVX2 selection that a default build runs for the 1-D family
+// (src/avx/dct.rs:172-196: f32(coeff) * f32(q) * 2^-16), see orc_fullstep_scalar_target.
+int g_fullstep_scalar_target = 0;
+inline void quant_fullstep(int32_t* cf, float* tf, int idx, int32_t coeff, int32_t q_mult) {
+    int64_t prod = (int64_t)coeff * (int64_t)q_mult;
+    if (g_fullstep_scalar_target)
+        tf[idx] = (float)prod * (1.0f / 65536.0f);
+    else
+        tf[idx] = (float)coeff * (float)q_mult * (1.0f / 65536.0f);
+    uint64_t mag = prod < 0 ? (uint64_t)(-(prod + 1)) + 1 : (uint64_t)prod;
+    if (mag < 65536) {
+        cf[idx] = 0;
+    } else {
+        int32_t l = (int32_t)((mag + 32768) >> 16);
+        cf[idx] = prod >= 0 ? l : -l;
+    }
+}
+
+// dct1d_4_i32 (src/dct.rs:171)
+void dct4(int32_t* b) {
+    int32_t t0 = b[0] + b[3], t1 = b[1] + b[2];
+    int32_t e0 = t0 + t1, e1 = t0 - t1;
+    int32_t t2 = mul_q16(b[0] - b[3], WC4_0);
+    int32_t t3 = mul_q16(b[1] - b[2], WC4_1);
+    int32_t o0 = t2 + t3, o1 = t2 - t3;
+    o0 = fmla_sqrt2(o0, o1);
+    b[0] = e0;
+    b[2] = e1;
+    b[1] = o0;
+    b[3] = o1;
+}
+
+// dct1d_8_i32 (src/dct.rs:194)
+void dct8(int32_t* b) {
+    int32_t t[8];
+    for (int i = 0; i < 4; i++) t[i] = b[i] + b[7 - i];
+    dct4(t);
+    for (int i = 0; i < 4; i++) t[4 + i] = mul_q16(b[i] - b[7 - i], WC8[i]);
+    dct4(t + 4);
+    t[4] = fmla_sqrt2(t[4], t[5]);
+    t[5] += t[6];
+    t[6] += t[7];
+    for (int i = 0; i < 4; i++) {
+        b[2 * i] = t[i];
+        b[2 * i + 1] = t[4 + i];
+    }
+}
+
+// dct1d_16_i32 (src/dct.rs:241)
+void dct16(int32_t* b) {
+    int32_t t[16];
+    for (int i = 0; i < 8; i++) {
+        t[i] = b[i] + b[15 - i];
+        t[8 + i] = b[i] - b[15 - i];
+    }
+    dct8(t);
+    for (int i = 0; i < 8; i++) t[8 + i] = mul_q16(t[8 + i], WC16[i]);
+    dct8(t + 8);
+    t[8] = fmla_sqrt2(t[8], t[9]);
+    for (int i = 9; i < 15; i++) t[i] += t[i + 1];
+    for (int i = 0; i < 8; i++) {
+        b[2 * i] = t[i];
+        b[2 * i + 1] = t[8 + i];
+    }
+}
+
+// dct1d_32_i32 (src/dct.rs:595)
+void dct32(int32_t* b) {
+    int32_t t[32];
+    for (int i = 0; i < 16; i++) {
+        t[i] = b[i] + b[31 - i];
+        t[16 + i] = b[i] - b[31 - i];
+    }
+    dct16(t);
+    for (int i = 0; i < 16; i++) t[16 + i] = mul_q16(t[16 + i], WC32[i]);
+    dct16(t + 16);
+    t[16] = fmla_sqrt2(t[16], t[17]);
+    for (int i = 17; i < 31; i++) t[i] += t[i + 1];
+    for (int i = 0; i < 16; i++) {
+        b[2 * i] = t[i];
+        b[2 * i + 1] = t[16 + i];
+    }
+}
+
+// fwd_adst{4,8,16}_1d (src/dct.rs:3186, :316, :513): Q12 matrix, i64 accumulation.
+void fadst(int n, const int32_t* in, int32_t* out) {
+    for (int i = 0; i < n; i++) {
+        int64_t acc = 0;
+        for (int j = 0; j < n; j++) {
+            int32_t m = n == 4 ? MT_FADST4_Q12[i][j] : n == 8 ? MT_FADST8_Q12[i][j] : MT_FADST16_Q12[i][j];
+            acc += (int64_t)m * (int64_t)in[j];
+        }
+        out[i] = (int32_t)((acc + 2048) >> 12);
+    }
+}
+
+void dct_n(int n, int32_t* b) {
+    switch (n) {
+        case 4: dct4(b); break;
+        case 8: dct8(b); break;
+        case 16: dct16(b); break;
+        case 32: dct32(b); break;
+        default: abort();
+    }
+}
+
+// One 1-D forward kernel of a 2-D pass: DCT in place, or ADST through a copy.
+void fwd_1d(bool adst, int n, int32_t* b) {
+    if (adst) {
+        int32_t tmp[32];
+        fadst(n, b, tmp);
+        memcpy(b, tmp, sizeof(int32_t) * n);
+    } else {
+        dct_n(n, b);
+    }
+}
+
+// Post-scale applied to the second-pass output before quantization.
+enum Post { POST_NONE, POST_INV_SQRT2, POST_HALF, POST_SHR6, POST_32X32 };
+
+inline int32_t post_scale(Post p, int32_t v) {
+    switch (p) {
+        case POST_NONE: return v;
+        case POST_INV_SQRT2: return mul_q16(v, 46341);  // src/dct.rs:806
+        case POST_HALF: return mul_q16(v, 32768);       // src/dct.rs:551
+        case POST_SHR6: return (v + (1 << 5)) >> 6;     // src/dct.rs:3158
+        case POST_32X32: {                              // src/dct.rs:730
+            int64_t prod = (int64_t)v * 16384;
+            return (int32_t)((prod + (1ll << 21)) >> 22);
+        }
+    }
+    return v;
+}
+
+// The column-first forward shape shared by dct4x4 (src/dct.rs:3076), dct4x8 (:3105),
+// dct4x16 (:2977), dct16x4 (:2952), dct8x8 (:279), dct8x16 (:1161), dct16x8 (:3033),
+// dct16x16 (:557), dct16x32 (:3134), dct32x32 (:705) and every ADST/DCT hybrid of those
+// sizes (:812-978, :1222-1552): first pass down each of the W columns (height H), then
+// across each of the H rows (width W); coefficient (fx, fy) stored at fx*H + fy.
+void fwd_col_first(const int32_t* in, int W, int H, bool col_adst, bool row_adst, int pre_shift,
+                   Post post, int32_t mdc, int32_t mac, int32_t* cf, float* tf) {
+    int32_t tmp[1024];
+    for (int col = 0; col < W; col++) {
+        int32_t c[32];
+        for (int row = 0; row < H; row++) c[row] = (int32_t)((uint32_t)in[row * W + col] << pre_shift);
+        fwd_1d(col_adst, H, c);
+        for (int fy = 0; fy < H; fy++) tmp[fy * W + col] = c[fy];
+    }
+    for (int fy = 0; fy < H; fy++) {
+        int32_t r[32];
+        for (int col = 0; col < W; col++) r[col] = tmp[fy * W + col];
+        fwd_1d(row_adst, W, r);
+        for (int fx = 0; fx < W; fx++)
+            store_quant_target(cf, tf, fx * H + fy, post_scale(post, r[fx]), mdc, mac);
+    }
+}
+
+// The row-first forward shape of dct8x4 (src/dct.rs:3253), the 8x4 ADST family
+// (src/dct.rs:3284) and dct32x16 (src/dct.rs:3342): first across each of the H rows
+// (width W), then down each of the W frequency columns (height H).
+void fwd_row_first(const int32_t* in, int W, int H, bool col_adst, bool row_adst, int pre_shift,
+                   Post post, int32_t mdc, int32_t mac, int32_t* cf, float* tf) {
+    int32_t tmp[1024];
+    for (int row = 0; row < H; row++) {
+        int32_t c[32];
+        for (int col = 0; col < W; col++) c[col] = (int32_t)((uint32_t)in[row * W + col] << pre_shift);
+        fwd_1d(row_adst, W, c);
+        for (int fx = 0; fx < W; fx++) tmp[fx * H + row] = c[fx];
+    }
+    for (int fx = 0; fx < W; fx++) {
+        int32_t r[32];
+        for (int row = 0; row < H; row++) r[row] = tmp[fx * H + row];
+        fwd_1d(col_adst, H, r);
+        for (int fy = 0; fy < H; fy++)
+            store_quant_target(cf, tf, fx * H + fy, post_scale(post, r[fy]), mdc, mac);
+    }
+}
+
+// f1ddct4x4_quant_t_direct (src/dct.rs:391), f1ddct8x8_quant_t_direct (src/dct.rs:328),
+// f1ddct_rect_quant_t_direct (src/dct.rs:1044): DCT along one axis, identity along the
+// other, per-length scale, then the full-step quantizer.
+void fwd_1d_family(const int32_t* in, int W, int H, bool vertical, int32_t mdc, int32_t mac,
+                   int32_t* cf, float* tf) {
+    int32_t f[128];
+    auto scale = [&](int n, int32_t v) -> int32_t {
+        if (W == 4 && H == 4) return mul_q16(v * 2, SQRT2_Q16);  // src/dct.rs:406
+        if (n == 8) return v * 2;                               // src/dct.rs:344, :1063
+        return mul_q16(v, SQRT2_Q16);                           // 16-point, src/dct.rs:1072
+    };
+    if (vertical) {
+        for (int x = 0; x < W; x++) {
+            int32_t col[16];
+            for (int y = 0; y < H; y++) col[y] = in[y * W + x];
+            dct_n(H, col);
+            for (int y = 0; y < H; y++) f[y * W + x] = scale(H, col[y]);
+        }
+    } else {
+        for (int y = 0; y < H; y++) {
+            int32_t row[16];
+            for (int x = 0; x < W; x++) row[x] = in[y * W + x];
+            dct_n(W, row);
+            for (int x = 0; x < W; x++) f[y * W + x] = scale(W, row[x]);
+        }
+    }
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            int rc = y + x * H;
+            quant_fullstep(cf, tf, rc, f[y * W + x], rc == 0 ? mdc : mac);
+        }
+}
+
+// identity_quant_t_direct (src/dct.rs:1586): integer division by the dequant step.
+void fwd_identity(const int32_t* in, int W, int H, int32_t dc_q, int32_t ac_q, int32_t* cf,
+                  float* tf) {
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            int rc = y + x * H;
+            int32_t qd = rc == 0 ? dc_q : ac_q;
+            int32_t num = in[y * W + x] * 8;
+            tf[rc] = (float)num / (float)qd;
+            int32_t am = (int32_t)(num < 0 ? -(uint32_t)num : (uint32_t)num);
+            if (am < qd) {
+                cf[rc] = 0;
+            } else {
+                int32_t level = (am + qd / 2) / qd;
+                cf[rc] = num < 0 ? -level : level;
+            }
+        }
+}
+
+// dct64_basis (src/dct.rs:4132): 8x the orthonormal DCT-II basis, libm cos, f64.
+const double* dct64_basis() {
+    static double b[32 * 64];
+    static bool init = false;
+    if (!init) {
+        for (int k = 0; k < 32; k++) {
+            double a = k == 0 ? std::sqrt(1.0 / 64.0) : std::sqrt(2.0 / 64.0);
+            for (int i = 0; i < 64; i++)
+                b[k * 64 + i] = a * std::cos(M_PI * (double)(2 * i + 1) * (double)k / 128.0);
+        }
+        init = true;
+    }
+    return b;
+}
+
+// Rust f64::round: half away from zero.
+inline int32_t round_away(double v) { return (int32_t)std::round(v); }
+
+// dct64x64_coeffs (src/dct.rs:4155): vertical pass accumulated over y, then an ordered sum
+// over x; out[u*32 + v].
+void dct64x64_coeffs(const int32_t* in, int32_t* out) {
+    const double* b = dct64_basis();
+    static thread_local double tmp[32 * 64];
+    for (int v = 0; v < 32; v++) {
+        double* row = tmp + v * 64;
+        for (int x = 0; x < 64; x++) row[x] = 0.0;
+        for (int y = 0; y < 64; y++) {
+            double w = b[v * 64 + y];
+            for (int x = 0; x < 64; x++) row[x] += w * (double)in[y * 64 + x];
+        }
+    }
+    for (int v = 0; v < 32; v++) {
+        const double* row = tmp + v * 64;
+        for (int u = 0; u < 32; u++) {
+            double s = 0.0;
+            for (int x = 0; x < 64; x++) s += row[x] * b[u * 64 + x];
+            out[u * 32 + v] = round_away(s * 8.0);
+        }
+    }
+}
+
+int qm_offset(int w, int h) {  // src/quant.rs:59
+    if (w == 4 && h == 4) return 0;
+    if (w == 8 && h == 8) return 16;
+    if (w == 16 && h == 16) return 80;
+    if (w == 32 && h == 32) return 336;
+    if (w == 4 && h == 8) return 1360;
+    if (w == 8 && h == 4) return 1392;
+    if (w == 8 && h == 16) return 1424;
+    if (w == 16 && h == 8) return 1552;
+    if (w == 16 && h == 32) return 1680;
+    if (w == 32 && h == 16) return 2192;
+    if (w == 4 && h == 16) return 2704;
+    if (w == 16 && h == 4) return 2768;
+    if (w == 8 && h == 32) return 2832;
+    if (w == 32 && h == 8) return 3088;
+    abort();
+}
+
+// FORWARD_QM_SCALES (src/dct.rs:93): weight = (1024 + iwt/2) / iwt, as f32 / 32.
+inline float forward_qm_scale(int iwt) {
+    if (iwt <= 0) return 0.0f;
+    int weight = (1024 + iwt / 2) / iwt;
+    return (float)weight / 32.0f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- QM and quantizer state
+extern "C" float orc_fast_round(float x) {
+    // FastRound (src/util.rs:35-59): nudge by one ulp, then the 1.5*2^23 magic round.
+    const float MAGIC = (float)((1u << 23) + (1u << 22));
+    uint32_t bits;
+    memcpy(&bits, &x, 4);
+    bits += 1;
+    float y;
+    memcpy(&y, &bits, 4);
+    volatile float s = y + MAGIC;
+    return s - MAGIC;
+}
+
+extern "C" void orc_apply_qmatrix(int32_t* levels, float* targets, const uint8_t* iwt, int n,
+                                  int scalar_fastround) {
+    for (int i = 0; i < n; i++) {
+        float t = targets[i] * forward_qm_scale(iwt[i]);
+        targets[i] = t;
+        if (scalar_fastround) {
+            // apply_qmatrix (src/dct.rs:111)
+            levels[i] = std::fabs(t) < 0.5f ? 0 : (int32_t)orc_fast_round(t);
+        } else {
+            // apply_qmatrix_avx2 (src/avx/qmatrix.rs:57-66): truncate, then one step away
+            // from zero when the exact fraction reaches 0.5.
+            float tr = std::trunc(t);
+            float frac = std::fabs(t - tr);
+            float step = frac >= 0.5f ? (t < 0 || std::signbit(t) ? -1.0f : 1.0f) : 0.0f;
+            levels[i] = (int32_t)(tr + step);
+        }
+    }
+}
+
+extern "C" int orc_qm_weight(int level, int chroma, int rc, int w, int h) {
+    if (level >= 15) return 32;
+    return MT_IQM[level][chroma ? 1 : 0][qm_offset(w, h) + rc];
+}
+
+extern "C" int orc_dequant_step(const orc_quant* q, int rc, int w, int h) {
+    int base = rc == 0 ? q->dc : q->ac;  // src/quant.rs:281
+    if (q->qm_level >= 15) return base;
+    int weight = orc_qm_weight(q->qm_level, q->qm_chroma, rc, w, h);
+    return (base * weight + 16) >> 5;
+}
+
+static void itx_clips(int bd, orc_quant* q) {  // src/quant.rs:243
+    if (bd <= 8) {
+        q->rmin = q->cmin = -32768;
+        q->rmax = q->cmax = 32767;
+        q->cf_max = 32767;
+    } else {
+        int32_t row_max = (1 << (bd + 7)) - 1;
+        int32_t col_max = (1 << (bd + 5)) - 1;
+        q->rmin = ~row_max;
+        q->rmax = row_max;
+        q->cmin = ~col_max;
+        q->cmax = col_max;
+        q->cf_max = row_max;
+    }
+}
+
+static int bd_index(int bd) { return bd == 10 ? 1 : bd == 12 ? 2 : 0; }  // src/quant.rs:104
+
+extern "C" void orc_quant_new(int base_q_idx, int bd, int qm_level, orc_quant* out) {
+    // Quant::new_with_qm (src/quant.rs:354); luma DC delta -8 (src/quant.rs:350)
+    memset(out, 0, sizeof(*out));
+    itx_clips(bd, out);
+    int dc_idx = std::min(255, std::max(0, base_q_idx - 8));
+    out->dc = MT_DC_QLOOKUP[bd_index(bd)][dc_idx];
+    out->ac = MT_AC_QLOOKUP[bd_index(bd)][base_q_idx];
+    out->q_mult_dc = (int32_t)std::round(65536.0 / (double)out->dc);
+    out->q_mult_ac = (int32_t)std::round(65536.0 / (double)out->ac);
+    out->qm_level = std::min(qm_level, 15);
+    out->qm_chroma = 0;
+    out->qidx = base_q_idx;
+    out->bd = bd;
+}
+
+extern "C" void orc_quant_new_chroma(int base_q_idx, int dc_delta, int ac_delta, int bd,
+                                     int qm_level, orc_quant* out) {
+    // Quant::new_chroma_with_delta_qm (src/quant.rs:394)
+    memset(out, 0, sizeof(*out));
+    int dc_idx = std::min(255, std::max(0, base_q_idx + dc_delta));
+    int ac_idx = std::min(255, std::max(0, base_q_idx + ac_delta));
+    itx_clips(bd, out);
+    out->dc = MT_DC_QLOOKUP[bd_index(bd)][dc_idx];
+    out->ac = MT_AC_QLOOKUP[bd_index(bd)][ac_idx];
+    out->q_mult_dc = (int32_t)std::round(65536.0 / (double)out->dc);
+    out->q_mult_ac = (int32_t)std::round(65536.0 / (double)out->ac);
+    out->qm_level = std::min(qm_level, 15);
+    out->qm_chroma = 1;
+    out->qidx = ac_idx;
+    out->bd = bd;
+}
+
+extern "C" int orc_qm_level_law(int base_q_idx, int sub) {
+    // qm_level_law (src/quant.rs:137-154)
+    bool subsampled = sub != 0;
+    const float HI = 48.0f, LO = 128.0f;
+    float qi = (float)base_q_idx;
+    if (qi <= 20.0f) return subsampled ? 8 : 15;
+    if (qi <= HI) {
+        float top = subsampled ? 8.0f : 15.0f;
+        volatile float num = (8.0f - top) * (qi - 20.0f);
+        volatile float v = top + num / (HI - 20.0f);
+        return (int)(uint8_t)orc_fast_round(v);
+    }
+    if (qi >= LO) return 0;
+    volatile float num = 8.0f * (LO - qi);
+    volatile float v = num / (LO - HI);
+    return (int)(uint8_t)orc_fast_round(v);
+}
+
+// ---------------------------------------------------------------- size helpers
+extern "C" int orc_tx_w(int s) { return (s >= 0 && s < 19) ? kTxW[s] : 0; }
+extern "C" int orc_tx_h(int s) { return (s >= 0 && s < 19) ? kTxH[s] : 0; }
+extern "C" int orc_tx_ncoef(int s) {
+    if (s < 0 || s >= 19) return 0;
+    int w = std::min(kTxW[s], 32), h = std::min(kTxH[s], 32);
+    return w * h;
+}
+
+extern "C" int orc_tx_supported(int s, int t) {
+    // The reference's forward coverage (src/dct.rs:2416-2470, :2624-2950).
+    const bool hyb = t == DCT_DCT || t == ADST_ADST || t == ADST_DCT || t == DCT_ADST;
+    const bool all7 = hyb || t == V_DCT || t == H_DCT || t == IDTX;
+    switch (s) {
+        case TX_4X4: case TX_8X8: case TX_8X16: case TX_16X8: return all7;
+        case TX_4X8: case TX_8X4: return hyb;
+        case TX_16X16: return hyb || t == IDTX;
+        case TX_4X16: case TX_16X4: case TX_16X32: case TX_32X16: case TX_32X32: case TX_64X64:
+            return t == DCT_DCT;
+        default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------- forward dispatch
+extern "C" int orc_fwd_txfm(int s, int t, const int32_t* resid, const orc_quant* q,
+                            int32_t* levels, float* targets) {
+    if (!orc_tx_supported(s, t)) return -1;
+    const int W = kTxW[s], H = kTxH[s];
+    const bool col_adst = t == ADST_ADST || t == ADST_DCT;  // ADST_DCT = vertical ADST
+    const bool row_adst = t == ADST_ADST || t == DCT_ADST;
+    int qw = W, qh = H;  // QM geometry
+
+    if (t == IDTX) {
+        // DctDispatch::idtx*_t (src/dct.rs:2910-2948): dequant steps, no QM.
+        fwd_identity(resid, W, H, q->dc, q->ac, levels, targets);
+        return 0;
+    }
+    if (t == V_DCT || t == H_DCT) {
+        // dispatch_flat_methods (src/dct.rs:2507): FlatDct multipliers, no ratio, no QM.
+        fwd_1d_family(resid, W, H, t == V_DCT, q->q_mult_dc, q->q_mult_ac, levels, targets);
+        return 0;
+    }
+
+    int32_t mdc = q->q_mult_dc, mac = q->q_mult_ac;
+    switch (s) {
+        case TX_4X4:  // dct4x4_t (src/dct.rs:2624), dispatch_qm_ratio_methods (:2463)
+            mdc = mul_q16(mdc, RATIO_4X4_Q16);
+            mac = mul_q16(mac, RATIO_4X4_Q16);
+            fwd_col_first(resid, 4, 4, col_adst, row_adst, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_4X8:  // dct4x8_t (src/dct.rs:2640)
+            mdc = mul_q16(mdc, RATIO_4X8_Q16);
+            mac = mul_q16(mac, RATIO_4X8_Q16);
+            fwd_col_first(resid, 4, 8, col_adst, row_adst, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_8X4:  // dct8x4_t / adst8x4_t family (src/dct.rs:2656-2721)
+            mdc = mul_q16(mdc, RATIO_4X8_Q16);
+            mac = mul_q16(mac, RATIO_4X8_Q16);
+            fwd_row_first(resid, 8, 4, col_adst, row_adst, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_4X16:  // dct4x16_t (src/dct.rs:2723)
+            fwd_col_first(resid, 4, 16, false, false, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_16X4:  // dct16x4_t (src/dct.rs:2737)
+            fwd_col_first(resid, 16, 4, false, false, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_8X8:  // dct8x8_t (src/dct.rs:2751), adst8x8 family (:1222-1293)
+            fwd_col_first(resid, 8, 8, col_adst, row_adst, 0, POST_NONE, mdc, mac, levels, targets);
+            break;
+        case TX_8X16:  // dct8x16_t (src/dct.rs:2765), adst8x16 family (:812, :922, :951)
+            fwd_col_first(resid, 8, 16, col_adst, row_adst, 0, POST_INV_SQRT2, mdc, mac, levels, targets);
+            break;
+        case TX_16X8:  // dct16x8_t (src/dct.rs:2779), adst16x8 family (:837, :862, :892)
+            fwd_col_first(resid, 16, 8, col_adst, row_adst, 0, POST_INV_SQRT2, mdc, mac, levels, targets);
+            break;
+        case TX_16X16:  // dct16x16_t (src/dct.rs:2793), adst16x16 family (:1295-1396)
+            fwd_col_first(resid, 16, 16, col_adst, row_adst, 0, POST_HALF, mdc, mac, levels, targets);
+            break;
+        case TX_16X32:  // dct16x32_t (src/dct.rs:2807)
+            mdc = mul_q16(mdc, RATIO_16X32_Q16);
+            mac = mul_q16(mac, RATIO_16X32_Q16);
+            fwd_col_first(resid, 16, 32, false, false, 6, POST_SHR6, mdc, mac, levels, targets);
+            break;
+        case TX_32X16:  // dct32x16_t (src/dct.rs:2823)
+            mdc = mul_q16(mdc, RATIO_16X32_Q16);
+            mac = mul_q16(mac, RATIO_16X32_Q16);
+            fwd_row_first(resid, 32, 16, false, false, 6, POST_SHR6, mdc, mac, levels, targets);
+            break;
+        case TX_32X32:  // dct32x32_t (src/dct.rs:2862)
+            fwd_col_first(resid, 32, 32, false, false, 6, POST_32X32, mdc, mac, levels, targets);
+            break;
+        case TX_64X64: {  // dct64x64_t (src/dct.rs:2847): f64 core, 32x32 QM
+            int32_t coeffs[1024];
+            dct64x64_coeffs(resid, coeffs);
+            for (int i = 0; i < 1024; i++) store_quant_target(levels, targets, i, coeffs[i], mdc, mac);
+            qw = 32;
+            qh = 32;
+            break;
+        }
+        default:
+            return -1;
+    }
+    // apply_qmatrix_result (src/dct.rs:132)
+    if (q->qm_level < 15) {
+        const uint8_t* row = &MT_IQM[q->qm_level][q->qm_chroma ? 1 : 0][qm_offset(qw, qh)];
+        orc_apply_qmatrix(levels, targets, row, qw * qh, 0);
+    }
+    return 0;
+}
+
+extern "C" void orc_dct1d(int n, int32_t* buf) { dct_n(n, buf); }
+extern "C" void orc_fadst1d(int n, const int32_t* in, int32_t* out) { fadst(n, in, out); }
+extern "C" void orc_set_fullstep_scalar_target(int on) { g_fullstep_scalar_target = on; }
