@@ -32,6 +32,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="wall-time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the chain has no multi-GPU path); "
@@ -46,6 +48,16 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calib_copy:
+        # a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the
+        # chain's counter values to bytes
+        src = torch.ones(args.calib_copy << 18, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": args.calib_copy << 20, "copies": 5}))
+        return
     mt.init(0)
 
     n = args.blocks
@@ -81,13 +93,26 @@ def main():
     # dominant kernel: largest summed time over the timed region
     slot = max((k for k in prof if k != "bucket"), key=lambda k: prof[k][0])
     k_ms = prof[slot][0] / max(prof[slot][1], 1)
-    sel = blocks["tx_size"] == {v: k for k, v in zip(mt.TX_SIZES_REF, [mt.TX_NAMES[s] for s in mt.TX_SIZES_REF])}.get(slot, -1)
     size_id = [s for s in mt.TX_SIZES_REF if mt.TX_NAMES[s] == slot][0]
+    w, h = mt.tx_width(size_id), mt.tx_height(size_id)
     nblk = int((blocks["tx_size"] == size_id).sum())
-    bpb = 2 * mt.tx_width(size_id) * mt.tx_height(size_id) + 8 + 16  # int16 in, desc, result
-    alg_bytes = nblk * bpb
+    alg_bytes = nblk * (2 * w * h + 8 + 16)  # int16 residual in, descriptor, result
     achieved = alg_bytes / (k_ms / 1000.0) / 1e9
-    del sel
+    kname = "void tx64_kernel<0>(KArgs)" if w == 64 else f"void tx_tile_kernel<{w}, {h}, 0>(KArgs, int)"
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "tx_chain_2p20_pmc.json")
+    if os.path.exists(pmc_path) and n == 1 << 20:
+        pmc = json.load(open(pmc_path))
+        if kname in pmc["kernels"]:
+            traffic = pmc["kernels"][kname]["traffic_bytes"]
+    compute = None
+    if w == 64:
+        # f64 work of the reference's separable 64-point forward, one mul and one add per
+        # term (no fused multiply-add): 32x64x64 vertical + 32x32x64 horizontal terms
+        flops = nblk * 2 * (32 * 64 * 64 + 32 * 32 * 64)
+        tf = flops / (k_ms / 1000.0) / 1e12
+        compute = {"bound": "valu_f64", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                   "frac": round(tf / 78.6, 4), "flops_per_launch": flops}
 
     line = {
         "metric": "Mpx/s + RD candidates/s on 1 MI355X vs host CPU; BD-rate of pruned vs full search",
@@ -119,10 +144,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "bytes_per_launch": alg_bytes,
             "note": "algorithmic bytes = 2*W*H int16 in + 8 B descriptor + 16 B result per block; "
-                    "the chain is integer-VALU bound, not HBM bound",
+                    "the kernel is VALU bound (see roofline_compute), not HBM bound",
         },
+        "roofline_compute": compute,
     }
 
     if not args.no_cpu:

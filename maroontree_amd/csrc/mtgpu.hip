@@ -46,7 +46,6 @@ __constant__ int32_t c_fadst4[16];
 __constant__ int32_t c_fadst8[64];
 __constant__ int32_t c_fadst16[256];
 __device__ uint8_t g_iqm[MT_QM_LEVELS][2][MT_QM_USED_SIZE];
-__device__ double g_dct64_basis[32 * 64];
 
 enum TxKind { KD = 0, KA = 1, KI = 2 };  // DCT, ADST, identity
 
@@ -328,15 +327,16 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
     const int tid = threadIdx.x;
 
     // virtual tile space: chain mode walks the 7 type classes of this size
+    __shared__ uint32_t s_cnt[7], s_st[7];
+    if (MODE == MODE_CHAIN && tid < 7) {
+        s_cnt[tid] = a.cls_count[size_slot_idx * 7 + tid];
+        s_st[tid] = a.cls_start[size_slot_idx * 7 + tid];
+    }
+    __syncthreads();
     int total_tiles = 0;
-    uint32_t cnt[7], st[7];
     if (MODE == MODE_CHAIN) {
 #pragma unroll
-        for (int t = 0; t < 7; t++) {
-            cnt[t] = a.cls_count[size_slot_idx * 7 + t];
-            st[t] = a.cls_start[size_slot_idx * 7 + t];
-            total_tiles += (int)((cnt[t] + B - 1) / B);
-        }
+        for (int t = 0; t < 7; t++) total_tiles += (int)((s_cnt[t] + B - 1) / B);
     } else {
         total_tiles = (a.n + B - 1) / B;
     }
@@ -347,14 +347,14 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
             if (MODE == MODE_CHAIN) {
                 int rem = tile, t = 0;
                 for (; t < 7; t++) {
-                    const int nt = (int)((cnt[t] + B - 1) / B);
+                    const int nt = (int)((s_cnt[t] + B - 1) / B);
                     if (rem < nt) break;
                     rem -= nt;
                 }
                 const uint32_t first = (uint32_t)rem * B;
                 s_type = type_of_slot(t);
-                s_nb = (int)min((uint32_t)B, cnt[t] - first);
-                gidx[0] = st[t] + first;  // position in `order` of the first block
+                s_nb = (int)min((uint32_t)B, s_cnt[t] - first);
+                gidx[0] = s_st[t] + first;  // position in `order` of the first block
             } else {
                 s_type = a.type;
                 s_nb = min(B, a.n - tile * B);
@@ -656,162 +656,185 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
 // TX_64X64: the reference codes the low 32x32 of a 64-point DCT.  Forward = f64 separable
 // basis (dct64x64_coeffs, src/dct.rs:4155: vertical pass accumulated over y, then an
 // ordered sum over x, round half away from zero of 8x the sum); quantizer with the 32x32
-// QM; inverse = idct_dequant_64x64 (src/idct.rs:2677).  One 256-thread workgroup per block.
+// QM; inverse = idct_dequant_64x64 (src/idct.rs:2677).
+//
+// One wave per block.  Lane x holds column x in f64 registers and runs the 32 vertical
+// chains (basis rows come through the scalar unit from constant memory); lanes (u, v parity)
+// hold basis row u and run 16 horizontal chains each over the LDS-staged vertical output.
+// Every f64 product and sum is rounded separately in the reference's order.
+__constant__ double c_dct64_basis[32 * 64];    // [k][i]
+__constant__ double c_dct64_basis_t[64 * 32];  // [i][k], the vertical pass's scalar stream
+
 template <int MODE>
-__global__ void __launch_bounds__(256) tx64_kernel(KArgs a) {
-    constexpr int RS = 65;
-    __shared__ int32_t work[64 * RS];
-    __shared__ double vt[32 * 64];
-    __shared__ int16_t in16[MODE == MODE_CHAIN ? 4096 : 1];
-    __shared__ BlkParam prm;
-    __shared__ uint32_t s_g, s_off;
+__global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
+    __shared__ double vt[32 * 64];          // vertical-pass output; then the row-pass buffer
+    __shared__ int32_t coef[32 * 33];       // dequantized coded coefficients [fy][fx]
     __shared__ unsigned long long s_sse;
     __shared__ uint32_t s_nnz, s_hash;
-    __shared__ int s_n;
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        if (MODE == MODE_CHAIN) {
-            const int sl = 12 * 7;  // 64x64, DCT_DCT
-            s_n = (int)a.cls_count[sl];
-        } else {
-            s_n = a.n;
-        }
-    }
-    __syncthreads();
-    const int nblocks = s_n;
+    int32_t* rowbuf = reinterpret_cast<int32_t*>(vt);  // 32 x 64 int32 after the forward
+    const int lane = threadIdx.x;
+    const int nblocks = MODE == MODE_CHAIN ? (int)a.cls_count[12 * 7] : a.n;
     const uint32_t chain_start = MODE == MODE_CHAIN ? a.cls_start[12 * 7] : 0;
+
     for (int blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
-        if (tid == 0) {
-            if (MODE == MODE_CHAIN) {
-                const uint32_t g = a.order[chain_start + blk];
-                const mt_txblk d = a.blocks[g];
-                s_g = g;
-                s_off = d.offset;
-                param_from_desc(d, prm);
-            } else {
-                s_g = blk;
-                s_off = blk;
-                param_from_quant(a.quants[blk], prm);
-            }
+        BlkParam prm;
+        uint32_t g, off;
+        if (MODE == MODE_CHAIN) {
+            g = a.order[chain_start + blk];
+            const mt_txblk d = a.blocks[g];
+            off = d.offset;
+            param_from_desc(d, prm);
+        } else {
+            g = blk;
+            off = blk;
+            param_from_quant(a.quants[blk], prm);
+        }
+        if (lane == 0) {
             s_sse = 0;
             s_nnz = 0;
             s_hash = 0;
         }
-        __syncthreads();
-        const uint32_t g = s_g, off = s_off;
-        for (int i = tid; i < 64 * RS; i += 256) work[i] = 0;
-        __syncthreads();
+        uint32_t nnz = 0, hsh = 0;
+        const bool qm = prm.qm_level < 15;
+        const uint8_t* qmrow = &g_iqm[qm ? prm.qm_level : 0][prm.qm_chroma][336];
         if (MODE != MODE_INV) {
-            for (int p = tid; p < 4096; p += 256) {
-                int32_t v;
-                if (MODE == MODE_CHAIN) {
-                    const int16_t s = a.pool16[(size_t)off + p];
-                    in16[p] = s;
-                    v = s;
-                } else {
-                    v = a.in32[(size_t)g * 4096 + p];
-                }
-                work[(p >> 6) * RS + (p & 63)] = v;
-            }
+            // The block's residual is staged in LDS (aliasing vt) so the y loop needs no
+            // register array; loads are issued for the whole block at once.
+            int32_t* inbuf = reinterpret_cast<int32_t*>(vt);
+#pragma unroll 16
+            for (int y = 0; y < 64; y++)
+                inbuf[y * 64 + lane] = MODE == MODE_CHAIN ? (int32_t)a.pool16[(size_t)off + y * 64 + lane]
+                                                          : a.in32[(size_t)g * 4096 + y * 64 + lane];
             __syncthreads();
-            // vertical pass: vt[v][x] = sum_y basis[v][y] * in[y][x], y ascending
+            // vertical pass: vt[v][x] = sum_y basis[v][y] * in[y][x], y ascending.  Lane x
+            // carries the 32 chains of column x; basis[.][y] is wave-uniform and contiguous in
+            // the transposed copy, so it arrives through the scalar unit.
             {
-                const int x = tid & 63;
-                for (int v = tid >> 6; v < 32; v += 4) {
-                    double acc = 0.0;
-                    const double* bv = g_dct64_basis + v * 64;
-                    for (int y = 0; y < 64; y++)
-                        acc = __dadd_rn(acc, __dmul_rn(bv[y], (double)work[y * RS + x]));
-                    vt[v * 64 + x] = acc;
+                double acc[32];
+#pragma unroll
+                for (int v = 0; v < 32; v++) acc[v] = 0.0;
+#pragma unroll 2
+                for (int y = 0; y < 64; y++) {
+                    const double c = (double)inbuf[y * 64 + lane];
+#pragma unroll
+                    for (int v = 0; v < 32; v++)
+                        acc[v] = __dadd_rn(acc[v], __dmul_rn(c_dct64_basis_t[y * 32 + v], c));
                 }
+                __syncthreads();  // every lane is done reading the staged residual
+#pragma unroll
+                for (int v = 0; v < 32; v++) vt[v * 64 + lane] = acc[v];
             }
             __syncthreads();
-            // horizontal pass + quantization of coefficient (u, v), rc = u*32 + v
-            for (int i = tid; i < 1024; i += 256) {
-                const int u = i & 31, v = i >> 5;
-                const double* bu = g_dct64_basis + u * 64;
-                double s = -0.0;
-                for (int x = 0; x < 64; x++) s = __dadd_rn(s, __dmul_rn(vt[v * 64 + x], bu[x]));
-                const int32_t coeff = (int32_t)round(__dmul_rn(s, 8.0));
-                const int rc = u * 32 + v;
-                const int32_t m = rc == 0 ? prm.qdc : prm.qac;
-                int32_t lvl = quant_q16(coeff, m);
-                float tgt = q16_target(coeff, m);
-                const bool qm = prm.qm_level < 15;
-                int iwt = 32;
-                if (qm) {
-                    iwt = g_iqm[prm.qm_level][prm.qm_chroma][336 + rc];
-                    lvl = qm_apply(tgt, iwt);
+            // horizontal pass + quantization: lane -> (u = lane % 32, v parity = lane / 32),
+            // 16 chains per lane summed over x ascending; the vertical output is read as LDS
+            // broadcasts, basis[u][x] from one 256-byte row of the transposed copy per x.
+            {
+                const int u = lane & 31, vp = lane >> 5;
+                double acc[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) acc[k] = -0.0;
+#pragma unroll 4
+                for (int x = 0; x < 64; x++) {
+                    const double bu = c_dct64_basis_t[x * 32 + u];
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        acc[k] = __dadd_rn(acc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
                 }
-                if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
-                    const size_t o = (MODE == MODE_FWD ? (size_t)g * 1024 : (size_t)off) + rc;
-                    a.levels[o] = lvl;
-                    a.targets[o] = tgt;
-                }
-                if (MODE == MODE_CHAIN) {
-                    int32_t step = rc == 0 ? prm.dc : prm.ac;
-                    if (qm) step = (step * iwt + 16) >> 5;
-                    atomicAdd(&s_hash, coef_hash(rc, lvl, tgt));
-                    if (lvl) atomicAdd(&s_nnz, 1u);
-                    work[v * RS + u] = dequant(lvl, step, 2, prm.cf_max);
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    coef[(2 * k + vp) * 33 + u] = (int32_t)round(__dmul_rn(acc[k], 8.0));
+                // quantize the lane's 16 coefficients, leave their dequantized values in place
+#pragma unroll 1
+                for (int k = 0; k < 16; k++) {
+                    const int v = 2 * k + vp;
+                    const int32_t coeff = coef[v * 33 + u];
+                    const int rc = u * 32 + v;
+                    const int32_t m = rc == 0 ? prm.qdc : prm.qac;
+                    int32_t lvl = quant_q16(coeff, m);
+                    float tgt = q16_target(coeff, m);
+                    const int iwt = qm ? qmrow[rc] : 32;
+                    if (qm) lvl = qm_apply(tgt, iwt);
+                    if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
+                        const size_t o = (MODE == MODE_FWD ? (size_t)g * 1024 : (size_t)off) + rc;
+                        a.levels[o] = lvl;
+                        a.targets[o] = tgt;
+                    }
+                    if (MODE == MODE_CHAIN) {
+                        int32_t step = rc == 0 ? prm.dc : prm.ac;
+                        if (qm) step = (step * iwt + 16) >> 5;
+                        hsh += coef_hash(rc, lvl, tgt);
+                        nnz += lvl != 0;
+                        coef[v * 33 + u] = dequant(lvl, step, 2, prm.cf_max);
+                    }
                 }
             }
             __syncthreads();
         } else {
-            for (int rc = tid; rc < 1024; rc += 256) {
+            for (int rc = lane; rc < 1024; rc += 64) {
                 const int u = rc >> 5, v = rc & 31;
                 int32_t step = rc == 0 ? prm.dc : prm.ac;
-                if (prm.qm_level < 15)
-                    step = (step * (int32_t)g_iqm[prm.qm_level][prm.qm_chroma][336 + rc] + 16) >> 5;
-                work[v * RS + u] = dequant(a.in32[(size_t)g * 1024 + rc], step, 2, prm.cf_max);
+                if (qm) step = (step * (int32_t)qmrow[rc] + 16) >> 5;
+                coef[v * 33 + u] = dequant(a.in32[(size_t)g * 1024 + rc], step, 2, prm.cf_max);
             }
             __syncthreads();
         }
         if (MODE != MODE_FWD) {
             // row pass over the 32 coded rows, inter-pass (t + 2) >> 2 clip
-            if (tid < 32) {
-                const int y = tid;
+            if (lane < 32) {
+                const int y = lane;
                 int32_t v[64];
 #pragma unroll
-                for (int x = 0; x < 64; x++) v[x] = x < 32 ? work[y * RS + x] : 0;
+                for (int x = 0; x < 64; x++) v[x] = x < 32 ? coef[y * 33 + x] : 0;
                 idct64<1>(v, Clip{prm.rmin, prm.rmax});
                 const Clip cc{prm.cmin, prm.cmax};
 #pragma unroll
-                for (int x = 0; x < 64; x++) work[y * RS + x] = cc((v[x] + 2) >> 2);
+                for (int x = 0; x < 64; x++) rowbuf[y * 64 + x] = cc((v[x] + 2) >> 2);
             }
             __syncthreads();
-            if (tid < 64) {
-                const int x = tid;
+            {
+                const int x = lane;
                 int32_t v[64];
 #pragma unroll
-                for (int y = 0; y < 64; y++) v[y] = y < 32 ? work[y * RS + x] : 0;
+                for (int y = 0; y < 64; y++) v[y] = y < 32 ? rowbuf[y * 64 + x] : 0;
                 idct64<1>(v, Clip{prm.cmin, prm.cmax});
-                uint32_t sse = 0, hsh = 0;
-                const int P = 1 << (prm.bd - 1), maxv = (1 << prm.bd) - 1;
+                // column x of the result goes to the (now free) vt storage; lane x only ever
+                // touches column x, so no barrier is needed before reading it back
+                int32_t* outbuf = reinterpret_cast<int32_t*>(vt);
 #pragma unroll
-                for (int y = 0; y < 64; y++) {
-                    const int32_t r = (v[y] + 8) >> 4;
-                    const int j = y * 64 + x;
-                    if (MODE == MODE_INV) {
-                        a.recon[(size_t)g * 4096 + j] = r;
-                    } else {
-                        if (a.recon) a.recon[(size_t)off + j] = r;
-                        const int32_t s = in16[j];
-                        const int32_t src = min(max(P + s, 0), maxv);
-                        const int32_t rec = min(max(P + r, 0), maxv);
-                        const int32_t d = src - rec;
-                        sse += (uint32_t)(d * d);
-                        hsh += pix_hash(j, r);
+                for (int y = 0; y < 64; y++) outbuf[y * 64 + x] = (v[y] + 8) >> 4;
+                uint32_t sse = 0;
+                const int P = 1 << (prm.bd - 1), maxv = (1 << prm.bd) - 1;
+                if (MODE == MODE_INV) {
+#pragma unroll 8
+                    for (int y = 0; y < 64; y++) a.recon[(size_t)g * 4096 + y * 64 + x] = outbuf[y * 64 + x];
+                } else {
+#pragma unroll 2
+                    for (int y0 = 0; y0 < 64; y0 += 16) {
+                        int32_t src16[16];
+#pragma unroll
+                        for (int i = 0; i < 16; i++) src16[i] = a.pool16[(size_t)off + (y0 + i) * 64 + x];
+#pragma unroll
+                        for (int i = 0; i < 16; i++) {
+                            const int j = (y0 + i) * 64 + x;
+                            const int32_t r = outbuf[j];
+                            if (a.recon) a.recon[(size_t)off + j] = r;
+                            const int32_t src = min(max(P + src16[i], 0), maxv);
+                            const int32_t rec = min(max(P + r, 0), maxv);
+                            const int32_t d = src - rec;
+                            sse += (uint32_t)(d * d);
+                            hsh += pix_hash(j, r);
+                        }
                     }
                 }
                 if (MODE == MODE_CHAIN) {
                     atomicAdd(&s_sse, (unsigned long long)sse);
                     atomicAdd(&s_hash, hsh);
+                    atomicAdd(&s_nnz, nnz);
                 }
             }
         }
         __syncthreads();
-        if (MODE == MODE_CHAIN && tid == 0) {
+        if (MODE == MODE_CHAIN && lane == 0) {
             mt_txres r;
             r.sse = (int64_t)s_sse;
             r.nnz = s_nnz;
@@ -994,8 +1017,8 @@ template <int MODE> int launch_size(int tx_size, const KArgs& a, int tiles_hint,
         case MT_TX_32X16: return launch_tiles<32, 16, MODE>(a, 10, tiles_hint, cus, s);
         case MT_TX_32X32: return launch_tiles<32, 32, MODE>(a, 11, tiles_hint, cus, s);
         case MT_TX_64X64: {
-            const int grid = grid_for(tiles_hint, cus);
-            hipLaunchKernelGGL((tx64_kernel<MODE>), dim3(grid), dim3(256), 0, s, a);
+            const int grid = std::max(1, std::min(tiles_hint, cus * 16));
+            hipLaunchKernelGGL((tx64_kernel<MODE>), dim3(grid), dim3(64), 0, s, a);
             HIP_TRY(hipGetLastError());
             return MT_OK;
         }
@@ -1055,7 +1078,11 @@ extern "C" int mt_init(int device) {
         for (int i = 0; i < 64; i++)
             basis[k * 64 + i] = a * std::cos(M_PI * (double)(2 * i + 1) * (double)k / 128.0);
     }
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_dct64_basis), basis.data(), basis.size() * sizeof(double)));
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_dct64_basis), basis.data(), basis.size() * sizeof(double)));
+    std::vector<double> basis_t(64 * 32);
+    for (int k = 0; k < 32; k++)
+        for (int i = 0; i < 64; i++) basis_t[i * 32 + k] = basis[k * 64 + i];
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_dct64_basis_t), basis_t.data(), basis_t.size() * sizeof(double)));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     st.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
