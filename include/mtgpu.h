@@ -116,6 +116,32 @@ int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* levels_d,
 int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                 int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream);
 
+/* ---- coefficient rate ------------------------------------------------------------- */
+/* One transform block to price: 32 bytes.  The rate is real_block_bits_bounded
+ * (src/rate.rs:149) against the frame-initial coefficient CDFs Cdfs::new(qctx)
+ * (src/coder/cdfs.rs:400): the exact symbols the coefficient coder would emit.
+ * eob_bin picks the eob_bin CDF as the reference's caller does (numbering: 0 16 luma,
+ * 1 16 chroma, 2 16 luma 1-D, 3 32 luma, 4 32 chroma, 5 64 luma, 6 64 chroma, 7 64 luma 1-D,
+ * 8 128 luma, 9 128 chroma, 10 128 luma 1-D, 11 256 luma, 12 256 chroma, 13 512 luma,
+ * 14 512 chroma, 15 1024 luma, 16 1024 chroma).  txtp_set: 0 none, 1 TX_4X4 intra set,
+ * 2 TX_8X8 set, 3 TX_16X16 set (txtp_mode = luma intra mode row, txtp = index in the set).
+ * flags bit 0: use skip_one / skip_zero (adaptive all-zero price) instead of txb_skip.
+ * A descriptor whose eob_bin table is not the one of the block's coded coefficient count,
+ * or whose indices are out of range, gets NaN. */
+typedef struct {
+    uint32_t offset;   /* element offset of the block's levels (ncoef, x-major) */
+    uint8_t tx_size, plane, eob_bin, skip_ctx;
+    uint8_t dcs_ctx, txtp_set, txtp_mode, txtp;
+    uint8_t qctx, flags, pad0, pad1;
+    float skip_one, skip_zero;
+    float bound;       /* early-out bound (bits); +inf to always price fully */
+    uint32_t pad2;
+} mt_rateblk;
+
+/* Bits of n blocks (bits_d[i] = +inf when the walk exceeded blocks[i].bound). */
+int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                  void* stream);
+
 /* Per-kernel timing of mt_tx_chain (no counterpart in the reference; measurement hook).
  * Between begin and end every chain launch is bracketed by HIP events on its stream;
  * end synchronizes them and returns, per slot, the summed milliseconds and launch count.

@@ -36,6 +36,10 @@ class MtTxRes(ctypes.Structure):
     _fields_ = [("sse", ctypes.c_int64), ("nnz", ctypes.c_uint32), ("hash", ctypes.c_uint32)]
 
 
+RATEBLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("plane", "u1"), ("eob_bin", "u1"),
+                          ("skip_ctx", "u1"), ("dcs_ctx", "u1"), ("txtp_set", "u1"), ("txtp_mode", "u1"),
+                          ("txtp", "u1"), ("qctx", "u1"), ("flags", "u1"), ("pad0", "u1"), ("pad1", "u1"),
+                          ("skip_one", "<f4"), ("skip_zero", "<f4"), ("bound", "<f4"), ("pad2", "<u4")])
 BLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("tx_type", "u1"), ("qidx", "u1"),
                       ("bd_qm", "u1")])
 RES_DTYPE = np.dtype([("sse", "<i8"), ("nnz", "<u4"), ("hash", "<u4")])
@@ -64,6 +68,7 @@ def lib():
         L.mt_inv_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp]
         L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
+        L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
@@ -201,6 +206,17 @@ def profile_end():
     cnt = (ctypes.c_int * n)()
     _check(lib().mt_profile_end(ms, cnt, n))
     return {PROFILE_SLOTS[i]: (ms[i], cnt[i]) for i in range(n)}
+
+
+def block_rate(blocks, levels, stream=None):
+    """Coefficient rate of n blocks: blocks is a uint8 cuda tensor of n*32 bytes
+    (RATEBLK_DTYPE records), levels an int32 cuda tensor; returns float32 bits (n,)."""
+    import torch
+    n = blocks.numel() // RATEBLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    out = torch.empty(n, dtype=torch.float32, device=blocks.device)
+    _check(lib().mt_block_rate(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(out), _stream_ptr(stream)))
+    return out
 
 
 def gen_tx_blocks(seed, n, all_sizes=False):

@@ -1,0 +1,376 @@
+// Coefficient rate of transform blocks on the GPU: the decision-side mirror of the AV1
+// coefficient coder (reference src/rate.rs:149 real_block_bits_bounded, src/cost.rs:159-262,
+// src/coeffs.rs:46 get_lo_ctx_2d) against the frame-initial CDFs Cdfs::new(qctx).
+//
+// The walk is sequential per block (every f32 addition in the reference's order), so one
+// lane prices one block and a wave prices 64 blocks at once.  The neighbour "levels" plane
+// the reference fills while walking the reverse scan needs no scratch here: at step i a
+// position holds its level byte exactly when its scan index is above i, so every context
+// is read straight from the block's levels through the inverse scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mtgpu.h"
+#include "av1_cdf_tables.h"
+#include "mt_common.h"
+
+namespace {
+
+// Costs (bits) of every symbol the rate walk prices, for one qctx.
+struct RateTables {
+    float skip[5][13][2];
+    float bt[5][2][42][4];    // CoefCostTables::bt (src/rate.rs:50)
+    float br[4][2][21][13];   // CoefCostTables::br, cumulative base-range ladders
+    float eob_base[5][2][4][3];
+    float eob_hi[5][2][11][2];
+    float dc_sign[2][3][2];
+    float eob_bin[17][11];
+    float txtp[3][13][7];     // intra tx-type sets of TX_4X4 / TX_8X8 / TX_16X16
+};
+
+// Scans (scan index -> x-major rc) and inverse scans, concatenated per scan size.
+constexpr int kScanTotal = 16 + 32 + 32 + 64 + 64 + 64 + 128 + 128 + 256 + 512 + 512 + 1024;
+
+}  // namespace
+
+__device__ RateTables g_rate[4];
+__constant__ uint16_t c_scan[kScanTotal];
+__constant__ uint16_t c_iscan[kScanTotal];
+__constant__ uint8_t c_lo_off[3][5][5];  // square, w < h, w > h (src/tables.rs:354, :456, :466)
+
+namespace {
+
+// Offset of a TxSize's scan in c_scan (64x64 prices its coded 32x32 with the 32x32 scan).
+__host__ __device__ inline int scan_offset(int s) {
+    switch (s) {
+        case MT_TX_4X4: return 0;
+        case MT_TX_4X8: return 16;
+        case MT_TX_8X4: return 48;
+        case MT_TX_8X8: return 80;
+        case MT_TX_4X16: return 144;
+        case MT_TX_16X4: return 208;
+        case MT_TX_8X16: return 272;
+        case MT_TX_16X8: return 400;
+        case MT_TX_16X16: return 528;
+        case MT_TX_16X32: return 784;
+        case MT_TX_32X16: return 1296;
+        case MT_TX_32X32: case MT_TX_64X64: return 1808;
+        default: return -1;
+    }
+}
+
+__device__ inline uint32_t uabs32(int32_t v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
+
+// level_byte (src/tables.rs:365)
+__device__ inline uint32_t level_byte(uint32_t m) {
+    return m == 0 ? 0u : m <= 2 ? m * 0x41u : (m < 15u ? m : 15u) + 0xC0u;
+}
+
+__device__ inline float golomb_bits(uint32_t v) {
+    return (float)(2 * (32 - __clz((int)(v + 1))) - 1);
+}
+
+// The 8-bit wrap of the reference's u8 level plane is irrelevant: entries stay below 256.
+__global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict__ blocks, int n,
+                                                  const int32_t* __restrict__ levels,
+                                                  float* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const mt_rateblk d = blocks[b];
+    const int s = d.tx_size;
+    const int so = s < 19 ? scan_offset(s) : -1;
+    if (so < 0 || d.plane > 1 || d.eob_bin > 16 || d.skip_ctx > 12 || d.dcs_ctx > 2 || d.qctx > 3 ||
+        d.txtp_set > 3 || d.txtp_mode > 12 || d.txtp > 6 || (s == MT_TX_64X64 && d.plane)) {
+        out[b] = __int_as_float(0x7fc00000);  // invalid descriptor: NaN
+        return;
+    }
+    const int wf = s == MT_TX_4X4 || s == MT_TX_4X8 || s == MT_TX_4X16 ? 4
+                 : s == MT_TX_8X8 || s == MT_TX_8X4 || s == MT_TX_8X16 ? 8
+                 : s == MT_TX_16X16 || s == MT_TX_16X8 || s == MT_TX_16X4 || s == MT_TX_16X32 ? 16
+                 : s == MT_TX_64X64 ? 64 : 32;
+    const int hf = s == MT_TX_4X4 || s == MT_TX_8X4 || s == MT_TX_16X4 ? 4
+                 : s == MT_TX_8X8 || s == MT_TX_4X8 || s == MT_TX_16X8 ? 8
+                 : s == MT_TX_16X16 || s == MT_TX_8X16 || s == MT_TX_4X16 || s == MT_TX_32X16 ? 16
+                 : s == MT_TX_64X64 ? 64 : 32;
+    // transform-size context of the coefficient CDFs: (log2(w/4) + log2(h/4) + 1) >> 1
+    const int cls = ((31 - __clz(wf)) - 2 + (31 - __clz(hf)) - 2 + 1) >> 1;
+    const int w = wf > 32 ? 32 : wf, h = hf > 32 ? 32 : hf, N = w * h;
+    // the eob_bin CDF must be the one of this coefficient count (the reference indexes its
+    // symbols by the eob's bin; a smaller table would run past its end)
+    const int ebn = d.eob_bin < 3 ? 16 : d.eob_bin < 5 ? 32 : d.eob_bin < 8 ? 64 : d.eob_bin < 11 ? 128
+                  : d.eob_bin < 13 ? 256 : d.eob_bin < 15 ? 512 : 1024;
+    if (ebn != N || (d.txtp_set == 3 && d.txtp > 4)) {
+        out[b] = __int_as_float(0x7fc00000);
+        return;
+    }
+    const int pl = d.plane;
+    const RateTables& T = g_rate[d.qctx];
+    const int32_t* cf = levels + d.offset;
+    const uint16_t* scan = c_scan + so;
+    const uint16_t* iscan = c_iscan + so;
+
+    int eob = N - 1;
+    while (eob >= 0 && cf[scan[eob]] == 0) eob--;
+    const bool adaptive_skip = d.flags & 1;
+    if (eob < 0) {
+        out[b] = adaptive_skip ? d.skip_one : T.skip[cls][d.skip_ctx][1];
+        return;
+    }
+    const int cb = cls < 3 ? cls : 3;
+    float bits = adaptive_skip ? d.skip_zero : T.skip[cls][d.skip_ctx][0];
+    if (d.txtp_set) bits += T.txtp[d.txtp_set - 1][d.txtp_mode][d.txtp];
+    if (eob == 0) {
+        const uint32_t m = uabs32(cf[0]);
+        const uint32_t t = m < 3 ? m : 3;
+        bits += T.eob_bin[d.eob_bin][0];
+        bits += T.eob_base[cls][pl][0][t - 1];
+        if (t == 3) {
+            float hi = T.br[cb][pl][0][(m - 3) < 12 ? (m - 3) : 12];
+            if (m >= 15) hi += golomb_bits(m - 15);
+            bits += hi;
+        }
+        bits += T.dc_sign[pl][d.dcs_ctx][cf[0] < 0 ? 1 : 0];
+        out[b] = bits;
+        return;
+    }
+    const int eob_bin = eob < 2 ? eob : 32 - __clz(eob);
+    bits += T.eob_bin[d.eob_bin][eob_bin];
+    if (eob_bin > 1) {
+        const int nbits = eob_bin - 2;
+        bits += T.eob_hi[cls][pl][eob_bin][(eob >> nbits) & 1];
+        bits += (float)nbits;
+    }
+    if (bits > d.bound) {
+        out[b] = INFINITY;
+        return;
+    }
+    const int log2h = 31 - __clz(h);
+    const int oi = w < h ? 1 : (w > h ? 2 : 0);
+    // level byte of x-major index k as the reference's plane holds it at step i
+    auto lvl_at = [&](int k, int i) -> uint32_t {
+        return (k < N && (int)iscan[k] > i) ? level_byte(uabs32(cf[k])) : 0u;
+    };
+    {
+        const int rc = scan[eob];
+        const int ex = rc >> log2h, ey = rc & (h - 1);
+        const uint32_t m = uabs32(cf[rc]);
+        const uint32_t et = (m < 3 ? m : 3) - 1;
+        const int ctx_e = 1 + (eob > (N >> 3) ? 1 : 0) + (eob > (N >> 2) ? 1 : 0);
+        bits += T.eob_base[cls][pl][ctx_e][et];
+        if (et == 2) {
+            const int bc = (ex | ey) > 1 ? 14 : 7;
+            float hi = T.br[cb][pl][bc][(m - 3) < 12 ? (m - 3) : 12];
+            if (m >= 15) hi += golomb_bits(m - 15);
+            bits += hi;
+        }
+    }
+    uint32_t ac_nonzero = 1;
+    for (int i = eob - 1; i >= 1; i--) {
+        if ((i & 15) == 0 && bits > d.bound) {
+            out[b] = INFINITY;
+            return;
+        }
+        const int rc = scan[i];
+        const int x = rc >> log2h, y = rc & (h - 1);
+        const int base = x * h + y;
+        const uint32_t hm = lvl_at(base + 1, i) + lvl_at(base + h, i) + lvl_at(base + h + 1, i);
+        const uint32_t mag = hm + lvl_at(base + 2, i) + lvl_at(base + 2 * h, i);
+        const int ctx = c_lo_off[oi][y < 4 ? y : 4][x < 4 ? x : 4] + (mag > 512 ? 4 : (int)((mag + 64) >> 7));
+        const uint32_t m = uabs32(cf[rc]);
+        const uint32_t tok = m < 3 ? m : 3;
+        bits += T.bt[cls][pl][ctx][tok];
+        if (tok == 3) {
+            const uint32_t mg = hm & 63;
+            const int bc = ((y | x) > 1 ? 14 : 7) + (mg > 12 ? 6 : (int)((mg + 1) >> 1));
+            bits += T.br[cb][pl][bc][(m - 3) < 12 ? (m - 3) : 12];
+            if (m >= 15) bits += golomb_bits(m - 15);
+        }
+        ac_nonzero += m != 0;
+    }
+    const uint32_t dm = uabs32(cf[0]);
+    const uint32_t dt = dm < 3 ? dm : 3;
+    bits += T.bt[cls][pl][0][dt];
+    if (dt == 3) {
+        const uint32_t mg = (lvl_at(1, 0) + lvl_at(h, 0) + lvl_at(h + 1, 0)) & 63;
+        const int bc = mg > 12 ? 6 : (int)((mg + 1) >> 1);
+        float hi = T.br[cb][pl][bc][(dm - 3) < 12 ? (dm - 3) : 12];
+        if (dm >= 15) hi += golomb_bits(dm - 15);
+        bits += hi;
+    }
+    if (cf[0] != 0) bits += T.dc_sign[pl][d.dcs_ctx][cf[0] < 0 ? 1 : 0];
+    for (uint32_t k = 0; k < ac_nonzero; k++) bits += 1.0f;
+    out[b] = bits;
+}
+
+// ---------------------------------------------------------------- host-side tables
+// cost_q_table (src/cost.rs:159): Q22 -log2(p / 32768) of the C library's log2f, as the
+// reference builds it at run time (so it is built on the host and uploaded, never
+// recomputed on the device).
+const uint32_t* cost_q() {
+    static uint32_t t[32769];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        t[0] = 0;
+        for (int p = 1; p <= 32768; p++) {
+            const float bits = -log2f((float)p * (1.0f / 32768.0f));
+            t[p] = (uint32_t)roundf(bits * (float)(1u << 22));
+        }
+    });
+    return t;
+}
+
+// Cost of symbol s of a raw AV1 CDF with `k` stored values (cdf_cost_with_table on the
+// icdf form, src/cost.rs:185 with icdf src/tables.rs:31).
+float sym_cost(const uint16_t* raw, int k, int s) {
+    const int fl = s > 0 ? 32768 - raw[s - 1] : 32768;
+    const int fh = s < k ? 32768 - raw[s] : 0;
+    const int p = fl - fh < 1 ? 1 : fl - fh;
+    return (float)cost_q()[p] * (1.0f / (float)(1u << 22));
+}
+
+void ladder(const uint16_t* raw3, float* row) {  // br_cum_row_with_table (src/cost.rs:204)
+    float c[4];
+    for (int s = 0; s < 4; s++) c[s] = sym_cost(raw3, 3, s);
+    const float c3x2 = c[3] + c[3];
+    const float c3x3 = c3x2 + c[3];
+    const float c3x4 = c3x3 + c[3];
+    const float r[13] = {c[0], c[1], c[2], c[3] + c[0], c[3] + c[1], c[3] + c[2], c3x2 + c[0],
+                         c3x2 + c[1], c3x2 + c[2], c3x3 + c[0], c3x3 + c[1], c3x3 + c[2], c3x4};
+    memcpy(row, r, sizeof(r));
+}
+
+void build_tables(int q, RateTables& t) {
+    memset(&t, 0, sizeof(t));
+    const uint16_t* skip[5] = {MT_SKIP_TX4[q], MT_SKIP_TX8[q], MT_SKIP_TX16[q], MT_SKIP_TX32[q],
+                               MT_SKIP_TX64[q]};
+    for (int c = 0; c < 5; c++)
+        for (int x = 0; x < 13; x++)
+            for (int s = 0; s < 2; s++) t.skip[c][x][s] = sym_cost(&skip[c][x], 1, s);
+    const uint16_t(*bt[5][2])[3] = {{MT_BASE_TOK_TX4_LUMA_Q[q], MT_BASE_TOK_TX4_CHROMA_Q[q]},
+                                    {MT_BASE_TOK_TX8_LUMA_Q[q], MT_BASE_TOK_TX8_CHROMA_Q[q]},
+                                    {MT_BASE_TOK_TX16_LUMA_Q[q], MT_BASE_TOK_TX16_CHROMA_Q[q]},
+                                    {MT_BASE_TOK_TX32_LUMA_Q[q], MT_BASE_TOK_TX32_CHROMA_Q[q]},
+                                    {MT_BASE_TOK_TX64_LUMA_Q[q], nullptr}};
+    const int nctx[5] = {41, 41, 42, 42, 42};
+    // Contexts past a table's end stay 0.0, as in the reference's zero-initialised table
+    // (the 2-D context walk never reaches them).
+    for (int c = 0; c < 5; c++)
+        for (int p = 0; p < 2; p++)
+            if (bt[c][p])
+                for (int x = 0; x < nctx[c]; x++)
+                    for (int s = 0; s < 4; s++) t.bt[c][p][x][s] = sym_cost(bt[c][p][x], 3, s);
+    const uint16_t(*br[4][2])[3] = {{MT_BR_TOK_TX4_LUMA_Q[q], MT_BR_TOK_TX4_CHROMA_Q[q]},
+                                    {MT_BR_TOK_TX8_LUMA_Q[q], MT_BR_TOK_TX8_CHROMA_Q[q]},
+                                    {MT_BR_TOK_TX16_LUMA_Q[q], MT_BR_TOK_TX16_CHROMA_Q[q]},
+                                    {MT_BR_TOK_TX32_LUMA_Q[q], MT_BR_TOK_TX32_CHROMA_Q[q]}};
+    for (int c = 0; c < 4; c++)
+        for (int p = 0; p < 2; p++)
+            for (int x = 0; x < 21; x++) ladder(br[c][p][x], t.br[c][p][x]);
+    const uint16_t(*eb[5][2])[2] = {{MT_EOB_BASE_TX4_LUMA_Q[q], MT_EOB_BASE_TX4_CHROMA_Q[q]},
+                                    {MT_EOB_BASE_TX8_LUMA_Q[q], MT_EOB_BASE_TX8_CHROMA_Q[q]},
+                                    {MT_EOB_BASE_TX16_LUMA_Q[q], MT_EOB_BASE_TX16_CHROMA_Q[q]},
+                                    {MT_EOB_BASE_TX32_LUMA_Q[q], MT_EOB_BASE_TX32_CHROMA_Q[q]},
+                                    {MT_EOB_BASE_TX64_LUMA_Q[q], nullptr}};
+    for (int c = 0; c < 5; c++)
+        for (int p = 0; p < 2; p++)
+            if (eb[c][p])
+                for (int x = 0; x < 4; x++)
+                    for (int s = 0; s < 3; s++) t.eob_base[c][p][x][s] = sym_cost(eb[c][p][x], 2, s);
+    const uint16_t* eh[5][2] = {{MT_EOB_HI_TX4_LUMA[q], MT_EOB_HI_TX4_CHROMA[q]},
+                                {MT_EOB_HI_TX8_LUMA[q], MT_EOB_HI_TX8_CHROMA[q]},
+                                {MT_EOB_HI_TX16_LUMA[q], MT_EOB_HI_TX16_CHROMA[q]},
+                                {MT_EOB_HI_TX32_LUMA[q], MT_EOB_HI_TX32_CHROMA[q]},
+                                {MT_EOB_HI_TX64_LUMA[q], nullptr}};
+    for (int c = 0; c < 5; c++)
+        for (int p = 0; p < 2; p++)
+            if (eh[c][p])
+                for (int x = 0; x < 11; x++)
+                    for (int s = 0; s < 2; s++) t.eob_hi[c][p][x][s] = sym_cost(&eh[c][p][x], 1, s);
+    for (int p = 0; p < 2; p++)
+        for (int x = 0; x < 3; x++)
+            for (int s = 0; s < 2; s++) t.dc_sign[p][x][s] = sym_cost(&MT_DC_SIGN_Q[q][p][x], 1, s);
+    struct {
+        const uint16_t* raw;
+        int k;
+    } bins[17] = {{MT_EOB_BIN_16_LUMA[q], 4},   {MT_EOB_BIN_16_CHROMA[q], 4},  {MT_EOB_BIN_16_LUMA_1D[q], 4},
+                  {MT_EOB_BIN_32_LUMA[q], 5},   {MT_EOB_BIN_32_CHROMA[q], 5},  {MT_EOB_BIN_64_LUMA[q], 6},
+                  {MT_EOB_BIN_64_CHROMA[q], 6}, {MT_EOB_BIN_64_LUMA_1D[q], 6}, {MT_EOB_BIN_128_LUMA[q], 7},
+                  {MT_EOB_BIN_128_CHROMA[q], 7}, {MT_EOB_BIN_128_LUMA_1D[q], 7}, {MT_EOB_BIN_256_LUMA[q], 8},
+                  {MT_EOB_BIN_256_CHROMA[q], 8}, {MT_EOB_BIN_512_LUMA[q], 9},  {MT_EOB_BIN_512_CHROMA[q], 9},
+                  {MT_EOB_BIN_1024_LUMA[q], 10}, {MT_EOB_BIN_1024_CHROMA[q], 10}};
+    for (int e = 0; e < 17; e++)
+        for (int s = 0; s <= bins[e].k; s++) t.eob_bin[e][s] = sym_cost(bins[e].raw, bins[e].k, s);
+    for (int m = 0; m < 13; m++) {
+        for (int s = 0; s <= 6; s++) t.txtp[0][m][s] = sym_cost(MT_TXTP_INTRA1_TX4[m], 6, s);
+        for (int s = 0; s <= 6; s++) t.txtp[1][m][s] = sym_cost(MT_TXTP_INTRA1_TX8[m], 6, s);
+        for (int s = 0; s <= 4; s++) t.txtp[2][m][s] = sym_cost(MT_TXTP_INTRA2_TX16[m], 4, s);
+    }
+}
+
+struct ScanTabs {
+    uint16_t scan[kScanTotal], iscan[kScanTotal];
+};
+
+void build_scans(ScanTabs& st) {
+    struct {
+        int s;
+        const uint32_t* tab;
+        int n;
+    } list[12] = {{MT_TX_4X4, MT_SCAN_4X4, 16},       {MT_TX_4X8, MT_SCAN_4X8, 32},
+                  {MT_TX_8X4, MT_SCAN_8X4, 32},       {MT_TX_8X8, MT_SCAN_8X8, 64},
+                  {MT_TX_4X16, MT_SCAN_4X16, 64},     {MT_TX_16X4, MT_SCAN_16X4, 64},
+                  {MT_TX_8X16, MT_SCAN_8X16, 128},    {MT_TX_16X8, MT_SCAN_16X8, 128},
+                  {MT_TX_16X16, MT_SCAN_16X16, 256},  {MT_TX_16X32, MT_SCAN_16X32, 512},
+                  {MT_TX_32X16, MT_SCAN_32X16, 512},  {MT_TX_32X32, MT_SCAN_32X32, 1024}};
+    for (auto& e : list) {
+        const int o = scan_offset(e.s);
+        for (int i = 0; i < e.n; i++) {
+            st.scan[o + i] = (uint16_t)e.tab[i];
+            st.iscan[o + e.tab[i]] = (uint16_t)i;
+        }
+    }
+}
+
+}  // namespace
+
+namespace mtlib {
+int rate_init_device(int /*device*/) {
+    static RateTables host[4];
+    static ScanTabs scans;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int q = 0; q < 4; q++) build_tables(q, host[q]);
+        build_scans(scans);
+    });
+    uint8_t lo[3][5][5];
+    for (int y = 0; y < 5; y++)
+        for (int x = 0; x < 5; x++) {
+            lo[0][y][x] = (uint8_t)MT_LO_CTX_OFF[y][x];
+            lo[1][y][x] = (uint8_t)MT_LO_CTX_OFF_WLH[y][x];
+            lo[2][y][x] = (uint8_t)MT_LO_CTX_OFF_WGH[y][x];
+        }
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_rate), host, sizeof(host)));
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_scan), scans.scan, sizeof(scans.scan)));
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_iscan), scans.iscan, sizeof(scans.iscan)));
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_lo_off), lo, sizeof(lo)));
+    return MT_OK;
+}
+}  // namespace mtlib
+
+extern "C" int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                             void* stream) {
+    int dev = 0;
+    MT_HIP_TRY(hipGetDevice(&dev));
+    if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !bits_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_block_rate: bad pointer or count");
+    if (n == 0) return MT_OK;
+    hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, blocks_d, n,
+                       levels_d, bits_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}

@@ -20,6 +20,7 @@
 
 #include "../../include/mtgpu.h"
 #include "av1_tables.h"
+#include "mt_common.h"
 #include "mt_tx1d.h"
 
 using namespace mt;
@@ -27,6 +28,10 @@ using namespace mt;
 // ------------------------------------------------------------------------- error state
 static thread_local std::string g_err;
 static int set_err(int code, const char* what) {
+    g_err = what;
+    return code;
+}
+int mtlib::set_error(int code, const std::string& what) {
     g_err = what;
     return code;
 }
@@ -1086,6 +1091,10 @@ extern "C" int mt_init(int device) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     st.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    {
+        int rc = mtlib::rate_init_device(device);
+        if (rc) return rc;
+    }
     HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&st.cursor, kClasses * sizeof(uint32_t)));

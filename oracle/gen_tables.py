@@ -28,6 +28,60 @@ REF = "/root/reference/src"
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_H = os.path.join(HERE, "..", "maroontree_amd", "csrc", "av1_tables.h")
 OUT_J = os.path.join(HERE, "..", "tests", "golden", "tables_checksum.json")
+OUT_H2 = os.path.join(HERE, "..", "maroontree_amd", "csrc", "av1_cdf_tables.h")
+
+# Default coefficient-coding CDFs (AV1 spec defaults, per qctx) in the raw (non-inverted) form
+# the reference stores (src/coef_q.rs), the AV1 scans (src/coef_q.rs, src/tables.rs:348-455)
+# and the coeff_base context offsets (src/tables.rs:354, :456, :466), plus the intra mode /
+# tx-type CDFs of the mode rate (src/tables.rs).
+CDF_TABLES_COEF_Q = [
+    "SKIP_TX4", "SKIP_TX8", "SKIP_TX16", "SKIP_TX32", "SKIP_TX64",
+    "BASE_TOK_TX4_LUMA_Q", "BASE_TOK_TX4_CHROMA_Q", "BASE_TOK_TX8_LUMA_Q", "BASE_TOK_TX8_CHROMA_Q",
+    "BASE_TOK_TX16_LUMA_Q", "BASE_TOK_TX16_CHROMA_Q", "BASE_TOK_TX32_LUMA_Q", "BASE_TOK_TX32_CHROMA_Q",
+    "BASE_TOK_TX64_LUMA_Q",
+    "BR_TOK_TX4_LUMA_Q", "BR_TOK_TX4_CHROMA_Q", "BR_TOK_TX8_LUMA_Q", "BR_TOK_TX8_CHROMA_Q",
+    "BR_TOK_TX16_LUMA_Q", "BR_TOK_TX16_CHROMA_Q", "BR_TOK_TX32_LUMA_Q", "BR_TOK_TX32_CHROMA_Q",
+    "EOB_BASE_TX4_LUMA_Q", "EOB_BASE_TX4_CHROMA_Q", "EOB_BASE_TX8_LUMA_Q", "EOB_BASE_TX8_CHROMA_Q",
+    "EOB_BASE_TX16_LUMA_Q", "EOB_BASE_TX16_CHROMA_Q", "EOB_BASE_TX32_LUMA_Q", "EOB_BASE_TX32_CHROMA_Q",
+    "EOB_BASE_TX64_LUMA_Q",
+    "EOB_HI_TX4_LUMA", "EOB_HI_TX4_CHROMA", "EOB_HI_TX8_LUMA", "EOB_HI_TX8_CHROMA",
+    "EOB_HI_TX16_LUMA", "EOB_HI_TX16_CHROMA", "EOB_HI_TX32_LUMA", "EOB_HI_TX32_CHROMA",
+    "EOB_HI_TX64_LUMA",
+    "DC_SIGN_Q",
+    "EOB_BIN_16_LUMA", "EOB_BIN_16_CHROMA", "EOB_BIN_16_LUMA_1D",
+    "EOB_BIN_32_LUMA", "EOB_BIN_32_CHROMA",
+    "EOB_BIN_64_LUMA", "EOB_BIN_64_CHROMA", "EOB_BIN_64_LUMA_1D",
+    "EOB_BIN_128_LUMA", "EOB_BIN_128_CHROMA", "EOB_BIN_128_LUMA_1D",
+    "EOB_BIN_256_LUMA", "EOB_BIN_256_CHROMA",
+    "EOB_BIN_512_LUMA", "EOB_BIN_512_CHROMA",
+    "EOB_BIN_1024_LUMA", "EOB_BIN_1024_CHROMA",
+    "SCAN_16X16", "SCAN_16X4", "SCAN_4X16", "SCAN_8X16", "SCAN_16X8", "SCAN_32X32",
+    "SCAN_16X32", "SCAN_32X16",
+]
+CDF_TABLES_TABLES = [
+    "SCAN_4X4", "SCAN_4X8", "SCAN_8X4", "SCAN_8X8",
+    "LO_CTX_OFF", "LO_CTX_OFF_WLH", "LO_CTX_OFF_WGH",
+    "KF_Y_MODE_CDF", "UV_MODE_NOCFL_CDF", "UV_MODE_CFL_CDF", "TXTP_INTRA1_TX8", "TXTP_INTRA2_TX16",
+    "TXTP_INTRA1_TX4", "ANGLE_DELTA_CDF", "INTRA_FILTER_TAPS", "FILTER_INTRA_CDF",
+    "FILTER_INTRA_MODE_CDF",
+]
+
+
+def _typed_table(text, name):
+    """(ctype, dims, values) of `static NAME: <rust array type> = [...]`."""
+    m = re.search(r"static " + name + r":\s*([^=]+)=", text)
+    ty = m.group(1).strip()
+    dims = [int(d) for d in re.findall(r";\s*(\d+)\s*\]", ty)]
+    base = re.search(r"(u8|u16|u32|i8|i16|i32)", ty).group(1)
+    count = 1
+    for d in dims:
+        count *= d
+    body = text[m.end():]
+    vals = [int(v) for v in re.findall(r"-?\b\d+\b", body)][:count]
+    assert len(vals) == count, name
+    ctype = {"u8": "uint8_t", "u16": "uint16_t", "u32": "uint32_t", "i8": "int8_t",
+             "i16": "int16_t", "i32": "int32_t"}[base]
+    return ctype, list(reversed(dims)), vals
 
 
 def _ints_after(text, name, count):
@@ -115,6 +169,22 @@ def main():
     out.append("};")
     with open(OUT_H, "w") as f:
         f.write("\n".join(out) + "\n")
+    tables_rs = open(os.path.join(REF, "tables.rs")).read()
+    out2 = ["// Generated by oracle/gen_tables.py -- do not edit by hand.",
+            "// AV1 default coefficient CDFs (raw form, per qctx), scans, coeff_base context",
+            "// offsets and intra mode / tx-type CDFs, with the values of the maroontree encoder",
+            "// (reference src/coef_q.rs, src/tables.rs).",
+            "#pragma once", "#include <stdint.h>", ""]
+    for src_text, names in ((coef_q, CDF_TABLES_COEF_Q), (tables_rs, CDF_TABLES_TABLES)):
+        for name in names:
+            ctype, dims, vals = _typed_table(src_text, name)
+            sums["cdf_" + name.lower()] = fnv1a(vals, 4)
+            dim_s = "".join(f"[{d}]" for d in dims)
+            out2.append(f"static const {ctype} MT_{name}{dim_s} = {{")
+            out2.append(arr(vals, 16))
+            out2.append("};")
+    with open(OUT_H2, "w") as f:
+        f.write("\n".join(out2) + "\n")
     with open(OUT_J, "w") as f:
         json.dump({k: f"0x{v:016x}" for k, v in sums.items()}, f, indent=1)
         f.write("\n")

@@ -97,6 +97,29 @@ int orc_chain_batch(int n, const uint8_t* tx_size, const uint8_t* tx_type, const
 uint32_t orc_block_hash(int ncoef, const int32_t* levels, const float* targets, int npx,
                         const int32_t* recon);
 
+/* ---- coefficient rate (src/rate.rs:149, src/cost.rs:159-262) ---------------------- */
+/* eob_bin: which eob_bin CDF the caller prices with (the reference picks it by size/plane):
+ *  0 16 luma, 1 16 chroma, 2 16 luma 1-D, 3 32 luma, 4 32 chroma, 5 64 luma, 6 64 chroma,
+ *  7 64 luma 1-D, 8 128 luma, 9 128 chroma, 10 128 luma 1-D, 11 256 luma, 12 256 chroma,
+ *  13 512 luma, 14 512 chroma, 15 1024 luma, 16 1024 chroma.
+ * txtp_set: 0 none (chroma), 1 the TX_4X4 intra set, 2 the TX_8X8 set, 3 the TX_16X16 set;
+ * txtp_mode the luma intra mode row, txtp the set index of the type. */
+typedef struct {
+    int32_t qctx, cls, plane, w, h, eob_bin, skip_ctx, dcs_ctx;
+    int32_t txtp_set, txtp_mode, txtp;
+    int32_t use_skip_bits;
+    float skip_one, skip_zero;
+} orc_rate_ctx;
+
+/* real_block_bits_bounded against Cdfs::new(qctx); cf x-major, scan of the block size. */
+float orc_block_rate(const int32_t* cf, const uint32_t* scan, const orc_rate_ctx* c, float bound);
+/* cost_q_table entry (Q22 bits) and the cost of one symbol of a raw AV1 CDF */
+uint32_t orc_cost_q(int p);
+float orc_cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s);
+float orc_skip_cost(int qctx, int cls, int ctx, int s);
+/* the scan table of an AV1 TxSize (64x64 uses the 32x32 scan); NULL if none */
+const uint32_t* orc_scan(int tx_size);
+
 #ifdef __cplusplus
 }
 #endif

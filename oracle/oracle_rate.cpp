@@ -1,0 +1,332 @@
+// CPU oracle: coefficient rate model (the decision-side mirror of the coefficient coder).
+//
+// TEST INFRASTRUCTURE ONLY -- the checker for the HIP library, never part of the product.
+// Restates src/cost.rs:159-262 (cost table, CDF cost, Golomb, base-range ladder),
+// src/rate.rs:50-301 (CoefCostTables, real_block_bits_bounded) and src/coeffs.rs:46
+// (get_lo_ctx_2d), with Cdfs::new(qctx)'s coefficient part (src/coder/cdfs.rs:400-629).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "oracle.h"
+#include "../maroontree_amd/csrc/av1_cdf_tables.h"
+
+namespace {
+
+inline uint32_t uabs(int32_t v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
+
+// cost_q_table (src/cost.rs:159): Q22 -log2(p / 32768) from the C library's log2f.
+const uint32_t* cost_q_table() {
+    static uint32_t t[32769];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        t[0] = 0;
+        for (int p = 1; p <= 32768; p++) {
+            float bits = -log2f((float)p * (1.0f / 32768.0f));
+            t[p] = (uint32_t)roundf(bits * (float)(1u << 22));
+        }
+    });
+    return t;
+}
+
+// cdf_cost_with_table (src/cost.rs:185) on an inverse CDF (icdf form: 32768 - cdf values).
+float cdf_cost(const uint16_t* icdf, int s) {
+    int fl = s > 0 ? icdf[s - 1] : 32768;
+    int fh = icdf[s];
+    int p = fl - fh < 1 ? 1 : fl - fh;
+    return (float)cost_q_table()[p] * (1.0f / (float)(1u << 22));
+}
+
+// golomb_cost (src/cost.rs:194)
+float golomb_cost(uint32_t v) {
+    uint32_t len = 32 - __builtin_clz(v + 1);
+    return (float)(2 * len - 1);
+}
+
+// hi_tok_cost_with_table (src/cost.rs:235)
+float hi_tok_cost(uint32_t m, const uint16_t* br_icdf) {
+    int total_br = std::min((int)m - 3, 12);
+    int coded = 0;
+    float bits = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        int s = std::min(total_br - coded, 3);
+        bits += cdf_cost(br_icdf, s);
+        coded += s;
+        if (s < 3) break;
+    }
+    if (m >= 15) bits += golomb_cost(m - 15);
+    return bits;
+}
+
+// br_cum_row_with_table (src/cost.rs:204)
+void br_cum_row(const uint16_t* br_icdf, float* row) {
+    float c[4];
+    for (int s = 0; s < 4; s++) c[s] = cdf_cost(br_icdf, s);
+    float c3x2 = c[3] + c[3];
+    float c3x3 = c3x2 + c[3];
+    float c3x4 = c3x3 + c[3];
+    const float r[13] = {c[0], c[1], c[2], c[3] + c[0], c[3] + c[1], c[3] + c[2], c3x2 + c[0],
+                         c3x2 + c[1], c3x2 + c[2], c3x3 + c[0], c3x3 + c[1], c3x3 + c[2], c3x4};
+    memcpy(row, r, sizeof(r));
+}
+
+// level_byte (src/tables.rs:365)
+uint8_t level_byte(uint32_t m) {
+    if (m == 0) return 0;
+    if (m <= 2) return (uint8_t)(m * 0x41);
+    return (uint8_t)(std::min(m, 15u) + (3u << 6));
+}
+
+// The coefficient CDFs of Cdfs::new(qctx) in icdf form (src/coder/cdfs.rs:400-629,
+// icdf src/tables.rs:31): every symbol row stored with its terminal 0.
+struct CoefCdfs {
+    uint16_t txb_skip[5][13][2];
+    uint16_t base_tok[5][2][42][4];
+    uint16_t br_tok[4][2][21][4];
+    uint16_t eob_base[5][2][4][3];
+    uint16_t eob_hi[5][2][11][2];
+    uint16_t dc_sign[2][3][2];
+    uint16_t eob_bin[17][11];  // see the eob-bin numbering in oracle.h
+    uint16_t txtp4[13][7], txtp8[13][7], txtp16[13][5];
+    // CoefCostTables (src/rate.rs:50-80)
+    float bt[5][2][42][4];
+    float br[4][2][21][13];
+};
+
+template <int K> void icdf_row(const uint16_t* raw, uint16_t* out) {
+    for (int i = 0; i < K; i++) out[i] = (uint16_t)(32768 - raw[i]);
+    out[K] = 0;
+}
+
+const CoefCdfs& coef_cdfs(int qctx) {
+    static CoefCdfs c[4];
+    static std::once_flag once[4];
+    std::call_once(once[qctx], [qctx] {
+        CoefCdfs& d = c[qctx];
+        memset(&d, 0, sizeof(d));
+        const uint16_t* skip[5] = {MT_SKIP_TX4[qctx], MT_SKIP_TX8[qctx], MT_SKIP_TX16[qctx],
+                                   MT_SKIP_TX32[qctx], MT_SKIP_TX64[qctx]};
+        for (int cls = 0; cls < 5; cls++)
+            for (int ctx = 0; ctx < 13; ctx++) icdf_row<1>(&skip[cls][ctx], d.txb_skip[cls][ctx]);
+        const uint16_t(*bt_src[5][2])[3] = {
+            {MT_BASE_TOK_TX4_LUMA_Q[qctx], MT_BASE_TOK_TX4_CHROMA_Q[qctx]},
+            {MT_BASE_TOK_TX8_LUMA_Q[qctx], MT_BASE_TOK_TX8_CHROMA_Q[qctx]},
+            {MT_BASE_TOK_TX16_LUMA_Q[qctx], MT_BASE_TOK_TX16_CHROMA_Q[qctx]},
+            {MT_BASE_TOK_TX32_LUMA_Q[qctx], MT_BASE_TOK_TX32_CHROMA_Q[qctx]},
+            {MT_BASE_TOK_TX64_LUMA_Q[qctx], nullptr}};
+        const int bt_n[5] = {41, 41, 42, 42, 42};
+        for (int cls = 0; cls < 5; cls++)
+            for (int pl = 0; pl < 2; pl++)
+                if (bt_src[cls][pl])
+                    for (int ctx = 0; ctx < bt_n[cls]; ctx++) icdf_row<3>(bt_src[cls][pl][ctx], d.base_tok[cls][pl][ctx]);
+        const uint16_t(*br_src[4][2])[3] = {
+            {MT_BR_TOK_TX4_LUMA_Q[qctx], MT_BR_TOK_TX4_CHROMA_Q[qctx]},
+            {MT_BR_TOK_TX8_LUMA_Q[qctx], MT_BR_TOK_TX8_CHROMA_Q[qctx]},
+            {MT_BR_TOK_TX16_LUMA_Q[qctx], MT_BR_TOK_TX16_CHROMA_Q[qctx]},
+            {MT_BR_TOK_TX32_LUMA_Q[qctx], MT_BR_TOK_TX32_CHROMA_Q[qctx]}};
+        for (int cls = 0; cls < 4; cls++)
+            for (int pl = 0; pl < 2; pl++)
+                for (int ctx = 0; ctx < 21; ctx++) icdf_row<3>(br_src[cls][pl][ctx], d.br_tok[cls][pl][ctx]);
+        const uint16_t(*eb_src[5][2])[2] = {
+            {MT_EOB_BASE_TX4_LUMA_Q[qctx], MT_EOB_BASE_TX4_CHROMA_Q[qctx]},
+            {MT_EOB_BASE_TX8_LUMA_Q[qctx], MT_EOB_BASE_TX8_CHROMA_Q[qctx]},
+            {MT_EOB_BASE_TX16_LUMA_Q[qctx], MT_EOB_BASE_TX16_CHROMA_Q[qctx]},
+            {MT_EOB_BASE_TX32_LUMA_Q[qctx], MT_EOB_BASE_TX32_CHROMA_Q[qctx]},
+            {MT_EOB_BASE_TX64_LUMA_Q[qctx], nullptr}};
+        for (int cls = 0; cls < 5; cls++)
+            for (int pl = 0; pl < 2; pl++)
+                if (eb_src[cls][pl])
+                    for (int ctx = 0; ctx < 4; ctx++) icdf_row<2>(eb_src[cls][pl][ctx], d.eob_base[cls][pl][ctx]);
+        const uint16_t* eh_src[5][2] = {
+            {MT_EOB_HI_TX4_LUMA[qctx], MT_EOB_HI_TX4_CHROMA[qctx]},
+            {MT_EOB_HI_TX8_LUMA[qctx], MT_EOB_HI_TX8_CHROMA[qctx]},
+            {MT_EOB_HI_TX16_LUMA[qctx], MT_EOB_HI_TX16_CHROMA[qctx]},
+            {MT_EOB_HI_TX32_LUMA[qctx], MT_EOB_HI_TX32_CHROMA[qctx]},
+            {MT_EOB_HI_TX64_LUMA[qctx], nullptr}};
+        for (int cls = 0; cls < 5; cls++)
+            for (int pl = 0; pl < 2; pl++)
+                if (eh_src[cls][pl])
+                    for (int b = 0; b < 11; b++) icdf_row<1>(&eh_src[cls][pl][b], d.eob_hi[cls][pl][b]);
+        for (int pl = 0; pl < 2; pl++)
+            for (int ctx = 0; ctx < 3; ctx++) icdf_row<1>(&MT_DC_SIGN_Q[qctx][pl][ctx], d.dc_sign[pl][ctx]);
+        icdf_row<4>(MT_EOB_BIN_16_LUMA[qctx], d.eob_bin[0]);
+        icdf_row<4>(MT_EOB_BIN_16_CHROMA[qctx], d.eob_bin[1]);
+        icdf_row<4>(MT_EOB_BIN_16_LUMA_1D[qctx], d.eob_bin[2]);
+        icdf_row<5>(MT_EOB_BIN_32_LUMA[qctx], d.eob_bin[3]);
+        icdf_row<5>(MT_EOB_BIN_32_CHROMA[qctx], d.eob_bin[4]);
+        icdf_row<6>(MT_EOB_BIN_64_LUMA[qctx], d.eob_bin[5]);
+        icdf_row<6>(MT_EOB_BIN_64_CHROMA[qctx], d.eob_bin[6]);
+        icdf_row<6>(MT_EOB_BIN_64_LUMA_1D[qctx], d.eob_bin[7]);
+        icdf_row<7>(MT_EOB_BIN_128_LUMA[qctx], d.eob_bin[8]);
+        icdf_row<7>(MT_EOB_BIN_128_CHROMA[qctx], d.eob_bin[9]);
+        icdf_row<7>(MT_EOB_BIN_128_LUMA_1D[qctx], d.eob_bin[10]);
+        icdf_row<8>(MT_EOB_BIN_256_LUMA[qctx], d.eob_bin[11]);
+        icdf_row<8>(MT_EOB_BIN_256_CHROMA[qctx], d.eob_bin[12]);
+        icdf_row<9>(MT_EOB_BIN_512_LUMA[qctx], d.eob_bin[13]);
+        icdf_row<9>(MT_EOB_BIN_512_CHROMA[qctx], d.eob_bin[14]);
+        icdf_row<10>(MT_EOB_BIN_1024_LUMA[qctx], d.eob_bin[15]);
+        icdf_row<10>(MT_EOB_BIN_1024_CHROMA[qctx], d.eob_bin[16]);
+        for (int m = 0; m < 13; m++) {
+            icdf_row<6>(MT_TXTP_INTRA1_TX4[m], d.txtp4[m]);
+            icdf_row<6>(MT_TXTP_INTRA1_TX8[m], d.txtp8[m]);
+            icdf_row<4>(MT_TXTP_INTRA2_TX16[m], d.txtp16[m]);
+        }
+        // CoefCostTables::build (src/rate.rs:58)
+        for (int cls = 0; cls < 5; cls++)
+            for (int pl = 0; pl < 2; pl++) {
+                for (int ctx = 0; ctx < 42; ctx++)
+                    for (int tok = 0; tok < 4; tok++) d.bt[cls][pl][ctx][tok] = cdf_cost(d.base_tok[cls][pl][ctx], tok);
+                if (cls == 4) continue;
+                for (int bc = 0; bc < 21; bc++) br_cum_row(d.br_tok[cls][pl][bc], d.br[cls][pl][bc]);
+            }
+    });
+    return c[qctx];
+}
+
+// get_lo_ctx_2d (src/coeffs.rs:46)
+inline void lo_ctx(const uint8_t* levels, int x, int y, const uint32_t (*off)[5], int stride,
+                   int* ctx, uint32_t* hi_mag) {
+    auto g = [&](int dx, int dy) { return (uint32_t)levels[(x + dx) * stride + (y + dy)]; };
+    uint32_t hm = g(0, 1) + g(1, 0) + g(1, 1);
+    uint32_t mag = hm + g(0, 2) + g(2, 0);
+    uint32_t offset = off[std::min(y, 4)][std::min(x, 4)];
+    *ctx = (int)(offset + (mag > 512 ? 4 : (mag + 64) >> 7));
+    *hi_mag = hm;
+}
+
+}  // namespace
+
+extern "C" uint32_t orc_cost_q(int p) { return cost_q_table()[p]; }
+
+extern "C" float orc_cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s) {
+    // cost of symbol s against a raw (AV1 spec) CDF of nsym-1 values
+    uint16_t ic[17];
+    for (int i = 0; i < nsym - 1; i++) ic[i] = (uint16_t)(32768 - raw_cdf[i]);
+    ic[nsym - 1] = 0;
+    return cdf_cost(ic, s);
+}
+
+// real_block_bits_bounded (src/rate.rs:149)
+extern "C" float orc_block_rate(const int32_t* cf, const uint32_t* scan, const orc_rate_ctx* c,
+                                float bound) {
+    const CoefCdfs& d = coef_cdfs(c->qctx);
+    const int cls = c->cls, pl = c->plane, w = c->w, h = c->h, n = w * h;
+    const uint16_t* skip_cdf = d.txb_skip[cls][c->skip_ctx];
+    int eob = -1;
+    for (int i = n - 1; i >= 0; i--)
+        if (cf[scan[i]] != 0) {
+            eob = i;
+            break;
+        }
+    if (eob < 0) return c->use_skip_bits ? c->skip_one : cdf_cost(skip_cdf, 1);
+
+    const uint16_t(*br_tok)[4] = d.br_tok[std::min(cls, 3)][pl];
+    const uint16_t(*eob_base)[3] = d.eob_base[cls][pl];
+    const uint16_t(*eob_hi)[2] = d.eob_hi[cls][pl];
+    const uint16_t* dc_sign = d.dc_sign[pl][c->dcs_ctx];
+    const uint16_t* eob_bin_cdf = d.eob_bin[c->eob_bin];
+
+    float bits = c->use_skip_bits ? c->skip_zero : cdf_cost(skip_cdf, 0);
+    if (c->txtp_set > 0) {
+        const uint16_t* tc = c->txtp_set == 1 ? d.txtp4[c->txtp_mode]
+                           : c->txtp_set == 2 ? d.txtp8[c->txtp_mode] : d.txtp16[c->txtp_mode];
+        bits += cdf_cost(tc, c->txtp);
+    }
+    if (eob == 0) {
+        uint32_t m = uabs(cf[0]);
+        bits += cdf_cost(eob_bin_cdf, 0);
+        bits += cdf_cost(eob_base[0], (int)std::min(m, 3u) - 1);
+        if (std::min(m, 3u) == 3) bits += hi_tok_cost(m, br_tok[0]);
+        bits += cdf_cost(dc_sign, cf[0] < 0 ? 1 : 0);
+        return bits;
+    }
+    const int eob_bin = eob < 2 ? eob : 32 - __builtin_clz((uint32_t)eob);
+    bits += cdf_cost(eob_bin_cdf, eob_bin);
+    if (eob_bin > 1) {
+        const int nbits = eob_bin - 2;
+        bits += cdf_cost(eob_hi[eob_bin], (eob >> nbits) & 1);
+        bits += (float)nbits;
+    }
+    if (bits > bound) return INFINITY;
+
+    const int log2h = __builtin_ctz(h);
+    const int stride = h;
+    const uint32_t(*offsets)[5] = w < h ? MT_LO_CTX_OFF_WLH : w > h ? MT_LO_CTX_OFF_WGH : MT_LO_CTX_OFF;
+    const float(*bt_c)[4] = d.bt[cls][pl];
+    const float(*br_c)[13] = d.br[std::min(cls, 3)][pl];
+    std::vector<uint8_t> levels((size_t)(w + 2) * (h + 2), 0);
+
+    const int ctx_e = 1 + (eob > (n >> 3) ? 1 : 0) + (eob > (n >> 2) ? 1 : 0);
+    {
+        const int rc = (int)scan[eob];
+        const int ex = rc >> log2h, ey = rc & (h - 1);
+        const uint32_t m = uabs(cf[rc]);
+        const uint32_t eob_tok = std::min(m, 3u) - 1;
+        bits += cdf_cost(eob_base[ctx_e], (int)eob_tok);
+        if (eob_tok == 2) {
+            const int bc = (ex | ey) > 1 ? 14 : 7;
+            bits += hi_tok_cost(m, br_tok[bc]);
+        }
+        levels[ex * stride + ey] = level_byte(m);
+    }
+    uint32_t ac_nonzero = 1;
+    for (int i = eob - 1; i >= 1; i--) {
+        if ((i & 15) == 0 && bits > bound) return INFINITY;
+        const int rc = (int)scan[i];
+        const int x = rc >> log2h, y = rc & (h - 1);
+        int ctx;
+        uint32_t hi_mag;
+        lo_ctx(levels.data(), x, y, offsets, stride, &ctx, &hi_mag);
+        const uint32_t m = uabs(cf[rc]);
+        const uint32_t tok = std::min(m, 3u);
+        bits += bt_c[ctx][tok];
+        if (tok == 3) {
+            const uint32_t mag = hi_mag & 63;
+            const int bc = ((y | x) > 1 ? 14 : 7) + (mag > 12 ? 6 : (int)((mag + 1) >> 1));
+            const int total_br = std::min((int)m - 3, 12);
+            bits += br_c[bc][total_br];
+            if (m >= 15) bits += golomb_cost(m - 15);
+        }
+        if (m != 0) {
+            levels[x * stride + y] = level_byte(m);
+            ac_nonzero++;
+        }
+    }
+    const uint32_t dm = uabs(cf[0]);
+    const uint32_t dc_tok = std::min(dm, 3u);
+    bits += cdf_cost(d.base_tok[cls][pl][0], (int)dc_tok);
+    if (dc_tok == 3) {
+        const uint32_t mag = ((uint32_t)levels[1] + levels[stride] + levels[stride + 1]) & 63;
+        const int bc = mag > 12 ? 6 : (int)((mag + 1) >> 1);
+        bits += hi_tok_cost(dm, br_tok[bc]);
+    }
+    if (cf[0] != 0) bits += cdf_cost(dc_sign, cf[0] < 0 ? 1 : 0);
+    for (uint32_t k = 0; k < ac_nonzero; k++) bits += 1.0f;
+    return bits;
+}
+
+extern "C" const uint32_t* orc_scan(int tx_size) {
+    switch (tx_size) {
+        case 0: return MT_SCAN_4X4;
+        case 1: return MT_SCAN_8X8;
+        case 2: return MT_SCAN_16X16;
+        case 3: case 4: return MT_SCAN_32X32;
+        case 5: return MT_SCAN_4X8;
+        case 6: return MT_SCAN_8X4;
+        case 7: return MT_SCAN_8X16;
+        case 8: return MT_SCAN_16X8;
+        case 9: return MT_SCAN_16X32;
+        case 10: return MT_SCAN_32X16;
+        case 13: return MT_SCAN_4X16;
+        case 14: return MT_SCAN_16X4;
+        default: return nullptr;
+    }
+}
+
+extern "C" float orc_skip_cost(int qctx, int cls, int ctx, int s) {
+    return cdf_cost(coef_cdfs(qctx).txb_skip[cls][ctx], s);
+}

@@ -136,3 +136,59 @@ def chain_batch(sizes, types, qidx, bd, qm, offsets, pool, nthreads=8, with_hash
                           ptr(pool, ctypes.c_int16), ptr(sse, ctypes.c_int64),
                           ptr(h, ctypes.c_uint32) if with_hash else None, nthreads)
     return sse, h
+
+
+class OrcRateCtx(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "qctx", "cls", "plane", "w", "h", "eob_bin", "skip_ctx", "dcs_ctx", "txtp_set",
+        "txtp_mode", "txtp", "use_skip_bits")] + [("skip_one", ctypes.c_float),
+                                                   ("skip_zero", ctypes.c_float)]
+
+
+def _rate_sigs(L):
+    P = ctypes.POINTER
+    L.orc_block_rate.argtypes = [P(ctypes.c_int32), P(ctypes.c_uint32), P(OrcRateCtx), ctypes.c_float]
+    L.orc_block_rate.restype = ctypes.c_float
+    L.orc_cost_q.argtypes = [ctypes.c_int]
+    L.orc_cost_q.restype = ctypes.c_uint32
+    L.orc_cdf_cost_raw.argtypes = [P(ctypes.c_uint16), ctypes.c_int, ctypes.c_int]
+    L.orc_cdf_cost_raw.restype = ctypes.c_float
+    L.orc_skip_cost.argtypes = [ctypes.c_int] * 4
+    L.orc_skip_cost.restype = ctypes.c_float
+    L.orc_scan.argtypes = [ctypes.c_int]
+    L.orc_scan.restype = P(ctypes.c_uint32)
+
+
+def scan(tx_size):
+    L = lib()
+    if not getattr(L, "_rate_sigs", False):
+        _rate_sigs(L)
+        L._rate_sigs = True
+    p = L.orc_scan(tx_size)
+    n = ncoef(tx_size)
+    return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+
+def rate_ctx(qctx=0, cls=1, plane=0, w=8, h=8, eob_bin=5, skip_ctx=0, dcs_ctx=0, txtp_set=0,
+             txtp_mode=0, txtp=0, skip_bits=None):
+    c = OrcRateCtx(qctx, cls, plane, w, h, eob_bin, skip_ctx, dcs_ctx, txtp_set, txtp_mode, txtp,
+                   1 if skip_bits else 0, *(skip_bits or (0.0, 0.0)))
+    return c
+
+
+def block_rate(cf, scan_tab, ctx, bound=float("inf")):
+    L = lib()
+    if not getattr(L, "_rate_sigs", False):
+        _rate_sigs(L)
+        L._rate_sigs = True
+    cf = np.ascontiguousarray(cf, dtype=np.int32)
+    sc = np.ascontiguousarray(scan_tab, dtype=np.uint32)
+    return L.orc_block_rate(ptr(cf, ctypes.c_int32), ptr(sc, ctypes.c_uint32), ctypes.byref(ctx), bound)
+
+
+def skip_cost(qctx, cls, ctx, s):
+    L = lib()
+    if not getattr(L, "_rate_sigs", False):
+        _rate_sigs(L)
+        L._rate_sigs = True
+    return L.orc_skip_cost(qctx, cls, ctx, s)

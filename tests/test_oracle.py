@@ -462,3 +462,18 @@ def test_sse_recon_brute_force():
                                     O.ptr(src, ctypes.c_uint16), w, w, h, 1023)
         rec = np.clip(pred + resid, 0, 1023)
         assert got == int(((src.astype(np.int64) - rec) ** 2).sum())
+
+
+def test_cdf_tables_match_reference_checksums():
+    """Every table of av1_cdf_tables.h (default coefficient CDFs, scans, context offsets, intra
+    mode CDFs; reference src/coef_q.rs, src/tables.rs) against the FNV-1a the recipe recorded
+    from the reference's text."""
+    hdr = open(os.path.join(ROOT, "maroontree_amd", "csrc", "av1_cdf_tables.h")).read()
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "tables_checksum.json")))
+    names = re.findall(r"static const \w+ MT_(\w+?)(?:\[\d+\])+ = \{", hdr)
+    assert len(names) >= 70
+    for name in names:
+        s = hdr.index(f"MT_{name}[")
+        body = hdr[hdr.index("{", s) + 1:hdr.index("};", s)]
+        vals = [int(x) for x in re.findall(r"-?\d+", body)]
+        assert _fnv(vals, 4) == int(golden["cdf_" + name.lower()], 16), name
