@@ -183,6 +183,29 @@ def main():
             out2.append(f"static const {ctype} MT_{name}{dim_s} = {{")
             out2.append(arr(vals, 16))
             out2.append("};")
+    # Intra prediction tables (reference src/intrapred.rs): the directional derivatives
+    # (:1081), the SMOOTH weights of sm_weights() (:1091) laid out so that the weights of
+    # size n start at index n, and the CfL CDFs (:1048, :1051).
+    intra = open(os.path.join(REF, "intrapred.rs")).read()
+    for name in ("DR_INTRA_DERIVATIVE", "CFL_SIGN_CDF", "CFL_ALPHA_CDF"):
+        ctype, dims, vals = _typed_table(intra, name)
+        sums["cdf_" + name.lower()] = fnv1a(vals, 4)
+        dim_s = "".join(f"[{d}]" for d in dims)
+        out2.append(f"static const {ctype} MT_{name}{dim_s} = {{")
+        out2.append(arr(vals, 16))
+        out2.append("};")
+    body = intra[intra.index("fn sm_weights("):]
+    body = body[:body.index("_ => unreachable")]
+    sm = [0] * 128
+    for n in (4, 8, 16, 32, 64):
+        m = re.search(r"\b%d => &\[([^\]]*)\]" % n, body)
+        w = [int(v) for v in re.findall(r"\d+", m.group(1))]
+        assert len(w) == n, n
+        sm[n:2 * n] = w
+    sums["cdf_sm_weights"] = fnv1a(sm, 4)
+    out2.append("static const int32_t MT_SM_WEIGHTS[128] = {")
+    out2.append(arr(sm, 16))
+    out2.append("};")
     with open(OUT_H2, "w") as f:
         f.write("\n".join(out2) + "\n")
     with open(OUT_J, "w") as f:

@@ -120,6 +120,54 @@ float orc_skip_cost(int qctx, int cls, int ctx, int s);
 /* the scan table of an AV1 TxSize (64x64 uses the 32x32 scan); NULL if none */
 const uint32_t* orc_scan(int tx_size);
 
+
+/* ---- intra prediction (src/intrapred.rs) ------------------------------------------- */
+/* Edges are passed as arrays of 132 samples whose element 2 is logical index 0 (the
+ * reference's IntraEdge, origin 2). */
+int orc_edge_filter_strength(int w, int h, int filter_type, int delta); /* :917 */
+int orc_use_edge_upsample(int w, int h, int filter_type, int delta);    /* :1021 */
+void orc_filter_edge(int32_t* edge_origin2, int size, int strength);    /* :977 */
+void orc_upsample_edge(int32_t* edge_origin2, int num_px, int bd);      /* :1026 */
+/* dr_predict_scalar (:629); zone 1, 2 or 3 */
+void orc_dr_predict(int zone, int bw, int bh, int edge_len, int dx, int dy, int up_above,
+                    int up_left, const int32_t* above_origin2, const int32_t* left_origin2,
+                    int32_t* out);
+int32_t orc_dc_pred_from_sum(int32_t edge_sum, int w, int h, int have_top, int have_left, int bd);
+int32_t orc_dc_pred(const uint16_t* recon, int stride, int ox, int oy, int w, int h, int bd);
+/* filter intra mode 0..4 (:473); w multiple of 4, h even, both <= 32 */
+int orc_filter_intra(int fmode, const uint16_t* recon, int stride, int ox, int oy, int w, int h,
+                     int bd, int32_t* out);
+/* predict_nd_ad (:359) for modes 1..12; mode 0 fills dc_pred; 16+k filter intra mode k.
+ * recon: plane (stride samples per row); (ox, oy) block origin; fw/fh the frame size of the
+ * px_have rules.  out: bw*bh row-major.  Returns 0, or -1 for an invalid call. */
+int orc_intra_pred(int mode, int angle_delta, const uint16_t* recon, int stride, int ox, int oy,
+                   int bw, int bh, int have_tr, int have_bl, int fw, int fh, int filter_type,
+                   int bd, int32_t* out);
+
+/* ---- CfL (src/intrapred.rs:1112-1260) ---------------------------------------------- */
+void orc_cfl_ac_444(const uint16_t* luma, int w, int h, int32_t* ac);
+void orc_cfl_ac_sub(const uint16_t* luma, int lstride, int cw, int ch, int ss_hor, int ss_ver,
+                    int32_t* ac);
+int32_t orc_cfl_pred_pixel(int32_t dc, int32_t ac, int32_t alpha, int bd);
+int32_t orc_cfl_best_alpha(const int32_t* ac, int ac_stride, const uint16_t* src, int src_stride,
+                           int w, int h, int32_t dc, int bd);
+
+/* ---- residual and distortion (src/rd_sse.rs, src/partition_rd.rs) ----------------- */
+void orc_residual_pred(const uint16_t* src, int stride, const int32_t* pred, int w, int h,
+                       int32_t* dst);
+/* plain SSE of clamp(pred + resid) over the visible vw x vh corner; resid may be NULL */
+int64_t orc_sse_recon_vis(const int32_t* pred, const int32_t* resid, const uint16_t* src,
+                          int stride, int w, int vw, int vh, int bd);
+void orc_wsse_tables(float* dct16, float* weights16);
+float orc_weighted_sse_scalar(const int32_t* err, int w, int h);
+float orc_weighted_sse_avx2(const int32_t* err, int w, int h);
+/* weighted_dist: err_vis packed vw wide; avx2 selects the op order of weighted_sse_avx2 */
+int64_t orc_weighted_dist(const int32_t* err_vis, int vw, int vh, float scale, int avx2);
+uint64_t orc_luma_satd(const uint16_t* src, int stride, int w, int h, int vw, int vh, int bd,
+                       const int32_t* pred, const int32_t* resid);
+uint64_t orc_satd_sad_proxy(const uint16_t* src, int stride, int w, int h, int vw, int vh,
+                            const int32_t* pred);
+
 #ifdef __cplusplus
 }
 #endif

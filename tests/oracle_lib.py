@@ -192,3 +192,87 @@ def skip_cost(qctx, cls, ctx, s):
         _rate_sigs(L)
         L._rate_sigs = True
     return L.orc_skip_cost(qctx, cls, ctx, s)
+
+
+# ---- intra prediction, CfL and distortion --------------------------------------------
+
+def _pred_sigs(L):
+    if getattr(L, "_pred_sigs", False):
+        return
+    P, c_int = ctypes.POINTER, ctypes.c_int
+    i32p, u16p = P(ctypes.c_int32), P(ctypes.c_uint16)
+    L.orc_edge_filter_strength.argtypes = [c_int] * 4
+    L.orc_use_edge_upsample.argtypes = [c_int] * 4
+    L.orc_filter_edge.argtypes = [i32p, c_int, c_int]
+    L.orc_upsample_edge.argtypes = [i32p, c_int, c_int]
+    L.orc_dr_predict.argtypes = [c_int] * 8 + [i32p, i32p, i32p]
+    L.orc_dc_pred_from_sum.argtypes = [ctypes.c_int32] + [c_int] * 5
+    L.orc_dc_pred_from_sum.restype = ctypes.c_int32
+    L.orc_dc_pred.argtypes = [u16p] + [c_int] * 6
+    L.orc_dc_pred.restype = ctypes.c_int32
+    L.orc_filter_intra.argtypes = [c_int, u16p] + [c_int] * 6 + [i32p]
+    L.orc_intra_pred.argtypes = [c_int, c_int, u16p] + [c_int] * 11 + [i32p]
+    L.orc_cfl_ac_444.argtypes = [u16p, c_int, c_int, i32p]
+    L.orc_cfl_ac_sub.argtypes = [u16p] + [c_int] * 5 + [i32p]
+    L.orc_cfl_pred_pixel.argtypes = [ctypes.c_int32] * 3 + [c_int]
+    L.orc_cfl_pred_pixel.restype = ctypes.c_int32
+    L.orc_cfl_best_alpha.argtypes = [i32p, c_int, u16p, c_int, c_int, c_int, ctypes.c_int32, c_int]
+    L.orc_cfl_best_alpha.restype = ctypes.c_int32
+    L.orc_residual_pred.argtypes = [u16p, c_int, i32p, c_int, c_int, i32p]
+    L.orc_sse_recon_vis.argtypes = [i32p, i32p, u16p] + [c_int] * 5
+    L.orc_sse_recon_vis.restype = ctypes.c_int64
+    L.orc_wsse_tables.argtypes = [P(ctypes.c_float), P(ctypes.c_float)]
+    L.orc_weighted_sse_scalar.argtypes = [i32p, c_int, c_int]
+    L.orc_weighted_sse_scalar.restype = ctypes.c_float
+    L.orc_weighted_sse_avx2.argtypes = [i32p, c_int, c_int]
+    L.orc_weighted_sse_avx2.restype = ctypes.c_float
+    L.orc_weighted_dist.argtypes = [i32p, c_int, c_int, ctypes.c_float, c_int]
+    L.orc_weighted_dist.restype = ctypes.c_int64
+    L.orc_luma_satd.argtypes = [u16p] + [c_int] * 6 + [i32p, i32p]
+    L.orc_luma_satd.restype = ctypes.c_uint64
+    L.orc_satd_sad_proxy.argtypes = [u16p] + [c_int] * 5 + [i32p]
+    L.orc_satd_sad_proxy.restype = ctypes.c_uint64
+    L._pred_sigs = True
+
+
+def plib():
+    L = lib()
+    _pred_sigs(L)
+    return L
+
+
+def intra_pred(mode, delta, recon, ox, oy, bw, bh, have_tr, have_bl, fw, fh, filter_type, bd):
+    """predict_nd_ad / dc / filter intra on a 2-D uint16 plane; returns (bh, bw) int32."""
+    L = plib()
+    recon = np.ascontiguousarray(recon, dtype=np.uint16)
+    out = np.zeros(bw * bh, np.int32)
+    rc = L.orc_intra_pred(mode, delta, ptr(recon, ctypes.c_uint16), recon.shape[1], ox, oy, bw, bh,
+                          int(have_tr), int(have_bl), fw, fh, int(filter_type), bd,
+                          ptr(out, ctypes.c_int32))
+    assert rc == 0, (mode, delta, bw, bh)
+    return out.reshape(bh, bw)
+
+
+def dr_predict(zone, bw, bh, edge_len, dx, dy, upa, upl, above132, left132):
+    L = plib()
+    a = np.ascontiguousarray(above132, dtype=np.int32)
+    l_ = np.ascontiguousarray(left132, dtype=np.int32)
+    out = np.zeros(bw * bh, np.int32)
+    L.orc_dr_predict(zone, bw, bh, edge_len, dx, dy, upa, upl, ptr(a, ctypes.c_int32),
+                     ptr(l_, ctypes.c_int32), ptr(out, ctypes.c_int32))
+    return out.reshape(bh, bw)
+
+
+def weighted_sse(err, w, h, avx2=True):
+    L = plib()
+    e = np.ascontiguousarray(err, dtype=np.int32)
+    f = L.orc_weighted_sse_avx2 if avx2 else L.orc_weighted_sse_scalar
+    return f(ptr(e, ctypes.c_int32), w, h)
+
+
+def wsse_tables():
+    L = plib()
+    d = np.zeros(16, np.float32)
+    w = np.zeros(16, np.float32)
+    L.orc_wsse_tables(ptr(d, ctypes.c_float), ptr(w, ctypes.c_float))
+    return d.reshape(4, 4), w.reshape(4, 4)
