@@ -142,11 +142,58 @@ typedef struct {
 int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
                   void* stream);
 
-/* Per-kernel timing of mt_tx_chain (no counterpart in the reference; measurement hook).
- * Between begin and end every chain launch is bracketed by HIP events on its stream;
- * end synchronizes them and returns, per slot, the summed milliseconds and launch count.
- * Slot 0 = bucketing (3 small kernels), slots 1..13 = the size kernels in the order
- * 4x4, 4x8, 8x4, 4x16, 16x4, 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 64x64. */
+/* ---- planes and intra prediction -------------------------------------------------- */
+/* A plane of uint16 samples in device memory.  width/height: the frame size the intra
+ * edge rules clip to (predict_nd_ad's fw/fh, src/intrapred.rs:359); every block must lie
+ * inside it.  vis_w/vis_h: the displayed size distortion is measured over (SrcBlock::new's
+ * plane_vis_w/h, src/rd_sse.rs:76).  The allocation holds at least stride*height samples. */
+typedef struct {
+    const uint16_t* data;
+    int32_t stride;
+    int32_t width, height;
+    int32_t vis_w, vis_h;
+    int32_t pad;
+} mt_plane;
+#define MT_MAX_PLANES 4
+
+/* Intra modes (the reference's numbering, src/intrapred.rs:54-82) and filter intra. */
+enum {
+    MT_PRED_DC = 0, MT_PRED_V = 1, MT_PRED_H = 2, MT_PRED_D45 = 3, MT_PRED_D135 = 4,
+    MT_PRED_D113 = 5, MT_PRED_D157 = 6, MT_PRED_D203 = 7, MT_PRED_D67 = 8,
+    MT_PRED_SMOOTH = 9, MT_PRED_SMOOTH_V = 10, MT_PRED_SMOOTH_H = 11, MT_PRED_PAETH = 12,
+    MT_PRED_FILTER0 = 16, MT_PRED_FILTER4 = 20  /* filter intra modes 0..4 */
+};
+enum { MT_PRED_HAVE_TR = 1, MT_PRED_HAVE_BL = 2, MT_PRED_SMOOTH_NB = 4 };
+
+/* One prediction: 16 bytes in HBM. */
+typedef struct {
+    uint32_t out_offset;  /* int32 element offset of the bw*bh row-major prediction */
+    uint16_t x, y;        /* block origin in the plane */
+    uint8_t bw, bh;       /* 4, 8, 16, 32, 64 (filter intra: <= 32) */
+    uint8_t mode;         /* MT_PRED_* */
+    int8_t angle_delta;   /* -3..3, directional modes */
+    uint8_t flags;        /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB (filter_type) */
+    uint8_t bd;           /* 8, 10, 12 */
+    uint8_t plane;        /* index into the plane array */
+    uint8_t pad;
+} mt_predblk;
+
+/* Intra prediction of n blocks from reconstructed planes:
+ *   modes 1..12: IntraPredDispatch::predict_nd_ad (src/intrapred.rs:359, edge filter on);
+ *   MT_PRED_DC: the block filled with IntraPredDispatch::dc_pred (src/intrapred.rs:232);
+ *   filter intra k: IntraPredDispatch::filter_predict (src/intrapred.rs:399).
+ * planes: host array of nplanes (1..MT_MAX_PLANES) views.  pred_d receives each block's
+ * prediction at its out_offset.  status_d (optional, n int32): 0, or MT_ERR_ARG for a
+ * descriptor that is invalid or reaches outside its plane (its output is left untouched). */
+int mt_intra_pred(const mt_predblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+                  int32_t* pred_d, int32_t* status_d, void* stream);
+
+/* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
+ * and end every launch is bracketed by HIP events on its stream; end synchronizes them and
+ * returns, per slot, the summed milliseconds and launch count.  Slot 0 = chain bucketing
+ * (3 small kernels), slots 1..13 = the chain's size kernels in the order 4x4, 4x8, 8x4,
+ * 4x16, 16x4, 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 64x64; 14 = rate, 15 = intra
+ * prediction, 16 = distortion, 17 = candidate evaluation. */
 int mt_profile_begin(void);
 int mt_profile_end(double* kernel_ms, int* launches, int max_slots);
 

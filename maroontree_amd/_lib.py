@@ -44,6 +44,18 @@ BLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("tx_type", "u1"), (
                       ("bd_qm", "u1")])
 RES_DTYPE = np.dtype([("sse", "<i8"), ("nnz", "<u4"), ("hash", "<u4")])
 QUANT_DTYPE = np.dtype([(n, "<i4") for n, _ in MtQuant._fields_])
+PREDBLK_DTYPE = np.dtype([("out_offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("bw", "u1"),
+                          ("bh", "u1"), ("mode", "u1"), ("angle_delta", "i1"), ("flags", "u1"),
+                          ("bd", "u1"), ("plane", "u1"), ("pad", "u1")])
+PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
+PRED_FILTER0 = 16
+MAX_PLANES = 4
+
+
+class MtPlane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("vis_w", ctypes.c_int32), ("vis_h", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 _lib = None
 
@@ -69,6 +81,7 @@ def lib():
         L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
+        L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
@@ -192,7 +205,7 @@ def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=No
 
 
 PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",
-                 "16x32", "32x16", "32x32", "64x64"]
+                 "16x32", "32x16", "32x32", "64x64", "rate", "intra_pred", "distortion", "eval"]
 
 
 def profile_begin():
@@ -228,3 +241,30 @@ def gen_tx_blocks(seed, n, all_sizes=False):
     _check(L.mt_gen_tx_blocks(seed, n, int(all_sizes), blocks.ctypes.data_as(ctypes.c_void_p),
                               pool.ctypes.data_as(ctypes.c_void_p), total))
     return blocks, pool
+
+
+def plane_views(planes):
+    """ctypes array of mt_plane from [(uint16 cuda tensor 2-D, width, height, vis_w, vis_h)].
+    width/height default to the tensor's shape, vis_w/vis_h to width/height."""
+    arr = (MtPlane * max(1, len(planes)))()
+    for i, p in enumerate(planes):
+        t = p[0]
+        h, stride = t.shape
+        w = p[1] if len(p) > 1 else stride
+        hh = p[2] if len(p) > 2 else h
+        vw = p[3] if len(p) > 3 else w
+        vh = p[4] if len(p) > 4 else hh
+        arr[i] = MtPlane(_dev_ptr(t), stride, w, hh, vw, vh, 0)
+    return arr
+
+
+def intra_pred(blocks, planes, pred, status=None, stream=None):
+    """Intra prediction of n blocks.  blocks: uint8 cuda tensor of n*16 bytes (PREDBLK_DTYPE
+    records); planes: list as plane_views(); pred: int32 cuda tensor receiving each block's
+    prediction at its out_offset; status: optional int32 cuda tensor (n,)."""
+    n = blocks.numel() // PREDBLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    pv = plane_views(planes)
+    _check(lib().mt_intra_pred(_dev_ptr(blocks), n, pv, len(planes), _dev_ptr(pred),
+                               _dev_ptr(status) if status is not None else None,
+                               _stream_ptr(stream)))

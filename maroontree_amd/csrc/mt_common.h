@@ -4,11 +4,36 @@
 
 #include <string>
 
+#include "../../include/mtgpu.h"
+
 namespace mtlib {
 // Records the calling thread's last error message and returns `code`.
 int set_error(int code, const std::string& what);
 // Per-device initialisation hooks of the other translation units, called by mt_init.
 int rate_init_device(int device);
+int pred_init_device(int device);
+// The current device if mt_init has set it up (else MT_ERR_NOT_INIT), and its CU count.
+int device_ready(int* dev, int* cus);
+// Copies a host plane array into a by-value kernel argument (a PlaneSet), validating it.
+int planes_from_host(const mt_plane* planes, int nplanes, const char* who, void* dst);
+
+// Kernel timing slots of mt_profile_end.
+enum {
+    kProfBucket = 0,   // 1..13: the transform chain per size
+    kProfRate = 14,
+    kProfPred = 15,
+    kProfDist = 16,
+    kProfEval = 17,
+    kProfSlots = 18
+};
+// While profiling is on (mt_profile_begin), brackets a launch with HIP events on `s`.
+struct ProfScope {
+    int slot;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(int slot_, hipStream_t s_);
+    ~ProfScope();
+};
 }  // namespace mtlib
 
 #define MT_HIP_TRY(expr)                                                                  \

@@ -369,6 +369,7 @@ extern "C" int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* l
     if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !bits_d)))
         return mtlib::set_error(MT_ERR_ARG, "mt_block_rate: bad pointer or count");
     if (n == 0) return MT_OK;
+    mtlib::ProfScope prof(mtlib::kProfRate, (hipStream_t)stream);
     hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, blocks_d, n,
                        levels_d, bits_d);
     MT_HIP_TRY(hipGetLastError());

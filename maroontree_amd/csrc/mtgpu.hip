@@ -959,29 +959,42 @@ hipEvent_t prof_event() {
     return e;
 }
 
-struct ProfScope {
-    int slot;
-    hipStream_t s;
-    hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(int slot_, hipStream_t s_) : slot(slot_), s(s_) {
-        if (!g_prof.on) return;
-        a = prof_event();
-        b = prof_event();
-        if (a) (void)hipEventRecord(a, s);
-    }
-    ~ProfScope() {
-        if (!a || !b) return;
-        (void)hipEventRecord(b, s);
-        g_prof.recs.push_back({slot, {a, b}});
-    }
-};
-
 int current_device(int* dev) {
     HIP_TRY(hipGetDevice(dev));
     if (*dev < 0 || *dev >= 64) return set_err(MT_ERR_ARG, "device index out of range");
     if (!g_dev[*dev].init) return set_err(MT_ERR_NOT_INIT, "mt_init was not called for this device");
     return MT_OK;
 }
+
+}  // namespace
+
+namespace mtlib {
+ProfScope::ProfScope(int slot_, hipStream_t s_) : slot(slot_), s(s_) {
+    if (!g_prof.on) return;
+    a = prof_event();
+    b = prof_event();
+    if (a) (void)hipEventRecord(a, s);
+}
+
+ProfScope::~ProfScope() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, s);
+    g_prof.recs.push_back({slot, {a, b}});
+}
+
+int device_ready(int* dev, int* cus) {
+    int d = 0;
+    MT_HIP_TRY(hipGetDevice(&d));
+    if (d < 0 || d >= 64) return set_error(MT_ERR_ARG, "device index out of range");
+    if (!g_dev[d].init) return set_error(MT_ERR_NOT_INIT, "mt_init was not called for this device");
+    if (dev) *dev = d;
+    if (cus) *cus = g_dev[d].cus;
+    return MT_OK;
+}
+}  // namespace mtlib
+
+namespace {
+using mtlib::ProfScope;
 
 int ensure_order(DeviceState& st, size_t n) {
     if (st.order_cap >= n && st.order) return MT_OK;
@@ -1093,6 +1106,8 @@ extern "C" int mt_init(int device) {
     st.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     {
         int rc = mtlib::rate_init_device(device);
+        if (rc) return rc;
+        rc = mtlib::pred_init_device(device);
         if (rc) return rc;
     }
     HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));

@@ -104,3 +104,14 @@ def test_generator_is_deterministic_and_shaped():
         assert np.abs(blk).max() <= lim
     # discrete Laplace(12): mean |x| = 2 e^(1/12) / (e^(1/6) - 1) / ... ~ 12
     assert 10.0 < np.abs(p1[:200000]).mean() < 14.0
+
+
+def test_descriptor_layouts():
+    """The numpy / ctypes records match the C structs of include/mtgpu.h byte for byte."""
+    assert mt.PREDBLK_DTYPE.itemsize == 16
+    assert ctypes.sizeof(mt.MtPlane) == 32
+    assert mt.RATEBLK_DTYPE.itemsize == 32
+    assert mt.BLK_DTYPE.itemsize == 8 and mt.RES_DTYPE.itemsize == 16
+    assert [mt.PREDBLK_DTYPE.fields[f][1] for f in ("out_offset", "x", "y", "bw", "bh", "mode",
+                                                     "angle_delta", "flags", "bd", "plane")] == \
+        [0, 4, 6, 8, 9, 10, 11, 12, 13, 14]
