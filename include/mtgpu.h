@@ -188,6 +188,39 @@ typedef struct {
 int mt_intra_pred(const mt_predblk* blocks_d, int n, const mt_plane* planes, int nplanes,
                   int32_t* pred_d, int32_t* status_d, void* stream);
 
+/* ---- distortion ---------------------------------------------------------------------- */
+enum { MT_DIST_NO_RESID = 1 };
+
+/* One block to measure: 24 bytes in HBM. */
+typedef struct {
+    uint32_t pred_offset;   /* int32 element offset of the bw*bh row-major prediction */
+    uint32_t resid_offset;  /* int32 element offset of the bw*bh decoded residual */
+    uint16_t x, y;          /* block origin in the source plane */
+    uint8_t bw, bh;         /* 4, 8, 16, 32, 64 */
+    uint8_t bd, plane;      /* bit depth; index into the plane array */
+    uint8_t flags;          /* MT_DIST_NO_RESID: measure the prediction alone */
+    uint8_t pad[3];
+    float wsse_scale;       /* RdDispatch::wsse_scale: > 0 weighted decision distortion */
+} mt_distblk;
+
+/* Measures of one block: 32 bytes. */
+typedef struct {
+    int64_t sse;     /* sum (src - clamp(pred + resid))^2 over the visible part:
+                        sse_recon / chroma_sse with plain SSE (src/rd_sse.rs:454, :682); -1 invalid */
+    int64_t dist;    /* what sse_recon returns under wsse_scale: round(scale * weighted SSE of the
+                        visible error zero-padded to 4x4 tiles), in the op order of
+                        weighted_sse_avx2 (src/rd_sse.rs:244, src/avx/rd.rs:767); = sse if scale <= 0 */
+    uint64_t satd;   /* luma_satd (src/rd_sse.rs:635, src/partition_rd.rs:86): 4x4 Hadamard L1
+                        of the same error, invisible samples at zero error */
+    uint64_t proxy;  /* satd_sad_proxy (src/rd_sse.rs:552, :1143): SAD + SATD/4 of src - pred */
+} mt_distres;
+
+/* Distortion of n reconstructed candidates against their source plane (planes: host array
+ * of views; the vis_w / vis_h of a plane clip every measure, as SrcBlock does). resid_d may
+ * be NULL when every block has MT_DIST_NO_RESID. */
+int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+                  const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d, void* stream);
+
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and
  * returns, per slot, the summed milliseconds and launch count.  Slot 0 = chain bucketing

@@ -47,6 +47,11 @@ QUANT_DTYPE = np.dtype([(n, "<i4") for n, _ in MtQuant._fields_])
 PREDBLK_DTYPE = np.dtype([("out_offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("bw", "u1"),
                           ("bh", "u1"), ("mode", "u1"), ("angle_delta", "i1"), ("flags", "u1"),
                           ("bd", "u1"), ("plane", "u1"), ("pad", "u1")])
+DISTBLK_DTYPE = np.dtype([("pred_offset", "<u4"), ("resid_offset", "<u4"), ("x", "<u2"), ("y", "<u2"),
+                          ("bw", "u1"), ("bh", "u1"), ("bd", "u1"), ("plane", "u1"), ("flags", "u1"),
+                          ("pad", "u1", 3), ("wsse_scale", "<f4")])
+DISTRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("proxy", "<u8")])
+DIST_NO_RESID = 1
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
 PRED_FILTER0 = 16
 MAX_PLANES = 4
@@ -82,6 +87,7 @@ def lib():
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
+        L.mt_distortion.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
@@ -268,3 +274,17 @@ def intra_pred(blocks, planes, pred, status=None, stream=None):
     _check(lib().mt_intra_pred(_dev_ptr(blocks), n, pv, len(planes), _dev_ptr(pred),
                                _dev_ptr(status) if status is not None else None,
                                _stream_ptr(stream)))
+
+
+def distortion(blocks, planes, pred, resid=None, stream=None):
+    """Distortion measures of n blocks.  blocks: uint8 cuda tensor of n*24 bytes
+    (DISTBLK_DTYPE records); returns a uint8 cuda tensor of n*32 bytes (DISTRES_DTYPE)."""
+    import torch
+    n = blocks.numel() // DISTBLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    out = torch.empty(n * DISTRES_DTYPE.itemsize, dtype=torch.uint8, device=blocks.device)
+    pv = plane_views(planes)
+    _check(lib().mt_distortion(_dev_ptr(blocks), n, pv, len(planes), _dev_ptr(pred),
+                               _dev_ptr(resid) if resid is not None else None, _dev_ptr(out),
+                               _stream_ptr(stream)))
+    return out

@@ -14,8 +14,14 @@ int rate_init_device(int device);
 int pred_init_device(int device);
 // The current device if mt_init has set it up (else MT_ERR_NOT_INIT), and its CU count.
 int device_ready(int* dev, int* cus);
-// Copies a host plane array into a by-value kernel argument (a PlaneSet), validating it.
-int planes_from_host(const mt_plane* planes, int nplanes, const char* who, void* dst);
+// The planes of one launch, passed to the kernel by value.
+struct PlaneSet {
+    mt_plane p[MT_MAX_PLANES];
+    int n;
+};
+// Copies a host plane array into a PlaneSet, validating it.
+int planes_from_host(const mt_plane* planes, int nplanes, const char* who, PlaneSet* ps);
+int dist_init_device(int device);
 
 // Kernel timing slots of mt_profile_end.
 enum {

@@ -1,0 +1,215 @@
+// Distortion of reconstructed candidates on the GPU: the reference encoder's RdDispatch
+// measures (src/rd_sse.rs:244-271, :454, :552, :635, :682) over a batch of blocks.
+//
+// One wave per block, persistent over the batch.  Phase 1 builds the block's error
+// src - clamp(pred + resid) in LDS, zero outside the displayed part, and reduces the plain
+// SSE.  Phase 2 takes one 4x4 tile per lane for the Hadamard measures.  Phase 3 is the
+// weighted SSE in the op order of the reference's x86 kernel weighted_sse_avx2
+// (src/avx/rd.rs:767-849): lane a < 32 owns accumulator (k, half, l) = (a >> 3, (a >> 2) & 1,
+// a & 3) and runs its fused chain over the tile pairs in order; lanes 0..3 then apply the
+// fixed reduction tree and the 4-wide tail column.  Every product is rounded as the
+// reference rounds it (__fmul_rn / __fmaf_rn / __fadd_rn).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/mtgpu.h"
+#include "mt_common.h"
+
+namespace {
+
+using mtlib::PlaneSet;
+
+__constant__ float c_wd[4][4];  // WSSE_DCT4 (src/rd_sse.rs:1420)
+__constant__ float c_ww[4][4];  // WSSE_WEIGHTS (src/rd_sse.rs:1402)
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+__device__ __forceinline__ void had4(int a, int b, int c, int d, int* o) {
+    const int e = a + c, f = a - c, g = b + d, h = b - d;
+    o[0] = e + g;
+    o[1] = f + h;
+    o[2] = f - h;
+    o[3] = e - g;
+}
+
+// Sum |H e H^T| of one 4x4 tile (had4 on rows, then on columns; luma_satd_scalar,
+// src/partition_rd.rs:86).
+template <typename T> __device__ uint32_t tile_satd(const T* e, int stride) {
+    int rows[4][4];
+    for (int r = 0; r < 4; ++r) had4(e[r * stride], e[r * stride + 1], e[r * stride + 2], e[r * stride + 3], rows[r]);
+    uint32_t s = 0;
+    for (int x = 0; x < 4; ++x) {
+        int col[4];
+        had4(rows[0][x], rows[1][x], rows[2][x], rows[3][x], col);
+        for (int i = 0; i < 4; ++i) s += (uint32_t)abs(col[i]);
+    }
+    return s;
+}
+
+// Coefficient l of row frequency k of the 4x4 tile at (r0, c0) of the error block, in the
+// vector program's order: t[j] = e0j*D[k][0] then three fmas down the column; then
+// c = D[l][0]*t0 and three fmas across.
+__device__ __forceinline__ float wsse_coef(const int16_t* e, int stride, int r0, int c0, int k, int l) {
+    float t[4];
+    for (int j = 0; j < 4; ++j) {
+        const int16_t* col = e + r0 * stride + c0 + j;
+        float v = __fmul_rn((float)col[0], c_wd[k][0]);
+        v = __fmaf_rn((float)col[stride], c_wd[k][1], v);
+        v = __fmaf_rn((float)col[2 * stride], c_wd[k][2], v);
+        v = __fmaf_rn((float)col[3 * stride], c_wd[k][3], v);
+        t[j] = v;
+    }
+    float c = __fmul_rn(c_wd[l][0], t[0]);
+    c = __fmaf_rn(c_wd[l][1], t[1], c);
+    c = __fmaf_rn(c_wd[l][2], t[2], c);
+    c = __fmaf_rn(c_wd[l][3], t[3], c);
+    return c;
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+constexpr int kES = 65;  // LDS error row stride (64 + 1)
+
+__global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__ blocks, int n,
+                                                  PlaneSet planes, const int32_t* __restrict__ pred,
+                                                  const int32_t* __restrict__ resid,
+                                                  mt_distres* __restrict__ out) {
+    __shared__ int16_t s_e[64 * kES];  // error src - recon (|e| < 2^12), zero where invisible
+    __shared__ float s_acc[32];
+    __shared__ float s_sum[4];
+    const int lane = threadIdx.x;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const mt_distblk d = blocks[b];
+        const int bw = d.bw, bh = d.bh, bd = d.bd;
+        bool ok = (bw == 4 || bw == 8 || bw == 16 || bw == 32 || bw == 64) &&
+                  (bh == 4 || bh == 8 || bh == 16 || bh == 32 || bh == 64) &&
+                  (bd == 8 || bd == 10 || bd == 12) && d.plane < planes.n;
+        const mt_plane pl = planes.p[ok ? d.plane : 0];
+        ok = ok && (int)d.x + bw <= pl.width && (int)d.y + bh <= pl.height;
+        if (!ok) {
+            if (lane == 0) out[b] = mt_distres{-1, -1, 0, 0};
+            continue;
+        }
+        const int vw = min(max(pl.vis_w - (int)d.x, 0), bw);
+        const int vh = min(max(pl.vis_h - (int)d.y, 0), bh);
+        const int maxv = (1 << bd) - 1;
+        const bool has_res = !(d.flags & MT_DIST_NO_RESID);
+        const int32_t* pr = pred + d.pred_offset;
+        const int32_t* rs = resid + d.resid_offset;
+        const uint16_t* src = pl.data + (size_t)d.y * pl.stride + d.x;
+        const int lw = __builtin_ctz(bw);
+        // ---- phase 1: error blocks and the plain SSE (sse_recon / chroma_sse)
+        int64_t sse = 0;
+        for (int p = lane; p < bw * bh; p += 64) {
+            const int y = p >> lw, x = p & (bw - 1);
+            int e = 0;
+            if (x < vw && y < vh)
+                e = (int)src[(size_t)y * pl.stride + x] - clampi(pr[p] + (has_res ? rs[p] : 0), 0, maxv);
+            s_e[y * kES + x] = (int16_t)e;
+            sse += (int64_t)e * e;
+        }
+        sse = wave_sum(sse);
+        __syncthreads();
+        // ---- phase 2: SATD of the reconstruction error, SAD + SATD/4 of the prediction error
+        uint64_t satd = 0, psatd = 0, sad = 0;
+        const int tiles = (bw >> 2) * (bh >> 2);
+        for (int t = lane; t < tiles; t += 64) {
+            const int ty = (t / (bw >> 2)) * 4, tx = (t % (bw >> 2)) * 4;
+            satd += tile_satd(s_e + ty * kES + tx, kES);
+            int ep[16];  // src - pred of the tile, zero where invisible
+            for (int r = 0; r < 4; ++r)
+                for (int x = 0; x < 4; ++x) {
+                    const int yy = ty + r, xx = tx + x;
+                    ep[r * 4 + x] = (xx < vw && yy < vh)
+                                        ? (int)src[(size_t)yy * pl.stride + xx] - pr[yy * bw + xx] : 0;
+                    sad += (uint64_t)abs(ep[r * 4 + x]);
+                }
+            psatd += tile_satd(ep, 4);
+        }
+        satd = wave_sum(satd);
+        psatd = wave_sum(psatd);
+        sad = wave_sum(sad);
+        // ---- phase 3: weighted SSE of the visible error, zero-padded to whole 4x4 tiles
+        int64_t dist = sse;
+        if (d.wsse_scale > 0.0f) {
+            const int pw = (vw + 3) & ~3, ph = (vh + 3) & ~3;
+            float total = 0.0f;
+            if (pw > 0 && ph > 0) {
+                const int pairs = pw / 8 * 8;
+                if (lane < 32) {
+                    const int k = lane >> 3, half = (lane >> 2) & 1, l = lane & 3;
+                    float acc = 0.0f;
+                    const float wk = c_ww[k][l];
+                    for (int r0 = 0; r0 < ph; r0 += 4)
+                        for (int c0 = 0; c0 < pairs; c0 += 8) {
+                            const float c = wsse_coef(s_e, kES, r0, c0 + 4 * half, k, l);
+                            acc = __fmaf_rn(__fmul_rn(wk, c), c, acc);
+                        }
+                    s_acc[lane] = acc;
+                }
+                __syncthreads();
+                if (lane < 4) {
+                    const int l = lane;
+                    auto A = [&](int k, int half) { return s_acc[k * 8 + half * 4 + l]; };
+                    const float s01 = __fadd_rn(__fadd_rn(A(0, 0), A(0, 1)), __fadd_rn(A(1, 0), A(1, 1)));
+                    const float s23 = __fadd_rn(__fadd_rn(A(2, 0), A(2, 1)), __fadd_rn(A(3, 0), A(3, 1)));
+                    float sum = __fadd_rn(s01, s23);
+                    if (pairs < pw)
+                        for (int r0 = 0; r0 < ph; r0 += 4)
+                            for (int k = 0; k < 4; ++k) {
+                                const float c = wsse_coef(s_e, kES, r0, pairs, k, l);
+                                sum = __fmaf_rn(__fmul_rn(c_ww[k][l], c), c, sum);
+                            }
+                    s_sum[l] = sum;
+                }
+                __syncthreads();
+                total = __fadd_rn(__fadd_rn(s_sum[0], s_sum[2]), __fadd_rn(s_sum[1], s_sum[3]));
+            }
+            dist = (int64_t)roundf(__fmul_rn(total, d.wsse_scale));
+        }
+        if (lane == 0) out[b] = mt_distres{sse, dist, satd, sad + (psatd >> 2)};
+        __syncthreads();  // LDS reuse by the next block
+    }
+}
+
+}  // namespace
+
+namespace mtlib {
+int dist_init_device(int /*device*/) {
+    // WSSE_DCT4: the f32 literals of the reference; WSSE_WEIGHTS: the raw table divided by
+    // its mean (0.4625) in f32, as the reference's const evaluation rounds it.
+    const float a = 0.5f, bb = 0.6532815f, c = 0.27059805f;
+    const float wd[4][4] = {{a, a, a, a}, {bb, c, -c, -bb}, {a, -a, -a, a}, {c, -bb, bb, -c}};
+    volatile float mean = 0.4625f;
+    const float dc = 1.8f / mean, diag = 0.8f / mean, axis = 0.4f / mean, hf = 0.2f / mean,
+                ac1 = 1.0f / mean;
+    const float ww[4][4] = {{dc, ac1, axis, hf}, {ac1, diag, hf, hf}, {axis, hf, hf, hf}, {hf, hf, hf, hf}};
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wd), wd, sizeof(wd)));
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_ww), ww, sizeof(ww)));
+    return MT_OK;
+}
+}  // namespace mtlib
+
+extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+                             const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d,
+                             void* stream) {
+    if (n < 0 || (n > 0 && (!blocks_d || !pred_d || !out_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_distortion: bad pointer or count");
+    PlaneSet ps{};
+    int rc = mtlib::planes_from_host(planes, nplanes, "mt_distortion", &ps);
+    if (rc != MT_OK) return rc;
+    if (n == 0) return MT_OK;
+    int dev = 0, cus = 0;
+    rc = mtlib::device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    const int grid = n < cus * 32 ? n : cus * 32;
+    mtlib::ProfScope prof(mtlib::kProfDist, (hipStream_t)stream);
+    hipLaunchKernelGGL(dist_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, blocks_d, n, ps,
+                       pred_d, resid_d ? resid_d : pred_d, out_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}

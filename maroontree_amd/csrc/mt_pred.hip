@@ -24,10 +24,7 @@ __constant__ int8_t c_taps[5][8][7];  // INTRA_FILTER_TAPS (src/tables.rs:50)
 __constant__ int32_t c_deriv[44];     // DR_INTRA_DERIVATIVE (src/intrapred.rs:1081)
 __constant__ int32_t c_smw[128];      // sm_weights(n) at [n, 2n) (src/intrapred.rs:1091)
 
-struct PlaneSet {
-    mt_plane p[MT_MAX_PLANES];
-    int n;
-};
+using mtlib::PlaneSet;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
@@ -338,8 +335,7 @@ int pred_init_device(int /*device*/) {
     return MT_OK;
 }
 
-int planes_from_host(const mt_plane* planes, int nplanes, const char* who, void* dst) {
-    PlaneSet* ps = static_cast<PlaneSet*>(dst);
+int planes_from_host(const mt_plane* planes, int nplanes, const char* who, PlaneSet* ps) {
     if (nplanes < 1 || nplanes > MT_MAX_PLANES || !planes)
         return set_error(MT_ERR_ARG, std::string(who) + ": 1..4 planes required");
     for (int i = 0; i < nplanes; ++i) {

@@ -1109,6 +1109,8 @@ extern "C" int mt_init(int device) {
         if (rc) return rc;
         rc = mtlib::pred_init_device(device);
         if (rc) return rc;
+        rc = mtlib::dist_init_device(device);
+        if (rc) return rc;
     }
     HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));

@@ -109,6 +109,7 @@ def test_generator_is_deterministic_and_shaped():
 def test_descriptor_layouts():
     """The numpy / ctypes records match the C structs of include/mtgpu.h byte for byte."""
     assert mt.PREDBLK_DTYPE.itemsize == 16
+    assert mt.DISTBLK_DTYPE.itemsize == 24 and mt.DISTRES_DTYPE.itemsize == 32
     assert ctypes.sizeof(mt.MtPlane) == 32
     assert mt.RATEBLK_DTYPE.itemsize == 32
     assert mt.BLK_DTYPE.itemsize == 8 and mt.RES_DTYPE.itemsize == 16
