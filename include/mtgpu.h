@@ -110,7 +110,8 @@ int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* levels_d,
 /* fwd -> quant(+QM) -> dequant -> inverse -> SSE for n blocks of mixed size/type/q/bd.
  * pool_d holds the int16 residuals.  res_d: n results.  levels_d / targets_d / recon_d
  * are optional (NULL) full outputs laid out like pool_d: block i's levels, targets
- * (ncoef, x-major) and reconstructed residual (W*H, int32) start at blocks[i].offset.
+ * (ncoef, x-major) and reconstructed residual (W*H, int32) start at blocks[i].offset;
+ * targets_d needs levels_d.
  * Blocks of unsupported (size, type) get sse = -1.  Uses a library-owned workspace on
  * the device (allocated on first use, grown as needed). */
 int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
@@ -220,6 +221,50 @@ typedef struct {
  * be NULL when every block has MT_DIST_NO_RESID. */
 int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
                   const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d, void* stream);
+
+/* ---- batched candidate evaluation -------------------------------------------------- */
+/* One intra candidate: 48 bytes in HBM.  The block is one transform block of tx_size. */
+typedef struct {
+    uint32_t offset;      /* element offset of the candidate's W*H slots in the work pools */
+    uint16_t x, y;        /* block origin (the source and reconstruction planes share it) */
+    uint8_t tx_size;      /* AV1 TxSize, one of the 13 sizes the reference transforms */
+    uint8_t tx_type;      /* AV1 TxType */
+    uint8_t mode;         /* MT_PRED_* */
+    int8_t angle_delta;   /* -3..3 */
+    uint8_t pflags;       /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB */
+    uint8_t planes;       /* bits 0-1: source plane index, bits 2-3: reconstruction plane index */
+    uint8_t qidx;         /* base_q_idx of the luma Quant::new_with_qm */
+    uint8_t bd_qm;        /* as mt_txblk: bits 0-3 QM level, bits 4-5 bit depth (0 8, 1 10, 2 12) */
+    uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;  /* rate context, as mt_rateblk */
+    uint8_t txtp_set, txtp_mode, txtp, qctx;
+    uint8_t rflags, pad0, pad1, pad2;
+    float skip_one, skip_zero;
+    float wsse_scale;     /* RdDispatch::wsse_scale: > 0 prices the weighted distortion */
+    float lambda;         /* rd_cost multiplier */
+    float extra_bits;     /* added to the coefficient bits (mode, angle delta, tx type ...) */
+} mt_cand;
+
+/* Result of one candidate: 40 bytes. */
+typedef struct {
+    int64_t sse;      /* plain SSE of the reconstruction over the displayed part */
+    int64_t dist;     /* decision distortion (weighted when wsse_scale > 0), as sse_recon */
+    uint64_t satd;    /* luma_satd of the reconstruction error */
+    float bits;       /* real_block_bits_bounded + extra_bits */
+    float cost;       /* rd_cost(dist, lambda, bits) = (float)dist + lambda * bits; +inf if invalid */
+    uint32_t nnz;     /* nonzero levels */
+    int32_t status;   /* 0; MT_ERR_ARG invalid candidate; MT_ERR_UNSUPPORTED (size, type) */
+} mt_candres;
+
+/* The reference's candidate loop body (src/coder/partition_search.rs:471-520) for n
+ * candidates at once: prediction from the reconstruction plane, residual against the source
+ * plane, forward transform + quant (+QM), dequant + inverse, reconstruction distortion
+ * (mt_distortion), coefficient rate (mt_block_rate) and rd_cost (src/partition_rd.rs:140).
+ * Trellis is not applied (the Speed::Fast path).  pool_elems bounds every candidate's
+ * offset + W*H; the stage pools (predictions, residuals, levels, reconstructed residuals)
+ * are library-owned device memory of that many elements, grown on demand.  Calls on one
+ * device share them: issue them on one stream. */
+int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes, int nplanes,
+                       uint64_t pool_elems, mt_candres* res_d, void* stream);
 
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and

@@ -52,6 +52,15 @@ DISTBLK_DTYPE = np.dtype([("pred_offset", "<u4"), ("resid_offset", "<u4"), ("x",
                           ("pad", "u1", 3), ("wsse_scale", "<f4")])
 DISTRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("proxy", "<u8")])
 DIST_NO_RESID = 1
+CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size", "u1"), ("tx_type", "u1"),
+                       ("mode", "u1"), ("angle_delta", "i1"), ("pflags", "u1"), ("planes", "u1"),
+                       ("qidx", "u1"), ("bd_qm", "u1"), ("rplane", "u1"), ("eob_bin", "u1"),
+                       ("skip_ctx", "u1"), ("dcs_ctx", "u1"), ("txtp_set", "u1"), ("txtp_mode", "u1"),
+                       ("txtp", "u1"), ("qctx", "u1"), ("rflags", "u1"), ("pad", "u1", 3),
+                       ("skip_one", "<f4"), ("skip_zero", "<f4"), ("wsse_scale", "<f4"),
+                       ("lambda", "<f4"), ("extra_bits", "<f4")])
+CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
+                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
 PRED_FILTER0 = 16
 MAX_PLANES = 4
@@ -88,6 +97,7 @@ def lib():
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
         L.mt_distortion.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
+        L.mt_eval_candidates.argtypes = [vp, i32, P(MtPlane), i32, u64, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
@@ -287,4 +297,19 @@ def distortion(blocks, planes, pred, resid=None, stream=None):
     _check(lib().mt_distortion(_dev_ptr(blocks), n, pv, len(planes), _dev_ptr(pred),
                                _dev_ptr(resid) if resid is not None else None, _dev_ptr(out),
                                _stream_ptr(stream)))
+    return out
+
+
+def eval_candidates(cands, planes, pool_elems, stream=None, out=None):
+    """Evaluate n intra candidates.  cands: uint8 cuda tensor of n*48 bytes (CAND_DTYPE
+    records); planes: list as plane_views(); returns a uint8 cuda tensor of n*40 bytes
+    (CANDRES_DTYPE)."""
+    import torch
+    n = cands.numel() // CAND_DTYPE.itemsize
+    init(cands.device.index or 0)
+    if out is None:
+        out = torch.empty(n * CANDRES_DTYPE.itemsize, dtype=torch.uint8, device=cands.device)
+    pv = plane_views(planes)
+    _check(lib().mt_eval_candidates(_dev_ptr(cands), n, pv, len(planes), int(pool_elems),
+                                    _dev_ptr(out), _stream_ptr(stream)))
     return out

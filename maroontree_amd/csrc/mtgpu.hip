@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
             if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
                 const size_t o = (MODE == MODE_FWD ? (size_t)gidx[b] * N : (size_t)goff[b]) + rc;
                 a.levels[o] = lvl;
-                a.targets[o] = tgt;
+                if (MODE == MODE_FWD || a.targets) a.targets[o] = tgt;
             }
             if (MODE == MODE_CHAIN) {
                 atomicAdd(&s_hash[b], coef_hash(rc, lvl, tgt));
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                     if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
                         const size_t o = (MODE == MODE_FWD ? (size_t)g * 1024 : (size_t)off) + rc;
                         a.levels[o] = lvl;
-                        a.targets[o] = tgt;
+                        if (MODE == MODE_FWD || a.targets) a.targets[o] = tgt;
                     }
                     if (MODE == MODE_CHAIN) {
                         int32_t step = rc == 0 ? prm.dc : prm.ac;
@@ -1240,8 +1240,8 @@ extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!blocks_d || !pool_d || !res_d)))
         return set_err(MT_ERR_ARG, "mt_tx_chain: bad pointer or count");
-    if ((levels_d == nullptr) != (targets_d == nullptr))
-        return set_err(MT_ERR_ARG, "mt_tx_chain: levels and targets go together");
+    if (targets_d && !levels_d)
+        return set_err(MT_ERR_ARG, "mt_tx_chain: targets need levels");
     if (n == 0) return MT_OK;
     std::lock_guard<std::mutex> lk(g_mu);
     DeviceState& st = g_dev[dev];
