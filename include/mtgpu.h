@@ -189,6 +189,32 @@ typedef struct {
 int mt_intra_pred(const mt_predblk* blocks_d, int n, const mt_plane* planes, int nplanes,
                   int32_t* pred_d, int32_t* status_d, void* stream);
 
+/* ---- chroma from luma ------------------------------------------------------------- */
+enum { MT_CFL_SS_HOR = 1, MT_CFL_SS_VER = 2, MT_CFL_ALPHA_GIVEN = 4 };
+
+/* One CfL block: 24 bytes. */
+typedef struct {
+    uint32_t out_offset;    /* int32 element offset of the cw*ch prediction (and AC) */
+    uint16_t lx, ly;        /* origin of the covered luma in the luma plane */
+    uint16_t x, y;          /* chroma block origin in the source plane */
+    uint8_t cw, ch;         /* 4, 8, 16, 32 */
+    uint8_t flags;          /* MT_CFL_SS_HOR | MT_CFL_SS_VER | MT_CFL_ALPHA_GIVEN */
+    uint8_t bd;
+    uint8_t luma_plane, src_plane;  /* indices into the plane array */
+    int8_t alpha;           /* used with MT_CFL_ALPHA_GIVEN */
+    uint8_t pad;
+    int32_t dc;             /* the chroma DC prediction the AC is added to */
+} mt_cflblk;
+
+/* Chroma-from-luma for n blocks: the luma AC of IntraPredDispatch::cfl_ac_sub (cfl_ac_444
+ * when not subsampled; src/intrapred.rs:1112, :1133) from the reconstructed luma plane,
+ * the alpha of cfl_best_alpha (analytic optimum, then the +/-3 energy search over the
+ * displayed part of the source, :1218) unless given, and cfl_pred_pixel (:1194).
+ * pred_d: the prediction at out_offset; ac_d (optional): the AC at out_offset;
+ * alpha_d: n alphas (-128 for an invalid descriptor, whose outputs are left untouched). */
+int mt_cfl(const mt_cflblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+           int32_t* pred_d, int32_t* ac_d, int32_t* alpha_d, void* stream);
+
 /* ---- distortion ---------------------------------------------------------------------- */
 enum { MT_DIST_NO_RESID = 1 };
 

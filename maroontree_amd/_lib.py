@@ -52,6 +52,10 @@ DISTBLK_DTYPE = np.dtype([("pred_offset", "<u4"), ("resid_offset", "<u4"), ("x",
                           ("pad", "u1", 3), ("wsse_scale", "<f4")])
 DISTRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("proxy", "<u8")])
 DIST_NO_RESID = 1
+CFLBLK_DTYPE = np.dtype([("out_offset", "<u4"), ("lx", "<u2"), ("ly", "<u2"), ("x", "<u2"), ("y", "<u2"),
+                         ("cw", "u1"), ("ch", "u1"), ("flags", "u1"), ("bd", "u1"), ("luma_plane", "u1"),
+                         ("src_plane", "u1"), ("alpha", "i1"), ("pad", "u1"), ("dc", "<i4")])
+CFL_SS_HOR, CFL_SS_VER, CFL_ALPHA_GIVEN = 1, 2, 4
 CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size", "u1"), ("tx_type", "u1"),
                        ("mode", "u1"), ("angle_delta", "i1"), ("pflags", "u1"), ("planes", "u1"),
                        ("qidx", "u1"), ("bd_qm", "u1"), ("rplane", "u1"), ("eob_bin", "u1"),
@@ -97,6 +101,7 @@ def lib():
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
         L.mt_distortion.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
+        L.mt_cfl.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
         L.mt_eval_candidates.argtypes = [vp, i32, P(MtPlane), i32, u64, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
@@ -313,3 +318,16 @@ def eval_candidates(cands, planes, pool_elems, stream=None, out=None):
     _check(lib().mt_eval_candidates(_dev_ptr(cands), n, pv, len(planes), int(pool_elems),
                                     _dev_ptr(out), _stream_ptr(stream)))
     return out
+
+
+def cfl(blocks, planes, pred, ac=None, stream=None):
+    """Chroma from luma for n blocks (CFLBLK_DTYPE records in a uint8 cuda tensor); writes
+    the predictions (and the AC) at each out_offset; returns the alphas (int32 cuda (n,))."""
+    import torch
+    n = blocks.numel() // CFLBLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    alpha = torch.empty(n, dtype=torch.int32, device=blocks.device)
+    pv = plane_views(planes)
+    _check(lib().mt_cfl(_dev_ptr(blocks), n, pv, len(planes), _dev_ptr(pred),
+                        _dev_ptr(ac) if ac is not None else None, _dev_ptr(alpha), _stream_ptr(stream)))
+    return alpha

@@ -325,6 +325,124 @@ __global__ __launch_bounds__(64) void intra_pred_kernel(const mt_predblk* __rest
     }
 }
 
+
+// ---- chroma from luma (src/intrapred.rs:1112-1260) -----------------------------------------
+
+__device__ __forceinline__ int cfl_pixel(int dc, int ac, int alpha, int maxv) {
+    const int diff = alpha * ac;
+    const int mag = (abs(diff) + 32) >> 6;
+    return clampi(dc + (diff < 0 ? -mag : mag), 0, maxv);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One wave per block: cfl_ac_sub's subsampled, scaled luma minus its rounded mean;
+// cfl_best_alpha_strided's analytic alpha and its +/-3 energy search over the displayed part
+// of the source; cfl_pred_pixel over the whole block.
+__global__ __launch_bounds__(64) void cfl_kernel(const mt_cflblk* __restrict__ blocks, int n,
+                                                 PlaneSet planes, int32_t* __restrict__ pred,
+                                                 int32_t* __restrict__ ac_out,
+                                                 int32_t* __restrict__ alpha_out) {
+    __shared__ int32_t s_ac[32 * 32];
+    const int lane = threadIdx.x;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const mt_cflblk d = blocks[b];
+        const int cw = d.cw, ch = d.ch, bd = d.bd;
+        const bool sh = d.flags & MT_CFL_SS_HOR, sv = d.flags & MT_CFL_SS_VER;
+        bool ok = (cw == 4 || cw == 8 || cw == 16 || cw == 32) &&
+                  (ch == 4 || ch == 8 || ch == 16 || ch == 32) && (bd == 8 || bd == 10 || bd == 12) &&
+                  d.luma_plane < planes.n && d.src_plane < planes.n;
+        const mt_plane lp = planes.p[ok ? d.luma_plane : 0];
+        const mt_plane sp = planes.p[ok ? d.src_plane : 0];
+        ok = ok && (int)d.lx + (cw << (sh ? 1 : 0)) <= lp.width && (int)d.ly + (ch << (sv ? 1 : 0)) <= lp.height &&
+             (int)d.x + cw <= sp.width && (int)d.y + ch <= sp.height;
+        if (!ok) {
+            if (lane == 0) alpha_out[b] = -128;
+            continue;
+        }
+        const int lcw = __builtin_ctz(cw), npx = cw * ch;
+        const int shift = 1 + (sv ? 0 : 1) + (sh ? 0 : 1);
+        const uint16_t* L = lp.data + (size_t)d.ly * lp.stride + d.lx;
+        int64_t part = 0;
+        for (int p = lane; p < npx; p += 64) {
+            const int y = p >> lcw, x = p & (cw - 1);
+            const uint16_t* r0 = L + (size_t)(y << (sv ? 1 : 0)) * lp.stride + (x << (sh ? 1 : 0));
+            int s = r0[0];
+            if (sh) s += r0[1];
+            if (sv) {
+                s += r0[lp.stride];
+                if (sh) s += r0[lp.stride + 1];
+            }
+            s <<= shift;
+            s_ac[p] = s;
+            part += s;
+        }
+        const int log2sz = lcw + __builtin_ctz(ch);
+        const int64_t sum = wave_sum(part) + ((int64_t(1) << log2sz) >> 1);
+        const int mean = (int)(sum >> log2sz);
+        __syncthreads();
+        for (int p = lane; p < npx; p += 64) s_ac[p] -= mean;
+        __syncthreads();
+        const int maxv = (1 << bd) - 1;
+        const int dc = d.dc;
+        const int vw = min(max(sp.vis_w - (int)d.x, 0), cw), vh = min(max(sp.vis_h - (int)d.y, 0), ch);
+        const uint16_t* S = sp.data + (size_t)d.y * sp.stride + d.x;
+        int alpha = d.alpha;
+        if (!(d.flags & MT_CFL_ALPHA_GIVEN)) {
+            int64_t num = 0, den = 0;
+            for (int p = lane; p < npx; p += 64) {
+                const int y = p >> lcw, x = p & (cw - 1);
+                if (x < vw && y < vh) {
+                    const int64_t a = s_ac[p];
+                    num += (int64_t)((int)S[(size_t)y * sp.stride + x] - dc) * a;
+                    den += a * a;
+                }
+            }
+            num = wave_sum(num);
+            den = wave_sum(den);
+            if (den == 0 || vw == 0 || vh == 0) {
+                alpha = 0;
+            } else {
+                const int64_t sg = num > 0 ? 1 : (num < 0 ? -1 : 0);
+                int64_t a0 = (64 * num + (den >> 1) * sg) / den;
+                a0 = a0 < -16 ? -16 : (a0 > 16 ? 16 : a0);
+                int64_t e[7] = {0, 0, 0, 0, 0, 0, 0};
+                for (int p = lane; p < npx; p += 64) {
+                    const int y = p >> lcw, x = p & (cw - 1);
+                    if (x < vw && y < vh) {
+                        const int s = S[(size_t)y * sp.stride + x];
+                        for (int k = 0; k < 7; ++k) {
+                            const int64_t dd = s - cfl_pixel(dc, s_ac[p], (int)a0 - 3 + k, maxv);
+                            e[k] += dd * dd;
+                        }
+                    }
+                }
+                int64_t best_e = INT64_MAX;
+                alpha = 0;
+                for (int k = 0; k < 7; ++k) {
+                    const int c = (int)a0 - 3 + k;
+                    const int64_t ek = wave_sum(e[k]);
+                    if (c < -16 || c > 16) continue;
+                    if (ek < best_e) {
+                        best_e = ek;
+                        alpha = c;
+                    }
+                }
+            }
+        }
+        if (lane == 0) alpha_out[b] = alpha;
+        int32_t* dst = pred + d.out_offset;
+        for (int p = lane; p < npx; p += 64) {
+            dst[p] = cfl_pixel(dc, s_ac[p], alpha, maxv);
+            if (ac_out) ac_out[d.out_offset + p] = s_ac[p];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 namespace mtlib {
@@ -365,6 +483,25 @@ extern "C" int mt_intra_pred(const mt_predblk* blocks_d, int n, const mt_plane* 
     mtlib::ProfScope prof(mtlib::kProfPred, (hipStream_t)stream);
     hipLaunchKernelGGL(intra_pred_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, blocks_d, n,
                        ps, pred_d, status_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+
+extern "C" int mt_cfl(const mt_cflblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+                      int32_t* pred_d, int32_t* ac_d, int32_t* alpha_d, void* stream) {
+    if (n < 0 || (n > 0 && (!blocks_d || !pred_d || !alpha_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_cfl: bad pointer or count");
+    PlaneSet ps{};
+    int rc = mtlib::planes_from_host(planes, nplanes, "mt_cfl", &ps);
+    if (rc != MT_OK) return rc;
+    if (n == 0) return MT_OK;
+    int dev = 0, cus = 0;
+    rc = mtlib::device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    const int grid = n < cus * 32 ? n : cus * 32;
+    mtlib::ProfScope prof(mtlib::kProfPred, (hipStream_t)stream);
+    hipLaunchKernelGGL(cfl_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, blocks_d, n, ps,
+                       pred_d, ac_d, alpha_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }
