@@ -1,0 +1,111 @@
+"""Synthetic workloads of the candidate evaluation: a seeded test image and the full luma
+candidate set of its superblocks.
+
+The image follows the reference benchmark's recipe: four linear gradients of random
+orientation, three octaves of value noise (amplitude 40 at 8 bits), and 20 hard-edged
+rectangles, axis-aligned or oblique, of uniform random colour; scaled to the bit depth.
+
+The candidate set of one 64x64 superblock is every square block of the partition quadtree
+(4x4 ... 64x64) under each of the eleven non-directional-plus-directional luma modes the
+reference's intra search evaluates (ND_LUMA_MODES, src/intrapred.rs:1712) with DCT_DCT: the
+work of one exhaustive luma mode decision per block.  Predictions read a synthetic
+reconstruction plane (source plus small noise), so every candidate is independent.
+"""
+import numpy as np
+
+from . import _lib
+
+ND_LUMA_MODES = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 12]
+SQUARE_TX = {4: 0, 8: 1, 16: 2, 32: 3, 64: 4}
+EOB_BIN_LUMA = {16: 0, 64: 5, 256: 11, 1024: 15}
+
+
+def synth_image(seed, width, height, bd=10):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    img = np.zeros((height, width))
+    for _ in range(4):
+        th = rng.uniform(0, 2 * np.pi)
+        proj = x * np.cos(th) + y * np.sin(th)
+        proj = (proj - proj.min()) / max(proj.max() - proj.min(), 1e-9)
+        img += rng.uniform(20, 70) * proj
+    for octave, amp in ((64, 40.0), (32, 20.0), (16, 10.0)):
+        gh, gw = height // octave + 2, width // octave + 2
+        g = rng.uniform(-1, 1, (gh, gw))
+        fy, fx = y / octave, x / octave
+        iy, ix = fy.astype(int), fx.astype(int)
+        ty, tx = fy - iy, fx - ix
+        v = (g[iy, ix] * (1 - tx) * (1 - ty) + g[iy, ix + 1] * tx * (1 - ty) +
+             g[iy + 1, ix] * (1 - tx) * ty + g[iy + 1, ix + 1] * tx * ty)
+        img += amp * v
+    for _ in range(20):
+        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+        hw, hh = rng.uniform(8, width / 6), rng.uniform(8, height / 6)
+        th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
+        u = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
+        v = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
+        img[(np.abs(u) <= hw) & (np.abs(v) <= hh)] = rng.uniform(0, 255)
+    img = np.clip(np.rint(img), 0, 255) * (1 << (bd - 8))
+    return img.astype(np.uint16)
+
+
+def synth_recon(src, seed, bd=10, amp=6):
+    """A stand-in for the decision-time reconstruction: the source plus small noise."""
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    noise = rng.integers(-amp, amp + 1, src.shape) << max(bd - 8, 0)
+    return np.clip(src.astype(np.int64) + noise, 0, (1 << bd) - 1).astype(np.uint16)
+
+
+def sb_candidates(width, height, bd=10, qidx=91, qm_level=None, lam=64.0, wsse_scale=1.0):
+    """Every (square block, luma mode) candidate of every full 64x64 superblock of a
+    width x height luma plane, as CAND_DTYPE records; returns (records, pool elements)."""
+    bdi = {8: 0, 10: 1, 12: 2}[bd]
+    if qm_level is None:
+        qm_level = _lib.qm_level_law(qidx, 0)
+    sbw, sbh = width // 64, height // 64
+    per_sb = []
+    for bs in (4, 8, 16, 32, 64):
+        n = 64 // bs
+        by, bx = np.mgrid[0:n, 0:n]
+        for m in ND_LUMA_MODES:
+            per_sb.append((bx.ravel() * bs, by.ravel() * bs, bs, m))
+    cnt = sum(len(p[0]) for p in per_sb)
+    total = cnt * sbw * sbh
+    c = np.zeros(total, _lib.CAND_DTYPE)
+    sy, sx = np.mgrid[0:sbh, 0:sbw]
+    sx, sy = sx.ravel() * 64, sy.ravel() * 64
+    pos = 0
+    off = 0
+    for ox, oy, bs, m in per_sb:
+        k = len(ox)
+        for j in range(k):
+            sl = slice(pos, pos + sbw * sbh)
+            x = sx + ox[j]
+            y = sy + oy[j]
+            c["x"][sl] = x
+            c["y"][sl] = y
+            c["tx_size"][sl] = SQUARE_TX[bs]
+            c["mode"][sl] = m
+            # top-right / bottom-left availability in quadtree decoding order: the row above
+            # the superblock is decoded, and so is the right sibling's top edge for a left
+            # child, but not the next superblock's; bottom-left only along the decoded
+            # superblock to the left (the sample counts clip at the frame edge by themselves)
+            tr = (oy[j] == 0 or (ox[j] // bs) % 2 == 0) and not (ox[j] + bs == 64 and oy[j] > 0)
+            bl = ox[j] == 0 and oy[j] + bs < 64
+            c["pflags"][sl] = (1 if tr else 0) | (2 if bl else 0)
+            c["offset"][sl] = off + np.arange(sbw * sbh, dtype=np.uint64) * (bs * bs)
+            off += sbw * sbh * bs * bs
+            pos += sbw * sbh
+    c["planes"] = 1 << 2
+    c["qidx"] = qidx
+    c["bd_qm"] = (qm_level & 15) | (bdi << 4)
+    ncoef = np.minimum(c["tx_size"].astype(np.int64), 3)
+    c["eob_bin"] = np.array([EOB_BIN_LUMA[16], EOB_BIN_LUMA[64], EOB_BIN_LUMA[256],
+                             EOB_BIN_LUMA[1024]])[ncoef]
+    c["txtp_set"] = 0
+    # coefficient CDF context of the frame's qindex (get_q_ctx: 20 / 60 / 120)
+    c["qctx"] = 0 if qidx <= 20 else 1 if qidx <= 60 else 2 if qidx <= 120 else 3
+    c["wsse_scale"] = wsse_scale
+    c["lambda"] = lam
+    assert off < (1 << 32)
+    return c, off

@@ -168,6 +168,35 @@ uint64_t orc_luma_satd(const uint16_t* src, int stride, int w, int h, int vw, in
 uint64_t orc_satd_sad_proxy(const uint16_t* src, int stride, int w, int h, int vw, int vh,
                             const int32_t* pred);
 
+
+/* ---- one intra candidate end to end (src/coder/partition_search.rs:471-520) ---------- */
+/* Same layout as the library's candidate record (48 bytes). */
+typedef struct {
+    uint32_t offset;
+    uint16_t x, y;
+    uint8_t tx_size, tx_type, mode;
+    int8_t angle_delta;
+    uint8_t pflags, planes, qidx, bd_qm;
+    uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;
+    uint8_t txtp_set, txtp_mode, txtp, qctx;
+    uint8_t rflags, pad0, pad1, pad2;
+    float skip_one, skip_zero, wsse_scale, lambda, extra_bits;
+} orc_cand;
+typedef struct {
+    int64_t sse, dist;
+    uint64_t satd;
+    float bits, cost;
+    uint32_t nnz;
+    int32_t status; /* 0, -1 invalid, -2 unsupported (size, type) */
+} orc_candres;
+typedef struct {
+    const uint16_t* data;
+    int32_t stride, width, height, vis_w, vis_h, pad;
+} orc_plane;
+void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, int nplanes, orc_candres* r);
+void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,
+                    orc_candres* r, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

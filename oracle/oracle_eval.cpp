@@ -1,0 +1,123 @@
+// CPU oracle: one intra candidate evaluated end to end, and a threaded batch of them.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  Composes the oracle's restated kernels in the
+// order of the reference's candidate loop body (src/coder/partition_search.rs:471-520):
+// predict_nd_ad / dc_pred / filter_predict, residual_pred, DctDispatch forward + quant
+// (+QM), IdctDispatch inverse, sse_recon / weighted distortion / luma_satd over the
+// displayed part (src/rd_sse.rs:454, :244, :635), real_block_bits_bounded
+// (src/rate.rs:149) plus the caller's extra bits, rd_cost (src/partition_rd.rs:140).
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+
+namespace {
+
+int tx_dim_log2(int v) { return __builtin_ctz((unsigned)v); }
+
+void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candres& r) {
+    std::memset(&r, 0, sizeof(r));
+    r.sse = -1;
+    r.dist = -1;
+    r.cost = __builtin_inff();
+    const int s = c.tx_size, t = c.tx_type;
+    if (s > 18) {
+        r.status = -1;
+        return;
+    }
+    const int w = orc_tx_w(s), h = orc_tx_h(s);
+    const int si = c.planes & 3, ri = (c.planes >> 2) & 3, bdi = (c.bd_qm >> 4) & 3;
+    if (si >= nplanes || ri >= nplanes || bdi > 2) {
+        r.status = -1;
+        return;
+    }
+    const int bd = bdi == 0 ? 8 : bdi == 1 ? 10 : 12, qm = c.bd_qm & 15;
+    const orc_plane& sp = planes[si];
+    const orc_plane& rp = planes[ri];
+    if (c.x + w > sp.width || c.y + h > sp.height || c.x + w > rp.width || c.y + h > rp.height) {
+        r.status = -1;
+        return;
+    }
+    if (!orc_tx_supported(s, t)) {
+        r.status = -2;
+        return;
+    }
+    std::vector<int32_t> pred(w * h), resid(w * h), inv(w * h), levels(orc_tx_ncoef(s));
+    std::vector<float> targets(orc_tx_ncoef(s));
+    if (orc_intra_pred(c.mode, c.angle_delta, rp.data, rp.stride, c.x, c.y, w, h, c.pflags & 1,
+                       (c.pflags >> 1) & 1, rp.width, rp.height, (c.pflags >> 2) & 1, bd,
+                       pred.data()) != 0) {
+        r.status = -1;
+        return;
+    }
+    const uint16_t* src = sp.data + (size_t)c.y * sp.stride + c.x;
+    orc_residual_pred(src, sp.stride, pred.data(), w, h, resid.data());
+    orc_quant q;
+    orc_quant_new(c.qidx, bd, qm, &q);
+    orc_fwd_txfm(s, t, resid.data(), &q, levels.data(), targets.data());
+    orc_inv_txfm(s, t, levels.data(), &q, inv.data());
+    const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
+    const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);
+    r.sse = orc_sse_recon_vis(pred.data(), inv.data(), src, sp.stride, w, vw, vh, bd);
+    if (c.wsse_scale > 0.0f) {
+        const int maxv = (1 << bd) - 1;
+        std::vector<int32_t> err((size_t)std::max(vw * vh, 1));
+        for (int y = 0; y < vh; ++y)
+            for (int x = 0; x < vw; ++x) {
+                const int i = y * w + x;
+                const int rec = std::min(std::max(pred[i] + inv[i], 0), maxv);
+                err[y * vw + x] = (int32_t)src[(size_t)y * sp.stride + x] - rec;
+            }
+        r.dist = orc_weighted_dist(err.data(), vw, vh, c.wsse_scale, 1);
+    } else {
+        r.dist = r.sse;
+    }
+    r.satd = orc_luma_satd(src, sp.stride, w, h, vw, vh, bd, pred.data(), inv.data());
+    orc_rate_ctx rc;
+    std::memset(&rc, 0, sizeof(rc));
+    rc.qctx = c.qctx;
+    rc.cls = (tx_dim_log2(w) - 2 + tx_dim_log2(h) - 2 + 1) >> 1;
+    rc.plane = c.rplane;
+    rc.w = std::min(w, 32);
+    rc.h = std::min(h, 32);
+    rc.eob_bin = c.eob_bin;
+    rc.skip_ctx = c.skip_ctx;
+    rc.dcs_ctx = c.dcs_ctx;
+    rc.txtp_set = c.txtp_set;
+    rc.txtp_mode = c.txtp_mode;
+    rc.txtp = c.txtp;
+    rc.use_skip_bits = c.rflags & 1;
+    rc.skip_one = c.skip_one;
+    rc.skip_zero = c.skip_zero;
+    const float bits = orc_block_rate(levels.data(), orc_scan(s), &rc, __builtin_inff());
+    r.bits = bits + c.extra_bits;
+    r.cost = (float)r.dist + c.lambda * r.bits;
+    uint32_t nnz = 0;
+    for (int32_t v : levels) nnz += v != 0;
+    r.nnz = nnz;
+    r.status = 0;
+}
+
+}  // namespace
+
+extern "C" void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, int nplanes,
+                                   orc_candres* r) {
+    eval_one(*c, planes, nplanes, *r);
+}
+
+extern "C" void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,
+                               orc_candres* r, int nthreads) {
+    if (nthreads <= 1) {
+        for (int i = 0; i < n; ++i) eval_one(c[i], planes, nplanes, r[i]);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nthreads; ++k)
+        th.emplace_back([=] {
+            for (int i = k; i < n; i += nthreads) eval_one(c[i], planes, nplanes, r[i]);
+        });
+    for (auto& x : th) x.join();
+}

@@ -276,3 +276,34 @@ def wsse_tables():
     w = np.zeros(16, np.float32)
     L.orc_wsse_tables(ptr(d, ctypes.c_float), ptr(w, ctypes.c_float))
     return d.reshape(4, 4), w.reshape(4, 4)
+
+
+# ---- whole candidates ----------------------------------------------------------------
+
+class OrcPlane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("vis_w", ctypes.c_int32), ("vis_h", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
+                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
+
+
+def eval_batch(cands, planes, nthreads=8):
+    """orc_eval_batch over a structured array of 48-byte candidate records.  planes: list of
+    (uint16 2-D array, width, height, vis_w, vis_h).  Returns CANDRES_DTYPE records."""
+    L = plib()
+    if not getattr(L, "_eval_sigs", False):
+        L.orc_eval_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcPlane),
+                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        L._eval_sigs = True
+    keep = [np.ascontiguousarray(p[0], dtype=np.uint16) for p in planes]
+    arr = (OrcPlane * len(planes))()
+    for i, (p, a) in enumerate(zip(planes, keep)):
+        arr[i] = OrcPlane(a.ctypes.data, a.shape[1], p[1], p[2], p[3], p[4], 0)
+    c = np.ascontiguousarray(cands)
+    assert c.dtype.itemsize == 48
+    out = np.zeros(len(c), CANDRES_DTYPE)
+    L.orc_eval_batch(c.ctypes.data, len(c), arr, len(planes), out.ctypes.data, nthreads)
+    return out
