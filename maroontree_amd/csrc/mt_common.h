@@ -22,6 +22,13 @@ struct PlaneSet {
 // Copies a host plane array into a PlaneSet, validating it.
 int planes_from_host(const mt_plane* planes, int nplanes, const char* who, PlaneSet* ps);
 int dist_init_device(int device);
+// mt_intra_pred with an optional residual output (src - pred, int16, at each out_offset; the
+// source plane index in each descriptor's pad byte).
+int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int32_t* pred_d,
+                      int32_t* status_d, int16_t* resid_d, hipStream_t s);
+// mt_block_rate whose eob search uses the chain's nonzero counts.
+int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                         const mt_txres* nnz_d, hipStream_t s);
 
 // Kernel timing slots of mt_profile_end.
 enum {

@@ -11,10 +11,13 @@
 // reference rounds it (__fmul_rn / __fmaf_rn / __fadd_rn).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/mtgpu.h"
 #include "mt_common.h"
+#include "mt_lists.h"
 
 namespace {
 
@@ -74,7 +77,9 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 
 constexpr int kES = 65;  // LDS error row stride (64 + 1)
 
-__global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__ blocks, int n,
+__global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__ blocks,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ count,
                                                   PlaneSet planes, const int32_t* __restrict__ pred,
                                                   const int32_t* __restrict__ resid,
                                                   mt_distres* __restrict__ out) {
@@ -82,18 +87,12 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
     __shared__ float s_acc[32];
     __shared__ float s_sum[4];
     const int lane = threadIdx.x;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+    const int n = (int)*count;
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const uint32_t b = list[li];
         const mt_distblk d = blocks[b];
         const int bw = d.bw, bh = d.bh, bd = d.bd;
-        bool ok = (bw == 4 || bw == 8 || bw == 16 || bw == 32 || bw == 64) &&
-                  (bh == 4 || bh == 8 || bh == 16 || bh == 32 || bh == 64) &&
-                  (bd == 8 || bd == 10 || bd == 12) && d.plane < planes.n;
-        const mt_plane pl = planes.p[ok ? d.plane : 0];
-        ok = ok && (int)d.x + bw <= pl.width && (int)d.y + bh <= pl.height;
-        if (!ok) {
-            if (lane == 0) out[b] = mt_distres{-1, -1, 0, 0};
-            continue;
-        }
+        const mt_plane pl = planes.p[d.plane];  // validated by dist_classify_kernel
         const int vw = min(max(pl.vis_w - (int)d.x, 0), bw);
         const int vh = min(max(pl.vis_h - (int)d.y, 0), bh);
         const int maxv = (1 << bd) - 1;
@@ -176,6 +175,175 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
     }
 }
 
+// ---- blocks of at most 64 samples: one thread per block, everything in registers ----------
+
+// t[k][j] of the 4x4 tile at (r0, c0): column j through row frequency k, in the vector
+// program's order (e0j*D[k][0], then three fmas down the column).
+__device__ __forceinline__ void wsse_rows_reg(const int* e, int stride, int r0, int c0, float (&t)[4][4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int* col = e + r0 * stride + c0 + j;
+            float v = __fmul_rn((float)col[0], c_wd[k][0]);
+            v = __fmaf_rn((float)col[stride], c_wd[k][1], v);
+            v = __fmaf_rn((float)col[2 * stride], c_wd[k][2], v);
+            v = __fmaf_rn((float)col[3 * stride], c_wd[k][3], v);
+            t[k][j] = v;
+        }
+}
+
+// Coefficient (k, l) from t[k]: D[l][0]*t0, then three fmas across.
+__device__ __forceinline__ float wsse_col(const float (&t)[4][4], int k, int l) {
+    float c = __fmul_rn(c_wd[l][0], t[k][0]);
+    c = __fmaf_rn(c_wd[l][1], t[k][1], c);
+    c = __fmaf_rn(c_wd[l][2], t[k][2], c);
+    c = __fmaf_rn(c_wd[l][3], t[k][3], c);
+    return c;
+}
+
+// The same four measures as dist_kernel for one W x H block, by one thread.  Loops run over
+// the block's static tile grid (so the error stays in registers); the displayed extent only
+// predicates them.  The weighted SSE keeps weighted_sse_avx2's accumulators and order.
+template <int W, int H>
+__device__ mt_distres small_measures(const uint16_t* src, int sstride, const int32_t* pr,
+                                     const int32_t* rs, bool has_res, int vw, int vh, int maxv,
+                                     float scale) {
+    int e[W * H];
+    int64_t sse = 0;
+    uint64_t satd = 0, psatd = 0, sad = 0;
+#pragma unroll
+    for (int ty = 0; ty < H; ty += 4)
+#pragma unroll
+        for (int tx = 0; tx < W; tx += 4) {
+            int ep[16];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int y = ty + r, x = tx + c, i = y * W + x;
+                    int ev = 0, epv = 0;
+                    if (x < vw && y < vh) {
+                        const int sv = src[(size_t)y * sstride + x];
+                        const int p = pr[i];
+                        ev = sv - clampi(p + (has_res ? rs[i] : 0), 0, maxv);
+                        epv = sv - p;
+                    }
+                    e[i] = ev;
+                    ep[r * 4 + c] = epv;
+                    sse += (int64_t)ev * ev;
+                    sad += (uint64_t)abs(epv);
+                }
+            satd += tile_satd(e + ty * W + tx, W);
+            psatd += tile_satd(ep, 4);
+        }
+    int64_t dist = sse;
+    if (scale > 0.0f) {
+        const int pw = (vw + 3) & ~3, ph = (vh + 3) & ~3, pairs = pw / 8 * 8;
+        float total = 0.0f;
+        if (pw > 0 && ph > 0) {
+            float acc[4][2][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) acc[k][hf][l] = 0.0f;
+#pragma unroll
+            for (int r0 = 0; r0 < H; r0 += 4)
+#pragma unroll
+                for (int c0 = 0; c0 + 8 <= W; c0 += 8)
+                    if (r0 < ph && c0 < pairs)
+#pragma unroll
+                        for (int hf = 0; hf < 2; ++hf) {
+                            float t[4][4];
+                            wsse_rows_reg(e, W, r0, c0 + 4 * hf, t);
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                                for (int l = 0; l < 4; ++l) {
+                                    const float c = wsse_col(t, k, l);
+                                    acc[k][hf][l] = __fmaf_rn(__fmul_rn(c_ww[k][l], c), c, acc[k][hf][l]);
+                                }
+                        }
+            float sum[4];
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const float s01 = __fadd_rn(__fadd_rn(acc[0][0][l], acc[0][1][l]), __fadd_rn(acc[1][0][l], acc[1][1][l]));
+                const float s23 = __fadd_rn(__fadd_rn(acc[2][0][l], acc[2][1][l]), __fadd_rn(acc[3][0][l], acc[3][1][l]));
+                sum[l] = __fadd_rn(s01, s23);
+            }
+            if (pairs < pw) {
+#pragma unroll
+                for (int r0 = 0; r0 < H; r0 += 4)
+#pragma unroll
+                    for (int tc = 0; tc < W; tc += 4)
+                        if (r0 < ph && tc == pairs) {
+                            float t[4][4];
+                            wsse_rows_reg(e, W, r0, tc, t);
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                                for (int l = 0; l < 4; ++l) {
+                                    const float c = wsse_col(t, k, l);
+                                    sum[l] = __fmaf_rn(__fmul_rn(c_ww[k][l], c), c, sum[l]);
+                                }
+                        }
+            }
+            total = __fadd_rn(__fadd_rn(sum[0], sum[2]), __fadd_rn(sum[1], sum[3]));
+        }
+        dist = (int64_t)roundf(__fmul_rn(total, scale));
+    }
+    return mt_distres{sse, dist, satd, sad + (psatd >> 2)};
+}
+
+using mtlib::kSmallShapes;
+
+template <int W, int H>
+__global__ void __launch_bounds__(256) dist_small_kernel(const mt_distblk* __restrict__ blocks,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ count,
+                                                         PlaneSet planes, const int32_t* __restrict__ pred,
+                                                         const int32_t* __restrict__ resid,
+                                                         mt_distres* __restrict__ out) {
+    const int n = (int)*count;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x) {
+        const uint32_t b = list[li];
+        const mt_distblk d = blocks[b];
+        const mt_plane pl = planes.p[d.plane];
+        const int vw = min(max(pl.vis_w - (int)d.x, 0), W);
+        const int vh = min(max(pl.vis_h - (int)d.y, 0), H);
+        out[b] = small_measures<W, H>(pl.data + (size_t)d.y * pl.stride + d.x, pl.stride,
+                                      pred + d.pred_offset, resid + d.resid_offset,
+                                      !(d.flags & MT_DIST_NO_RESID), vw, vh, (1 << d.bd) - 1,
+                                      d.wsse_scale);
+    }
+}
+
+// Validates every descriptor (invalid ones are answered here) and sorts the valid ones into
+// one list per shape class (list c at lists + c * cap).  cnt[0..6] must be zero on entry.
+__global__ void __launch_bounds__(256) dist_classify_kernel(const mt_distblk* __restrict__ blocks, int n,
+                                                            PlaneSet planes, uint32_t* __restrict__ lists,
+                                                            size_t cap, uint32_t* __restrict__ cnt,
+                                                            mt_distres* __restrict__ out) {
+    auto cls = [&](int b, bool first) -> int {
+        const mt_distblk d = blocks[b];
+        const int bw = d.bw, bh = d.bh, bd = d.bd;
+        bool ok = (bw == 4 || bw == 8 || bw == 16 || bw == 32 || bw == 64) &&
+                  (bh == 4 || bh == 8 || bh == 16 || bh == 32 || bh == 64) &&
+                  (bd == 8 || bd == 10 || bd == 12) && d.plane < planes.n;
+        const mt_plane pl = planes.p[ok ? d.plane : 0];
+        ok = ok && (int)d.x + bw <= pl.width && (int)d.y + bh <= pl.height;
+        if (!ok) {
+            if (first) out[b] = mt_distres{-1, -1, 0, 0};
+            return -1;
+        }
+        return mtlib::small_shape_class(bw, bh);
+    };
+    mtlib::classify_chunk<kSmallShapes + 1>(n, cls, lists, cap, cnt);
+}
+
+
 }  // namespace
 
 namespace mtlib {
@@ -194,6 +362,17 @@ int dist_init_device(int /*device*/) {
 }
 }  // namespace mtlib
 
+namespace {
+// Library-owned lists of one device, grown on demand: calls on one device share them.
+struct DistWork {
+    size_t cap = 0;
+    uint32_t* lists = nullptr;  // kSmallShapes + 1 lists of cap entries
+    uint32_t* cnt = nullptr;
+};
+std::mutex g_dist_mu;
+DistWork g_dist_work[64];
+}  // namespace
+
 extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
                              const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d,
                              void* stream) {
@@ -206,10 +385,34 @@ extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* 
     int dev = 0, cus = 0;
     rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
-    const int grid = n < cus * 32 ? n : cus * 32;
-    mtlib::ProfScope prof(mtlib::kProfDist, (hipStream_t)stream);
-    hipLaunchKernelGGL(dist_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream, blocks_d, n, ps,
-                       pred_d, resid_d ? resid_d : pred_d, out_d);
+    std::lock_guard<std::mutex> lk(g_dist_mu);
+    DistWork& w = g_dist_work[dev];
+    if (w.cap < (size_t)n) {
+        if (w.lists) MT_HIP_TRY(hipFree(w.lists));
+        w.lists = nullptr;
+        const size_t cap = n < 65536 ? 65536 : (size_t)n;
+        MT_HIP_TRY(hipMalloc(&w.lists, (kSmallShapes + 1) * cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, (kSmallShapes + 1) * sizeof(uint32_t)));
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t* resid = resid_d ? resid_d : pred_d;
+    mtlib::ProfScope prof(mtlib::kProfDist, s);
+    hipLaunchKernelGGL(mtlib::zero_counts_kernel<kSmallShapes + 1>, dim3(1), dim3(64), 0, s, w.cnt);
+    hipLaunchKernelGGL(dist_classify_kernel, dim3(std::max(1, std::min((n + 255) / 256, cus * 4))),
+                       dim3(256), 0, s, blocks_d, n, ps, w.lists, w.cap, w.cnt, out_d);
+    const int gs = std::max(1, std::min((n + 255) / 256, cus * 8));
+    const uint32_t* L = w.lists;
+    const size_t cap = w.cap;
+    hipLaunchKernelGGL((dist_small_kernel<4, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L, w.cnt, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<4, 8>), dim3(gs), dim3(256), 0, s, blocks_d, L + cap, w.cnt + 1, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<8, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L + 2 * cap, w.cnt + 2, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<8, 8>), dim3(gs), dim3(256), 0, s, blocks_d, L + 3 * cap, w.cnt + 3, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<4, 16>), dim3(gs), dim3(256), 0, s, blocks_d, L + 4 * cap, w.cnt + 4, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<16, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L + 5 * cap, w.cnt + 5, ps, pred_d, resid, out_d);
+    const int gw = n < cus * 32 ? n : cus * 32;
+    hipLaunchKernelGGL(dist_kernel, dim3(gw), dim3(64), 0, s, blocks_d, L + 6 * cap, w.cnt + 6, ps,
+                       pred_d, resid, out_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -1,6 +1,6 @@
 // Batched intra candidate evaluation: one call runs the reference's candidate loop body
 // (src/coder/partition_search.rs:471-520 and its siblings) for every candidate of a batch:
-//   predict (predict_nd_ad / dc_pred / filter_predict)  -> residual_pred
+//   predict (predict_nd_ad / dc_pred / filter_predict) + residual_pred
 //   -> forward transform + quant (+QM) -> dequant + inverse          (the transform chain)
 //   -> sse_recon / weighted distortion / luma_satd                  (distortion)
 //   -> real_block_bits_bounded + the caller's mode bits             (rate)
@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     p.flags = c.pflags;
     p.bd = (uint8_t)bd;
     p.plane = (uint8_t)ri;
+    p.pad = (uint8_t)si;  // source plane of the residual the predictor writes alongside
     pb[i] = p;
     mt_txblk t;
     t.offset = c.offset;
@@ -101,28 +102,6 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     r.skip_zero = c.skip_zero;
     r.bound = __int_as_float(0x7f800000);  // +inf: price fully
     rb[i] = r;
-}
-
-// residual_pred (src/rd_sse.rs:1179): src - pred into the chain's int16 pool, one wave per
-// candidate (|src - pred| < 2^12).
-__global__ void __launch_bounds__(64) eval_resid_kernel(const mt_cand* __restrict__ cands, int n,
-                                                        PlaneSet planes,
-                                                        const int32_t* __restrict__ pred,
-                                                        const int32_t* __restrict__ pstatus,
-                                                        const int32_t* __restrict__ status,
-                                                        int16_t* __restrict__ resid) {
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        if (status[i] != MT_OK || pstatus[i] != MT_OK) continue;
-        const mt_cand c = cands[i];
-        const int w = tx_w(c.tx_size), h = tx_h(c.tx_size);
-        const mt_plane& s = planes.p[c.planes & 3];
-        const uint16_t* src = s.data + (size_t)c.y * s.stride + c.x;
-        const int lw = __builtin_ctz(w);
-        for (int p = threadIdx.x; p < w * h; p += 64) {
-            const int y = p >> lw, x = p & (w - 1);
-            resid[c.offset + p] = (int16_t)((int)src[(size_t)y * s.stride + x] - pred[c.offset + p]);
-        }
-    }
 }
 
 __global__ void __launch_bounds__(256) eval_combine_kernel(const mt_cand* __restrict__ cands, int n,
@@ -230,14 +209,11 @@ extern "C" int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane*
     hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
                        w.pb, w.tb, w.db, w.rb, w.status);
     MT_HIP_TRY(hipGetLastError());
-    if ((rc = mt_intra_pred(w.pb, n, planes, nplanes, w.pred, w.pstatus, stream))) return rc;
-    const int gw = n < cus * 32 ? n : cus * 32;
-    hipLaunchKernelGGL(eval_resid_kernel, dim3(gw), dim3(64), 0, s, cands_d, n, ps, w.pred, w.pstatus,
-                       w.status, w.resid);
-    MT_HIP_TRY(hipGetLastError());
+    // prediction and residual_pred in one pass
+    if ((rc = mtlib::intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
     if ((rc = mt_tx_chain(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, stream))) return rc;
     if ((rc = mt_distortion(w.db, n, planes, nplanes, w.pred, w.recon, w.dr, stream))) return rc;
-    if ((rc = mt_block_rate(w.rb, n, w.levels, w.bits, stream))) return rc;
+    if ((rc = mtlib::rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
     hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,
                        w.pstatus, w.txr, w.dr, w.bits, res_d);
     MT_HIP_TRY(hipGetLastError());

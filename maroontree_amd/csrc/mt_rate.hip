@@ -76,9 +76,12 @@ __device__ inline float golomb_bits(uint32_t v) {
 }
 
 // The 8-bit wrap of the reference's u8 level plane is irrelevant: entries stay below 256.
+// nnz (optional): the transform chain's per-block results; their nonzero-level counts let
+// the eob search stop after the last nonzero level instead of walking every scan position.
 __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict__ blocks, int n,
                                                   const int32_t* __restrict__ levels,
-                                                  float* __restrict__ out) {
+                                                  float* __restrict__ out,
+                                                  const mt_txres* __restrict__ nnz) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
     const mt_rateblk d = blocks[b];
@@ -115,7 +118,29 @@ __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict_
     const uint16_t* iscan = c_iscan + so;
 
     int eob = N - 1;
-    while (eob >= 0 && cf[scan[eob]] == 0) eob--;
+    if (nnz) {
+        // eob = the largest scan index of a nonzero level: walk the x-major levels four at a
+        // time until all of the block's nonzero levels have been seen
+        const uint32_t want = nnz[b].nnz;
+        uint32_t seen = 0;
+        eob = -1;
+        if ((d.offset & 3) == 0) {
+            const int4* cf4 = reinterpret_cast<const int4*>(cf);
+            for (int k4 = 0; k4 < (N >> 2) && seen < want; k4++) {
+                const int4 v = cf4[k4];
+                const int k = k4 << 2;
+                if (v.x) { seen++; eob = max(eob, (int)iscan[k]); }
+                if (v.y) { seen++; eob = max(eob, (int)iscan[k + 1]); }
+                if (v.z) { seen++; eob = max(eob, (int)iscan[k + 2]); }
+                if (v.w) { seen++; eob = max(eob, (int)iscan[k + 3]); }
+            }
+        } else {
+            for (int k = 0; k < N && seen < want; k++)
+                if (cf[k]) { seen++; eob = max(eob, (int)iscan[k]); }
+        }
+    } else {
+        while (eob >= 0 && cf[scan[eob]] == 0) eob--;
+    }
     const bool adaptive_skip = d.flags & 1;
     if (eob < 0) {
         out[b] = adaptive_skip ? d.skip_one : T.skip[cls][d.skip_ctx][1];
@@ -371,7 +396,19 @@ extern "C" int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* l
     if (n == 0) return MT_OK;
     mtlib::ProfScope prof(mtlib::kProfRate, (hipStream_t)stream);
     hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, blocks_d, n,
-                       levels_d, bits_d);
+                       levels_d, bits_d, (const mt_txres*)nullptr);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }
+
+namespace mtlib {
+int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                         const mt_txres* nnz_d, hipStream_t s) {  // levels of any alignment
+    if (n == 0) return MT_OK;
+    ProfScope prof(kProfRate, s);
+    hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, s, blocks_d, n, levels_d,
+                       bits_d, nnz_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+}  // namespace mtlib
