@@ -2,8 +2,9 @@
 // coefficient coder (reference src/rate.rs:149 real_block_bits_bounded, src/cost.rs:159-262,
 // src/coeffs.rs:46 get_lo_ctx_2d) against the frame-initial CDFs Cdfs::new(qctx).
 //
-// The walk is sequential per block (every f32 addition in the reference's order), so one
-// lane prices one block and a wave prices 64 blocks at once.  The neighbour "levels" plane
+// The walk is sequential per block (every f32 addition in the reference's order).  Blocks of
+// up to 128 coefficients take one lane each; larger ones one wave each, whose lanes price the
+// scan positions in parallel before one lane adds the costs in order.  The neighbour "levels" plane
 // the reference fills while walking the reverse scan needs no scratch here: at step i a
 // position holds its level byte exactly when its scan index is above i, so every context
 // is read straight from the block's levels through the inverse scan.
@@ -18,6 +19,7 @@
 #include "../../include/mtgpu.h"
 #include "av1_cdf_tables.h"
 #include "mt_common.h"
+#include "mt_lists.h"
 
 namespace {
 
@@ -75,23 +77,18 @@ __device__ inline float golomb_bits(uint32_t v) {
     return (float)(2 * (32 - __clz((int)(v + 1))) - 1);
 }
 
-// The 8-bit wrap of the reference's u8 level plane is irrelevant: entries stay below 256.
-// nnz (optional): the transform chain's per-block results; their nonzero-level counts let
-// the eob search stop after the last nonzero level instead of walking every scan position.
-__global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict__ blocks, int n,
-                                                  const int32_t* __restrict__ levels,
-                                                  float* __restrict__ out,
-                                                  const mt_txres* __restrict__ nnz) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const mt_rateblk d = blocks[b];
+// A decoded, validated rate descriptor.
+struct RateBlk {
+    int so, cls, cb, w, h, N, log2h, oi;
+};
+
+// Validation and the size-derived quantities of one descriptor (false: invalid, priced NaN).
+__device__ inline bool rate_decode(const mt_rateblk& d, RateBlk& r) {
     const int s = d.tx_size;
-    const int so = s < 19 ? scan_offset(s) : -1;
-    if (so < 0 || d.plane > 1 || d.eob_bin > 16 || d.skip_ctx > 12 || d.dcs_ctx > 2 || d.qctx > 3 ||
-        d.txtp_set > 3 || d.txtp_mode > 12 || d.txtp > 6 || (s == MT_TX_64X64 && d.plane)) {
-        out[b] = __int_as_float(0x7fc00000);  // invalid descriptor: NaN
-        return;
-    }
+    r.so = s < 19 ? scan_offset(s) : -1;
+    if (r.so < 0 || d.plane > 1 || d.eob_bin > 16 || d.skip_ctx > 12 || d.dcs_ctx > 2 || d.qctx > 3 ||
+        d.txtp_set > 3 || d.txtp_mode > 12 || d.txtp > 6 || (s == MT_TX_64X64 && d.plane))
+        return false;
     const int wf = s == MT_TX_4X4 || s == MT_TX_4X8 || s == MT_TX_4X16 ? 4
                  : s == MT_TX_8X8 || s == MT_TX_8X4 || s == MT_TX_8X16 ? 8
                  : s == MT_TX_16X16 || s == MT_TX_16X8 || s == MT_TX_16X4 || s == MT_TX_16X32 ? 16
@@ -101,16 +98,41 @@ __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict_
                  : s == MT_TX_16X16 || s == MT_TX_8X16 || s == MT_TX_4X16 || s == MT_TX_32X16 ? 16
                  : s == MT_TX_64X64 ? 64 : 32;
     // transform-size context of the coefficient CDFs: (log2(w/4) + log2(h/4) + 1) >> 1
-    const int cls = ((31 - __clz(wf)) - 2 + (31 - __clz(hf)) - 2 + 1) >> 1;
-    const int w = wf > 32 ? 32 : wf, h = hf > 32 ? 32 : hf, N = w * h;
+    r.cls = ((31 - __clz(wf)) - 2 + (31 - __clz(hf)) - 2 + 1) >> 1;
+    r.cb = r.cls < 3 ? r.cls : 3;
+    r.w = wf > 32 ? 32 : wf;
+    r.h = hf > 32 ? 32 : hf;
+    r.N = r.w * r.h;
+    r.log2h = 31 - __clz(r.h);
+    r.oi = r.w < r.h ? 1 : (r.w > r.h ? 2 : 0);
     // the eob_bin CDF must be the one of this coefficient count (the reference indexes its
     // symbols by the eob's bin; a smaller table would run past its end)
     const int ebn = d.eob_bin < 3 ? 16 : d.eob_bin < 5 ? 32 : d.eob_bin < 8 ? 64 : d.eob_bin < 11 ? 128
                   : d.eob_bin < 13 ? 256 : d.eob_bin < 15 ? 512 : 1024;
-    if (ebn != N || (d.txtp_set == 3 && d.txtp > 4)) {
-        out[b] = __int_as_float(0x7fc00000);
+    return ebn == r.N && !(d.txtp_set == 3 && d.txtp > 4);
+}
+
+// Blocks of at least this many coded coefficients take the wave path when split.
+constexpr int kRateWaveN = 256;
+
+// The 8-bit wrap of the reference's u8 level plane is irrelevant: entries stay below 256.
+// nnz (optional): the transform chain's per-block results; their nonzero-level counts let
+// the eob search stop after the last nonzero level instead of walking every scan position.
+// skip_large: leave valid blocks of kRateWaveN or more coefficients to rate_wave_kernel.
+__global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict__ blocks, int n,
+                                                  const int32_t* __restrict__ levels,
+                                                  float* __restrict__ out,
+                                                  const mt_txres* __restrict__ nnz, int skip_large) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const mt_rateblk d = blocks[b];
+    RateBlk R;
+    if (!rate_decode(d, R)) {
+        out[b] = __int_as_float(0x7fc00000);  // invalid descriptor: NaN
         return;
     }
+    if (skip_large && R.N >= kRateWaveN) return;
+    const int so = R.so, cls = R.cls, w = R.w, h = R.h, N = R.N;
     const int pl = d.plane;
     const RateTables& T = g_rate[d.qctx];
     const int32_t* cf = levels + d.offset;
@@ -146,7 +168,7 @@ __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict_
         out[b] = adaptive_skip ? d.skip_one : T.skip[cls][d.skip_ctx][1];
         return;
     }
-    const int cb = cls < 3 ? cls : 3;
+    const int cb = R.cb;
     float bits = adaptive_skip ? d.skip_zero : T.skip[cls][d.skip_ctx][0];
     if (d.txtp_set) bits += T.txtp[d.txtp_set - 1][d.txtp_mode][d.txtp];
     if (eob == 0) {
@@ -174,8 +196,9 @@ __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict_
         out[b] = INFINITY;
         return;
     }
-    const int log2h = 31 - __clz(h);
-    const int oi = w < h ? 1 : (w > h ? 2 : 0);
+    const int log2h = R.log2h;
+    const int oi = R.oi;
+    (void)w;
     // level byte of x-major index k as the reference's plane holds it at step i
     auto lvl_at = [&](int k, int i) -> uint32_t {
         return (k < N && (int)iscan[k] > i) ? level_byte(uabs32(cf[k])) : 0u;
@@ -230,6 +253,170 @@ __global__ void __launch_bounds__(256) rate_kernel(const mt_rateblk* __restrict_
     if (cf[0] != 0) bits += T.dc_sign[pl][d.dcs_ctx][cf[0] < 0 ? 1 : 0];
     for (uint32_t k = 0; k < ac_nonzero; k++) bits += 1.0f;
     out[b] = bits;
+}
+
+// Blocks of kRateWaveN or more coefficients, one wave per block.  The lanes stage the level
+// bytes and the inverse scan in LDS, find the eob, and price every scan position's symbols
+// in parallel (the contexts only read levels coded before the position, which the staged
+// plane already holds); lane 0 then adds the per-position costs in the reference's order,
+// with the same bound checks, so the f32 sum is the sequential walk's.
+__global__ void __launch_bounds__(64) rate_wave_kernel(const mt_rateblk* __restrict__ blocks,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ count,
+                                                       const int32_t* __restrict__ levels,
+                                                       float* __restrict__ out) {
+    __shared__ uint8_t s_lb[1024];    // level byte of |cf[k]|, x-major
+    __shared__ uint16_t s_is[1024];   // inverse scan
+    // per scan position: base token and base range costs, and the Golomb bits; a symbol the
+    // position does not code costs +0.0, whose addition leaves every finite sum unchanged
+    __shared__ float2 s_t01[1024];
+    __shared__ float s_t2[1024];
+    const int lane = threadIdx.x;
+    const int nlist = (int)*count;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const uint32_t b = list[li];
+        const mt_rateblk d = blocks[b];
+        RateBlk R;
+        rate_decode(d, R);  // valid: only valid blocks are listed
+        const int cls = R.cls, cb = R.cb, h = R.h, N = R.N, log2h = R.log2h, oi = R.oi;
+        const int pl = d.plane;
+        const RateTables& T = g_rate[d.qctx];
+        const int32_t* cf = levels + d.offset;
+        const uint16_t* scan = c_scan + R.so;
+        int emax = -1, nzc = 0;
+        for (int k = lane; k < N; k += 64) {
+            const int32_t v = cf[k];
+            const int is = c_iscan[R.so + k];
+            s_lb[k] = (uint8_t)level_byte(uabs32(v));
+            s_is[k] = (uint16_t)is;
+            if (v) emax = max(emax, is);
+        }
+        for (int off = 32; off > 0; off >>= 1) emax = max(emax, __shfl_xor(emax, off));
+        const int eob = emax;
+        __syncthreads();
+        const bool adaptive_skip = d.flags & 1;
+        if (eob < 1) {
+            // all zero, or DC only: the thread kernel's short paths
+            if (lane == 0) {
+                float bits;
+                if (eob < 0) {
+                    bits = adaptive_skip ? d.skip_one : T.skip[cls][d.skip_ctx][1];
+                } else {
+                    bits = adaptive_skip ? d.skip_zero : T.skip[cls][d.skip_ctx][0];
+                    if (d.txtp_set) bits += T.txtp[d.txtp_set - 1][d.txtp_mode][d.txtp];
+                    const uint32_t m = uabs32(cf[0]);
+                    const uint32_t t = m < 3 ? m : 3;
+                    bits += T.eob_bin[d.eob_bin][0];
+                    bits += T.eob_base[cls][pl][0][t - 1];
+                    if (t == 3) {
+                        float hi = T.br[cb][pl][0][(m - 3) < 12 ? (m - 3) : 12];
+                        if (m >= 15) hi += golomb_bits(m - 15);
+                        bits += hi;
+                    }
+                    bits += T.dc_sign[pl][d.dcs_ctx][cf[0] < 0 ? 1 : 0];
+                }
+                out[b] = bits;
+            }
+            __syncthreads();
+            continue;
+        }
+        auto lvl_at = [&](int k, int i) -> uint32_t {
+            return (k < N && (int)s_is[k] > i) ? (uint32_t)s_lb[k] : 0u;
+        };
+        // every position 1 .. eob-1: its symbols' costs
+        for (int i = 1 + lane; i < eob; i += 64) {
+            const int rc = scan[i];
+            const int x = rc >> log2h, y = rc & (h - 1);
+            const int base = x * h + y;
+            const uint32_t hm = lvl_at(base + 1, i) + lvl_at(base + h, i) + lvl_at(base + h + 1, i);
+            const uint32_t mag = hm + lvl_at(base + 2, i) + lvl_at(base + 2 * h, i);
+            const int ctx = c_lo_off[oi][y < 4 ? y : 4][x < 4 ? x : 4] + (mag > 512 ? 4 : (int)((mag + 64) >> 7));
+            const uint32_t m = uabs32(cf[rc]);
+            const uint32_t tok = m < 3 ? m : 3;
+            float t1 = 0.0f, t2 = 0.0f;
+            if (tok == 3) {
+                const uint32_t mg = hm & 63;
+                const int bc = ((y | x) > 1 ? 14 : 7) + (mg > 12 ? 6 : (int)((mg + 1) >> 1));
+                t1 = T.br[cb][pl][bc][(m - 3) < 12 ? (m - 3) : 12];
+                if (m >= 15) t2 = golomb_bits(m - 15);
+            }
+            s_t01[i] = make_float2(T.bt[cls][pl][ctx][tok], t1);
+            s_t2[i] = t2;
+            nzc += m != 0;
+        }
+        for (int off = 32; off > 0; off >>= 1) nzc += __shfl_xor(nzc, off);
+        __syncthreads();
+        if (lane == 0) {
+            float bits = adaptive_skip ? d.skip_zero : T.skip[cls][d.skip_ctx][0];
+            if (d.txtp_set) bits += T.txtp[d.txtp_set - 1][d.txtp_mode][d.txtp];
+            const int eob_bin = eob < 2 ? eob : 32 - __clz(eob);
+            bits += T.eob_bin[d.eob_bin][eob_bin];
+            if (eob_bin > 1) {
+                const int nbits = eob_bin - 2;
+                bits += T.eob_hi[cls][pl][eob_bin][(eob >> nbits) & 1];
+                bits += (float)nbits;
+            }
+            bool over = bits > d.bound;
+            if (!over) {
+                const int rc = scan[eob];
+                const int ex = rc >> log2h, ey = rc & (h - 1);
+                const uint32_t m = uabs32(cf[rc]);
+                const uint32_t et = (m < 3 ? m : 3) - 1;
+                const int ctx_e = 1 + (eob > (N >> 3) ? 1 : 0) + (eob > (N >> 2) ? 1 : 0);
+                bits += T.eob_base[cls][pl][ctx_e][et];
+                if (et == 2) {
+                    const int bc = (ex | ey) > 1 ? 14 : 7;
+                    float hi = T.br[cb][pl][bc][(m - 3) < 12 ? (m - 3) : 12];
+                    if (m >= 15) hi += golomb_bits(m - 15);
+                    bits += hi;
+                }
+                // positions i .. down to the next multiple of 16 form one run between the
+                // reference's bound checks (made before pricing every 16th position)
+                int i = eob - 1;
+                while (i >= 1) {
+                    if ((i & 15) == 0 && bits > d.bound) {
+                        over = true;
+                        break;
+                    }
+                    const int lo = max(1, (i & 15) == 0 ? i - 15 : (i & ~15) + 1);
+                    for (; i >= lo; i--) {
+                        const float2 t01 = s_t01[i];
+                        bits += t01.x;
+                        bits += t01.y;
+                        bits += s_t2[i];
+                    }
+                }
+            }
+            if (!over) {
+                const uint32_t dm = uabs32(cf[0]);
+                const uint32_t dt = dm < 3 ? dm : 3;
+                bits += T.bt[cls][pl][0][dt];
+                if (dt == 3) {
+                    const uint32_t mg = (lvl_at(1, 0) + lvl_at(h, 0) + lvl_at(h + 1, 0)) & 63;
+                    const int bc = mg > 12 ? 6 : (int)((mg + 1) >> 1);
+                    float hi = T.br[cb][pl][bc][(dm - 3) < 12 ? (dm - 3) : 12];
+                    if (dm >= 15) hi += golomb_bits(dm - 15);
+                    bits += hi;
+                }
+                if (cf[0] != 0) bits += T.dc_sign[pl][d.dcs_ctx][cf[0] < 0 ? 1 : 0];
+                const uint32_t ac_nonzero = 1 + (uint32_t)nzc;
+                for (uint32_t k = 0; k < ac_nonzero; k++) bits += 1.0f;
+            }
+            out[b] = over ? INFINITY : bits;
+        }
+        __syncthreads();
+    }
+}
+
+// Lists the valid blocks of kRateWaveN or more coefficients.
+__global__ void __launch_bounds__(256) rate_classify_kernel(const mt_rateblk* __restrict__ blocks, int n,
+                                                            uint32_t* __restrict__ list,
+                                                            uint32_t* __restrict__ cnt) {
+    auto cls = [&](int b, bool) -> int {
+        RateBlk R;
+        return (rate_decode(blocks[b], R) && R.N >= kRateWaveN) ? 0 : -1;
+    };
+    mtlib::classify_chunk<1>(n, cls, list, 0, cnt);
 }
 
 // ---------------------------------------------------------------- host-side tables
@@ -387,27 +574,56 @@ int rate_init_device(int /*device*/) {
 }
 }  // namespace mtlib
 
-extern "C" int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
-                             void* stream) {
-    int dev = 0;
-    MT_HIP_TRY(hipGetDevice(&dev));
-    if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !bits_d)))
-        return mtlib::set_error(MT_ERR_ARG, "mt_block_rate: bad pointer or count");
-    if (n == 0) return MT_OK;
-    mtlib::ProfScope prof(mtlib::kProfRate, (hipStream_t)stream);
-    hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, blocks_d, n,
-                       levels_d, bits_d, (const mt_txres*)nullptr);
-    MT_HIP_TRY(hipGetLastError());
-    return MT_OK;
+namespace mtlib {
+int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                         const mt_txres* nnz_d, hipStream_t s);
 }
 
+extern "C" int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
+                             void* stream) {
+    if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !bits_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_block_rate: bad pointer or count");
+    return mtlib::rate_launch_with_nnz(blocks_d, n, levels_d, bits_d, nullptr, (hipStream_t)stream);
+}
+
+namespace {
+struct RateWork {
+    size_t cap = 0;
+    uint32_t* list = nullptr;
+    uint32_t* cnt = nullptr;
+};
+std::mutex g_rate_mu;
+RateWork g_rate_work[64];
+}  // namespace
+
 namespace mtlib {
+// mt_block_rate for the candidate pipeline: the small blocks one lane each with the chain's
+// nonzero counts steering the eob search, the large blocks one wave each.
 int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
                          const mt_txres* nnz_d, hipStream_t s) {  // levels of any alignment
     if (n == 0) return MT_OK;
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_rate_mu);
+    RateWork& w = g_rate_work[dev];
+    if (w.cap < (size_t)n) {
+        if (w.list) MT_HIP_TRY(hipFree(w.list));
+        w.list = nullptr;
+        const size_t cap = n < 65536 ? 65536 : (size_t)n;
+        MT_HIP_TRY(hipMalloc(&w.list, cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, sizeof(uint32_t)));
     ProfScope prof(kProfRate, s);
+    hipLaunchKernelGGL(zero_counts_kernel<1>, dim3(1), dim3(64), 0, s, w.cnt);
+    hipLaunchKernelGGL(rate_classify_kernel, dim3(std::max(1, std::min((n + 255) / 256, cus * 4))),
+                       dim3(256), 0, s, blocks_d, n, w.list, w.cnt);
     hipLaunchKernelGGL(rate_kernel, dim3((n + 255) / 256), dim3(256), 0, s, blocks_d, n, levels_d,
-                       bits_d, nnz_d);
+                       bits_d, nnz_d, 1);
+    const int gw = n < cus * 32 ? n : cus * 32;
+    hipLaunchKernelGGL(rate_wave_kernel, dim3(gw), dim3(64), 0, s, blocks_d, w.list, w.cnt, levels_d,
+                       bits_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }
