@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eval", action="store_true", help="skip the candidate-evaluation leg")
     ap.add_argument("--eval-steps", type=int, default=10)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="profiling aid: run only the candidate-evaluation leg and print its object")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -68,6 +70,9 @@ def main():
         print(json.dumps({"calib_copy_bytes": args.calib_copy << 20, "copies": 5}))
         return
     mt.init(0)
+    if args.eval_only:
+        print(json.dumps(bench_eval(args, dev, torch, mt)), flush=True)
+        return
 
     n = args.blocks
     t0 = time.time()
