@@ -663,16 +663,18 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
 // ordered sum over x, round half away from zero of 8x the sum); quantizer with the 32x32
 // QM; inverse = idct_dequant_64x64 (src/idct.rs:2677).
 //
-// One wave per block.  Lane x holds column x in f64 registers and runs the 32 vertical
-// chains (basis rows come through the scalar unit from constant memory); lanes (u, v parity)
-// hold basis row u and run 16 horizontal chains each over the LDS-staged vertical output.
-// Every f64 product and sum is rounded separately in the reference's order.
+// One wave per block.  Lane x holds the 32 vertical chains of column x in f64 registers
+// (basis rows come through the scalar unit from constant memory); the vertical output goes
+// through LDS 16 rows at a time to the horizontal pass, where lanes (u, v parity) run 8
+// chains each.  Every f64 product and sum is rounded separately in the reference's order.
+// LDS stays at 12 KB per wave (the row-pass buffer and the column pass's output halves reuse
+// the same 8 KB), so about three blocks share each SIMD.
 __constant__ double c_dct64_basis[32 * 64];    // [k][i]
 __constant__ double c_dct64_basis_t[64 * 32];  // [i][k], the vertical pass's scalar stream
 
 template <int MODE>
 __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
-    __shared__ double vt[32 * 64];          // vertical-pass output; then the row-pass buffer
+    __shared__ double vt[16 * 64];          // half of the vertical output; then the row pass
     __shared__ int32_t coef[32 * 33];       // dequantized coded coefficients [fy][fx]
     __shared__ unsigned long long s_sse;
     __shared__ uint32_t s_nnz, s_hash;
@@ -703,52 +705,56 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
         const bool qm = prm.qm_level < 15;
         const uint8_t* qmrow = &g_iqm[qm ? prm.qm_level : 0][prm.qm_chroma][336];
         if (MODE != MODE_INV) {
-            // The block's residual is staged in LDS (aliasing vt) so the y loop needs no
-            // register array; loads are issued for the whole block at once.
-            int32_t* inbuf = reinterpret_cast<int32_t*>(vt);
+            // Vertical pass: lane x runs the 32 chains of column x over y ascending; the
+            // residual is staged in LDS as int16 (the chain's input type; the parity entry
+            // point reads its int32 input directly), the basis arrives wave-uniform through
+            // the scalar unit.  Its output goes through LDS in two halves of 16 rows, each
+            // followed by the horizontal pass of those rows (lane (u, parity) runs 8 chains
+            // over x ascending on LDS broadcasts).
+            int16_t* in16 = reinterpret_cast<int16_t*>(vt);
+            if (MODE == MODE_CHAIN) {
 #pragma unroll 16
-            for (int y = 0; y < 64; y++)
-                inbuf[y * 64 + lane] = MODE == MODE_CHAIN ? (int32_t)a.pool16[(size_t)off + y * 64 + lane]
-                                                          : a.in32[(size_t)g * 4096 + y * 64 + lane];
-            __syncthreads();
-            // vertical pass: vt[v][x] = sum_y basis[v][y] * in[y][x], y ascending.  Lane x
-            // carries the 32 chains of column x; basis[.][y] is wave-uniform and contiguous in
-            // the transposed copy, so it arrives through the scalar unit.
-            {
-                double acc[32];
-#pragma unroll
-                for (int v = 0; v < 32; v++) acc[v] = 0.0;
-#pragma unroll 2
-                for (int y = 0; y < 64; y++) {
-                    const double c = (double)inbuf[y * 64 + lane];
-#pragma unroll
-                    for (int v = 0; v < 32; v++)
-                        acc[v] = __dadd_rn(acc[v], __dmul_rn(c_dct64_basis_t[y * 32 + v], c));
-                }
-                __syncthreads();  // every lane is done reading the staged residual
-#pragma unroll
-                for (int v = 0; v < 32; v++) vt[v * 64 + lane] = acc[v];
+                for (int y = 0; y < 64; y++) in16[y * 64 + lane] = a.pool16[(size_t)off + y * 64 + lane];
+                __syncthreads();
             }
-            __syncthreads();
-            // horizontal pass + quantization: lane -> (u = lane % 32, v parity = lane / 32),
-            // 16 chains per lane summed over x ascending; the vertical output is read as LDS
-            // broadcasts, basis[u][x] from one 256-byte row of the transposed copy per x.
+            double acc[32];
+#pragma unroll
+            for (int j = 0; j < 32; j++) acc[j] = 0.0;
+#pragma unroll 2
+            for (int y = 0; y < 64; y++) {
+                const double c = MODE == MODE_CHAIN ? (double)in16[y * 64 + lane]
+                                                    : (double)a.in32[(size_t)g * 4096 + y * 64 + lane];
+#pragma unroll
+                for (int j = 0; j < 32; j++)
+                    acc[j] = __dadd_rn(acc[j], __dmul_rn(c_dct64_basis_t[y * 32 + j], c));
+            }
+            __syncthreads();  // the staged residual is dead
+#pragma unroll
+            for (int hf = 0; hf < 2; hf++) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) vt[j * 64 + lane] = acc[hf * 16 + j];
+                __syncthreads();
+                {
+                    const int u = lane & 31, vp = lane >> 5;
+                    double hacc[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) hacc[k] = -0.0;
+#pragma unroll 2
+                    for (int x = 0; x < 64; x++) {
+                        const double bu = c_dct64_basis_t[x * 32 + u];
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        coef[(hf * 16 + 2 * k + vp) * 33 + u] = (int32_t)round(__dmul_rn(hacc[k], 8.0));
+                }
+                __syncthreads();  // the next half overwrites vt
+            }
+            // quantize the lane's 16 coefficients, leave their dequantized values in place
             {
                 const int u = lane & 31, vp = lane >> 5;
-                double acc[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++) acc[k] = -0.0;
-#pragma unroll 4
-                for (int x = 0; x < 64; x++) {
-                    const double bu = c_dct64_basis_t[x * 32 + u];
-#pragma unroll
-                    for (int k = 0; k < 16; k++)
-                        acc[k] = __dadd_rn(acc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
-                }
-#pragma unroll
-                for (int k = 0; k < 16; k++)
-                    coef[(2 * k + vp) * 33 + u] = (int32_t)round(__dmul_rn(acc[k], 8.0));
-                // quantize the lane's 16 coefficients, leave their dequantized values in place
 #pragma unroll 1
                 for (int k = 0; k < 16; k++) {
                     const int v = 2 * k + vp;
@@ -802,39 +808,35 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
 #pragma unroll
                 for (int y = 0; y < 64; y++) v[y] = y < 32 ? rowbuf[y * 64 + x] : 0;
                 idct64<1>(v, Clip{prm.cmin, prm.cmax});
-                // column x of the result goes to the (now free) vt storage; lane x only ever
-                // touches column x, so no barrier is needed before reading it back
-                int32_t* outbuf = reinterpret_cast<int32_t*>(vt);
-#pragma unroll
-                for (int y = 0; y < 64; y++) outbuf[y * 64 + x] = (v[y] + 8) >> 4;
-                uint32_t sse = 0;
-                const int P = 1 << (prm.bd - 1), maxv = (1 << prm.bd) - 1;
                 if (MODE == MODE_INV) {
-#pragma unroll 8
-                    for (int y = 0; y < 64; y++) a.recon[(size_t)g * 4096 + y * 64 + x] = outbuf[y * 64 + x];
+#pragma unroll
+                    for (int y = 0; y < 64; y++) a.recon[(size_t)g * 4096 + y * 64 + x] = (v[y] + 8) >> 4;
                 } else {
-#pragma unroll 2
-                    for (int y0 = 0; y0 < 64; y0 += 16) {
-                        int32_t src16[16];
+                    // column x is measured in two halves of 32 rows, each parked in the
+                    // lane's own column of the row-pass buffer (which the lane alone read)
+                    uint32_t sse = 0;
+                    const int P = 1 << (prm.bd - 1), maxv = (1 << prm.bd) - 1;
 #pragma unroll
-                        for (int i = 0; i < 16; i++) src16[i] = a.pool16[(size_t)off + (y0 + i) * 64 + x];
+                    for (int h2 = 0; h2 < 2; h2++) {
 #pragma unroll
-                        for (int i = 0; i < 16; i++) {
-                            const int j = (y0 + i) * 64 + x;
-                            const int32_t r = outbuf[j];
+                        for (int y = 0; y < 32; y++) rowbuf[y * 64 + x] = (v[h2 * 32 + y] + 8) >> 4;
+#pragma unroll 4
+                        for (int y = 0; y < 32; y++) {
+                            const int j = (h2 * 32 + y) * 64 + x;
+                            const int32_t r = rowbuf[y * 64 + x];
                             if (a.recon) a.recon[(size_t)off + j] = r;
-                            const int32_t src = min(max(P + src16[i], 0), maxv);
+                            const int32_t src = min(max(P + (int32_t)a.pool16[(size_t)off + j], 0), maxv);
                             const int32_t rec = min(max(P + r, 0), maxv);
-                            const int32_t d = src - rec;
-                            sse += (uint32_t)(d * d);
+                            const int32_t dd = src - rec;
+                            sse += (uint32_t)(dd * dd);
                             hsh += pix_hash(j, r);
                         }
                     }
-                }
-                if (MODE == MODE_CHAIN) {
-                    atomicAdd(&s_sse, (unsigned long long)sse);
-                    atomicAdd(&s_hash, hsh);
-                    atomicAdd(&s_nnz, nnz);
+                    if (MODE == MODE_CHAIN) {
+                        atomicAdd(&s_sse, (unsigned long long)sse);
+                        atomicAdd(&s_hash, hsh);
+                        atomicAdd(&s_nnz, nnz);
+                    }
                 }
             }
         }
