@@ -61,8 +61,11 @@ typedef struct {
     uint8_t tx_size;   /* AV1 TxSize */
     uint8_t tx_type;   /* AV1 TxType */
     uint8_t qidx;      /* base_q_idx of Quant::new_with_qm */
-    uint8_t bd_qm;     /* bits 0-3: QM level (15 = flat); bits 4-5: 0 8-bit, 1 10-bit, 2 12-bit */
+    uint8_t bd_qm;     /* bits 0-3: QM level (15 = flat); bits 4-5: 0 8-bit, 1 10-bit, 2 12-bit;
+                          bit 6 (MT_TXBLK_CHROMA): chroma quantizer, new_chroma_with_delta_qm
+                          (src/quant.rs:394) with zero qindex deltas; else the luma new_with_qm */
 } mt_txblk;
+#define MT_TXBLK_CHROMA 0x40
 
 /* Per-block result of the chain: 16 bytes in HBM. */
 typedef struct {
@@ -259,11 +262,13 @@ typedef struct {
     int8_t angle_delta;   /* -3..3 */
     uint8_t pflags;       /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB */
     uint8_t planes;       /* bits 0-1: source plane index, bits 2-3: reconstruction plane index */
-    uint8_t qidx;         /* base_q_idx of the luma Quant::new_with_qm */
-    uint8_t bd_qm;        /* as mt_txblk: bits 0-3 QM level, bits 4-5 bit depth (0 8, 1 10, 2 12) */
+    uint8_t qidx;         /* base_q_idx of the quantizer */
+    uint8_t bd_qm;        /* as mt_txblk: bits 0-3 QM level, bits 4-5 bit depth (0 8, 1 10, 2 12),
+                             bit 6 chroma: new_chroma_with_delta_qm with dc_delta / ac_delta */
     uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;  /* rate context, as mt_rateblk */
     uint8_t txtp_set, txtp_mode, txtp, qctx;
-    uint8_t rflags, pad0, pad1, pad2;
+    uint8_t rflags, pad0;
+    int8_t dc_delta, ac_delta;  /* chroma qindex deltas (src/quant.rs:394) */
     float skip_one, skip_zero;
     float wsse_scale;     /* RdDispatch::wsse_scale: > 0 prices the weighted distortion */
     float lambda;         /* rd_cost multiplier */
@@ -283,7 +288,8 @@ typedef struct {
 
 /* The reference's candidate loop body (src/coder/partition_search.rs:471-520) for n
  * candidates at once: prediction from the reconstruction plane, residual against the source
- * plane, forward transform + quant (+QM), dequant + inverse, reconstruction distortion
+ * plane, forward transform + quant (+QM; luma or chroma quantizer), dequant + inverse,
+ * reconstruction distortion
  * (mt_distortion), coefficient rate (mt_block_rate) and rd_cost (src/partition_rd.rs:140).
  * Trellis is not applied (the Speed::Fast path).  pool_elems bounds every candidate's
  * offset + W*H; the stage pools (predictions, residuals, levels, reconstructed residuals)

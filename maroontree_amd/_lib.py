@@ -60,12 +60,14 @@ CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size"
                        ("mode", "u1"), ("angle_delta", "i1"), ("pflags", "u1"), ("planes", "u1"),
                        ("qidx", "u1"), ("bd_qm", "u1"), ("rplane", "u1"), ("eob_bin", "u1"),
                        ("skip_ctx", "u1"), ("dcs_ctx", "u1"), ("txtp_set", "u1"), ("txtp_mode", "u1"),
-                       ("txtp", "u1"), ("qctx", "u1"), ("rflags", "u1"), ("pad", "u1", 3),
+                       ("txtp", "u1"), ("qctx", "u1"), ("rflags", "u1"), ("pad0", "u1"),
+                       ("dc_delta", "i1"), ("ac_delta", "i1"),
                        ("skip_one", "<f4"), ("skip_zero", "<f4"), ("wsse_scale", "<f4"),
                        ("lambda", "<f4"), ("extra_bits", "<f4")])
 CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
                           ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
+TXBLK_CHROMA = 0x40
 PRED_FILTER0 = 16
 MAX_PLANES = 4
 

@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
                                                          mt_txblk* __restrict__ tb,
                                                          mt_distblk* __restrict__ db,
                                                          mt_rateblk* __restrict__ rb,
+                                                         int8_t* __restrict__ qd,
                                                          int32_t* __restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -75,6 +76,8 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     t.qidx = c.qidx;
     t.bd_qm = c.bd_qm;
     tb[i] = t;
+    qd[2 * (size_t)i] = c.dc_delta;
+    qd[2 * (size_t)i + 1] = c.ac_delta;
     mt_distblk d{};
     d.pred_offset = c.offset;
     d.resid_offset = c.offset;
@@ -143,6 +146,7 @@ struct EvalWork {
     mt_txblk* tb = nullptr;
     mt_distblk* db = nullptr;
     mt_rateblk* rb = nullptr;
+    int8_t* qd = nullptr;
     int32_t* status = nullptr;
     int32_t* pstatus = nullptr;
     mt_txres* txr = nullptr;
@@ -168,7 +172,7 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
     if (w.cap_n < n) {
         const size_t c = n < 4096 ? 4096 : n;
         if ((rc = grow(w.pb, c)) || (rc = grow(w.tb, c)) || (rc = grow(w.db, c)) ||
-            (rc = grow(w.rb, c)) || (rc = grow(w.status, c)) || (rc = grow(w.pstatus, c)) ||
+            (rc = grow(w.rb, c)) || (rc = grow(w.qd, 2 * c)) || (rc = grow(w.status, c)) || (rc = grow(w.pstatus, c)) ||
             (rc = grow(w.txr, c)) || (rc = grow(w.dr, c)) || (rc = grow(w.bits, c)))
             return rc;
         w.cap_n = c;
@@ -207,11 +211,12 @@ extern "C" int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane*
     mtlib::ProfScope prof(mtlib::kProfEval, s);
     const int g256 = (n + 255) / 256;
     hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
-                       w.pb, w.tb, w.db, w.rb, w.status);
+                       w.pb, w.tb, w.db, w.rb, w.qd, w.status);
     MT_HIP_TRY(hipGetLastError());
     // prediction and residual_pred in one pass
     if ((rc = mtlib::intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
-    if ((rc = mt_tx_chain(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, stream))) return rc;
+    if ((rc = mtlib::tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s)))
+        return rc;
     if ((rc = mt_distortion(w.db, n, planes, nplanes, w.pred, w.recon, w.dr, stream))) return rc;
     if ((rc = mtlib::rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
     hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,

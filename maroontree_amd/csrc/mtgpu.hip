@@ -164,14 +164,25 @@ struct BlkParam {
     int32_t qm_level, qm_chroma, bd;
 };
 
-__device__ inline void param_from_desc(const mt_txblk& d, BlkParam& p) {
+// The quantizer of a chain descriptor: Quant::new_with_qm (src/quant.rs:354) for luma; with
+// MT_TXBLK_CHROMA, Quant::new_chroma_with_delta_qm (:394) with the block's (dc, ac) qindex
+// deltas from `qdelta` (two int8 per block; both 0 without the array).
+__device__ inline void param_from_desc(const mt_txblk& d, BlkParam& p, const int8_t* qdelta,
+                                       uint32_t g) {
     const int qm = d.bd_qm & 15;
     const int bdi = (d.bd_qm >> 4) & 3;
     const int bd = bdi == 0 ? 8 : bdi == 1 ? 10 : 12;
     const int q = d.qidx;
-    const int dci = q - 8 < 0 ? 0 : q - 8;  // luma DC delta -8 (src/quant.rs:350)
-    p.dc = c_dcq[bdi][dci];
-    p.ac = c_acq[bdi][q];
+    const bool chroma = (d.bd_qm & MT_TXBLK_CHROMA) != 0;
+    if (chroma) {
+        const int ddc = qdelta ? qdelta[2 * (size_t)g] : 0, dac = qdelta ? qdelta[2 * (size_t)g + 1] : 0;
+        p.dc = c_dcq[bdi][min(255, max(0, q + ddc))];
+        p.ac = c_acq[bdi][min(255, max(0, q + dac))];
+    } else {
+        const int dci = q - 8 < 0 ? 0 : q - 8;  // luma DC delta -8 (src/quant.rs:350)
+        p.dc = c_dcq[bdi][dci];
+        p.ac = c_acq[bdi][q];
+    }
     // round(65536 / step) of the reference's f64 (src/quant.rs:362): no step divides
     // 131072 oddly, so the integer round-half-up below is the same value.
     p.qdc = (131072 + p.dc) / (2 * p.dc);
@@ -189,7 +200,7 @@ __device__ inline void param_from_desc(const mt_txblk& d, BlkParam& p) {
         p.cf_max = rmax;
     }
     p.qm_level = qm;
-    p.qm_chroma = 0;
+    p.qm_chroma = chroma ? 1 : 0;
     p.bd = bd;
 }
 
@@ -310,6 +321,7 @@ struct KArgs {
     int32_t* levels;
     float* targets;
     int32_t* recon;
+    const int8_t* qdelta;  // chain mode, optional: chroma (dc, ac) qindex deltas per block
     int n;      // parity modes: block count
     int type;   // parity modes: tx type
 };
@@ -378,7 +390,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                     const mt_txblk d = a.blocks[g];
                     gidx[b] = g;
                     goff[b] = d.offset;
-                    param_from_desc(d, prm[b]);
+                    param_from_desc(d, prm[b], a.qdelta, g);
                 } else {
                     const uint32_t g = base + b;
                     gidx[b] = g;
@@ -690,7 +702,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             g = a.order[chain_start + blk];
             const mt_txblk d = a.blocks[g];
             off = d.offset;
-            param_from_desc(d, prm);
+            param_from_desc(d, prm, a.qdelta, g);
         } else {
             g = blk;
             off = blk;
@@ -1237,6 +1249,13 @@ extern "C" int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* level
 
 extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                            int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream) {
+    return mtlib::tx_chain_launch(blocks_d, n, pool_d, res_d, levels_d, targets_d, recon_d, nullptr,
+                                  (hipStream_t)stream);
+}
+
+int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
+                           int32_t* levels_d, float* targets_d, int32_t* recon_d,
+                           const int8_t* qdelta_d, hipStream_t stream) {
     int dev;
     int rc = current_device(&dev);
     if (rc) return rc;
@@ -1274,6 +1293,7 @@ extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_
     a.levels = levels_d;
     a.targets = targets_d;
     a.recon = recon_d;
+    a.qdelta = qdelta_d;
     // The class sizes live on the device; each launch is persistent over its own tiles, so
     // the grid only needs an upper bound: n blocks can never need more tiles than this.
     for (int slot = 0; slot < 13; slot++) {

@@ -56,7 +56,10 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     const uint16_t* src = sp.data + (size_t)c.y * sp.stride + c.x;
     orc_residual_pred(src, sp.stride, pred.data(), w, h, resid.data());
     orc_quant q;
-    orc_quant_new(c.qidx, bd, qm, &q);
+    if (c.bd_qm & 0x40)  // Quant::new_chroma_with_delta_qm (src/quant.rs:394)
+        orc_quant_new_chroma(c.qidx, c.dc_delta, c.ac_delta, bd, qm, &q);
+    else  // Quant::new_with_qm (src/quant.rs:354)
+        orc_quant_new(c.qidx, bd, qm, &q);
     orc_fwd_txfm(s, t, resid.data(), &q, levels.data(), targets.data());
     orc_inv_txfm(s, t, levels.data(), &q, inv.data());
     const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);

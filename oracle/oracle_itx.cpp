@@ -664,7 +664,12 @@ extern "C" int orc_chain_batch(int n, const uint8_t* tx_size, const uint8_t* tx_
         std::vector<float> tg(1024);
         for (int i = begin; i < end; i++) {
             orc_quant q;
-            orc_quant_new(qidx[i], bd[i], qm_level[i], &q);
+            // qm_level bit 6: the chroma quantizer with zero qindex deltas
+            // (Quant::new_chroma_with_delta_qm, src/quant.rs:394)
+            if (qm_level[i] & 0x40)
+                orc_quant_new_chroma(qidx[i], 0, 0, bd[i], qm_level[i] & 15, &q);
+            else
+                orc_quant_new(qidx[i], bd[i], qm_level[i] & 15, &q);
             int64_t sse = orc_chain_block(tx_size[i], tx_type[i], pool + offset[i], &q, lv.data(),
                                           tg.data(), rec.data());
             sse_out[i] = sse;

@@ -55,6 +55,10 @@ def make_cands(rng, n, bd):
         r["qidx"] = int(rng.integers(0, 256))
         qm = 15 if rng.random() < 0.4 else int(rng.integers(0, 15))
         r["bd_qm"] = qm | (bdi << 4)
+        if i % 3 == 2:  # chroma quantizer with qindex deltas
+            r["bd_qm"] |= mt.TXBLK_CHROMA
+            r["dc_delta"] = int(rng.integers(-20, 21))
+            r["ac_delta"] = int(rng.integers(-20, 21))
         nc = mt.tx_ncoef(s)
         r["rplane"] = 0 if s == 4 else int(rng.integers(0, 2))
         r["eob_bin"] = EOB_BINS[nc][int(rng.integers(0, len(EOB_BINS[nc])))]
@@ -91,7 +95,10 @@ def oracle_candidate(r, planes_np, views):
                         rfw, rfh, pf & 4, bd).ravel().astype(np.int32)
     sblk = np.ascontiguousarray(src[y:y + h, x:x + w])
     resid = sblk.astype(np.int32).ravel() - pred
-    q = O.quant(int(r["qidx"]), bd, qm)
+    if int(r["bd_qm"]) & mt.TXBLK_CHROMA:
+        q = O.quant_chroma(int(r["qidx"]), int(r["dc_delta"]), int(r["ac_delta"]), bd, qm)
+    else:
+        q = O.quant(int(r["qidx"]), bd, qm)
     levels, _ = O.fwd(s, t, resid, q)
     inv = O.inv(s, t, levels, q)
     L = O.plib()

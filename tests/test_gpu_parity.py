@@ -203,6 +203,29 @@ def test_chain_edge_batches(dev):
     assert (got["sse"] == sse).all() and (got["hash"] == h).all()
 
 
+def test_chain_chroma_quantizer(dev):
+    """Blocks with the chroma bit (bd_qm bit 6) take new_chroma_with_delta_qm (zero deltas:
+    the chroma QM planes); the others the luma quantizer, in one mixed batch."""
+    n = 20000
+    blocks, pool = mt.gen_tx_blocks(SEED + 4, n)
+    rng = np.random.default_rng(SEED + 4)
+    chroma = rng.random(n) < 0.5
+    blocks["bd_qm"][chroma] |= mt.TXBLK_CHROMA
+    say(f"chain chroma quantizer: {n} blocks, {int(chroma.sum())} chroma")
+    bt, pt = _upload(blocks, pool, dev)
+    res = torch.zeros(n * 16, dtype=torch.uint8, device=dev)
+    mt.tx_chain(bt, pt, res)
+    got = res.cpu().numpy().view(mt._lib.RES_DTYPE)
+    sse, h = O.chain_batch(blocks["tx_size"], blocks["tx_type"], blocks["qidx"], _bd(blocks),
+                           blocks["bd_qm"] & 0x4f, blocks["offset"].astype(np.uint64), pool)
+    bad = np.nonzero((got["sse"] != sse) | (got["hash"] != h))[0]
+    assert len(bad) == 0, f"{len(bad)} blocks differ, first {bad[:10]}"
+    # the chroma planes of the QM change the result of some blocks
+    sse_l, h_l = O.chain_batch(blocks["tx_size"], blocks["tx_type"], blocks["qidx"], _bd(blocks),
+                               blocks["bd_qm"] & 15, blocks["offset"].astype(np.uint64), pool)
+    assert (h_l[chroma] != h[chroma]).any()
+
+
 def test_chain_full_workload_every_block(dev):
     """The full 2^20-block microbench workload: SSE and checksum of every block equal the
     oracle's (the checksum covers every level, target and reconstructed sample)."""
