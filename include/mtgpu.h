@@ -298,12 +298,92 @@ typedef struct {
 int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes, int nplanes,
                        uint64_t pool_elems, mt_candres* res_d, void* stream);
 
+/* ---- whole-frame luma intra RDO over the superblock wavefront ----------------------- */
+/* The candidate generators and argmin of the reference's decision code (A19:
+ * rd_cost_square src/coder/partition_search.rs:338, pick_luma16 :4755, pick_block8
+ * src/coder/block8.rs:689, pick_luma32 src/coder/block32.rs:728, rd_pick_luma64
+ * src/coder/block64.rs:1812, decode_sb src/coder/superblock.rs:813) replaced by one
+ * exhaustive, data-parallel procedure of this library:
+ *
+ *  - 64x64 superblocks (SB) run along the anti-diagonal wavefront: SB (r, c) in step
+ *    c + 2r, so its left, above and above-right SBs are reconstructed before it starts;
+ *  - in a step, every quadtree node (64 .. 8, or .. 4 with MT_RDO_MIN4) of every SB of
+ *    the step that lies inside the frame evaluates every allowed mode as one DCT_DCT
+ *    candidate of mt_eval_candidates: the 13 intra modes, with MT_RDO_ANGLE_DELTAS the 6
+ *    non-zero angle deltas of the 8 directional modes at >= 8x8, with MT_RDO_FILTER_INTRA
+ *    the 5 filter-intra modes at <= 32x32; it predicts from the reconstruction, whose
+ *    samples inside the SB still hold the source at decision time;
+ *  - rate = coefficient bits (real_block_bits_bounded, txb skip / dc sign context 0, the
+ *    intra tx-set symbol of DCT_DCT at <= 16x16) + luma_intra_syntax_bits
+ *    (src/coder/superblock.rs:192: kf_y mode, angle delta, use_filter_intra / mode);
+ *    cost = weighted distortion + lambda * rate (rd_cost);
+ *  - bottom-up, a node keeps PARTITION_NONE (its cheapest mode; ties to the lower mode
+ *    id) or SPLIT (its children's sum), each adding lambda * the partition symbol bits
+ *    (partition_bits_bl, src/coder/lossy_state.rs:1718, frame-initial CDFs); nodes that
+ *    cross the frame edge split without a symbol;
+ *  - the chosen leaves are reconstructed in coding order, one wave per SB;
+ *  - decision-time contexts (kf_y mode context, edge-filter type, partition context) read
+ *    the final maps outside the SB and count neighbours inside it as DC_PRED / unsplit;
+ *  - the final accounting re-prices every leaf and partition symbol with the final
+ *    contexts against the final reconstruction.
+ * Mode ids: 0..12 the intra modes (MT_PRED_*, delta 0); 13 + 6 (m - 1) + j directional
+ * mode m in 1..8 with delta {-3, -2, -1, 1, 2, 3}[j]; 61..65 filter intra 0..4. */
+enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4 };
+#define MT_RDO_NODES 341          /* quadtree nodes of one SB: 1 + 4 + 16 + 64 + 256 */
+#define MT_RDO_ALL_MODES 0x3FFF   /* prune mask bits 0-12: intra modes; bit 13: filter intra */
+#define MT_RDO_FORCE_SPLIT 0x4000 /* node must split (ignored at the minimum size) */
+#define MT_RDO_NO_SPLIT 0x8000    /* node is not split (ignored where it crosses the frame) */
+
+/* One frame.  Node n of an SB is level L = 0 (64x64) .. 4 (4x4), n = base[L] + iy * 2^L + ix
+ * with base = {0, 1, 5, 21, 85}; SBs in raster order. */
+typedef struct {
+    mt_plane src;          /* luma source (device), width and height multiples of 8 */
+    uint16_t* recon;       /* out: reconstruction (device), recon_stride samples per row */
+    int32_t recon_stride;
+    uint8_t bd;            /* 8, 10, 12 */
+    uint8_t qidx;          /* base_q_idx */
+    uint8_t qm_level;      /* 0..15 (15 flat) */
+    uint8_t pad0;
+    float lambda;          /* > 0 (mt_rdo_lambda gives the reference's decision lambda) */
+    float wsse_scale;      /* RdDispatch::wsse_scale (1.0 shipped; 0 plain SSE) */
+    const uint16_t* prune; /* optional (device): [nsb][MT_RDO_NODES] node masks; NULL = full */
+    void* nodes;           /* out (device): [nsb][MT_RDO_NODES] mt_rdo_node */
+} mt_rdo_frame;
+
+typedef struct {
+    float cost;       /* the node's decided rd cost (+inf when the node was not searched) */
+    float none_cost;  /* PARTITION_NONE cost incl. its partition symbol (+inf if none) */
+    uint8_t split;    /* 0 leaf, 1 split, 2 not coded (outside the frame / not searched) */
+    uint8_t mode;     /* mode id of the cheapest NONE candidate */
+    uint8_t pad0, pad1;
+} mt_rdo_node;
+
+typedef struct {
+    double bits;         /* final rate: coefficients + mode syntax + partition symbols */
+    int64_t sse;         /* SSE of the reconstruction against the source */
+    int64_t candidates;  /* decision candidates evaluated */
+    int32_t leaves, sbs;
+    int32_t steps, pad;
+} mt_rdo_stats;
+
+/* Runs the RDO of one frame on `stream` and waits for it.  Returns MT_OK, MT_ERR_ARG for
+ * invalid parameters, or a device error. */
+int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream);
+
+/* The reference's decision lambda lambda_d = mlam / K_PART_LAMBDA
+ * (src/coder/lossy_state.rs:1769-1790): mode_lambda_q(dc_q, bd) (src/cost.rs:285) times
+ * mode_lambda_weight(qidx, sub) (src/cost.rs:132, shipped tuning) over 2.775.
+ * sub: 0 4:4:4, 1 4:2:2, 2 4:2:0.  mt_mode_lambda returns mlam. */
+float mt_rdo_lambda(int qidx, int bd, int sub);
+float mt_mode_lambda(int qidx, int bd, int sub);
+
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and
  * returns, per slot, the summed milliseconds and launch count.  Slot 0 = chain bucketing
  * (3 small kernels), slots 1..13 = the chain's size kernels in the order 4x4, 4x8, 8x4,
  * 4x16, 16x4, 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 64x64; 14 = rate, 15 = intra
- * prediction, 16 = distortion, 17 = candidate evaluation. */
+ * prediction, 16 = distortion, 17 = candidate evaluation, 18 = the RDO's candidate
+ * generation / argmin / decision kernels, 19 = the RDO's superblock reconstruction. */
 int mt_profile_begin(void);
 int mt_profile_end(double* kernel_ms, int* launches, int max_slots);
 

@@ -108,6 +108,9 @@ def lib():
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
+        L.mt_intra_rdo.argtypes = [vp, i32, vp, vp]
+        L.mt_rdo_lambda.argtypes = L.mt_mode_lambda.argtypes = [i32, i32, i32]
+        L.mt_rdo_lambda.restype = L.mt_mode_lambda.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -228,7 +231,8 @@ def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=No
 
 
 PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",
-                 "16x32", "32x16", "32x32", "64x64", "rate", "intra_pred", "distortion", "eval"]
+                 "16x32", "32x16", "32x32", "64x64", "rate", "intra_pred", "distortion", "eval",
+                 "rdo_decide", "rdo_recon"]
 
 
 def profile_begin():

@@ -1,6 +1,7 @@
 """Builds libmtgpu.so (HIP, gfx950) in-tree, and the CPU oracle used by the tests."""
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -15,17 +16,34 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = [os.path.join(PKG, "csrc", f) for f in ("mtgpu.hip", "mt_rate.hip", "mt_pred.hip", "mt_dist.hip", "mt_eval.hip")]
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("mtgpu.hip", "mt_rate.hip", "mt_pred.hip", "mt_dist.hip", "mt_eval.hip", "mt_rdo.hip")]
+
+
+def _compile(src):
+    obj = os.path.join(PKG, "csrc", "_obj", os.path.basename(src) + ".o")
+    cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, r
 
 
 def build_library(verbose=False):
-    cmd = [HIPCC] + HIP_FLAGS + ["-o", LIB + ".tmp"] + SOURCES
+    """Compiles every source for gfx950 (one process per file) and links libmtgpu.so."""
+    os.makedirs(os.path.join(PKG, "csrc", "_obj"), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
+        results = list(ex.map(_compile, SOURCES))
+    msgs = []
+    for obj, r in results:
+        out = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed:\n" + out)
+        if out.strip():
+            msgs.append(out)
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + [o for o, _ in results]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + out)
-    if verbose and out.strip():
-        print(out)
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    if verbose and msgs:
+        print("\n".join(msgs))
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

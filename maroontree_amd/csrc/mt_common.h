@@ -34,6 +34,23 @@ int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int
 int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
                          const mt_txres* nnz_d, hipStream_t s);
 
+// mt_distortion on a prepared plane set.
+int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, const int32_t* pred_d,
+                      const int32_t* resid_d, mt_distres* out_d, hipStream_t s);
+// mt_eval_candidates on a prepared plane set.
+int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
+                mt_candres* res_d, hipStream_t s);
+// The RDO's superblock reconstruction: one wave per superblock b walks the leaves
+// leaves_d[first_d[b] .. + count_d[b]) in coding order, predicting from and writing to the
+// reconstruction plane rec (rstride samples per row; frame size of src).
+int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
+                    int nsb, const mt_plane& src, uint16_t* rec, int rstride, int qidx, int bd_qm,
+                    hipStream_t s);
+// Cost in bits of symbol s of a raw AV1 CDF of nsym symbols (nsym - 1 values), through the
+// reference's cost_q_table (host).
+float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s);
+int rdo_init_device(int device);
+
 // Kernel timing slots of mt_profile_end.
 enum {
     kProfBucket = 0,   // 1..13: the transform chain per size
@@ -41,7 +58,9 @@ enum {
     kProfPred = 15,
     kProfDist = 16,
     kProfEval = 17,
-    kProfSlots = 18
+    kProfRdo = 18,     // the RDO's candidate generation, argmin and decision kernels
+    kProfRecon = 19,   // the RDO's superblock reconstruction
+    kProfSlots = 20
 };
 // While profiling is on (mt_profile_begin), brackets a launch with HIP events on `s`.
 struct ProfScope {

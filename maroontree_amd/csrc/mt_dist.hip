@@ -373,17 +373,12 @@ std::mutex g_dist_mu;
 DistWork g_dist_work[64];
 }  // namespace
 
-extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
-                             const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d,
-                             void* stream) {
-    if (n < 0 || (n > 0 && (!blocks_d || !pred_d || !out_d)))
-        return mtlib::set_error(MT_ERR_ARG, "mt_distortion: bad pointer or count");
-    PlaneSet ps{};
-    int rc = mtlib::planes_from_host(planes, nplanes, "mt_distortion", &ps);
-    if (rc != MT_OK) return rc;
+namespace mtlib {
+int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, const int32_t* pred_d,
+                      const int32_t* resid_d, mt_distres* out_d, hipStream_t s) {
     if (n == 0) return MT_OK;
     int dev = 0, cus = 0;
-    rc = mtlib::device_ready(&dev, &cus);
+    int rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_dist_mu);
     DistWork& w = g_dist_work[dev];
@@ -395,7 +390,6 @@ extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* 
         w.cap = cap;
     }
     if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, (kSmallShapes + 1) * sizeof(uint32_t)));
-    hipStream_t s = (hipStream_t)stream;
     const int32_t* resid = resid_d ? resid_d : pred_d;
     mtlib::ProfScope prof(mtlib::kProfDist, s);
     hipLaunchKernelGGL(mtlib::zero_counts_kernel<kSmallShapes + 1>, dim3(1), dim3(64), 0, s, w.cnt);
@@ -415,4 +409,16 @@ extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* 
                        pred_d, resid, out_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
+}
+}  // namespace mtlib
+
+extern "C" int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int nplanes,
+                             const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d,
+                             void* stream) {
+    if (n < 0 || (n > 0 && (!blocks_d || !pred_d || !out_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_distortion: bad pointer or count");
+    PlaneSet ps{};
+    int rc = mtlib::planes_from_host(planes, nplanes, "mt_distortion", &ps);
+    if (rc != MT_OK) return rc;
+    return mtlib::distortion_launch(blocks_d, n, ps, pred_d, resid_d, out_d, (hipStream_t)stream);
 }

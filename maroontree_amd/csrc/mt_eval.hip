@@ -189,6 +189,35 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
 
 }  // namespace
 
+namespace mtlib {
+int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
+                mt_candres* res_d, hipStream_t s) {
+    if (n == 0) return MT_OK;
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_eval_mu);
+    EvalWork& w = g_work[dev];
+    rc = ensure(w, (size_t)n, (size_t)pool_elems);
+    if (rc != MT_OK) return rc;
+    ProfScope prof(kProfEval, s);
+    const int g256 = (n + 255) / 256;
+    hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
+                       w.pb, w.tb, w.db, w.rb, w.qd, w.status);
+    MT_HIP_TRY(hipGetLastError());
+    // prediction and residual_pred in one pass
+    if ((rc = intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
+    if ((rc = tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s)))
+        return rc;
+    if ((rc = distortion_launch(w.db, n, ps, w.pred, w.recon, w.dr, s))) return rc;
+    if ((rc = rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
+    hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,
+                       w.pstatus, w.txr, w.dr, w.bits, res_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+}  // namespace mtlib
+
 extern "C" int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes,
                                   int nplanes, uint64_t pool_elems, mt_candres* res_d,
                                   void* stream) {
@@ -199,28 +228,5 @@ extern "C" int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane*
     PlaneSet ps{};
     int rc = mtlib::planes_from_host(planes, nplanes, "mt_eval_candidates", &ps);
     if (rc != MT_OK) return rc;
-    if (n == 0) return MT_OK;
-    int dev = 0, cus = 0;
-    rc = mtlib::device_ready(&dev, &cus);
-    if (rc != MT_OK) return rc;
-    std::lock_guard<std::mutex> lk(g_eval_mu);
-    EvalWork& w = g_work[dev];
-    rc = ensure(w, (size_t)n, (size_t)pool_elems);
-    if (rc != MT_OK) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    mtlib::ProfScope prof(mtlib::kProfEval, s);
-    const int g256 = (n + 255) / 256;
-    hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
-                       w.pb, w.tb, w.db, w.rb, w.qd, w.status);
-    MT_HIP_TRY(hipGetLastError());
-    // prediction and residual_pred in one pass
-    if ((rc = mtlib::intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
-    if ((rc = mtlib::tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s)))
-        return rc;
-    if ((rc = mt_distortion(w.db, n, planes, nplanes, w.pred, w.recon, w.dr, stream))) return rc;
-    if ((rc = mtlib::rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
-    hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,
-                       w.pstatus, w.txr, w.dr, w.bits, res_d);
-    MT_HIP_TRY(hipGetLastError());
-    return MT_OK;
+    return mtlib::eval_launch(cands_d, n, ps, pool_elems, res_d, (hipStream_t)stream);
 }

@@ -628,3 +628,7 @@ int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* level
     return MT_OK;
 }
 }  // namespace mtlib
+
+namespace mtlib {
+float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s) { return sym_cost(raw_cdf, nsym - 1, s); }
+}  // namespace mtlib

@@ -1,0 +1,738 @@
+// Whole-frame luma intra RDO on the GPU: the superblock anti-diagonal wavefront of
+// mt_intra_rdo (the procedure is documented at its declaration in include/mtgpu.h).
+//
+// Per wavefront step (every SB (r, c) with c + 2r = step):
+//   rdo_gen_kernel     one wave per SB: node flags, then per node a ballot over the 66 mode
+//                      ids writes the allowed candidates (compacted, in node / id order)
+//                      with their decision-time contexts and mode syntax bits;
+//   eval_launch        the batched candidate evaluation (prediction ... rd_cost);
+//   rdo_argmin_kernel  one thread per candidate: atomicMin of (cost bits, mode id) per node
+//                      (costs are >= 0, so their bit patterns order like the values);
+//   rdo_decide_kernel  one wave per SB: bottom-up NONE / SPLIT, leaves in coding order,
+//                      the final mode / partition maps and the leaf descriptors;
+//   sb_recon_launch    one wave per SB reconstructs its leaves (mt_sbrecon.h).
+// Afterwards every leaf is re-priced with the final contexts in one batched evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mtgpu.h"
+#include "av1_cdf_tables.h"
+#include "mt_common.h"
+
+namespace {
+
+using mtlib::PlaneSet;
+
+constexpr int kNodes = MT_RDO_NODES;
+constexpr int kModeIds = 66;
+constexpr int kMaxLeaves = 256;
+
+// Symbol bits of the frame-initial decision CDFs (Cdfs::new, src/coder/cdfs.rs:400).
+struct SymBits {
+    float kf[5][5][13];   // kf_y mode [above ctx][left ctx][mode]
+    float angle[8][7];    // angle delta [mode - V_PRED][delta + 3]
+    float fuse[5][2];     // use_filter_intra at 4x4 .. 64x64 (av1 block sizes 0, 3, 6, 9, 12)
+    float fmode[5];       // filter intra mode
+    float part[4][4][2];  // [bl - 1][ctx][NONE, SPLIT]: 64, 32, 16 (10 symbols), 8 (4)
+};
+__constant__ SymBits c_sym;
+
+__host__ __device__ inline int level_base(int L) { return L == 0 ? 0 : L == 1 ? 1 : L == 2 ? 5 : L == 3 ? 21 : 85; }
+__host__ __device__ inline int node_level(int id) {
+    return id >= 85 ? 4 : id >= 21 ? 3 : id >= 5 ? 2 : id >= 1 ? 1 : 0;
+}
+__host__ __device__ inline int node_id(int L, int ix, int iy) { return level_base(L) + iy * (1 << L) + ix; }
+
+// Coding order of the s-aligned cell (ix, iy) inside a superblock: bit-interleaved, y high.
+__host__ __device__ inline int zidx(int ix, int iy) {
+    int z = 0;
+    for (int b = 0; b < 4; ++b) z |= (((ix >> b) & 1) << (2 * b)) | (((iy >> b) & 1) << (2 * b + 1));
+    return z;
+}
+// Top-right / bottom-left neighbour of the same size coded before the s x s block at
+// (xin, yin) of its SB (the frame edge clips the sample counts inside the predictor).
+__host__ __device__ inline bool have_tr(int xin, int yin, int s) {
+    if (yin == 0) return true;
+    if (xin + s >= 64) return false;
+    return zidx(xin / s + 1, yin / s - 1) < zidx(xin / s, yin / s);
+}
+__host__ __device__ inline bool have_bl(int xin, int yin, int s) {
+    if (yin + s >= 64) return false;
+    if (xin == 0) return true;
+    return zidx(xin / s - 1, yin / s + 1) < zidx(xin / s, yin / s);
+}
+
+__host__ __device__ inline int id_mode(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 16 + id - 61; }
+__host__ __device__ inline int id_delta(int id) {
+    if (id < 13 || id > 60) return 0;
+    const int j = (id - 13) % 6;
+    return j < 3 ? j - 3 : j - 2;
+}
+__host__ __device__ inline int id_base(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 0; }
+__host__ __device__ inline int id_bit(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 13; }
+__host__ __device__ inline bool id_valid(int id, int s, int flags) {
+    if (id <= 12) return true;
+    if (id <= 60) return (flags & MT_RDO_ANGLE_DELTAS) && s >= 8;
+    return (flags & MT_RDO_FILTER_INTRA) && s <= 32;
+}
+__host__ __device__ inline bool id_allowed(int id, uint16_t mask) {
+    return id == 0 || ((mask >> id_bit(id)) & 1);  // DC_PRED is always evaluated
+}
+
+enum { kSearched = 1, kInside = 2, kNoneOk = 4, kMaySplit = 8 };
+
+struct FrameArgs {
+    mt_plane src;
+    const uint16_t* prune;
+    mt_rdo_node* nodes;
+    uint8_t* mode4;  // (W / 4) x (H / 4) luma modes (filter intra as DC_PRED)
+    uint8_t* part8;  // (W / 8) x (H / 8) partition context bytes
+    int W, H, C;
+    int bd, qidx, qm, flags, minsz;
+    float lambda, wsse;
+};
+
+// One SB of a step: its raster index and where its candidates / pool slots start.
+struct SbTask {
+    uint32_t sb;
+    uint32_t cand_base;
+    uint32_t pool_base;
+    uint32_t pad;
+};
+
+// Flags of every node of SB (r, c), in node order (parents before children).
+__device__ void node_flags(const FrameArgs& f, int r, int c, uint8_t* fl) {
+    const uint16_t* mask = f.prune ? f.prune + (size_t)(r * f.C + c) * kNodes : nullptr;
+    for (int id = 0; id < kNodes; ++id) {
+        const int L = node_level(id), k = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int ix = k % n, iy = k / n;
+        const int x = c * 64 + ix * s, y = r * 64 + iy * s;
+        const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
+        bool sr = id == 0 || ((fl[node_id(L - 1, ix >> 1, iy >> 1)] & kMaySplit) && x < f.W && y < f.H);
+        const bool in = x + s <= f.W && y + s <= f.H;
+        const bool none_ok = sr && in && (s == f.minsz || !(m & MT_RDO_FORCE_SPLIT) || (m & MT_RDO_NO_SPLIT));
+        const bool may_split = sr && s > f.minsz && (!in || !(m & MT_RDO_NO_SPLIT));
+        fl[id] = (uint8_t)((sr ? kSearched : 0) | (in ? kInside : 0) | (none_ok ? kNoneOk : 0) |
+                           (may_split ? kMaySplit : 0));
+    }
+}
+
+__device__ inline int map_mode(const FrameArgs& f, int x, int y) {
+    return f.mode4[(size_t)(y >> 2) * (f.W >> 2) + (x >> 2)];
+}
+__device__ inline int map_part(const FrameArgs& f, int x, int y) {
+    return f.part8[(size_t)(y >> 3) * (f.W >> 3) + (x >> 3)];
+}
+
+// Partition symbol bits of a node at (x, y), level L (bl = L + 1); decision-time contexts
+// read only the neighbours outside the SB at (x0, y0).
+__device__ float part_bits(const FrameArgs& f, int x, int y, int L, int sym, int x0, int y0, bool fin) {
+    const int bl = L + 1, sh = 4 - bl;
+    const int a = (y > 0 && (fin || y == y0)) ? map_part(f, x, y - 1) : 0;
+    const int l = (x > 0 && (fin || x == x0)) ? map_part(f, x - 1, y) : 0;
+    const int ctx = ((a >> sh) & 1) + (((l >> sh) & 1) << 1);
+    return c_sym.part[bl - 1][ctx][sym];
+}
+
+__constant__ int kIntraModeCtx[13] = {0, 1, 2, 3, 4, 4, 4, 4, 3, 0, 1, 2, 0};
+__constant__ int kFilterDir[5] = {0, 1, 2, 6, 0};  // Filter_Intra_Mode_To_Intra_Dir
+
+// The candidate of mode id `id` on the s x s block at (x, y) of the SB at (x0, y0).
+__device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, int x0, int y0, bool fin,
+                             uint32_t offset) {
+    mt_cand c{};
+    c.offset = offset;
+    c.x = (uint16_t)x;
+    c.y = (uint16_t)y;
+    c.tx_size = (uint8_t)(s == 4 ? MT_TX_4X4 : s == 8 ? MT_TX_8X8 : s == 16 ? MT_TX_16X16
+                          : s == 32 ? MT_TX_32X32 : MT_TX_64X64);
+    c.tx_type = MT_TXTP_DCT_DCT;
+    c.mode = (uint8_t)id_mode(id);
+    c.angle_delta = (int8_t)id_delta(id);
+    const bool above_ok = y > 0 && (fin || y == y0);
+    const bool left_ok = x > 0 && (fin || x == x0);
+    const int am = above_ok ? map_mode(f, x, y - 1) : 0, lm = left_ok ? map_mode(f, x - 1, y) : 0;
+    const bool smooth = (am >= MT_PRED_SMOOTH && am <= MT_PRED_SMOOTH_H) ||
+                        (lm >= MT_PRED_SMOOTH && lm <= MT_PRED_SMOOTH_H);
+    c.pflags = (uint8_t)((have_tr(x - x0, y - y0, s) ? MT_PRED_HAVE_TR : 0) |
+                         (have_bl(x - x0, y - y0, s) ? MT_PRED_HAVE_BL : 0) | (smooth ? MT_PRED_SMOOTH_NB : 0));
+    c.planes = 0 | (1 << 2);
+    c.qidx = (uint8_t)f.qidx;
+    const int bdi = f.bd == 8 ? 0 : f.bd == 10 ? 1 : 2;
+    c.bd_qm = (uint8_t)(f.qm | (bdi << 4));
+    c.rplane = 0;
+    c.eob_bin = (uint8_t)(s == 4 ? 0 : s == 8 ? 5 : s == 16 ? 11 : 15);
+    c.txtp_set = (uint8_t)(s == 4 ? 1 : s == 8 ? 2 : s == 16 ? 3 : 0);
+    c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
+    c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT's index in the intra tx sets
+    c.qctx = (uint8_t)(f.qidx <= 20 ? 0 : f.qidx <= 60 ? 1 : f.qidx <= 120 ? 2 : 3);
+    c.wsse_scale = f.wsse;
+    c.lambda = f.lambda;
+    // luma_intra_syntax_bits (src/coder/superblock.rs:192)
+    const int base = id_base(id);
+    const int lv = s == 64 ? 4 : s == 32 ? 3 : s == 16 ? 2 : s == 8 ? 1 : 0;
+    float bits = c_sym.kf[kIntraModeCtx[am]][kIntraModeCtx[lm]][base];
+    if (base >= MT_PRED_V && base <= MT_PRED_D67 && s >= 8) {
+        bits = __fadd_rn(bits, c_sym.angle[base - 1][id_delta(id) + 3]);
+    } else if ((f.flags & MT_RDO_FILTER_INTRA) && base == MT_PRED_DC && s <= 32) {
+        const int use = id >= 61;
+        bits = __fadd_rn(bits, c_sym.fuse[lv][use]);
+        if (use) bits = __fadd_rn(bits, c_sym.fmode[id - 61]);
+    }
+    c.extra_bits = bits;
+    return c;
+}
+
+// Candidate count and pool elements of every SB of the frame.
+__global__ __launch_bounds__(64) void rdo_count_kernel(FrameArgs f, uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ pools) {
+    __shared__ uint8_t fl[kNodes];
+    const int sb = blockIdx.x, lane = threadIdx.x;
+    const int r = sb / f.C, c = sb % f.C;
+    if (lane == 0) node_flags(f, r, c, fl);
+    __syncthreads();
+    const uint16_t* mask = f.prune ? f.prune + (size_t)sb * kNodes : nullptr;
+    uint32_t cnt = 0, pool = 0;
+    for (int id = 0; id < kNodes; ++id) {
+        if (!(fl[id] & kNoneOk)) continue;
+        const int s = 64 >> node_level(id);
+        const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
+        for (int j = lane; j < 128; j += 64) {
+            const bool ok = j < kModeIds && id_valid(j, s, f.flags) && id_allowed(j, m);
+            const uint32_t k = (uint32_t)__popcll(__ballot(ok));
+            cnt += k;
+            pool += k * (uint32_t)(s * s);
+        }
+    }
+    if (lane == 0) {
+        counts[sb] = cnt;
+        pools[sb] = pool;
+    }
+}
+
+// Writes the candidates of the step's SBs (one wave each) and their node / id references
+// ((local SB << 16) | (node << 7) | id); saves the node flags for the decision.
+__global__ __launch_bounds__(64) void rdo_gen_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+                                                     mt_cand* __restrict__ cands,
+                                                     uint32_t* __restrict__ refs,
+                                                     uint8_t* __restrict__ nflags) {
+    __shared__ uint8_t fl[kNodes];
+    const int local = blockIdx.x, lane = threadIdx.x;
+    const SbTask t = tasks[local];
+    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
+    if (lane == 0) node_flags(f, r, c, fl);
+    __syncthreads();
+    for (int id = lane; id < kNodes; id += 64) nflags[(size_t)local * kNodes + id] = fl[id];
+    const uint16_t* mask = f.prune ? f.prune + (size_t)t.sb * kNodes : nullptr;
+    uint32_t k0 = t.cand_base, p0 = t.pool_base;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int id = 0; id < kNodes; ++id) {
+        if (!(fl[id] & kNoneOk)) continue;
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+        const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
+        for (int j = lane; j < 128; j += 64) {
+            const bool ok = j < kModeIds && id_valid(j, s, f.flags) && id_allowed(j, m);
+            const uint64_t b = __ballot(ok);
+            if (ok) {
+                const uint32_t rank = (uint32_t)__popcll(b & below);
+                const uint32_t k = k0 + rank;
+                cands[k] = make_cand(f, x, y, s, j, x0, y0, false, p0 + rank * (uint32_t)(s * s));
+                refs[k] = ((uint32_t)local << 16) | ((uint32_t)id << 7) | (uint32_t)j;
+            }
+            const uint32_t tot = (uint32_t)__popcll(b);
+            k0 += tot;
+            p0 += tot * (uint32_t)(s * s);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) rdo_fill_u64_kernel(unsigned long long* p, size_t n, unsigned long long v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void __launch_bounds__(256) rdo_fill_u8_kernel(uint8_t* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+__global__ void __launch_bounds__(256) rdo_copy_plane_kernel(mt_plane src, uint16_t* __restrict__ dst, int dstride) {
+    const int W = src.width;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)W * src.height;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t y = i / W, x = i % W;
+        dst[y * dstride + x] = src.data[y * src.stride + x];
+    }
+}
+
+__global__ void __launch_bounds__(256) rdo_argmin_kernel(const uint32_t* __restrict__ refs,
+                                                         const mt_candres* __restrict__ res, int n,
+                                                         unsigned long long* __restrict__ keys) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t ref = refs[k];
+    const uint32_t local = ref >> 16, node = (ref >> 7) & 511, id = ref & 127;
+    const float cost = res[k].cost;  // >= 0, or +inf for an invalid candidate
+    const unsigned long long key = ((unsigned long long)__float_as_uint(cost) << 32) | id;
+    atomicMin(&keys[(size_t)local * kNodes + node], key);
+}
+
+// Bottom-up decision of one SB (lane 0), its leaves in coding order, maps and descriptors.
+__global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+                                                        const unsigned long long* __restrict__ keys,
+                                                        const uint8_t* __restrict__ nflags,
+                                                        mt_predblk* __restrict__ leaves,
+                                                        uint32_t* __restrict__ first,
+                                                        uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ sb_leaves,
+                                                        uint32_t* __restrict__ sb_leaf_pool) {
+    __shared__ float cost[kNodes];
+    __shared__ uint8_t split[kNodes];
+    __shared__ uint8_t mode[kNodes];
+    __shared__ uint16_t lv[kMaxLeaves];
+    const int local = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const SbTask t = tasks[local];
+    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
+    const uint8_t* fl = nflags + (size_t)local * kNodes;
+    mt_rdo_node* out = f.nodes + (size_t)t.sb * kNodes;
+    for (int id = kNodes - 1; id >= 0; --id) {
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int ix = kk % n, iy = kk / n, x = x0 + ix * s, y = y0 + iy * s;
+        const unsigned long long key = keys[(size_t)local * kNodes + id];
+        const float best = key == ~0ull ? __int_as_float(0x7f800000) : __uint_as_float((uint32_t)(key >> 32));
+        const int bid = key == ~0ull ? 0 : (int)(key & 127);
+        mode[id] = (uint8_t)bid;
+        mt_rdo_node nd{};
+        nd.mode = (uint8_t)bid;
+        if (!(fl[id] & kSearched)) {
+            cost[id] = 0.0f;
+            split[id] = 2;
+            nd.cost = __int_as_float(0x7f800000);
+            nd.none_cost = nd.cost;
+            nd.split = 2;
+            out[id] = nd;
+            continue;
+        }
+        const float inf = __int_as_float(0x7f800000);
+        float none = inf, spl = inf;
+        if ((fl[id] & kNoneOk) && best < inf) {
+            none = best;
+            if (s >= 8) none = __fadd_rn(none, __fmul_rn(f.lambda, part_bits(f, x, y, L, 0, x0, y0, false)));
+        }
+        if (fl[id] & kMaySplit) {
+            float sum = 0.0f;
+            for (int q = 0; q < 4; ++q) sum = __fadd_rn(sum, cost[node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1))]);
+            spl = (fl[id] & kInside) ? __fadd_rn(sum, __fmul_rn(f.lambda, part_bits(f, x, y, L, 1, x0, y0, false)))
+                                     : sum;
+        }
+        if (none <= spl) {
+            cost[id] = none;
+            split[id] = 0;
+        } else {
+            cost[id] = spl;
+            split[id] = 1;
+        }
+        nd.cost = cost[id];
+        nd.none_cost = none;
+        nd.split = split[id];
+        out[id] = nd;
+    }
+    // leaves in coding order (children pushed in reverse so they pop 0, 1, 2, 3)
+    int nl = 0;
+    uint16_t stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const int id = stack[--sp];
+        if (split[id] == 2) continue;
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L;
+        if (split[id] == 1) {
+            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+            continue;
+        }
+        lv[nl++] = (uint16_t)id;
+    }
+    // the final maps of the SB's leaves (partition context bytes of part_ctx_bytes,
+    // src/coder/superblock.rs:1062: NONE at 64 / 32 / 16 / 8, and the 8x8 split into 4x4)
+    const uint8_t pbyte[5] = {0x10, 0x18, 0x1c, 0x1e, 0x1f};
+    uint32_t pool = 0;
+    for (int i = 0; i < nl; ++i) {
+        const int id = lv[i];
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+        const int base = id_base(mode[id]);
+        for (int yy = y; yy < y + s; yy += 4)
+            for (int xx = x; xx < x + s; xx += 4) f.mode4[(size_t)(yy >> 2) * (f.W >> 2) + (xx >> 2)] = (uint8_t)base;
+        for (int yy = y; yy < y + s; yy += 8)
+            for (int xx = x; xx < x + s; xx += 8) f.part8[(size_t)(yy >> 3) * (f.W >> 3) + (xx >> 3)] = pbyte[L];
+        pool += (uint32_t)(s * s);
+    }
+    for (int i = 0; i < nl; ++i) {
+        const int id = lv[i];
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+        const mt_cand cd = make_cand(f, x, y, s, mode[id], x0, y0, true, 0);
+        mt_predblk p{};
+        p.x = cd.x;
+        p.y = cd.y;
+        p.bw = (uint8_t)s;
+        p.bh = (uint8_t)s;
+        p.mode = cd.mode;
+        p.angle_delta = cd.angle_delta;
+        p.flags = cd.pflags;
+        p.bd = (uint8_t)f.bd;
+        leaves[(size_t)local * kMaxLeaves + i] = p;
+    }
+    first[local] = (uint32_t)local * kMaxLeaves;
+    count[local] = (uint32_t)nl;
+    sb_leaves[t.sb] = (uint32_t)nl;
+    sb_leaf_pool[t.sb] = pool;
+}
+
+// The SB's leaves with the final contexts, at leaf_base[sb] / pool_base[sb] (one thread per SB).
+__global__ void __launch_bounds__(64) rdo_final_gen_kernel(FrameArgs f, int nsb, const uint32_t* __restrict__ leaf_base,
+                                                          const uint32_t* __restrict__ pool_base,
+                                                          mt_cand* __restrict__ cands) {
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nsb) return;
+    const int r = sb / f.C, c = sb % f.C, x0 = c * 64, y0 = r * 64;
+    const mt_rdo_node* nd = f.nodes + (size_t)sb * kNodes;
+    uint32_t k = leaf_base[sb], p = pool_base[sb];
+    uint16_t stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const int id = stack[--sp];
+        const int sp_ = nd[id].split;
+        if (sp_ == 2) continue;
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        if (sp_ == 1) {
+            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+            continue;
+        }
+        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+        cands[k++] = make_cand(f, x, y, s, nd[id].mode, x0, y0, true, p);
+        p += (uint32_t)(s * s);
+    }
+}
+
+// Per SB, in coding order: partition symbol bits of every coded node inside the frame and
+// the re-priced bits and SSE of every leaf (f64 sum of the f32 values; i64 SSE).
+__global__ void __launch_bounds__(64) rdo_final_sum_kernel(FrameArgs f, int nsb, const uint32_t* __restrict__ leaf_base,
+                                                          const mt_candres* __restrict__ res,
+                                                          double* __restrict__ sb_bits,
+                                                          long long* __restrict__ sb_sse) {
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nsb) return;
+    const int r = sb / f.C, c = sb % f.C, x0 = c * 64, y0 = r * 64;
+    const mt_rdo_node* nd = f.nodes + (size_t)sb * kNodes;
+    uint32_t k = leaf_base[sb];
+    double b = 0.0;
+    long long e = 0;
+    uint16_t stack[32];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const int id = stack[--sp];
+        const int sp_ = nd[id].split;
+        if (sp_ == 2) continue;
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+        if (s >= 8 && x + s <= f.W && y + s <= f.H)
+            b = __dadd_rn(b, (double)part_bits(f, x, y, L, sp_, x0, y0, true));
+        if (sp_ == 1) {
+            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+            continue;
+        }
+        b = __dadd_rn(b, (double)res[k].bits);
+        e += res[k].sse;
+        ++k;
+    }
+    sb_bits[sb] = b;
+    sb_sse[sb] = e;
+}
+
+// Library-owned buffers of one device, grown on demand.
+struct RdoWork {
+    size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map4 = 0, cap_map8 = 0;
+    uint32_t *counts = nullptr, *pools = nullptr, *sb_leaves = nullptr, *sb_leaf_pool = nullptr;
+    uint32_t *leaf_base = nullptr, *pool_base = nullptr;
+    double* sb_bits = nullptr;
+    long long* sb_sse = nullptr;
+    SbTask* tasks = nullptr;
+    unsigned long long* keys = nullptr;
+    uint8_t* nflags = nullptr;
+    mt_predblk* leaves = nullptr;
+    uint32_t *first = nullptr, *count = nullptr;
+    mt_cand* cands = nullptr;
+    uint32_t* refs = nullptr;
+    mt_candres* cres = nullptr;
+    mt_cand* fcands = nullptr;
+    mt_candres* fres = nullptr;
+    uint8_t *mode4 = nullptr, *part8 = nullptr;
+};
+std::mutex g_rdo_mu;
+RdoWork g_rdo[64];
+
+template <typename T> int regrow(T*& p, size_t n) {
+    if (p) MT_HIP_TRY(hipFree(p));
+    p = nullptr;
+    MT_HIP_TRY(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    return MT_OK;
+}
+
+}  // namespace
+
+namespace mtlib {
+int rdo_init_device(int /*device*/) {
+    SymBits b;
+    for (int a = 0; a < 5; ++a)
+        for (int l = 0; l < 5; ++l)
+            for (int m = 0; m < 13; ++m) b.kf[a][l][m] = cdf_cost_raw(MT_KF_Y_MODE_CDF[a][l], 13, m);
+    for (int m = 0; m < 8; ++m)
+        for (int j = 0; j < 7; ++j) b.angle[m][j] = cdf_cost_raw(MT_ANGLE_DELTA_CDF[m], 7, j);
+    const int bsz[5] = {0, 3, 6, 9, 12};  // av1_block_size_index (src/coder.rs:279) of 4x4 .. 64x64
+    for (int L = 0; L < 5; ++L)
+        for (int u = 0; u < 2; ++u) b.fuse[L][u] = cdf_cost_raw(&MT_FILTER_INTRA_CDF[bsz[L]], 2, u);
+    for (int k = 0; k < 5; ++k) b.fmode[k] = cdf_cost_raw(MT_FILTER_INTRA_MODE_CDF, 5, k);
+    for (int ctx = 0; ctx < 4; ++ctx) {
+        for (int bl = 1; bl <= 3; ++bl) {
+            b.part[bl - 1][ctx][0] = cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 0);
+            b.part[bl - 1][ctx][1] = cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 3);
+        }
+        b.part[3][ctx][0] = cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 0);
+        b.part[3][ctx][1] = cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 3);
+    }
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sym), &b, sizeof(b)));
+    return MT_OK;
+}
+}  // namespace mtlib
+
+extern "C" float mt_mode_lambda(int qidx, int bd, int sub) {
+    if (qidx < 0 || qidx > 255 || (bd != 8 && bd != 10 && bd != 12)) return -1.0f;
+    // mode_lambda_q (src/cost.rs:285) on the luma DC step of Quant::new (src/quant.rs:350)
+    mt_quant q;
+    if (mt_quant_new(qidx, bd, 15, &q) != MT_OK) return -1.0f;
+    const float dc_q = (float)q.dc;
+    const float dc_q8 = dc_q / (float)(1u << ((bd > 8 ? bd : 8) - 8));
+    const float mult = 3.3f + 0.0015f * dc_q8;  // def_kf_rd_multiplier (src/cost.rs:116)
+    const float lq = 0.009005174719460433f * 1.0f * dc_q * dc_q * mult;
+    // mode_lambda_weight (src/cost.rs:132) with the shipped tuning (src/tuning.rs:761-763,
+    // :913-915): per-format scale 0.8 / 0.65 / 0.6, top-band tilt floor 0, full 72, weight 3
+    const float scale = sub == 2 ? 0.8f : sub == 1 ? 0.65f : 0.6f;
+    int wi = ((255 - qidx) * 3) / 4;
+    wi = wi < 0 ? 0 : wi > 72 ? 72 : wi;
+    const float w = (float)(wi + 128) * (1.0f / 128.0f);  // aom_ssimulacra2_rdmult_weight (:125)
+    float tw;
+    if (sub == 1) {
+        tw = w * scale;
+    } else {
+        float r = ((float)qidx - 0.0f) / (72.0f - 0.0f);
+        r = r < 0.0f ? 0.0f : r > 1.0f ? 1.0f : r;
+        tw = (3.0f + (w - 3.0f) * r) * scale;
+    }
+    return lq * tw;
+}
+
+extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
+    const float m = mt_mode_lambda(qidx, bd, sub);
+    return m < 0.0f ? m : m / 2.775f;  // K_PART_LAMBDA (src/cost.rs:278)
+}
+
+extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream) {
+    if (!frame || !stats) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
+    const mt_rdo_frame& fr = *frame;
+    const int W = fr.src.width, H = fr.src.height;
+    if (!fr.src.data || !fr.recon || !fr.nodes || W <= 0 || H <= 0 || (W & 7) || (H & 7) ||
+        W > 65535 || H > 65535 || fr.src.stride < W || fr.recon_stride < W)
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad planes (sizes must be multiples of 8)");
+    if ((fr.bd != 8 && fr.bd != 10 && fr.bd != 12) || fr.qm_level > 15 || !(fr.lambda > 0.0f) ||
+        !std::isfinite(fr.lambda) || !(fr.wsse_scale >= 0.0f) || !std::isfinite(fr.wsse_scale) ||
+        (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad parameters");
+    int dev = 0, cus = 0;
+    int rc = mtlib::device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_rdo_mu);
+    RdoWork& w = g_rdo[dev];
+    hipStream_t s = (hipStream_t)stream;
+    const int C = (W + 63) / 64, R = (H + 63) / 64, nsb = C * R, nsteps = (C - 1) + 2 * (R - 1) + 1;
+    const size_t n4 = (size_t)(W / 4) * (H / 4), n8 = (size_t)(W / 8) * (H / 8);
+    if (w.cap_sb < (size_t)nsb) {
+        if ((rc = regrow(w.counts, nsb)) || (rc = regrow(w.pools, nsb)) || (rc = regrow(w.sb_leaves, nsb)) ||
+            (rc = regrow(w.sb_leaf_pool, nsb)) || (rc = regrow(w.leaf_base, nsb)) ||
+            (rc = regrow(w.pool_base, nsb)) || (rc = regrow(w.sb_bits, nsb)) || (rc = regrow(w.sb_sse, nsb)) ||
+            (rc = regrow(w.tasks, nsb)))
+            return rc;
+        w.cap_sb = nsb;
+    }
+    if (w.cap_map4 < n4) {
+        if ((rc = regrow(w.mode4, n4))) return rc;
+        w.cap_map4 = n4;
+    }
+    if (w.cap_map8 < n8) {
+        if ((rc = regrow(w.part8, n8))) return rc;
+        w.cap_map8 = n8;
+    }
+    FrameArgs f{};
+    f.src = fr.src;
+    f.src.vis_w = W;
+    f.src.vis_h = H;
+    f.prune = fr.prune;
+    f.nodes = (mt_rdo_node*)fr.nodes;
+    f.mode4 = w.mode4;
+    f.part8 = w.part8;
+    f.W = W;
+    f.H = H;
+    f.C = C;
+    f.bd = fr.bd;
+    f.qidx = fr.qidx;
+    f.qm = fr.qm_level;
+    f.flags = flags;
+    f.minsz = (flags & MT_RDO_MIN4) ? 4 : 8;
+    f.lambda = fr.lambda;
+    f.wsse = fr.wsse_scale;
+    PlaneSet ps{};
+    ps.n = 2;
+    ps.p[0] = f.src;
+    ps.p[1] = mt_plane{fr.recon, fr.recon_stride, W, H, W, H, 0};
+    const int bdi = fr.bd == 8 ? 0 : fr.bd == 10 ? 1 : 2;
+    const int bd_qm = fr.qm_level | (bdi << 4);
+    const int gfill = std::max(1, std::min(cus * 4, (int)((std::max(n4, (size_t)W * H) + 255) / 256)));
+
+    // the reconstruction starts as the source (the decision's stand-in inside each SB)
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        hipLaunchKernelGGL(rdo_copy_plane_kernel, dim3(gfill), dim3(256), 0, s, f.src, fr.recon, fr.recon_stride);
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(gfill), dim3(256), 0, s, w.mode4, n4);
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(gfill), dim3(256), 0, s, w.part8, n8);
+        hipLaunchKernelGGL(rdo_count_kernel, dim3(nsb), dim3(64), 0, s, f, w.counts, w.pools);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    std::vector<uint32_t> counts(nsb), pools(nsb);
+    MT_HIP_TRY(hipMemcpyAsync(counts.data(), w.counts, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(pools.data(), w.pools, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+
+    // steps: SB lists and candidate / pool bases
+    std::vector<SbTask> tasks;
+    tasks.reserve(nsb);
+    std::vector<int> step_first(nsteps + 1);
+    std::vector<uint32_t> step_n(nsteps), step_pool(nsteps);
+    size_t max_cand = 0, max_sb = 0;
+    uint64_t max_pool = 0;
+    int64_t total_cands = 0;
+    for (int k = 0; k < nsteps; ++k) {
+        step_first[k] = (int)tasks.size();
+        uint64_t nc = 0, np = 0;
+        for (int r = 0; r < R; ++r) {
+            const int c = k - 2 * r;
+            if (c < 0 || c >= C) continue;
+            const int sb = r * C + c;
+            tasks.push_back(SbTask{(uint32_t)sb, (uint32_t)nc, (uint32_t)np, 0});
+            nc += counts[sb];
+            np += pools[sb];
+        }
+        if (np > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: frame too large");
+        step_n[k] = (uint32_t)nc;
+        step_pool[k] = (uint32_t)np;
+        total_cands += (int64_t)nc;
+        max_cand = std::max(max_cand, (size_t)nc);
+        max_pool = std::max(max_pool, np);
+        max_sb = std::max(max_sb, tasks.size() - step_first[k]);
+    }
+    step_first[nsteps] = (int)tasks.size();
+    if (w.cap_step_sb < max_sb) {
+        if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.nflags, max_sb * kNodes)) ||
+            (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
+            (rc = regrow(w.count, max_sb)))
+            return rc;
+        w.cap_step_sb = max_sb;
+    }
+    if (w.cap_cand < max_cand) {
+        if ((rc = regrow(w.cands, max_cand)) || (rc = regrow(w.refs, max_cand)) || (rc = regrow(w.cres, max_cand)))
+            return rc;
+        w.cap_cand = max_cand;
+    }
+    MT_HIP_TRY(hipMemcpyAsync(w.tasks, tasks.data(), tasks.size() * sizeof(SbTask), hipMemcpyHostToDevice, s));
+
+    for (int k = 0; k < nsteps; ++k) {
+        const int nk = step_first[k + 1] - step_first[k];
+        if (nk == 0) continue;
+        const SbTask* tk = w.tasks + step_first[k];
+        const int n = (int)step_n[k];
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
+                               w.keys, (size_t)nk * kNodes, ~0ull);
+            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.cands, w.refs, w.nflags);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        if ((rc = mtlib::eval_launch(w.cands, n, ps, step_pool[k], w.cres, s))) return rc;
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            if (n > 0)
+                hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n, w.keys);
+            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.keys, w.nflags, w.leaves,
+                               w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        {
+            mtlib::ProfScope prof(mtlib::kProfRecon, s);
+            if ((rc = mtlib::sb_recon_launch(w.leaves, w.first, w.count, nk, f.src, fr.recon, fr.recon_stride,
+                                             fr.qidx, bd_qm, s)))
+                return rc;
+        }
+    }
+
+    // final accounting with the final contexts
+    std::vector<uint32_t> nleaf(nsb), lpool(nsb), lbase(nsb), pbase(nsb);
+    MT_HIP_TRY(hipMemcpyAsync(nleaf.data(), w.sb_leaves, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(lpool.data(), w.sb_leaf_pool, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+    uint64_t tl = 0, tp = 0;
+    for (int i = 0; i < nsb; ++i) {
+        lbase[i] = (uint32_t)tl;
+        pbase[i] = (uint32_t)tp;
+        tl += nleaf[i];
+        tp += lpool[i];
+    }
+    if (tp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: frame too large");
+    if (w.cap_final < tl) {
+        if ((rc = regrow(w.fcands, tl)) || (rc = regrow(w.fres, tl))) return rc;
+        w.cap_final = tl;
+    }
+    MT_HIP_TRY(hipMemcpyAsync(w.leaf_base, lbase.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    MT_HIP_TRY(hipMemcpyAsync(w.pool_base, pbase.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        hipLaunchKernelGGL(rdo_final_gen_kernel, dim3((nsb + 63) / 64), dim3(64), 0, s, f, nsb, w.leaf_base,
+                           w.pool_base, w.fcands);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    if ((rc = mtlib::eval_launch(w.fcands, (int)tl, ps, tp, w.fres, s))) return rc;
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        hipLaunchKernelGGL(rdo_final_sum_kernel, dim3((nsb + 63) / 64), dim3(64), 0, s, f, nsb, w.leaf_base, w.fres,
+                           w.sb_bits, w.sb_sse);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> bits(nsb);
+    std::vector<long long> sse(nsb);
+    MT_HIP_TRY(hipMemcpyAsync(bits.data(), w.sb_bits, nsb * sizeof(double), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(sse.data(), w.sb_sse, nsb * sizeof(long long), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+    std::memset(stats, 0, sizeof(*stats));
+    for (int i = 0; i < nsb; ++i) {
+        stats->bits += bits[i];
+        stats->sse += sse[i];
+    }
+    stats->candidates = total_cands;
+    stats->leaves = (int32_t)tl;
+    stats->sbs = nsb;
+    stats->steps = nsteps;
+    return MT_OK;
+}

@@ -1,0 +1,235 @@
+// Superblock reconstruction of the intra RDO (part of the mtgpu.hip translation unit: it
+// uses that unit's transform tables).
+//
+// One wave per 64x64 superblock walks the superblock's chosen leaves in coding order.  Per
+// leaf: the prediction (wave_predict, reading the reconstruction plane: the superblock's
+// earlier leaves and the finished superblocks around it), the residual against the source,
+// the forward DCT_DCT + quantization (+QM), dequantization and the inverse (the chain of
+// the reference's DctDispatch / IdctDispatch for square sizes, src/dct.rs:2624-2950,
+// src/idct.rs:168-728, :2677), and the reconstruction clamp(pred + residual) written back
+// to the plane.  Every stage keeps the batched kernels' arithmetic, so the reconstruction
+// equals the one the candidate evaluation measured for the same prediction.
+#pragma once
+#include "mt_pred_wave.h"
+
+namespace {
+
+// Square DCT_DCT chain of one N x N block (N <= 32) by one wave: lane i runs line i of
+// each pass.  res: the residual (LDS, row-major); out: the reconstructed residual (may
+// alias res); work: LDS of N * (N + 1) ints.
+template <int N>
+__device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p) {
+    using R = TxRules<N, N>;
+    constexpr int RS = N + 1;
+    const int lane = threadIdx.x & 63;
+    // forward column pass (length N)
+    if (lane < N) {
+        int32_t v[N];
+#pragma unroll
+        for (int y = 0; y < N; y++) v[y] = (int32_t)((uint32_t)res[y * N + lane] << R::pre_shift);
+        fwd_line<N>(false, v);
+#pragma unroll
+        for (int y = 0; y < N; y++) work[y * RS + lane] = v[y];
+    }
+    __syncthreads();
+    // forward row pass, post-scale, quant (+QM), dequant, in place
+    if (lane < N) {
+        const int fy = lane;
+        int32_t v[N];
+#pragma unroll
+        for (int x = 0; x < N; x++) v[x] = work[fy * RS + x];
+        fwd_line<N>(false, v);
+#pragma unroll
+        for (int fx = 0; fx < N; fx++) {
+            int32_t c = v[fx];
+            if (R::post == 2) c = mulq16(c, 32768);
+            else if (R::post == 4) c = (int32_t)(((int64_t)c * 16384 + (1ll << 21)) >> 22);
+            const int rc = fx * N + fy;
+            int32_t m = rc == 0 ? p.qdc : p.qac;
+            if (R::ratio) m = mulq16(m, R::ratio);
+            int32_t lvl = quant_q16(c, m);
+            int32_t step = rc == 0 ? p.dc : p.ac;
+            if (p.qm_level < 15) {
+                float tgt = q16_target(c, m);
+                const int iwt = g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc];
+                lvl = qm_apply(tgt, iwt);
+                step = (step * iwt + 16) >> 5;
+            }
+            work[fy * RS + fx] = dequant(lvl, step, R::dq_shift, p.cf_max);
+        }
+    }
+    __syncthreads();
+    // inverse row pass (length N) with the row clip, inter-pass shift and column clip
+    if (lane < N) {
+        const int y = lane;
+        int32_t v[N];
+#pragma unroll
+        for (int x = 0; x < N; x++) v[x] = work[y * RS + x];
+        inv_line<N>(KD, v, Clip{p.rmin, p.rmax});
+        const Clip cc{p.cmin, p.cmax};
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            int32_t t = v[x];
+            if (R::mid) t = (t + (1 << (R::mid - 1))) >> R::mid;
+            work[y * RS + x] = cc(t);
+        }
+    }
+    __syncthreads();
+    // inverse column pass (length N), (t + 8) >> 4
+    if (lane < N) {
+        const int x = lane;
+        int32_t v[N];
+#pragma unroll
+        for (int y = 0; y < N; y++) v[y] = work[y * RS + x];
+        inv_line<N>(KD, v, Clip{p.cmin, p.cmax});
+#pragma unroll
+        for (int y = 0; y < N; y++) out[y * N + x] = (v[y] + 8) >> 4;
+    }
+    __syncthreads();
+}
+
+// TX_64X64 chain of one block by one wave: the f64 separable forward of dct64x64_coeffs
+// (src/dct.rs:4155) with every product and sum rounded on its own in the reference's order,
+// the 32x32-QM quantizer, and idct_dequant_64x64 (src/idct.rs:2677); the arithmetic of the
+// batched 64x64 kernel.  vt: LDS of 16 * 64 doubles; coef: LDS of 32 * 33 ints.
+__device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32_t* coef,
+                             const BlkParam& p) {
+    const int lane = threadIdx.x & 63;
+    int32_t* rowbuf = reinterpret_cast<int32_t*>(vt);  // 32 x 64 int32 after the forward
+    const bool qm = p.qm_level < 15;
+    const uint8_t* qmrow = &g_iqm[qm ? p.qm_level : 0][p.qm_chroma][336];
+    double acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) acc[j] = 0.0;
+#pragma unroll 2
+    for (int y = 0; y < 64; y++) {
+        const double c = (double)res[y * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < 32; j++) acc[j] = __dadd_rn(acc[j], __dmul_rn(c_dct64_basis_t[y * 32 + j], c));
+    }
+#pragma unroll
+    for (int hf = 0; hf < 2; hf++) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) vt[j * 64 + lane] = acc[hf * 16 + j];
+        __syncthreads();
+        {
+            const int u = lane & 31, vp = lane >> 5;
+            double hacc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) hacc[k] = -0.0;
+#pragma unroll 2
+            for (int x = 0; x < 64; x++) {
+                const double bu = c_dct64_basis_t[x * 32 + u];
+#pragma unroll
+                for (int k = 0; k < 8; k++) hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                coef[(hf * 16 + 2 * k + vp) * 33 + u] = (int32_t)round(__dmul_rn(hacc[k], 8.0));
+        }
+        __syncthreads();
+    }
+    {
+        const int u = lane & 31, vp = lane >> 5;
+#pragma unroll 1
+        for (int k = 0; k < 16; k++) {
+            const int v = 2 * k + vp;
+            const int32_t coeff = coef[v * 33 + u];
+            const int rc = u * 32 + v;
+            const int32_t m = rc == 0 ? p.qdc : p.qac;
+            int32_t lvl = quant_q16(coeff, m);
+            float tgt = q16_target(coeff, m);
+            const int iwt = qm ? qmrow[rc] : 32;
+            if (qm) lvl = qm_apply(tgt, iwt);
+            int32_t step = rc == 0 ? p.dc : p.ac;
+            if (qm) step = (step * iwt + 16) >> 5;
+            coef[v * 33 + u] = dequant(lvl, step, 2, p.cf_max);
+        }
+    }
+    __syncthreads();
+    if (lane < 32) {
+        const int y = lane;
+        int32_t v[64];
+#pragma unroll
+        for (int x = 0; x < 64; x++) v[x] = x < 32 ? coef[y * 33 + x] : 0;
+        idct64<1>(v, Clip{p.rmin, p.rmax});
+        const Clip cc{p.cmin, p.cmax};
+#pragma unroll
+        for (int x = 0; x < 64; x++) rowbuf[y * 64 + x] = cc((v[x] + 2) >> 2);
+    }
+    __syncthreads();
+    {
+        const int x = lane;
+        int32_t v[64];
+#pragma unroll
+        for (int y = 0; y < 64; y++) v[y] = y < 32 ? rowbuf[y * 64 + x] : 0;
+        idct64<1>(v, Clip{p.cmin, p.cmax});
+#pragma unroll
+        for (int y = 0; y < 64; y++) out[y * 64 + x] = (v[y] + 8) >> 4;
+    }
+    __syncthreads();
+}
+
+// One workgroup (one wave) per superblock: leaves[first[b] .. first[b] + count[b]) in
+// coding order.  Each leaf descriptor carries its prediction (x, y, size, mode, angle
+// delta, availability and filter-type flags, bit depth); q holds the quantizer (qidx,
+// bd_qm) shared by the frame.  rec: the reconstruction plane (frame size of src).
+__global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restrict__ leaves,
+                                                      const uint32_t* __restrict__ first,
+                                                      const uint32_t* __restrict__ count,
+                                                      mt_plane src, uint16_t* __restrict__ rec,
+                                                      int rstride, mt_txblk q) {
+    __shared__ int32_t s_pred[4096];
+    __shared__ int32_t s_res[4096];
+    __shared__ int32_t s_a[kEdge], s_l[kEdge];
+    __shared__ int32_t s_fi[33 * 33];
+    __shared__ double s_vt[16 * 64];
+    __shared__ int32_t s_work[32 * 33];
+    const int lane = threadIdx.x;
+    const uint32_t f0 = first[blockIdx.x], n = count[blockIdx.x];
+    BlkParam prm;
+    param_from_desc(q, prm, nullptr, 0);
+    const int maxv = (1 << prm.bd) - 1;
+    for (uint32_t i = 0; i < n; i++) {
+        const mt_predblk d = leaves[f0 + i];
+        const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
+        const int x0 = d.x, y0 = d.y;
+        wave_predict(d, src.width, src.height,
+                     [=](int x, int y) { return (int)rec[(size_t)y * rstride + x]; }, s_pred, s_a, s_l,
+                     s_fi);
+        __syncthreads();
+        for (int p = lane; p < npx; p += 64)
+            s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
+        __syncthreads();
+        switch (s) {
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm); break;
+            case 32: wave_chain_sq<32>(s_res, s_res, s_work, prm); break;
+            default: wave_chain64(s_res, s_res, s_vt, s_work, prm); break;
+        }
+        for (int p = lane; p < npx; p += 64) {
+            const int v = s_pred[p] + s_res[p];
+            rec[(size_t)(y0 + (p >> lw)) * rstride + x0 + (p & (s - 1))] =
+                (uint16_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+        }
+        __syncthreads();  // the next leaf predicts from these samples
+    }
+}
+
+}  // namespace
+
+namespace mtlib {
+int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
+                    int nsb, const mt_plane& src, uint16_t* rec, int rstride, int qidx, int bd_qm,
+                    hipStream_t s) {
+    if (nsb <= 0) return MT_OK;
+    mt_txblk q{};
+    q.qidx = (uint8_t)qidx;
+    q.bd_qm = (uint8_t)bd_qm;
+    hipLaunchKernelGGL(sb_recon_kernel, dim3(nsb), dim3(64), 0, s, leaves_d, first_d, count_d, src,
+                       rec, rstride, q);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+}  // namespace mtlib

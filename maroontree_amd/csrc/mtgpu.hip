@@ -864,6 +864,9 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------- RDO reconstruction
+#include "mt_sbrecon.h"
+
 // ------------------------------------------------------------------------- bucketing
 // Counting sort of the block list by (size slot, type slot): 13 x 7 classes.  Blocks of an
 // unsupported class are answered directly (sse = -1).
@@ -1102,6 +1105,7 @@ extern "C" int mt_init(int device) {
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst8), MT_FADST8_Q12, sizeof(MT_FADST8_Q12)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst16), MT_FADST16_Q12, sizeof(MT_FADST16_Q12)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_iqm), MT_IQM, sizeof(MT_IQM)));
+    HIP_TRY(upload_pred_tables());  // this unit's copy, for the RDO reconstruction
     // dct64_basis (src/dct.rs:4132): built on the host with the C library's cos, as the
     // reference builds it, so every f64 weight is the same.
     std::vector<double> basis(32 * 64);
@@ -1124,6 +1128,8 @@ extern "C" int mt_init(int device) {
         rc = mtlib::pred_init_device(device);
         if (rc) return rc;
         rc = mtlib::dist_init_device(device);
+        if (rc) return rc;
+        rc = mtlib::rdo_init_device(device);
         if (rc) return rc;
     }
     HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));

@@ -33,7 +33,8 @@ OUT_H2 = os.path.join(HERE, "..", "maroontree_amd", "csrc", "av1_cdf_tables.h")
 # Default coefficient-coding CDFs (AV1 spec defaults, per qctx) in the raw (non-inverted) form
 # the reference stores (src/coef_q.rs), the AV1 scans (src/coef_q.rs, src/tables.rs:348-455)
 # and the coeff_base context offsets (src/tables.rs:354, :456, :466), plus the intra mode /
-# tx-type CDFs of the mode rate (src/tables.rs).
+# tx-type CDFs of the mode rate, and the partition CDFs of the decision snapshot Cdfs::new
+# (src/tables.rs:377, :403; src/coder/cdfs.rs:544-555).
 CDF_TABLES_COEF_Q = [
     "SKIP_TX4", "SKIP_TX8", "SKIP_TX16", "SKIP_TX32", "SKIP_TX64",
     "BASE_TOK_TX4_LUMA_Q", "BASE_TOK_TX4_CHROMA_Q", "BASE_TOK_TX8_LUMA_Q", "BASE_TOK_TX8_CHROMA_Q",
@@ -63,7 +64,7 @@ CDF_TABLES_TABLES = [
     "LO_CTX_OFF", "LO_CTX_OFF_WLH", "LO_CTX_OFF_WGH",
     "KF_Y_MODE_CDF", "UV_MODE_NOCFL_CDF", "UV_MODE_CFL_CDF", "TXTP_INTRA1_TX8", "TXTP_INTRA2_TX16",
     "TXTP_INTRA1_TX4", "ANGLE_DELTA_CDF", "INTRA_FILTER_TAPS", "FILTER_INTRA_CDF",
-    "FILTER_INTRA_MODE_CDF",
+    "FILTER_INTRA_MODE_CDF", "PART_SPLIT_CDF", "PART_BL8_CDF",
 ]
 
 

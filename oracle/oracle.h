@@ -197,6 +197,41 @@ typedef struct {
 void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, int nplanes, orc_candres* r);
 void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,
                     orc_candres* r, int nthreads);
+/* orc_eval_candidate that also returns the reconstruction clamp(pred + inverse), w*h */
+void orc_eval_candidate_recon(const orc_cand* c, const orc_plane* planes, int nplanes,
+                              orc_candres* r, int32_t* recon);
+
+/* ---- whole-frame luma intra RDO over the superblock wavefront ---------------------- */
+/* The library's mt_intra_rdo procedure (see include/mtgpu.h) on the CPU, candidate by
+ * candidate through orc_eval_candidate. */
+typedef struct {
+    int32_t width, height;  /* multiples of 8 */
+    int32_t bd, qidx, qm_level, flags;
+    float lambda, wsse_scale;
+} orc_rdo_params;
+typedef struct {
+    float cost, none_cost;
+    uint8_t split, mode, pad0, pad1;
+} orc_rdo_node;
+typedef struct {
+    double bits;
+    int64_t sse, candidates;
+    int32_t leaves, sbs;
+} orc_rdo_stats;
+/* src: width x height samples, src_stride per row; prune: NULL or [nsb][341] node masks
+ * (nsb in SB raster order); recon: width x height out (stride width); nodes: [nsb][341]
+ * out; modes (optional): (width/4) x (height/4) out, the final luma mode map (filter intra
+ * as DC_PRED).  Returns 0, or -1 for invalid parameters. */
+int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_params* p,
+                  const uint16_t* prune, uint16_t* recon, orc_rdo_node* nodes,
+                  uint8_t* modes, orc_rdo_stats* stats, int nthreads);
+/* the decision lambda of the reference (src/coder/lossy_state.rs:1769-1790, src/cost.rs:132,
+ * :285): mode_lambda_q(dc_q, bd) * mode_lambda_weight(qidx, sub) / K_PART_LAMBDA */
+float orc_rdo_lambda(int qidx, int bd, int sub);
+float orc_mode_lambda(int qidx, int bd, int sub);
+/* the RDO's top-right / bottom-left availability of an s x s block at (xin, yin) of its SB */
+int orc_rdo_have_tr(int xin, int yin, int s);
+int orc_rdo_have_bl(int xin, int yin, int s);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,9 @@ namespace {
 
 int tx_dim_log2(int v) { return __builtin_ctz((unsigned)v); }
 
-void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candres& r) {
+// recon (optional): clamp(pred + inverse residual), w*h row-major.
+void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candres& r,
+              int32_t* recon = nullptr) {
     std::memset(&r, 0, sizeof(r));
     r.sse = -1;
     r.dist = -1;
@@ -65,6 +67,8 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
     const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);
     r.sse = orc_sse_recon_vis(pred.data(), inv.data(), src, sp.stride, w, vw, vh, bd);
+    if (recon)
+        for (int i = 0; i < w * h; ++i) recon[i] = std::min(std::max(pred[i] + inv[i], 0), (1 << bd) - 1);
     if (c.wsse_scale > 0.0f) {
         const int maxv = (1 << bd) - 1;
         std::vector<int32_t> err((size_t)std::max(vw * vh, 1));
@@ -109,6 +113,11 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
 extern "C" void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, int nplanes,
                                    orc_candres* r) {
     eval_one(*c, planes, nplanes, *r);
+}
+
+extern "C" void orc_eval_candidate_recon(const orc_cand* c, const orc_plane* planes, int nplanes,
+                                         orc_candres* r, int32_t* recon) {
+    eval_one(*c, planes, nplanes, *r, recon);
 }
 
 extern "C" void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,

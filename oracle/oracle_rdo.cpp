@@ -1,0 +1,448 @@
+// CPU oracle: the whole-frame luma intra RDO of the library (mt_intra_rdo), restated on the
+// CPU with the oracle's candidate evaluation.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  The decision procedure is the library's own
+// (the reference's ~35k-line decision heuristics are not part of the path); every quantity
+// it combines is a reference kernel already restated here: the candidate loop body
+// (src/coder/partition_search.rs:471-520, orc_eval_candidate), the mode syntax bits
+// (luma_intra_syntax_bits, src/coder/superblock.rs:192; mode_bits,
+// src/coder/lossy_state.rs:1290), the partition symbol bits (partition_bits_bl,
+// src/coder/lossy_state.rs:1718 with get_partition_ctx, src/coeffs.rs:2917, and the context
+// bytes of part_ctx_bytes, src/coder/superblock.rs:1062), rd_cost
+// (src/partition_rd.rs:140) and the decision lambda (src/coder/lossy_state.rs:1769-1790).
+//
+// Procedure, per 64x64 superblock (SB) in wavefront order (SB (r, c) in step c + 2r):
+//  1. every quadtree node (64 .. 8, or .. 4) inside the frame evaluates every allowed mode
+//     (13 intra modes, the 6 angle deltas of the 8 directional modes at >= 8x8, the 5
+//     filter-intra modes at <= 32x32) as one DCT_DCT transform block, predicting from the
+//     reconstruction plane; samples of the SB itself still hold the source there (the
+//     decision sees the source inside its own SB and the final reconstruction outside);
+//  2. bottom-up, each node keeps PARTITION_NONE (its best mode) or SPLIT (the sum of its
+//     children), each priced with its partition symbol;
+//  3. the chosen leaves are reconstructed in coding (Z) order from the reconstruction.
+// Decision-time contexts read the final mode / partition maps outside the SB and count
+// neighbours inside the SB as DC_PRED / unsplit.  The final accounting re-evaluates every
+// leaf with the final contexts.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+#include "../maroontree_amd/csrc/av1_cdf_tables.h"
+
+namespace {
+
+constexpr int kNodes = 341;
+constexpr int kLevelBase[5] = {0, 1, 5, 21, 85};
+enum { kAngle = 1, kFilter = 2, kMin4 = 4 };
+enum { kForceSplit = 0x4000, kNoSplit = 0x8000 };
+
+struct Geo {
+    int L, ix, iy, s;
+};
+Geo node_geo(int id) {
+    const int L = id >= 85 ? 4 : id >= 21 ? 3 : id >= 5 ? 2 : id >= 1 ? 1 : 0;
+    const int k = id - kLevelBase[L], n = 1 << L;
+    return {L, k % n, k / n, 64 >> L};
+}
+int node_id(int L, int ix, int iy) { return kLevelBase[L] + iy * (1 << L) + ix; }
+
+// Coding order of the s-aligned cell (ix, iy) inside a superblock: bit-interleaved, y high.
+int zidx(int ix, int iy) {
+    int z = 0;
+    for (int b = 0; b < 4; ++b) z |= (((ix >> b) & 1) << (2 * b)) | (((iy >> b) & 1) << (2 * b + 1));
+    return z;
+}
+// Whether the s x s block at (xin, yin) inside its SB has its top-right / bottom-left
+// neighbour of the same size coded before it (the frame edge clips the count separately).
+bool have_tr(int xin, int yin, int s) {
+    if (yin == 0) return true;
+    if (xin + s >= 64) return false;
+    return zidx(xin / s + 1, yin / s - 1) < zidx(xin / s, yin / s);
+}
+bool have_bl(int xin, int yin, int s) {
+    if (yin + s >= 64) return false;
+    if (xin == 0) return true;
+    return zidx(xin / s - 1, yin / s + 1) < zidx(xin / s, yin / s);
+}
+
+// Mode ids 0..65: 0..12 the intra modes; 13 + 6 (m - 1) + j the directional mode m with
+// delta {-3, -2, -1, 1, 2, 3}[j]; 61..65 filter intra 0..4.
+constexpr int kModeIds = 66;
+int id_mode(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 16 + id - 61; }
+int id_delta(int id) {
+    static const int d[6] = {-3, -2, -1, 1, 2, 3};
+    return id >= 13 && id <= 60 ? d[(id - 13) % 6] : 0;
+}
+int id_base(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 0; }
+int id_bit(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 13; }
+bool id_valid(int id, int s, int flags) {
+    if (id <= 12) return true;
+    if (id <= 60) return (flags & kAngle) && s >= 8;
+    return (flags & kFilter) && s <= 32;
+}
+
+// Symbol costs of the frame-initial decision CDFs (Cdfs::new, src/coder/cdfs.rs:400).
+struct SymBits {
+    float kf[5][5][13];
+    float angle[8][7];
+    float fuse[5][2];  // use_filter_intra at 4x4, 8x8, 16x16, 32x32
+    float fmode[5];
+    float part[4][4][2];  // [bl - 1][ctx][NONE, SPLIT]
+};
+const SymBits& sym_bits() {
+    static SymBits t = [] {
+        SymBits b;
+        for (int a = 0; a < 5; ++a)
+            for (int l = 0; l < 5; ++l)
+                for (int m = 0; m < 13; ++m) b.kf[a][l][m] = orc_cdf_cost_raw(MT_KF_Y_MODE_CDF[a][l], 13, m);
+        for (int m = 0; m < 8; ++m)
+            for (int j = 0; j < 7; ++j) b.angle[m][j] = orc_cdf_cost_raw(MT_ANGLE_DELTA_CDF[m], 7, j);
+        static const int bsz[5] = {0, 3, 6, 9, 12};  // av1_block_size_index of 4x4 .. 64x64
+        for (int L = 0; L < 5; ++L)
+            for (int u = 0; u < 2; ++u) b.fuse[L][u] = orc_cdf_cost_raw(&MT_FILTER_INTRA_CDF[bsz[L]], 2, u);
+        for (int k = 0; k < 5; ++k) b.fmode[k] = orc_cdf_cost_raw(MT_FILTER_INTRA_MODE_CDF, 5, k);
+        for (int ctx = 0; ctx < 4; ++ctx) {
+            for (int bl = 1; bl <= 3; ++bl) {
+                b.part[bl - 1][ctx][0] = orc_cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 0);
+                b.part[bl - 1][ctx][1] = orc_cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 3);
+            }
+            b.part[3][ctx][0] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 0);
+            b.part[3][ctx][1] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 3);
+        }
+        return b;
+    }();
+    return t;
+}
+
+const int kIntraModeCtx[13] = {0, 1, 2, 3, 4, 4, 4, 4, 3, 0, 1, 2, 0};  // src/intrapred.rs:1088
+const int kFilterDir[5] = {0, 1, 2, 6, 0};  // Filter_Intra_Mode_To_Intra_Dir
+const uint8_t kPartByte[5] = {0x10, 0x18, 0x1c, 0x1e, 0x1f};  // leaf of 64 .. 8; 4x4 split
+
+int sq_tx(int s) { return s == 4 ? 0 : s == 8 ? 1 : s == 16 ? 2 : s == 32 ? 3 : 4; }
+int bdi_of(int bd) { return bd == 8 ? 0 : bd == 10 ? 1 : 2; }
+int qctx_of(int q) { return q <= 20 ? 0 : q <= 60 ? 1 : q <= 120 ? 2 : 3; }
+
+// luma_intra_syntax_bits (src/coder/superblock.rs:192) of mode id `id` on an s x s block.
+float mode_extra_bits(int actx, int lctx, int id, int s, int flags) {
+    const SymBits& T = sym_bits();
+    const int base = id_base(id);
+    const int L = s == 64 ? 0 : s == 32 ? 1 : s == 16 ? 2 : s == 8 ? 3 : 4;
+    float bits = T.kf[actx][lctx][base];
+    if (base >= 1 && base <= 8 && s >= 8) {
+        bits = bits + T.angle[base - 1][id_delta(id) + 3];
+    } else if ((flags & kFilter) && base == 0 && s <= 32) {
+        const int use = id >= 61;
+        bits = bits + T.fuse[4 - L][use];
+        if (use) bits = bits + T.fmode[id - 61];
+    }
+    return bits;
+}
+
+struct Frame {
+    const uint16_t* src;
+    int sstride, W, H, bd, qidx, qm, flags;
+    float lambda, wsse;
+    std::vector<uint16_t> rec;     // W x H
+    std::vector<uint8_t> mode4;    // (W/4) x (H/4) luma modes
+    std::vector<uint8_t> part8;    // (W/8) x (H/8) partition context bytes
+    orc_plane planes[2];
+    int minsz;
+
+    int mode_at(int x, int y) const { return mode4[(size_t)(y >> 2) * (W >> 2) + (x >> 2)]; }
+    int part_at(int x, int y) const { return part8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
+
+    orc_cand cand(int x, int y, int s, int id, int sb_x0, int sb_y0, bool final_ctx) const {
+        orc_cand c;
+        std::memset(&c, 0, sizeof(c));
+        c.x = (uint16_t)x;
+        c.y = (uint16_t)y;
+        c.tx_size = (uint8_t)sq_tx(s);
+        c.tx_type = 0;
+        c.mode = (uint8_t)id_mode(id);
+        c.angle_delta = (int8_t)id_delta(id);
+        // neighbours: final maps, except inside the SB at decision time (DC, not smooth)
+        const bool above_ok = y > 0 && (final_ctx || y == sb_y0);
+        const bool left_ok = x > 0 && (final_ctx || x == sb_x0);
+        const int am = above_ok ? mode_at(x, y - 1) : 0, lm = left_ok ? mode_at(x - 1, y) : 0;
+        const bool smooth = (am >= 9 && am <= 11) || (lm >= 9 && lm <= 11);
+        c.pflags = (uint8_t)((have_tr(x - sb_x0, y - sb_y0, s) ? 1 : 0) |
+                             (have_bl(x - sb_x0, y - sb_y0, s) ? 2 : 0) | (smooth ? 4 : 0));
+        c.planes = 0 | (1 << 2);
+        c.qidx = (uint8_t)qidx;
+        c.bd_qm = (uint8_t)(qm | (bdi_of(bd) << 4));
+        c.rplane = 0;
+        c.eob_bin = (uint8_t)(s == 4 ? 0 : s == 8 ? 5 : s == 16 ? 11 : 15);
+        c.txtp_set = (uint8_t)(s == 4 ? 1 : s == 8 ? 2 : s == 16 ? 3 : 0);
+        c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
+        c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT in the intra sets
+        c.qctx = (uint8_t)qctx_of(qidx);
+        c.wsse_scale = wsse;
+        c.lambda = lambda;
+        c.extra_bits = mode_extra_bits(kIntraModeCtx[am], kIntraModeCtx[lm], id, s, flags);
+        return c;
+    }
+
+    float part_bits(int x, int y, int L, int sym, int sb_x0, int sb_y0, bool final_ctx) const {
+        const int bl = L + 1, sh = 4 - bl;
+        const int a = (y > 0 && (final_ctx || y == sb_y0)) ? part_at(x, y - 1) : 0;
+        const int l = (x > 0 && (final_ctx || x == sb_x0)) ? part_at(x - 1, y) : 0;
+        const int ctx = ((a >> sh) & 1) + (((l >> sh) & 1) << 1);
+        return sym_bits().part[bl - 1][ctx][sym];
+    }
+};
+
+struct SbOut {
+    int64_t candidates = 0;
+    std::vector<int> leaves;  // node ids in coding order
+};
+
+// Decide and reconstruct one superblock (its neighbours of earlier steps are final).
+void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, SbOut& out) {
+    const int x0 = c * 64, y0 = r * 64;
+    bool searched[kNodes], inside[kNodes], none_ok[kNodes], may_split[kNodes];
+    float best[kNodes];
+    int best_id[kNodes];
+    for (int id = 0; id < kNodes; ++id) {
+        const Geo g = node_geo(id);
+        const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+        const uint16_t m = mask ? mask[id] : 0x3FFF;
+        bool sr;
+        if (id == 0) {
+            sr = true;
+        } else {
+            const int pid = node_id(g.L - 1, g.ix >> 1, g.iy >> 1);
+            sr = may_split[pid] && x < f.W && y < f.H;
+        }
+        searched[id] = sr;
+        inside[id] = x + g.s <= f.W && y + g.s <= f.H;
+        none_ok[id] = sr && inside[id] && (g.s == f.minsz || !(m & kForceSplit) || (m & kNoSplit));
+        may_split[id] = sr && g.s > f.minsz && (!inside[id] || !(m & kNoSplit));
+        best[id] = __builtin_inff();
+        best_id[id] = 0;
+        if (!none_ok[id]) continue;
+        for (int mid = 0; mid < kModeIds; ++mid) {
+            if (!id_valid(mid, g.s, f.flags)) continue;
+            if (mid != 0 && !((m >> id_bit(mid)) & 1)) continue;
+            const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, false);
+            orc_candres res;
+            orc_eval_candidate(&cd, f.planes, 2, &res);
+            ++out.candidates;
+            if (res.cost < best[id]) {
+                best[id] = res.cost;
+                best_id[id] = mid;
+            }
+        }
+    }
+    // bottom-up decision
+    float cost[kNodes];
+    uint8_t split[kNodes];
+    for (int id = kNodes - 1; id >= 0; --id) {
+        const Geo g = node_geo(id);
+        const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+        orc_rdo_node& nd = nodes[id];
+        std::memset(&nd, 0, sizeof(nd));
+        nd.mode = (uint8_t)best_id[id];
+        if (!searched[id]) {
+            cost[id] = 0.0f;
+            split[id] = 2;
+            nd.cost = __builtin_inff();
+            nd.none_cost = __builtin_inff();
+            nd.split = 2;
+            continue;
+        }
+        float none = __builtin_inff(), spl = __builtin_inff();
+        if (none_ok[id] && best[id] < __builtin_inff()) {
+            none = best[id];
+            if (g.s >= 8) none = none + f.lambda * f.part_bits(x, y, g.L, 0, x0, y0, false);
+        }
+        if (may_split[id]) {
+            float sum = 0.0f;
+            for (int k = 0; k < 4; ++k) sum = sum + cost[node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1))];
+            spl = inside[id] ? sum + f.lambda * f.part_bits(x, y, g.L, 1, x0, y0, false) : sum;
+        }
+        if (none <= spl) {
+            cost[id] = none;
+            split[id] = 0;
+        } else {
+            cost[id] = spl;
+            split[id] = 1;
+        }
+        nd.cost = cost[id];
+        nd.none_cost = none;
+        nd.split = split[id];
+    }
+    // leaves in coding order; maps; reconstruction
+    int stack[64], sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const int id = stack[--sp];
+        if (split[id] == 2) continue;
+        const Geo g = node_geo(id);
+        if (split[id] == 1) {
+            for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
+            continue;
+        }
+        out.leaves.push_back(id);
+        const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+        const int mid = best_id[id];
+        const int base = id_base(mid);
+        for (int yy = y; yy < y + g.s; yy += 4)
+            for (int xx = x; xx < x + g.s; xx += 4) f.mode4[(size_t)(yy >> 2) * (f.W >> 2) + (xx >> 2)] = (uint8_t)base;
+        for (int yy = y; yy < y + g.s; yy += 8)
+            for (int xx = x; xx < x + g.s; xx += 8) f.part8[(size_t)(yy >> 3) * (f.W >> 3) + (xx >> 3)] = kPartByte[g.L];
+        const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, true);
+        orc_candres res;
+        std::vector<int32_t> rc((size_t)g.s * g.s);
+        orc_eval_candidate_recon(&cd, f.planes, 2, &res, rc.data());
+        for (int yy = 0; yy < g.s; ++yy)
+            for (int xx = 0; xx < g.s; ++xx) f.rec[(size_t)(y + yy) * f.W + x + xx] = (uint16_t)rc[yy * g.s + xx];
+    }
+}
+
+}  // namespace
+
+extern "C" int orc_rdo_have_tr(int xin, int yin, int s) { return have_tr(xin, yin, s) ? 1 : 0; }
+extern "C" int orc_rdo_have_bl(int xin, int yin, int s) { return have_bl(xin, yin, s) ? 1 : 0; }
+
+extern "C" float orc_mode_lambda(int qidx, int bd, int sub) {
+    // mode_lambda_q (src/cost.rs:285) on the luma DC step of Quant::new (src/quant.rs:350)
+    orc_quant q;
+    orc_quant_new(qidx, bd, 15, &q);
+    const float dc_q = (float)q.dc;
+    const float dc_q8 = dc_q / (float)(1u << ((bd > 8 ? bd : 8) - 8));
+    const float mult = 3.3f + 0.0015f * dc_q8;  // def_kf_rd_multiplier (src/cost.rs:116)
+    const float lq = 0.009005174719460433f * 1.0f * dc_q * dc_q * mult;
+    // mode_lambda_weight (src/cost.rs:132) with the shipped tuning (src/tuning.rs:761-763,
+    // :913-915): scale 0.8 / 0.65 / 0.6, top-band tilt floor 0, full 72, weight 3
+    const float scale = sub == 2 ? 0.8f : sub == 1 ? 0.65f : 0.6f;
+    int wi = ((255 - qidx) * 3) / 4;
+    wi = wi < 0 ? 0 : wi > 72 ? 72 : wi;
+    const float w = (float)(wi + 128) * (1.0f / 128.0f);  // aom_ssimulacra2_rdmult_weight
+    float tw;
+    if (sub == 1) {
+        tw = w * scale;
+    } else {
+        float r = ((float)qidx - 0.0f) / (72.0f - 0.0f);
+        r = r < 0.0f ? 0.0f : r > 1.0f ? 1.0f : r;
+        tw = (3.0f + (w - 3.0f) * r) * scale;
+    }
+    return lq * tw;
+}
+
+extern "C" float orc_rdo_lambda(int qidx, int bd, int sub) {
+    return orc_mode_lambda(qidx, bd, sub) / 2.775f;  // lambda_d = mlam / K_PART_LAMBDA
+}
+
+extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_params* p,
+                             const uint16_t* prune, uint16_t* recon, orc_rdo_node* nodes,
+                             uint8_t* modes, orc_rdo_stats* stats, int nthreads) {
+    if (!p || !src || !recon || !nodes || !stats) return -1;
+    if (p->width <= 0 || p->height <= 0 || (p->width & 7) || (p->height & 7) || p->width > 65535 ||
+        p->height > 65535 || src_stride < p->width)
+        return -1;
+    if (p->bd != 8 && p->bd != 10 && p->bd != 12) return -1;
+    if (p->qidx < 0 || p->qidx > 255 || p->qm_level < 0 || p->qm_level > 15) return -1;
+    Frame f;
+    f.src = src;
+    f.sstride = src_stride;
+    f.W = p->width;
+    f.H = p->height;
+    f.bd = p->bd;
+    f.qidx = p->qidx;
+    f.qm = p->qm_level;
+    f.flags = p->flags;
+    f.lambda = p->lambda;
+    f.wsse = p->wsse_scale;
+    f.minsz = (p->flags & kMin4) ? 4 : 8;
+    f.rec.resize((size_t)f.W * f.H);
+    for (int y = 0; y < f.H; ++y) std::memcpy(&f.rec[(size_t)y * f.W], src + (size_t)y * src_stride, f.W * 2);
+    f.mode4.assign((size_t)(f.W / 4) * (f.H / 4), 0);
+    f.part8.assign((size_t)(f.W / 8) * (f.H / 8), 0);
+    f.planes[0] = {src, src_stride, f.W, f.H, f.W, f.H, 0};
+    f.planes[1] = {f.rec.data(), f.W, f.W, f.H, f.W, f.H, 0};
+    const int C = (f.W + 63) / 64, R = (f.H + 63) / 64, nsb = C * R;
+    std::vector<SbOut> outs(nsb);
+    const int nsteps = (C - 1) + 2 * (R - 1) + 1;
+    for (int k = 0; k < nsteps; ++k) {
+        std::vector<int> sbs;
+        for (int r = 0; r < R; ++r) {
+            const int c = k - 2 * r;
+            if (c >= 0 && c < C) sbs.push_back(r * C + c);
+        }
+        auto run = [&](int i) {
+            const int sb = sbs[i];
+            do_sb(f, sb / C, sb % C, prune ? prune + (size_t)sb * kNodes : nullptr,
+                  nodes + (size_t)sb * kNodes, outs[sb]);
+        };
+        if (nthreads <= 1 || sbs.size() == 1) {
+            for (size_t i = 0; i < sbs.size(); ++i) run((int)i);
+        } else {
+            std::vector<std::thread> th;
+            const int nt = std::min<int>(nthreads, (int)sbs.size());
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t] {
+                    for (int i = t; i < (int)sbs.size(); i += nt) run(i);
+                });
+            for (auto& x : th) x.join();
+        }
+    }
+    // final accounting with the final contexts
+    double bits = 0.0;
+    int64_t sse = 0, cands = 0;
+    int32_t leaves = 0;
+    std::vector<double> sb_bits(nsb, 0.0);
+    std::vector<int64_t> sb_sse(nsb, 0);
+    auto account = [&](int sb) {
+        const int r = sb / C, c = sb % C, x0 = c * 64, y0 = r * 64;
+        const orc_rdo_node* nd = nodes + (size_t)sb * kNodes;
+        double b = 0.0;
+        int64_t e = 0;
+        int stack[64], sp = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const int id = stack[--sp];
+            if (nd[id].split == 2) continue;
+            const Geo g = node_geo(id);
+            const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+            const bool in = x + g.s <= f.W && y + g.s <= f.H;
+            if (g.s >= 8 && in) b += (double)f.part_bits(x, y, g.L, nd[id].split, x0, y0, true);
+            if (nd[id].split == 1) {
+                for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
+                continue;
+            }
+            const orc_cand cd = f.cand(x, y, g.s, nd[id].mode, x0, y0, true);
+            orc_candres res;
+            orc_eval_candidate(&cd, f.planes, 2, &res);
+            b += (double)res.bits;
+            e += res.sse;
+        }
+        sb_bits[sb] = b;
+        sb_sse[sb] = e;
+    };
+    {
+        std::vector<std::thread> th;
+        const int nt = nthreads <= 1 ? 1 : std::min(nthreads, nsb);
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (int sb = t; sb < nsb; sb += nt) account(sb);
+            });
+        for (auto& x : th) x.join();
+    }
+    for (int sb = 0; sb < nsb; ++sb) {
+        bits += sb_bits[sb];
+        sse += sb_sse[sb];
+        cands += outs[sb].candidates;
+        leaves += (int32_t)outs[sb].leaves.size();
+    }
+    std::memcpy(recon, f.rec.data(), f.rec.size() * 2);
+    if (modes) std::memcpy(modes, f.mode4.data(), f.mode4.size());
+    stats->bits = bits;
+    stats->sse = sse;
+    stats->candidates = cands;
+    stats->leaves = leaves;
+    stats->sbs = nsb;
+    return 0;
+}

@@ -307,3 +307,83 @@ def eval_batch(cands, planes, nthreads=8):
     out = np.zeros(len(c), CANDRES_DTYPE)
     L.orc_eval_batch(c.ctypes.data, len(c), arr, len(planes), out.ctypes.data, nthreads)
     return out
+
+
+# ---- whole-frame luma intra RDO (oracle/oracle_rdo.cpp) ---------------------------------
+RDO_NODE_DTYPE = np.dtype([("cost", "<f4"), ("none_cost", "<f4"), ("split", "u1"), ("mode", "u1"),
+                           ("pad", "u1", (2,))])
+
+
+class OrcRdoParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("bd", ctypes.c_int32),
+                ("qidx", ctypes.c_int32), ("qm_level", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("lambda_", ctypes.c_float), ("wsse_scale", ctypes.c_float)]
+
+
+class OrcRdoStats(ctypes.Structure):
+    _fields_ = [("bits", ctypes.c_double), ("sse", ctypes.c_int64), ("candidates", ctypes.c_int64),
+                ("leaves", ctypes.c_int32), ("sbs", ctypes.c_int32)]
+
+
+def _rdo_sigs(L):
+    if getattr(L, "_rdo_sigs", False):
+        return
+    L.orc_intra_rdo.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcRdoParams),
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.POINTER(OrcRdoStats), ctypes.c_int]
+    L.orc_rdo_lambda.argtypes = [ctypes.c_int] * 3
+    L.orc_rdo_lambda.restype = ctypes.c_float
+    L.orc_mode_lambda.argtypes = [ctypes.c_int] * 3
+    L.orc_mode_lambda.restype = ctypes.c_float
+    L._rdo_sigs = True
+
+
+def rdo_lambda(qidx, bd, sub):
+    L = lib()
+    _rdo_sigs(L)
+    return L.orc_rdo_lambda(qidx, bd, sub)
+
+
+def mode_lambda(qidx, bd, sub):
+    L = lib()
+    _rdo_sigs(L)
+    return L.orc_mode_lambda(qidx, bd, sub)
+
+
+def intra_rdo(src, bd, qidx, qm_level, flags, lam, wsse_scale=1.0, prune=None, nthreads=8):
+    """orc_intra_rdo on a 2-D uint16 frame.  Returns (recon, nodes [nsb, 341], modes
+    [H/4, W/4], stats dict)."""
+    L = lib()
+    _rdo_sigs(L)
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    H, W = src.shape
+    nsb = ((W + 63) // 64) * ((H + 63) // 64)
+    p = OrcRdoParams(W, H, bd, qidx, qm_level, flags, lam, wsse_scale)
+    recon = np.zeros((H, W), np.uint16)
+    nodes = np.zeros((nsb, 341), RDO_NODE_DTYPE)
+    modes = np.zeros((H // 4, W // 4), np.uint8)
+    st = OrcRdoStats()
+    pr = None
+    if prune is not None:
+        pr = np.ascontiguousarray(prune, dtype=np.uint16)
+        assert pr.shape == (nsb, 341)
+    rc = L.orc_intra_rdo(src.ctypes.data, W, ctypes.byref(p), None if pr is None else pr.ctypes.data,
+                         recon.ctypes.data, nodes.ctypes.data, modes.ctypes.data, ctypes.byref(st),
+                         nthreads)
+    if rc != 0:
+        raise ValueError("orc_intra_rdo: invalid parameters")
+    return recon, nodes, modes, {"bits": st.bits, "sse": st.sse, "candidates": st.candidates,
+                                 "leaves": st.leaves, "sbs": st.sbs}
+
+
+def cdf_tables():
+    """The CDF / table arrays of maroontree_amd/csrc/av1_cdf_tables.h as numpy arrays, by
+    name without the MT_ prefix (shapes as declared)."""
+    import re
+    hdr = open(os.path.join(ROOT, "maroontree_amd", "csrc", "av1_cdf_tables.h")).read()
+    out = {}
+    for m in re.finditer(r"static const (\w+) MT_(\w+)((?:\[\d+\])+) = \{(.*?)\};", hdr, re.S):
+        dims = [int(d) for d in re.findall(r"\[(\d+)\]", m.group(3))]
+        vals = np.array([int(v) for v in re.findall(r"-?\d+", m.group(4))], np.int64)
+        out[m.group(2)] = vals.reshape(dims)
+    return out
