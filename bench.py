@@ -43,6 +43,10 @@ def main():
     ap.add_argument("--eval-steps", type=int, default=10)
     ap.add_argument("--eval-only", action="store_true",
                     help="profiling aid: run only the candidate-evaluation leg and print its object")
+    ap.add_argument("--no-rdo", action="store_true", help="skip the whole-frame RDO leg")
+    ap.add_argument("--rdo-steps", type=int, default=3)
+    ap.add_argument("--rdo-only", action="store_true",
+                    help="profiling aid: run only the whole-frame RDO leg and print its object")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -72,6 +76,9 @@ def main():
     mt.init(0)
     if args.eval_only:
         print(json.dumps(bench_eval(args, dev, torch, mt)), flush=True)
+        return
+    if args.rdo_only:
+        print(json.dumps(bench_rdo(args, dev, torch, mt)), flush=True)
         return
 
     n = args.blocks
@@ -193,6 +200,8 @@ def main():
         }
     if not args.no_eval:
         line["eval"] = bench_eval(args, dev, torch, mt)
+    if not args.no_rdo:
+        line["rdo"] = bench_rdo(args, dev, torch, mt)
     print(json.dumps(line), flush=True)
 
 
@@ -268,6 +277,74 @@ def bench_eval(args, dev, torch, mt):
             "sample": f"{done} candidates drawn uniformly from the same set ({px / 1e6:.1f} Mpx "
                       f"of blocks), {cpu_s:.1f} s wall, std::thread over candidates",
         }
+    return r
+
+
+def bench_rdo(args, dev, torch, mt):
+    """Exact full intra RDO of a seeded 3840x2160 10-bit frame (luma) at qindex 91: one step =
+    one mt_intra_rdo call (wavefront decisions, reconstruction, final accounting)."""
+    from maroontree_amd import rdo as R
+    from maroontree_amd import workload as W
+    width, height, bd, qidx = 3840, 2160, 10, 91
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    img = W.synth_image(SEED, width, height, bd)
+    qm = mt.qm_level_law(qidx, 0)
+    lam = R.rdo_lambda(qidx, bd, 2)
+    src = torch.from_numpy(img.view(np.int16)).to(dev)
+    rec = torch.empty_like(src)
+    nsb = ((width + 63) // 64) * ((height + 63) // 64)
+    nodes = torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(args.warmup, 1)):
+        _, _, st = R.intra_rdo(src, bd, qidx, qm, flags, lam, recon=rec, nodes=nodes, stream=stream)
+    torch.cuda.synchronize()
+    mt.profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.rdo_steps):
+        _, _, st = R.intra_rdo(src, bd, qidx, qm, flags, lam, recon=rec, nodes=nodes, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = mt.profile_end()
+    ms = (t1 - t0) * 1000.0 / args.rdo_steps
+    npx = width * height
+    print(f"[bench] rdo: {ms:.2f} ms/frame, {npx / ms / 1e3:.1f} Mpx/s, "
+          f"{st['candidates'] / ms / 1e3:.1f} M candidates/s", file=sys.stderr, flush=True)
+    r = {
+        "workload": "exact full luma intra RDO of a seeded 3840x2160 10-bit synthetic frame at qindex 91 "
+                    "(qm_level_law QM, the reference's decision lambda for 4:2:0, weighted SSE): "
+                    "64x64 superblocks on the anti-diagonal wavefront, partitions 64..4, 13 modes + "
+                    "angle deltas + filter intra, DCT_DCT",
+        "ms_per_frame": round(ms, 3),
+        "mpx_per_s": round(npx / (ms / 1000.0) / 1e6, 2),
+        "candidates": st["candidates"],
+        "candidates_per_s": round(st["candidates"] / (ms / 1000.0), 1),
+        "leaves": st["leaves"],
+        "steps": st["steps"],
+        "bits": round(st["bits"], 1),
+        "psnr_y": round(R.psnr(st["sse"], npx, bd), 4),
+        "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = min(16, os.cpu_count() or 1)
+        # a bounded crop of the same frame: 16 x 8 superblocks
+        cw, ch = 1024, 512
+        crop = np.ascontiguousarray(img[:ch, :cw])
+        t0 = time.perf_counter()
+        _, _, _, cst = O.intra_rdo(crop, bd, qidx, qm, flags, lam, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        r["cpu_baseline"] = {
+            "value": round(cw * ch / cpu_s / 1e6, 4),
+            "unit": "Mpx/s",
+            "candidates_per_s": round(cst["candidates"] / cpu_s, 1),
+            "cores": threads,
+            "kind": "port",
+            "sample": f"the top-left {cw}x{ch} crop of the same frame ({cst['candidates']} candidates), "
+                      f"{cpu_s:.1f} s wall, std::thread over the superblocks of each wavefront step",
+        }
+        r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
     return r
 
 

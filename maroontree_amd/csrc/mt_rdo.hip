@@ -2,14 +2,16 @@
 // mt_intra_rdo (the procedure is documented at its declaration in include/mtgpu.h).
 //
 // Per wavefront step (every SB (r, c) with c + 2r = step):
-//   rdo_gen_kernel     one wave per SB: node flags, then per node a ballot over the 66 mode
-//                      ids writes the allowed candidates (compacted, in node / id order)
-//                      with their decision-time contexts and mode syntax bits;
+//   rdo_plan_kernel    one wave per SB: node flags (level by level), per-node candidate
+//                      counts and their prefix in node order;
+//   rdo_gen_kernel     one wave per (SB, node): a ballot over the 66 mode ids writes the
+//                      allowed candidates (compacted, in id order) with their
+//                      decision-time contexts and mode syntax bits;
 //   eval_launch        the batched candidate evaluation (prediction ... rd_cost);
 //   rdo_argmin_kernel  one thread per candidate: atomicMin of (cost bits, mode id) per node
 //                      (costs are >= 0, so their bit patterns order like the values);
-//   rdo_decide_kernel  one wave per SB: bottom-up NONE / SPLIT, leaves in coding order,
-//                      the final mode / partition maps and the leaf descriptors;
+//   rdo_decide_kernel  one wave per SB: bottom-up NONE / SPLIT level by level, leaves in
+//                      coding order, the final mode / partition maps and leaf descriptors;
 //   sb_recon_launch    one wave per SB reconstructs its leaves (mt_sbrecon.h).
 // Afterwards every leaf is re-priced with the final contexts in one batched evaluation.
 #include <hip/hip_runtime.h>
@@ -105,21 +107,34 @@ struct SbTask {
     uint32_t pad;
 };
 
-// Flags of every node of SB (r, c), in node order (parents before children).
+// Flags of every node of SB (r, c) into LDS, level by level (a node's search depends on its
+// parent); called by all 64 lanes of the wave.
 __device__ void node_flags(const FrameArgs& f, int r, int c, uint8_t* fl) {
     const uint16_t* mask = f.prune ? f.prune + (size_t)(r * f.C + c) * kNodes : nullptr;
-    for (int id = 0; id < kNodes; ++id) {
-        const int L = node_level(id), k = id - level_base(L), n = 1 << L, s = 64 >> L;
-        const int ix = k % n, iy = k / n;
-        const int x = c * 64 + ix * s, y = r * 64 + iy * s;
-        const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
-        bool sr = id == 0 || ((fl[node_id(L - 1, ix >> 1, iy >> 1)] & kMaySplit) && x < f.W && y < f.H);
-        const bool in = x + s <= f.W && y + s <= f.H;
-        const bool none_ok = sr && in && (s == f.minsz || !(m & MT_RDO_FORCE_SPLIT) || (m & MT_RDO_NO_SPLIT));
-        const bool may_split = sr && s > f.minsz && (!in || !(m & MT_RDO_NO_SPLIT));
-        fl[id] = (uint8_t)((sr ? kSearched : 0) | (in ? kInside : 0) | (none_ok ? kNoneOk : 0) |
-                           (may_split ? kMaySplit : 0));
+    for (int L = 0; L < 5; ++L) {
+        const int n = 1 << L, s = 64 >> L;
+        for (int k = threadIdx.x; k < n * n; k += 64) {
+            const int id = level_base(L) + k, ix = k % n, iy = k / n;
+            const int x = c * 64 + ix * s, y = r * 64 + iy * s;
+            const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
+            const bool sr = id == 0 || ((fl[node_id(L - 1, ix >> 1, iy >> 1)] & kMaySplit) && x < f.W && y < f.H);
+            const bool in = x + s <= f.W && y + s <= f.H;
+            const bool none_ok = sr && in && (s == f.minsz || !(m & MT_RDO_FORCE_SPLIT) || (m & MT_RDO_NO_SPLIT));
+            const bool may_split = sr && s > f.minsz && (!in || !(m & MT_RDO_NO_SPLIT));
+            fl[id] = (uint8_t)((sr ? kSearched : 0) | (in ? kInside : 0) | (none_ok ? kNoneOk : 0) |
+                               (may_split ? kMaySplit : 0));
+        }
+        __syncthreads();
     }
+}
+
+// Number of mode ids node `id` (flags fl) evaluates under mask m.
+__device__ inline int node_count(int id, uint8_t fl, uint16_t m, int flags) {
+    if (!(fl & kNoneOk)) return 0;
+    const int s = 64 >> node_level(id);
+    int k = 0;
+    for (int j = 0; j < kModeIds; ++j) k += (id_valid(j, s, flags) && id_allowed(j, m)) ? 1 : 0;
+    return k;
 }
 
 __device__ inline int map_mode(const FrameArgs& f, int x, int y) {
@@ -194,20 +209,18 @@ __global__ __launch_bounds__(64) void rdo_count_kernel(FrameArgs f, uint32_t* __
     __shared__ uint8_t fl[kNodes];
     const int sb = blockIdx.x, lane = threadIdx.x;
     const int r = sb / f.C, c = sb % f.C;
-    if (lane == 0) node_flags(f, r, c, fl);
-    __syncthreads();
+    node_flags(f, r, c, fl);
     const uint16_t* mask = f.prune ? f.prune + (size_t)sb * kNodes : nullptr;
     uint32_t cnt = 0, pool = 0;
-    for (int id = 0; id < kNodes; ++id) {
-        if (!(fl[id] & kNoneOk)) continue;
+    for (int id = lane; id < kNodes; id += 64) {
         const int s = 64 >> node_level(id);
-        const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
-        for (int j = lane; j < 128; j += 64) {
-            const bool ok = j < kModeIds && id_valid(j, s, f.flags) && id_allowed(j, m);
-            const uint32_t k = (uint32_t)__popcll(__ballot(ok));
-            cnt += k;
-            pool += k * (uint32_t)(s * s);
-        }
+        const uint32_t k = (uint32_t)node_count(id, fl[id], mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES, f.flags);
+        cnt += k;
+        pool += k * (uint32_t)(s * s);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o);
+        pool += __shfl_xor(pool, o);
     }
     if (lane == 0) {
         counts[sb] = cnt;
@@ -215,40 +228,80 @@ __global__ __launch_bounds__(64) void rdo_count_kernel(FrameArgs f, uint32_t* __
     }
 }
 
-// Writes the candidates of the step's SBs (one wave each) and their node / id references
-// ((local SB << 16) | (node << 7) | id); saves the node flags for the decision.
-__global__ __launch_bounds__(64) void rdo_gen_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
-                                                     mt_cand* __restrict__ cands,
-                                                     uint32_t* __restrict__ refs,
-                                                     uint8_t* __restrict__ nflags) {
+// Per-node plan of a step's SB: where its candidates and pool slots start, how many, its
+// flags and mask.
+struct NodePlan {
+    uint32_t cand_base, pool_base;
+    uint16_t mask;
+    uint8_t count, flags;
+};
+
+// One wave per SB of the step: node flags, per-node counts and their prefix in node order.
+__global__ __launch_bounds__(64) void rdo_plan_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+                                                      NodePlan* __restrict__ plan) {
     __shared__ uint8_t fl[kNodes];
+    __shared__ uint32_t cnt[kNodes], pl[kNodes];
     const int local = blockIdx.x, lane = threadIdx.x;
     const SbTask t = tasks[local];
-    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
-    if (lane == 0) node_flags(f, r, c, fl);
-    __syncthreads();
-    for (int id = lane; id < kNodes; id += 64) nflags[(size_t)local * kNodes + id] = fl[id];
+    const int r = t.sb / f.C, c = t.sb % f.C;
+    node_flags(f, r, c, fl);
     const uint16_t* mask = f.prune ? f.prune + (size_t)t.sb * kNodes : nullptr;
-    uint32_t k0 = t.cand_base, p0 = t.pool_base;
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    for (int id = 0; id < kNodes; ++id) {
-        if (!(fl[id] & kNoneOk)) continue;
-        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
-        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+    for (int id = lane; id < kNodes; id += 64) {
+        const int s = 64 >> node_level(id);
         const uint16_t m = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
-        for (int j = lane; j < 128; j += 64) {
-            const bool ok = j < kModeIds && id_valid(j, s, f.flags) && id_allowed(j, m);
-            const uint64_t b = __ballot(ok);
-            if (ok) {
-                const uint32_t rank = (uint32_t)__popcll(b & below);
-                const uint32_t k = k0 + rank;
-                cands[k] = make_cand(f, x, y, s, j, x0, y0, false, p0 + rank * (uint32_t)(s * s));
-                refs[k] = ((uint32_t)local << 16) | ((uint32_t)id << 7) | (uint32_t)j;
-            }
-            const uint32_t tot = (uint32_t)__popcll(b);
-            k0 += tot;
-            p0 += tot * (uint32_t)(s * s);
+        cnt[id] = (uint32_t)node_count(id, fl[id], m, f.flags);
+        pl[id] = cnt[id] * (uint32_t)(s * s);
+    }
+    __syncthreads();
+    if (lane == 0) {  // exclusive prefix in node order
+        uint32_t a = t.cand_base, b = t.pool_base;
+        for (int id = 0; id < kNodes; ++id) {
+            const uint32_t ca = cnt[id], pa = pl[id];
+            cnt[id] = a;
+            pl[id] = b;
+            a += ca;
+            b += pa;
         }
+    }
+    __syncthreads();
+    for (int id = lane; id < kNodes; id += 64) {
+        NodePlan p;
+        p.cand_base = cnt[id];
+        p.pool_base = pl[id];
+        p.mask = mask ? mask[id] : (uint16_t)MT_RDO_ALL_MODES;
+        p.flags = fl[id];
+        p.count = (uint8_t)node_count(id, fl[id], p.mask, f.flags);
+        plan[(size_t)local * kNodes + id] = p;
+    }
+}
+
+// Writes the candidates of the step: one wave per (SB, node); lane j takes mode ids j and
+// j + 64 and the allowed ones are compacted in id order.  refs[k] = (local SB << 16) |
+// (node << 7) | id.
+__global__ __launch_bounds__(64) void rdo_gen_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+                                                     const NodePlan* __restrict__ plan,
+                                                     mt_cand* __restrict__ cands,
+                                                     uint32_t* __restrict__ refs) {
+    const int local = blockIdx.x / kNodes, id = blockIdx.x % kNodes, lane = threadIdx.x;
+    const NodePlan p = plan[blockIdx.x];
+    if (p.count == 0) return;
+    const SbTask t = tasks[local];
+    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
+    const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+    const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t k0 = p.cand_base, p0 = p.pool_base;
+    for (int j = lane; j < 128; j += 64) {
+        const bool ok = j < kModeIds && id_valid(j, s, f.flags) && id_allowed(j, p.mask);
+        const uint64_t b = __ballot(ok);
+        if (ok) {
+            const uint32_t rank = (uint32_t)__popcll(b & below);
+            cands[k0 + rank] = make_cand(f, x, y, s, j, x0, y0, false, p0 + rank * (uint32_t)(s * s));
+            refs[k0 + rank] = ((uint32_t)local << 16) | ((uint32_t)id << 7) | (uint32_t)j;
+        }
+        const uint32_t tot = (uint32_t)__popcll(b);
+        k0 += tot;
+        p0 += tot * (uint32_t)(s * s);
     }
 }
 
@@ -279,10 +332,12 @@ __global__ void __launch_bounds__(256) rdo_argmin_kernel(const uint32_t* __restr
     atomicMin(&keys[(size_t)local * kNodes + node], key);
 }
 
-// Bottom-up decision of one SB (lane 0), its leaves in coding order, maps and descriptors.
+// Decision of one SB per wave: bottom-up NONE / SPLIT level by level (lanes over the
+// level's nodes), the leaves in coding order (lane 0 walks the tree in LDS), the final maps
+// (lanes over the SB's 4x4 and 8x8 units) and the leaf descriptors (lanes over leaves).
 __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
                                                         const unsigned long long* __restrict__ keys,
-                                                        const uint8_t* __restrict__ nflags,
+                                                        const NodePlan* __restrict__ plan,
                                                         mt_predblk* __restrict__ leaves,
                                                         uint32_t* __restrict__ first,
                                                         uint32_t* __restrict__ count,
@@ -292,85 +347,93 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTas
     __shared__ uint8_t split[kNodes];
     __shared__ uint8_t mode[kNodes];
     __shared__ uint16_t lv[kMaxLeaves];
-    const int local = blockIdx.x;
-    if (threadIdx.x != 0) return;
+    __shared__ int s_nl;
+    const int local = blockIdx.x, lane = threadIdx.x;
     const SbTask t = tasks[local];
     const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
-    const uint8_t* fl = nflags + (size_t)local * kNodes;
     mt_rdo_node* out = f.nodes + (size_t)t.sb * kNodes;
-    for (int id = kNodes - 1; id >= 0; --id) {
-        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
-        const int ix = kk % n, iy = kk / n, x = x0 + ix * s, y = y0 + iy * s;
-        const unsigned long long key = keys[(size_t)local * kNodes + id];
-        const float best = key == ~0ull ? __int_as_float(0x7f800000) : __uint_as_float((uint32_t)(key >> 32));
-        const int bid = key == ~0ull ? 0 : (int)(key & 127);
-        mode[id] = (uint8_t)bid;
-        mt_rdo_node nd{};
-        nd.mode = (uint8_t)bid;
-        if (!(fl[id] & kSearched)) {
-            cost[id] = 0.0f;
-            split[id] = 2;
-            nd.cost = __int_as_float(0x7f800000);
-            nd.none_cost = nd.cost;
-            nd.split = 2;
-            out[id] = nd;
-            continue;
-        }
-        const float inf = __int_as_float(0x7f800000);
-        float none = inf, spl = inf;
-        if ((fl[id] & kNoneOk) && best < inf) {
-            none = best;
-            if (s >= 8) none = __fadd_rn(none, __fmul_rn(f.lambda, part_bits(f, x, y, L, 0, x0, y0, false)));
-        }
-        if (fl[id] & kMaySplit) {
-            float sum = 0.0f;
-            for (int q = 0; q < 4; ++q) sum = __fadd_rn(sum, cost[node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1))]);
-            spl = (fl[id] & kInside) ? __fadd_rn(sum, __fmul_rn(f.lambda, part_bits(f, x, y, L, 1, x0, y0, false)))
+    const float inf = __int_as_float(0x7f800000);
+    for (int L = 4; L >= 0; --L) {
+        const int n = 1 << L, s = 64 >> L;
+        for (int kk = lane; kk < n * n; kk += 64) {
+            const int id = level_base(L) + kk, ix = kk % n, iy = kk / n, x = x0 + ix * s, y = y0 + iy * s;
+            const uint8_t fl = plan[(size_t)local * kNodes + id].flags;
+            const unsigned long long key = keys[(size_t)local * kNodes + id];
+            const float best = key == ~0ull ? inf : __uint_as_float((uint32_t)(key >> 32));
+            const int bid = key == ~0ull ? 0 : (int)(key & 127);
+            mode[id] = (uint8_t)bid;
+            mt_rdo_node nd{};
+            nd.mode = (uint8_t)bid;
+            if (!(fl & kSearched)) {
+                cost[id] = 0.0f;
+                split[id] = 2;
+                nd.cost = inf;
+                nd.none_cost = inf;
+                nd.split = 2;
+                out[id] = nd;
+                continue;
+            }
+            float none = inf, spl = inf;
+            if ((fl & kNoneOk) && best < inf) {
+                none = best;
+                if (s >= 8) none = __fadd_rn(none, __fmul_rn(f.lambda, part_bits(f, x, y, L, 0, x0, y0, false)));
+            }
+            if (fl & kMaySplit) {
+                float sum = 0.0f;
+                for (int q = 0; q < 4; ++q) sum = __fadd_rn(sum, cost[node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1))]);
+                spl = (fl & kInside) ? __fadd_rn(sum, __fmul_rn(f.lambda, part_bits(f, x, y, L, 1, x0, y0, false)))
                                      : sum;
+            }
+            const bool leaf = none <= spl;
+            cost[id] = leaf ? none : spl;
+            split[id] = leaf ? 0 : 1;
+            nd.cost = cost[id];
+            nd.none_cost = none;
+            nd.split = split[id];
+            out[id] = nd;
         }
-        if (none <= spl) {
-            cost[id] = none;
-            split[id] = 0;
-        } else {
-            cost[id] = spl;
-            split[id] = 1;
-        }
-        nd.cost = cost[id];
-        nd.none_cost = none;
-        nd.split = split[id];
-        out[id] = nd;
+        __syncthreads();
     }
-    // leaves in coding order (children pushed in reverse so they pop 0, 1, 2, 3)
-    int nl = 0;
-    uint16_t stack[32];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-        const int id = stack[--sp];
-        if (split[id] == 2) continue;
-        const int L = node_level(id), kk = id - level_base(L), n = 1 << L;
-        if (split[id] == 1) {
-            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
-            continue;
+    if (lane == 0) {  // leaves in coding order (children pushed in reverse: they pop 0, 1, 2, 3)
+        int nl = 0, sp = 0;
+        uint16_t stack[32];
+        stack[sp++] = 0;
+        while (sp) {
+            const int id = stack[--sp];
+            if (split[id] == 2) continue;
+            const int L = node_level(id), kk = id - level_base(L), n = 1 << L;
+            if (split[id] == 1) {
+                for (int q = 3; q >= 0; --q)
+                    stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+                continue;
+            }
+            lv[nl++] = (uint16_t)id;
         }
-        lv[nl++] = (uint16_t)id;
+        s_nl = nl;
     }
-    // the final maps of the SB's leaves (partition context bytes of part_ctx_bytes,
-    // src/coder/superblock.rs:1062: NONE at 64 / 32 / 16 / 8, and the 8x8 split into 4x4)
+    // the final maps: each 4x4 (and 8x8) unit inside the frame finds its leaf from the root;
+    // partition context bytes of part_ctx_bytes (src/coder/superblock.rs:1062): NONE at
+    // 64 / 32 / 16 / 8, and an 8x8 split into 4x4
     const uint8_t pbyte[5] = {0x10, 0x18, 0x1c, 0x1e, 0x1f};
-    uint32_t pool = 0;
-    for (int i = 0; i < nl; ++i) {
-        const int id = lv[i];
-        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
-        const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
-        const int base = id_base(mode[id]);
-        for (int yy = y; yy < y + s; yy += 4)
-            for (int xx = x; xx < x + s; xx += 4) f.mode4[(size_t)(yy >> 2) * (f.W >> 2) + (xx >> 2)] = (uint8_t)base;
-        for (int yy = y; yy < y + s; yy += 8)
-            for (int xx = x; xx < x + s; xx += 8) f.part8[(size_t)(yy >> 3) * (f.W >> 3) + (xx >> 3)] = pbyte[L];
-        pool += (uint32_t)(s * s);
+    for (int u = lane; u < 256 + 64; u += 64) {
+        const bool u4 = u < 256;
+        const int g = u4 ? 16 : 8, k = u4 ? u : u - 256, cell = u4 ? 4 : 8;
+        const int ux = (k % g) * cell, uy = (k / g) * cell;
+        if (x0 + ux >= f.W || y0 + uy >= f.H) continue;
+        int id = 0, L = 0;
+        while (split[id] == 1) {
+            ++L;
+            const int s = 64 >> L;
+            id = node_id(L, ux / s, uy / s);
+        }
+        if (split[id] != 0) continue;
+        if (u4) f.mode4[(size_t)((y0 + uy) >> 2) * (f.W >> 2) + ((x0 + ux) >> 2)] = (uint8_t)id_base(mode[id]);
+        else f.part8[(size_t)((y0 + uy) >> 3) * (f.W >> 3) + ((x0 + ux) >> 3)] = pbyte[L];
     }
-    for (int i = 0; i < nl; ++i) {
+    __syncthreads();  // the maps and the leaf list are complete
+    const int nl = s_nl;
+    uint32_t pool = 0;
+    for (int i = lane; i < nl; i += 64) {
         const int id = lv[i];
         const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
         const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
@@ -385,11 +448,15 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTas
         p.flags = cd.pflags;
         p.bd = (uint8_t)f.bd;
         leaves[(size_t)local * kMaxLeaves + i] = p;
+        pool += (uint32_t)(s * s);
     }
-    first[local] = (uint32_t)local * kMaxLeaves;
-    count[local] = (uint32_t)nl;
-    sb_leaves[t.sb] = (uint32_t)nl;
-    sb_leaf_pool[t.sb] = pool;
+    for (int o = 32; o > 0; o >>= 1) pool += __shfl_xor(pool, o);
+    if (lane == 0) {
+        first[local] = (uint32_t)local * kMaxLeaves;
+        count[local] = (uint32_t)nl;
+        sb_leaves[t.sb] = (uint32_t)nl;
+        sb_leaf_pool[t.sb] = pool;
+    }
 }
 
 // The SB's leaves with the final contexts, at leaf_base[sb] / pool_base[sb] (one thread per SB).
@@ -464,7 +531,7 @@ struct RdoWork {
     long long* sb_sse = nullptr;
     SbTask* tasks = nullptr;
     unsigned long long* keys = nullptr;
-    uint8_t* nflags = nullptr;
+    NodePlan* plan = nullptr;
     mt_predblk* leaves = nullptr;
     uint32_t *first = nullptr, *count = nullptr;
     mt_cand* cands = nullptr;
@@ -646,7 +713,7 @@ extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* 
     }
     step_first[nsteps] = (int)tasks.size();
     if (w.cap_step_sb < max_sb) {
-        if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.nflags, max_sb * kNodes)) ||
+        if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.plan, max_sb * kNodes)) ||
             (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
             (rc = regrow(w.count, max_sb)))
             return rc;
@@ -668,7 +735,8 @@ extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* 
             mtlib::ProfScope prof(mtlib::kProfRdo, s);
             hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
                                w.keys, (size_t)nk * kNodes, ~0ull);
-            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.cands, w.refs, w.nflags);
+            hipLaunchKernelGGL(rdo_plan_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.plan);
+            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk * kNodes), dim3(64), 0, s, f, tk, w.plan, w.cands, w.refs);
             MT_HIP_TRY(hipGetLastError());
         }
         if ((rc = mtlib::eval_launch(w.cands, n, ps, step_pool[k], w.cres, s))) return rc;
@@ -676,7 +744,7 @@ extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* 
             mtlib::ProfScope prof(mtlib::kProfRdo, s);
             if (n > 0)
                 hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n, w.keys);
-            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.keys, w.nflags, w.leaves,
+            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.keys, w.plan, w.leaves,
                                w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
             MT_HIP_TRY(hipGetLastError());
         }
