@@ -47,6 +47,18 @@ def main():
     ap.add_argument("--rdo-steps", type=int, default=3)
     ap.add_argument("--rdo-only", action="store_true",
                     help="profiling aid: run only the whole-frame RDO leg and print its object")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruning leg")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="run only the learned-pruning leg and print its object")
+    ap.add_argument("--learned-train", type=int, default=48, help="training frames (1920x1080)")
+    ap.add_argument("--learned-eval", type=int, default=12, help="held-out frames per quality")
+    ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
+    ap.add_argument("--stream-images", type=int, default=48)
+    ap.add_argument("--stream-max-side", type=int, default=2048)
+    ap.add_argument("--topk", type=int, default=5)
+    ap.add_argument("--lo", type=float, default=0.15)
+    ap.add_argument("--hi", type=float, default=0.85)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -79,6 +91,9 @@ def main():
         return
     if args.rdo_only:
         print(json.dumps(bench_rdo(args, dev, torch, mt)), flush=True)
+        return
+    if args.learned_only:
+        print(json.dumps(bench_learned(args, dev, torch, mt)), flush=True)
         return
 
     n = args.blocks
@@ -202,6 +217,11 @@ def main():
         line["eval"] = bench_eval(args, dev, torch, mt)
     if not args.no_rdo:
         line["rdo"] = bench_rdo(args, dev, torch, mt)
+    net = None
+    if not args.no_learned:
+        line["learned"], net = bench_learned(args, dev, torch, mt, return_net=True)
+    if not args.no_stream:
+        line["stream"] = bench_stream(args, dev, torch, mt, net)
     print(json.dumps(line), flush=True)
 
 
@@ -345,6 +365,145 @@ def bench_rdo(args, dev, torch, mt):
                       f"{cpu_s:.1f} s wall, std::thread over the superblocks of each wavefront step",
         }
         r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
+    return r
+
+
+def bench_learned(args, dev, torch, mt, return_net=False):
+    """Learned pruning: the network trained on the GPU on exact-RDO labels of seeded 1920x1080
+    8-bit frames, then the pruned RDO against the full RDO on held-out frames at quality
+    30 / 50 / 70 / 90 (qindex 140 / 91 / 53 / 17): speedup and BD-rate (PSNR-Y, rate = the
+    RDO's final bits), and the same policy with the untrained network."""
+    from maroontree_amd import prune as PR
+    from maroontree_amd import rdo as R
+    from maroontree_amd import workload as W
+    width, height, bd = 1920, 1080, 8
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    qs = [140, 91, 53, 17]
+    log = lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    train_seeds = [10000 + i for i in range(args.learned_train)]
+    data = PR.make_dataset(dev, train_seeds, width, height, bd, qs, flags)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    log(f"{data[0].shape[0]} superblocks labelled by the exact RDO in {t_data:.1f} s")
+    net = PR.PruneNet(PR.init_params(1), dev)
+    untrained = PR.PruneNet(PR.init_params(1), dev)
+    t0 = time.perf_counter()
+    hist = PR.train(net, data, bd, epochs=args.learned_epochs, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del data
+    eval_imgs = [W.synth_image(90000 + i, width, height, bd) for i in range(args.learned_eval)]
+    srcs = [torch.from_numpy(img.view(np.int16)).to(dev) for img in eval_imgs]
+    npx = width * height * len(srcs)
+
+    def run(q, model):
+        frames = [{"src": s, "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, 0)} for s in srcs]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        if model is not None:
+            for fr in frames:
+                fr["prune"] = PR.prune_masks(model, fr["src"], bd, q, args.topk, args.lo, args.hi)
+        out = R.intra_rdo_batch(frames, flags)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        bits = sum(o[2]["bits"] for o in out)
+        sse = sum(o[2]["sse"] for o in out)
+        cands = sum(o[2]["candidates"] for o in out)
+        return dt, bits, R.psnr(sse, npx, bd), cands
+
+    res = {"full": [], "learned": [], "untrained": []}
+    for q in qs:
+        run(q, None)  # warm-up of this batch shape
+        for name, model in (("full", None), ("learned", net), ("untrained", untrained)):
+            res[name].append(run(q, model))
+        log(f"q {q}: full {res['full'][-1][0]:.3f} s, learned {res['learned'][-1][0]:.3f} s")
+    out = {
+        "workload": f"{args.learned_train} training and {args.learned_eval} held-out seeded 1920x1080 8-bit "
+                    "frames; labels from the exact full RDO (partitions 64..4, 13 modes + angle deltas + "
+                    "filter intra); pruned RDO at qindex 140/91/53/17 vs the full RDO",
+        "network": "conv3x3s2 1-16-32-64, pool, fc 257-64-98; 46178 params; bf16 MFMA, fp32 accumulation",
+        "train_superblocks": int(len(train_seeds) * ((width + 63) // 64) * ((height + 63) // 64)),
+        "label_seconds": round(t_data, 2),
+        "train_seconds": round(t_train, 2),
+        "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
+        "policy": {"topk_modes": args.topk, "no_split_below": args.lo, "force_split_above": args.hi},
+    }
+    full_t = sum(r[0] for r in res["full"])
+    for name in ("learned", "untrained"):
+        t = sum(r[0] for r in res[name])
+        out[name] = {
+            "speedup_vs_full": round(full_t / t, 3),
+            "bd_rate_pct": round(PR.bd_rate([r[1] for r in res["full"]], [r[2] for r in res["full"]],
+                                            [r[1] for r in res[name]], [r[2] for r in res[name]]), 3),
+            "candidate_fraction": round(sum(r[3] for r in res[name]) / sum(r[3] for r in res["full"]), 4),
+            "mpx_per_s": round(npx * len(qs) / t / 1e6, 2),
+        }
+    out["full"] = {"mpx_per_s": round(npx * len(qs) / full_t / 1e6, 2),
+                   "points": [[round(r[1], 1), round(r[2], 4)] for r in res["full"]]}
+    out["learned"]["points"] = [[round(r[1], 1), round(r[2], 4)] for r in res["learned"]]
+    return (out, net) if return_net else out
+
+
+def bench_stream(args, dev, torch, mt, net=None):
+    """Uneven stream: seeded frames of log-uniform sizes 64 .. max side (mixed aspect
+    ratios, 8 / 10-bit, 4:2:0 / 4:4:4 lambda laws, qindex 140 / 91 / 53 / 17), all queued at
+    t = 0 and coded in groups of 16 frames of similar wavefront length (one
+    mt_intra_rdo_batch each).  Aggregate Mpx/s and per-frame latency (completion of its
+    group); with the learned pruning too when a trained network is given."""
+    from maroontree_amd import prune as PR
+    from maroontree_amd import rdo as R
+    from maroontree_amd import workload as W
+    rng = np.random.default_rng(SEED)
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    specs = []
+    for i in range(args.stream_images):
+        w, h = (int(np.exp(rng.uniform(np.log(64), np.log(args.stream_max_side)))) // 8 * 8 for _ in range(2))
+        bd = int(rng.choice([8, 10]))
+        sub = int(rng.choice([0, 2]))
+        q = int(rng.choice([140, 91, 53, 17]))
+        specs.append((w, h, bd, sub, q))
+    imgs = [W.synth_image(70000 + i, w, h, bd) for i, (w, h, bd, _, _) in enumerate(specs)]
+    srcs = [torch.from_numpy(im.view(np.int16)).to(dev) for im in imgs]
+    steps = [((w + 63) // 64 - 1) + 2 * ((h + 63) // 64 - 1) + 1 for (w, h, _, _, _) in specs]
+    order = sorted(range(len(specs)), key=lambda i: -steps[i])
+    groups = [order[i:i + 16] for i in range(0, len(order), 16)]
+    npx = sum(w * h for (w, h, _, _, _) in specs)
+
+    def run(model):
+        lat = [0.0] * len(specs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cands = 0
+        for g in groups:
+            frames = []
+            for i in g:
+                w, h, bd, sub, q = specs[i]
+                fr = {"src": srcs[i], "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, sub),
+                      "lam": R.rdo_lambda(q, bd, sub)}
+                if model is not None:
+                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi)
+                frames.append(fr)
+            out = R.intra_rdo_batch(frames, flags)
+            torch.cuda.synchronize()
+            now = time.perf_counter() - t0
+            for i, o in zip(g, out):
+                lat[i] = now
+                cands += o[2]["candidates"]
+        total = time.perf_counter() - t0
+        return {"seconds": round(total, 3), "mpx_per_s": round(npx / total / 1e6, 2),
+                "candidates_per_s": round(cands / total, 1),
+                "latency_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                "latency_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1)}
+
+    run(None)  # warm-up
+    r = {"workload": f"{len(specs)} seeded frames, sides log-uniform 64..{args.stream_max_side} (multiples of 8), "
+                     "8/10-bit, 4:2:0/4:4:4 lambda, qindex 140/91/53/17, exact full luma RDO; groups of 16 "
+                     "frames of similar wavefront length per mt_intra_rdo_batch",
+         "frames": len(specs), "mpx": round(npx / 1e6, 2), "pruning_off": run(None)}
+    if net is not None:
+        r["pruning_on"] = run(net)
+    print(f"[bench] stream: {r}", file=sys.stderr, flush=True)
     return r
 
 

@@ -158,7 +158,7 @@ typedef struct {
     int32_t vis_w, vis_h;
     int32_t pad;
 } mt_plane;
-#define MT_MAX_PLANES 4
+#define MT_MAX_PLANES 32
 
 /* Intra modes (the reference's numbering, src/intrapred.rs:54-82) and filter intra. */
 enum {
@@ -261,13 +261,16 @@ typedef struct {
     uint8_t mode;         /* MT_PRED_* */
     int8_t angle_delta;   /* -3..3 */
     uint8_t pflags;       /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB */
-    uint8_t planes;       /* bits 0-1: source plane index, bits 2-3: reconstruction plane index */
+    uint8_t planes;       /* bits 0-1: source plane index, bits 2-3: reconstruction plane index
+                             (low bits; the high bits are in planes_hi) */
     uint8_t qidx;         /* base_q_idx of the quantizer */
     uint8_t bd_qm;        /* as mt_txblk: bits 0-3 QM level, bits 4-5 bit depth (0 8, 1 10, 2 12),
                              bit 6 chroma: new_chroma_with_delta_qm with dc_delta / ac_delta */
     uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;  /* rate context, as mt_rateblk */
     uint8_t txtp_set, txtp_mode, txtp, qctx;
-    uint8_t rflags, pad0;
+    uint8_t rflags;
+    uint8_t planes_hi;    /* bits 0-3: source plane index >> 2, bits 4-7: reconstruction >> 2
+                             (0 with up to 4 planes) */
     int8_t dc_delta, ac_delta;  /* chroma qindex deltas (src/quant.rs:394) */
     float skip_one, skip_zero;
     float wsse_scale;     /* RdDispatch::wsse_scale: > 0 prices the weighted distortion */
@@ -369,6 +372,12 @@ typedef struct {
 /* Runs the RDO of one frame on `stream` and waits for it.  Returns MT_OK, MT_ERR_ARG for
  * invalid parameters, or a device error. */
 int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream);
+/* The RDO of several frames of any sizes, bit depths and quantizers at once (stats: one per
+ * frame): up to MT_MAX_PLANES / 2 frames share every wavefront step, so each step's batch
+ * evaluation holds the candidates of all of them; larger counts run in such groups.  Each
+ * frame's result equals its own mt_intra_rdo. */
+int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
+                       void* stream);
 
 /* The reference's decision lambda lambda_d = mlam / K_PART_LAMBDA
  * (src/coder/lossy_state.rs:1769-1790): mode_lambda_q(dc_q, bd) (src/cost.rs:285) times
@@ -376,6 +385,50 @@ int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void
  * sub: 0 4:4:4, 1 4:2:2, 2 4:2:0.  mt_mode_lambda returns mlam. */
 float mt_rdo_lambda(int qidx, int bd, int sub);
 float mt_mode_lambda(int qidx, int bd, int sub);
+
+/* ---- learned candidate pruning (bf16 CNN on MFMA) ---------------------------------- */
+/* A small network replaces / feeds the reference's hand pruning (proxy_mode_beam
+ * src/coder/partition_search.rs:187, split_breakout_k :135-160, rank_luma_modes
+ * src/coder/superblock.rs:102): 64x64 luma superblock + qindex -> conv 3x3 s2 (1->16) ->
+ * conv 3x3 s2 (16->32) -> conv 3x3 s2 (32->64), ReLU each -> 4x4 average pool ++ qindex/255
+ * -> fc 257->64 ReLU -> fc 64->98: outputs 0..84 the split logits of quadtree nodes 0..84
+ * (64x64 .. 8x8), 85..97 the logits of intra modes 0..12 being used by the superblock's
+ * leaves.  Parameters: one flat fp32 array (weights [K][N] row-major then biases, layer by
+ * layer: conv1 9x16 + 16, conv2 144x32 + 32 (k = tap * 16 + channel), conv3 288x64 + 64,
+ * fc1 257x64 + 64, fc2 64x98 + 98); every product runs in bf16 with fp32 accumulation. */
+#define MT_PNET_PARAMS 46178
+#define MT_PNET_OUTPUTS 98
+
+/* C[m][n] = sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn], A and B bf16 (device), C fp32 [M][N]:
+ * the MFMA GEMM every layer uses. */
+int mt_gemm_bf16(int M, int N, int K, const uint16_t* A_d, long sam, long sak, const uint16_t* B_d,
+                 long sbk, long sbn, float* C_d, void* stream);
+/* Device workspace (bytes) of a batch: bf16 weights, activations, columns, gradients. */
+size_t mt_pnet_workspace_bytes(int batch);
+/* The frame's 64x64 superblocks in raster order into pix_d [nsb][4096] (edge samples
+ * replicated past the frame). */
+int mt_pnet_extract(const mt_plane* plane, uint16_t* pix_d, void* stream);
+/* Forward of `batch` superblocks: sample b = pix_d[idx_d[b]] (idx_d NULL: b) with qindex
+ * qidx_d[idx_d[b]]; logits_d (optional): [batch][98].  Keeps the activations in work_d
+ * for mt_pnet_backward. */
+int mt_pnet_forward(const float* params_d, const uint16_t* pix_d, const int32_t* idx_d,
+                    const uint8_t* qidx_d, int bd, int batch, void* work_d, float* logits_d,
+                    void* stream);
+/* Backward of the last forward for masked binary cross-entropy with logits against
+ * targets_d / mask_d [batch][98]; the split terms (0..84) are weighted by norm_split, the
+ * mode terms by norm_mode.  grads_d: MT_PNET_PARAMS fp32 (overwritten); loss_terms_d
+ * (optional): the [batch][98] weighted loss terms. */
+int mt_pnet_backward(const float* params_d, const float* targets_d, const float* mask_d, int batch,
+                     float norm_split, float norm_mode, void* work_d, float* grads_d,
+                     float* loss_terms_d, void* stream);
+/* One Adam step (bias-corrected; `step` counts from 1) on the fp32 parameters. */
+int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, int step, float lr,
+                 float beta1, float beta2, float eps, void* stream);
+/* mt_intra_rdo prune masks [batch][MT_RDO_NODES] from the logits: the top-k intra modes of
+ * each superblock (DC_PRED and filter intra always kept) on every node, and for nodes 0..84
+ * MT_RDO_NO_SPLIT when sigmoid(split logit) < lo, MT_RDO_FORCE_SPLIT when > hi. */
+int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, float lo, float hi,
+                        uint16_t* prune_d, void* stream);
 
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and

@@ -60,7 +60,7 @@ CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size"
                        ("mode", "u1"), ("angle_delta", "i1"), ("pflags", "u1"), ("planes", "u1"),
                        ("qidx", "u1"), ("bd_qm", "u1"), ("rplane", "u1"), ("eob_bin", "u1"),
                        ("skip_ctx", "u1"), ("dcs_ctx", "u1"), ("txtp_set", "u1"), ("txtp_mode", "u1"),
-                       ("txtp", "u1"), ("qctx", "u1"), ("rflags", "u1"), ("pad0", "u1"),
+                       ("txtp", "u1"), ("qctx", "u1"), ("rflags", "u1"), ("planes_hi", "u1"),
                        ("dc_delta", "i1"), ("ac_delta", "i1"),
                        ("skip_one", "<f4"), ("skip_zero", "<f4"), ("wsse_scale", "<f4"),
                        ("lambda", "<f4"), ("extra_bits", "<f4")])
@@ -69,7 +69,7 @@ CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bi
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
 TXBLK_CHROMA = 0x40
 PRED_FILTER0 = 16
-MAX_PLANES = 4
+MAX_PLANES = 32
 
 
 class MtPlane(ctypes.Structure):
@@ -109,6 +109,7 @@ def lib():
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
         L.mt_intra_rdo.argtypes = [vp, i32, vp, vp]
+        L.mt_intra_rdo_batch.argtypes = [vp, i32, i32, vp, vp]
         L.mt_rdo_lambda.argtypes = L.mt_mode_lambda.argtypes = [i32, i32, i32]
         L.mt_rdo_lambda.restype = L.mt_mode_lambda.restype = ctypes.c_float
         _lib = L

@@ -40,12 +40,19 @@ int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, con
 // mt_eval_candidates on a prepared plane set.
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
                 mt_candres* res_d, hipStream_t s);
+// One frame of the RDO's reconstruction: its source, reconstruction plane (rstride samples
+// per row, frame size of src) and quantizer (base_q_idx, bd_qm as in mt_txblk).
+struct ReconFrame {
+    mt_plane src;
+    uint16_t* rec;
+    int32_t rstride;
+    int32_t qidx, bd_qm, pad;
+};
 // The RDO's superblock reconstruction: one wave per superblock b walks the leaves
 // leaves_d[first_d[b] .. + count_d[b]) in coding order, predicting from and writing to the
-// reconstruction plane rec (rstride samples per row; frame size of src).
+// reconstruction plane of frame frames_d[leaf.plane] (device array).
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
-                    int nsb, const mt_plane& src, uint16_t* rec, int rstride, int qidx, int bd_qm,
-                    hipStream_t s);
+                    int nsb, const ReconFrame* frames_d, hipStream_t s);
 // Cost in bits of symbol s of a raw AV1 CDF of nsym symbols (nsym - 1 values), through the
 // reference's cost_q_table (host).
 float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s);

@@ -103,6 +103,7 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
         const int lw = __builtin_ctz(bw);
         // ---- phase 1: error blocks and the plain SSE (sse_recon / chroma_sse)
         int64_t sse = 0;
+#pragma unroll 8
         for (int p = lane; p < bw * bh; p += 64) {
             const int y = p >> lw, x = p & (bw - 1);
             int e = 0;
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
         // ---- phase 2: SATD of the reconstruction error, SAD + SATD/4 of the prediction error
         uint64_t satd = 0, psatd = 0, sad = 0;
         const int tiles = (bw >> 2) * (bh >> 2);
+#pragma unroll 4
         for (int t = lane; t < tiles; t += 64) {
             const int ty = (t / (bw >> 2)) * 4, tx = (t % (bw >> 2)) * 4;
             satd += tile_satd(s_e + ty * kES + tx, kES);

@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     if (i >= n) return;
     const mt_cand c = cands[i];
     const int w = tx_w(c.tx_size), h = tx_h(c.tx_size);
-    const int si = c.planes & 3, ri = (c.planes >> 2) & 3;
+    const int si = (c.planes & 3) | ((c.planes_hi & 15) << 2);
+    const int ri = ((c.planes >> 2) & 3) | ((c.planes_hi >> 4) << 2);
     const int bdi = (c.bd_qm >> 4) & 3;
     const int bd = bdi == 0 ? 8 : bdi == 1 ? 10 : 12;
     bool ok = w > 0 && bdi < 3 && si < planes.n && ri < planes.n &&

@@ -447,7 +447,7 @@ int pred_init_device(int /*device*/) {
 
 int planes_from_host(const mt_plane* planes, int nplanes, const char* who, PlaneSet* ps) {
     if (nplanes < 1 || nplanes > MT_MAX_PLANES || !planes)
-        return set_error(MT_ERR_ARG, std::string(who) + ": 1..4 planes required");
+        return set_error(MT_ERR_ARG, std::string(who) + ": 1..32 planes required");
     for (int i = 0; i < nplanes; ++i) {
         const mt_plane& p = planes[i];
         if (!p.data || p.stride <= 0 || p.width <= 0 || p.height <= 0 || p.width > p.stride ||

@@ -97,14 +97,17 @@ struct FrameArgs {
     int W, H, C;
     int bd, qidx, qm, flags, minsz;
     float lambda, wsse;
+    int frame;     // index in the batch: planes 2 * frame (source) and 2 * frame + 1 (recon)
+    uint32_t sb0;  // index of the frame's first SB in the batch's SB arrays
 };
 
-// One SB of a step: its raster index and where its candidates / pool slots start.
+// One SB of a step: its frame, its raster index in the frame and where its candidates /
+// pool slots start.
 struct SbTask {
     uint32_t sb;
     uint32_t cand_base;
     uint32_t pool_base;
-    uint32_t pad;
+    uint32_t frame;
 };
 
 // Flags of every node of SB (r, c) into LDS, level by level (a node's search depends on its
@@ -128,12 +131,15 @@ __device__ void node_flags(const FrameArgs& f, int r, int c, uint8_t* fl) {
     }
 }
 
-// Number of mode ids node `id` (flags fl) evaluates under mask m.
+// Number of mode ids node `id` (flags fl) evaluates under mask m: the allowed intra modes
+// (DC always), 6 angle deltas per allowed directional mode at >= 8x8, 5 filter-intra modes.
 __device__ inline int node_count(int id, uint8_t fl, uint16_t m, int flags) {
     if (!(fl & kNoneOk)) return 0;
     const int s = 64 >> node_level(id);
-    int k = 0;
-    for (int j = 0; j < kModeIds; ++j) k += (id_valid(j, s, flags) && id_allowed(j, m)) ? 1 : 0;
+    const uint32_t mm = (uint32_t)m | 1u;
+    int k = __popc(mm & 0x1FFFu);
+    if ((flags & MT_RDO_ANGLE_DELTAS) && s >= 8) k += 6 * __popc(mm & 0x1FEu);
+    if ((flags & MT_RDO_FILTER_INTRA) && s <= 32 && (mm & (1u << 13))) k += 5;
     return k;
 }
 
@@ -176,7 +182,9 @@ __device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, in
                         (lm >= MT_PRED_SMOOTH && lm <= MT_PRED_SMOOTH_H);
     c.pflags = (uint8_t)((have_tr(x - x0, y - y0, s) ? MT_PRED_HAVE_TR : 0) |
                          (have_bl(x - x0, y - y0, s) ? MT_PRED_HAVE_BL : 0) | (smooth ? MT_PRED_SMOOTH_NB : 0));
-    c.planes = 0 | (1 << 2);
+    const int sp = 2 * f.frame, rp = 2 * f.frame + 1;
+    c.planes = (uint8_t)((sp & 3) | ((rp & 3) << 2));
+    c.planes_hi = (uint8_t)((sp >> 2) | ((rp >> 2) << 4));
     c.qidx = (uint8_t)f.qidx;
     const int bdi = f.bd == 8 ? 0 : f.bd == 10 ? 1 : 2;
     c.bd_qm = (uint8_t)(f.qm | (bdi << 4));
@@ -203,11 +211,17 @@ __device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, in
     return c;
 }
 
-// Candidate count and pool elements of every SB of the frame.
-__global__ __launch_bounds__(64) void rdo_count_kernel(FrameArgs f, uint32_t* __restrict__ counts,
+// Candidate count and pool elements of every SB of the batch (blockIdx = position in the
+// batch's SB list).
+__global__ __launch_bounds__(64) void rdo_count_kernel(const FrameArgs* __restrict__ frames,
+                                                       const SbTask* __restrict__ all,
+                                                       uint32_t* __restrict__ counts,
                                                        uint32_t* __restrict__ pools) {
     __shared__ uint8_t fl[kNodes];
-    const int sb = blockIdx.x, lane = threadIdx.x;
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const SbTask t = all[g];
+    const FrameArgs& f = frames[t.frame];
+    const int sb = (int)t.sb;
     const int r = sb / f.C, c = sb % f.C;
     node_flags(f, r, c, fl);
     const uint16_t* mask = f.prune ? f.prune + (size_t)sb * kNodes : nullptr;
@@ -223,8 +237,8 @@ __global__ __launch_bounds__(64) void rdo_count_kernel(FrameArgs f, uint32_t* __
         pool += __shfl_xor(pool, o);
     }
     if (lane == 0) {
-        counts[sb] = cnt;
-        pools[sb] = pool;
+        counts[g] = cnt;
+        pools[g] = pool;
     }
 }
 
@@ -237,12 +251,13 @@ struct NodePlan {
 };
 
 // One wave per SB of the step: node flags, per-node counts and their prefix in node order.
-__global__ __launch_bounds__(64) void rdo_plan_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+__global__ __launch_bounds__(64) void rdo_plan_kernel(const FrameArgs* __restrict__ frames, const SbTask* __restrict__ tasks,
                                                       NodePlan* __restrict__ plan) {
     __shared__ uint8_t fl[kNodes];
     __shared__ uint32_t cnt[kNodes], pl[kNodes];
     const int local = blockIdx.x, lane = threadIdx.x;
     const SbTask t = tasks[local];
+    const FrameArgs& f = frames[t.frame];
     const int r = t.sb / f.C, c = t.sb % f.C;
     node_flags(f, r, c, fl);
     const uint16_t* mask = f.prune ? f.prune + (size_t)t.sb * kNodes : nullptr;
@@ -253,14 +268,34 @@ __global__ __launch_bounds__(64) void rdo_plan_kernel(FrameArgs f, const SbTask*
         pl[id] = cnt[id] * (uint32_t)(s * s);
     }
     __syncthreads();
-    if (lane == 0) {  // exclusive prefix in node order
-        uint32_t a = t.cand_base, b = t.pool_base;
-        for (int id = 0; id < kNodes; ++id) {
-            const uint32_t ca = cnt[id], pa = pl[id];
-            cnt[id] = a;
-            pl[id] = b;
-            a += ca;
-            b += pa;
+    {  // exclusive prefix in node order: lane l sums nodes 6l .. 6l + 5, then a wave scan
+        uint32_t a = 0, b = 0;
+        for (int j = 0; j < 6; ++j) {
+            const int id = 6 * lane + j;
+            if (id < kNodes) {
+                a += cnt[id];
+                b += pl[id];
+            }
+        }
+        uint32_t ia = a, ib = b;  // inclusive scan over lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t ua = __shfl_up(ia, o), ub = __shfl_up(ib, o);
+            if (lane >= o) {
+                ia += ua;
+                ib += ub;
+            }
+        }
+        a = t.cand_base + ia - a;
+        b = t.pool_base + ib - b;
+        for (int j = 0; j < 6; ++j) {
+            const int id = 6 * lane + j;
+            if (id < kNodes) {
+                const uint32_t ca = cnt[id], pa = pl[id];
+                cnt[id] = a;
+                pl[id] = b;
+                a += ca;
+                b += pa;
+            }
         }
     }
     __syncthreads();
@@ -278,7 +313,7 @@ __global__ __launch_bounds__(64) void rdo_plan_kernel(FrameArgs f, const SbTask*
 // Writes the candidates of the step: one wave per (SB, node); lane j takes mode ids j and
 // j + 64 and the allowed ones are compacted in id order.  refs[k] = (local SB << 16) |
 // (node << 7) | id.
-__global__ __launch_bounds__(64) void rdo_gen_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+__global__ __launch_bounds__(64) void rdo_gen_kernel(const FrameArgs* __restrict__ frames, const SbTask* __restrict__ tasks,
                                                      const NodePlan* __restrict__ plan,
                                                      mt_cand* __restrict__ cands,
                                                      uint32_t* __restrict__ refs) {
@@ -286,6 +321,7 @@ __global__ __launch_bounds__(64) void rdo_gen_kernel(FrameArgs f, const SbTask* 
     const NodePlan p = plan[blockIdx.x];
     if (p.count == 0) return;
     const SbTask t = tasks[local];
+    const FrameArgs& f = frames[t.frame];
     const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
     const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
     const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
@@ -335,7 +371,7 @@ __global__ void __launch_bounds__(256) rdo_argmin_kernel(const uint32_t* __restr
 // Decision of one SB per wave: bottom-up NONE / SPLIT level by level (lanes over the
 // level's nodes), the leaves in coding order (lane 0 walks the tree in LDS), the final maps
 // (lanes over the SB's 4x4 and 8x8 units) and the leaf descriptors (lanes over leaves).
-__global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTask* __restrict__ tasks,
+__global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restrict__ frames, const SbTask* __restrict__ tasks,
                                                         const unsigned long long* __restrict__ keys,
                                                         const NodePlan* __restrict__ plan,
                                                         mt_predblk* __restrict__ leaves,
@@ -350,6 +386,7 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTas
     __shared__ int s_nl;
     const int local = blockIdx.x, lane = threadIdx.x;
     const SbTask t = tasks[local];
+    const FrameArgs& f = frames[t.frame];
     const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
     mt_rdo_node* out = f.nodes + (size_t)t.sb * kNodes;
     const float inf = __int_as_float(0x7f800000);
@@ -447,6 +484,7 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTas
         p.angle_delta = cd.angle_delta;
         p.flags = cd.pflags;
         p.bd = (uint8_t)f.bd;
+        p.plane = (uint8_t)f.frame;
         leaves[(size_t)local * kMaxLeaves + i] = p;
         pool += (uint32_t)(s * s);
     }
@@ -454,20 +492,26 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(FrameArgs f, const SbTas
     if (lane == 0) {
         first[local] = (uint32_t)local * kMaxLeaves;
         count[local] = (uint32_t)nl;
-        sb_leaves[t.sb] = (uint32_t)nl;
-        sb_leaf_pool[t.sb] = pool;
+        sb_leaves[f.sb0 + t.sb] = (uint32_t)nl;
+        sb_leaf_pool[f.sb0 + t.sb] = pool;
     }
 }
 
-// The SB's leaves with the final contexts, at leaf_base[sb] / pool_base[sb] (one thread per SB).
-__global__ void __launch_bounds__(64) rdo_final_gen_kernel(FrameArgs f, int nsb, const uint32_t* __restrict__ leaf_base,
+// The SB's leaves with the final contexts, at leaf_base[g] / pool_base[g] (one thread per SB
+// of the batch's SB list).
+__global__ void __launch_bounds__(64) rdo_final_gen_kernel(const FrameArgs* __restrict__ frames,
+                                                          const SbTask* __restrict__ all, int nall,
+                                                          const uint32_t* __restrict__ leaf_base,
                                                           const uint32_t* __restrict__ pool_base,
                                                           mt_cand* __restrict__ cands) {
-    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= nsb) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nall) return;
+    const SbTask t = all[g];
+    const FrameArgs& f = frames[t.frame];
+    const int sb = (int)t.sb;
     const int r = sb / f.C, c = sb % f.C, x0 = c * 64, y0 = r * 64;
     const mt_rdo_node* nd = f.nodes + (size_t)sb * kNodes;
-    uint32_t k = leaf_base[sb], p = pool_base[sb];
+    uint32_t k = leaf_base[g], p = pool_base[g];
     uint16_t stack[32];
     int sp = 0;
     stack[sp++] = 0;
@@ -488,15 +532,20 @@ __global__ void __launch_bounds__(64) rdo_final_gen_kernel(FrameArgs f, int nsb,
 
 // Per SB, in coding order: partition symbol bits of every coded node inside the frame and
 // the re-priced bits and SSE of every leaf (f64 sum of the f32 values; i64 SSE).
-__global__ void __launch_bounds__(64) rdo_final_sum_kernel(FrameArgs f, int nsb, const uint32_t* __restrict__ leaf_base,
+__global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __restrict__ frames,
+                                                          const SbTask* __restrict__ all, int nall,
+                                                          const uint32_t* __restrict__ leaf_base,
                                                           const mt_candres* __restrict__ res,
                                                           double* __restrict__ sb_bits,
                                                           long long* __restrict__ sb_sse) {
-    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= nsb) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nall) return;
+    const SbTask t = all[g];
+    const FrameArgs& f = frames[t.frame];
+    const int sb = (int)t.sb;
     const int r = sb / f.C, c = sb % f.C, x0 = c * 64, y0 = r * 64;
     const mt_rdo_node* nd = f.nodes + (size_t)sb * kNodes;
-    uint32_t k = leaf_base[sb];
+    uint32_t k = leaf_base[g];
     double b = 0.0;
     long long e = 0;
     uint16_t stack[32];
@@ -518,18 +567,19 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(FrameArgs f, int nsb,
         e += res[k].sse;
         ++k;
     }
-    sb_bits[sb] = b;
-    sb_sse[sb] = e;
+    sb_bits[g] = b;
+    sb_sse[g] = e;
 }
 
 // Library-owned buffers of one device, grown on demand.
 struct RdoWork {
-    size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map4 = 0, cap_map8 = 0;
+    size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map = 0, cap_frames = 0;
     uint32_t *counts = nullptr, *pools = nullptr, *sb_leaves = nullptr, *sb_leaf_pool = nullptr;
     uint32_t *leaf_base = nullptr, *pool_base = nullptr;
     double* sb_bits = nullptr;
     long long* sb_sse = nullptr;
-    SbTask* tasks = nullptr;
+    SbTask* all = nullptr;    // the batch's SBs, frame by frame, raster order
+    SbTask* tasks = nullptr;  // the same SBs in step order
     unsigned long long* keys = nullptr;
     NodePlan* plan = nullptr;
     mt_predblk* leaves = nullptr;
@@ -539,7 +589,9 @@ struct RdoWork {
     mt_candres* cres = nullptr;
     mt_cand* fcands = nullptr;
     mt_candres* fres = nullptr;
-    uint8_t *mode4 = nullptr, *part8 = nullptr;
+    uint8_t* maps = nullptr;  // every frame's mode and partition maps
+    FrameArgs* fargs = nullptr;
+    mtlib::ReconFrame* rframes = nullptr;
 };
 std::mutex g_rdo_mu;
 RdoWork g_rdo[64];
@@ -548,6 +600,242 @@ template <typename T> int regrow(T*& p, size_t n) {
     if (p) MT_HIP_TRY(hipFree(p));
     p = nullptr;
     MT_HIP_TRY(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    return MT_OK;
+}
+
+// Frames of one batch: the eval's plane set holds two planes per frame.
+constexpr int kBatchFrames = MT_MAX_PLANES / 2;
+
+int check_frame(const mt_rdo_frame& fr) {
+    const int W = fr.src.width, H = fr.src.height;
+    if (!fr.src.data || !fr.recon || !fr.nodes || W <= 0 || H <= 0 || (W & 7) || (H & 7) || W > 65535 ||
+        H > 65535 || fr.src.stride < W || fr.recon_stride < W)
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad planes (sizes must be multiples of 8)");
+    if ((fr.bd != 8 && fr.bd != 10 && fr.bd != 12) || fr.qm_level > 15 || !(fr.lambda > 0.0f) ||
+        !std::isfinite(fr.lambda) || !(fr.wsse_scale >= 0.0f) || !std::isfinite(fr.wsse_scale))
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad parameters");
+    return MT_OK;
+}
+
+// The wavefront of up to kBatchFrames frames at once: step k covers step k of every frame.
+int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_stats* stats, int cus,
+              hipStream_t s) {
+    int rc = MT_OK;
+    std::vector<FrameArgs> fa(nf);
+    std::vector<mtlib::ReconFrame> rf(nf);
+    std::vector<int> C(nf), Rr(nf), nsteps(nf);
+    size_t map_bytes = 0;
+    uint32_t nall = 0;
+    int max_steps = 0;
+    for (int i = 0; i < nf; ++i) {
+        const mt_rdo_frame& fr = frames[i];
+        const int W = fr.src.width, H = fr.src.height;
+        C[i] = (W + 63) / 64;
+        Rr[i] = (H + 63) / 64;
+        nsteps[i] = (C[i] - 1) + 2 * (Rr[i] - 1) + 1;
+        max_steps = std::max(max_steps, nsteps[i]);
+        FrameArgs& f = fa[i];
+        f = FrameArgs{};
+        f.src = fr.src;
+        f.src.vis_w = W;
+        f.src.vis_h = H;
+        f.prune = fr.prune;
+        f.nodes = (mt_rdo_node*)fr.nodes;
+        f.W = W;
+        f.H = H;
+        f.C = C[i];
+        f.bd = fr.bd;
+        f.qidx = fr.qidx;
+        f.qm = fr.qm_level;
+        f.flags = flags;
+        f.minsz = (flags & MT_RDO_MIN4) ? 4 : 8;
+        f.lambda = fr.lambda;
+        f.wsse = fr.wsse_scale;
+        f.frame = i;
+        f.sb0 = nall;
+        nall += (uint32_t)(C[i] * Rr[i]);
+        map_bytes += (size_t)(W / 4) * (H / 4) + (size_t)(W / 8) * (H / 8);
+        const int bdi = fr.bd == 8 ? 0 : fr.bd == 10 ? 1 : 2;
+        rf[i] = mtlib::ReconFrame{f.src, fr.recon, fr.recon_stride, fr.qidx, fr.qm_level | (bdi << 4), 0};
+    }
+    if (w.cap_sb < nall) {
+        if ((rc = regrow(w.counts, nall)) || (rc = regrow(w.pools, nall)) || (rc = regrow(w.sb_leaves, nall)) ||
+            (rc = regrow(w.sb_leaf_pool, nall)) || (rc = regrow(w.leaf_base, nall)) ||
+            (rc = regrow(w.pool_base, nall)) || (rc = regrow(w.sb_bits, nall)) || (rc = regrow(w.sb_sse, nall)) ||
+            (rc = regrow(w.all, nall)) || (rc = regrow(w.tasks, nall)))
+            return rc;
+        w.cap_sb = nall;
+    }
+    if (w.cap_map < map_bytes) {
+        if ((rc = regrow(w.maps, map_bytes))) return rc;
+        w.cap_map = map_bytes;
+    }
+    if (w.cap_frames < (size_t)nf) {
+        if ((rc = regrow(w.fargs, kBatchFrames)) || (rc = regrow(w.rframes, kBatchFrames))) return rc;
+        w.cap_frames = kBatchFrames;
+    }
+    {
+        size_t off = 0;
+        for (int i = 0; i < nf; ++i) {
+            fa[i].mode4 = w.maps + off;
+            off += (size_t)(fa[i].W / 4) * (fa[i].H / 4);
+            fa[i].part8 = w.maps + off;
+            off += (size_t)(fa[i].W / 8) * (fa[i].H / 8);
+        }
+    }
+    std::vector<SbTask> all(nall);
+    for (int i = 0; i < nf; ++i)
+        for (int sb = 0; sb < C[i] * Rr[i]; ++sb) all[fa[i].sb0 + sb] = SbTask{(uint32_t)sb, 0, 0, (uint32_t)i};
+    MT_HIP_TRY(hipMemcpyAsync(w.fargs, fa.data(), nf * sizeof(FrameArgs), hipMemcpyHostToDevice, s));
+    MT_HIP_TRY(hipMemcpyAsync(w.rframes, rf.data(), nf * sizeof(mtlib::ReconFrame), hipMemcpyHostToDevice, s));
+    MT_HIP_TRY(hipMemcpyAsync(w.all, all.data(), nall * sizeof(SbTask), hipMemcpyHostToDevice, s));
+    PlaneSet ps{};
+    ps.n = 2 * nf;
+    for (int i = 0; i < nf; ++i) {
+        ps.p[2 * i] = fa[i].src;
+        ps.p[2 * i + 1] = mt_plane{frames[i].recon, frames[i].recon_stride, fa[i].W, fa[i].H, fa[i].W, fa[i].H, 0};
+    }
+
+    // each reconstruction starts as its source (the decision's stand-in inside each SB)
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        for (int i = 0; i < nf; ++i) {
+            const size_t npx = (size_t)fa[i].W * fa[i].H;
+            hipLaunchKernelGGL(rdo_copy_plane_kernel, dim3(std::max(1, std::min(cus * 4, (int)((npx + 255) / 256)))),
+                               dim3(256), 0, s, fa[i].src, frames[i].recon, frames[i].recon_stride);
+        }
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(std::max(1, std::min(cus * 4, (int)((map_bytes + 255) / 256)))),
+                           dim3(256), 0, s, w.maps, map_bytes);
+        hipLaunchKernelGGL(rdo_count_kernel, dim3(nall), dim3(64), 0, s, w.fargs, w.all, w.counts, w.pools);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    std::vector<uint32_t> counts(nall), pools(nall);
+    MT_HIP_TRY(hipMemcpyAsync(counts.data(), w.counts, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(pools.data(), w.pools, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+
+    // steps: the SBs of step k of every frame, their candidate / pool bases
+    std::vector<SbTask> tasks;
+    tasks.reserve(nall);
+    std::vector<int> step_first(max_steps + 1);
+    std::vector<uint32_t> step_n(max_steps), step_pool(max_steps);
+    std::vector<int64_t> frame_cands(nf, 0);
+    size_t max_cand = 0, max_sb = 0;
+    for (int k = 0; k < max_steps; ++k) {
+        step_first[k] = (int)tasks.size();
+        uint64_t nc = 0, np = 0;
+        for (int i = 0; i < nf; ++i) {
+            for (int r = 0; r < Rr[i]; ++r) {
+                const int c = k - 2 * r;
+                if (c < 0 || c >= C[i]) continue;
+                const uint32_t sb = (uint32_t)(r * C[i] + c), g = fa[i].sb0 + sb;
+                tasks.push_back(SbTask{sb, (uint32_t)nc, (uint32_t)np, (uint32_t)i});
+                nc += counts[g];
+                np += pools[g];
+                frame_cands[i] += counts[g];
+            }
+        }
+        if (np > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
+        step_n[k] = (uint32_t)nc;
+        step_pool[k] = (uint32_t)np;
+        max_cand = std::max(max_cand, (size_t)nc);
+        max_sb = std::max(max_sb, tasks.size() - step_first[k]);
+    }
+    step_first[max_steps] = (int)tasks.size();
+    if (max_sb >= 65536) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
+    if (w.cap_step_sb < max_sb) {
+        if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.plan, max_sb * kNodes)) ||
+            (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
+            (rc = regrow(w.count, max_sb)))
+            return rc;
+        w.cap_step_sb = max_sb;
+    }
+    if (w.cap_cand < max_cand) {
+        if ((rc = regrow(w.cands, max_cand)) || (rc = regrow(w.refs, max_cand)) || (rc = regrow(w.cres, max_cand)))
+            return rc;
+        w.cap_cand = max_cand;
+    }
+    MT_HIP_TRY(hipMemcpyAsync(w.tasks, tasks.data(), tasks.size() * sizeof(SbTask), hipMemcpyHostToDevice, s));
+
+    for (int k = 0; k < max_steps; ++k) {
+        const int nk = step_first[k + 1] - step_first[k];
+        if (nk == 0) continue;
+        const SbTask* tk = w.tasks + step_first[k];
+        const int n = (int)step_n[k];
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
+                               w.keys, (size_t)nk * kNodes, ~0ull);
+            hipLaunchKernelGGL(rdo_plan_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.plan);
+            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk * kNodes), dim3(64), 0, s, w.fargs, tk, w.plan, w.cands,
+                               w.refs);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        if ((rc = mtlib::eval_launch(w.cands, n, ps, step_pool[k], w.cres, s))) return rc;
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            if (n > 0)
+                hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n, w.keys);
+            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.keys, w.plan, w.leaves,
+                               w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        {
+            mtlib::ProfScope prof(mtlib::kProfRecon, s);
+            if ((rc = mtlib::sb_recon_launch(w.leaves, w.first, w.count, nk, w.rframes, s))) return rc;
+        }
+    }
+
+    // final accounting with the final contexts
+    std::vector<uint32_t> nleaf(nall), lpool(nall), lbase(nall), pbase(nall);
+    MT_HIP_TRY(hipMemcpyAsync(nleaf.data(), w.sb_leaves, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(lpool.data(), w.sb_leaf_pool, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+    uint64_t tl = 0, tp = 0;
+    for (uint32_t i = 0; i < nall; ++i) {
+        lbase[i] = (uint32_t)tl;
+        pbase[i] = (uint32_t)tp;
+        tl += nleaf[i];
+        tp += lpool[i];
+    }
+    if (tp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch too large");
+    if (w.cap_final < tl) {
+        if ((rc = regrow(w.fcands, tl)) || (rc = regrow(w.fres, tl))) return rc;
+        w.cap_final = tl;
+    }
+    MT_HIP_TRY(hipMemcpyAsync(w.leaf_base, lbase.data(), nall * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    MT_HIP_TRY(hipMemcpyAsync(w.pool_base, pbase.data(), nall * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        hipLaunchKernelGGL(rdo_final_gen_kernel, dim3((nall + 63) / 64), dim3(64), 0, s, w.fargs, w.all, (int)nall,
+                           w.leaf_base, w.pool_base, w.fcands);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    if ((rc = mtlib::eval_launch(w.fcands, (int)tl, ps, tp, w.fres, s))) return rc;
+    {
+        mtlib::ProfScope prof(mtlib::kProfRdo, s);
+        hipLaunchKernelGGL(rdo_final_sum_kernel, dim3((nall + 63) / 64), dim3(64), 0, s, w.fargs, w.all, (int)nall,
+                           w.leaf_base, w.fres, w.sb_bits, w.sb_sse);
+        MT_HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> bits(nall);
+    std::vector<long long> sse(nall);
+    MT_HIP_TRY(hipMemcpyAsync(bits.data(), w.sb_bits, nall * sizeof(double), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(sse.data(), w.sb_sse, nall * sizeof(long long), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < nf; ++i) {
+        mt_rdo_stats& st = stats[i];
+        std::memset(&st, 0, sizeof(st));
+        const uint32_t n = (uint32_t)(C[i] * Rr[i]);
+        for (uint32_t sb = 0; sb < n; ++sb) {
+            st.bits += bits[fa[i].sb0 + sb];
+            st.sse += sse[fa[i].sb0 + sb];
+            st.leaves += (int32_t)nleaf[fa[i].sb0 + sb];
+        }
+        st.candidates = frame_cands[i];
+        st.sbs = (int32_t)n;
+        st.steps = nsteps[i];
+    }
     return MT_OK;
 }
 
@@ -609,198 +897,27 @@ extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
     return m < 0.0f ? m : m / 2.775f;  // K_PART_LAMBDA (src/cost.rs:278)
 }
 
-extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream) {
-    if (!frame || !stats) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
-    const mt_rdo_frame& fr = *frame;
-    const int W = fr.src.width, H = fr.src.height;
-    if (!fr.src.data || !fr.recon || !fr.nodes || W <= 0 || H <= 0 || (W & 7) || (H & 7) ||
-        W > 65535 || H > 65535 || fr.src.stride < W || fr.recon_stride < W)
-        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad planes (sizes must be multiples of 8)");
-    if ((fr.bd != 8 && fr.bd != 10 && fr.bd != 12) || fr.qm_level > 15 || !(fr.lambda > 0.0f) ||
-        !std::isfinite(fr.lambda) || !(fr.wsse_scale >= 0.0f) || !std::isfinite(fr.wsse_scale) ||
-        (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4)))
-        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad parameters");
+extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
+                                  void* stream) {
+    if (!frames || !stats || nframes < 0) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
+    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4))
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad flags");
+    int rc = MT_OK;
+    for (int i = 0; i < nframes; ++i)
+        if ((rc = check_frame(frames[i]))) return rc;
+    if (nframes == 0) return MT_OK;
     int dev = 0, cus = 0;
-    int rc = mtlib::device_ready(&dev, &cus);
+    rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_rdo_mu);
     RdoWork& w = g_rdo[dev];
-    hipStream_t s = (hipStream_t)stream;
-    const int C = (W + 63) / 64, R = (H + 63) / 64, nsb = C * R, nsteps = (C - 1) + 2 * (R - 1) + 1;
-    const size_t n4 = (size_t)(W / 4) * (H / 4), n8 = (size_t)(W / 8) * (H / 8);
-    if (w.cap_sb < (size_t)nsb) {
-        if ((rc = regrow(w.counts, nsb)) || (rc = regrow(w.pools, nsb)) || (rc = regrow(w.sb_leaves, nsb)) ||
-            (rc = regrow(w.sb_leaf_pool, nsb)) || (rc = regrow(w.leaf_base, nsb)) ||
-            (rc = regrow(w.pool_base, nsb)) || (rc = regrow(w.sb_bits, nsb)) || (rc = regrow(w.sb_sse, nsb)) ||
-            (rc = regrow(w.tasks, nsb)))
-            return rc;
-        w.cap_sb = nsb;
+    for (int i0 = 0; i0 < nframes; i0 += kBatchFrames) {
+        const int nf = std::min(kBatchFrames, nframes - i0);
+        if ((rc = run_batch(w, frames + i0, nf, flags, stats + i0, cus, (hipStream_t)stream))) return rc;
     }
-    if (w.cap_map4 < n4) {
-        if ((rc = regrow(w.mode4, n4))) return rc;
-        w.cap_map4 = n4;
-    }
-    if (w.cap_map8 < n8) {
-        if ((rc = regrow(w.part8, n8))) return rc;
-        w.cap_map8 = n8;
-    }
-    FrameArgs f{};
-    f.src = fr.src;
-    f.src.vis_w = W;
-    f.src.vis_h = H;
-    f.prune = fr.prune;
-    f.nodes = (mt_rdo_node*)fr.nodes;
-    f.mode4 = w.mode4;
-    f.part8 = w.part8;
-    f.W = W;
-    f.H = H;
-    f.C = C;
-    f.bd = fr.bd;
-    f.qidx = fr.qidx;
-    f.qm = fr.qm_level;
-    f.flags = flags;
-    f.minsz = (flags & MT_RDO_MIN4) ? 4 : 8;
-    f.lambda = fr.lambda;
-    f.wsse = fr.wsse_scale;
-    PlaneSet ps{};
-    ps.n = 2;
-    ps.p[0] = f.src;
-    ps.p[1] = mt_plane{fr.recon, fr.recon_stride, W, H, W, H, 0};
-    const int bdi = fr.bd == 8 ? 0 : fr.bd == 10 ? 1 : 2;
-    const int bd_qm = fr.qm_level | (bdi << 4);
-    const int gfill = std::max(1, std::min(cus * 4, (int)((std::max(n4, (size_t)W * H) + 255) / 256)));
-
-    // the reconstruction starts as the source (the decision's stand-in inside each SB)
-    {
-        mtlib::ProfScope prof(mtlib::kProfRdo, s);
-        hipLaunchKernelGGL(rdo_copy_plane_kernel, dim3(gfill), dim3(256), 0, s, f.src, fr.recon, fr.recon_stride);
-        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(gfill), dim3(256), 0, s, w.mode4, n4);
-        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(gfill), dim3(256), 0, s, w.part8, n8);
-        hipLaunchKernelGGL(rdo_count_kernel, dim3(nsb), dim3(64), 0, s, f, w.counts, w.pools);
-        MT_HIP_TRY(hipGetLastError());
-    }
-    std::vector<uint32_t> counts(nsb), pools(nsb);
-    MT_HIP_TRY(hipMemcpyAsync(counts.data(), w.counts, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipMemcpyAsync(pools.data(), w.pools, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipStreamSynchronize(s));
-
-    // steps: SB lists and candidate / pool bases
-    std::vector<SbTask> tasks;
-    tasks.reserve(nsb);
-    std::vector<int> step_first(nsteps + 1);
-    std::vector<uint32_t> step_n(nsteps), step_pool(nsteps);
-    size_t max_cand = 0, max_sb = 0;
-    uint64_t max_pool = 0;
-    int64_t total_cands = 0;
-    for (int k = 0; k < nsteps; ++k) {
-        step_first[k] = (int)tasks.size();
-        uint64_t nc = 0, np = 0;
-        for (int r = 0; r < R; ++r) {
-            const int c = k - 2 * r;
-            if (c < 0 || c >= C) continue;
-            const int sb = r * C + c;
-            tasks.push_back(SbTask{(uint32_t)sb, (uint32_t)nc, (uint32_t)np, 0});
-            nc += counts[sb];
-            np += pools[sb];
-        }
-        if (np > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: frame too large");
-        step_n[k] = (uint32_t)nc;
-        step_pool[k] = (uint32_t)np;
-        total_cands += (int64_t)nc;
-        max_cand = std::max(max_cand, (size_t)nc);
-        max_pool = std::max(max_pool, np);
-        max_sb = std::max(max_sb, tasks.size() - step_first[k]);
-    }
-    step_first[nsteps] = (int)tasks.size();
-    if (w.cap_step_sb < max_sb) {
-        if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.plan, max_sb * kNodes)) ||
-            (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
-            (rc = regrow(w.count, max_sb)))
-            return rc;
-        w.cap_step_sb = max_sb;
-    }
-    if (w.cap_cand < max_cand) {
-        if ((rc = regrow(w.cands, max_cand)) || (rc = regrow(w.refs, max_cand)) || (rc = regrow(w.cres, max_cand)))
-            return rc;
-        w.cap_cand = max_cand;
-    }
-    MT_HIP_TRY(hipMemcpyAsync(w.tasks, tasks.data(), tasks.size() * sizeof(SbTask), hipMemcpyHostToDevice, s));
-
-    for (int k = 0; k < nsteps; ++k) {
-        const int nk = step_first[k + 1] - step_first[k];
-        if (nk == 0) continue;
-        const SbTask* tk = w.tasks + step_first[k];
-        const int n = (int)step_n[k];
-        {
-            mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
-                               w.keys, (size_t)nk * kNodes, ~0ull);
-            hipLaunchKernelGGL(rdo_plan_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.plan);
-            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk * kNodes), dim3(64), 0, s, f, tk, w.plan, w.cands, w.refs);
-            MT_HIP_TRY(hipGetLastError());
-        }
-        if ((rc = mtlib::eval_launch(w.cands, n, ps, step_pool[k], w.cres, s))) return rc;
-        {
-            mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            if (n > 0)
-                hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n, w.keys);
-            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, f, tk, w.keys, w.plan, w.leaves,
-                               w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
-            MT_HIP_TRY(hipGetLastError());
-        }
-        {
-            mtlib::ProfScope prof(mtlib::kProfRecon, s);
-            if ((rc = mtlib::sb_recon_launch(w.leaves, w.first, w.count, nk, f.src, fr.recon, fr.recon_stride,
-                                             fr.qidx, bd_qm, s)))
-                return rc;
-        }
-    }
-
-    // final accounting with the final contexts
-    std::vector<uint32_t> nleaf(nsb), lpool(nsb), lbase(nsb), pbase(nsb);
-    MT_HIP_TRY(hipMemcpyAsync(nleaf.data(), w.sb_leaves, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipMemcpyAsync(lpool.data(), w.sb_leaf_pool, nsb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipStreamSynchronize(s));
-    uint64_t tl = 0, tp = 0;
-    for (int i = 0; i < nsb; ++i) {
-        lbase[i] = (uint32_t)tl;
-        pbase[i] = (uint32_t)tp;
-        tl += nleaf[i];
-        tp += lpool[i];
-    }
-    if (tp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: frame too large");
-    if (w.cap_final < tl) {
-        if ((rc = regrow(w.fcands, tl)) || (rc = regrow(w.fres, tl))) return rc;
-        w.cap_final = tl;
-    }
-    MT_HIP_TRY(hipMemcpyAsync(w.leaf_base, lbase.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    MT_HIP_TRY(hipMemcpyAsync(w.pool_base, pbase.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    {
-        mtlib::ProfScope prof(mtlib::kProfRdo, s);
-        hipLaunchKernelGGL(rdo_final_gen_kernel, dim3((nsb + 63) / 64), dim3(64), 0, s, f, nsb, w.leaf_base,
-                           w.pool_base, w.fcands);
-        MT_HIP_TRY(hipGetLastError());
-    }
-    if ((rc = mtlib::eval_launch(w.fcands, (int)tl, ps, tp, w.fres, s))) return rc;
-    {
-        mtlib::ProfScope prof(mtlib::kProfRdo, s);
-        hipLaunchKernelGGL(rdo_final_sum_kernel, dim3((nsb + 63) / 64), dim3(64), 0, s, f, nsb, w.leaf_base, w.fres,
-                           w.sb_bits, w.sb_sse);
-        MT_HIP_TRY(hipGetLastError());
-    }
-    std::vector<double> bits(nsb);
-    std::vector<long long> sse(nsb);
-    MT_HIP_TRY(hipMemcpyAsync(bits.data(), w.sb_bits, nsb * sizeof(double), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipMemcpyAsync(sse.data(), w.sb_sse, nsb * sizeof(long long), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipStreamSynchronize(s));
-    std::memset(stats, 0, sizeof(*stats));
-    for (int i = 0; i < nsb; ++i) {
-        stats->bits += bits[i];
-        stats->sse += sse[i];
-    }
-    stats->candidates = total_cands;
-    stats->leaves = (int32_t)tl;
-    stats->sbs = nsb;
-    stats->steps = nsteps;
     return MT_OK;
+}
+
+extern "C" int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream) {
+    return mt_intra_rdo_batch(frame, 1, flags, stats, stream);
 }

@@ -171,14 +171,13 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
 }
 
 // One workgroup (one wave) per superblock: leaves[first[b] .. first[b] + count[b]) in
-// coding order.  Each leaf descriptor carries its prediction (x, y, size, mode, angle
-// delta, availability and filter-type flags, bit depth); q holds the quantizer (qidx,
-// bd_qm) shared by the frame.  rec: the reconstruction plane (frame size of src).
+// coding order, all of one frame.  Each leaf descriptor carries its prediction (x, y, size,
+// mode, angle delta, availability and filter-type flags, bit depth) and, in `plane`, its
+// frame's index in `frames` (source, reconstruction plane of the frame size, quantizer).
 __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restrict__ leaves,
                                                       const uint32_t* __restrict__ first,
                                                       const uint32_t* __restrict__ count,
-                                                      mt_plane src, uint16_t* __restrict__ rec,
-                                                      int rstride, mt_txblk q) {
+                                                      const mtlib::ReconFrame* __restrict__ frames) {
     __shared__ int32_t s_pred[4096];
     __shared__ int32_t s_res[4096];
     __shared__ int32_t s_a[kEdge], s_l[kEdge];
@@ -187,6 +186,14 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
     __shared__ int32_t s_work[32 * 33];
     const int lane = threadIdx.x;
     const uint32_t f0 = first[blockIdx.x], n = count[blockIdx.x];
+    if (n == 0) return;
+    const mtlib::ReconFrame fr = frames[leaves[f0].plane];
+    const mt_plane src = fr.src;
+    uint16_t* __restrict__ rec = fr.rec;
+    const int rstride = fr.rstride;
+    mt_txblk q{};
+    q.qidx = (uint8_t)fr.qidx;
+    q.bd_qm = (uint8_t)fr.bd_qm;
     BlkParam prm;
     param_from_desc(q, prm, nullptr, 0);
     const int maxv = (1 << prm.bd) - 1;
@@ -221,14 +228,9 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
 
 namespace mtlib {
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
-                    int nsb, const mt_plane& src, uint16_t* rec, int rstride, int qidx, int bd_qm,
-                    hipStream_t s) {
+                    int nsb, const ReconFrame* frames_d, hipStream_t s) {
     if (nsb <= 0) return MT_OK;
-    mt_txblk q{};
-    q.qidx = (uint8_t)qidx;
-    q.bd_qm = (uint8_t)bd_qm;
-    hipLaunchKernelGGL(sb_recon_kernel, dim3(nsb), dim3(64), 0, s, leaves_d, first_d, count_d, src,
-                       rec, rstride, q);
+    hipLaunchKernelGGL(sb_recon_kernel, dim3(nsb), dim3(64), 0, s, leaves_d, first_d, count_d, frames_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -87,6 +87,51 @@ def intra_rdo(src, bd, qidx, qm_level=15, flags=ANGLE_DELTAS | FILTER_INTRA | MI
                           "leaves": st.leaves, "sbs": st.sbs, "steps": st.steps}
 
 
+def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale=1.0, stream=None):
+    """RDO of several frames in one call (mt_intra_rdo_batch).  frames: list of dicts with
+    src (16-bit cuda tensor (H, W)), bd, qidx and optionally qm_level (15), lam (the
+    reference's decision lambda for 4:2:0), prune, recon, nodes.  Returns a list of
+    (recon, nodes, stats) in the same order."""
+    import torch
+    n = len(frames)
+    arr = (MtRdoFrame * max(n, 1))()
+    outs = []
+    for i, fr in enumerate(frames):
+        src = fr["src"]
+        H, W = src.shape
+        dev = src.device
+        init(dev.index or 0)
+        nsb = ((W + 63) // 64) * ((H + 63) // 64)
+        rec = fr.get("recon")
+        if rec is None:
+            rec = torch.empty((H, W), dtype=src.dtype, device=dev)
+        nodes = fr.get("nodes")
+        if nodes is None:
+            nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        lam = fr.get("lam")
+        if lam is None:
+            lam = rdo_lambda(fr["qidx"], fr["bd"], 2)
+        f = arr[i]
+        f.src = MtPlane(_dev_ptr(src).value, src.stride(0), W, H, W, H, 0)
+        f.recon = _dev_ptr(rec).value
+        f.recon_stride = rec.stride(0)
+        f.bd, f.qidx, f.qm_level = fr["bd"], fr["qidx"], fr.get("qm_level", 15)
+        f.lambda_ = lam
+        f.wsse_scale = wsse_scale
+        pr = fr.get("prune")
+        if pr is not None:
+            if tuple(pr.shape) != (nsb, NODES):
+                raise _lib.MtError("prune must have shape (nsb, 341)")
+            f.prune = _dev_ptr(pr).value
+        f.nodes = _dev_ptr(nodes).value
+        outs.append((rec, nodes))
+    st = (MtRdoStats * max(n, 1))()
+    _check(lib().mt_intra_rdo_batch(arr, n, flags, st, _stream_ptr(stream)))
+    return [(outs[i][0], outs[i][1], {"bits": st[i].bits, "sse": st[i].sse, "candidates": st[i].candidates,
+                                       "leaves": st[i].leaves, "sbs": st[i].sbs, "steps": st[i].steps})
+            for i in range(n)]
+
+
 def nodes_to_np(nodes):
     """Device node records -> structured numpy array (nsb, 341) of NODE_DTYPE."""
     a = nodes.cpu().numpy().view(NODE_DTYPE)

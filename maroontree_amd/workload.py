@@ -40,7 +40,7 @@ def synth_image(seed, width, height, bd=10):
         img += amp * v
     for _ in range(20):
         cx, cy = rng.uniform(0, width), rng.uniform(0, height)
-        hw, hh = rng.uniform(8, width / 6), rng.uniform(8, height / 6)
+        hw, hh = rng.uniform(8, max(width / 6, 9)), rng.uniform(8, max(height / 6, 9))
         th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
         u = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
         v = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)

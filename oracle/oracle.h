@@ -179,7 +179,7 @@ typedef struct {
     uint8_t pflags, planes, qidx, bd_qm;
     uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;
     uint8_t txtp_set, txtp_mode, txtp, qctx;
-    uint8_t rflags, pad0;
+    uint8_t rflags, planes_hi; /* high bits of the plane indices (bits 0-3 source, 4-7 recon) */
     int8_t dc_delta, ac_delta; /* chroma (bd_qm bit 6) qindex deltas */
     float skip_one, skip_zero, wsse_scale, lambda, extra_bits;
 } orc_cand;

@@ -31,7 +31,9 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
         return;
     }
     const int w = orc_tx_w(s), h = orc_tx_h(s);
-    const int si = c.planes & 3, ri = (c.planes >> 2) & 3, bdi = (c.bd_qm >> 4) & 3;
+    const int si = (c.planes & 3) | ((c.planes_hi & 15) << 2);
+    const int ri = ((c.planes >> 2) & 3) | ((c.planes_hi >> 4) << 2);
+    const int bdi = (c.bd_qm >> 4) & 3;
     if (si >= nplanes || ri >= nplanes || bdi > 2) {
         r.status = -1;
         return;

@@ -83,6 +83,33 @@ def test_rdo_plain_sse_matches_oracle(dev):
     assert_same(*out)
 
 
+def test_rdo_batch_of_uneven_frames_matches_oracle(dev):
+    """Frames of different sizes, bit depths, quantizers and prune masks in one call: each
+    frame's result equals the oracle's for that frame alone."""
+    rng = np.random.default_rng(17)
+    specs = [(200, 136, 10, 91), (64, 64, 8, 200), (136, 264, 12, 30), (72, 40, 8, 140), (128, 72, 10, 17)]
+    frames, want = [], []
+    for i, (w, h, bd, q) in enumerate(specs):
+        img = W.synth_image(500 + i, w, h, bd)
+        qm = mt.qm_level_law(q, 0)
+        lam = R.rdo_lambda(q, bd, 2)
+        prune = None
+        if i == 3:
+            nsb = ((w + 63) // 64) * ((h + 63) // 64)
+            prune = rng.integers(0, 1 << 16, (nsb, R.NODES)).astype(np.uint16)
+        fr = {"src": torch.from_numpy(img.view(np.int16)).to(dev), "bd": bd, "qidx": q, "qm_level": qm,
+              "lam": lam}
+        if prune is not None:
+            fr["prune"] = torch.from_numpy(prune.view(np.int16)).to(dev)
+        frames.append(fr)
+        want.append((img, O.intra_rdo(img, bd, q, qm, ALL, lam, prune=prune)))
+    print("rdo batch", len(frames), flush=True)
+    got = R.intra_rdo_batch(frames, ALL)
+    torch.cuda.synchronize()
+    for (rec, nodes, st), (img, (o_rec, o_nodes, _, o_st)) in zip(got, want):
+        assert_same(rec.cpu().numpy().view(np.uint16), R.nodes_to_np(nodes), st, o_rec, o_nodes, o_st)
+
+
 def test_rdo_repeatable(dev):
     img = W.synth_image(31, 128, 128, 10)
     src = torch.from_numpy(img.view(np.int16)).to(dev)
