@@ -51,17 +51,21 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-pruning leg and print its object")
     ap.add_argument("--learned-train", type=int, default=48, help="training frames (1920x1080)")
-    ap.add_argument("--learned-eval", type=int, default=12, help="held-out frames per quality")
-    ap.add_argument("--learned-epochs", type=int, default=12)
+    ap.add_argument("--learned-eval", type=int, default=32, help="held-out frames per quality")
+    ap.add_argument("--learned-epochs", type=int, default=30)
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
     ap.add_argument("--stream-images", type=int, default=48)
     ap.add_argument("--stream-max-side", type=int, default=2048)
-    ap.add_argument("--topk", type=int, default=5)
-    ap.add_argument("--lo", type=float, default=0.15)
-    ap.add_argument("--hi", type=float, default=0.85)
+    ap.add_argument("--topk", type=int, default=13, help="intra modes kept per superblock")
+    ap.add_argument("--lo", default="0.002,0.005,0.02,0.05",
+                    help="NO_SPLIT below this split probability, per node level 64,32,16,8")
+    ap.add_argument("--hi", default="0.998,0.995,0.98,0.95",
+                    help="FORCE_SPLIT above this split probability, per node level 64,32,16,8")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    args.lo = [float(v) for v in str(args.lo).split(",")] * (1 if "," in str(args.lo) else 4)
+    args.hi = [float(v) for v in str(args.hi).split(",")] * (1 if "," in str(args.hi) else 4)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the chain has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
@@ -217,6 +221,8 @@ def main():
         line["eval"] = bench_eval(args, dev, torch, mt)
     if not args.no_rdo:
         line["rdo"] = bench_rdo(args, dev, torch, mt)
+    if not args.no_rdo:
+        line["plain_512"] = bench_plain512(args, dev, torch, mt)
     net = None
     if not args.no_learned:
         line["learned"], net = bench_learned(args, dev, torch, mt, return_net=True)
@@ -443,6 +449,40 @@ def bench_learned(args, dev, torch, mt, return_net=False):
                    "points": [[round(r[1], 1), round(r[2], 4)] for r in res["full"]]}
     out["learned"]["points"] = [[round(r[1], 1), round(r[2], 4)] for r in res["learned"]]
     return (out, net) if return_net else out
+
+
+def bench_plain512(args, dev, torch, mt):
+    """The plain-encode configuration: a seeded 512x512 RGB image, its full-range BT.601
+    luma coded at quality 60 (qindex 70), 4:4:4 lambda, by the exact RDO on the GPU and by
+    the oracle's RDO on the host (no entropy coder: the rate is the RDO's final bits)."""
+    from maroontree_amd import rdo as R
+    from maroontree_amd import workload as W
+    luma = W.rgb_to_luma(W.synth_rgb(SEED, 512, 512))
+    q, bd = 70, 8
+    qm = mt.qm_level_law(q, 0)
+    lam = R.rdo_lambda(q, bd, 0)
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    src = torch.from_numpy(luma.view(np.int16)).to(dev)
+    R.intra_rdo(src, bd, q, qm, flags, lam)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, st = R.intra_rdo(src, bd, q, qm, flags, lam)
+    torch.cuda.synchronize()
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    r = {"workload": "seeded 512x512 RGB (gradients, noise, rectangles) -> BT.601 luma, quality 60 (qindex 70), "
+                     "4:4:4 lambda, exact full luma RDO",
+         "bits": round(st["bits"], 1), "bytes_rate_model": round(st["bits"] / 8.0, 1),
+         "psnr_y": round(R.psnr(st["sse"], 512 * 512, bd), 4), "gpu_ms": round(gpu_ms, 2)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        _, _, _, cst = O.intra_rdo(luma, bd, q, qm, flags, lam, nthreads=threads)
+        r["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        r["cpu_cores"] = threads
+        r["identical_to_cpu"] = bool(cst["bits"] == st["bits"] and cst["sse"] == st["sse"])
+    return r
 
 
 def bench_stream(args, dev, torch, mt, net=None):

@@ -136,7 +136,8 @@ typedef struct {
     uint32_t offset;   /* element offset of the block's levels (ncoef, x-major) */
     uint8_t tx_size, plane, eob_bin, skip_ctx;
     uint8_t dcs_ctx, txtp_set, txtp_mode, txtp;
-    uint8_t qctx, flags, pad0, pad1;
+    uint8_t qctx;      /* rate slot: 0..3 Cdfs::new(qctx), 4..7 loaded snapshots */
+    uint8_t flags, pad0, pad1;
     float skip_one, skip_zero;
     float bound;       /* early-out bound (bits); +inf to always price fully */
     uint32_t pad2;
@@ -145,6 +146,34 @@ typedef struct {
 /* Bits of n blocks (bits_d[i] = +inf when the walk exceeded blocks[i].bound). */
 int mt_block_rate(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
                   void* stream);
+
+/* The coefficient part of a decision CDF snapshot (the reference's Cdfs, src/coder/cdfs.rs:
+ * 108-172) in the raw AV1 form of the default tables: each row holds the cumulative
+ * probabilities of the first k symbols (the last symbol's 32768 implied).  Rate slots
+ * (the qctx field of mt_rateblk / mt_cand): 0..3 the frame-initial Cdfs::new(qctx),
+ * MT_RATE_SLOTS_FIXED..MT_RATE_SLOTS-1 snapshots the caller loads, e.g. adapted or
+ * pilot-warmed CDFs (switch_decision_cdfs, src/coder/lossy_state.rs:319). */
+#define MT_RATE_SLOTS_FIXED 4
+#define MT_RATE_SLOTS 8
+typedef struct {
+    uint16_t skip[5][13];             /* txb_skip [tx class][ctx] (k = 1) */
+    uint16_t base_tok[5][2][42][3];   /* coeff_base [class][plane][ctx] (41 ctx at 4x4 / 8x8) */
+    uint16_t br_tok[4][2][21][3];     /* coeff_br [class][plane][ctx] */
+    uint16_t eob_base[5][2][4][2];    /* coeff_base_eob [class][plane][ctx] */
+    uint16_t eob_hi[5][2][11];        /* eob_extra [class][plane][eob bin] (k = 1) */
+    uint16_t dc_sign[2][3];           /* [plane][ctx] (k = 1) */
+    uint16_t eob_bin[17][10];         /* eob_bin CDFs in the numbering of mt_rateblk.eob_bin;
+                                         row e holds 4,4,4,5,5,6,6,6,7,7,7,8,8,9,9,10,10 values */
+    uint16_t txtp4[13][6], txtp8[13][6], txtp16[13][4];  /* intra tx-type sets [intra mode] */
+    uint16_t pad;
+} mt_coef_cdfs;
+
+/* The coefficient CDFs of Cdfs::new(qctx), qctx 0..3 (host). */
+int mt_rate_default_cdfs(int qctx, mt_coef_cdfs* out);
+/* Builds the rate tables of `cdfs` (CoefCostTables::build, src/rate.rs:58, with the
+ * reference's cost_q_table) into slot MT_RATE_SLOTS_FIXED..MT_RATE_SLOTS-1 of the current
+ * device; synchronous, and no rate evaluation may be in flight on the device meanwhile. */
+int mt_rate_load_cdfs(int slot, const mt_coef_cdfs* cdfs);
 
 /* ---- planes and intra prediction -------------------------------------------------- */
 /* A plane of uint16 samples in device memory.  width/height: the frame size the intra
@@ -426,9 +455,11 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
                  float beta1, float beta2, float eps, void* stream);
 /* mt_intra_rdo prune masks [batch][MT_RDO_NODES] from the logits: the top-k intra modes of
  * each superblock (DC_PRED and filter intra always kept) on every node, and for nodes 0..84
- * MT_RDO_NO_SPLIT when sigmoid(split logit) < lo, MT_RDO_FORCE_SPLIT when > hi. */
-int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, float lo, float hi,
-                        uint16_t* prune_d, void* stream);
+ * of level L (0: 64x64 .. 3: 8x8) MT_RDO_NO_SPLIT when sigmoid(split logit) < lo4[L],
+ * MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays of 4).  Between the thresholds a node keeps
+ * both choices: the full search there. */
+int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, const float* lo4,
+                        const float* hi4, uint16_t* prune_d, void* stream);
 
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and

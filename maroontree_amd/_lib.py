@@ -66,6 +66,13 @@ CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size"
                        ("lambda", "<f4"), ("extra_bits", "<f4")])
 CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
                           ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
+COEF_CDFS_DTYPE = np.dtype([("skip", "<u2", (5, 13)), ("base_tok", "<u2", (5, 2, 42, 3)),
+                            ("br_tok", "<u2", (4, 2, 21, 3)), ("eob_base", "<u2", (5, 2, 4, 2)),
+                            ("eob_hi", "<u2", (5, 2, 11)), ("dc_sign", "<u2", (2, 3)),
+                            ("eob_bin", "<u2", (17, 10)), ("txtp4", "<u2", (13, 6)),
+                            ("txtp8", "<u2", (13, 6)), ("txtp16", "<u2", (13, 4)), ("pad", "<u2")])
+EOB_BIN_K = [4, 4, 4, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 9, 9, 10, 10]
+RATE_SLOTS_FIXED, RATE_SLOTS = 4, 8
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
 TXBLK_CHROMA = 0x40
 PRED_FILTER0 = 16
@@ -109,6 +116,8 @@ def lib():
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
         L.mt_intra_rdo.argtypes = [vp, i32, vp, vp]
+        L.mt_rate_default_cdfs.argtypes = [i32, vp]
+        L.mt_rate_load_cdfs.argtypes = [i32, vp]
         L.mt_intra_rdo_batch.argtypes = [vp, i32, i32, vp, vp]
         L.mt_rdo_lambda.argtypes = L.mt_mode_lambda.argtypes = [i32, i32, i32]
         L.mt_rdo_lambda.restype = L.mt_mode_lambda.restype = ctypes.c_float
@@ -258,6 +267,20 @@ def block_rate(blocks, levels, stream=None):
     out = torch.empty(n, dtype=torch.float32, device=blocks.device)
     _check(lib().mt_block_rate(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(out), _stream_ptr(stream)))
     return out
+
+
+def rate_default_cdfs(qctx):
+    """The coefficient CDFs of Cdfs::new(qctx) as a COEF_CDFS_DTYPE record."""
+    out = np.zeros(1, COEF_CDFS_DTYPE)
+    _check(lib().mt_rate_default_cdfs(qctx, out.ctypes.data_as(ctypes.c_void_p)))
+    return out[0]
+
+
+def rate_load_cdfs(slot, cdfs, device=0):
+    """Loads a COEF_CDFS_DTYPE snapshot into rate slot 4..7 of `device`."""
+    init(device)
+    rec = np.ascontiguousarray(np.asarray(cdfs, COEF_CDFS_DTYPE).reshape(1))
+    _check(lib().mt_rate_load_cdfs(slot, rec.ctypes.data_as(ctypes.c_void_p)))
 
 
 def gen_tx_blocks(seed, n, all_sizes=False):

@@ -276,12 +276,17 @@ __global__ void __launch_bounds__(256) sb_extract_kernel(mt_plane p, int C, int 
     }
 }
 
+// Per-level split thresholds of the prune masks (levels 0..3: 64x64 .. 8x8 nodes).
+struct SplitThresholds {
+    float lo[4], hi[4];
+};
+
 // Pruning masks from the network's logits, one thread per (SB, node): the SB's top-k intra
 // modes by score (ties to the lower mode; DC always; filter intra with DC) and, for nodes
-// 0..84, NO_SPLIT when P(split) < lo, FORCE_SPLIT when P(split) > hi.  Nodes 85..340 (4x4)
-// take the mode mask only.
-__global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict__ logits, int B, int topk, float lo,
-                                                         float hi, uint16_t* __restrict__ prune) {
+// 0..84 of level L, NO_SPLIT when P(split) < lo[L], FORCE_SPLIT when P(split) > hi[L].
+// Nodes 85..340 (4x4) take the mode mask only.
+__global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict__ logits, int B, int topk,
+                                                         SplitThresholds th, uint16_t* __restrict__ prune) {
     const long n = (long)B * MT_RDO_NODES;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long b = i / MT_RDO_NODES;
@@ -294,9 +299,10 @@ __global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict
             if (rank < topk) m |= (uint16_t)(1u << j);
         }
         if (node < 85) {
+            const int L = node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
             const float ps = 1.0f / (1.0f + expf(-z[node]));
-            if (ps < lo) m |= MT_RDO_NO_SPLIT;
-            else if (ps > hi) m |= MT_RDO_FORCE_SPLIT;
+            if (ps < th.lo[L]) m |= MT_RDO_NO_SPLIT;
+            else if (ps > th.hi[L]) m |= MT_RDO_FORCE_SPLIT;
         }
         prune[i] = m;
     }
@@ -532,15 +538,21 @@ extern "C" int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, f
     return MT_OK;
 }
 
-extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, float lo, float hi, uint16_t* prune_d,
-                                   void* stream) {
-    if (!logits_d || !prune_d || batch <= 0 || topk < 0 || topk > 13 || !(lo <= hi))
+extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, const float* lo4, const float* hi4,
+                                   uint16_t* prune_d, void* stream) {
+    if (!logits_d || !prune_d || !lo4 || !hi4 || batch <= 0 || topk < 0 || topk > 13)
         return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: bad arguments");
+    SplitThresholds th;
+    for (int L = 0; L < 4; ++L) {
+        if (!(lo4[L] <= hi4[L])) return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: lo > hi");
+        th.lo[L] = lo4[L];
+        th.hi[L] = hi4[L];
+    }
     int dev = 0, cus = 0;
     int rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     hipLaunchKernelGGL(prune_mask_kernel, dim3(grid_for((long)batch * MT_RDO_NODES, cus)), dim3(256), 0,
-                       (hipStream_t)stream, logits_d, batch, topk, lo, hi, prune_d);
+                       (hipStream_t)stream, logits_d, batch, topk, th, prune_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

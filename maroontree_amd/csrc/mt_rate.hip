@@ -40,7 +40,7 @@ constexpr int kScanTotal = 16 + 32 + 32 + 64 + 64 + 64 + 128 + 128 + 256 + 512 +
 
 }  // namespace
 
-__device__ RateTables g_rate[4];
+__device__ RateTables g_rate[MT_RATE_SLOTS];
 __constant__ uint16_t c_scan[kScanTotal];
 __constant__ uint16_t c_iscan[kScanTotal];
 __constant__ uint8_t c_lo_off[3][5][5];  // square, w < h, w > h (src/tables.rs:354, :456, :466)
@@ -86,7 +86,7 @@ struct RateBlk {
 __device__ inline bool rate_decode(const mt_rateblk& d, RateBlk& r) {
     const int s = d.tx_size;
     r.so = s < 19 ? scan_offset(s) : -1;
-    if (r.so < 0 || d.plane > 1 || d.eob_bin > 16 || d.skip_ctx > 12 || d.dcs_ctx > 2 || d.qctx > 3 ||
+    if (r.so < 0 || d.plane > 1 || d.eob_bin > 16 || d.skip_ctx > 12 || d.dcs_ctx > 2 || d.qctx >= MT_RATE_SLOTS ||
         d.txtp_set > 3 || d.txtp_mode > 12 || d.txtp > 6 || (s == MT_TX_64X64 && d.plane))
         return false;
     const int wf = s == MT_TX_4X4 || s == MT_TX_4X8 || s == MT_TX_4X16 ? 4
@@ -445,6 +445,9 @@ float sym_cost(const uint16_t* raw, int k, int s) {
     return (float)cost_q()[p] * (1.0f / (float)(1u << 22));
 }
 
+// Stored values of the eob_bin CDF of each mt_rateblk.eob_bin.
+constexpr int kEobBinK[17] = {4, 4, 4, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 9, 9, 10, 10};
+
 void ladder(const uint16_t* raw3, float* row) {  // br_cum_row_with_table (src/cost.rs:204)
     float c[4];
     for (int s = 0; s < 4; s++) c[s] = sym_cost(raw3, 3, s);
@@ -456,71 +459,91 @@ void ladder(const uint16_t* raw3, float* row) {  // br_cum_row_with_table (src/c
     memcpy(row, r, sizeof(r));
 }
 
-void build_tables(int q, RateTables& t) {
-    memset(&t, 0, sizeof(t));
-    const uint16_t* skip[5] = {MT_SKIP_TX4[q], MT_SKIP_TX8[q], MT_SKIP_TX16[q], MT_SKIP_TX32[q],
-                               MT_SKIP_TX64[q]};
-    for (int c = 0; c < 5; c++)
-        for (int x = 0; x < 13; x++)
-            for (int s = 0; s < 2; s++) t.skip[c][x][s] = sym_cost(&skip[c][x], 1, s);
+// The coefficient CDFs of Cdfs::new(q) (src/coder/cdfs.rs:400) in raw form.
+void default_cdfs(int q, mt_coef_cdfs& c) {
+    memset(&c, 0, sizeof(c));
+    const uint16_t* skip[5] = {MT_SKIP_TX4[q], MT_SKIP_TX8[q], MT_SKIP_TX16[q], MT_SKIP_TX32[q], MT_SKIP_TX64[q]};
+    for (int k = 0; k < 5; k++) memcpy(c.skip[k], skip[k], 13 * sizeof(uint16_t));
     const uint16_t(*bt[5][2])[3] = {{MT_BASE_TOK_TX4_LUMA_Q[q], MT_BASE_TOK_TX4_CHROMA_Q[q]},
                                     {MT_BASE_TOK_TX8_LUMA_Q[q], MT_BASE_TOK_TX8_CHROMA_Q[q]},
                                     {MT_BASE_TOK_TX16_LUMA_Q[q], MT_BASE_TOK_TX16_CHROMA_Q[q]},
                                     {MT_BASE_TOK_TX32_LUMA_Q[q], MT_BASE_TOK_TX32_CHROMA_Q[q]},
                                     {MT_BASE_TOK_TX64_LUMA_Q[q], nullptr}};
     const int nctx[5] = {41, 41, 42, 42, 42};
-    // Contexts past a table's end stay 0.0, as in the reference's zero-initialised table
-    // (the 2-D context walk never reaches them).
-    for (int c = 0; c < 5; c++)
+    for (int k = 0; k < 5; k++)
         for (int p = 0; p < 2; p++)
-            if (bt[c][p])
-                for (int x = 0; x < nctx[c]; x++)
-                    for (int s = 0; s < 4; s++) t.bt[c][p][x][s] = sym_cost(bt[c][p][x], 3, s);
+            if (bt[k][p]) memcpy(c.base_tok[k][p], bt[k][p], nctx[k] * 3 * sizeof(uint16_t));
     const uint16_t(*br[4][2])[3] = {{MT_BR_TOK_TX4_LUMA_Q[q], MT_BR_TOK_TX4_CHROMA_Q[q]},
                                     {MT_BR_TOK_TX8_LUMA_Q[q], MT_BR_TOK_TX8_CHROMA_Q[q]},
                                     {MT_BR_TOK_TX16_LUMA_Q[q], MT_BR_TOK_TX16_CHROMA_Q[q]},
                                     {MT_BR_TOK_TX32_LUMA_Q[q], MT_BR_TOK_TX32_CHROMA_Q[q]}};
-    for (int c = 0; c < 4; c++)
-        for (int p = 0; p < 2; p++)
-            for (int x = 0; x < 21; x++) ladder(br[c][p][x], t.br[c][p][x]);
+    for (int k = 0; k < 4; k++)
+        for (int p = 0; p < 2; p++) memcpy(c.br_tok[k][p], br[k][p], 21 * 3 * sizeof(uint16_t));
     const uint16_t(*eb[5][2])[2] = {{MT_EOB_BASE_TX4_LUMA_Q[q], MT_EOB_BASE_TX4_CHROMA_Q[q]},
                                     {MT_EOB_BASE_TX8_LUMA_Q[q], MT_EOB_BASE_TX8_CHROMA_Q[q]},
                                     {MT_EOB_BASE_TX16_LUMA_Q[q], MT_EOB_BASE_TX16_CHROMA_Q[q]},
                                     {MT_EOB_BASE_TX32_LUMA_Q[q], MT_EOB_BASE_TX32_CHROMA_Q[q]},
                                     {MT_EOB_BASE_TX64_LUMA_Q[q], nullptr}};
-    for (int c = 0; c < 5; c++)
+    for (int k = 0; k < 5; k++)
         for (int p = 0; p < 2; p++)
-            if (eb[c][p])
-                for (int x = 0; x < 4; x++)
-                    for (int s = 0; s < 3; s++) t.eob_base[c][p][x][s] = sym_cost(eb[c][p][x], 2, s);
+            if (eb[k][p]) memcpy(c.eob_base[k][p], eb[k][p], 4 * 2 * sizeof(uint16_t));
     const uint16_t* eh[5][2] = {{MT_EOB_HI_TX4_LUMA[q], MT_EOB_HI_TX4_CHROMA[q]},
                                 {MT_EOB_HI_TX8_LUMA[q], MT_EOB_HI_TX8_CHROMA[q]},
                                 {MT_EOB_HI_TX16_LUMA[q], MT_EOB_HI_TX16_CHROMA[q]},
                                 {MT_EOB_HI_TX32_LUMA[q], MT_EOB_HI_TX32_CHROMA[q]},
                                 {MT_EOB_HI_TX64_LUMA[q], nullptr}};
-    for (int c = 0; c < 5; c++)
+    for (int k = 0; k < 5; k++)
         for (int p = 0; p < 2; p++)
-            if (eh[c][p])
+            if (eh[k][p]) memcpy(c.eob_hi[k][p], eh[k][p], 11 * sizeof(uint16_t));
+    memcpy(c.dc_sign, MT_DC_SIGN_Q[q], sizeof(c.dc_sign));
+    const uint16_t* bins[17] = {MT_EOB_BIN_16_LUMA[q],   MT_EOB_BIN_16_CHROMA[q],  MT_EOB_BIN_16_LUMA_1D[q],
+                                MT_EOB_BIN_32_LUMA[q],   MT_EOB_BIN_32_CHROMA[q],  MT_EOB_BIN_64_LUMA[q],
+                                MT_EOB_BIN_64_CHROMA[q], MT_EOB_BIN_64_LUMA_1D[q], MT_EOB_BIN_128_LUMA[q],
+                                MT_EOB_BIN_128_CHROMA[q], MT_EOB_BIN_128_LUMA_1D[q], MT_EOB_BIN_256_LUMA[q],
+                                MT_EOB_BIN_256_CHROMA[q], MT_EOB_BIN_512_LUMA[q],  MT_EOB_BIN_512_CHROMA[q],
+                                MT_EOB_BIN_1024_LUMA[q], MT_EOB_BIN_1024_CHROMA[q]};
+    for (int e = 0; e < 17; e++) memcpy(c.eob_bin[e], bins[e], kEobBinK[e] * sizeof(uint16_t));
+    memcpy(c.txtp4, MT_TXTP_INTRA1_TX4, sizeof(c.txtp4));
+    memcpy(c.txtp8, MT_TXTP_INTRA1_TX8, sizeof(c.txtp8));
+    memcpy(c.txtp16, MT_TXTP_INTRA2_TX16, sizeof(c.txtp16));
+}
+
+// Symbol costs of a snapshot (CoefCostTables::build, src/rate.rs:58).
+void build_tables(const mt_coef_cdfs& c, RateTables& t) {
+    memset(&t, 0, sizeof(t));
+    for (int k = 0; k < 5; k++)
+        for (int x = 0; x < 13; x++)
+            for (int s = 0; s < 2; s++) t.skip[k][x][s] = sym_cost(&c.skip[k][x], 1, s);
+    // Contexts past a table's end stay 0.0, as in the reference's zero-initialised table
+    // (the 2-D context walk never reaches them).
+    const int nctx[5] = {41, 41, 42, 42, 42};
+    for (int k = 0; k < 5; k++)
+        for (int p = 0; p < 2; p++)
+            if (!(k == 4 && p == 1))
+                for (int x = 0; x < nctx[k]; x++)
+                    for (int s = 0; s < 4; s++) t.bt[k][p][x][s] = sym_cost(c.base_tok[k][p][x], 3, s);
+    for (int k = 0; k < 4; k++)
+        for (int p = 0; p < 2; p++)
+            for (int x = 0; x < 21; x++) ladder(c.br_tok[k][p][x], t.br[k][p][x]);
+    for (int k = 0; k < 5; k++)
+        for (int p = 0; p < 2; p++)
+            if (!(k == 4 && p == 1))
+                for (int x = 0; x < 4; x++)
+                    for (int s = 0; s < 3; s++) t.eob_base[k][p][x][s] = sym_cost(c.eob_base[k][p][x], 2, s);
+    for (int k = 0; k < 5; k++)
+        for (int p = 0; p < 2; p++)
+            if (!(k == 4 && p == 1))
                 for (int x = 0; x < 11; x++)
-                    for (int s = 0; s < 2; s++) t.eob_hi[c][p][x][s] = sym_cost(&eh[c][p][x], 1, s);
+                    for (int s = 0; s < 2; s++) t.eob_hi[k][p][x][s] = sym_cost(&c.eob_hi[k][p][x], 1, s);
     for (int p = 0; p < 2; p++)
         for (int x = 0; x < 3; x++)
-            for (int s = 0; s < 2; s++) t.dc_sign[p][x][s] = sym_cost(&MT_DC_SIGN_Q[q][p][x], 1, s);
-    struct {
-        const uint16_t* raw;
-        int k;
-    } bins[17] = {{MT_EOB_BIN_16_LUMA[q], 4},   {MT_EOB_BIN_16_CHROMA[q], 4},  {MT_EOB_BIN_16_LUMA_1D[q], 4},
-                  {MT_EOB_BIN_32_LUMA[q], 5},   {MT_EOB_BIN_32_CHROMA[q], 5},  {MT_EOB_BIN_64_LUMA[q], 6},
-                  {MT_EOB_BIN_64_CHROMA[q], 6}, {MT_EOB_BIN_64_LUMA_1D[q], 6}, {MT_EOB_BIN_128_LUMA[q], 7},
-                  {MT_EOB_BIN_128_CHROMA[q], 7}, {MT_EOB_BIN_128_LUMA_1D[q], 7}, {MT_EOB_BIN_256_LUMA[q], 8},
-                  {MT_EOB_BIN_256_CHROMA[q], 8}, {MT_EOB_BIN_512_LUMA[q], 9},  {MT_EOB_BIN_512_CHROMA[q], 9},
-                  {MT_EOB_BIN_1024_LUMA[q], 10}, {MT_EOB_BIN_1024_CHROMA[q], 10}};
+            for (int s = 0; s < 2; s++) t.dc_sign[p][x][s] = sym_cost(&c.dc_sign[p][x], 1, s);
     for (int e = 0; e < 17; e++)
-        for (int s = 0; s <= bins[e].k; s++) t.eob_bin[e][s] = sym_cost(bins[e].raw, bins[e].k, s);
+        for (int s = 0; s <= kEobBinK[e]; s++) t.eob_bin[e][s] = sym_cost(c.eob_bin[e], kEobBinK[e], s);
     for (int m = 0; m < 13; m++) {
-        for (int s = 0; s <= 6; s++) t.txtp[0][m][s] = sym_cost(MT_TXTP_INTRA1_TX4[m], 6, s);
-        for (int s = 0; s <= 6; s++) t.txtp[1][m][s] = sym_cost(MT_TXTP_INTRA1_TX8[m], 6, s);
-        for (int s = 0; s <= 4; s++) t.txtp[2][m][s] = sym_cost(MT_TXTP_INTRA2_TX16[m], 4, s);
+        for (int s = 0; s <= 6; s++) t.txtp[0][m][s] = sym_cost(c.txtp4[m], 6, s);
+        for (int s = 0; s <= 6; s++) t.txtp[1][m][s] = sym_cost(c.txtp8[m], 6, s);
+        for (int s = 0; s <= 4; s++) t.txtp[2][m][s] = sym_cost(c.txtp16[m], 4, s);
     }
 }
 
@@ -552,11 +575,15 @@ void build_scans(ScanTabs& st) {
 
 namespace mtlib {
 int rate_init_device(int /*device*/) {
-    static RateTables host[4];
+    static RateTables host[MT_RATE_SLOTS];
     static ScanTabs scans;
     static std::once_flag once;
     std::call_once(once, [] {
-        for (int q = 0; q < 4; q++) build_tables(q, host[q]);
+        for (int q = 0; q < MT_RATE_SLOTS; q++) {  // the loadable slots start as Cdfs::new(0)
+            mt_coef_cdfs c;
+            default_cdfs(q < MT_RATE_SLOTS_FIXED ? q : 0, c);
+            build_tables(c, host[q]);
+        }
         build_scans(scans);
     });
     uint8_t lo[3][5][5];
@@ -632,3 +659,24 @@ int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* level
 namespace mtlib {
 float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s) { return sym_cost(raw_cdf, nsym - 1, s); }
 }  // namespace mtlib
+
+extern "C" int mt_rate_default_cdfs(int qctx, mt_coef_cdfs* out) {
+    if (qctx < 0 || qctx >= MT_RATE_SLOTS_FIXED || !out)
+        return mtlib::set_error(MT_ERR_ARG, "mt_rate_default_cdfs: bad argument");
+    default_cdfs(qctx, *out);
+    return MT_OK;
+}
+
+extern "C" int mt_rate_load_cdfs(int slot, const mt_coef_cdfs* cdfs) {
+    if (slot < MT_RATE_SLOTS_FIXED || slot >= MT_RATE_SLOTS || !cdfs)
+        return mtlib::set_error(MT_ERR_ARG, "mt_rate_load_cdfs: bad slot or pointer");
+    int dev = 0, cus = 0;
+    int rc = mtlib::device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    RateTables t;
+    build_tables(*cdfs, t);
+    MT_HIP_TRY(hipDeviceSynchronize());
+    MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_rate), &t, sizeof(t), (size_t)slot * sizeof(t)));
+    MT_HIP_TRY(hipDeviceSynchronize());
+    return MT_OK;
+}

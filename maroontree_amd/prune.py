@@ -32,7 +32,7 @@ def _sigs():
     L.mt_pnet_forward.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, vp]
     L.mt_pnet_backward.argtypes = [vp, vp, vp, i32, f32, f32, vp, vp, vp, vp]
     L.mt_pnet_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, vp]
-    L.mt_pnet_prune_masks.argtypes = [vp, i32, i32, f32, f32, vp, vp]
+    L.mt_pnet_prune_masks.argtypes = [vp, i32, i32, ctypes.POINTER(f32), ctypes.POINTER(f32), vp, vp]
     L._pnet_sigs = True
     return L
 
@@ -109,10 +109,13 @@ class PruneNet:
                                     lr, b1, b2, eps, _stream_ptr(stream)))
 
     def masks(self, logits, topk, lo, hi, stream=None):
+        """Prune masks; lo / hi: a split threshold each, or 4 per node level (64x64 .. 8x8)."""
         import torch
         B = logits.shape[0]
+        lo4 = (ctypes.c_float * 4)(*([lo] * 4 if np.isscalar(lo) else lo))
+        hi4 = (ctypes.c_float * 4)(*([hi] * 4 if np.isscalar(hi) else hi))
         out = torch.empty((B, R.NODES), dtype=torch.int16, device=self.dev)
-        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, topk, lo, hi, _p(out), _stream_ptr(stream)))
+        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, topk, lo4, hi4, _p(out), _stream_ptr(stream)))
         return out
 
 

@@ -49,6 +49,41 @@ def synth_image(seed, width, height, bd=10):
     return img.astype(np.uint16)
 
 
+def synth_rgb(seed, width, height):
+    """The same recipe on three 8-bit channels (rectangles of one shape get an RGB colour);
+    returns uint8 (height, width, 3)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    img = np.zeros((height, width, 3))
+    for _ in range(4):
+        th = rng.uniform(0, 2 * np.pi)
+        proj = x * np.cos(th) + y * np.sin(th)
+        proj = (proj - proj.min()) / max(proj.max() - proj.min(), 1e-9)
+        img += rng.uniform(20, 70, 3) * proj[..., None]
+    for octave, amp in ((64, 40.0), (32, 20.0), (16, 10.0)):
+        gh, gw = height // octave + 2, width // octave + 2
+        g = rng.uniform(-1, 1, (gh, gw, 3))
+        fy, fx = y / octave, x / octave
+        iy, ix = fy.astype(int), fx.astype(int)
+        ty, tx = (fy - iy)[..., None], (fx - ix)[..., None]
+        img += amp * (g[iy, ix] * (1 - tx) * (1 - ty) + g[iy, ix + 1] * tx * (1 - ty) +
+                      g[iy + 1, ix] * (1 - tx) * ty + g[iy + 1, ix + 1] * tx * ty)
+    for _ in range(20):
+        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+        hw, hh = rng.uniform(8, max(width / 6, 9)), rng.uniform(8, max(height / 6, 9))
+        th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
+        u = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
+        v = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
+        img[(np.abs(u) <= hw) & (np.abs(v) <= hh)] = rng.uniform(0, 255, 3)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def rgb_to_luma(rgb):
+    """Full-range BT.601 luma (the matrix of the sRGB CICP's YCbCr), rounded, uint16."""
+    r, g, b = (rgb[..., i].astype(np.float64) for i in range(3))
+    return np.clip(np.rint(0.299 * r + 0.587 * g + 0.114 * b), 0, 255).astype(np.uint16)
+
+
 def synth_recon(src, seed, bd=10, amp=6):
     """A stand-in for the decision-time reconstruction: the source plus small noise."""
     rng = np.random.default_rng(seed ^ 0x5EED)

@@ -111,6 +111,24 @@ typedef struct {
     float skip_one, skip_zero;
 } orc_rate_ctx;
 
+/* The coefficient part of a decision CDF snapshot in raw form: the library's mt_coef_cdfs
+ * layout. */
+typedef struct {
+    uint16_t skip[5][13];
+    uint16_t base_tok[5][2][42][3];
+    uint16_t br_tok[4][2][21][3];
+    uint16_t eob_base[5][2][4][2];
+    uint16_t eob_hi[5][2][11];
+    uint16_t dc_sign[2][3];
+    uint16_t eob_bin[17][10];
+    uint16_t txtp4[13][6], txtp8[13][6], txtp16[13][4];
+    uint16_t pad;
+} orc_coef_cdfs;
+/* The raw snapshot of Cdfs::new(qctx), and loading one into slot 4..7 (the qctx of
+ * orc_rate_ctx selects the slot; 0..3 are the frame-initial snapshots). */
+void orc_rate_default_cdfs(int qctx, orc_coef_cdfs* out);
+int orc_rate_load_cdfs(int slot, const orc_coef_cdfs* cdfs);
+
 /* real_block_bits_bounded against Cdfs::new(qctx); cf x-major, scan of the block size. */
 float orc_block_rate(const int32_t* cf, const uint32_t* scan, const orc_rate_ctx* c, float bound);
 /* cost_q_table entry (Q22 bits) and the cost of one symbol of a raw AV1 CDF */

@@ -102,89 +102,129 @@ template <int K> void icdf_row(const uint16_t* raw, uint16_t* out) {
     out[K] = 0;
 }
 
-const CoefCdfs& coef_cdfs(int qctx) {
-    static CoefCdfs c[4];
-    static std::once_flag once[4];
-    std::call_once(once[qctx], [qctx] {
-        CoefCdfs& d = c[qctx];
-        memset(&d, 0, sizeof(d));
-        const uint16_t* skip[5] = {MT_SKIP_TX4[qctx], MT_SKIP_TX8[qctx], MT_SKIP_TX16[qctx],
-                                   MT_SKIP_TX32[qctx], MT_SKIP_TX64[qctx]};
-        for (int cls = 0; cls < 5; cls++)
-            for (int ctx = 0; ctx < 13; ctx++) icdf_row<1>(&skip[cls][ctx], d.txb_skip[cls][ctx]);
-        const uint16_t(*bt_src[5][2])[3] = {
-            {MT_BASE_TOK_TX4_LUMA_Q[qctx], MT_BASE_TOK_TX4_CHROMA_Q[qctx]},
-            {MT_BASE_TOK_TX8_LUMA_Q[qctx], MT_BASE_TOK_TX8_CHROMA_Q[qctx]},
-            {MT_BASE_TOK_TX16_LUMA_Q[qctx], MT_BASE_TOK_TX16_CHROMA_Q[qctx]},
-            {MT_BASE_TOK_TX32_LUMA_Q[qctx], MT_BASE_TOK_TX32_CHROMA_Q[qctx]},
-            {MT_BASE_TOK_TX64_LUMA_Q[qctx], nullptr}};
-        const int bt_n[5] = {41, 41, 42, 42, 42};
-        for (int cls = 0; cls < 5; cls++)
-            for (int pl = 0; pl < 2; pl++)
-                if (bt_src[cls][pl])
-                    for (int ctx = 0; ctx < bt_n[cls]; ctx++) icdf_row<3>(bt_src[cls][pl][ctx], d.base_tok[cls][pl][ctx]);
-        const uint16_t(*br_src[4][2])[3] = {
-            {MT_BR_TOK_TX4_LUMA_Q[qctx], MT_BR_TOK_TX4_CHROMA_Q[qctx]},
-            {MT_BR_TOK_TX8_LUMA_Q[qctx], MT_BR_TOK_TX8_CHROMA_Q[qctx]},
-            {MT_BR_TOK_TX16_LUMA_Q[qctx], MT_BR_TOK_TX16_CHROMA_Q[qctx]},
-            {MT_BR_TOK_TX32_LUMA_Q[qctx], MT_BR_TOK_TX32_CHROMA_Q[qctx]}};
-        for (int cls = 0; cls < 4; cls++)
-            for (int pl = 0; pl < 2; pl++)
-                for (int ctx = 0; ctx < 21; ctx++) icdf_row<3>(br_src[cls][pl][ctx], d.br_tok[cls][pl][ctx]);
-        const uint16_t(*eb_src[5][2])[2] = {
-            {MT_EOB_BASE_TX4_LUMA_Q[qctx], MT_EOB_BASE_TX4_CHROMA_Q[qctx]},
-            {MT_EOB_BASE_TX8_LUMA_Q[qctx], MT_EOB_BASE_TX8_CHROMA_Q[qctx]},
-            {MT_EOB_BASE_TX16_LUMA_Q[qctx], MT_EOB_BASE_TX16_CHROMA_Q[qctx]},
-            {MT_EOB_BASE_TX32_LUMA_Q[qctx], MT_EOB_BASE_TX32_CHROMA_Q[qctx]},
-            {MT_EOB_BASE_TX64_LUMA_Q[qctx], nullptr}};
-        for (int cls = 0; cls < 5; cls++)
-            for (int pl = 0; pl < 2; pl++)
-                if (eb_src[cls][pl])
-                    for (int ctx = 0; ctx < 4; ctx++) icdf_row<2>(eb_src[cls][pl][ctx], d.eob_base[cls][pl][ctx]);
-        const uint16_t* eh_src[5][2] = {
-            {MT_EOB_HI_TX4_LUMA[qctx], MT_EOB_HI_TX4_CHROMA[qctx]},
-            {MT_EOB_HI_TX8_LUMA[qctx], MT_EOB_HI_TX8_CHROMA[qctx]},
-            {MT_EOB_HI_TX16_LUMA[qctx], MT_EOB_HI_TX16_CHROMA[qctx]},
-            {MT_EOB_HI_TX32_LUMA[qctx], MT_EOB_HI_TX32_CHROMA[qctx]},
-            {MT_EOB_HI_TX64_LUMA[qctx], nullptr}};
-        for (int cls = 0; cls < 5; cls++)
-            for (int pl = 0; pl < 2; pl++)
-                if (eh_src[cls][pl])
-                    for (int b = 0; b < 11; b++) icdf_row<1>(&eh_src[cls][pl][b], d.eob_hi[cls][pl][b]);
+constexpr int kEobK[17] = {4, 4, 4, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 9, 9, 10, 10};
+
+// The raw coefficient CDFs of Cdfs::new(qctx) (src/coder/cdfs.rs:400-629).
+void raw_defaults(int qctx, orc_coef_cdfs& r) {
+    memset(&r, 0, sizeof(r));
+    const uint16_t* skip[5] = {MT_SKIP_TX4[qctx], MT_SKIP_TX8[qctx], MT_SKIP_TX16[qctx], MT_SKIP_TX32[qctx],
+                               MT_SKIP_TX64[qctx]};
+    for (int cls = 0; cls < 5; cls++)
+        for (int ctx = 0; ctx < 13; ctx++) r.skip[cls][ctx] = skip[cls][ctx];
+    const uint16_t(*bt_src[5][2])[3] = {
+        {MT_BASE_TOK_TX4_LUMA_Q[qctx], MT_BASE_TOK_TX4_CHROMA_Q[qctx]},
+        {MT_BASE_TOK_TX8_LUMA_Q[qctx], MT_BASE_TOK_TX8_CHROMA_Q[qctx]},
+        {MT_BASE_TOK_TX16_LUMA_Q[qctx], MT_BASE_TOK_TX16_CHROMA_Q[qctx]},
+        {MT_BASE_TOK_TX32_LUMA_Q[qctx], MT_BASE_TOK_TX32_CHROMA_Q[qctx]},
+        {MT_BASE_TOK_TX64_LUMA_Q[qctx], nullptr}};
+    const int bt_n[5] = {41, 41, 42, 42, 42};
+    for (int cls = 0; cls < 5; cls++)
         for (int pl = 0; pl < 2; pl++)
-            for (int ctx = 0; ctx < 3; ctx++) icdf_row<1>(&MT_DC_SIGN_Q[qctx][pl][ctx], d.dc_sign[pl][ctx]);
-        icdf_row<4>(MT_EOB_BIN_16_LUMA[qctx], d.eob_bin[0]);
-        icdf_row<4>(MT_EOB_BIN_16_CHROMA[qctx], d.eob_bin[1]);
-        icdf_row<4>(MT_EOB_BIN_16_LUMA_1D[qctx], d.eob_bin[2]);
-        icdf_row<5>(MT_EOB_BIN_32_LUMA[qctx], d.eob_bin[3]);
-        icdf_row<5>(MT_EOB_BIN_32_CHROMA[qctx], d.eob_bin[4]);
-        icdf_row<6>(MT_EOB_BIN_64_LUMA[qctx], d.eob_bin[5]);
-        icdf_row<6>(MT_EOB_BIN_64_CHROMA[qctx], d.eob_bin[6]);
-        icdf_row<6>(MT_EOB_BIN_64_LUMA_1D[qctx], d.eob_bin[7]);
-        icdf_row<7>(MT_EOB_BIN_128_LUMA[qctx], d.eob_bin[8]);
-        icdf_row<7>(MT_EOB_BIN_128_CHROMA[qctx], d.eob_bin[9]);
-        icdf_row<7>(MT_EOB_BIN_128_LUMA_1D[qctx], d.eob_bin[10]);
-        icdf_row<8>(MT_EOB_BIN_256_LUMA[qctx], d.eob_bin[11]);
-        icdf_row<8>(MT_EOB_BIN_256_CHROMA[qctx], d.eob_bin[12]);
-        icdf_row<9>(MT_EOB_BIN_512_LUMA[qctx], d.eob_bin[13]);
-        icdf_row<9>(MT_EOB_BIN_512_CHROMA[qctx], d.eob_bin[14]);
-        icdf_row<10>(MT_EOB_BIN_1024_LUMA[qctx], d.eob_bin[15]);
-        icdf_row<10>(MT_EOB_BIN_1024_CHROMA[qctx], d.eob_bin[16]);
-        for (int m = 0; m < 13; m++) {
-            icdf_row<6>(MT_TXTP_INTRA1_TX4[m], d.txtp4[m]);
-            icdf_row<6>(MT_TXTP_INTRA1_TX8[m], d.txtp8[m]);
-            icdf_row<4>(MT_TXTP_INTRA2_TX16[m], d.txtp16[m]);
+            if (bt_src[cls][pl])
+                for (int ctx = 0; ctx < bt_n[cls]; ctx++)
+                    for (int i = 0; i < 3; i++) r.base_tok[cls][pl][ctx][i] = bt_src[cls][pl][ctx][i];
+    const uint16_t(*br_src[4][2])[3] = {
+        {MT_BR_TOK_TX4_LUMA_Q[qctx], MT_BR_TOK_TX4_CHROMA_Q[qctx]},
+        {MT_BR_TOK_TX8_LUMA_Q[qctx], MT_BR_TOK_TX8_CHROMA_Q[qctx]},
+        {MT_BR_TOK_TX16_LUMA_Q[qctx], MT_BR_TOK_TX16_CHROMA_Q[qctx]},
+        {MT_BR_TOK_TX32_LUMA_Q[qctx], MT_BR_TOK_TX32_CHROMA_Q[qctx]}};
+    for (int cls = 0; cls < 4; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            for (int ctx = 0; ctx < 21; ctx++)
+                for (int i = 0; i < 3; i++) r.br_tok[cls][pl][ctx][i] = br_src[cls][pl][ctx][i];
+    const uint16_t(*eb_src[5][2])[2] = {
+        {MT_EOB_BASE_TX4_LUMA_Q[qctx], MT_EOB_BASE_TX4_CHROMA_Q[qctx]},
+        {MT_EOB_BASE_TX8_LUMA_Q[qctx], MT_EOB_BASE_TX8_CHROMA_Q[qctx]},
+        {MT_EOB_BASE_TX16_LUMA_Q[qctx], MT_EOB_BASE_TX16_CHROMA_Q[qctx]},
+        {MT_EOB_BASE_TX32_LUMA_Q[qctx], MT_EOB_BASE_TX32_CHROMA_Q[qctx]},
+        {MT_EOB_BASE_TX64_LUMA_Q[qctx], nullptr}};
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            if (eb_src[cls][pl])
+                for (int ctx = 0; ctx < 4; ctx++)
+                    for (int i = 0; i < 2; i++) r.eob_base[cls][pl][ctx][i] = eb_src[cls][pl][ctx][i];
+    const uint16_t* eh_src[5][2] = {
+        {MT_EOB_HI_TX4_LUMA[qctx], MT_EOB_HI_TX4_CHROMA[qctx]},
+        {MT_EOB_HI_TX8_LUMA[qctx], MT_EOB_HI_TX8_CHROMA[qctx]},
+        {MT_EOB_HI_TX16_LUMA[qctx], MT_EOB_HI_TX16_CHROMA[qctx]},
+        {MT_EOB_HI_TX32_LUMA[qctx], MT_EOB_HI_TX32_CHROMA[qctx]},
+        {MT_EOB_HI_TX64_LUMA[qctx], nullptr}};
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            if (eh_src[cls][pl])
+                for (int b = 0; b < 11; b++) r.eob_hi[cls][pl][b] = eh_src[cls][pl][b];
+    for (int pl = 0; pl < 2; pl++)
+        for (int ctx = 0; ctx < 3; ctx++) r.dc_sign[pl][ctx] = MT_DC_SIGN_Q[qctx][pl][ctx];
+    const uint16_t* bins[17] = {MT_EOB_BIN_16_LUMA[qctx],    MT_EOB_BIN_16_CHROMA[qctx],   MT_EOB_BIN_16_LUMA_1D[qctx],
+                                MT_EOB_BIN_32_LUMA[qctx],    MT_EOB_BIN_32_CHROMA[qctx],   MT_EOB_BIN_64_LUMA[qctx],
+                                MT_EOB_BIN_64_CHROMA[qctx],  MT_EOB_BIN_64_LUMA_1D[qctx],  MT_EOB_BIN_128_LUMA[qctx],
+                                MT_EOB_BIN_128_CHROMA[qctx], MT_EOB_BIN_128_LUMA_1D[qctx], MT_EOB_BIN_256_LUMA[qctx],
+                                MT_EOB_BIN_256_CHROMA[qctx], MT_EOB_BIN_512_LUMA[qctx],    MT_EOB_BIN_512_CHROMA[qctx],
+                                MT_EOB_BIN_1024_LUMA[qctx],  MT_EOB_BIN_1024_CHROMA[qctx]};
+    for (int e = 0; e < 17; e++)
+        for (int i = 0; i < kEobK[e]; i++) r.eob_bin[e][i] = bins[e][i];
+    for (int m = 0; m < 13; m++) {
+        for (int i = 0; i < 6; i++) r.txtp4[m][i] = MT_TXTP_INTRA1_TX4[m][i];
+        for (int i = 0; i < 6; i++) r.txtp8[m][i] = MT_TXTP_INTRA1_TX8[m][i];
+        for (int i = 0; i < 4; i++) r.txtp16[m][i] = MT_TXTP_INTRA2_TX16[m][i];
+    }
+}
+
+// The icdf rows and CoefCostTables (src/rate.rs:50-80) of a raw snapshot.
+void build_coef(const orc_coef_cdfs& r, CoefCdfs& d) {
+    memset(&d, 0, sizeof(d));
+    for (int cls = 0; cls < 5; cls++)
+        for (int ctx = 0; ctx < 13; ctx++) icdf_row<1>(&r.skip[cls][ctx], d.txb_skip[cls][ctx]);
+    const int bt_n[5] = {41, 41, 42, 42, 42};
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            if (!(cls == 4 && pl == 1))
+                for (int ctx = 0; ctx < bt_n[cls]; ctx++) icdf_row<3>(r.base_tok[cls][pl][ctx], d.base_tok[cls][pl][ctx]);
+    for (int cls = 0; cls < 4; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            for (int ctx = 0; ctx < 21; ctx++) icdf_row<3>(r.br_tok[cls][pl][ctx], d.br_tok[cls][pl][ctx]);
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            if (!(cls == 4 && pl == 1))
+                for (int ctx = 0; ctx < 4; ctx++) icdf_row<2>(r.eob_base[cls][pl][ctx], d.eob_base[cls][pl][ctx]);
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++)
+            if (!(cls == 4 && pl == 1))
+                for (int b = 0; b < 11; b++) icdf_row<1>(&r.eob_hi[cls][pl][b], d.eob_hi[cls][pl][b]);
+    for (int pl = 0; pl < 2; pl++)
+        for (int ctx = 0; ctx < 3; ctx++) icdf_row<1>(&r.dc_sign[pl][ctx], d.dc_sign[pl][ctx]);
+    for (int e = 0; e < 17; e++) {
+        for (int i = 0; i < kEobK[e]; i++) d.eob_bin[e][i] = (uint16_t)(32768 - r.eob_bin[e][i]);
+        d.eob_bin[e][kEobK[e]] = 0;
+    }
+    for (int m = 0; m < 13; m++) {
+        icdf_row<6>(r.txtp4[m], d.txtp4[m]);
+        icdf_row<6>(r.txtp8[m], d.txtp8[m]);
+        icdf_row<4>(r.txtp16[m], d.txtp16[m]);
+    }
+    // CoefCostTables::build (src/rate.rs:58)
+    for (int cls = 0; cls < 5; cls++)
+        for (int pl = 0; pl < 2; pl++) {
+            for (int ctx = 0; ctx < 42; ctx++)
+                for (int tok = 0; tok < 4; tok++) d.bt[cls][pl][ctx][tok] = cdf_cost(d.base_tok[cls][pl][ctx], tok);
+            if (cls == 4) continue;
+            for (int bc = 0; bc < 21; bc++) br_cum_row(d.br_tok[cls][pl][bc], d.br[cls][pl][bc]);
         }
-        // CoefCostTables::build (src/rate.rs:58)
-        for (int cls = 0; cls < 5; cls++)
-            for (int pl = 0; pl < 2; pl++) {
-                for (int ctx = 0; ctx < 42; ctx++)
-                    for (int tok = 0; tok < 4; tok++) d.bt[cls][pl][ctx][tok] = cdf_cost(d.base_tok[cls][pl][ctx], tok);
-                if (cls == 4) continue;
-                for (int bc = 0; bc < 21; bc++) br_cum_row(d.br_tok[cls][pl][bc], d.br[cls][pl][bc]);
-            }
+}
+
+// Snapshot slots: 0..3 Cdfs::new(qctx); 4..7 loaded by orc_rate_load_cdfs (Cdfs::new(0) until then).
+CoefCdfs g_slots[8];
+std::once_flag g_slots_once;
+
+const CoefCdfs& coef_cdfs(int slot) {
+    std::call_once(g_slots_once, [] {
+        for (int q = 0; q < 8; q++) {
+            orc_coef_cdfs r;
+            raw_defaults(q < 4 ? q : 0, r);
+            build_coef(r, g_slots[q]);
+        }
     });
-    return c[qctx];
+    return g_slots[slot];
 }
 
 // get_lo_ctx_2d (src/coeffs.rs:46)
@@ -329,4 +369,13 @@ extern "C" const uint32_t* orc_scan(int tx_size) {
 
 extern "C" float orc_skip_cost(int qctx, int cls, int ctx, int s) {
     return cdf_cost(coef_cdfs(qctx).txb_skip[cls][ctx], s);
+}
+
+extern "C" void orc_rate_default_cdfs(int qctx, orc_coef_cdfs* out) { raw_defaults(qctx, *out); }
+
+extern "C" int orc_rate_load_cdfs(int slot, const orc_coef_cdfs* cdfs) {
+    if (slot < 4 || slot > 7 || !cdfs) return -1;
+    coef_cdfs(0);
+    build_coef(*cdfs, g_slots[slot]);
+    return 0;
 }

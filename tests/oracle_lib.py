@@ -387,3 +387,43 @@ def cdf_tables():
         vals = np.array([int(v) for v in re.findall(r"-?\d+", m.group(4))], np.int64)
         out[m.group(2)] = vals.reshape(dims)
     return out
+
+
+def rate_default_cdfs(qctx):
+    import maroontree_amd as mt
+    L = lib()
+    L.orc_rate_default_cdfs.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    out = np.zeros(1, mt.COEF_CDFS_DTYPE)
+    L.orc_rate_default_cdfs(qctx, out.ctypes.data)
+    return out[0]
+
+
+def rate_load_cdfs(slot, cdfs):
+    import maroontree_amd as mt
+    L = lib()
+    L.orc_rate_load_cdfs.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    rec = np.ascontiguousarray(np.asarray(cdfs, mt.COEF_CDFS_DTYPE).reshape(1))
+    assert L.orc_rate_load_cdfs(slot, rec.ctypes.data) == 0
+
+
+def random_cdfs(rng):
+    """A valid random snapshot: every row strictly increasing within 1..32767."""
+    import maroontree_amd as mt
+    c = np.zeros(1, mt.COEF_CDFS_DTYPE)[0]
+
+    def rows(shape, k):
+        a = np.sort(rng.integers(1, 32768, shape + (k,)), axis=-1)
+        a += np.arange(k)  # strictly increasing
+        return np.minimum(a, 32767 - (k - 1) + np.arange(k)).astype(np.uint16)
+    c["skip"] = rows((5, 13), 1)[..., 0]
+    c["base_tok"] = rows((5, 2, 42), 3)
+    c["br_tok"] = rows((4, 2, 21), 3)
+    c["eob_base"] = rows((5, 2, 4), 2)
+    c["eob_hi"] = rows((5, 2, 11), 1)[..., 0]
+    c["dc_sign"] = rows((2, 3), 1)[..., 0]
+    for e, k in enumerate(mt.EOB_BIN_K):
+        c["eob_bin"][e, :k] = rows((), k)
+    c["txtp4"] = rows((13,), 6)
+    c["txtp8"] = rows((13,), 6)
+    c["txtp16"] = rows((13,), 4)
+    return c

@@ -114,7 +114,8 @@ def test_prune_masks_match_numpy(dev):
     z = rng.standard_normal((20, 98)).astype(np.float32) * 3
     z[3, 85:] = 0.5  # all mode scores tied: the lower modes win
     net = PR.PruneNet(PO.init_params(1), dev)
-    got = net.masks(torch.from_numpy(z).to(dev), 5, 0.2, 0.8).cpu().numpy().view(np.uint16)
+    lo, hi = [0.2, 0.1, 0.3, 0.25], [0.8, 0.9, 0.6, 0.75]
+    got = net.masks(torch.from_numpy(z).to(dev), 5, lo, hi).cpu().numpy().view(np.uint16)
     for b in range(20):
         order = sorted(range(13), key=lambda j: (-z[b, 85 + j], j))
         mm = (1 << 0) | (1 << 13)
@@ -123,8 +124,9 @@ def test_prune_masks_match_numpy(dev):
         for n in range(R.NODES):
             want = mm
             if n < 85:
+                L = R.node_geometry(n)[0]
                 ps = 1.0 / (1.0 + np.exp(-np.float64(z[b, n])))
-                want |= R.NO_SPLIT if ps < 0.2 else R.FORCE_SPLIT if ps > 0.8 else 0
+                want |= R.NO_SPLIT if ps < lo[L] else R.FORCE_SPLIT if ps > hi[L] else 0
             assert got[b, n] == want, (b, n)
 
 

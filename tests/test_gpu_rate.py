@@ -18,7 +18,7 @@ def cls_of(s):
     return (int(math.log2(w)) - 2 + int(math.log2(h)) - 2 + 1) >> 1
 
 
-def make_blocks(rng, n):
+def make_blocks(rng, n, qctxs=(0, 1, 2, 3)):
     sizes = [0, 5, 6, 13, 14, 1, 7, 8, 2, 9, 10, 3, 4]
     recs = np.zeros(n, mt.RATEBLK_DTYPE)
     pool = []
@@ -53,7 +53,7 @@ def make_blocks(rng, n):
             r["txtp_set"] = rng.integers(1, 4)
             r["txtp_mode"] = rng.integers(0, 13)
             r["txtp"] = rng.integers(0, 5 if r["txtp_set"] == 3 else 7)
-        r["qctx"] = rng.integers(0, 4)
+        r["qctx"] = qctxs[rng.integers(0, len(qctxs))]
         if rng.random() < 0.2:
             r["flags"] = 1
             r["skip_one"] = rng.uniform(0.01, 9.0)
@@ -126,3 +126,21 @@ def test_block_rate_invalid_descriptor_is_nan(dev):
     got = mt.block_rate(torch.from_numpy(recs.view(np.uint8).copy()).to(dev),
                         torch.zeros(1024, dtype=torch.int32, device=dev)).cpu().numpy()
     assert np.isnan(got[0]) and np.isnan(got[1]) and not np.isnan(got[2])
+
+
+def test_block_rate_with_a_loaded_snapshot(dev):
+    """Random decision CDF snapshots in slots 4 and 7: the device rate equals the oracle's
+    against the same snapshot, bit for bit."""
+    import oracle_lib as O2
+    rng = np.random.default_rng(77)
+    for slot in (4, 7):
+        snap = O2.random_cdfs(rng)
+        mt.rate_load_cdfs(slot, snap)
+        O2.rate_load_cdfs(slot, snap)
+    recs, pool = make_blocks(rng, 3000, qctxs=(4, 7))
+    print("rate snapshots loaded", flush=True)
+    got = mt.block_rate(torch.from_numpy(recs.view(np.uint8).copy()).to(dev),
+                        torch.from_numpy(pool).to(dev)).cpu().numpy()
+    exp = oracle_bits(recs, pool)
+    same = (got.view(np.uint32) == exp.view(np.uint32))
+    assert same.all(), f"{(~same).sum()} blocks differ; first {np.nonzero(~same)[0][:5]}"

@@ -88,3 +88,35 @@ def test_scans_are_permutations_in_diagonal_order():
         sc = O.scan(s)
         assert sorted(sc.tolist()) == list(range(O.ncoef(s)))
         assert sc[0] == 0
+
+
+# ---- decision CDF snapshots ------------------------------------------------------------
+def test_default_snapshot_reloaded_prices_like_its_slot():
+    """Cdfs::new(2) loaded into slot 5 prices every block exactly as slot 2 does."""
+    O.rate_load_cdfs(5, O.rate_default_cdfs(2))
+    rng = np.random.default_rng(3)
+    s8 = O.scan(1)
+    for _ in range(200):
+        cf = np.where(rng.random(64) < 0.3, rng.integers(-40, 41, 64), 0).astype(np.int32)
+        a = O.block_rate(cf, s8, O.rate_ctx(qctx=2))
+        b = O.block_rate(cf, s8, O.rate_ctx(qctx=5))
+        assert np.float32(a).view(np.uint32) == np.float32(b).view(np.uint32)
+
+
+def test_snapshot_skip_symbol_is_priced_from_the_loaded_cdf():
+    """An all-zero 8x8 block costs its txb_skip symbol: -log2((32768 - c) / 32768) for the
+    loaded skip CDF value c (the reference's cost table, Q22)."""
+    snap = O.rate_default_cdfs(0)
+    for c in (1024, 16384, 30000):
+        snap["skip"][1, 0] = c
+        O.rate_load_cdfs(6, snap)
+        got = O.block_rate(np.zeros(64, np.int32), O.scan(1), O.rate_ctx(qctx=6))
+        want = -math.log2((32768 - c) / 32768.0)
+        assert abs(got - want) < 1e-5, (c, got, want)
+
+
+def test_library_default_snapshots_match_oracle():
+    import maroontree_amd as mt
+    for q in range(4):
+        a, b = mt.rate_default_cdfs(q), O.rate_default_cdfs(q)
+        assert a.tobytes() == b.tobytes()
