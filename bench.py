@@ -50,20 +50,22 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruning leg")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-pruning leg and print its object")
-    ap.add_argument("--learned-train", type=int, default=48, help="training frames (1920x1080)")
-    ap.add_argument("--learned-eval", type=int, default=32, help="held-out frames per quality")
-    ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--learned-train", type=int, default=96, help="training frames (1920x1080)")
+    ap.add_argument("--learned-eval", type=int, default=16, help="held-out frames per quality")
+    ap.add_argument("--learned-epochs", type=int, default=40)
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
     ap.add_argument("--stream-images", type=int, default=48)
     ap.add_argument("--stream-max-side", type=int, default=2048)
-    ap.add_argument("--topk", type=int, default=13, help="intra modes kept per superblock")
-    ap.add_argument("--lo", default="0.002,0.005,0.02,0.05",
+    ap.add_argument("--topk", default="13,13,13,13,13",
+                    help="intra modes kept per superblock, per node level 64,32,16,8,4")
+    ap.add_argument("--lo", default="0,0.01,0.005,0",
                     help="NO_SPLIT below this split probability, per node level 64,32,16,8")
-    ap.add_argument("--hi", default="0.998,0.995,0.98,0.95",
+    ap.add_argument("--hi", default="1,1,1,1",
                     help="FORCE_SPLIT above this split probability, per node level 64,32,16,8")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    args.topk = [int(v) for v in str(args.topk).split(",")] * (1 if "," in str(args.topk) else 5)
     args.lo = [float(v) for v in str(args.lo).split(",")] * (1 if "," in str(args.lo) else 4)
     args.hi = [float(v) for v in str(args.hi).split(",")] * (1 if "," in str(args.hi) else 4)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -376,9 +378,12 @@ def bench_rdo(args, dev, torch, mt):
 
 def bench_learned(args, dev, torch, mt, return_net=False):
     """Learned pruning: the network trained on the GPU on exact-RDO labels of seeded 1920x1080
-    8-bit frames, then the pruned RDO against the full RDO on held-out frames at quality
-    30 / 50 / 70 / 90 (qindex 140 / 91 / 53 / 17): speedup and BD-rate (PSNR-Y, rate = the
-    RDO's final bits), and the same policy with the untrained network."""
+    8-bit frames, then, on held-out frames at quality 30 / 50 / 70 / 90 (qindex 140 / 91 /
+    53 / 17): the plain GPU form (one mt_intra_rdo call per frame, full search), the batched
+    form (mt_intra_rdo_batch over the frames, full search: identical results), and the
+    batched + learned form (one network forward over all superblocks, pruned search);
+    speedups, BD-rate (PSNR-Y, rate = the RDO's final bits) of the pruned search, and the
+    same policy with the untrained network."""
     from maroontree_amd import prune as PR
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
@@ -391,7 +396,8 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     data = PR.make_dataset(dev, train_seeds, width, height, bd, qs, flags)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    log(f"{data[0].shape[0]} superblocks labelled by the exact RDO in {t_data:.1f} s")
+    n_train_sb = int(data[0].shape[0])
+    log(f"{n_train_sb} superblocks labelled by the exact RDO in {t_data:.1f} s")
     net = PR.PruneNet(PR.init_params(1), dev)
     untrained = PR.PruneNet(PR.init_params(1), dev)
     t0 = time.perf_counter()
@@ -403,51 +409,69 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     srcs = [torch.from_numpy(img.view(np.int16)).to(dev) for img in eval_imgs]
     npx = width * height * len(srcs)
 
-    def run(q, model):
-        frames = [{"src": s, "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, 0)} for s in srcs]
+    def frames_at(q):
+        return [{"src": s, "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, 0)} for s in srcs]
+
+    def plain(q):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = [R.intra_rdo_batch([fr], flags)[0] for fr in frames_at(q)]
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    def batched(q, model):
+        frames = frames_at(q)
         torch.cuda.synchronize()
         t = time.perf_counter()
         if model is not None:
-            for fr in frames:
-                fr["prune"] = PR.prune_masks(model, fr["src"], bd, q, args.topk, args.lo, args.hi)
+            for fr, m in zip(frames, PR.prune_masks_batch(model, srcs, bd, q, args.topk, args.lo, args.hi)):
+                fr["prune"] = m
         out = R.intra_rdo_batch(frames, flags)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        bits = sum(o[2]["bits"] for o in out)
-        sse = sum(o[2]["sse"] for o in out)
-        cands = sum(o[2]["candidates"] for o in out)
-        return dt, bits, R.psnr(sse, npx, bd), cands
+        return time.perf_counter() - t, out
 
-    res = {"full": [], "learned": [], "untrained": []}
+    def summary(dt, out):
+        return (dt, sum(o[2]["bits"] for o in out), R.psnr(sum(o[2]["sse"] for o in out), npx, bd),
+                sum(o[2]["candidates"] for o in out))
+
+    res = {"plain": [], "batched": [], "learned": [], "untrained": []}
     for q in qs:
-        run(q, None)  # warm-up of this batch shape
-        for name, model in (("full", None), ("learned", net), ("untrained", untrained)):
-            res[name].append(run(q, model))
-        log(f"q {q}: full {res['full'][-1][0]:.3f} s, learned {res['learned'][-1][0]:.3f} s")
+        batched(q, None)  # warm-up of this batch shape
+        res["plain"].append(summary(*plain(q)))
+        res["batched"].append(summary(*batched(q, None)))
+        res["learned"].append(summary(*batched(q, net)))
+        res["untrained"].append(summary(*batched(q, untrained)))
+        log(f"q {q}: plain {res['plain'][-1][0]:.3f} s, batched {res['batched'][-1][0]:.3f} s, "
+            f"batched+learned {res['learned'][-1][0]:.3f} s")
+    T = {k: sum(r[0] for r in v) for k, v in res.items()}
+    full_pts = res["batched"]
     out = {
         "workload": f"{args.learned_train} training and {args.learned_eval} held-out seeded 1920x1080 8-bit "
                     "frames; labels from the exact full RDO (partitions 64..4, 13 modes + angle deltas + "
-                    "filter intra); pruned RDO at qindex 140/91/53/17 vs the full RDO",
+                    "filter intra); qindex 140/91/53/17",
         "network": "conv3x3s2 1-16-32-64, pool, fc 257-64-98; 46178 params; bf16 MFMA, fp32 accumulation",
-        "train_superblocks": int(len(train_seeds) * ((width + 63) // 64) * ((height + 63) // 64)),
+        "train_superblocks": n_train_sb,
         "label_seconds": round(t_data, 2),
         "train_seconds": round(t_train, 2),
         "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
         "policy": {"topk_modes": args.topk, "no_split_below": args.lo, "force_split_above": args.hi},
+        "plain_mpx_per_s": round(npx * len(qs) / T["plain"] / 1e6, 2),
+        "batched_mpx_per_s": round(npx * len(qs) / T["batched"] / 1e6, 2),
+        "identical_plain_batched": all(abs(a[1] - b[1]) == 0 and a[2] == b[2]
+                                       for a, b in zip(res["plain"], res["batched"])),
+        "full_points": [[round(r[1], 1), round(r[2], 4)] for r in full_pts],
     }
-    full_t = sum(r[0] for r in res["full"])
     for name in ("learned", "untrained"):
-        t = sum(r[0] for r in res[name])
+        t = T[name]
         out[name] = {
-            "speedup_vs_full": round(full_t / t, 3),
-            "bd_rate_pct": round(PR.bd_rate([r[1] for r in res["full"]], [r[2] for r in res["full"]],
-                                            [r[1] for r in res[name]], [r[2] for r in res[name]]), 3),
-            "candidate_fraction": round(sum(r[3] for r in res[name]) / sum(r[3] for r in res["full"]), 4),
             "mpx_per_s": round(npx * len(qs) / t / 1e6, 2),
+            "speedup_vs_plain": round(T["plain"] / t, 3),
+            "speedup_vs_batched": round(T["batched"] / t, 3),
+            "bd_rate_pct": round(PR.bd_rate([r[1] for r in full_pts], [r[2] for r in full_pts],
+                                            [r[1] for r in res[name]], [r[2] for r in res[name]]), 3),
+            "candidate_fraction": round(sum(r[3] for r in res[name]) / sum(r[3] for r in full_pts), 4),
+            "points": [[round(r[1], 1), round(r[2], 4)] for r in res[name]],
         }
-    out["full"] = {"mpx_per_s": round(npx * len(qs) / full_t / 1e6, 2),
-                   "points": [[round(r[1], 1), round(r[2], 4)] for r in res["full"]]}
-    out["learned"]["points"] = [[round(r[1], 1), round(r[2], 4)] for r in res["learned"]]
     return (out, net) if return_net else out
 
 
@@ -522,7 +546,7 @@ def bench_stream(args, dev, torch, mt, net=None):
                 fr = {"src": srcs[i], "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, sub),
                       "lam": R.rdo_lambda(q, bd, sub)}
                 if model is not None:
-                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi)
+                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi)  # per frame
                 frames.append(fr)
             out = R.intra_rdo_batch(frames, flags)
             torch.cuda.synchronize()

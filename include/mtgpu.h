@@ -453,12 +453,12 @@ int mt_pnet_backward(const float* params_d, const float* targets_d, const float*
 /* One Adam step (bias-corrected; `step` counts from 1) on the fp32 parameters. */
 int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, int step, float lr,
                  float beta1, float beta2, float eps, void* stream);
-/* mt_intra_rdo prune masks [batch][MT_RDO_NODES] from the logits: the top-k intra modes of
- * each superblock (DC_PRED and filter intra always kept) on every node, and for nodes 0..84
- * of level L (0: 64x64 .. 3: 8x8) MT_RDO_NO_SPLIT when sigmoid(split logit) < lo4[L],
- * MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays of 4).  Between the thresholds a node keeps
- * both choices: the full search there. */
-int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, const float* lo4,
+/* mt_intra_rdo prune masks [batch][MT_RDO_NODES] from the logits: on a node of level L
+ * (0: 64x64 .. 4: 4x4) the superblock's topk5[L] best-scored intra modes (DC_PRED and
+ * filter intra always kept), and for nodes 0..84 MT_RDO_NO_SPLIT when sigmoid(split logit)
+ * < lo4[L], MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays).  Between the thresholds a node
+ * keeps both choices: the full search there. */
+int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
                         const float* hi4, uint16_t* prune_d, void* stream);
 
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin

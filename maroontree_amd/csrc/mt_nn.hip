@@ -276,22 +276,26 @@ __global__ void __launch_bounds__(256) sb_extract_kernel(mt_plane p, int C, int 
     }
 }
 
-// Per-level split thresholds of the prune masks (levels 0..3: 64x64 .. 8x8 nodes).
+// Per-level policy of the prune masks: split thresholds of levels 0..3 (64x64 .. 8x8
+// nodes) and the number of intra modes kept at levels 0..4 (64x64 .. 4x4).
 struct SplitThresholds {
     float lo[4], hi[4];
+    int topk[5];
 };
 
 // Pruning masks from the network's logits, one thread per (SB, node): the SB's top-k intra
-// modes by score (ties to the lower mode; DC always; filter intra with DC) and, for nodes
-// 0..84 of level L, NO_SPLIT when P(split) < lo[L], FORCE_SPLIT when P(split) > hi[L].
-// Nodes 85..340 (4x4) take the mode mask only.
-__global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict__ logits, int B, int topk,
+// modes by score, k = topk[level of the node] (ties to the lower mode; DC always; filter
+// intra with DC) and, for nodes 0..84 of level L, NO_SPLIT when P(split) < lo[L],
+// FORCE_SPLIT when P(split) > hi[L].  Nodes 85..340 (4x4) take the mode mask only.
+__global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict__ logits, int B,
                                                          SplitThresholds th, uint16_t* __restrict__ prune) {
     const long n = (long)B * MT_RDO_NODES;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long b = i / MT_RDO_NODES;
         const int node = (int)(i % MT_RDO_NODES);
         const float* z = logits + b * kOut;
+        const int lv = node >= 85 ? 4 : node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
+        const int topk = th.topk[lv];
         uint16_t m = (1u << 0) | (1u << 13);
         for (int j = 0; j < 13; ++j) {
             int rank = 0;
@@ -538,11 +542,15 @@ extern "C" int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, f
     return MT_OK;
 }
 
-extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, const float* lo4, const float* hi4,
-                                   uint16_t* prune_d, void* stream) {
-    if (!logits_d || !prune_d || !lo4 || !hi4 || batch <= 0 || topk < 0 || topk > 13)
+extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
+                                   const float* hi4, uint16_t* prune_d, void* stream) {
+    if (!logits_d || !prune_d || !lo4 || !hi4 || !topk5 || batch <= 0)
         return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: bad arguments");
     SplitThresholds th;
+    for (int L = 0; L < 5; ++L) {
+        if (topk5[L] < 0 || topk5[L] > 13) return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: topk");
+        th.topk[L] = topk5[L];
+    }
     for (int L = 0; L < 4; ++L) {
         if (!(lo4[L] <= hi4[L])) return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: lo > hi");
         th.lo[L] = lo4[L];
@@ -552,7 +560,7 @@ extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, int topk, c
     int rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     hipLaunchKernelGGL(prune_mask_kernel, dim3(grid_for((long)batch * MT_RDO_NODES, cus)), dim3(256), 0,
-                       (hipStream_t)stream, logits_d, batch, topk, th, prune_d);
+                       (hipStream_t)stream, logits_d, batch, th, prune_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

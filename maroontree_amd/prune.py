@@ -32,7 +32,8 @@ def _sigs():
     L.mt_pnet_forward.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, vp]
     L.mt_pnet_backward.argtypes = [vp, vp, vp, i32, f32, f32, vp, vp, vp, vp]
     L.mt_pnet_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, vp]
-    L.mt_pnet_prune_masks.argtypes = [vp, i32, i32, ctypes.POINTER(f32), ctypes.POINTER(f32), vp, vp]
+    L.mt_pnet_prune_masks.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(f32), ctypes.POINTER(f32),
+                                      vp, vp]
     L._pnet_sigs = True
     return L
 
@@ -109,13 +110,15 @@ class PruneNet:
                                     lr, b1, b2, eps, _stream_ptr(stream)))
 
     def masks(self, logits, topk, lo, hi, stream=None):
-        """Prune masks; lo / hi: a split threshold each, or 4 per node level (64x64 .. 8x8)."""
+        """Prune masks; topk: modes kept, one value or 5 per node level (64x64 .. 4x4); lo /
+        hi: split thresholds, one value or 4 per node level (64x64 .. 8x8)."""
         import torch
         B = logits.shape[0]
+        tk = (ctypes.c_int * 5)(*([topk] * 5 if np.isscalar(topk) else topk))
         lo4 = (ctypes.c_float * 4)(*([lo] * 4 if np.isscalar(lo) else lo))
         hi4 = (ctypes.c_float * 4)(*([hi] * 4 if np.isscalar(hi) else hi))
         out = torch.empty((B, R.NODES), dtype=torch.int16, device=self.dev)
-        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, topk, lo4, hi4, _p(out), _stream_ptr(stream)))
+        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, tk, lo4, hi4, _p(out), _stream_ptr(stream)))
         return out
 
 
@@ -207,6 +210,21 @@ def train(net, data, bd, epochs=12, batch=256, lr=2e-3, seed=0, log=None):
         if log:
             log(f"epoch {ep}: loss {hist[-1]:.4f}")
     return hist
+
+
+def prune_masks_batch(net, srcs, bd, qidx, topk, lo, hi):
+    """The network's masks for frames of one bit depth and qindex with one forward over all
+    their superblocks: a list of (nsb_i, 341) int16 cuda tensors."""
+    import torch
+    sbs = [extract_superblocks(s) for s in srcs]
+    allp = torch.cat(sbs)
+    q = torch.full((allp.shape[0],), qidx, dtype=torch.uint8, device=allp.device)
+    masks = net.masks(net.forward(allp, q, bd), topk, lo, hi)
+    out, o = [], 0
+    for s in sbs:
+        out.append(masks[o:o + s.shape[0]])
+        o += s.shape[0]
+    return out
 
 
 def prune_masks(net, src, bd, qidx, topk, lo, hi):

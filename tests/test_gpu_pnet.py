@@ -115,13 +115,14 @@ def test_prune_masks_match_numpy(dev):
     z[3, 85:] = 0.5  # all mode scores tied: the lower modes win
     net = PR.PruneNet(PO.init_params(1), dev)
     lo, hi = [0.2, 0.1, 0.3, 0.25], [0.8, 0.9, 0.6, 0.75]
-    got = net.masks(torch.from_numpy(z).to(dev), 5, lo, hi).cpu().numpy().view(np.uint16)
+    topk = [3, 5, 7, 13, 0]
+    got = net.masks(torch.from_numpy(z).to(dev), topk, lo, hi).cpu().numpy().view(np.uint16)
     for b in range(20):
         order = sorted(range(13), key=lambda j: (-z[b, 85 + j], j))
-        mm = (1 << 0) | (1 << 13)
-        for j in order[:5]:
-            mm |= 1 << j
         for n in range(R.NODES):
+            mm = (1 << 0) | (1 << 13)
+            for j in order[:topk[R.node_geometry(n)[0]]]:
+                mm |= 1 << j
             want = mm
             if n < 85:
                 L = R.node_geometry(n)[0]
