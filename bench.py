@@ -50,14 +50,19 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruning leg")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-pruning leg and print its object")
-    ap.add_argument("--learned-train", type=int, default=96, help="training frames (1920x1080)")
-    ap.add_argument("--learned-eval", type=int, default=16, help="held-out frames per quality")
+    ap.add_argument("--learned-train", type=int, default=200, help="training frames (1920x1080)")
+    ap.add_argument("--learned-eval", type=int, default=50, help="held-out frames per quality")
     ap.add_argument("--learned-epochs", type=int, default=40)
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
-    ap.add_argument("--stream-images", type=int, default=48)
-    ap.add_argument("--stream-max-side", type=int, default=2048)
-    ap.add_argument("--topk", default="13,13,13,13,13",
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run only the uneven-stream leg (pruning off) and print its object")
+    ap.add_argument("--stream-images", type=int, default=256)
+    ap.add_argument("--stream-max-side", type=int, default=8192)
+    ap.add_argument("--topk", default="8,10,12,13,13",
                     help="intra modes kept per superblock, per node level 64,32,16,8,4")
+    ap.add_argument("--mode-cover", type=float, default=0.0,
+                    help="keep all modes where the kept ones cover the SB's modes with a lower "
+                         "estimated probability")
     ap.add_argument("--lo", default="0,0.01,0.005,0",
                     help="NO_SPLIT below this split probability, per node level 64,32,16,8")
     ap.add_argument("--hi", default="1,1,1,1",
@@ -100,6 +105,9 @@ def main():
         return
     if args.learned_only:
         print(json.dumps(bench_learned(args, dev, torch, mt)), flush=True)
+        return
+    if args.stream_only:
+        print(json.dumps(bench_stream(args, dev, torch, mt)), flush=True)
         return
 
     n = args.blocks
@@ -405,7 +413,7 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del data
-    eval_imgs = [W.synth_image(90000 + i, width, height, bd) for i in range(args.learned_eval)]
+    eval_imgs = W.synth_images([(90000 + i, width, height, bd) for i in range(args.learned_eval)])
     srcs = [torch.from_numpy(img.view(np.int16)).to(dev) for img in eval_imgs]
     npx = width * height * len(srcs)
 
@@ -424,7 +432,8 @@ def bench_learned(args, dev, torch, mt, return_net=False):
         torch.cuda.synchronize()
         t = time.perf_counter()
         if model is not None:
-            for fr, m in zip(frames, PR.prune_masks_batch(model, srcs, bd, q, args.topk, args.lo, args.hi)):
+            for fr, m in zip(frames, PR.prune_masks_batch(model, srcs, bd, q, args.topk, args.lo, args.hi,
+                                                          args.mode_cover)):
                 fr["prune"] = m
         out = R.intra_rdo_batch(frames, flags)
         torch.cuda.synchronize()
@@ -454,7 +463,8 @@ def bench_learned(args, dev, torch, mt, return_net=False):
         "label_seconds": round(t_data, 2),
         "train_seconds": round(t_train, 2),
         "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
-        "policy": {"topk_modes": args.topk, "no_split_below": args.lo, "force_split_above": args.hi},
+        "policy": {"topk_modes": args.topk, "no_split_below": args.lo, "force_split_above": args.hi,
+                   "mode_cover": args.mode_cover},
         "plain_mpx_per_s": round(npx * len(qs) / T["plain"] / 1e6, 2),
         "batched_mpx_per_s": round(npx * len(qs) / T["batched"] / 1e6, 2),
         "identical_plain_batched": all(abs(a[1] - b[1]) == 0 and a[2] == b[2]
@@ -527,7 +537,7 @@ def bench_stream(args, dev, torch, mt, net=None):
         sub = int(rng.choice([0, 2]))
         q = int(rng.choice([140, 91, 53, 17]))
         specs.append((w, h, bd, sub, q))
-    imgs = [W.synth_image(70000 + i, w, h, bd) for i, (w, h, bd, _, _) in enumerate(specs)]
+    imgs = W.synth_images([(70000 + i, w, h, bd) for i, (w, h, bd, _, _) in enumerate(specs)])
     srcs = [torch.from_numpy(im.view(np.int16)).to(dev) for im in imgs]
     steps = [((w + 63) // 64 - 1) + 2 * ((h + 63) // 64 - 1) + 1 for (w, h, _, _, _) in specs]
     order = sorted(range(len(specs)), key=lambda i: -steps[i])
@@ -546,7 +556,8 @@ def bench_stream(args, dev, torch, mt, net=None):
                 fr = {"src": srcs[i], "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, sub),
                       "lam": R.rdo_lambda(q, bd, sub)}
                 if model is not None:
-                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi)  # per frame
+                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi,
+                                                 args.mode_cover)  # per frame
                 frames.append(fr)
             out = R.intra_rdo_batch(frames, flags)
             torch.cuda.synchronize()

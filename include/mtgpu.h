@@ -457,9 +457,12 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
  * (0: 64x64 .. 4: 4x4) the superblock's topk5[L] best-scored intra modes (DC_PRED and
  * filter intra always kept), and for nodes 0..84 MT_RDO_NO_SPLIT when sigmoid(split logit)
  * < lo4[L], MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays).  Between the thresholds a node
- * keeps both choices: the full search there. */
+ * keeps both choices: the full search there.  Modes fall back the same way: when the
+ * product over the dropped modes j of (1 - sigmoid(mode logit j)) -- the estimated
+ * probability that the kept modes cover every mode the superblock uses -- is below
+ * mode_cover (0..1; 0 = never), the node keeps all 13 modes. */
 int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
-                        const float* hi4, uint16_t* prune_d, void* stream);
+                        const float* hi4, float mode_cover, uint16_t* prune_d, void* stream);
 
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and

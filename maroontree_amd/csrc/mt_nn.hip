@@ -277,16 +277,22 @@ __global__ void __launch_bounds__(256) sb_extract_kernel(mt_plane p, int C, int 
 }
 
 // Per-level policy of the prune masks: split thresholds of levels 0..3 (64x64 .. 8x8
-// nodes) and the number of intra modes kept at levels 0..4 (64x64 .. 4x4).
+// nodes), the number of intra modes kept at levels 0..4 (64x64 .. 4x4) and the confidence
+// below which a level falls back to all modes.
 struct SplitThresholds {
     float lo[4], hi[4];
     int topk[5];
+    float cover;
 };
 
 // Pruning masks from the network's logits, one thread per (SB, node): the SB's top-k intra
 // modes by score, k = topk[level of the node] (ties to the lower mode; DC always; filter
 // intra with DC) and, for nodes 0..84 of level L, NO_SPLIT when P(split) < lo[L],
 // FORCE_SPLIT when P(split) > hi[L].  Nodes 85..340 (4x4) take the mode mask only.
+// Confidence fallback of the modes: the mode scores are independent probabilities that a
+// leaf of the SB uses the mode, so prod over the dropped modes of (1 - p_j) (ascending j)
+// estimates the probability that the kept modes cover every mode the SB uses; below
+// `cover` the node keeps all 13 modes.
 __global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict__ logits, int B,
                                                          SplitThresholds th, uint16_t* __restrict__ prune) {
     const long n = (long)B * MT_RDO_NODES;
@@ -297,11 +303,14 @@ __global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict
         const int lv = node >= 85 ? 4 : node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
         const int topk = th.topk[lv];
         uint16_t m = (1u << 0) | (1u << 13);
+        float covered = 1.0f;
         for (int j = 0; j < 13; ++j) {
             int rank = 0;
             for (int q = 0; q < 13; ++q) rank += (z[85 + q] > z[85 + j] || (z[85 + q] == z[85 + j] && q < j)) ? 1 : 0;
             if (rank < topk) m |= (uint16_t)(1u << j);
+            else if (j != 0) covered *= 1.0f - 1.0f / (1.0f + expf(-z[85 + j]));
         }
+        if (covered < th.cover) m |= 0x1fffu;
         if (node < 85) {
             const int L = node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
             const float ps = 1.0f / (1.0f + expf(-z[node]));
@@ -543,10 +552,13 @@ extern "C" int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, f
 }
 
 extern "C" int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
-                                   const float* hi4, uint16_t* prune_d, void* stream) {
+                                   const float* hi4, float mode_cover, uint16_t* prune_d, void* stream) {
     if (!logits_d || !prune_d || !lo4 || !hi4 || !topk5 || batch <= 0)
         return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: bad arguments");
+    if (!(mode_cover >= 0.0f && mode_cover <= 1.0f))
+        return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: mode_cover outside [0, 1]");
     SplitThresholds th;
+    th.cover = mode_cover;
     for (int L = 0; L < 5; ++L) {
         if (topk5[L] < 0 || topk5[L] > 13) return mtlib::set_error(MT_ERR_ARG, "mt_pnet_prune_masks: topk");
         th.topk[L] = topk5[L];

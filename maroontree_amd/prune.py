@@ -33,7 +33,7 @@ def _sigs():
     L.mt_pnet_backward.argtypes = [vp, vp, vp, i32, f32, f32, vp, vp, vp, vp]
     L.mt_pnet_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, vp]
     L.mt_pnet_prune_masks.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(f32), ctypes.POINTER(f32),
-                                      vp, vp]
+                                      f32, vp, vp]
     L._pnet_sigs = True
     return L
 
@@ -109,16 +109,18 @@ class PruneNet:
         _check(_sigs().mt_pnet_adam(_p(self.params), _p(self.grads), _p(self.m), _p(self.v), self.step,
                                     lr, b1, b2, eps, _stream_ptr(stream)))
 
-    def masks(self, logits, topk, lo, hi, stream=None):
+    def masks(self, logits, topk, lo, hi, cover=0.0, stream=None):
         """Prune masks; topk: modes kept, one value or 5 per node level (64x64 .. 4x4); lo /
-        hi: split thresholds, one value or 4 per node level (64x64 .. 8x8)."""
+        hi: split thresholds, one value or 4 per node level (64x64 .. 8x8); cover: the
+        estimated coverage of the kept modes below which a node keeps all modes."""
         import torch
         B = logits.shape[0]
         tk = (ctypes.c_int * 5)(*([topk] * 5 if np.isscalar(topk) else topk))
         lo4 = (ctypes.c_float * 4)(*([lo] * 4 if np.isscalar(lo) else lo))
         hi4 = (ctypes.c_float * 4)(*([hi] * 4 if np.isscalar(hi) else hi))
         out = torch.empty((B, R.NODES), dtype=torch.int16, device=self.dev)
-        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, tk, lo4, hi4, _p(out), _stream_ptr(stream)))
+        _check(_sigs().mt_pnet_prune_masks(_p(logits), B, tk, lo4, hi4, float(cover), _p(out),
+                                           _stream_ptr(stream)))
         return out
 
 
@@ -171,8 +173,9 @@ def make_dataset(dev, seeds, width, height, bd, qidxs, flags, images=None):
     import torch
     from . import workload as W
     frames = []
-    for i, seed in enumerate(seeds):
-        img = images[i] if images is not None else W.synth_image(seed, width, height, bd)
+    if images is None:
+        images = W.synth_images([(seed, width, height, bd) for seed in seeds])
+    for i, img in enumerate(images):
         q = int(qidxs[i % len(qidxs)])
         frames.append({"src": torch.from_numpy(img.view(np.int16)).to(dev), "bd": bd, "qidx": q,
                        "qm_level": _lib.qm_level_law(q, 0)})
@@ -212,14 +215,14 @@ def train(net, data, bd, epochs=12, batch=256, lr=2e-3, seed=0, log=None):
     return hist
 
 
-def prune_masks_batch(net, srcs, bd, qidx, topk, lo, hi):
+def prune_masks_batch(net, srcs, bd, qidx, topk, lo, hi, cover=0.0):
     """The network's masks for frames of one bit depth and qindex with one forward over all
     their superblocks: a list of (nsb_i, 341) int16 cuda tensors."""
     import torch
     sbs = [extract_superblocks(s) for s in srcs]
     allp = torch.cat(sbs)
     q = torch.full((allp.shape[0],), qidx, dtype=torch.uint8, device=allp.device)
-    masks = net.masks(net.forward(allp, q, bd), topk, lo, hi)
+    masks = net.masks(net.forward(allp, q, bd), topk, lo, hi, cover)
     out, o = [], 0
     for s in sbs:
         out.append(masks[o:o + s.shape[0]])
@@ -227,13 +230,13 @@ def prune_masks_batch(net, srcs, bd, qidx, topk, lo, hi):
     return out
 
 
-def prune_masks(net, src, bd, qidx, topk, lo, hi):
+def prune_masks(net, src, bd, qidx, topk, lo, hi, cover=0.0):
     """The network's masks for one frame: (nsb, 341) int16 cuda tensor."""
     import torch
     sbs = extract_superblocks(src)
     q = torch.full((sbs.shape[0],), qidx, dtype=torch.uint8, device=src.device)
     logits = net.forward(sbs, q, bd)
-    return net.masks(logits, topk, lo, hi)
+    return net.masks(logits, topk, lo, hi, cover)
 
 
 def init_params(seed):

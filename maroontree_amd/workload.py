@@ -20,61 +20,82 @@ SQUARE_TX = {4: 0, 8: 1, 16: 2, 32: 3, 64: 4}
 EOB_BIN_LUMA = {16: 0, 64: 5, 256: 11, 1024: 15}
 
 
+def _noise_octave(g, octave, height, width, ys, xs):
+    """Bilinear value noise of the lattice g at cell size `octave` (g[iy, ix] gathered by
+    repeating lattice values, which equals the direct gather)."""
+    fy, fx = ys / octave, xs / octave
+    iy, ix = fy.astype(int), fx.astype(int)
+    ty, tx = fy - iy, fx - ix
+    if g.ndim == 3:
+        ty, tx = ty[..., None], tx[..., None]
+    up = g.repeat(octave, axis=0).repeat(octave, axis=1)  # up[y, x] = g[y // octave, x // octave]
+    g00, g01 = up[:height, :width], up[:height, octave:octave + width]
+    g10, g11 = up[octave:octave + height, :width], up[octave:octave + height, octave:octave + width]
+    return g00 * (1 - tx) * (1 - ty) + g01 * tx * (1 - ty) + g10 * (1 - tx) * ty + g11 * tx * ty
+
+
+def _rect_mask(rng, width, height, xs, ys):
+    """One rectangle of the recipe: centre, half sizes and (half the time) an oblique angle;
+    returns (y0, x0, mask over its bounding box) or None when the box is empty."""
+    cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+    hw, hh = rng.uniform(8, max(width / 6, 9)), rng.uniform(8, max(height / 6, 9))
+    th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
+    r = np.hypot(hw, hh) + 2
+    x0, x1 = max(int(cx - r), 0), min(int(cx + r) + 1, width)
+    y0, y1 = max(int(cy - r), 0), min(int(cy + r) + 1, height)
+    bx, by = xs[:, x0:x1], ys[y0:y1]
+    u = (bx - cx) * np.cos(th) + (by - cy) * np.sin(th)
+    v = -(bx - cx) * np.sin(th) + (by - cy) * np.cos(th)
+    return y0, x0, (np.abs(u) <= hw) & (np.abs(v) <= hh)
+
+
 def synth_image(seed, width, height, bd=10):
+    """The seeded test image, uint16 (height, width) at bit depth bd."""
     rng = np.random.default_rng(seed)
-    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    ys = np.arange(height, dtype=np.float64)[:, None]
+    xs = np.arange(width, dtype=np.float64)[None, :]
     img = np.zeros((height, width))
     for _ in range(4):
         th = rng.uniform(0, 2 * np.pi)
-        proj = x * np.cos(th) + y * np.sin(th)
+        proj = xs * np.cos(th) + ys * np.sin(th)
         proj = (proj - proj.min()) / max(proj.max() - proj.min(), 1e-9)
         img += rng.uniform(20, 70) * proj
     for octave, amp in ((64, 40.0), (32, 20.0), (16, 10.0)):
-        gh, gw = height // octave + 2, width // octave + 2
-        g = rng.uniform(-1, 1, (gh, gw))
-        fy, fx = y / octave, x / octave
-        iy, ix = fy.astype(int), fx.astype(int)
-        ty, tx = fy - iy, fx - ix
-        v = (g[iy, ix] * (1 - tx) * (1 - ty) + g[iy, ix + 1] * tx * (1 - ty) +
-             g[iy + 1, ix] * (1 - tx) * ty + g[iy + 1, ix + 1] * tx * ty)
-        img += amp * v
+        g = rng.uniform(-1, 1, (height // octave + 2, width // octave + 2))
+        img += amp * _noise_octave(g, octave, height, width, ys, xs)
     for _ in range(20):
-        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
-        hw, hh = rng.uniform(8, max(width / 6, 9)), rng.uniform(8, max(height / 6, 9))
-        th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
-        u = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
-        v = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
-        img[(np.abs(u) <= hw) & (np.abs(v) <= hh)] = rng.uniform(0, 255)
+        y0, x0, m = _rect_mask(rng, width, height, xs, ys)
+        img[y0:y0 + m.shape[0], x0:x0 + m.shape[1]][m] = rng.uniform(0, 255)
     img = np.clip(np.rint(img), 0, 255) * (1 << (bd - 8))
     return img.astype(np.uint16)
+
+
+def synth_images(specs, threads=16):
+    """synth_image over a list of (seed, width, height, bd) on a thread pool (numpy's array
+    work releases the GIL); returns the images in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(lambda s: synth_image(*s), specs))
 
 
 def synth_rgb(seed, width, height):
     """The same recipe on three 8-bit channels (rectangles of one shape get an RGB colour);
     returns uint8 (height, width, 3)."""
     rng = np.random.default_rng(seed)
-    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    ys = np.arange(height, dtype=np.float64)[:, None]
+    xs = np.arange(width, dtype=np.float64)[None, :]
     img = np.zeros((height, width, 3))
     for _ in range(4):
         th = rng.uniform(0, 2 * np.pi)
-        proj = x * np.cos(th) + y * np.sin(th)
+        proj = xs * np.cos(th) + ys * np.sin(th)
         proj = (proj - proj.min()) / max(proj.max() - proj.min(), 1e-9)
         img += rng.uniform(20, 70, 3) * proj[..., None]
     for octave, amp in ((64, 40.0), (32, 20.0), (16, 10.0)):
-        gh, gw = height // octave + 2, width // octave + 2
-        g = rng.uniform(-1, 1, (gh, gw, 3))
-        fy, fx = y / octave, x / octave
-        iy, ix = fy.astype(int), fx.astype(int)
-        ty, tx = (fy - iy)[..., None], (fx - ix)[..., None]
-        img += amp * (g[iy, ix] * (1 - tx) * (1 - ty) + g[iy, ix + 1] * tx * (1 - ty) +
-                      g[iy + 1, ix] * (1 - tx) * ty + g[iy + 1, ix + 1] * tx * ty)
+        g = rng.uniform(-1, 1, (height // octave + 2, width // octave + 2, 3))
+        img += amp * _noise_octave(g, octave, height, width, ys, xs)
     for _ in range(20):
-        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
-        hw, hh = rng.uniform(8, max(width / 6, 9)), rng.uniform(8, max(height / 6, 9))
-        th = rng.uniform(0, np.pi) if rng.random() < 0.5 else 0.0
-        u = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
-        v = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
-        img[(np.abs(u) <= hw) & (np.abs(v) <= hh)] = rng.uniform(0, 255, 3)
+        y0, x0, m = _rect_mask(rng, width, height, xs, ys)
+        img[y0:y0 + m.shape[0], x0:x0 + m.shape[1]][m] = rng.uniform(0, 255, 3)
     return np.clip(np.rint(img), 0, 255).astype(np.uint8)
 
 

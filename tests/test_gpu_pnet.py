@@ -116,19 +116,28 @@ def test_prune_masks_match_numpy(dev):
     net = PR.PruneNet(PO.init_params(1), dev)
     lo, hi = [0.2, 0.1, 0.3, 0.25], [0.8, 0.9, 0.6, 0.75]
     topk = [3, 5, 7, 13, 0]
-    got = net.masks(torch.from_numpy(z).to(dev), topk, lo, hi).cpu().numpy().view(np.uint16)
+    cover = 0.02
+    got = net.masks(torch.from_numpy(z).to(dev), topk, lo, hi, cover).cpu().numpy().view(np.uint16)
+    fallbacks = 0
     for b in range(20):
         order = sorted(range(13), key=lambda j: (-z[b, 85 + j], j))
+        p = 1.0 / (1.0 + np.exp(-z[b, 85:].astype(np.float64)))
         for n in range(R.NODES):
             mm = (1 << 0) | (1 << 13)
-            for j in order[:topk[R.node_geometry(n)[0]]]:
+            k = topk[R.node_geometry(n)[0]]
+            for j in order[:k]:
                 mm |= 1 << j
+            covered = np.prod([1.0 - p[j] for j in order[k:] if j != 0])
+            if covered < cover:
+                mm |= 0x1FFF
+                fallbacks += 1
             want = mm
             if n < 85:
                 L = R.node_geometry(n)[0]
                 ps = 1.0 / (1.0 + np.exp(-np.float64(z[b, n])))
                 want |= R.NO_SPLIT if ps < lo[L] else R.FORCE_SPLIT if ps > hi[L] else 0
             assert got[b, n] == want, (b, n)
+    assert 0 < fallbacks < 20 * R.NODES  # the fallback fires on some nodes, not all
 
 
 def test_training_on_rdo_labels_lowers_the_loss(dev):
