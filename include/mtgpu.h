@@ -187,13 +187,15 @@ typedef struct {
     int32_t vis_w, vis_h;
     int32_t pad;
 } mt_plane;
-#define MT_MAX_PLANES 32
+#define MT_MAX_PLANES 64
 
-/* Intra modes (the reference's numbering, src/intrapred.rs:54-82) and filter intra. */
+/* Intra modes (the reference's numbering, src/intrapred.rs:54-82), chroma-from-luma (a
+ * candidate mode of mt_eval_candidates only) and filter intra. */
 enum {
     MT_PRED_DC = 0, MT_PRED_V = 1, MT_PRED_H = 2, MT_PRED_D45 = 3, MT_PRED_D135 = 4,
     MT_PRED_D113 = 5, MT_PRED_D157 = 6, MT_PRED_D203 = 7, MT_PRED_D67 = 8,
     MT_PRED_SMOOTH = 9, MT_PRED_SMOOTH_V = 10, MT_PRED_SMOOTH_H = 11, MT_PRED_PAETH = 12,
+    MT_PRED_CFL = 13,
     MT_PRED_FILTER0 = 16, MT_PRED_FILTER4 = 20  /* filter intra modes 0..4 */
 };
 enum { MT_PRED_HAVE_TR = 1, MT_PRED_HAVE_BL = 2, MT_PRED_SMOOTH_NB = 4 };
@@ -281,15 +283,25 @@ int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int
                   const int32_t* pred_d, const int32_t* resid_d, mt_distres* out_d, void* stream);
 
 /* ---- batched candidate evaluation -------------------------------------------------- */
-/* One intra candidate: 48 bytes in HBM.  The block is one transform block of tx_size. */
+/* One intra candidate: 48 bytes in HBM.  The block is one transform block of tx_size.
+ * A chroma-from-luma candidate (mode MT_PRED_CFL, square 4..32 chroma blocks) predicts
+ * cfl_pred_pixel(dc_pred, alpha, luma AC) (src/intrapred.rs:1194) with the DC prediction
+ * from the reconstruction plane and the AC of cfl_ac_sub / cfl_ac_444 (:1112, :1133) from
+ * the luma reconstruction plane at (x << ss_x, y << ss_y); the alpha is cfl_best_alpha's
+ * (:1218, against the source) unless MT_CAND_CFL_ALPHA_GIVEN; mt_candres.alpha reports it.
+ * For it, pflags holds MT_CAND_CFL_* (not the MT_PRED_* flags), angle_delta the given
+ * alpha, and txtp_mode the luma reconstruction plane's index (txtp_set must be 0: chroma
+ * codes no transform type). */
+enum { MT_CAND_CFL_SS_HOR = 8, MT_CAND_CFL_SS_VER = 16, MT_CAND_CFL_ALPHA_GIVEN = 32 };
 typedef struct {
     uint32_t offset;      /* element offset of the candidate's W*H slots in the work pools */
     uint16_t x, y;        /* block origin (the source and reconstruction planes share it) */
     uint8_t tx_size;      /* AV1 TxSize, one of the 13 sizes the reference transforms */
     uint8_t tx_type;      /* AV1 TxType */
     uint8_t mode;         /* MT_PRED_* */
-    int8_t angle_delta;   /* -3..3 */
-    uint8_t pflags;       /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB */
+    int8_t angle_delta;   /* -3..3 (MT_PRED_CFL: the given alpha, -16..16) */
+    uint8_t pflags;       /* MT_PRED_HAVE_TR | MT_PRED_HAVE_BL | MT_PRED_SMOOTH_NB (MT_PRED_CFL:
+                             MT_CAND_CFL_*) */
     uint8_t planes;       /* bits 0-1: source plane index, bits 2-3: reconstruction plane index
                              (low bits; the high bits are in planes_hi) */
     uint8_t qidx;         /* base_q_idx of the quantizer */
@@ -307,7 +319,7 @@ typedef struct {
     float extra_bits;     /* added to the coefficient bits (mode, angle delta, tx type ...) */
 } mt_cand;
 
-/* Result of one candidate: 40 bytes. */
+/* Result of one candidate: 48 bytes. */
 typedef struct {
     int64_t sse;      /* plain SSE of the reconstruction over the displayed part */
     int64_t dist;     /* decision distortion (weighted when wsse_scale > 0), as sse_recon */
@@ -316,6 +328,8 @@ typedef struct {
     float cost;       /* rd_cost(dist, lambda, bits) = (float)dist + lambda * bits; +inf if invalid */
     uint32_t nnz;     /* nonzero levels */
     int32_t status;   /* 0; MT_ERR_ARG invalid candidate; MT_ERR_UNSUPPORTED (size, type) */
+    int32_t alpha;    /* MT_PRED_CFL: the CfL alpha used; 0 otherwise */
+    int32_t pad;
 } mt_candres;
 
 /* The reference's candidate loop body (src/coder/partition_search.rs:471-520) for n
@@ -359,8 +373,35 @@ int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes, in
  *  - the final accounting re-prices every leaf and partition symbol with the final
  *    contexts against the final reconstruction.
  * Mode ids: 0..12 the intra modes (MT_PRED_*, delta 0); 13 + 6 (m - 1) + j directional
- * mode m in 1..8 with delta {-3, -2, -1, 1, 2, 3}[j]; 61..65 filter intra 0..4. */
-enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4 };
+ * mode m in 1..8 with delta {-3, -2, -1, 1, 2, 3}[j]; 61..65 filter intra 0..4.
+ *
+ * Chroma (mt_rdo_frame.chroma 4:2:0 or 4:4:4), in the same wavefront step right after the
+ * SB's luma reconstruction:
+ *  - chroma blocks: 4:2:0: each luma leaf of >= 8x8 at (x, y), size s, owns the s/2 chroma
+ *    block at (x/2, y/2); an 8x8 node split into 4x4 leaves owns one 4x4 chroma block, coded
+ *    with (and taking the luma mode of) its last 4x4 leaf.  4:4:4: each leaf owns the chroma
+ *    block of its own size and place; a 64x64 one is four 32x32 transform blocks in raster
+ *    order (the largest chroma transform), each predicted in turn;
+ *  - per chroma block, the uv modes 0..12 (angle delta 0) and, with MT_RDO_CFL where CfL is
+ *    allowed (luma block <= 32x32), CfL (13): DC prediction + alpha * the AC of the final
+ *    luma reconstruction, each plane's alpha from cfl_best_alpha; CfL with both alphas 0 is
+ *    DC_PRED and is not a CfL candidate.  Each (mode, plane, transform block) is one
+ *    candidate: the transform type the decoder derives from the uv mode
+ *    (chroma_tx_for_mode, src/coder/chroma_tx.rs:46; DCT_DCT at 32x32), the chroma
+ *    quantizer (qidx with uv_dc_delta / uv_ac_delta, uv_qm_level), plain SSE, the plane-1
+ *    coefficient rate with txb skip context 7 and dc sign context 0;
+ *  - cost(mode) = the rd_costs summed over U then V, each over its transform blocks in
+ *    raster order, + lambda * uv_mode_bits (src/coder/lossy_state.rs:1302: the uv_mode
+ *    symbol under the luma mode, from the CfL-allowed or the no-CfL CDF as AV1 codes it; the
+ *    delta-0 angle symbol of a directional mode at >= 8x8 blocks; the CfL sign and alpha
+ *    symbols); the lowest cost wins, ties to the lower uv mode;
+ *  - decision-time chroma samples inside the SB are the source, outside the final
+ *    reconstruction; the chroma edge-filter type reads the final uv modes outside the SB
+ *    and counts neighbours inside it as not smooth; the chosen chroma blocks are then
+ *    reconstructed in coding order, and the final accounting re-prices them with the final
+ *    contexts (mt_rdo_stats bits_uv, sse_u, sse_v). */
+enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4, MT_RDO_CFL = 8 };
+enum { MT_RDO_LUMA_ONLY = 0, MT_RDO_CHROMA_420 = 1, MT_RDO_CHROMA_444 = 2 };
 #define MT_RDO_NODES 341          /* quadtree nodes of one SB: 1 + 4 + 16 + 64 + 256 */
 #define MT_RDO_ALL_MODES 0x3FFF   /* prune mask bits 0-12: intra modes; bit 13: filter intra */
 #define MT_RDO_FORCE_SPLIT 0x4000 /* node must split (ignored at the minimum size) */
@@ -375,11 +416,19 @@ typedef struct {
     uint8_t bd;            /* 8, 10, 12 */
     uint8_t qidx;          /* base_q_idx */
     uint8_t qm_level;      /* 0..15 (15 flat) */
-    uint8_t pad0;
+    uint8_t chroma;        /* MT_RDO_LUMA_ONLY, MT_RDO_CHROMA_420, MT_RDO_CHROMA_444 */
     float lambda;          /* > 0 (mt_rdo_lambda gives the reference's decision lambda) */
     float wsse_scale;      /* RdDispatch::wsse_scale (1.0 shipped; 0 plain SSE) */
     const uint16_t* prune; /* optional (device): [nsb][MT_RDO_NODES] node masks; NULL = full */
     void* nodes;           /* out (device): [nsb][MT_RDO_NODES] mt_rdo_node */
+    /* chroma (ignored when luma only): sources of the chroma size (width / 2 x height / 2
+     * at 4:2:0), reconstructions out, the chroma quantizer (mt_chroma_q_deltas and
+     * mt_qm_chroma_level_law give the reference's laws) */
+    mt_plane src_u, src_v;
+    uint16_t *recon_u, *recon_v;
+    int32_t recon_uv_stride;
+    int8_t uv_dc_delta, uv_ac_delta;
+    uint8_t uv_qm_level, pad1;
 } mt_rdo_frame;
 
 typedef struct {
@@ -387,24 +436,32 @@ typedef struct {
     float none_cost;  /* PARTITION_NONE cost incl. its partition symbol (+inf if none) */
     uint8_t split;    /* 0 leaf, 1 split, 2 not coded (outside the frame / not searched) */
     uint8_t mode;     /* mode id of the cheapest NONE candidate */
-    uint8_t pad0, pad1;
+    uint8_t uv_mode;  /* the chroma block coded at this node: uv mode 0..12, 13 CfL; 255 none
+                         (chroma blocks sit at leaves, and at 4:2:0 8x8 nodes split into 4x4) */
+    uint8_t pad0;
+    int8_t cfl_alpha_u, cfl_alpha_v;  /* CfL alphas (0 unless uv_mode is 13) */
+    uint8_t pad1[2];
 } mt_rdo_node;
 
 typedef struct {
-    double bits;         /* final rate: coefficients + mode syntax + partition symbols */
-    int64_t sse;         /* SSE of the reconstruction against the source */
-    int64_t candidates;  /* decision candidates evaluated */
+    double bits;         /* final rate: coefficients + mode syntax + partition symbols (luma
+                            and chroma) */
+    int64_t sse;         /* luma SSE of the reconstruction against the source */
+    int64_t candidates;  /* decision candidates evaluated (luma and chroma) */
     int32_t leaves, sbs;
     int32_t steps, pad;
+    double bits_uv;      /* the chroma part of bits (coefficients and uv mode syntax) */
+    int64_t sse_u, sse_v;
 } mt_rdo_stats;
 
 /* Runs the RDO of one frame on `stream` and waits for it.  Returns MT_OK, MT_ERR_ARG for
  * invalid parameters, or a device error. */
 int mt_intra_rdo(const mt_rdo_frame* frame, int flags, mt_rdo_stats* stats, void* stream);
-/* The RDO of several frames of any sizes, bit depths and quantizers at once (stats: one per
- * frame): up to MT_MAX_PLANES / 2 frames share every wavefront step, so each step's batch
- * evaluation holds the candidates of all of them; larger counts run in such groups.  Each
- * frame's result equals its own mt_intra_rdo. */
+/* The RDO of several frames of any sizes, bit depths, chroma formats and quantizers at once
+ * (stats: one per frame): up to 16 frames (and MT_MAX_PLANES planes: 2 per luma-only frame,
+ * 6 with chroma) share every wavefront step, so each step's batch evaluation holds the
+ * candidates of all of them; larger counts run in such groups.  Each frame's result equals
+ * its own mt_intra_rdo. */
 int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
                        void* stream);
 
@@ -414,6 +471,10 @@ int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rd
  * sub: 0 4:4:4, 1 4:2:2, 2 4:2:0.  mt_mode_lambda returns mlam. */
 float mt_rdo_lambda(int qidx, int bd, int sub);
 float mt_mode_lambda(int qidx, int bd, int sub);
+/* The reference's frame chroma quantizer laws (shipped tuning): chroma_dc_delta /
+ * chroma_ac_delta (src/quant.rs:204-240) and qm_chroma_level_law (:155).  sub as above. */
+int mt_chroma_q_deltas(int qidx, int sub, int* dc_delta, int* ac_delta);
+int mt_qm_chroma_level_law(int qidx, int sub);
 
 /* ---- learned candidate pruning (bf16 CNN on MFMA) ---------------------------------- */
 /* A small network replaces / feeds the reference's hand pruning (proxy_mode_beam

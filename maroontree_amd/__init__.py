@@ -9,7 +9,8 @@ from ._lib import (  # noqa: F401
     qm_level_law, tx_width, tx_height, tx_ncoef, tx_supported, fwd_txfm, inv_txfm, tx_chain,
     gen_tx_blocks, block_rate, RATEBLK_DTYPE, BLK_DTYPE, RES_DTYPE, PREDBLK_DTYPE, MtPlane,
     plane_views, intra_pred, distortion, DISTBLK_DTYPE, DISTRES_DTYPE, DIST_NO_RESID,
-    PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB, PRED_FILTER0, MAX_PLANES,
+    PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB, PRED_FILTER0, PRED_CFL, CAND_CFL_SS_HOR, CAND_CFL_SS_VER,
+    CAND_CFL_ALPHA_GIVEN, MAX_PLANES,
     eval_candidates, CAND_DTYPE, CANDRES_DTYPE, TXBLK_CHROMA, cfl, CFLBLK_DTYPE, CFL_SS_HOR, CFL_SS_VER,
     CFL_ALPHA_GIVEN, COEF_CDFS_DTYPE, EOB_BIN_K, RATE_SLOTS, RATE_SLOTS_FIXED, rate_default_cdfs,
     rate_load_cdfs, profile_begin, profile_end, PROFILE_SLOTS, TX_SIZES_REF, TX_SIZES_ALL, TX_TYPES_REF, TX_NAMES, TXTP_NAMES,

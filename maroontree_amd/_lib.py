@@ -65,7 +65,8 @@ CAND_DTYPE = np.dtype([("offset", "<u4"), ("x", "<u2"), ("y", "<u2"), ("tx_size"
                        ("skip_one", "<f4"), ("skip_zero", "<f4"), ("wsse_scale", "<f4"),
                        ("lambda", "<f4"), ("extra_bits", "<f4")])
 CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
-                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
+                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4"), ("alpha", "<i4"),
+                          ("pad", "<i4")])
 COEF_CDFS_DTYPE = np.dtype([("skip", "<u2", (5, 13)), ("base_tok", "<u2", (5, 2, 42, 3)),
                             ("br_tok", "<u2", (4, 2, 21, 3)), ("eob_base", "<u2", (5, 2, 4, 2)),
                             ("eob_hi", "<u2", (5, 2, 11)), ("dc_sign", "<u2", (2, 3)),
@@ -76,7 +77,9 @@ RATE_SLOTS_FIXED, RATE_SLOTS = 4, 8
 PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB = 1, 2, 4
 TXBLK_CHROMA = 0x40
 PRED_FILTER0 = 16
-MAX_PLANES = 32
+PRED_CFL = 13
+CAND_CFL_SS_HOR, CAND_CFL_SS_VER, CAND_CFL_ALPHA_GIVEN = 8, 16, 32
+MAX_PLANES = 64
 
 
 class MtPlane(ctypes.Structure):
@@ -337,7 +340,7 @@ def distortion(blocks, planes, pred, resid=None, stream=None):
 
 def eval_candidates(cands, planes, pool_elems, stream=None, out=None):
     """Evaluate n intra candidates.  cands: uint8 cuda tensor of n*48 bytes (CAND_DTYPE
-    records); planes: list as plane_views(); returns a uint8 cuda tensor of n*40 bytes
+    records); planes: list as plane_views(); returns a uint8 cuda tensor of n*48 bytes
     (CANDRES_DTYPE)."""
     import torch
     n = cands.numel() // CAND_DTYPE.itemsize

@@ -30,6 +30,11 @@ int tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_t
 // source plane index in each descriptor's pad byte).
 int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int32_t* pred_d,
                       int32_t* status_d, int16_t* resid_d, hipStream_t s);
+// The CfL candidates list_d[0 .. *count_d) (count <= n_max) of a candidate evaluation: the
+// CfL prediction from the DC prediction already in pred_d, src - prediction into resid_d,
+// the alpha into alpha_d[candidate].
+int cfl_eval_launch(const mt_cand* cands_d, const uint32_t* list_d, const uint32_t* count_d, int n_max,
+                    const PlaneSet& ps, int32_t* pred_d, int16_t* resid_d, int32_t* alpha_d, hipStream_t s);
 // mt_block_rate whose eob search uses the chain's nonzero counts.
 int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
                          const mt_txres* nnz_d, hipStream_t s);
@@ -47,12 +52,21 @@ struct ReconFrame {
     uint16_t* rec;
     int32_t rstride;
     int32_t qidx, bd_qm, pad;
+    // chroma: U and V sources and reconstructions, the chroma quantizer, the subsampling
+    mt_plane src_uv[2];
+    uint16_t* rec_uv[2];
+    int32_t uv_stride, uv_bd_qm, uv_dc, uv_ac, ss, pad2;
 };
 // The RDO's superblock reconstruction: one wave per superblock b walks the leaves
 // leaves_d[first_d[b] .. + count_d[b]) in coding order, predicting from and writing to the
 // reconstruction plane of frame frames_d[leaf.plane] (device array).
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
                     int nsb, const ReconFrame* frames_d, hipStream_t s);
+// The RDO's chroma reconstruction: one wave per superblock b walks the chroma transform
+// blocks desc_d[first_d[b] .. + count_d[b]) (descriptor fields as sb_recon_uv_kernel reads
+// them) of frame frames_d[desc.plane].
+int sb_recon_uv_launch(const mt_predblk* desc_d, const uint32_t* first_d, const uint32_t* count_d, int nsb,
+                       const ReconFrame* frames_d, hipStream_t s);
 // Cost in bits of symbol s of a raw AV1 CDF of nsym symbols (nsym - 1 values), through the
 // reference's cost_q_table (host).
 float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s);

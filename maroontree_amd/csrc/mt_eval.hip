@@ -21,6 +21,10 @@ namespace {
 
 using mtlib::PlaneSet;
 
+__global__ void __launch_bounds__(64) zero_u32_kernel(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += 64) p[i] = 0;
+}
+
 __device__ __forceinline__ int tx_w(int s) {
     constexpr int w[19] = {4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 4, 16, 8, 32, 16, 64};
     return s < 19 ? w[s] : 0;
@@ -31,7 +35,8 @@ __device__ __forceinline__ int tx_h(int s) {
 }
 
 // Splits each candidate into the stage descriptors; a candidate outside its planes or its
-// pool slot gets status MT_ERR_ARG and descriptors the stages reject.
+// pool slot gets status MT_ERR_ARG and descriptors the stages reject.  CfL candidates
+// predict DC_PRED first and are listed for the CfL stage (cfl_list, *cfl_count).
 __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restrict__ cands, int n,
                                                          PlaneSet planes, uint64_t pool_elems,
                                                          mt_predblk* __restrict__ pb,
@@ -39,7 +44,9 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
                                                          mt_distblk* __restrict__ db,
                                                          mt_rateblk* __restrict__ rb,
                                                          int8_t* __restrict__ qd,
-                                                         int32_t* __restrict__ status) {
+                                                         int32_t* __restrict__ status,
+                                                         uint32_t* __restrict__ cfl_list,
+                                                         uint32_t* __restrict__ cfl_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const mt_cand c = cands[i];
@@ -56,6 +63,17 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
         ok = (int)c.x + w <= s.width && (int)c.y + h <= s.height && (int)c.x + w <= r.width &&
              (int)c.y + h <= r.height;
     }
+    const bool cfl = c.mode == MT_PRED_CFL;
+    if (ok && cfl) {  // square 4..32, a luma plane holding the covered luma, no tx-type symbol
+        const int sx = (c.pflags & MT_CAND_CFL_SS_HOR) ? 1 : 0, sy = (c.pflags & MT_CAND_CFL_SS_VER) ? 1 : 0;
+        ok = w == h && w <= 32 && c.txtp_set == 0 && c.txtp_mode < planes.n && c.angle_delta >= -16 &&
+             c.angle_delta <= 16;
+        if (ok) {
+            const mt_plane& l = planes.p[c.txtp_mode];
+            ok = (((int)c.x + w) << sx) <= l.width && (((int)c.y + h) << sy) <= l.height;
+        }
+        if (ok) cfl_list[atomicAdd(cfl_count, 1u)] = (uint32_t)i;
+    }
     status[i] = ok ? MT_OK : MT_ERR_ARG;
     mt_predblk p{};
     p.out_offset = c.offset;
@@ -63,9 +81,9 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     p.y = c.y;
     p.bw = ok ? (uint8_t)w : 0;  // bw 0: the predictor rejects it
     p.bh = (uint8_t)h;
-    p.mode = c.mode;
-    p.angle_delta = c.angle_delta;
-    p.flags = c.pflags;
+    p.mode = cfl ? (uint8_t)MT_PRED_DC : c.mode;
+    p.angle_delta = cfl ? (int8_t)0 : c.angle_delta;
+    p.flags = cfl ? (uint8_t)0 : c.pflags;
     p.bd = (uint8_t)bd;
     p.plane = (uint8_t)ri;
     p.pad = (uint8_t)si;  // source plane of the residual the predictor writes alongside
@@ -98,7 +116,7 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     r.skip_ctx = c.skip_ctx;
     r.dcs_ctx = c.dcs_ctx;
     r.txtp_set = c.txtp_set;
-    r.txtp_mode = c.txtp_mode;
+    r.txtp_mode = cfl ? (uint8_t)0 : c.txtp_mode;  // CfL: txtp_mode names the luma plane
     r.txtp = c.txtp;
     r.qctx = c.qctx;
     r.flags = c.rflags;
@@ -114,6 +132,7 @@ __global__ void __launch_bounds__(256) eval_combine_kernel(const mt_cand* __rest
                                                            const mt_txres* __restrict__ txr,
                                                            const mt_distres* __restrict__ dr,
                                                            const float* __restrict__ bits,
+                                                           const int32_t* __restrict__ alpha,
                                                            mt_candres* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -132,6 +151,7 @@ __global__ void __launch_bounds__(256) eval_combine_kernel(const mt_cand* __rest
         r.bits = __fadd_rn(bits[i], c.extra_bits);
         // rd_cost (src/partition_rd.rs:140): distortion + lambda * rate, in f32
         r.cost = __fadd_rn((float)r.dist, __fmul_rn(c.lambda, r.bits));
+        r.alpha = c.mode == MT_PRED_CFL ? alpha[i] : 0;
     } else {
         r.sse = -1;
         r.dist = -1;
@@ -150,6 +170,9 @@ struct EvalWork {
     int8_t* qd = nullptr;
     int32_t* status = nullptr;
     int32_t* pstatus = nullptr;
+    uint32_t* cfl_list = nullptr;
+    uint32_t* cfl_count = nullptr;
+    int32_t* alpha = nullptr;
     mt_txres* txr = nullptr;
     mt_distres* dr = nullptr;
     float* bits = nullptr;
@@ -174,8 +197,10 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
         const size_t c = n < 4096 ? 4096 : n;
         if ((rc = grow(w.pb, c)) || (rc = grow(w.tb, c)) || (rc = grow(w.db, c)) ||
             (rc = grow(w.rb, c)) || (rc = grow(w.qd, 2 * c)) || (rc = grow(w.status, c)) || (rc = grow(w.pstatus, c)) ||
-            (rc = grow(w.txr, c)) || (rc = grow(w.dr, c)) || (rc = grow(w.bits, c)))
+            (rc = grow(w.txr, c)) || (rc = grow(w.dr, c)) || (rc = grow(w.bits, c)) ||
+            (rc = grow(w.cfl_list, c)) || (rc = grow(w.alpha, c)))
             return rc;
+        if (!w.cfl_count && (rc = grow(w.cfl_count, 1))) return rc;
         w.cap_n = c;
     }
     if (w.cap_pool < pool) {
@@ -203,17 +228,19 @@ int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool
     if (rc != MT_OK) return rc;
     ProfScope prof(kProfEval, s);
     const int g256 = (n + 255) / 256;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w.cfl_count, 1);
     hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
-                       w.pb, w.tb, w.db, w.rb, w.qd, w.status);
+                       w.pb, w.tb, w.db, w.rb, w.qd, w.status, w.cfl_list, w.cfl_count);
     MT_HIP_TRY(hipGetLastError());
-    // prediction and residual_pred in one pass
+    // prediction and residual_pred in one pass, then chroma-from-luma on the listed candidates
     if ((rc = intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
+    if ((rc = cfl_eval_launch(cands_d, w.cfl_list, w.cfl_count, n, ps, w.pred, w.resid, w.alpha, s))) return rc;
     if ((rc = tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s)))
         return rc;
     if ((rc = distortion_launch(w.db, n, ps, w.pred, w.recon, w.dr, s))) return rc;
     if ((rc = rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
     hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,
-                       w.pstatus, w.txr, w.dr, w.bits, res_d);
+                       w.pstatus, w.txr, w.dr, w.bits, w.alpha, res_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -320,22 +320,7 @@ __global__ void __launch_bounds__(256) pred_small_kernel(const mt_predblk* __res
     }
 }
 
-// ---- chroma from luma (src/intrapred.rs:1112-1260) -----------------------------------------
-
-__device__ __forceinline__ int cfl_pixel(int dc, int ac, int alpha, int maxv) {
-    const int diff = alpha * ac;
-    const int mag = (abs(diff) + 32) >> 6;
-    return clampi(dc + (diff < 0 ? -mag : mag), 0, maxv);
-}
-
-template <typename T> __device__ __forceinline__ T wave_sum(T v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-// One wave per block: cfl_ac_sub's subsampled, scaled luma minus its rounded mean;
-// cfl_best_alpha_strided's analytic alpha and its +/-3 energy search over the displayed part
-// of the source; cfl_pred_pixel over the whole block.
+// One wave per block of mt_cfl.
 __global__ __launch_bounds__(64) void cfl_kernel(const mt_cflblk* __restrict__ blocks, int n,
                                                  PlaneSet planes, int32_t* __restrict__ pred,
                                                  int32_t* __restrict__ ac_out,
@@ -357,83 +342,37 @@ __global__ __launch_bounds__(64) void cfl_kernel(const mt_cflblk* __restrict__ b
             if (lane == 0) alpha_out[b] = -128;
             continue;
         }
-        const int lcw = __builtin_ctz(cw), npx = cw * ch;
-        const int shift = 1 + (sv ? 0 : 1) + (sh ? 0 : 1);
-        const uint16_t* L = lp.data + (size_t)d.ly * lp.stride + d.lx;
-        int64_t part = 0;
-        for (int p = lane; p < npx; p += 64) {
-            const int y = p >> lcw, x = p & (cw - 1);
-            const uint16_t* r0 = L + (size_t)(y << (sv ? 1 : 0)) * lp.stride + (x << (sh ? 1 : 0));
-            int s = r0[0];
-            if (sh) s += r0[1];
-            if (sv) {
-                s += r0[lp.stride];
-                if (sh) s += r0[lp.stride + 1];
-            }
-            s <<= shift;
-            s_ac[p] = s;
-            part += s;
-        }
-        const int log2sz = lcw + __builtin_ctz(ch);
-        const int64_t sum = wave_sum(part) + ((int64_t(1) << log2sz) >> 1);
-        const int mean = (int)(sum >> log2sz);
-        __syncthreads();
-        for (int p = lane; p < npx; p += 64) s_ac[p] -= mean;
-        __syncthreads();
-        const int maxv = (1 << bd) - 1;
-        const int dc = d.dc;
-        const int vw = min(max(sp.vis_w - (int)d.x, 0), cw), vh = min(max(sp.vis_h - (int)d.y, 0), ch);
-        const uint16_t* S = sp.data + (size_t)d.y * sp.stride + d.x;
-        int alpha = d.alpha;
-        if (!(d.flags & MT_CFL_ALPHA_GIVEN)) {
-            int64_t num = 0, den = 0;
-            for (int p = lane; p < npx; p += 64) {
-                const int y = p >> lcw, x = p & (cw - 1);
-                if (x < vw && y < vh) {
-                    const int64_t a = s_ac[p];
-                    num += (int64_t)((int)S[(size_t)y * sp.stride + x] - dc) * a;
-                    den += a * a;
-                }
-            }
-            num = wave_sum(num);
-            den = wave_sum(den);
-            if (den == 0 || vw == 0 || vh == 0) {
-                alpha = 0;
-            } else {
-                const int64_t sg = num > 0 ? 1 : (num < 0 ? -1 : 0);
-                int64_t a0 = (64 * num + (den >> 1) * sg) / den;
-                a0 = a0 < -16 ? -16 : (a0 > 16 ? 16 : a0);
-                int64_t e[7] = {0, 0, 0, 0, 0, 0, 0};
-                for (int p = lane; p < npx; p += 64) {
-                    const int y = p >> lcw, x = p & (cw - 1);
-                    if (x < vw && y < vh) {
-                        const int s = S[(size_t)y * sp.stride + x];
-                        for (int k = 0; k < 7; ++k) {
-                            const int64_t dd = s - cfl_pixel(dc, s_ac[p], (int)a0 - 3 + k, maxv);
-                            e[k] += dd * dd;
-                        }
-                    }
-                }
-                int64_t best_e = INT64_MAX;
-                alpha = 0;
-                for (int k = 0; k < 7; ++k) {
-                    const int c = (int)a0 - 3 + k;
-                    const int64_t ek = wave_sum(e[k]);
-                    if (c < -16 || c > 16) continue;
-                    if (ek < best_e) {
-                        best_e = ek;
-                        alpha = c;
-                    }
-                }
-            }
-        }
+        const int alpha = cfl_block(lp, d.lx, d.ly, sh, sv, sp, d.x, d.y, cw, ch, bd, d.dc,
+                                    (d.flags & MT_CFL_ALPHA_GIVEN) != 0, d.alpha, s_ac, pred + d.out_offset,
+                                    ac_out ? ac_out + d.out_offset : nullptr, nullptr);
         if (lane == 0) alpha_out[b] = alpha;
-        int32_t* dst = pred + d.out_offset;
-        for (int p = lane; p < npx; p += 64) {
-            dst[p] = cfl_pixel(dc, s_ac[p], alpha, maxv);
-            if (ac_out) ac_out[d.out_offset + p] = s_ac[p];
-        }
-        __syncthreads();
+    }
+}
+
+// The CfL candidates of a candidate evaluation (indices list[0 .. *count)): their DC
+// prediction is already in the prediction pool (its first sample is the DC value); the CfL
+// prediction and src - prediction replace the pool's prediction and residual.
+__global__ __launch_bounds__(64) void cfl_eval_kernel(const mt_cand* __restrict__ cands,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count, PlaneSet planes,
+                                                      int32_t* __restrict__ pred, int16_t* __restrict__ resid,
+                                                      int32_t* __restrict__ alpha_out) {
+    __shared__ int32_t s_ac[32 * 32];
+    const uint32_t n = *count;
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
+        const uint32_t i = list[b];
+        const mt_cand c = cands[i];
+        const int w = 4 << c.tx_size;  // square 4..32 (checked by the setup)
+        const int si = (c.planes & 3) | ((c.planes_hi & 15) << 2);
+        const bool sh = c.pflags & MT_CAND_CFL_SS_HOR, sv = c.pflags & MT_CAND_CFL_SS_VER;
+        const int bdi = (c.bd_qm >> 4) & 3;
+        const int bd = bdi == 0 ? 8 : bdi == 1 ? 10 : 12;
+        const int dc = pred[c.offset];
+        const int alpha = cfl_block(planes.p[c.txtp_mode], (int)c.x << (sh ? 1 : 0), (int)c.y << (sv ? 1 : 0), sh,
+                                    sv, planes.p[si], c.x, c.y, w, w, bd, dc,
+                                    (c.pflags & MT_CAND_CFL_ALPHA_GIVEN) != 0, c.angle_delta, s_ac,
+                                    pred + c.offset, nullptr, resid + c.offset);
+        if ((threadIdx.x & 63) == 0) alpha_out[i] = alpha;
     }
 }
 
@@ -447,7 +386,8 @@ int pred_init_device(int /*device*/) {
 
 int planes_from_host(const mt_plane* planes, int nplanes, const char* who, PlaneSet* ps) {
     if (nplanes < 1 || nplanes > MT_MAX_PLANES || !planes)
-        return set_error(MT_ERR_ARG, std::string(who) + ": 1..32 planes required");
+        return set_error(MT_ERR_ARG, std::string(who) + ": 1.." + std::to_string(MT_MAX_PLANES) +
+                                         " planes required");
     for (int i = 0; i < nplanes; ++i) {
         const mt_plane& p = planes[i];
         if (!p.data || p.stride <= 0 || p.width <= 0 || p.height <= 0 || p.width > p.stride ||
@@ -507,6 +447,21 @@ int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int
     const int gw = n < cus * 32 ? n : cus * 32;
     hipLaunchKernelGGL(intra_pred_kernel, dim3(gw), dim3(64), 0, s, blocks_d, L + 6 * cap, w.cnt + 6, ps,
                        pred_d, resid_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
+}  // namespace mtlib
+
+namespace mtlib {
+int cfl_eval_launch(const mt_cand* cands_d, const uint32_t* list_d, const uint32_t* count_d, int n_max,
+                    const PlaneSet& ps, int32_t* pred_d, int16_t* resid_d, int32_t* alpha_d, hipStream_t s) {
+    if (n_max <= 0) return MT_OK;
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    const int grid = n_max < cus * 16 ? n_max : cus * 16;
+    hipLaunchKernelGGL(cfl_eval_kernel, dim3(grid), dim3(64), 0, s, cands_d, list_d, count_d, ps, pred_d, resid_d,
+                       alpha_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -331,4 +331,106 @@ inline hipError_t upload_pred_tables() {
     return e;
 }
 
+// ---- chroma from luma (src/intrapred.rs:1112-1260) -----------------------------------------
+
+__device__ __forceinline__ int cfl_pixel(int dc, int ac, int alpha, int maxv) {
+    const int diff = alpha * ac;
+    const int mag = (abs(diff) + 32) >> 6;
+    return clampi(dc + (diff < 0 ? -mag : mag), 0, maxv);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One CfL block by one wave: cfl_ac_sub's subsampled, scaled luma minus its rounded mean
+// (cfl_ac_444 without subsampling) from the luma plane at (lx, ly); unless alpha_given,
+// cfl_best_alpha_strided's analytic alpha and its +/-3 energy search over the displayed part
+// of the source block at (x, y); cfl_pred_pixel over the whole block into dst (and the AC
+// into ac_dst, src - prediction into resid_dst, when given).  s_ac: LDS of cw * ch ints.
+// Returns the alpha.  The caller has checked that both blocks lie inside their planes.
+__device__ int cfl_block(const mt_plane& lp, int lx, int ly, bool sh, bool sv, const mt_plane& sp, int x, int y,
+                         int cw, int ch, int bd, int dc, bool alpha_given, int alpha_in, int32_t* s_ac,
+                         int32_t* dst, int32_t* ac_dst, int16_t* resid_dst) {
+    const int lane = threadIdx.x & 63;
+    const int lcw = __builtin_ctz(cw), npx = cw * ch;
+    const int shift = 1 + (sv ? 0 : 1) + (sh ? 0 : 1);
+    const uint16_t* L = lp.data + (size_t)ly * lp.stride + lx;
+    int64_t part = 0;
+    for (int p = lane; p < npx; p += 64) {
+        const int yy = p >> lcw, xx = p & (cw - 1);
+        const uint16_t* r0 = L + (size_t)(yy << (sv ? 1 : 0)) * lp.stride + (xx << (sh ? 1 : 0));
+        int s = r0[0];
+        if (sh) s += r0[1];
+        if (sv) {
+            s += r0[lp.stride];
+            if (sh) s += r0[lp.stride + 1];
+        }
+        s <<= shift;
+        s_ac[p] = s;
+        part += s;
+    }
+    const int log2sz = lcw + __builtin_ctz(ch);
+    const int64_t sum = wave_sum(part) + ((int64_t(1) << log2sz) >> 1);
+    const int mean = (int)(sum >> log2sz);
+    __syncthreads();
+    for (int p = lane; p < npx; p += 64) s_ac[p] -= mean;
+    __syncthreads();
+    const int maxv = (1 << bd) - 1;
+    const int vw = min(max(sp.vis_w - x, 0), cw), vh = min(max(sp.vis_h - y, 0), ch);
+    const uint16_t* S = sp.data + (size_t)y * sp.stride + x;
+    int alpha = alpha_in;
+    if (!alpha_given) {
+        int64_t num = 0, den = 0;
+        for (int p = lane; p < npx; p += 64) {
+            const int yy = p >> lcw, xx = p & (cw - 1);
+            if (xx < vw && yy < vh) {
+                const int64_t a = s_ac[p];
+                num += (int64_t)((int)S[(size_t)yy * sp.stride + xx] - dc) * a;
+                den += a * a;
+            }
+        }
+        num = wave_sum(num);
+        den = wave_sum(den);
+        if (den == 0 || vw == 0 || vh == 0) {
+            alpha = 0;
+        } else {
+            const int64_t sg = num > 0 ? 1 : (num < 0 ? -1 : 0);
+            int64_t a0 = (64 * num + (den >> 1) * sg) / den;
+            a0 = a0 < -16 ? -16 : (a0 > 16 ? 16 : a0);
+            int64_t e[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int p = lane; p < npx; p += 64) {
+                const int yy = p >> lcw, xx = p & (cw - 1);
+                if (xx < vw && yy < vh) {
+                    const int s = S[(size_t)yy * sp.stride + xx];
+                    for (int k = 0; k < 7; ++k) {
+                        const int64_t dd = s - cfl_pixel(dc, s_ac[p], (int)a0 - 3 + k, maxv);
+                        e[k] += dd * dd;
+                    }
+                }
+            }
+            int64_t best_e = INT64_MAX;
+            alpha = 0;
+            for (int k = 0; k < 7; ++k) {
+                const int c = (int)a0 - 3 + k;
+                const int64_t ek = wave_sum(e[k]);
+                if (c < -16 || c > 16) continue;
+                if (ek < best_e) {
+                    best_e = ek;
+                    alpha = c;
+                }
+            }
+        }
+    }
+    for (int p = lane; p < npx; p += 64) {
+        const int v = cfl_pixel(dc, s_ac[p], alpha, maxv);
+        dst[p] = v;
+        if (ac_dst) ac_dst[p] = s_ac[p];
+        if (resid_dst) resid_dst[p] = (int16_t)((int)S[(size_t)(p >> lcw) * sp.stride + (p & (cw - 1))] - v);
+    }
+    __syncthreads();
+    return alpha;
+}
+
 }  // namespace

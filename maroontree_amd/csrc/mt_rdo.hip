@@ -13,7 +13,16 @@
 //   rdo_decide_kernel  one wave per SB: bottom-up NONE / SPLIT level by level, leaves in
 //                      coding order, the final mode / partition maps and leaf descriptors;
 //   sb_recon_launch    one wave per SB reconstructs its leaves (mt_sbrecon.h).
-// Afterwards every leaf is re-priced with the final contexts in one batched evaluation.
+// With chroma, in the same step:
+//   rdo_uv_gen_kernel     one wave per SB: the chroma blocks of the decided leaves in coding
+//                         order and every (uv mode, plane, transform block) candidate, in a
+//                         fixed slot range per SB (unused slots are marked invalid);
+//   eval_launch           the batched evaluation of the chroma candidates;
+//   rdo_uv_decide_kernel  one wave per SB: per chroma block the cheapest uv mode (U + V +
+//                         lambda * uv mode bits), the uv map and the reconstruction list;
+//   sb_recon_uv_launch    one wave per SB reconstructs its chroma blocks (mt_sbrecon.h).
+// Afterwards every leaf and chroma block is re-priced with the final contexts in one batched
+// evaluation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +42,8 @@ using mtlib::PlaneSet;
 constexpr int kNodes = MT_RDO_NODES;
 constexpr int kModeIds = 66;
 constexpr int kMaxLeaves = 256;
+constexpr int kMaxUvBlocks = 256;  // chroma blocks of one SB (4:4:4 with 4x4 leaves)
+constexpr int kMaxUvDesc = 512;    // chroma reconstruction blocks of one SB (U and V)
 
 // Symbol bits of the frame-initial decision CDFs (Cdfs::new, src/coder/cdfs.rs:400).
 struct SymBits {
@@ -41,6 +52,9 @@ struct SymBits {
     float fuse[5][2];     // use_filter_intra at 4x4 .. 64x64 (av1 block sizes 0, 3, 6, 9, 12)
     float fmode[5];       // filter intra mode
     float part[4][4][2];  // [bl - 1][ctx][NONE, SPLIT]: 64, 32, 16 (10 symbols), 8 (4)
+    float uv[2][13][14];  // uv_mode [CfL allowed][luma mode][uv mode] (13 or 14 symbols)
+    float cfl_sign[8];    // CfL joint sign - 1
+    float cfl_alpha[6][16];
 };
 __constant__ SymBits c_sym;
 
@@ -97,9 +111,32 @@ struct FrameArgs {
     int W, H, C;
     int bd, qidx, qm, flags, minsz;
     float lambda, wsse;
-    int frame;     // index in the batch: planes 2 * frame (source) and 2 * frame + 1 (recon)
+    int frame;     // index in the batch (its ReconFrame)
+    int pbase;     // its planes in the batch's plane set: pbase source, pbase + 1 reconstruction,
+                   // with chroma pbase + 2 / + 3 U source / reconstruction, pbase + 4 / + 5 V
     uint32_t sb0;  // index of the frame's first SB in the batch's SB arrays
+    // chroma
+    int chroma;    // MT_RDO_LUMA_ONLY, MT_RDO_CHROMA_420, MT_RDO_CHROMA_444
+    int ss;        // 1 at 4:2:0 (both directions), 0 at 4:4:4
+    int Wc, Hc;    // chroma plane size
+    int uv_dc, uv_ac, uv_qm;
+    uint8_t* uvmode4;  // (Wc / 4) x (Hc / 4) chosen uv modes (CfL as 13)
 };
+
+// One chroma block of an SB: the node it is coded at, its place and size in the chroma
+// plane, its transform blocks (4 at a 4:4:4 64x64, else 1), the luma mode of its uv_mode
+// context, the luma block its availability follows (SB-relative, luma units) and where its
+// candidates, pool slots and reconstruction descriptors start inside the SB's ranges.
+struct UvBlk {
+    uint16_t node, cx, cy;
+    uint8_t cs, nt, nm, ymode;
+    uint8_t flags;  // 1 CfL allowed (and searched when kUvCfl), 2 angle delta coded (>= 8x8)
+    uint8_t ls;
+    uint8_t lx, ly;
+    uint16_t desc;
+    uint32_t cand, pool;
+};
+enum { kUvCflAllowed = 1, kUvAngle = 2 };
 
 // One SB of a step: its frame, its raster index in the frame and where its candidates /
 // pool slots start.
@@ -182,7 +219,7 @@ __device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, in
                         (lm >= MT_PRED_SMOOTH && lm <= MT_PRED_SMOOTH_H);
     c.pflags = (uint8_t)((have_tr(x - x0, y - y0, s) ? MT_PRED_HAVE_TR : 0) |
                          (have_bl(x - x0, y - y0, s) ? MT_PRED_HAVE_BL : 0) | (smooth ? MT_PRED_SMOOTH_NB : 0));
-    const int sp = 2 * f.frame, rp = 2 * f.frame + 1;
+    const int sp = f.pbase, rp = f.pbase + 1;
     c.planes = (uint8_t)((sp & 3) | ((rp & 3) << 2));
     c.planes_hi = (uint8_t)((sp >> 2) | ((rp >> 2) << 4));
     c.qidx = (uint8_t)f.qidx;
@@ -401,6 +438,7 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restr
             mode[id] = (uint8_t)bid;
             mt_rdo_node nd{};
             nd.mode = (uint8_t)bid;
+            nd.uv_mode = 255;
             if (!(fl & kSearched)) {
                 cost[id] = 0.0f;
                 split[id] = 2;
@@ -497,6 +535,278 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restr
     }
 }
 
+// ---- chroma ----------------------------------------------------------------------------
+
+// The transform type the decoder derives from a uv mode (chroma_tx_for_mode,
+// src/coder/chroma_tx.rs:46); DCT_DCT at 32x32, where the intra set has no other.
+__host__ __device__ inline int uv_tx_type(int m, int cs) {
+    if (cs >= 32) return MT_TXTP_DCT_DCT;
+    switch (m) {
+        case MT_PRED_PAETH: case MT_PRED_SMOOTH: case MT_PRED_D135: return MT_TXTP_ADST_ADST;
+        case MT_PRED_SMOOTH_V: case MT_PRED_V: case MT_PRED_D113: case MT_PRED_D67: return MT_TXTP_ADST_DCT;
+        case MT_PRED_SMOOTH_H: case MT_PRED_H: case MT_PRED_D157: case MT_PRED_D203: return MT_TXTP_DCT_ADST;
+        default: return MT_TXTP_DCT_DCT;
+    }
+}
+
+// uv_mode_bits (src/coder/lossy_state.rs:1302) of uv mode m (13 CfL with alphas au, av)
+// under luma mode ym: the uv_mode symbol from the CfL-allowed or the no-CfL CDF, the delta-0
+// angle symbol of a directional mode where angle deltas are coded, the CfL joint sign and
+// the alpha magnitudes.
+__device__ float uv_mode_bits(int ym, int m, int au, int av, int flags) {
+    const int allowed = (flags & kUvCflAllowed) ? 1 : 0;
+    if (m == MT_PRED_CFL) {
+        const int su = au == 0 ? 0 : au < 0 ? 1 : 2, sv = av == 0 ? 0 : av < 0 ? 1 : 2;
+        float b = __fadd_rn(c_sym.uv[1][ym][MT_PRED_CFL], c_sym.cfl_sign[su * 3 + sv - 1]);
+        if (su) b = __fadd_rn(b, c_sym.cfl_alpha[(su == 2) * 3 + sv][abs(au) - 1]);
+        if (sv) b = __fadd_rn(b, c_sym.cfl_alpha[(sv == 2) * 3 + su][abs(av) - 1]);
+        return b;
+    }
+    float b = c_sym.uv[allowed][ym][m];
+    if ((flags & kUvAngle) && m >= MT_PRED_V && m <= MT_PRED_D67) b = __fadd_rn(b, c_sym.angle[m - 1][3]);
+    return b;
+}
+
+__device__ inline int uv_map(const FrameArgs& f, int cx, int cy) {
+    return f.uvmode4[(size_t)(cy >> 2) * (f.Wc >> 2) + (cx >> 2)];
+}
+
+// Visits the chroma blocks of the SB at (x0, y0) in coding order from its decided nodes.
+template <class V>
+__device__ void uv_walk(const FrameArgs& f, const mt_rdo_node* nd, int x0, int y0, bool cfl_search, V visit) {
+    int sp = 0;
+    uint16_t stack[32];
+    stack[sp++] = 0;
+    while (sp) {
+        const int id = stack[--sp];
+        const int sp_ = nd[id].split;
+        if (sp_ == 2) continue;
+        const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
+        const int ix = kk % n, iy = kk / n;
+        const bool group = f.ss && s == 8 && sp_ == 1;  // 4:2:0: 8x8 split into 4x4 leaves
+        if (sp_ == 1 && !group) {
+            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1));
+            continue;
+        }
+        UvBlk b{};
+        b.node = (uint16_t)id;
+        b.cx = (uint16_t)((x0 + ix * s) >> f.ss);
+        b.cy = (uint16_t)((y0 + iy * s) >> f.ss);
+        b.cs = (uint8_t)(f.ss ? (s == 8 ? 4 : s >> 1) : s);
+        b.nt = (uint8_t)(b.cs == 64 ? 4 : 1);
+        b.lx = (uint8_t)(ix * s);
+        b.ly = (uint8_t)(iy * s);
+        b.ls = (uint8_t)s;
+        b.ymode = (uint8_t)id_base(nd[group ? node_id(4, 2 * ix + 1, 2 * iy + 1) : id].mode);
+        const int luma_bs = group ? 4 : s;  // the block the chroma is coded with
+        b.flags = (uint8_t)((luma_bs <= 32 ? kUvCflAllowed : 0) | (luma_bs >= 8 ? kUvAngle : 0));
+        b.nm = (uint8_t)(13 + ((cfl_search && (b.flags & kUvCflAllowed)) ? 1 : 0));
+        visit(b);
+    }
+}
+
+// Candidate of uv mode m (13 CfL), plane p (0 U, 1 V), transform block t of chroma block b;
+// alpha >= -16 with the final contexts gives CfL its alpha.
+__device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p, int t, int x0, int y0, bool fin,
+                                int alpha, uint32_t offset) {
+    mt_cand c{};
+    const int ts = b.nt == 4 ? 32 : b.cs;
+    const int tx = b.cx + (t & 1) * ts, ty = b.cy + (t >> 1) * ts;
+    c.offset = offset;
+    c.x = (uint16_t)tx;
+    c.y = (uint16_t)ty;
+    c.tx_size = (uint8_t)(ts == 4 ? MT_TX_4X4 : ts == 8 ? MT_TX_8X8 : ts == 16 ? MT_TX_16X16 : MT_TX_32X32);
+    c.tx_type = (uint8_t)(m == MT_PRED_CFL ? MT_TXTP_DCT_DCT : uv_tx_type(m, ts));
+    c.mode = (uint8_t)m;
+    if (m == MT_PRED_CFL) {
+        c.pflags = (uint8_t)(f.ss ? (MT_CAND_CFL_SS_HOR | MT_CAND_CFL_SS_VER) : 0) |
+                   (uint8_t)(fin ? MT_CAND_CFL_ALPHA_GIVEN : 0);
+        c.angle_delta = (int8_t)(fin ? alpha : 0);
+        c.txtp_mode = (uint8_t)(f.pbase + 1);
+    } else {
+        // availability: the luma block's (4:4:4 64x64: each 32x32 transform block's)
+        const int ax = b.nt == 4 ? (t & 1) * 32 : b.lx, ay = b.nt == 4 ? (t >> 1) * 32 : b.ly;
+        const int as = b.nt == 4 ? 32 : b.ls;
+        const int cy0 = y0 >> f.ss, cx0 = x0 >> f.ss;
+        const bool above_ok = ty > 0 && (fin || ty == cy0);
+        const bool left_ok = tx > 0 && (fin || tx == cx0);
+        const int am = above_ok ? uv_map(f, tx, ty - 1) : 0, lm = left_ok ? uv_map(f, tx - 1, ty) : 0;
+        const bool smooth = (am >= MT_PRED_SMOOTH && am <= MT_PRED_SMOOTH_H) ||
+                            (lm >= MT_PRED_SMOOTH && lm <= MT_PRED_SMOOTH_H);
+        c.pflags = (uint8_t)((have_tr(ax, ay, as) ? MT_PRED_HAVE_TR : 0) | (have_bl(ax, ay, as) ? MT_PRED_HAVE_BL : 0) |
+                             (smooth ? MT_PRED_SMOOTH_NB : 0));
+    }
+    const int sp = f.pbase + 2 + 2 * p, rp = sp + 1;
+    c.planes = (uint8_t)((sp & 3) | ((rp & 3) << 2));
+    c.planes_hi = (uint8_t)((sp >> 2) | ((rp >> 2) << 4));
+    c.qidx = (uint8_t)f.qidx;
+    const int bdi = f.bd == 8 ? 0 : f.bd == 10 ? 1 : 2;
+    c.bd_qm = (uint8_t)(f.uv_qm | (bdi << 4) | MT_TXBLK_CHROMA);
+    c.dc_delta = (int8_t)f.uv_dc;
+    c.ac_delta = (int8_t)f.uv_ac;
+    c.rplane = 1;
+    c.eob_bin = (uint8_t)(ts == 4 ? 0 : ts == 8 ? 5 : ts == 16 ? 11 : 15);
+    c.skip_ctx = 7;
+    c.qctx = (uint8_t)(f.qidx <= 20 ? 0 : f.qidx <= 60 ? 1 : f.qidx <= 120 ? 2 : 3);
+    c.wsse_scale = 0.0f;
+    c.lambda = f.lambda;
+    return c;
+}
+
+// One wave per SB of the step: its chroma blocks and their candidates in slots
+// [local * maxc, + maxc) of the candidates / [local * maxp, + maxp) of the pool; the slots it
+// does not use get an invalid transform size (the evaluation skips them).
+__global__ __launch_bounds__(64) void rdo_uv_gen_kernel(const FrameArgs* __restrict__ frames,
+                                                        const SbTask* __restrict__ tasks, uint32_t maxc,
+                                                        uint32_t maxp, mt_cand* __restrict__ cands,
+                                                        UvBlk* __restrict__ blks, uint32_t* __restrict__ nblks) {
+    __shared__ UvBlk s_b[kMaxUvBlocks];
+    __shared__ int s_nb;
+    __shared__ uint32_t s_total;
+    const int local = blockIdx.x, lane = threadIdx.x;
+    const SbTask t = tasks[local];
+    const FrameArgs& f = frames[t.frame];
+    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
+    if (lane == 0) {
+        int nb = 0;
+        uint32_t co = 0, po = 0;
+        if (f.chroma) {
+            uv_walk(f, f.nodes + (size_t)t.sb * kNodes, x0, y0, (f.flags & MT_RDO_CFL) != 0,
+                    [&](const UvBlk& b) { s_b[nb++] = b; });
+            uint32_t de = 0;
+            for (int i = 0; i < nb; ++i) {
+                UvBlk& b = s_b[i];
+                const int ts = b.nt == 4 ? 32 : b.cs;
+                b.cand = co;
+                b.pool = po;
+                b.desc = (uint16_t)de;
+                co += (uint32_t)b.nm * 2u * b.nt;
+                po += (uint32_t)b.nm * 2u * b.nt * (uint32_t)(ts * ts);
+                de += 2u * b.nt;
+            }
+        }
+        s_nb = nb;
+        s_total = co;
+    }
+    __syncthreads();
+    const int nb = s_nb;
+    mt_cand* out = cands + (size_t)local * maxc;
+    const uint32_t pool0 = (uint32_t)local * maxp;
+    for (int i = 0; i < nb; ++i) {
+        const UvBlk b = s_b[i];
+        const int ts = b.nt == 4 ? 32 : b.cs, per = 2 * b.nt;
+        for (int k = lane; k < b.nm * per; k += 64) {
+            const int m = k / per, p = (k / b.nt) & 1, tt = k % b.nt;
+            out[b.cand + k] = make_uv_cand(f, b, m, p, tt, x0, y0, false, 0, pool0 + b.pool + (uint32_t)k * (ts * ts));
+        }
+        if (lane == 0) blks[(size_t)local * kMaxUvBlocks + i] = b;
+    }
+    for (uint32_t k = s_total + lane; k < maxc; k += 64) out[k].tx_size = 0xff;
+    if (lane == 0) nblks[local] = (uint32_t)nb;
+}
+
+// One wave per SB: per chroma block (lanes over blocks) the cheapest uv mode, the uv map and
+// the node record; then (final contexts) the reconstruction descriptors in coding order:
+// per block U then V, each over its transform blocks.  Descriptor fields: x, y, bw = bh,
+// mode (13 CfL), angle_delta (CfL alpha), flags, bd, plane = the frame's index;
+// out_offset = plane (0 U, 1 V) | transform type << 1.
+__global__ __launch_bounds__(64) void rdo_uv_decide_kernel(const FrameArgs* __restrict__ frames,
+                                                           const SbTask* __restrict__ tasks, uint32_t maxc,
+                                                           const mt_candres* __restrict__ res,
+                                                           const UvBlk* __restrict__ blks,
+                                                           const uint32_t* __restrict__ nblks,
+                                                           mt_predblk* __restrict__ desc,
+                                                           uint32_t* __restrict__ first, uint32_t* __restrict__ count,
+                                                           uint32_t* __restrict__ sb_uv_cands,
+                                                           uint32_t* __restrict__ sb_uv_pool,
+                                                           uint32_t* __restrict__ sb_uv_dec) {
+    __shared__ uint8_t s_m[kMaxUvBlocks];
+    __shared__ int8_t s_au[kMaxUvBlocks], s_av[kMaxUvBlocks];
+    const int local = blockIdx.x, lane = threadIdx.x;
+    const SbTask t = tasks[local];
+    const FrameArgs& f = frames[t.frame];
+    const int r = t.sb / f.C, c = t.sb % f.C, x0 = c * 64, y0 = r * 64;
+    const int nb = (int)nblks[local];
+    const UvBlk* bl = blks + (size_t)local * kMaxUvBlocks;
+    const mt_candres* rs = res + (size_t)local * maxc;
+    mt_rdo_node* nd = f.nodes + (size_t)t.sb * kNodes;
+    const float inf = __int_as_float(0x7f800000);
+    uint32_t dec = 0;
+    for (int i = lane; i < nb; i += 64) {
+        const UvBlk b = bl[i];
+        const int per = 2 * b.nt;
+        float best = inf;
+        int bm = 0, ba = 0, bv = 0;
+        for (int m = 0; m < b.nm; ++m) {
+            const mt_candres* q = rs + b.cand + m * per;
+            float sum = 0.0f;
+            for (int k = 0; k < per; ++k) sum = __fadd_rn(sum, q[k].cost);
+            int au = 0, av = 0;
+            if (m == MT_PRED_CFL) {
+                au = q[0].alpha;
+                av = q[b.nt].alpha;
+                if (au == 0 && av == 0) continue;  // CfL with both alphas 0 is DC_PRED
+            }
+            if (!(sum < inf)) continue;
+            const float cost = __fadd_rn(sum, __fmul_rn(f.lambda, uv_mode_bits(b.ymode, m, au, av, b.flags)));
+            if (cost < best) {
+                best = cost;
+                bm = m;
+                ba = au;
+                bv = av;
+            }
+        }
+        dec += (uint32_t)(b.nm * per);
+        s_m[i] = (uint8_t)bm;
+        s_au[i] = (int8_t)ba;
+        s_av[i] = (int8_t)bv;
+        nd[b.node].uv_mode = (uint8_t)bm;
+        nd[b.node].cfl_alpha_u = (int8_t)ba;
+        nd[b.node].cfl_alpha_v = (int8_t)bv;
+        for (int yy = 0; yy < b.cs; yy += 4)
+            for (int xx = 0; xx < b.cs; xx += 4)
+                f.uvmode4[(size_t)((b.cy + yy) >> 2) * (f.Wc >> 2) + ((b.cx + xx) >> 2)] = (uint8_t)bm;
+    }
+    __syncthreads();  // the uv map of the whole SB is final
+    uint32_t cands = 0, pool = 0;
+    for (int i = lane; i < nb; i += 64) {
+        const UvBlk b = bl[i];
+        const int ts = b.nt == 4 ? 32 : b.cs;
+        for (int p = 0; p < 2; ++p)
+            for (int tt = 0; tt < b.nt; ++tt) {
+                const mt_cand cd = make_uv_cand(f, b, s_m[i], p, tt, x0, y0, true, p ? s_av[i] : s_au[i], 0);
+                mt_predblk d{};
+                d.out_offset = (uint32_t)p | ((uint32_t)cd.tx_type << 1);
+                d.x = cd.x;
+                d.y = cd.y;
+                d.bw = (uint8_t)ts;
+                d.bh = (uint8_t)ts;
+                d.mode = cd.mode;
+                d.angle_delta = cd.angle_delta;
+                d.flags = cd.mode == MT_PRED_CFL ? (uint8_t)0 : cd.pflags;
+                d.bd = (uint8_t)f.bd;
+                d.plane = (uint8_t)f.frame;
+                desc[(size_t)local * kMaxUvDesc + b.desc + p * b.nt + tt] = d;
+            }
+        cands += 2u * b.nt;
+        pool += 2u * b.nt * (uint32_t)(ts * ts);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        cands += __shfl_xor(cands, o);
+        pool += __shfl_xor(pool, o);
+        dec += __shfl_xor(dec, o);
+    }
+    if (lane == 0) {
+        uint32_t nd_ = 0;
+        if (nb) nd_ = (uint32_t)bl[nb - 1].desc + 2u * bl[nb - 1].nt;
+        first[local] = (uint32_t)local * kMaxUvDesc;
+        count[local] = nd_;
+        sb_uv_cands[f.sb0 + t.sb] = cands;
+        sb_uv_pool[f.sb0 + t.sb] = pool;
+        sb_uv_dec[f.sb0 + t.sb] = dec;
+    }
+}
+
 // The SB's leaves with the final contexts, at leaf_base[g] / pool_base[g] (one thread per SB
 // of the batch's SB list).
 __global__ void __launch_bounds__(64) rdo_final_gen_kernel(const FrameArgs* __restrict__ frames,
@@ -528,6 +838,19 @@ __global__ void __launch_bounds__(64) rdo_final_gen_kernel(const FrameArgs* __re
         cands[k++] = make_cand(f, x, y, s, nd[id].mode, x0, y0, true, p);
         p += (uint32_t)(s * s);
     }
+    if (!f.chroma) return;
+    // the chroma blocks: U then V over the transform blocks, the uv mode bits on the first
+    uv_walk(f, nd, x0, y0, (f.flags & MT_RDO_CFL) != 0, [&](const UvBlk& b) {
+        const int ts = b.nt == 4 ? 32 : b.cs;
+        const mt_rdo_node& o = nd[b.node];
+        for (int pl = 0; pl < 2; ++pl)
+            for (int tt = 0; tt < b.nt; ++tt) {
+                mt_cand cd = make_uv_cand(f, b, o.uv_mode, pl, tt, x0, y0, true, pl ? o.cfl_alpha_v : o.cfl_alpha_u, p);
+                if (pl == 0 && tt == 0) cd.extra_bits = uv_mode_bits(b.ymode, o.uv_mode, o.cfl_alpha_u, o.cfl_alpha_v, b.flags);
+                cands[k++] = cd;
+                p += (uint32_t)(ts * ts);
+            }
+    });
 }
 
 // Per SB, in coding order: partition symbol bits of every coded node inside the frame and
@@ -537,7 +860,9 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
                                                           const uint32_t* __restrict__ leaf_base,
                                                           const mt_candres* __restrict__ res,
                                                           double* __restrict__ sb_bits,
-                                                          long long* __restrict__ sb_sse) {
+                                                          long long* __restrict__ sb_sse,
+                                                          double* __restrict__ sb_bits_uv,
+                                                          long long* __restrict__ sb_sse_uv) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nall) return;
     const SbTask t = all[g];
@@ -567,13 +892,39 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
         e += res[k].sse;
         ++k;
     }
+    double buv = 0.0;
+    long long eu = 0, ev = 0;
+    if (f.chroma) {
+        uv_walk(f, nd, x0, y0, (f.flags & MT_RDO_CFL) != 0, [&](const UvBlk& blk) {
+            for (int pl = 0; pl < 2; ++pl)
+                for (int tt = 0; tt < blk.nt; ++tt) {
+                    b = __dadd_rn(b, (double)res[k].bits);
+                    buv = __dadd_rn(buv, (double)res[k].bits);
+                    (pl ? ev : eu) += res[k].sse;
+                    ++k;
+                }
+        });
+    }
     sb_bits[g] = b;
     sb_sse[g] = e;
+    sb_bits_uv[g] = buv;
+    sb_sse_uv[2 * g] = eu;
+    sb_sse_uv[2 * g + 1] = ev;
 }
 
 // Library-owned buffers of one device, grown on demand.
 struct RdoWork {
     size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map = 0, cap_frames = 0;
+    size_t cap_uv = 0;
+    uint32_t *sb_uv_cands = nullptr, *sb_uv_pool = nullptr, *sb_uv_dec = nullptr;
+    double* sb_bits_uv = nullptr;
+    long long* sb_sse_uv = nullptr;
+    UvBlk* uvblk = nullptr;
+    uint32_t* nuvblk = nullptr;
+    mt_predblk* uvdesc = nullptr;
+    uint32_t *uvfirst = nullptr, *uvcount = nullptr;
+    mt_cand* ucands = nullptr;
+    mt_candres* ures = nullptr;
     uint32_t *counts = nullptr, *pools = nullptr, *sb_leaves = nullptr, *sb_leaf_pool = nullptr;
     uint32_t *leaf_base = nullptr, *pool_base = nullptr;
     double* sb_bits = nullptr;
@@ -603,8 +954,10 @@ template <typename T> int regrow(T*& p, size_t n) {
     return MT_OK;
 }
 
-// Frames of one batch: the eval's plane set holds two planes per frame.
-constexpr int kBatchFrames = MT_MAX_PLANES / 2;
+// Frames of one batch: at most 16, and the eval's plane set holds their planes (2 per frame,
+// 6 with chroma).
+constexpr int kBatchFrames = 16;
+int frame_planes(const mt_rdo_frame& fr) { return fr.chroma ? 6 : 2; }
 
 int check_frame(const mt_rdo_frame& fr) {
     const int W = fr.src.width, H = fr.src.height;
@@ -614,6 +967,16 @@ int check_frame(const mt_rdo_frame& fr) {
     if ((fr.bd != 8 && fr.bd != 10 && fr.bd != 12) || fr.qm_level > 15 || !(fr.lambda > 0.0f) ||
         !std::isfinite(fr.lambda) || !(fr.wsse_scale >= 0.0f) || !std::isfinite(fr.wsse_scale))
         return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad parameters");
+    if (fr.chroma > MT_RDO_CHROMA_444) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad chroma format");
+    if (fr.chroma) {
+        const int ss = fr.chroma == MT_RDO_CHROMA_420 ? 1 : 0, Wc = W >> ss, Hc = H >> ss;
+        for (const mt_plane* c : {&fr.src_u, &fr.src_v})
+            if (!c->data || c->width != Wc || c->height != Hc || c->stride < Wc)
+                return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: chroma sources must be the chroma size");
+        if (!fr.recon_u || !fr.recon_v || fr.recon_uv_stride < Wc || fr.uv_qm_level > 15 || fr.uv_dc_delta < -64 ||
+            fr.uv_dc_delta > 63 || fr.uv_ac_delta < -64 || fr.uv_ac_delta > 63)
+            return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad chroma parameters");
+    }
     return MT_OK;
 }
 
@@ -626,7 +989,9 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     std::vector<int> C(nf), Rr(nf), nsteps(nf);
     size_t map_bytes = 0;
     uint32_t nall = 0;
-    int max_steps = 0;
+    int max_steps = 0, nplanes = 0;
+    bool any_chroma = false;
+    uint32_t maxc = 0, maxp = 0;  // chroma candidate / pool slots per SB of a step
     for (int i = 0; i < nf; ++i) {
         const mt_rdo_frame& fr = frames[i];
         const int W = fr.src.width, H = fr.src.height;
@@ -652,17 +1017,55 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
         f.lambda = fr.lambda;
         f.wsse = fr.wsse_scale;
         f.frame = i;
+        f.pbase = nplanes;
+        nplanes += frame_planes(fr);
         f.sb0 = nall;
         nall += (uint32_t)(C[i] * Rr[i]);
         map_bytes += (size_t)(W / 4) * (H / 4) + (size_t)(W / 8) * (H / 8);
         const int bdi = fr.bd == 8 ? 0 : fr.bd == 10 ? 1 : 2;
-        rf[i] = mtlib::ReconFrame{f.src, fr.recon, fr.recon_stride, fr.qidx, fr.qm_level | (bdi << 4), 0};
+        rf[i] = mtlib::ReconFrame{};
+        rf[i].src = f.src;
+        rf[i].rec = fr.recon;
+        rf[i].rstride = fr.recon_stride;
+        rf[i].qidx = fr.qidx;
+        rf[i].bd_qm = fr.qm_level | (bdi << 4);
+        f.chroma = fr.chroma;
+        if (fr.chroma) {
+            any_chroma = true;
+            f.ss = fr.chroma == MT_RDO_CHROMA_420 ? 1 : 0;
+            f.Wc = W >> f.ss;
+            f.Hc = H >> f.ss;
+            f.uv_dc = fr.uv_dc_delta;
+            f.uv_ac = fr.uv_ac_delta;
+            f.uv_qm = fr.uv_qm_level;
+            map_bytes += (size_t)(f.Wc / 4) * (f.Hc / 4);
+            rf[i].src_uv[0] = fr.src_u;
+            rf[i].src_uv[1] = fr.src_v;
+            for (int k = 0; k < 2; ++k) {
+                rf[i].src_uv[k].vis_w = f.Wc;
+                rf[i].src_uv[k].vis_h = f.Hc;
+            }
+            rf[i].rec_uv[0] = fr.recon_u;
+            rf[i].rec_uv[1] = fr.recon_v;
+            rf[i].uv_stride = fr.recon_uv_stride;
+            rf[i].uv_bd_qm = fr.uv_qm_level | (bdi << 4) | MT_TXBLK_CHROMA;
+            rf[i].uv_dc = fr.uv_dc_delta;
+            rf[i].uv_ac = fr.uv_ac_delta;
+            rf[i].ss = f.ss;
+            // 4:2:0: <= 64 chroma blocks of <= 14 modes x 2 planes; 4:4:4: <= 256 (4x4 leaves)
+            const uint32_t c1 = (f.ss || f.minsz == 8) ? 64u * 28u : 256u * 28u;
+            const uint32_t p1 = f.ss ? 28u * 1024u : 28u * 4096u;
+            maxc = std::max(maxc, c1);
+            maxp = std::max(maxp, p1);
+        }
     }
     if (w.cap_sb < nall) {
         if ((rc = regrow(w.counts, nall)) || (rc = regrow(w.pools, nall)) || (rc = regrow(w.sb_leaves, nall)) ||
             (rc = regrow(w.sb_leaf_pool, nall)) || (rc = regrow(w.leaf_base, nall)) ||
             (rc = regrow(w.pool_base, nall)) || (rc = regrow(w.sb_bits, nall)) || (rc = regrow(w.sb_sse, nall)) ||
-            (rc = regrow(w.all, nall)) || (rc = regrow(w.tasks, nall)))
+            (rc = regrow(w.all, nall)) || (rc = regrow(w.tasks, nall)) || (rc = regrow(w.sb_uv_cands, nall)) ||
+            (rc = regrow(w.sb_uv_pool, nall)) || (rc = regrow(w.sb_uv_dec, nall)) ||
+            (rc = regrow(w.sb_bits_uv, nall)) || (rc = regrow(w.sb_sse_uv, 2 * (size_t)nall)))
             return rc;
         w.cap_sb = nall;
     }
@@ -681,6 +1084,10 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             off += (size_t)(fa[i].W / 4) * (fa[i].H / 4);
             fa[i].part8 = w.maps + off;
             off += (size_t)(fa[i].W / 8) * (fa[i].H / 8);
+            if (fa[i].chroma) {
+                fa[i].uvmode4 = w.maps + off;
+                off += (size_t)(fa[i].Wc / 4) * (fa[i].Hc / 4);
+            }
         }
     }
     std::vector<SbTask> all(nall);
@@ -690,10 +1097,18 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     MT_HIP_TRY(hipMemcpyAsync(w.rframes, rf.data(), nf * sizeof(mtlib::ReconFrame), hipMemcpyHostToDevice, s));
     MT_HIP_TRY(hipMemcpyAsync(w.all, all.data(), nall * sizeof(SbTask), hipMemcpyHostToDevice, s));
     PlaneSet ps{};
-    ps.n = 2 * nf;
+    ps.n = nplanes;
     for (int i = 0; i < nf; ++i) {
-        ps.p[2 * i] = fa[i].src;
-        ps.p[2 * i + 1] = mt_plane{frames[i].recon, frames[i].recon_stride, fa[i].W, fa[i].H, fa[i].W, fa[i].H, 0};
+        const int b = fa[i].pbase;
+        ps.p[b] = fa[i].src;
+        ps.p[b + 1] = mt_plane{frames[i].recon, frames[i].recon_stride, fa[i].W, fa[i].H, fa[i].W, fa[i].H, 0};
+        if (fa[i].chroma) {
+            const int Wc = fa[i].Wc, Hc = fa[i].Hc, us = frames[i].recon_uv_stride;
+            ps.p[b + 2] = rf[i].src_uv[0];
+            ps.p[b + 3] = mt_plane{frames[i].recon_u, us, Wc, Hc, Wc, Hc, 0};
+            ps.p[b + 4] = rf[i].src_uv[1];
+            ps.p[b + 5] = mt_plane{frames[i].recon_v, us, Wc, Hc, Wc, Hc, 0};
+        }
     }
 
     // each reconstruction starts as its source (the decision's stand-in inside each SB)
@@ -703,7 +1118,22 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             const size_t npx = (size_t)fa[i].W * fa[i].H;
             hipLaunchKernelGGL(rdo_copy_plane_kernel, dim3(std::max(1, std::min(cus * 4, (int)((npx + 255) / 256)))),
                                dim3(256), 0, s, fa[i].src, frames[i].recon, frames[i].recon_stride);
+            if (fa[i].chroma) {
+                const size_t cpx = (size_t)fa[i].Wc * fa[i].Hc;
+                const dim3 g(std::max(1, std::min(cus * 4, (int)((cpx + 255) / 256))));
+                hipLaunchKernelGGL(rdo_copy_plane_kernel, g, dim3(256), 0, s, rf[i].src_uv[0], frames[i].recon_u,
+                                   frames[i].recon_uv_stride);
+                hipLaunchKernelGGL(rdo_copy_plane_kernel, g, dim3(256), 0, s, rf[i].src_uv[1], frames[i].recon_v,
+                                   frames[i].recon_uv_stride);
+            }
         }
+        // no chroma stage runs for luma-only SBs: their chroma counts stay 0
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(std::max(1, std::min(cus * 4, (int)((nall * 4 + 255) / 256)))),
+                           dim3(256), 0, s, reinterpret_cast<uint8_t*>(w.sb_uv_cands), (size_t)nall * 4);
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(std::max(1, std::min(cus * 4, (int)((nall * 4 + 255) / 256)))),
+                           dim3(256), 0, s, reinterpret_cast<uint8_t*>(w.sb_uv_pool), (size_t)nall * 4);
+        hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(std::max(1, std::min(cus * 4, (int)((nall * 4 + 255) / 256)))),
+                           dim3(256), 0, s, reinterpret_cast<uint8_t*>(w.sb_uv_dec), (size_t)nall * 4);
         hipLaunchKernelGGL(rdo_fill_u8_kernel, dim3(std::max(1, std::min(cus * 4, (int)((map_bytes + 255) / 256)))),
                            dim3(256), 0, s, w.maps, map_bytes);
         hipLaunchKernelGGL(rdo_count_kernel, dim3(nall), dim3(64), 0, s, w.fargs, w.all, w.counts, w.pools);
@@ -746,9 +1176,18 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     if (w.cap_step_sb < max_sb) {
         if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.plan, max_sb * kNodes)) ||
             (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
-            (rc = regrow(w.count, max_sb)))
+            (rc = regrow(w.count, max_sb)) || (rc = regrow(w.uvblk, max_sb * kMaxUvBlocks)) ||
+            (rc = regrow(w.nuvblk, max_sb)) || (rc = regrow(w.uvdesc, max_sb * kMaxUvDesc)) ||
+            (rc = regrow(w.uvfirst, max_sb)) || (rc = regrow(w.uvcount, max_sb)))
             return rc;
         w.cap_step_sb = max_sb;
+    }
+    if (any_chroma) {
+        if ((uint64_t)max_sb * maxp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
+        if (w.cap_uv < max_sb * maxc) {
+            if ((rc = regrow(w.ucands, max_sb * maxc)) || (rc = regrow(w.ures, max_sb * maxc))) return rc;
+            w.cap_uv = max_sb * maxc;
+        }
     }
     if (w.cap_cand < max_cand) {
         if ((rc = regrow(w.cands, max_cand)) || (rc = regrow(w.refs, max_cand)) || (rc = regrow(w.cres, max_cand)))
@@ -784,19 +1223,40 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             mtlib::ProfScope prof(mtlib::kProfRecon, s);
             if ((rc = mtlib::sb_recon_launch(w.leaves, w.first, w.count, nk, w.rframes, s))) return rc;
         }
+        if (!any_chroma) continue;
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            hipLaunchKernelGGL(rdo_uv_gen_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, maxc, maxp, w.ucands,
+                               w.uvblk, w.nuvblk);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        if ((rc = mtlib::eval_launch(w.ucands, (int)(nk * maxc), ps, (uint64_t)nk * maxp, w.ures, s))) return rc;
+        {
+            mtlib::ProfScope prof(mtlib::kProfRdo, s);
+            hipLaunchKernelGGL(rdo_uv_decide_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, maxc, w.ures, w.uvblk,
+                               w.nuvblk, w.uvdesc, w.uvfirst, w.uvcount, w.sb_uv_cands, w.sb_uv_pool, w.sb_uv_dec);
+            MT_HIP_TRY(hipGetLastError());
+        }
+        {
+            mtlib::ProfScope prof(mtlib::kProfRecon, s);
+            if ((rc = mtlib::sb_recon_uv_launch(w.uvdesc, w.uvfirst, w.uvcount, nk, w.rframes, s))) return rc;
+        }
     }
 
     // final accounting with the final contexts
-    std::vector<uint32_t> nleaf(nall), lpool(nall), lbase(nall), pbase(nall);
+    std::vector<uint32_t> nleaf(nall), lpool(nall), lbase(nall), pbase(nall), uvc(nall), uvp(nall), uvd(nall);
     MT_HIP_TRY(hipMemcpyAsync(nleaf.data(), w.sb_leaves, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     MT_HIP_TRY(hipMemcpyAsync(lpool.data(), w.sb_leaf_pool, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(uvc.data(), w.sb_uv_cands, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(uvp.data(), w.sb_uv_pool, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(uvd.data(), w.sb_uv_dec, nall * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     MT_HIP_TRY(hipStreamSynchronize(s));
     uint64_t tl = 0, tp = 0;
     for (uint32_t i = 0; i < nall; ++i) {
         lbase[i] = (uint32_t)tl;
         pbase[i] = (uint32_t)tp;
-        tl += nleaf[i];
-        tp += lpool[i];
+        tl += nleaf[i] + uvc[i];
+        tp += lpool[i] + uvp[i];
     }
     if (tp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch too large");
     if (w.cap_final < tl) {
@@ -815,24 +1275,32 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     {
         mtlib::ProfScope prof(mtlib::kProfRdo, s);
         hipLaunchKernelGGL(rdo_final_sum_kernel, dim3((nall + 63) / 64), dim3(64), 0, s, w.fargs, w.all, (int)nall,
-                           w.leaf_base, w.fres, w.sb_bits, w.sb_sse);
+                           w.leaf_base, w.fres, w.sb_bits, w.sb_sse, w.sb_bits_uv, w.sb_sse_uv);
         MT_HIP_TRY(hipGetLastError());
     }
-    std::vector<double> bits(nall);
-    std::vector<long long> sse(nall);
+    std::vector<double> bits(nall), bits_uv(nall);
+    std::vector<long long> sse(nall), sse_uv(2 * (size_t)nall);
     MT_HIP_TRY(hipMemcpyAsync(bits.data(), w.sb_bits, nall * sizeof(double), hipMemcpyDeviceToHost, s));
     MT_HIP_TRY(hipMemcpyAsync(sse.data(), w.sb_sse, nall * sizeof(long long), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(bits_uv.data(), w.sb_bits_uv, nall * sizeof(double), hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(sse_uv.data(), w.sb_sse_uv, 2 * nall * sizeof(long long), hipMemcpyDeviceToHost, s));
     MT_HIP_TRY(hipStreamSynchronize(s));
     for (int i = 0; i < nf; ++i) {
         mt_rdo_stats& st = stats[i];
         std::memset(&st, 0, sizeof(st));
         const uint32_t n = (uint32_t)(C[i] * Rr[i]);
+        int64_t uv_dec = 0;
         for (uint32_t sb = 0; sb < n; ++sb) {
-            st.bits += bits[fa[i].sb0 + sb];
-            st.sse += sse[fa[i].sb0 + sb];
-            st.leaves += (int32_t)nleaf[fa[i].sb0 + sb];
+            const uint32_t g = fa[i].sb0 + sb;
+            st.bits += bits[g];
+            st.sse += sse[g];
+            st.leaves += (int32_t)nleaf[g];
+            st.bits_uv += bits_uv[g];
+            st.sse_u += sse_uv[2 * (size_t)g];
+            st.sse_v += sse_uv[2 * (size_t)g + 1];
+            uv_dec += uvd[g];
         }
-        st.candidates = frame_cands[i];
+        st.candidates = frame_cands[i] + uv_dec;
         st.sbs = (int32_t)n;
         st.steps = nsteps[i];
     }
@@ -861,6 +1329,15 @@ int rdo_init_device(int /*device*/) {
         b.part[3][ctx][0] = cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 0);
         b.part[3][ctx][1] = cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 3);
     }
+    for (int y = 0; y < 13; ++y) {
+        for (int m = 0; m < 14; ++m) {
+            b.uv[0][y][m] = m < 13 ? cdf_cost_raw(MT_UV_MODE_NOCFL_CDF[y], 13, m) : 0.0f;
+            b.uv[1][y][m] = cdf_cost_raw(MT_UV_MODE_CFL_CDF[y], 14, m);
+        }
+    }
+    for (int k = 0; k < 8; ++k) b.cfl_sign[k] = cdf_cost_raw(MT_CFL_SIGN_CDF, 8, k);
+    for (int c = 0; c < 6; ++c)
+        for (int k = 0; k < 16; ++k) b.cfl_alpha[c][k] = cdf_cost_raw(MT_CFL_ALPHA_CDF[c], 16, k);
     MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_sym), &b, sizeof(b)));
     return MT_OK;
 }
@@ -900,7 +1377,7 @@ extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
 extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
                                   void* stream) {
     if (!frames || !stats || nframes < 0) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
-    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4))
+    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL))
         return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad flags");
     int rc = MT_OK;
     for (int i = 0; i < nframes; ++i)
@@ -911,9 +1388,12 @@ extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int f
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_rdo_mu);
     RdoWork& w = g_rdo[dev];
-    for (int i0 = 0; i0 < nframes; i0 += kBatchFrames) {
-        const int nf = std::min(kBatchFrames, nframes - i0);
+    for (int i0 = 0; i0 < nframes;) {
+        int nf = 0, planes = 0;
+        while (i0 + nf < nframes && nf < kBatchFrames && planes + frame_planes(frames[i0 + nf]) <= MT_MAX_PLANES)
+            planes += frame_planes(frames[i0 + nf++]);
         if ((rc = run_batch(w, frames + i0, nf, flags, stats + i0, cus, (hipStream_t)stream))) return rc;
+        i0 += nf;
     }
     return MT_OK;
 }

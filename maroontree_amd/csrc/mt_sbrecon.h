@@ -14,20 +14,24 @@
 
 namespace {
 
-// Square DCT_DCT chain of one N x N block (N <= 32) by one wave: lane i runs line i of
-// each pass.  res: the residual (LDS, row-major); out: the reconstructed residual (may
-// alias res); work: LDS of N * (N + 1) ints.
+// Square chain of one N x N block (N <= 32) by one wave: lane i runs line i of each pass.
+// type: DCT_DCT, or at N <= 16 ADST_DCT / DCT_ADST / ADST_ADST (vertical kind first, the
+// 2-D types the chain kernels run with the same per-size rules).  res: the residual (LDS,
+// row-major); out: the reconstructed residual (may alias res); work: LDS of N * (N + 1) ints.
 template <int N>
-__device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p) {
+__device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p,
+                              int type = MT_TXTP_DCT_DCT) {
     using R = TxRules<N, N>;
     constexpr int RS = N + 1;
     const int lane = threadIdx.x & 63;
+    const bool col_adst = N <= 16 && (type == MT_TXTP_ADST_ADST || type == MT_TXTP_ADST_DCT);
+    const bool row_adst = N <= 16 && (type == MT_TXTP_ADST_ADST || type == MT_TXTP_DCT_ADST);
     // forward column pass (length N)
     if (lane < N) {
         int32_t v[N];
 #pragma unroll
         for (int y = 0; y < N; y++) v[y] = (int32_t)((uint32_t)res[y * N + lane] << R::pre_shift);
-        fwd_line<N>(false, v);
+        fwd_line<N>(col_adst, v);
 #pragma unroll
         for (int y = 0; y < N; y++) work[y * RS + lane] = v[y];
     }
@@ -38,7 +42,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
         int32_t v[N];
 #pragma unroll
         for (int x = 0; x < N; x++) v[x] = work[fy * RS + x];
-        fwd_line<N>(false, v);
+        fwd_line<N>(row_adst, v);
 #pragma unroll
         for (int fx = 0; fx < N; fx++) {
             int32_t c = v[fx];
@@ -65,7 +69,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
         int32_t v[N];
 #pragma unroll
         for (int x = 0; x < N; x++) v[x] = work[y * RS + x];
-        inv_line<N>(KD, v, Clip{p.rmin, p.rmax});
+        inv_line<N>(row_adst ? KA : KD, v, Clip{p.rmin, p.rmax});
         const Clip cc{p.cmin, p.cmax};
 #pragma unroll
         for (int x = 0; x < N; x++) {
@@ -81,7 +85,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
         int32_t v[N];
 #pragma unroll
         for (int y = 0; y < N; y++) v[y] = work[y * RS + x];
-        inv_line<N>(KD, v, Clip{p.cmin, p.cmax});
+        inv_line<N>(col_adst ? KA : KD, v, Clip{p.cmin, p.cmax});
 #pragma unroll
         for (int y = 0; y < N; y++) out[y * N + x] = (v[y] + 8) >> 4;
     }
@@ -224,9 +228,83 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
     }
 }
 
+// Chroma of the RDO: one wave per superblock walks its chroma transform blocks
+// desc[first[b] .. + count[b]) in coding order.  Per block: the prediction from the chroma
+// reconstruction (CfL, mode 13: the DC prediction plus alpha (angle_delta) times the AC of
+// the frame's luma reconstruction), the residual, the chain of the transform type in
+// out_offset >> 1 with the chroma quantizer, and the clamped reconstruction; out_offset & 1
+// selects V over U; `plane` is the frame's index in `frames`.
+__global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __restrict__ desc,
+                                                         const uint32_t* __restrict__ first,
+                                                         const uint32_t* __restrict__ count,
+                                                         const mtlib::ReconFrame* __restrict__ frames) {
+    __shared__ int32_t s_pred[1024];
+    __shared__ int32_t s_res[1024];
+    __shared__ int32_t s_a[kEdge], s_l[kEdge];
+    __shared__ int32_t s_fi[33 * 33];
+    __shared__ int32_t s_work[32 * 33];
+    const int lane = threadIdx.x;
+    const uint32_t f0 = first[blockIdx.x], n = count[blockIdx.x];
+    if (n == 0) return;
+    const mtlib::ReconFrame fr = frames[desc[f0].plane];
+    mt_txblk q{};
+    q.qidx = (uint8_t)fr.qidx;
+    q.bd_qm = (uint8_t)fr.uv_bd_qm;
+    const int8_t qd[2] = {(int8_t)fr.uv_dc, (int8_t)fr.uv_ac};
+    BlkParam prm;
+    param_from_desc(q, prm, qd, 0);
+    const int maxv = (1 << prm.bd) - 1;
+    const int ustride = fr.uv_stride;
+    const mt_plane luma{fr.rec, fr.rstride, fr.src.width, fr.src.height, fr.src.width, fr.src.height, 0};
+    for (uint32_t i = 0; i < n; i++) {
+        mt_predblk d = desc[f0 + i];
+        const int pl = (int)(d.out_offset & 1), type = (int)(d.out_offset >> 1);
+        const mt_plane src = fr.src_uv[pl];
+        uint16_t* __restrict__ rec = fr.rec_uv[pl];
+        const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
+        const int x0 = d.x, y0 = d.y;
+        const bool cfl = d.mode == MT_PRED_CFL;
+        const int alpha = d.angle_delta;
+        if (cfl) {
+            d.mode = MT_PRED_DC;
+            d.angle_delta = 0;
+        }
+        wave_predict(d, src.width, src.height,
+                     [=](int x, int y) { return (int)rec[(size_t)y * ustride + x]; }, s_pred, s_a, s_l, s_fi);
+        __syncthreads();
+        if (cfl) {
+            const int dc = s_pred[0];
+            __syncthreads();
+            cfl_block(luma, x0 << fr.ss, y0 << fr.ss, fr.ss != 0, fr.ss != 0, src, x0, y0, s, s, prm.bd, dc, true,
+                      alpha, s_work, s_pred, nullptr, nullptr);
+        }
+        for (int p = lane; p < npx; p += 64)
+            s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
+        __syncthreads();
+        switch (s) {
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, type); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, type); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, type); break;
+            default: wave_chain_sq<32>(s_res, s_res, s_work, prm, type); break;
+        }
+        for (int p = lane; p < npx; p += 64) {
+            const int v = s_pred[p] + s_res[p];
+            rec[(size_t)(y0 + (p >> lw)) * ustride + x0 + (p & (s - 1))] = (uint16_t)(v < 0 ? 0 : v > maxv ? maxv : v);
+        }
+        __syncthreads();  // the next block predicts from these samples
+    }
+}
+
 }  // namespace
 
 namespace mtlib {
+int sb_recon_uv_launch(const mt_predblk* desc_d, const uint32_t* first_d, const uint32_t* count_d, int nsb,
+                       const ReconFrame* frames_d, hipStream_t s) {
+    if (nsb <= 0) return MT_OK;
+    hipLaunchKernelGGL(sb_recon_uv_kernel, dim3(nsb), dim3(64), 0, s, desc_d, first_d, count_d, frames_d);
+    MT_HIP_TRY(hipGetLastError());
+    return MT_OK;
+}
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
                     int nsb, const ReconFrame* frames_d, hipStream_t s) {
     if (nsb <= 0) return MT_OK;

@@ -1206,6 +1206,45 @@ extern "C" int mt_qm_level_law(int base_q_idx, int sub) {
     return (int)(uint8_t)fast_round(v);
 }
 
+// Frame chroma quantizer laws with the shipped tuning (src/tuning.rs:750-791: top_ease 1,
+// qm_hq_knee 12, qm_hq_width 5, top_444_uac 6, uv420_tail_delta 28, uv420_mid_d 28).
+static int uv420_delta_law(int qi) {
+    // uv420_mid_delta (src/quant.rs:170) with tail = d = 28
+    const int d = 28, tail = 28;
+    if (qi <= 20) return 0;
+    if (qi < 64) return -(d * (qi - 20)) / 44;
+    if (qi <= 96) return -d;
+    if (qi < 136) return -tail - ((d - tail) * (136 - qi)) / 40;
+    return -tail;
+}
+
+extern "C" int mt_chroma_q_deltas(int base_q_idx, int sub, int* dc_delta, int* ac_delta) {
+    if (base_q_idx < 0 || base_q_idx > 255 || (sub != 0 && sub != 2) || !dc_delta || !ac_delta)
+        return mtlib::set_error(MT_ERR_ARG, "mt_chroma_q_deltas: qidx 0..255, sub 0 (4:4:4) or 2 (4:2:0)");
+    const int q = base_q_idx;
+    // chroma_ac_delta (src/quant.rs:204)
+    *ac_delta = sub == 2 ? uv420_delta_law(q) : q <= 20 ? 6 : q >= 96 ? 24 : 24 * (q - 20) / 76;
+    // chroma_dc_delta (src/quant.rs:223)
+    const int deep = q <= 160 ? 0 : -std::min((q - 160) / 3, 32);
+    *dc_delta = sub == 2 ? deep + uv420_delta_law(q) : deep;
+    return MT_OK;
+}
+
+extern "C" int mt_qm_chroma_level_law(int base_q_idx, int sub) {
+    // qm_chroma_level_law (src/quant.rs:155)
+    if (sub != 0) return mt_qm_level_law(base_q_idx, sub);
+    const int pinned = std::min(2, mt_qm_level_law(base_q_idx, 0));
+    volatile float t = (12.0f - (float)base_q_idx) / 5.0f;  // qm_high_quality_t
+    float tc = t < 0.0f ? 0.0f : t > 1.0f ? 1.0f : (float)t;
+    tc = 1.0f * tc;  // top_ease
+    if (tc > 0.0f) {
+        volatile float prod = (15.0f - (float)pinned) * tc;
+        volatile float v = (float)pinned + prod;
+        return (int)(uint8_t)std::round((float)v);
+    }
+    return pinned;
+}
+
 extern "C" int mt_tx_width(int s) { return (s >= 0 && s < 19) ? kTxW[s] : 0; }
 extern "C" int mt_tx_height(int s) { return (s >= 0 && s < 19) ? kTxH[s] : 0; }
 extern "C" int mt_tx_ncoef(int s) {

@@ -13,25 +13,42 @@ import numpy as np
 from . import _lib
 from ._lib import MtPlane, _check, _dev_ptr, _stream_ptr, init, lib
 
-ANGLE_DELTAS, FILTER_INTRA, MIN4 = 1, 2, 4
+ANGLE_DELTAS, FILTER_INTRA, MIN4, CFL = 1, 2, 4, 8
+LUMA_ONLY, CHROMA_420, CHROMA_444 = 0, 1, 2
 NODES = 341
 ALL_MODES, FORCE_SPLIT, NO_SPLIT = 0x3FFF, 0x4000, 0x8000
 LEVEL_BASE = [0, 1, 5, 21, 85]
 NODE_DTYPE = np.dtype([("cost", "<f4"), ("none_cost", "<f4"), ("split", "u1"), ("mode", "u1"),
-                       ("pad", "u1", (2,))])
+                       ("uv_mode", "u1"), ("pad0", "u1"), ("cfl_alpha_u", "i1"), ("cfl_alpha_v", "i1"),
+                       ("pad1", "u1", (2,))])
 
 
 class MtRdoFrame(ctypes.Structure):
     _fields_ = [("src", MtPlane), ("recon", ctypes.c_void_p), ("recon_stride", ctypes.c_int32),
                 ("bd", ctypes.c_uint8), ("qidx", ctypes.c_uint8), ("qm_level", ctypes.c_uint8),
-                ("pad0", ctypes.c_uint8), ("lambda_", ctypes.c_float), ("wsse_scale", ctypes.c_float),
-                ("prune", ctypes.c_void_p), ("nodes", ctypes.c_void_p)]
+                ("chroma", ctypes.c_uint8), ("lambda_", ctypes.c_float), ("wsse_scale", ctypes.c_float),
+                ("prune", ctypes.c_void_p), ("nodes", ctypes.c_void_p),
+                ("src_u", MtPlane), ("src_v", MtPlane), ("recon_u", ctypes.c_void_p),
+                ("recon_v", ctypes.c_void_p), ("recon_uv_stride", ctypes.c_int32),
+                ("uv_dc_delta", ctypes.c_int8), ("uv_ac_delta", ctypes.c_int8),
+                ("uv_qm_level", ctypes.c_uint8), ("pad1", ctypes.c_uint8)]
 
 
 class MtRdoStats(ctypes.Structure):
     _fields_ = [("bits", ctypes.c_double), ("sse", ctypes.c_int64), ("candidates", ctypes.c_int64),
                 ("leaves", ctypes.c_int32), ("sbs", ctypes.c_int32), ("steps", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("bits_uv", ctypes.c_double), ("sse_u", ctypes.c_int64),
+                ("sse_v", ctypes.c_int64)]
+
+
+def _sigs():
+    L = lib()
+    if not getattr(L, "_rdo_chroma_sigs", False):
+        L.mt_chroma_q_deltas.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]
+        L.mt_qm_chroma_level_law.argtypes = [ctypes.c_int, ctypes.c_int]
+        L._rdo_chroma_sigs = True
+    return L
 
 
 def rdo_lambda(qidx, bd, sub=2):
@@ -43,6 +60,15 @@ def mode_lambda(qidx, bd, sub=2):
     return float(lib().mt_mode_lambda(qidx, bd, sub))
 
 
+def chroma_laws(qidx, sub):
+    """The reference's frame chroma quantizer (dc delta, ac delta, QM level) for sub 2 (4:2:0)
+    or 0 (4:4:4): chroma_dc_delta, chroma_ac_delta, qm_chroma_level_law."""
+    L = _sigs()
+    dc, ac = ctypes.c_int(), ctypes.c_int()
+    _check(L.mt_chroma_q_deltas(qidx, sub, ctypes.byref(dc), ctypes.byref(ac)))
+    return dc.value, ac.value, int(L.mt_qm_chroma_level_law(qidx, sub))
+
+
 def node_geometry(node):
     """(level, ix, iy, size) of quadtree node `node` of a superblock."""
     L = 4 if node >= 85 else 3 if node >= 21 else 2 if node >= 5 else 1 if node >= 1 else 0
@@ -51,53 +77,37 @@ def node_geometry(node):
     return L, k % n, k // n, 64 >> L
 
 
+def _plane(t, w, h):
+    return MtPlane(_dev_ptr(t).value, t.stride(0), w, h, w, h, 0)
+
+
 def intra_rdo(src, bd, qidx, qm_level=15, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, lam=None,
-              wsse_scale=1.0, prune=None, recon=None, nodes=None, stream=None):
+              wsse_scale=1.0, prune=None, recon=None, nodes=None, stream=None, **chroma):
     """RDO of one frame.  src: uint16 cuda tensor (H, W), W and H multiples of 8; prune:
-    optional uint16 cuda tensor (nsb, 341) of node masks.  Returns (recon uint16 (H, W) cuda
-    tensor, nodes uint8 cuda tensor (nsb, 341 * 12), stats dict)."""
-    import torch
-    if src.dtype != torch.int16 and src.dtype != torch.uint16:
-        raise _lib.MtError("src must be a 16-bit tensor")
-    H, W = src.shape
-    dev = src.device
-    init(dev.index or 0)
-    nsb = ((W + 63) // 64) * ((H + 63) // 64)
-    if recon is None:
-        recon = torch.empty((H, W), dtype=src.dtype, device=dev)
-    if nodes is None:
-        nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
-    if lam is None:
-        lam = rdo_lambda(qidx, bd, 2)
-    f = MtRdoFrame()
-    f.src = MtPlane(_dev_ptr(src).value, src.stride(0), W, H, W, H, 0)
-    f.recon = _dev_ptr(recon).value
-    f.recon_stride = recon.stride(0)
-    f.bd, f.qidx, f.qm_level = bd, qidx, qm_level
-    f.lambda_ = lam
-    f.wsse_scale = wsse_scale
-    if prune is not None:
-        if tuple(prune.shape) != (nsb, NODES) or prune.dtype not in (torch.int16, torch.uint16):
-            raise _lib.MtError("prune must be a 16-bit tensor of shape (nsb, 341)")
-        f.prune = _dev_ptr(prune).value
-    f.nodes = _dev_ptr(nodes).value
-    st = MtRdoStats()
-    _check(lib().mt_intra_rdo(ctypes.byref(f), flags, ctypes.byref(st), _stream_ptr(stream)))
-    return recon, nodes, {"bits": st.bits, "sse": st.sse, "candidates": st.candidates,
-                          "leaves": st.leaves, "sbs": st.sbs, "steps": st.steps}
+    optional uint16 cuda tensor (nsb, 341) of node masks; chroma keywords as
+    intra_rdo_batch's frame dicts.  Returns (recon uint16 (H, W) cuda tensor, nodes uint8
+    cuda tensor (nsb, 341 * 16), stats dict)."""
+    fr = dict(src=src, bd=bd, qidx=qidx, qm_level=qm_level, lam=lam, prune=prune, recon=recon, nodes=nodes)
+    fr.update(chroma)
+    return intra_rdo_batch([fr], flags, wsse_scale, stream)[0]
 
 
 def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale=1.0, stream=None):
     """RDO of several frames in one call (mt_intra_rdo_batch).  frames: list of dicts with
     src (16-bit cuda tensor (H, W)), bd, qidx and optionally qm_level (15), lam (the
-    reference's decision lambda for 4:2:0), prune, recon, nodes.  Returns a list of
-    (recon, nodes, stats) in the same order."""
+    reference's decision lambda for 4:2:0), prune, recon, nodes; with chroma: chroma
+    (CHROMA_420 or CHROMA_444), src_u and src_v (16-bit cuda tensors of the chroma size),
+    optionally uv_dc, uv_ac, uv_qm (default: the reference's laws), recon_u, recon_v.
+    Returns a list of (recon, nodes, stats) in the same order; with chroma, stats holds
+    recon_uv = (recon_u, recon_v), bits_uv, sse_u and sse_v."""
     import torch
     n = len(frames)
     arr = (MtRdoFrame * max(n, 1))()
     outs = []
     for i, fr in enumerate(frames):
         src = fr["src"]
+        if src.dtype != torch.int16 and src.dtype != torch.uint16:
+            raise _lib.MtError("src must be a 16-bit tensor")
         H, W = src.shape
         dev = src.device
         init(dev.index or 0)
@@ -108,11 +118,13 @@ def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale
         nodes = fr.get("nodes")
         if nodes is None:
             nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        ch = int(fr.get("chroma", LUMA_ONLY))
+        sub = 2 if ch == CHROMA_420 else 0
         lam = fr.get("lam")
         if lam is None:
-            lam = rdo_lambda(fr["qidx"], fr["bd"], 2)
+            lam = rdo_lambda(fr["qidx"], fr["bd"], sub if ch else 2)
         f = arr[i]
-        f.src = MtPlane(_dev_ptr(src).value, src.stride(0), W, H, W, H, 0)
+        f.src = _plane(src, W, H)
         f.recon = _dev_ptr(rec).value
         f.recon_stride = rec.stride(0)
         f.bd, f.qidx, f.qm_level = fr["bd"], fr["qidx"], fr.get("qm_level", 15)
@@ -124,12 +136,40 @@ def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale
                 raise _lib.MtError("prune must have shape (nsb, 341)")
             f.prune = _dev_ptr(pr).value
         f.nodes = _dev_ptr(nodes).value
-        outs.append((rec, nodes))
+        uv = None
+        if ch:
+            su, sv = fr["src_u"], fr["src_v"]
+            Hc, Wc = su.shape
+            ru = fr.get("recon_u")
+            rv = fr.get("recon_v")
+            if ru is None:
+                ru = torch.empty((Hc, Wc), dtype=src.dtype, device=dev)
+            if rv is None:
+                rv = torch.empty((Hc, Wc), dtype=src.dtype, device=dev)
+            if ru.stride(0) != rv.stride(0):
+                raise _lib.MtError("recon_u and recon_v must share a stride")
+            dc, ac, qm = chroma_laws(fr["qidx"], sub)
+            f.chroma = ch
+            f.src_u = _plane(su, Wc, Hc)
+            f.src_v = _plane(sv, Wc, Hc)
+            f.recon_u = _dev_ptr(ru).value
+            f.recon_v = _dev_ptr(rv).value
+            f.recon_uv_stride = ru.stride(0)
+            f.uv_dc_delta = fr.get("uv_dc", dc)
+            f.uv_ac_delta = fr.get("uv_ac", ac)
+            f.uv_qm_level = fr.get("uv_qm", qm)
+            uv = (ru, rv)
+        outs.append((rec, nodes, uv))
     st = (MtRdoStats * max(n, 1))()
     _check(lib().mt_intra_rdo_batch(arr, n, flags, st, _stream_ptr(stream)))
-    return [(outs[i][0], outs[i][1], {"bits": st[i].bits, "sse": st[i].sse, "candidates": st[i].candidates,
-                                       "leaves": st[i].leaves, "sbs": st[i].sbs, "steps": st[i].steps})
-            for i in range(n)]
+    res = []
+    for i in range(n):
+        d = {"bits": st[i].bits, "sse": st[i].sse, "candidates": st[i].candidates, "leaves": st[i].leaves,
+             "sbs": st[i].sbs, "steps": st[i].steps}
+        if outs[i][2] is not None:
+            d.update(recon_uv=outs[i][2], bits_uv=st[i].bits_uv, sse_u=st[i].sse_u, sse_v=st[i].sse_v)
+        res.append((outs[i][0], outs[i][1], d))
+    return res
 
 
 def nodes_to_np(nodes):

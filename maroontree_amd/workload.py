@@ -105,6 +105,26 @@ def rgb_to_luma(rgb):
     return np.clip(np.rint(0.299 * r + 0.587 * g + 0.114 * b), 0, 255).astype(np.uint16)
 
 
+def synth_yuv(seed, width, height, bd=8, sub=2):
+    """The recipe's RGB image as full-range BT.601 Y, Cb, Cr at bit depth bd (8-bit values
+    scaled by 2^(bd - 8)); sub 2: chroma 4:2:0 (2x2 means, rounded), 0: 4:4:4.  Returns
+    three uint16 planes."""
+    rgb = synth_rgb(seed, width, height).astype(np.float64)
+    r, g, b = rgb[..., 0], rgb[..., 1], rgb[..., 2]
+    y = 0.299 * r + 0.587 * g + 0.114 * b
+    cb = 128.0 - 0.168736 * r - 0.331264 * g + 0.5 * b
+    cr = 128.0 + 0.5 * r - 0.418688 * g - 0.081312 * b
+    sc = float(1 << (bd - 8))
+    mx = (1 << bd) - 1
+
+    def q(p):
+        return np.clip(np.rint(p * sc), 0, mx).astype(np.uint16)
+    if sub == 2:
+        cb = 0.25 * (cb[0::2, 0::2] + cb[1::2, 0::2] + cb[0::2, 1::2] + cb[1::2, 1::2])
+        cr = 0.25 * (cr[0::2, 0::2] + cr[1::2, 0::2] + cr[0::2, 1::2] + cr[1::2, 1::2])
+    return q(y), q(cb), q(cr)
+
+
 def synth_recon(src, seed, bd=10, amp=6):
     """A stand-in for the decision-time reconstruction: the source plus small noise."""
     rng = np.random.default_rng(seed ^ 0x5EED)

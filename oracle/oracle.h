@@ -36,6 +36,9 @@ void orc_quant_new_chroma(int base_q_idx, int dc_delta, int ac_delta, int bd, in
                           orc_quant* out);
 /* qm_level_law (src/quant.rs:137) */
 int orc_qm_level_law(int base_q_idx, int sub);
+int orc_qm_chroma_level_law(int base_q_idx, int sub);
+int orc_chroma_dc_delta(int base_q_idx, int sub);
+int orc_chroma_ac_delta(int base_q_idx, int sub);
 /* qm_weight (src/quant.rs:92); 32 when flat */
 int orc_qm_weight(int level, int chroma, int rc, int w, int h);
 /* Dct::dequant_step (src/quant.rs:281) */
@@ -188,7 +191,9 @@ uint64_t orc_satd_sad_proxy(const uint16_t* src, int stride, int w, int h, int v
 
 
 /* ---- one intra candidate end to end (src/coder/partition_search.rs:471-520) ---------- */
-/* Same layout as the library's candidate record (48 bytes). */
+/* Same layout as the library's candidate record (48 bytes).  Mode 13 is chroma-from-luma:
+ * DC prediction + alpha * the luma AC of plane txtp_mode at (x << ss_x, y << ss_y); pflags
+ * 8 / 16 subsampling, 32 alpha given (in angle_delta). */
 typedef struct {
     uint32_t offset;
     uint16_t x, y;
@@ -207,6 +212,8 @@ typedef struct {
     float bits, cost;
     uint32_t nnz;
     int32_t status; /* 0, -1 invalid, -2 unsupported (size, type) */
+    int32_t alpha;  /* CfL candidates (mode 13): the alpha used */
+    int32_t pad;
 } orc_candres;
 typedef struct {
     const uint16_t* data;
@@ -224,17 +231,23 @@ void orc_eval_candidate_recon(const orc_cand* c, const orc_plane* planes, int np
  * candidate through orc_eval_candidate. */
 typedef struct {
     int32_t width, height;  /* multiples of 8 */
-    int32_t bd, qidx, qm_level, flags;
+    int32_t bd, qidx, qm_level, flags;  /* flags: 1 angle deltas, 2 filter intra, 4 to 4x4, 8 CfL */
     float lambda, wsse_scale;
+    int32_t chroma;                     /* 0 luma only, 1 4:2:0, 2 4:4:4 */
+    int32_t uv_dc_delta, uv_ac_delta, uv_qm_level;
 } orc_rdo_params;
 typedef struct {
     float cost, none_cost;
-    uint8_t split, mode, pad0, pad1;
+    uint8_t split, mode, uv_mode, pad0;
+    int8_t cfl_alpha_u, cfl_alpha_v;
+    uint8_t pad1[2];
 } orc_rdo_node;
 typedef struct {
     double bits;
     int64_t sse, candidates;
     int32_t leaves, sbs;
+    double bits_uv;
+    int64_t sse_u, sse_v;
 } orc_rdo_stats;
 /* src: width x height samples, src_stride per row; prune: NULL or [nsb][341] node masks
  * (nsb in SB raster order); recon: width x height out (stride width); nodes: [nsb][341]
@@ -243,6 +256,12 @@ typedef struct {
 int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_params* p,
                   const uint16_t* prune, uint16_t* recon, orc_rdo_node* nodes,
                   uint8_t* modes, orc_rdo_stats* stats, int nthreads);
+/* with chroma (p->chroma 1 or 2): src_u / src_v of the chroma size (uv_stride per row),
+ * recon_u / recon_v out (stride: the chroma width) */
+int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint16_t* src_u, const uint16_t* src_v,
+                      int uv_stride, const orc_rdo_params* p, const uint16_t* prune, uint16_t* recon,
+                      uint16_t* recon_u, uint16_t* recon_v, orc_rdo_node* nodes, uint8_t* modes,
+                      orc_rdo_stats* stats, int nthreads);
 /* the decision lambda of the reference (src/coder/lossy_state.rs:1769-1790, src/cost.rs:132,
  * :285): mode_lambda_q(dc_q, bd) * mode_lambda_weight(qidx, sub) / K_PART_LAMBDA */
 float orc_rdo_lambda(int qidx, int bd, int sub);

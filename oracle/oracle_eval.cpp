@@ -51,13 +51,33 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     }
     std::vector<int32_t> pred(w * h), resid(w * h), inv(w * h), levels(orc_tx_ncoef(s));
     std::vector<float> targets(orc_tx_ncoef(s));
-    if (orc_intra_pred(c.mode, c.angle_delta, rp.data, rp.stride, c.x, c.y, w, h, c.pflags & 1,
-                       (c.pflags >> 1) & 1, rp.width, rp.height, (c.pflags >> 2) & 1, bd,
-                       pred.data()) != 0) {
+    const bool cfl = c.mode == 13;
+    const int ssx = (c.pflags >> 3) & 1, ssy = (c.pflags >> 4) & 1;
+    if (cfl && (w != h || w > 32 || c.txtp_set != 0 || c.txtp_mode >= nplanes || c.angle_delta < -16 ||
+                c.angle_delta > 16 || ((c.x + w) << ssx) > planes[c.txtp_mode].width ||
+                ((c.y + h) << ssy) > planes[c.txtp_mode].height)) {
+        r.status = -1;
+        return;
+    }
+    if (orc_intra_pred(cfl ? 0 : c.mode, cfl ? 0 : c.angle_delta, rp.data, rp.stride, c.x, c.y, w, h,
+                       cfl ? 0 : c.pflags & 1, cfl ? 0 : (c.pflags >> 1) & 1, rp.width, rp.height,
+                       cfl ? 0 : (c.pflags >> 2) & 1, bd, pred.data()) != 0) {
         r.status = -1;
         return;
     }
     const uint16_t* src = sp.data + (size_t)c.y * sp.stride + c.x;
+    if (cfl) {  // cfl_ac_sub / cfl_ac_444, cfl_best_alpha, cfl_pred_pixel (src/intrapred.rs:1112-1218)
+        const orc_plane& lp = planes[c.txtp_mode];
+        std::vector<int32_t> ac(w * h);
+        orc_cfl_ac_sub(lp.data + (size_t)(c.y << ssy) * lp.stride + (c.x << ssx), lp.stride, w, h, ssx, ssy,
+                       ac.data());
+        const int dc = pred[0];
+        const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
+        const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);
+        const int alpha = (c.pflags & 32) ? c.angle_delta : orc_cfl_best_alpha(ac.data(), w, src, sp.stride, vw, vh, dc, bd);
+        for (int i = 0; i < w * h; ++i) pred[i] = orc_cfl_pred_pixel(dc, ac[i], alpha, bd);
+        r.alpha = alpha;
+    }
     orc_residual_pred(src, sp.stride, pred.data(), w, h, resid.data());
     orc_quant q;
     if (c.bd_qm & 0x40)  // Quant::new_chroma_with_delta_qm (src/quant.rs:394)
@@ -96,7 +116,7 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     rc.skip_ctx = c.skip_ctx;
     rc.dcs_ctx = c.dcs_ctx;
     rc.txtp_set = c.txtp_set;
-    rc.txtp_mode = c.txtp_mode;
+    rc.txtp_mode = cfl ? 0 : c.txtp_mode;  // CfL: txtp_mode names the luma plane
     rc.txtp = c.txtp;
     rc.use_skip_bits = c.rflags & 1;
     rc.skip_one = c.skip_one;

@@ -23,6 +23,14 @@
 // Decision-time contexts read the final mode / partition maps outside the SB and count
 // neighbours inside the SB as DC_PRED / unsplit.  The final accounting re-evaluates every
 // leaf with the final contexts.
+// Chroma (4:2:0 / 4:4:4), after the SB's luma reconstruction: per chroma block (4:2:0: the
+// s/2 block of each >= 8x8 leaf, one 4x4 block per 8x8 split into 4x4 leaves; 4:4:4: one
+// per leaf, a 64x64 as four 32x32 transform blocks), the 13 uv modes and CfL (where the
+// luma block is <= 32x32) on U and V with the decoder's transform type for the uv mode
+// (chroma_tx_for_mode, src/coder/chroma_tx.rs:46), the chroma quantizer, plain SSE and the
+// plane-1 rate (txb skip context 7); cost = U + V rd costs + lambda * uv_mode_bits
+// (src/coder/lossy_state.rs:1302), ties to the lower mode; then the chroma reconstruction in
+// coding order and the final re-pricing.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -36,7 +44,7 @@ namespace {
 
 constexpr int kNodes = 341;
 constexpr int kLevelBase[5] = {0, 1, 5, 21, 85};
-enum { kAngle = 1, kFilter = 2, kMin4 = 4 };
+enum { kAngle = 1, kFilter = 2, kMin4 = 4, kCfl = 8 };
 enum { kForceSplit = 0x4000, kNoSplit = 0x8000 };
 
 struct Geo {
@@ -91,6 +99,9 @@ struct SymBits {
     float fuse[5][2];  // use_filter_intra at 4x4, 8x8, 16x16, 32x32
     float fmode[5];
     float part[4][4][2];  // [bl - 1][ctx][NONE, SPLIT]
+    float uv[2][13][14];  // uv_mode [CfL allowed][luma mode][uv mode]
+    float cfl_sign[8];
+    float cfl_alpha[6][16];
 };
 const SymBits& sym_bits() {
     static SymBits t = [] {
@@ -112,6 +123,14 @@ const SymBits& sym_bits() {
             b.part[3][ctx][0] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 0);
             b.part[3][ctx][1] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 3);
         }
+        for (int y = 0; y < 13; ++y)
+            for (int m = 0; m < 14; ++m) {
+                b.uv[0][y][m] = m < 13 ? orc_cdf_cost_raw(MT_UV_MODE_NOCFL_CDF[y], 13, m) : 0.0f;
+                b.uv[1][y][m] = orc_cdf_cost_raw(MT_UV_MODE_CFL_CDF[y], 14, m);
+            }
+        for (int k = 0; k < 8; ++k) b.cfl_sign[k] = orc_cdf_cost_raw(MT_CFL_SIGN_CDF, 8, k);
+        for (int c = 0; c < 6; ++c)
+            for (int k = 0; k < 16; ++k) b.cfl_alpha[c][k] = orc_cdf_cost_raw(MT_CFL_ALPHA_CDF[c], 16, k);
         return b;
     }();
     return t;
@@ -141,6 +160,41 @@ float mode_extra_bits(int actx, int lctx, int id, int s, int flags) {
     return bits;
 }
 
+// chroma_tx_for_mode (src/coder/chroma_tx.rs:46): ADST_ADST 3, ADST_DCT 1, DCT_ADST 2 (the
+// AV1 TxType numbering), DCT_DCT otherwise and at 32x32.
+int uv_txtp(int m, int cs) {
+    if (cs >= 32) return 0;
+    if (m == 12 || m == 9 || m == 4) return 3;
+    if (m == 10 || m == 1 || m == 5 || m == 8) return 1;
+    if (m == 11 || m == 2 || m == 6 || m == 7) return 2;
+    return 0;
+}
+
+// uv_mode_bits (src/coder/lossy_state.rs:1302): the uv_mode symbol (CfL-allowed or no-CfL
+// CDF as AV1 codes it), the delta-0 angle symbol where angle deltas are coded (>= 8x8), the
+// CfL joint sign and alpha magnitudes.
+float uv_bits(int ym, int m, int au, int av, bool cfl_allowed, bool angle_ok) {
+    const SymBits& T = sym_bits();
+    if (m == 13) {
+        const int su = au == 0 ? 0 : au < 0 ? 1 : 2, sv = av == 0 ? 0 : av < 0 ? 1 : 2;
+        float b = T.uv[1][ym][13] + T.cfl_sign[su * 3 + sv - 1];
+        if (su) b = b + T.cfl_alpha[(su == 2) * 3 + sv][std::abs(au) - 1];
+        if (sv) b = b + T.cfl_alpha[(sv == 2) * 3 + su][std::abs(av) - 1];
+        return b;
+    }
+    float b = T.uv[cfl_allowed ? 1 : 0][ym][m];
+    if (angle_ok && m >= 1 && m <= 8) b = b + T.angle[m - 1][3];
+    return b;
+}
+
+// A chroma block: its node, chroma origin and size, transform blocks, luma mode context,
+// whether CfL is allowed / angle deltas are coded, the luma block of its availability.
+struct UvB {
+    int node, cx, cy, cs, nt, ym;
+    bool cfl_ok, angle_ok;
+    int lx, ly, ls;
+};
+
 struct Frame {
     const uint16_t* src;
     int sstride, W, H, bd, qidx, qm, flags;
@@ -148,8 +202,89 @@ struct Frame {
     std::vector<uint16_t> rec;     // W x H
     std::vector<uint8_t> mode4;    // (W/4) x (H/4) luma modes
     std::vector<uint8_t> part8;    // (W/8) x (H/8) partition context bytes
-    orc_plane planes[2];
+    orc_plane planes[6];  // source, reconstruction; with chroma U source / reconstruction, V
+    int nplanes = 2;
     int minsz;
+    int chroma = 0, ss = 0, Wc = 0, Hc = 0, uv_dc = 0, uv_ac = 0, uv_qm = 15;
+    std::vector<uint16_t> rec_uv[2];  // Wc x Hc
+    std::vector<uint8_t> uvmode4;     // (Wc/4) x (Hc/4) uv modes
+
+    int uv_at(int cx, int cy) const { return uvmode4[(size_t)(cy >> 2) * (Wc >> 2) + (cx >> 2)]; }
+
+    // the chroma blocks of the SB at (x0, y0) in coding order
+    std::vector<UvB> uv_blocks(const orc_rdo_node* nd, int x0, int y0) const {
+        std::vector<UvB> out;
+        int stack[64], sp = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const int id = stack[--sp];
+            if (nd[id].split == 2) continue;
+            const Geo g = node_geo(id);
+            const bool group = ss && g.s == 8 && nd[id].split == 1;
+            if (nd[id].split == 1 && !group) {
+                for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
+                continue;
+            }
+            UvB b;
+            b.node = id;
+            b.cx = (x0 + g.ix * g.s) >> ss;
+            b.cy = (y0 + g.iy * g.s) >> ss;
+            b.cs = ss ? (g.s == 8 ? 4 : g.s / 2) : g.s;
+            b.nt = b.cs == 64 ? 4 : 1;
+            b.ym = id_base(nd[group ? node_id(4, 2 * g.ix + 1, 2 * g.iy + 1) : id].mode);
+            const int lbs = group ? 4 : g.s;
+            b.cfl_ok = lbs <= 32;
+            b.angle_ok = lbs >= 8;
+            b.lx = g.ix * g.s;
+            b.ly = g.iy * g.s;
+            b.ls = g.s;
+            out.push_back(b);
+        }
+        return out;
+    }
+
+    int uv_modes(const UvB& b) const { return 13 + ((flags & kCfl) && b.cfl_ok ? 1 : 0); }
+
+    // candidate of uv mode m, plane p (0 U, 1 V), transform block t; fin: final contexts and
+    // the given CfL alpha
+    orc_cand uv_cand(const UvB& b, int m, int p, int t, int x0, int y0, bool fin, int alpha) const {
+        orc_cand c;
+        std::memset(&c, 0, sizeof(c));
+        const int ts = b.nt == 4 ? 32 : b.cs;
+        const int tx = b.cx + (t & 1) * ts, ty = b.cy + (t >> 1) * ts;
+        c.x = (uint16_t)tx;
+        c.y = (uint16_t)ty;
+        c.tx_size = (uint8_t)sq_tx(ts);
+        c.tx_type = (uint8_t)(m == 13 ? 0 : uv_txtp(m, ts));
+        c.mode = (uint8_t)m;
+        if (m == 13) {
+            c.pflags = (uint8_t)((ss ? 8 | 16 : 0) | (fin ? 32 : 0));
+            c.angle_delta = (int8_t)(fin ? alpha : 0);
+            c.txtp_mode = 1;  // the luma reconstruction plane
+        } else {
+            const int ax = b.nt == 4 ? (t & 1) * 32 : b.lx, ay = b.nt == 4 ? (t >> 1) * 32 : b.ly;
+            const int as = b.nt == 4 ? 32 : b.ls;
+            const bool above_ok = ty > 0 && (fin || ty == (y0 >> ss));
+            const bool left_ok = tx > 0 && (fin || tx == (x0 >> ss));
+            const int am = above_ok ? uv_at(tx, ty - 1) : 0, lm = left_ok ? uv_at(tx - 1, ty) : 0;
+            const bool smooth = (am >= 9 && am <= 11) || (lm >= 9 && lm <= 11);
+            c.pflags = (uint8_t)((have_tr(ax, ay, as) ? 1 : 0) | (have_bl(ax, ay, as) ? 2 : 0) | (smooth ? 4 : 0));
+        }
+        const int sp = 2 + 2 * p, rp = sp + 1;
+        c.planes = (uint8_t)((sp & 3) | ((rp & 3) << 2));
+        c.planes_hi = (uint8_t)((sp >> 2) | ((rp >> 2) << 4));
+        c.qidx = (uint8_t)qidx;
+        c.bd_qm = (uint8_t)(uv_qm | (bdi_of(bd) << 4) | 0x40);
+        c.dc_delta = (int8_t)uv_dc;
+        c.ac_delta = (int8_t)uv_ac;
+        c.rplane = 1;
+        c.eob_bin = (uint8_t)(ts == 4 ? 0 : ts == 8 ? 5 : ts == 16 ? 11 : 15);
+        c.skip_ctx = 7;
+        c.qctx = (uint8_t)qctx_of(qidx);
+        c.wsse_scale = 0.0f;
+        c.lambda = lambda;
+        return c;
+    }
 
     int mode_at(int x, int y) const { return mode4[(size_t)(y >> 2) * (W >> 2) + (x >> 2)]; }
     int part_at(int x, int y) const { return part8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
@@ -228,7 +363,7 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
             if (mid != 0 && !((m >> id_bit(mid)) & 1)) continue;
             const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, false);
             orc_candres res;
-            orc_eval_candidate(&cd, f.planes, 2, &res);
+            orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
             ++out.candidates;
             if (res.cost < best[id]) {
                 best[id] = res.cost;
@@ -245,6 +380,7 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         orc_rdo_node& nd = nodes[id];
         std::memset(&nd, 0, sizeof(nd));
         nd.mode = (uint8_t)best_id[id];
+        nd.uv_mode = 255;
         if (!searched[id]) {
             cost[id] = 0.0f;
             split[id] = 2;
@@ -296,9 +432,64 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, true);
         orc_candres res;
         std::vector<int32_t> rc((size_t)g.s * g.s);
-        orc_eval_candidate_recon(&cd, f.planes, 2, &res, rc.data());
+        orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
         for (int yy = 0; yy < g.s; ++yy)
             for (int xx = 0; xx < g.s; ++xx) f.rec[(size_t)(y + yy) * f.W + x + xx] = (uint16_t)rc[yy * g.s + xx];
+    }
+    if (!f.chroma) return;
+    // chroma decisions (decision-time contexts), the uv map, then the reconstruction
+    const std::vector<UvB> blocks = f.uv_blocks(nodes, x0, y0);
+    std::vector<int> um(blocks.size()), ua(blocks.size()), uav(blocks.size());
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        const UvB& b = blocks[i];
+        float bestc = __builtin_inff();
+        int bm = 0, ba = 0, bv = 0;
+        for (int m = 0; m < f.uv_modes(b); ++m) {
+            float sum = 0.0f;
+            int au = 0, av = 0;
+            for (int pl = 0; pl < 2; ++pl)
+                for (int t = 0; t < b.nt; ++t) {
+                    const orc_cand cd = f.uv_cand(b, m, pl, t, x0, y0, false, 0);
+                    orc_candres res;
+                    orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
+                    ++out.candidates;
+                    sum = sum + res.cost;
+                    if (m == 13 && t == 0) (pl ? av : au) = res.alpha;
+                }
+            if (m == 13 && au == 0 && av == 0) continue;
+            if (!(sum < __builtin_inff())) continue;
+            const float cost = sum + f.lambda * uv_bits(b.ym, m, au, av, b.cfl_ok, b.angle_ok);
+            if (cost < bestc) {
+                bestc = cost;
+                bm = m;
+                ba = au;
+                bv = av;
+            }
+        }
+        um[i] = bm;
+        ua[i] = ba;
+        uav[i] = bv;
+        nodes[b.node].uv_mode = (uint8_t)bm;
+        nodes[b.node].cfl_alpha_u = (int8_t)ba;
+        nodes[b.node].cfl_alpha_v = (int8_t)bv;
+        for (int yy = 0; yy < b.cs; yy += 4)
+            for (int xx = 0; xx < b.cs; xx += 4)
+                f.uvmode4[(size_t)((b.cy + yy) >> 2) * (f.Wc >> 2) + ((b.cx + xx) >> 2)] = (uint8_t)bm;
+    }
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        const UvB& b = blocks[i];
+        const int ts = b.nt == 4 ? 32 : b.cs;
+        for (int pl = 0; pl < 2; ++pl)
+            for (int t = 0; t < b.nt; ++t) {
+                const orc_cand cd = f.uv_cand(b, um[i], pl, t, x0, y0, true, pl ? uav[i] : ua[i]);
+                orc_candres res;
+                std::vector<int32_t> rc((size_t)ts * ts);
+                orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
+                std::vector<uint16_t>& R = f.rec_uv[pl];
+                for (int yy = 0; yy < ts; ++yy)
+                    for (int xx = 0; xx < ts; ++xx)
+                        R[(size_t)(cd.y + yy) * f.Wc + cd.x + xx] = (uint16_t)rc[yy * ts + xx];
+            }
     }
 }
 
@@ -339,7 +530,19 @@ extern "C" float orc_rdo_lambda(int qidx, int bd, int sub) {
 extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_params* p,
                              const uint16_t* prune, uint16_t* recon, orc_rdo_node* nodes,
                              uint8_t* modes, orc_rdo_stats* stats, int nthreads) {
+    if (!p || p->chroma != 0) return -1;
+    return orc_intra_rdo_yuv(src, src_stride, nullptr, nullptr, 0, p, prune, recon, nullptr, nullptr, nodes, modes,
+                             stats, nthreads);
+}
+
+extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint16_t* src_u, const uint16_t* src_v,
+                                 int uv_stride, const orc_rdo_params* p, const uint16_t* prune, uint16_t* recon,
+                                 uint16_t* recon_u, uint16_t* recon_v, orc_rdo_node* nodes, uint8_t* modes,
+                                 orc_rdo_stats* stats, int nthreads) {
     if (!p || !src || !recon || !nodes || !stats) return -1;
+    if (p->chroma < 0 || p->chroma > 2) return -1;
+    if (p->chroma && (!src_u || !src_v || !recon_u || !recon_v || p->uv_qm_level < 0 || p->uv_qm_level > 15))
+        return -1;
     if (p->width <= 0 || p->height <= 0 || (p->width & 7) || (p->height & 7) || p->width > 65535 ||
         p->height > 65535 || src_stride < p->width)
         return -1;
@@ -363,6 +566,26 @@ extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_
     f.part8.assign((size_t)(f.W / 8) * (f.H / 8), 0);
     f.planes[0] = {src, src_stride, f.W, f.H, f.W, f.H, 0};
     f.planes[1] = {f.rec.data(), f.W, f.W, f.H, f.W, f.H, 0};
+    f.chroma = p->chroma;
+    if (f.chroma) {
+        f.ss = f.chroma == 1 ? 1 : 0;
+        f.Wc = f.W >> f.ss;
+        f.Hc = f.H >> f.ss;
+        if (uv_stride < f.Wc) return -1;
+        f.uv_dc = p->uv_dc_delta;
+        f.uv_ac = p->uv_ac_delta;
+        f.uv_qm = p->uv_qm_level;
+        const uint16_t* cs[2] = {src_u, src_v};
+        for (int k = 0; k < 2; ++k) {
+            f.rec_uv[k].resize((size_t)f.Wc * f.Hc);
+            for (int y = 0; y < f.Hc; ++y)
+                std::memcpy(&f.rec_uv[k][(size_t)y * f.Wc], cs[k] + (size_t)y * uv_stride, f.Wc * 2);
+            f.planes[2 + 2 * k] = {cs[k], uv_stride, f.Wc, f.Hc, f.Wc, f.Hc, 0};
+            f.planes[3 + 2 * k] = {f.rec_uv[k].data(), f.Wc, f.Wc, f.Hc, f.Wc, f.Hc, 0};
+        }
+        f.uvmode4.assign((size_t)(f.Wc / 4) * (f.Hc / 4), 0);
+        f.nplanes = 6;
+    }
     const int C = (f.W + 63) / 64, R = (f.H + 63) / 64, nsb = C * R;
     std::vector<SbOut> outs(nsb);
     const int nsteps = (C - 1) + 2 * (R - 1) + 1;
@@ -393,8 +616,8 @@ extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_
     double bits = 0.0;
     int64_t sse = 0, cands = 0;
     int32_t leaves = 0;
-    std::vector<double> sb_bits(nsb, 0.0);
-    std::vector<int64_t> sb_sse(nsb, 0);
+    std::vector<double> sb_bits(nsb, 0.0), sb_bits_uv(nsb, 0.0);
+    std::vector<int64_t> sb_sse(nsb, 0), sb_sse_u(nsb, 0), sb_sse_v(nsb, 0);
     auto account = [&](int sb) {
         const int r = sb / C, c = sb % C, x0 = c * 64, y0 = r * 64;
         const orc_rdo_node* nd = nodes + (size_t)sb * kNodes;
@@ -415,12 +638,33 @@ extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_
             }
             const orc_cand cd = f.cand(x, y, g.s, nd[id].mode, x0, y0, true);
             orc_candres res;
-            orc_eval_candidate(&cd, f.planes, 2, &res);
+            orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
             b += (double)res.bits;
             e += res.sse;
         }
+        double buv = 0.0;
+        int64_t euv[2] = {0, 0};
+        if (f.chroma) {
+            for (const UvB& ub : f.uv_blocks(nd, x0, y0)) {
+                const orc_rdo_node& o = nd[ub.node];
+                for (int pl = 0; pl < 2; ++pl)
+                    for (int t = 0; t < ub.nt; ++t) {
+                        orc_cand cd = f.uv_cand(ub, o.uv_mode, pl, t, x0, y0, true, pl ? o.cfl_alpha_v : o.cfl_alpha_u);
+                        if (pl == 0 && t == 0)
+                            cd.extra_bits = uv_bits(ub.ym, o.uv_mode, o.cfl_alpha_u, o.cfl_alpha_v, ub.cfl_ok, ub.angle_ok);
+                        orc_candres res;
+                        orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
+                        b += (double)res.bits;
+                        buv += (double)res.bits;
+                        euv[pl] += res.sse;
+                    }
+            }
+        }
         sb_bits[sb] = b;
         sb_sse[sb] = e;
+        sb_bits_uv[sb] = buv;
+        sb_sse_u[sb] = euv[0];
+        sb_sse_v[sb] = euv[1];
     };
     {
         std::vector<std::thread> th;
@@ -431,13 +675,25 @@ extern "C" int orc_intra_rdo(const uint16_t* src, int src_stride, const orc_rdo_
             });
         for (auto& x : th) x.join();
     }
+    double bits_uv = 0.0;
+    int64_t sse_u = 0, sse_v = 0;
     for (int sb = 0; sb < nsb; ++sb) {
         bits += sb_bits[sb];
         sse += sb_sse[sb];
+        bits_uv += sb_bits_uv[sb];
+        sse_u += sb_sse_u[sb];
+        sse_v += sb_sse_v[sb];
         cands += outs[sb].candidates;
         leaves += (int32_t)outs[sb].leaves.size();
     }
     std::memcpy(recon, f.rec.data(), f.rec.size() * 2);
+    if (f.chroma) {
+        std::memcpy(recon_u, f.rec_uv[0].data(), f.rec_uv[0].size() * 2);
+        std::memcpy(recon_v, f.rec_uv[1].data(), f.rec_uv[1].size() * 2);
+    }
+    stats->bits_uv = bits_uv;
+    stats->sse_u = sse_u;
+    stats->sse_v = sse_v;
     if (modes) std::memcpy(modes, f.mode4.data(), f.mode4.size());
     stats->bits = bits;
     stats->sse = sse;

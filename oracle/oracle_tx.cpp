@@ -481,6 +481,44 @@ extern "C" int orc_qm_level_law(int base_q_idx, int sub) {
     return (int)(uint8_t)orc_fast_round(v);
 }
 
+// qm_chroma_level_law (src/quant.rs:155) and the frame chroma qindex deltas chroma_dc_delta /
+// chroma_ac_delta (:204-240) with the shipped tuning (src/tuning.rs:750-791).
+namespace {
+int orc_uv420_mid_delta(int qi) {
+    const int d = 28, tail = 28;  // uv420_mid_d, uv420_tail_delta
+    if (qi <= 20) return 0;
+    if (qi < 64) return -(d * (qi - 20)) / 44;
+    if (qi <= 96) return -d;
+    if (qi < 136) return -tail - ((d - tail) * (136 - qi)) / 40;
+    return -tail;
+}
+}  // namespace
+
+extern "C" int orc_chroma_ac_delta(int q, int sub) {
+    if (sub == 2) return orc_uv420_mid_delta(q);
+    if (q <= 20) return 6;  // top_444_uac
+    if (q >= 96) return 24;
+    return 24 * (q - 20) / 76;
+}
+
+extern "C" int orc_chroma_dc_delta(int q, int sub) {
+    const int deep = q <= 160 ? 0 : -std::min((q - 160) / 3, 32);
+    return sub == 2 ? deep + orc_uv420_mid_delta(q) : deep;
+}
+
+extern "C" int orc_qm_chroma_level_law(int q, int sub) {
+    if (sub != 0) return orc_qm_level_law(q, sub);
+    const int pinned = std::min(2, orc_qm_level_law(q, 0));
+    volatile float hq = (12.0f - (float)q) / 5.0f;  // (qm_hq_knee - q) / qm_hq_width
+    const float t = 1.0f * std::min(std::max((float)hq, 0.0f), 1.0f);  // top_ease = 1
+    if (t > 0.0f) {
+        volatile float a = (15.0f - (float)pinned) * t;
+        volatile float b = (float)pinned + a;
+        return (int)(uint8_t)std::round((float)b);
+    }
+    return pinned;
+}
+
 // ---------------------------------------------------------------- size helpers
 extern "C" int orc_tx_w(int s) { return (s >= 0 && s < 19) ? kTxW[s] : 0; }
 extern "C" int orc_tx_h(int s) { return (s >= 0 && s < 19) ? kTxH[s] : 0; }

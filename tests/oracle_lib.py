@@ -287,7 +287,8 @@ class OrcPlane(ctypes.Structure):
 
 
 CANDRES_DTYPE = np.dtype([("sse", "<i8"), ("dist", "<i8"), ("satd", "<u8"), ("bits", "<f4"),
-                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4")])
+                          ("cost", "<f4"), ("nnz", "<u4"), ("status", "<i4"), ("alpha", "<i4"),
+                          ("pad", "<i4")])
 
 
 def eval_batch(cands, planes, nthreads=8):
@@ -311,18 +312,21 @@ def eval_batch(cands, planes, nthreads=8):
 
 # ---- whole-frame luma intra RDO (oracle/oracle_rdo.cpp) ---------------------------------
 RDO_NODE_DTYPE = np.dtype([("cost", "<f4"), ("none_cost", "<f4"), ("split", "u1"), ("mode", "u1"),
-                           ("pad", "u1", (2,))])
+                           ("uv_mode", "u1"), ("pad0", "u1"), ("cfl_alpha_u", "i1"), ("cfl_alpha_v", "i1"),
+                           ("pad1", "u1", (2,))])
 
 
 class OrcRdoParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("bd", ctypes.c_int32),
                 ("qidx", ctypes.c_int32), ("qm_level", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("lambda_", ctypes.c_float), ("wsse_scale", ctypes.c_float)]
+                ("lambda_", ctypes.c_float), ("wsse_scale", ctypes.c_float), ("chroma", ctypes.c_int32),
+                ("uv_dc_delta", ctypes.c_int32), ("uv_ac_delta", ctypes.c_int32), ("uv_qm_level", ctypes.c_int32)]
 
 
 class OrcRdoStats(ctypes.Structure):
     _fields_ = [("bits", ctypes.c_double), ("sse", ctypes.c_int64), ("candidates", ctypes.c_int64),
-                ("leaves", ctypes.c_int32), ("sbs", ctypes.c_int32)]
+                ("leaves", ctypes.c_int32), ("sbs", ctypes.c_int32), ("bits_uv", ctypes.c_double),
+                ("sse_u", ctypes.c_int64), ("sse_v", ctypes.c_int64)]
 
 
 def _rdo_sigs(L):
@@ -331,11 +335,54 @@ def _rdo_sigs(L):
     L.orc_intra_rdo.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcRdoParams),
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.POINTER(OrcRdoStats), ctypes.c_int]
+    vp = ctypes.c_void_p
+    L.orc_intra_rdo_yuv.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.POINTER(OrcRdoParams), vp, vp,
+                                    vp, vp, vp, vp, ctypes.POINTER(OrcRdoStats), ctypes.c_int]
+    for f in ("orc_qm_chroma_level_law", "orc_chroma_dc_delta", "orc_chroma_ac_delta"):
+        getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int]
     L.orc_rdo_lambda.argtypes = [ctypes.c_int] * 3
     L.orc_rdo_lambda.restype = ctypes.c_float
     L.orc_mode_lambda.argtypes = [ctypes.c_int] * 3
     L.orc_mode_lambda.restype = ctypes.c_float
     L._rdo_sigs = True
+
+
+def chroma_laws(qidx, sub):
+    """(chroma_dc_delta, chroma_ac_delta, qm_chroma_level_law) of the oracle."""
+    L = lib()
+    _rdo_sigs(L)
+    return L.orc_chroma_dc_delta(qidx, sub), L.orc_chroma_ac_delta(qidx, sub), L.orc_qm_chroma_level_law(qidx, sub)
+
+
+def intra_rdo_yuv(src, src_u, src_v, bd, qidx, qm_level, flags, lam, chroma, uv_dc, uv_ac, uv_qm,
+                  wsse_scale=1.0, prune=None, nthreads=8):
+    """orc_intra_rdo_yuv: the RDO with chroma (chroma 1 4:2:0, 2 4:4:4).  Returns (recon,
+    recon_u, recon_v, nodes [nsb, 341], stats dict)."""
+    L = lib()
+    _rdo_sigs(L)
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    su = np.ascontiguousarray(src_u, dtype=np.uint16)
+    sv = np.ascontiguousarray(src_v, dtype=np.uint16)
+    H, W = src.shape
+    Hc, Wc = su.shape
+    nsb = ((W + 63) // 64) * ((H + 63) // 64)
+    p = OrcRdoParams(W, H, bd, qidx, qm_level, flags, lam, wsse_scale, chroma, uv_dc, uv_ac, uv_qm)
+    recon = np.zeros((H, W), np.uint16)
+    ru = np.zeros((Hc, Wc), np.uint16)
+    rv = np.zeros((Hc, Wc), np.uint16)
+    nodes = np.zeros((nsb, 341), RDO_NODE_DTYPE)
+    st = OrcRdoStats()
+    pr = None
+    if prune is not None:
+        pr = np.ascontiguousarray(prune, dtype=np.uint16)
+    rc = L.orc_intra_rdo_yuv(src.ctypes.data, W, su.ctypes.data, sv.ctypes.data, Wc, ctypes.byref(p),
+                             None if pr is None else pr.ctypes.data, recon.ctypes.data, ru.ctypes.data,
+                             rv.ctypes.data, nodes.ctypes.data, None, ctypes.byref(st), nthreads)
+    if rc != 0:
+        raise ValueError("orc_intra_rdo_yuv: invalid parameters")
+    return recon, ru, rv, nodes, {"bits": st.bits, "sse": st.sse, "candidates": st.candidates,
+                                  "leaves": st.leaves, "sbs": st.sbs, "bits_uv": st.bits_uv,
+                                  "sse_u": st.sse_u, "sse_v": st.sse_v}
 
 
 def rdo_lambda(qidx, bd, sub):

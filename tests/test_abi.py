@@ -111,7 +111,7 @@ def test_descriptor_layouts():
     assert mt.PREDBLK_DTYPE.itemsize == 16
     assert mt.DISTBLK_DTYPE.itemsize == 24 and mt.DISTRES_DTYPE.itemsize == 32
     assert mt.CFLBLK_DTYPE.itemsize == 24
-    assert mt.CAND_DTYPE.itemsize == 48 and mt.CANDRES_DTYPE.itemsize == 40
+    assert mt.CAND_DTYPE.itemsize == 48 and mt.CANDRES_DTYPE.itemsize == 48
     assert mt.CAND_DTYPE.fields["rplane"][1] == 16 and mt.CAND_DTYPE.fields["skip_one"][1] == 28
     assert ctypes.sizeof(mt.MtPlane) == 32
     assert mt.RATEBLK_DTYPE.itemsize == 32
