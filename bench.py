@@ -317,40 +317,53 @@ def bench_eval(args, dev, torch, mt):
 
 
 def bench_rdo(args, dev, torch, mt):
-    """Exact full intra RDO of a seeded 3840x2160 10-bit frame (luma) at qindex 91: one step =
-    one mt_intra_rdo call (wavefront decisions, reconstruction, final accounting)."""
+    """Exact full intra RDO of a seeded 3840x2160 10-bit 4:2:0 frame at quality 50 (qindex 91):
+    one step = one mt_intra_rdo call (luma and chroma wavefront decisions with CfL,
+    reconstruction, final accounting).  The luma-only RDO of the same frame is timed too."""
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
     width, height, bd, qidx = 3840, 2160, 10, 91
-    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
-    img = W.synth_image(SEED, width, height, bd)
-    qm = mt.qm_level_law(qidx, 0)
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
+    y, u, v = W.synth_yuv(SEED, width, height, bd, 2)
+    qm = mt.qm_level_law(qidx, 2)
     lam = R.rdo_lambda(qidx, bd, 2)
-    src = torch.from_numpy(img.view(np.int16)).to(dev)
-    rec = torch.empty_like(src)
+    t = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
     nsb = ((width + 63) // 64) * ((height + 63) // 64)
-    nodes = torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    frame = {"src": t(y), "src_u": t(u), "src_v": t(v), "bd": bd, "qidx": qidx, "qm_level": qm, "lam": lam,
+             "chroma": R.CHROMA_420, "recon": torch.empty((height, width), dtype=torch.int16, device=dev),
+             "recon_u": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
+             "recon_v": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
+             "nodes": torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)}
+    luma = {k: frame[k] for k in ("src", "bd", "qidx", "qm_level", "lam", "recon", "nodes")}
     stream = torch.cuda.current_stream()
-    for _ in range(max(args.warmup, 1)):
-        _, _, st = R.intra_rdo(src, bd, qidx, qm, flags, lam, recon=rec, nodes=nodes, stream=stream)
-    torch.cuda.synchronize()
-    mt.profile_begin()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.rdo_steps):
-        _, _, st = R.intra_rdo(src, bd, qidx, qm, flags, lam, recon=rec, nodes=nodes, stream=stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = mt.profile_end()
-    ms = (t1 - t0) * 1000.0 / args.rdo_steps
+
+    def timed(fr, fl, steps, prof=False):
+        for _ in range(max(args.warmup, 1)):
+            st = R.intra_rdo_batch([fr], fl, stream=stream)[0][2]
+        torch.cuda.synchronize()
+        if prof:
+            mt.profile_begin()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = R.intra_rdo_batch([fr], fl, stream=stream)[0][2]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        return ms, st, (mt.profile_end() if prof else None)
+
+    ms, st, prof = timed(frame, flags, args.rdo_steps, prof=True)
+    nd = R.nodes_to_np(frame["nodes"])
+    cfl_blocks, chroma_blocks = int((nd["uv_mode"] == 13).sum()), int((nd["uv_mode"] != 255).sum())
+    ms_l, st_l, _ = timed(luma, flags & ~R.CFL, args.rdo_steps)
     npx = width * height
-    print(f"[bench] rdo: {ms:.2f} ms/frame, {npx / ms / 1e3:.1f} Mpx/s, "
-          f"{st['candidates'] / ms / 1e3:.1f} M candidates/s", file=sys.stderr, flush=True)
+    print(f"[bench] rdo 4:2:0: {ms:.2f} ms/frame, {npx / ms / 1e3:.1f} Mpx/s, "
+          f"{st['candidates'] / ms / 1e3:.1f} M candidates/s; luma only {ms_l:.2f} ms", file=sys.stderr, flush=True)
     r = {
-        "workload": "exact full luma intra RDO of a seeded 3840x2160 10-bit synthetic frame at qindex 91 "
-                    "(qm_level_law QM, the reference's decision lambda for 4:2:0, weighted SSE): "
-                    "64x64 superblocks on the anti-diagonal wavefront, partitions 64..4, 13 modes + "
-                    "angle deltas + filter intra, DCT_DCT",
+        "workload": "exact full intra RDO of a seeded 3840x2160 10-bit 4:2:0 synthetic frame (RGB recipe -> "
+                    "BT.601 YCbCr) at quality 50, qindex 91 (qm_level_law QMs, the reference's chroma qindex "
+                    "deltas and decision lambda, weighted luma SSE): 64x64 superblocks on the anti-diagonal "
+                    "wavefront, partitions 64..4, 13 luma modes + angle deltas + filter intra (DCT_DCT), "
+                    "13 uv modes + CfL per chroma block (decoder-derived chroma transforms)",
         "ms_per_frame": round(ms, 3),
         "mpx_per_s": round(npx / (ms / 1000.0) / 1e6, 2),
         "candidates": st["candidates"],
@@ -358,7 +371,14 @@ def bench_rdo(args, dev, torch, mt):
         "leaves": st["leaves"],
         "steps": st["steps"],
         "bits": round(st["bits"], 1),
+        "bits_uv": round(st["bits_uv"], 1),
         "psnr_y": round(R.psnr(st["sse"], npx, bd), 4),
+        "psnr_u": round(R.psnr(st["sse_u"], npx // 4, bd), 4),
+        "psnr_v": round(R.psnr(st["sse_v"], npx // 4, bd), 4),
+        "chroma_blocks": chroma_blocks,
+        "cfl_blocks": cfl_blocks,
+        "luma_only": {"ms_per_frame": round(ms_l, 3), "mpx_per_s": round(npx / (ms_l / 1000.0) / 1e6, 2),
+                      "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
     }
     if not args.no_cpu:
@@ -367,9 +387,11 @@ def bench_rdo(args, dev, torch, mt):
         threads = min(16, os.cpu_count() or 1)
         # a bounded crop of the same frame: 16 x 8 superblocks
         cw, ch = 1024, 512
-        crop = np.ascontiguousarray(img[:ch, :cw])
+        uvq = R.chroma_laws(qidx, 2)
         t0 = time.perf_counter()
-        _, _, _, cst = O.intra_rdo(crop, bd, qidx, qm, flags, lam, nthreads=threads)
+        _, _, _, _, cst = O.intra_rdo_yuv(np.ascontiguousarray(y[:ch, :cw]), np.ascontiguousarray(u[:ch // 2, :cw // 2]),
+                                          np.ascontiguousarray(v[:ch // 2, :cw // 2]), bd, qidx, qm, flags, lam,
+                                          R.CHROMA_420, *uvq, nthreads=threads)
         cpu_s = time.perf_counter() - t0
         r["cpu_baseline"] = {
             "value": round(cw * ch / cpu_s / 1e6, 4),
@@ -377,8 +399,8 @@ def bench_rdo(args, dev, torch, mt):
             "candidates_per_s": round(cst["candidates"] / cpu_s, 1),
             "cores": threads,
             "kind": "port",
-            "sample": f"the top-left {cw}x{ch} crop of the same frame ({cst['candidates']} candidates), "
-                      f"{cpu_s:.1f} s wall, std::thread over the superblocks of each wavefront step",
+            "sample": f"the top-left {cw}x{ch} crop of the same frame with its chroma ({cst['candidates']} "
+                      f"candidates), {cpu_s:.1f} s wall, std::thread over the superblocks of each wavefront step",
         }
         r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
     return r
@@ -521,15 +543,16 @@ def bench_plain512(args, dev, torch, mt):
 
 def bench_stream(args, dev, torch, mt, net=None):
     """Uneven stream: seeded frames of log-uniform sizes 64 .. max side (mixed aspect
-    ratios, 8 / 10-bit, 4:2:0 / 4:4:4 lambda laws, qindex 140 / 91 / 53 / 17), all queued at
-    t = 0 and coded in groups of 16 frames of similar wavefront length (one
-    mt_intra_rdo_batch each).  Aggregate Mpx/s and per-frame latency (completion of its
-    group); with the learned pruning too when a trained network is given."""
+    ratios, 8 / 10-bit, 4:2:0 / 4:4:4 chroma, qindex 140 / 91 / 53 / 17), all queued at
+    t = 0 and coded in groups of 10 frames of similar wavefront length (one
+    mt_intra_rdo_batch each: 6 planes per frame fill the plane set).  Aggregate Mpx/s and
+    per-frame latency (completion of its group); with the learned pruning too when a
+    trained network is given."""
     from maroontree_amd import prune as PR
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
     rng = np.random.default_rng(SEED)
-    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
     specs = []
     for i in range(args.stream_images):
         w, h = (int(np.exp(rng.uniform(np.log(64), np.log(args.stream_max_side)))) // 8 * 8 for _ in range(2))
@@ -537,11 +560,13 @@ def bench_stream(args, dev, torch, mt, net=None):
         sub = int(rng.choice([0, 2]))
         q = int(rng.choice([140, 91, 53, 17]))
         specs.append((w, h, bd, sub, q))
-    imgs = W.synth_images([(70000 + i, w, h, bd) for i, (w, h, bd, _, _) in enumerate(specs)])
-    srcs = [torch.from_numpy(im.view(np.int16)).to(dev) for im in imgs]
+    yuvs = W.synth_yuvs([(70000 + i, w, h, bd, sub) for i, (w, h, bd, sub, _) in enumerate(specs)])
+    t16 = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
+    srcs = [tuple(t16(p) for p in yuv) for yuv in yuvs]
+    del yuvs
     steps = [((w + 63) // 64 - 1) + 2 * ((h + 63) // 64 - 1) + 1 for (w, h, _, _, _) in specs]
     order = sorted(range(len(specs)), key=lambda i: -steps[i])
-    groups = [order[i:i + 16] for i in range(0, len(order), 16)]
+    groups = [order[i:i + 10] for i in range(0, len(order), 10)]
     npx = sum(w * h for (w, h, _, _, _) in specs)
 
     def run(model):
@@ -553,10 +578,11 @@ def bench_stream(args, dev, torch, mt, net=None):
             frames = []
             for i in g:
                 w, h, bd, sub, q = specs[i]
-                fr = {"src": srcs[i], "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, sub),
-                      "lam": R.rdo_lambda(q, bd, sub)}
+                fr = {"src": srcs[i][0], "src_u": srcs[i][1], "src_v": srcs[i][2], "bd": bd, "qidx": q,
+                      "qm_level": mt.qm_level_law(q, sub), "lam": R.rdo_lambda(q, bd, sub),
+                      "chroma": R.CHROMA_420 if sub == 2 else R.CHROMA_444}
                 if model is not None:
-                    fr["prune"] = PR.prune_masks(model, srcs[i], bd, q, args.topk, args.lo, args.hi,
+                    fr["prune"] = PR.prune_masks(model, srcs[i][0], bd, q, args.topk, args.lo, args.hi,
                                                  args.mode_cover)  # per frame
                 frames.append(fr)
             out = R.intra_rdo_batch(frames, flags)
@@ -572,9 +598,10 @@ def bench_stream(args, dev, torch, mt, net=None):
                 "latency_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1)}
 
     run(None)  # warm-up
-    r = {"workload": f"{len(specs)} seeded frames, sides log-uniform 64..{args.stream_max_side} (multiples of 8), "
-                     "8/10-bit, 4:2:0/4:4:4 lambda, qindex 140/91/53/17, exact full luma RDO; groups of 16 "
-                     "frames of similar wavefront length per mt_intra_rdo_batch",
+    r = {"workload": f"{len(specs)} seeded frames (RGB recipe -> BT.601 YCbCr), sides log-uniform "
+                     f"64..{args.stream_max_side} (multiples of 8), 8/10-bit, 4:2:0/4:4:4 chroma, qindex "
+                     "140/91/53/17, exact full RDO (luma, chroma, CfL); groups of 10 frames of similar wavefront "
+                     "length per mt_intra_rdo_batch",
          "frames": len(specs), "mpx": round(npx / 1e6, 2), "pruning_off": run(None)}
     if net is not None:
         r["pruning_on"] = run(net)

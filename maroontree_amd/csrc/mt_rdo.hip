@@ -13,14 +13,16 @@
 //   rdo_decide_kernel  one wave per SB: bottom-up NONE / SPLIT level by level, leaves in
 //                      coding order, the final mode / partition maps and leaf descriptors;
 //   sb_recon_launch    one wave per SB reconstructs its leaves (mt_sbrecon.h).
-// With chroma, in the same step:
-//   rdo_uv_gen_kernel     one wave per SB: the chroma blocks of the decided leaves in coding
-//                         order and every (uv mode, plane, transform block) candidate, in a
-//                         fixed slot range per SB (unused slots are marked invalid);
-//   eval_launch           the batched evaluation of the chroma candidates;
+// With chroma, the chroma of step k runs one step behind, inside step k + 1 (its
+// candidates need step k's luma reconstruction and step k - 1's chroma, nothing of step
+// k + 1), so one batched evaluation per step serves both:
+//   rdo_uv_gen_kernel     one wave per SB of step k: the chroma blocks of the decided leaves
+//                         in coding order and every (uv mode, plane, transform block)
+//                         candidate, in a fixed slot range per SB after step k + 1's luma
+//                         candidates (unused slots are marked invalid);
 //   rdo_uv_decide_kernel  one wave per SB: per chroma block the cheapest uv mode (U + V +
 //                         lambda * uv mode bits), the uv map and the reconstruction list;
-//   sb_recon_uv_launch    one wave per SB reconstructs its chroma blocks (mt_sbrecon.h).
+//   sb_recon_uv_launch    one wave per (SB, plane) reconstructs the chroma blocks.
 // Afterwards every leaf and chroma block is re-priced with the final contexts in one batched
 // evaluation.
 #include <hip/hip_runtime.h>
@@ -658,7 +660,8 @@ __device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p
 // does not use get an invalid transform size (the evaluation skips them).
 __global__ __launch_bounds__(64) void rdo_uv_gen_kernel(const FrameArgs* __restrict__ frames,
                                                         const SbTask* __restrict__ tasks, uint32_t maxc,
-                                                        uint32_t maxp, mt_cand* __restrict__ cands,
+                                                        uint32_t maxp, uint32_t pool_base,
+                                                        mt_cand* __restrict__ cands,
                                                         UvBlk* __restrict__ blks, uint32_t* __restrict__ nblks) {
     __shared__ UvBlk s_b[kMaxUvBlocks];
     __shared__ int s_nb;
@@ -691,7 +694,7 @@ __global__ __launch_bounds__(64) void rdo_uv_gen_kernel(const FrameArgs* __restr
     __syncthreads();
     const int nb = s_nb;
     mt_cand* out = cands + (size_t)local * maxc;
-    const uint32_t pool0 = (uint32_t)local * maxp;
+    const uint32_t pool0 = pool_base + (uint32_t)local * maxp;
     for (int i = 0; i < nb; ++i) {
         const UvBlk b = s_b[i];
         const int ts = b.nt == 4 ? 32 : b.cs, per = 2 * b.nt;
@@ -915,7 +918,6 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
 // Library-owned buffers of one device, grown on demand.
 struct RdoWork {
     size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map = 0, cap_frames = 0;
-    size_t cap_uv = 0;
     uint32_t *sb_uv_cands = nullptr, *sb_uv_pool = nullptr, *sb_uv_dec = nullptr;
     double* sb_bits_uv = nullptr;
     long long* sb_sse_uv = nullptr;
@@ -923,8 +925,6 @@ struct RdoWork {
     uint32_t* nuvblk = nullptr;
     mt_predblk* uvdesc = nullptr;
     uint32_t *uvfirst = nullptr, *uvcount = nullptr;
-    mt_cand* ucands = nullptr;
-    mt_candres* ures = nullptr;
     uint32_t *counts = nullptr, *pools = nullptr, *sb_leaves = nullptr, *sb_leaf_pool = nullptr;
     uint32_t *leaf_base = nullptr, *pool_base = nullptr;
     double* sb_bits = nullptr;
@@ -1173,6 +1173,17 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     }
     step_first[max_steps] = (int)tasks.size();
     if (max_sb >= 65536) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
+    // each step's evaluation: its luma candidates, then the chroma slots of the step before
+    auto step_nk = [&](int k) { return (k < 0 || k >= max_steps) ? 0 : step_first[k + 1] - step_first[k]; };
+    std::vector<uint64_t> eval_n(max_steps + 1), eval_pool(max_steps + 1);
+    for (int k = 0; k <= max_steps; ++k) {
+        const uint64_t pk = any_chroma ? (uint64_t)step_nk(k - 1) : 0;
+        eval_n[k] = (k < max_steps ? step_n[k] : 0) + pk * maxc;
+        eval_pool[k] = (k < max_steps ? step_pool[k] : 0) + pk * maxp;
+        if (eval_pool[k] > UINT32_MAX || eval_n[k] > (uint64_t)INT32_MAX)
+            return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
+        max_cand = std::max(max_cand, (size_t)eval_n[k]);
+    }
     if (w.cap_step_sb < max_sb) {
         if ((rc = regrow(w.keys, max_sb * kNodes)) || (rc = regrow(w.plan, max_sb * kNodes)) ||
             (rc = regrow(w.leaves, max_sb * kMaxLeaves)) || (rc = regrow(w.first, max_sb)) ||
@@ -1182,13 +1193,6 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             return rc;
         w.cap_step_sb = max_sb;
     }
-    if (any_chroma) {
-        if ((uint64_t)max_sb * maxp > UINT32_MAX) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: batch step too large");
-        if (w.cap_uv < max_sb * maxc) {
-            if ((rc = regrow(w.ucands, max_sb * maxc)) || (rc = regrow(w.ures, max_sb * maxc))) return rc;
-            w.cap_uv = max_sb * maxc;
-        }
-    }
     if (w.cap_cand < max_cand) {
         if ((rc = regrow(w.cands, max_cand)) || (rc = regrow(w.refs, max_cand)) || (rc = regrow(w.cres, max_cand)))
             return rc;
@@ -1196,50 +1200,50 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     }
     MT_HIP_TRY(hipMemcpyAsync(w.tasks, tasks.data(), tasks.size() * sizeof(SbTask), hipMemcpyHostToDevice, s));
 
-    for (int k = 0; k < max_steps; ++k) {
-        const int nk = step_first[k + 1] - step_first[k];
-        if (nk == 0) continue;
-        const SbTask* tk = w.tasks + step_first[k];
-        const int n = (int)step_n[k];
+    for (int k = 0; k <= max_steps; ++k) {
+        const int nk = step_nk(k), pk = any_chroma ? step_nk(k - 1) : 0;
+        if (nk == 0 && pk == 0) continue;
+        const SbTask* tk = w.tasks + step_first[std::min(k, max_steps)];
+        const SbTask* tp = pk ? w.tasks + step_first[k - 1] : nullptr;
+        const int n = k < max_steps ? (int)step_n[k] : 0;
         {
             mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
-                               w.keys, (size_t)nk * kNodes, ~0ull);
-            hipLaunchKernelGGL(rdo_plan_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.plan);
-            hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk * kNodes), dim3(64), 0, s, w.fargs, tk, w.plan, w.cands,
-                               w.refs);
+            if (nk) {
+                hipLaunchKernelGGL(rdo_fill_u64_kernel, dim3(std::max(1, (nk * kNodes + 255) / 256)), dim3(256), 0, s,
+                                   w.keys, (size_t)nk * kNodes, ~0ull);
+                hipLaunchKernelGGL(rdo_plan_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.plan);
+                hipLaunchKernelGGL(rdo_gen_kernel, dim3(nk * kNodes), dim3(64), 0, s, w.fargs, tk, w.plan, w.cands,
+                                   w.refs);
+            }
+            if (pk)  // the chroma candidates of step k - 1 after this step's luma candidates
+                hipLaunchKernelGGL(rdo_uv_gen_kernel, dim3(pk), dim3(64), 0, s, w.fargs, tp, maxc, maxp,
+                                   (uint32_t)(k < max_steps ? step_pool[k] : 0), w.cands + n, w.uvblk, w.nuvblk);
             MT_HIP_TRY(hipGetLastError());
         }
-        if ((rc = mtlib::eval_launch(w.cands, n, ps, step_pool[k], w.cres, s))) return rc;
-        {
-            mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            if (n > 0)
-                hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n, w.keys);
-            hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.keys, w.plan, w.leaves,
-                               w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
-            MT_HIP_TRY(hipGetLastError());
-        }
-        {
+        if ((rc = mtlib::eval_launch(w.cands, (int)eval_n[k], ps, eval_pool[k], w.cres, s))) return rc;
+        if (nk) {
+            {
+                mtlib::ProfScope prof(mtlib::kProfRdo, s);
+                if (n > 0)
+                    hipLaunchKernelGGL(rdo_argmin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w.refs, w.cres, n,
+                                       w.keys);
+                hipLaunchKernelGGL(rdo_decide_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, w.keys, w.plan, w.leaves,
+                                   w.first, w.count, w.sb_leaves, w.sb_leaf_pool);
+                MT_HIP_TRY(hipGetLastError());
+            }
             mtlib::ProfScope prof(mtlib::kProfRecon, s);
             if ((rc = mtlib::sb_recon_launch(w.leaves, w.first, w.count, nk, w.rframes, s))) return rc;
         }
-        if (!any_chroma) continue;
-        {
-            mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            hipLaunchKernelGGL(rdo_uv_gen_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, maxc, maxp, w.ucands,
-                               w.uvblk, w.nuvblk);
-            MT_HIP_TRY(hipGetLastError());
-        }
-        if ((rc = mtlib::eval_launch(w.ucands, (int)(nk * maxc), ps, (uint64_t)nk * maxp, w.ures, s))) return rc;
-        {
-            mtlib::ProfScope prof(mtlib::kProfRdo, s);
-            hipLaunchKernelGGL(rdo_uv_decide_kernel, dim3(nk), dim3(64), 0, s, w.fargs, tk, maxc, w.ures, w.uvblk,
-                               w.nuvblk, w.uvdesc, w.uvfirst, w.uvcount, w.sb_uv_cands, w.sb_uv_pool, w.sb_uv_dec);
-            MT_HIP_TRY(hipGetLastError());
-        }
-        {
+        if (pk) {
+            {
+                mtlib::ProfScope prof(mtlib::kProfRdo, s);
+                hipLaunchKernelGGL(rdo_uv_decide_kernel, dim3(pk), dim3(64), 0, s, w.fargs, tp, maxc, w.cres + n,
+                                   w.uvblk, w.nuvblk, w.uvdesc, w.uvfirst, w.uvcount, w.sb_uv_cands, w.sb_uv_pool,
+                                   w.sb_uv_dec);
+                MT_HIP_TRY(hipGetLastError());
+            }
             mtlib::ProfScope prof(mtlib::kProfRecon, s);
-            if ((rc = mtlib::sb_recon_uv_launch(w.uvdesc, w.uvfirst, w.uvcount, nk, w.rframes, s))) return rc;
+            if ((rc = mtlib::sb_recon_uv_launch(w.uvdesc, w.uvfirst, w.uvcount, pk, w.rframes, s))) return rc;
         }
     }
 

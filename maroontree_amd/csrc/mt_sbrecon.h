@@ -228,8 +228,9 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
     }
 }
 
-// Chroma of the RDO: one wave per superblock walks its chroma transform blocks
-// desc[first[b] .. + count[b]) in coding order.  Per block: the prediction from the chroma
+// Chroma of the RDO: one wave per (superblock, plane) walks the superblock's chroma
+// transform blocks of that plane (blockIdx.y: 0 U, 1 V) in desc[first[b] .. + count[b]),
+// in coding order (the planes do not depend on each other).  Per block: the prediction from the chroma
 // reconstruction (CfL, mode 13: the DC prediction plus alpha (angle_delta) times the AC of
 // the frame's luma reconstruction), the residual, the chain of the transform type in
 // out_offset >> 1 with the chroma quantizer, and the clamped reconstruction; out_offset & 1
@@ -259,6 +260,7 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
     for (uint32_t i = 0; i < n; i++) {
         mt_predblk d = desc[f0 + i];
         const int pl = (int)(d.out_offset & 1), type = (int)(d.out_offset >> 1);
+        if (pl != (int)blockIdx.y) continue;
         const mt_plane src = fr.src_uv[pl];
         uint16_t* __restrict__ rec = fr.rec_uv[pl];
         const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
@@ -301,7 +303,7 @@ namespace mtlib {
 int sb_recon_uv_launch(const mt_predblk* desc_d, const uint32_t* first_d, const uint32_t* count_d, int nsb,
                        const ReconFrame* frames_d, hipStream_t s) {
     if (nsb <= 0) return MT_OK;
-    hipLaunchKernelGGL(sb_recon_uv_kernel, dim3(nsb), dim3(64), 0, s, desc_d, first_d, count_d, frames_d);
+    hipLaunchKernelGGL(sb_recon_uv_kernel, dim3(nsb, 2), dim3(64), 0, s, desc_d, first_d, count_d, frames_d);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -125,6 +125,13 @@ def synth_yuv(seed, width, height, bd=8, sub=2):
     return q(y), q(cb), q(cr)
 
 
+def synth_yuvs(specs, threads=16):
+    """synth_yuv over a list of (seed, width, height, bd, sub) on a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(lambda s: synth_yuv(*s), specs))
+
+
 def synth_recon(src, seed, bd=10, amp=6):
     """A stand-in for the decision-time reconstruction: the source plus small noise."""
     rng = np.random.default_rng(seed ^ 0x5EED)
