@@ -45,6 +45,8 @@ def main():
                     help="profiling aid: run only the candidate-evaluation leg and print its object")
     ap.add_argument("--no-rdo", action="store_true", help="skip the whole-frame RDO leg")
     ap.add_argument("--rdo-steps", type=int, default=3)
+    ap.add_argument("--rdo-skip-luma", action="store_true",
+                    help="time only the 4:2:0 RDO (not the luma-only RDO beside it)")
     ap.add_argument("--rdo-only", action="store_true",
                     help="profiling aid: run only the whole-frame RDO leg and print its object")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruning leg")
@@ -53,6 +55,8 @@ def main():
     ap.add_argument("--learned-train", type=int, default=200, help="training frames (1920x1080)")
     ap.add_argument("--learned-eval", type=int, default=50, help="held-out frames per quality")
     ap.add_argument("--learned-epochs", type=int, default=40)
+    ap.add_argument("--learned-luma-only", action="store_true",
+                    help="run the learned-pruning comparison on the luma RDO alone")
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the uneven-stream leg (pruning off) and print its object")
@@ -354,10 +358,10 @@ def bench_rdo(args, dev, torch, mt):
     ms, st, prof = timed(frame, flags, args.rdo_steps, prof=True)
     nd = R.nodes_to_np(frame["nodes"])
     cfl_blocks, chroma_blocks = int((nd["uv_mode"] == 13).sum()), int((nd["uv_mode"] != 255).sum())
-    ms_l, st_l, _ = timed(luma, flags & ~R.CFL, args.rdo_steps)
+    ms_l, st_l, _ = timed(luma, flags & ~R.CFL, args.rdo_steps) if not args.rdo_skip_luma else (None, None, None)
     npx = width * height
     print(f"[bench] rdo 4:2:0: {ms:.2f} ms/frame, {npx / ms / 1e3:.1f} Mpx/s, "
-          f"{st['candidates'] / ms / 1e3:.1f} M candidates/s; luma only {ms_l:.2f} ms", file=sys.stderr, flush=True)
+          f"{st['candidates'] / ms / 1e3:.1f} M candidates/s; luma only {ms_l} ms", file=sys.stderr, flush=True)
     r = {
         "workload": "exact full intra RDO of a seeded 3840x2160 10-bit 4:2:0 synthetic frame (RGB recipe -> "
                     "BT.601 YCbCr) at quality 50, qindex 91 (qm_level_law QMs, the reference's chroma qindex "
@@ -377,8 +381,9 @@ def bench_rdo(args, dev, torch, mt):
         "psnr_v": round(R.psnr(st["sse_v"], npx // 4, bd), 4),
         "chroma_blocks": chroma_blocks,
         "cfl_blocks": cfl_blocks,
-        "luma_only": {"ms_per_frame": round(ms_l, 3), "mpx_per_s": round(npx / (ms_l / 1000.0) / 1e6, 2),
-                      "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
+        "luma_only": None if ms_l is None else {
+            "ms_per_frame": round(ms_l, 3), "mpx_per_s": round(npx / (ms_l / 1000.0) / 1e6, 2),
+            "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
     }
     if not args.no_cpu:
@@ -408,22 +413,26 @@ def bench_rdo(args, dev, torch, mt):
 
 def bench_learned(args, dev, torch, mt, return_net=False):
     """Learned pruning: the network trained on the GPU on exact-RDO labels of seeded 1920x1080
-    8-bit frames, then, on held-out frames at quality 30 / 50 / 70 / 90 (qindex 140 / 91 /
-    53 / 17): the plain GPU form (one mt_intra_rdo call per frame, full search), the batched
-    form (mt_intra_rdo_batch over the frames, full search: identical results), and the
-    batched + learned form (one network forward over all superblocks, pruned search);
-    speedups, BD-rate (PSNR-Y, rate = the RDO's final bits) of the pruned search, and the
-    same policy with the untrained network."""
+    8-bit 4:2:0 frames (its luma decisions), then, on held-out frames at quality 30 / 50 / 70
+    / 90 (qindex 140 / 91 / 53 / 17): the plain GPU form (one mt_intra_rdo call per frame,
+    full search), the batched form (mt_intra_rdo_batch over the frames, full search:
+    identical results), and the batched + learned form (one network forward over all
+    superblocks, pruned luma search); every form codes the chroma too (4:2:0, CfL) unless
+    --learned-luma-only.  Speedups, BD-rate (PSNR-Y against the RDO's final bits, luma and
+    chroma) of the pruned search, and the same policy with the untrained network."""
     from maroontree_amd import prune as PR
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
     width, height, bd = 1920, 1080, 8
-    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    chroma = not args.learned_luma_only
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | (R.CFL if chroma else 0)
     qs = [140, 91, 53, 17]
     log = lambda m: print(f"[bench] learned: {m}", file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
     train_seeds = [10000 + i for i in range(args.learned_train)]
-    data = PR.make_dataset(dev, train_seeds, width, height, bd, qs, flags)
+    train_y = [yuv[0] for yuv in W.synth_yuvs([(s, width, height, bd, 2) for s in train_seeds])]
+    data = PR.make_dataset(dev, train_seeds, width, height, bd, qs, flags & ~R.CFL, images=train_y)
+    del train_y
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     n_train_sb = int(data[0].shape[0])
@@ -435,12 +444,21 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del data
-    eval_imgs = W.synth_images([(90000 + i, width, height, bd) for i in range(args.learned_eval)])
-    srcs = [torch.from_numpy(img.view(np.int16)).to(dev) for img in eval_imgs]
+    eval_yuv = W.synth_yuvs([(90000 + i, width, height, bd, 2) for i in range(args.learned_eval)])
+    t16 = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
+    planes = [tuple(t16(p) for p in yuv) for yuv in eval_yuv]
+    del eval_yuv
+    srcs = [p[0] for p in planes]
     npx = width * height * len(srcs)
 
     def frames_at(q):
-        return [{"src": s, "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, 0)} for s in srcs]
+        out = []
+        for y, u, v in planes:
+            fr = {"src": y, "bd": bd, "qidx": q, "qm_level": mt.qm_level_law(q, 2)}
+            if chroma:
+                fr.update(src_u=u, src_v=v, chroma=R.CHROMA_420)
+            out.append(fr)
+        return out
 
     def plain(q):
         torch.cuda.synchronize()
@@ -478,8 +496,10 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     full_pts = res["batched"]
     out = {
         "workload": f"{args.learned_train} training and {args.learned_eval} held-out seeded 1920x1080 8-bit "
-                    "frames; labels from the exact full RDO (partitions 64..4, 13 modes + angle deltas + "
-                    "filter intra); qindex 140/91/53/17",
+                    f"{'4:2:0 ' if chroma else ''}frames (RGB recipe -> BT.601 YCbCr); labels from the exact full "
+                    "luma RDO (partitions 64..4, 13 modes + angle deltas + filter intra); qindex 140/91/53/17; "
+                    + ("every form codes the chroma (13 uv modes + CfL), the network prunes the luma search"
+                       if chroma else "luma only"),
         "network": "conv3x3s2 1-16-32-64, pool, fc 257-64-98; 46178 params; bf16 MFMA, fp32 accumulation",
         "train_superblocks": n_train_sb,
         "label_seconds": round(t_data, 2),
