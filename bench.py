@@ -528,36 +528,43 @@ def bench_learned(args, dev, torch, mt, return_net=False):
 
 
 def bench_plain512(args, dev, torch, mt):
-    """The plain-encode configuration: a seeded 512x512 RGB image, its full-range BT.601
-    luma coded at quality 60 (qindex 70), 4:4:4 lambda, by the exact RDO on the GPU and by
-    the oracle's RDO on the host (no entropy coder: the rate is the RDO's final bits)."""
+    """The plain-encode configuration: a seeded 512x512 RGB image as full-range BT.601 YCbCr
+    4:4:4, quality 60 (qindex 70), the reference's 4:4:4 lambda, chroma quantizer laws and
+    QMs, coded by the exact RDO (luma, chroma, CfL) on the GPU and by the oracle's RDO on the
+    host (no entropy coder: the rate is the RDO's final bits)."""
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
-    luma = W.rgb_to_luma(W.synth_rgb(SEED, 512, 512))
     q, bd = 70, 8
+    y, u, v = W.synth_yuv(SEED, 512, 512, bd, 0)
     qm = mt.qm_level_law(q, 0)
     lam = R.rdo_lambda(q, bd, 0)
-    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
-    src = torch.from_numpy(luma.view(np.int16)).to(dev)
-    R.intra_rdo(src, bd, q, qm, flags, lam)
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
+    t = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
+    fr = {"src": t(y), "src_u": t(u), "src_v": t(v), "bd": bd, "qidx": q, "qm_level": qm, "lam": lam,
+          "chroma": R.CHROMA_444}
+    R.intra_rdo_batch([fr], flags)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, _, st = R.intra_rdo(src, bd, q, qm, flags, lam)
+    _, _, st = R.intra_rdo_batch([fr], flags)[0]
     torch.cuda.synchronize()
     gpu_ms = (time.perf_counter() - t0) * 1e3
-    r = {"workload": "seeded 512x512 RGB (gradients, noise, rectangles) -> BT.601 luma, quality 60 (qindex 70), "
-                     "4:4:4 lambda, exact full luma RDO",
+    npx = 512 * 512
+    r = {"workload": "seeded 512x512 RGB (gradients, noise, rectangles) -> BT.601 YCbCr 4:4:4, quality 60 "
+                     "(qindex 70), 4:4:4 lambda and chroma laws, exact full RDO (luma, chroma, CfL)",
          "bits": round(st["bits"], 1), "bytes_rate_model": round(st["bits"] / 8.0, 1),
-         "psnr_y": round(R.psnr(st["sse"], 512 * 512, bd), 4), "gpu_ms": round(gpu_ms, 2)}
+         "psnr_y": round(R.psnr(st["sse"], npx, bd), 4), "psnr_u": round(R.psnr(st["sse_u"], npx, bd), 4),
+         "psnr_v": round(R.psnr(st["sse_v"], npx, bd), 4), "gpu_ms": round(gpu_ms, 2)}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
         threads = min(16, os.cpu_count() or 1)
+        uvq = R.chroma_laws(q, 0)
         t0 = time.perf_counter()
-        _, _, _, cst = O.intra_rdo(luma, bd, q, qm, flags, lam, nthreads=threads)
+        *_, cst = O.intra_rdo_yuv(y, u, v, bd, q, qm, flags, lam, R.CHROMA_444, *uvq, nthreads=threads)
         r["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         r["cpu_cores"] = threads
-        r["identical_to_cpu"] = bool(cst["bits"] == st["bits"] and cst["sse"] == st["sse"])
+        r["identical_to_cpu"] = bool(cst["bits"] == st["bits"] and cst["sse"] == st["sse"] and
+                                     cst["sse_u"] == st["sse_u"] and cst["sse_v"] == st["sse_v"])
     return r
 
 
