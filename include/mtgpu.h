@@ -389,7 +389,8 @@ int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes, in
  *    candidate: the transform type the decoder derives from the uv mode
  *    (chroma_tx_for_mode, src/coder/chroma_tx.rs:46; DCT_DCT at 32x32), the chroma
  *    quantizer (qidx with uv_dc_delta / uv_ac_delta, uv_qm_level), plain SSE, the plane-1
- *    coefficient rate with txb skip context 7 and dc sign context 0;
+ *    coefficient rate (the chroma eob_bin CDFs) with txb skip context 7 and dc sign
+ *    context 0;
  *  - cost(mode) = the rd_costs summed over U then V, each over its transform blocks in
  *    raster order, + lambda * uv_mode_bits (src/coder/lossy_state.rs:1302: the uv_mode
  *    symbol under the luma mode, from the CfL-allowed or the no-CfL CDF as AV1 codes it; the

@@ -647,7 +647,7 @@ __device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p
     c.dc_delta = (int8_t)f.uv_dc;
     c.ac_delta = (int8_t)f.uv_ac;
     c.rplane = 1;
-    c.eob_bin = (uint8_t)(ts == 4 ? 0 : ts == 8 ? 5 : ts == 16 ? 11 : 15);
+    c.eob_bin = (uint8_t)(ts == 4 ? 1 : ts == 8 ? 6 : ts == 16 ? 12 : 16);  // the chroma eob_bin CDFs
     c.skip_ctx = 7;
     c.qctx = (uint8_t)(f.qidx <= 20 ? 0 : f.qidx <= 60 ? 1 : f.qidx <= 120 ? 2 : 3);
     c.wsse_scale = 0.0f;

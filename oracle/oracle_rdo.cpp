@@ -278,7 +278,7 @@ struct Frame {
         c.dc_delta = (int8_t)uv_dc;
         c.ac_delta = (int8_t)uv_ac;
         c.rplane = 1;
-        c.eob_bin = (uint8_t)(ts == 4 ? 0 : ts == 8 ? 5 : ts == 16 ? 11 : 15);
+        c.eob_bin = (uint8_t)(ts == 4 ? 1 : ts == 8 ? 6 : ts == 16 ? 12 : 16);  // chroma eob_bin CDFs
         c.skip_ctx = 7;
         c.qctx = (uint8_t)qctx_of(qidx);
         c.wsse_scale = 0.0f;

@@ -142,7 +142,7 @@ def test_eval_cfl_and_chroma_candidates_match_oracle(dev):
         c["bd_qm"] = rng.integers(0, 16) | (1 << 4) | 0x40
         c["dc_delta"], c["ac_delta"] = rng.integers(-30, 30), rng.integers(-30, 30)
         c["rplane"] = 1
-        c["eob_bin"] = {4: 0, 8: 5, 16: 11, 32: 15}[s]
+        c["eob_bin"] = {4: 1, 8: 6, 16: 12, 32: 16}[s]
         c["skip_ctx"] = 7 + rng.integers(0, 3)
         c["qctx"] = rng.integers(0, 4)
         c["lambda"] = 40.0

@@ -226,7 +226,7 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
                 c["bd_qm"] = uvq[2] | (1 << 4) | 0x40
                 c["dc_delta"], c["ac_delta"] = uvq[0], uvq[1]
                 c["rplane"] = 1
-                c["eob_bin"] = {4: 0, 8: 5, 16: 11, 32: 15}[cs]
+                c["eob_bin"] = {4: 1, 8: 6, 16: 12, 32: 16}[cs]  # the chroma eob_bin CDFs
                 c["skip_ctx"] = 7
                 c["qctx"] = 2
                 c["lambda"] = lam
