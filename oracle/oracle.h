@@ -231,16 +231,17 @@ void orc_eval_candidate_recon(const orc_cand* c, const orc_plane* planes, int np
  * candidate through orc_eval_candidate. */
 typedef struct {
     int32_t width, height;  /* multiples of 8 */
-    int32_t bd, qidx, qm_level, flags;  /* flags: 1 angle deltas, 2 filter intra, 4 to 4x4, 8 CfL */
+    int32_t bd, qidx, qm_level, flags;  /* 1 angle deltas, 2 filter intra, 4 to 4x4, 8 CfL, 16 HORZ / VERT */
     float lambda, wsse_scale;
     int32_t chroma;                     /* 0 luma only, 1 4:2:0, 2 4:4:4 */
     int32_t uv_dc_delta, uv_ac_delta, uv_qm_level;
 } orc_rdo_params;
 typedef struct {
     float cost, none_cost;
-    uint8_t split, mode, uv_mode, pad0;
-    int8_t cfl_alpha_u, cfl_alpha_v;
-    uint8_t pad1[2];
+    uint8_t split, mode, rmode0, rmode1; /* split: 0 NONE, 1 SPLIT, 2 not coded, 3 HORZ, 4 VERT */
+    uint8_t uv_mode, uv_mode1;
+    int8_t cfl_alpha_u, cfl_alpha_v, cfl_alpha_u1, cfl_alpha_v1;
+    uint8_t pad[2];
 } orc_rdo_node;
 typedef struct {
     double bits;
@@ -269,6 +270,9 @@ float orc_mode_lambda(int qidx, int bd, int sub);
 /* the RDO's top-right / bottom-left availability of an s x s block at (xin, yin) of its SB */
 int orc_rdo_have_tr(int xin, int yin, int s);
 int orc_rdo_have_bl(int xin, int yin, int s);
+/* half k (0 / 1 top / bottom of HORZ, 2 / 3 left / right of VERT) of the s x s node at
+ * (xin, yin): bit 0 top-right, bit 1 bottom-left coded before it */
+int orc_rdo_rect_avail(int xin, int yin, int s, int k);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,5 @@
-// CPU oracle: the whole-frame luma intra RDO of the library (mt_intra_rdo), restated on the
-// CPU with the oracle's candidate evaluation.
+// CPU oracle: the whole-frame intra RDO of the library (mt_intra_rdo), restated on the CPU
+// with the oracle's candidate evaluation.
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.h).  The decision procedure is the library's own
 // (the reference's ~35k-line decision heuristics are not part of the path); every quantity
@@ -13,24 +13,27 @@
 //
 // Procedure, per 64x64 superblock (SB) in wavefront order (SB (r, c) in step c + 2r):
 //  1. every quadtree node (64 .. 8, or .. 4) inside the frame evaluates every allowed mode
-//     (13 intra modes, the 6 angle deltas of the 8 directional modes at >= 8x8, the 5
-//     filter-intra modes at <= 32x32) as one DCT_DCT transform block, predicting from the
-//     reconstruction plane; samples of the SB itself still hold the source there (the
-//     decision sees the source inside its own SB and the final reconstruction outside);
-//  2. bottom-up, each node keeps PARTITION_NONE (its best mode) or SPLIT (the sum of its
-//     children), each priced with its partition symbol;
-//  3. the chosen leaves are reconstructed in coding (Z) order from the reconstruction.
+//     (13 intra modes, the 6 angle deltas of the 8 directional modes on blocks of >= 8 in
+//     both directions, the 5 filter-intra modes up to 32 in both) as one DCT_DCT transform
+//     block, predicting from the reconstruction plane; samples of the SB itself still hold
+//     the source there (the decision sees the source inside its own SB and the final
+//     reconstruction outside); with the rectangular flag, the two halves of PARTITION_HORZ
+//     and of PARTITION_VERT of every 32x32 / 16x16 (and, down to 4x4, 8x8) node likewise;
+//  2. bottom-up, each node keeps PARTITION_NONE (its best mode), HORZ / VERT (the sums of
+//     their halves' best) or SPLIT (the sum of its children), each priced with its
+//     partition symbol; ties keep the first of NONE, HORZ, VERT, SPLIT;
+//  3. the chosen leaves are reconstructed in coding order from the reconstruction.
 // Decision-time contexts read the final mode / partition maps outside the SB and count
 // neighbours inside the SB as DC_PRED / unsplit.  The final accounting re-evaluates every
 // leaf with the final contexts.
 // Chroma (4:2:0 / 4:4:4), after the SB's luma reconstruction: per chroma block (4:2:0: the
-// s/2 block of each >= 8x8 leaf, one 4x4 block per 8x8 split into 4x4 leaves; 4:4:4: one
-// per leaf, a 64x64 as four 32x32 transform blocks), the 13 uv modes and CfL (where the
-// luma block is <= 32x32) on U and V with the decoder's transform type for the uv mode
-// (chroma_tx_for_mode, src/coder/chroma_tx.rs:46), the chroma quantizer, plain SSE and the
-// plane-1 rate (txb skip context 7); cost = U + V rd costs + lambda * uv_mode_bits
-// (src/coder/lossy_state.rs:1302), ties to the lower mode; then the chroma reconstruction in
-// coding order and the final re-pricing.
+// half-size block of each leaf of >= 8 in both directions, one 4x4 block per 8x8 node
+// split into 4x4 / 8x4 / 4x8 leaves; 4:4:4: one per leaf, a 64x64 as four 32x32 transform
+// blocks), the 13 uv modes and CfL (where the luma block is <= 32x32) on U and V with the
+// decoder's transform type for the uv mode (chroma_tx_for_mode, src/coder/chroma_tx.rs:46),
+// the chroma quantizer, plain SSE and the plane-1 rate (txb skip context 7); cost = U + V
+// rd costs + lambda * uv_mode_bits (src/coder/lossy_state.rs:1302), ties to the lower
+// mode; then the chroma reconstruction in coding order and the final re-pricing.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -44,8 +47,10 @@ namespace {
 
 constexpr int kNodes = 341;
 constexpr int kLevelBase[5] = {0, 1, 5, 21, 85};
-enum { kAngle = 1, kFilter = 2, kMin4 = 4, kCfl = 8 };
+enum { kAngle = 1, kFilter = 2, kMin4 = 4, kCfl = 8, kRect = 16 };
 enum { kForceSplit = 0x4000, kNoSplit = 0x8000 };
+// node decisions (the split field of a node record)
+enum { kNone = 0, kSplit = 1, kUncoded = 2, kHorz = 3, kVert = 4 };
 
 struct Geo {
     int L, ix, iy, s;
@@ -76,6 +81,26 @@ bool have_bl(int xin, int yin, int s) {
     return zidx(xin / s - 1, yin / s + 1) < zidx(xin / s, yin / s);
 }
 
+// A block to predict: position, size, and whether its top-right / bottom-left neighbours
+// are coded before it.
+struct Blk {
+    int x, y, w, h;
+    bool tr, bl;
+};
+// Half k of a node at (x, y) of size s with availability (ntr, nbl): 0 / 1 the top / bottom
+// of PARTITION_HORZ, 2 / 3 the left / right of PARTITION_VERT.  The top half's bottom-left
+// and the left half's top-right lie in blocks coded before the node; the bottom half's
+// top-right and the right half's bottom-left in blocks coded after it.
+Blk rect_blk(int x, int y, int s, int k, bool ntr, bool nbl) {
+    const int h = s / 2;
+    switch (k) {
+        case 0: return {x, y, s, h, ntr, true};
+        case 1: return {x, y + h, s, h, false, nbl};
+        case 2: return {x, y, h, s, true, nbl};
+        default: return {x + h, y, h, s, ntr, false};
+    }
+}
+
 // Mode ids 0..65: 0..12 the intra modes; 13 + 6 (m - 1) + j the directional mode m with
 // delta {-3, -2, -1, 1, 2, 3}[j]; 61..65 filter intra 0..4.
 constexpr int kModeIds = 66;
@@ -86,19 +111,47 @@ int id_delta(int id) {
 }
 int id_base(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 0; }
 int id_bit(int id) { return id <= 12 ? id : id <= 60 ? 1 + (id - 13) / 6 : 13; }
-bool id_valid(int id, int s, int flags) {
+// angle deltas are coded on blocks of >= 8x8 (av1_use_angle_delta: both sides >= 8 here),
+// filter intra on blocks up to 32x32
+bool id_valid(int id, int w, int h, int flags) {
     if (id <= 12) return true;
-    if (id <= 60) return (flags & kAngle) && s >= 8;
-    return (flags & kFilter) && s <= 32;
+    if (id <= 60) return (flags & kAngle) && w >= 8 && h >= 8;
+    return (flags & kFilter) && w <= 32 && h <= 32;
+}
+
+// AV1 BlockSize index of a w x h block (4x4 0, 4x8 1, 8x4 2, 8x8 3, ..., 64x64 12).
+int bsize_index(int w, int h) {
+    const int lw = __builtin_ctz((unsigned)w) - 2, lh = __builtin_ctz((unsigned)h) - 2;
+    return 3 * std::min(lw, lh) + (lw == lh ? 0 : lw < lh ? 1 : 2);
+}
+// AV1 TxSize of a w x h transform (the 13 reference sizes).
+int tx_of(int w, int h) {
+    if (w == h) return w == 4 ? 0 : w == 8 ? 1 : w == 16 ? 2 : w == 32 ? 3 : 4;
+    if (w == 4 && h == 8) return 5;
+    if (w == 8 && h == 4) return 6;
+    if (w == 8 && h == 16) return 7;
+    if (w == 16 && h == 8) return 8;
+    if (w == 16 && h == 32) return 9;
+    return 10;  // 32x16
+}
+// eob_bin CDF index of a block of n coefficients (numbering of the rate descriptor)
+int eob_luma(int n) { return n == 16 ? 0 : n == 32 ? 3 : n == 64 ? 5 : n == 128 ? 8 : n == 256 ? 11 : n == 512 ? 13 : 15; }
+int eob_chroma(int n) { return n == 16 ? 1 : n == 32 ? 4 : n == 64 ? 6 : n == 128 ? 9 : n == 256 ? 12 : n == 512 ? 14 : 16; }
+// the intra tx-type symbol set of a w x h luma transform: 1 / 2 the 7-type set with the
+// 4x4 / 8x8 tables (by the smaller side), 3 the 5-type set at 16x16, 0 none (32 and up)
+int txtp_set_of(int w, int h) {
+    if (w >= 32 || h >= 32) return 0;
+    if (w == 16 && h == 16) return 3;
+    return std::min(w, h) == 4 ? 1 : 2;
 }
 
 // Symbol costs of the frame-initial decision CDFs (Cdfs::new, src/coder/cdfs.rs:400).
 struct SymBits {
     float kf[5][5][13];
     float angle[8][7];
-    float fuse[5][2];  // use_filter_intra at 4x4, 8x8, 16x16, 32x32
+    float fuse[13][2];    // use_filter_intra by block size index
     float fmode[5];
-    float part[4][4][2];  // [bl - 1][ctx][NONE, SPLIT]
+    float part[4][4][4];  // [bl - 1][ctx][NONE, HORZ, VERT, SPLIT]
     float uv[2][13][14];  // uv_mode [CfL allowed][luma mode][uv mode]
     float cfl_sign[8];
     float cfl_alpha[6][16];
@@ -111,17 +164,14 @@ const SymBits& sym_bits() {
                 for (int m = 0; m < 13; ++m) b.kf[a][l][m] = orc_cdf_cost_raw(MT_KF_Y_MODE_CDF[a][l], 13, m);
         for (int m = 0; m < 8; ++m)
             for (int j = 0; j < 7; ++j) b.angle[m][j] = orc_cdf_cost_raw(MT_ANGLE_DELTA_CDF[m], 7, j);
-        static const int bsz[5] = {0, 3, 6, 9, 12};  // av1_block_size_index of 4x4 .. 64x64
-        for (int L = 0; L < 5; ++L)
-            for (int u = 0; u < 2; ++u) b.fuse[L][u] = orc_cdf_cost_raw(&MT_FILTER_INTRA_CDF[bsz[L]], 2, u);
+        for (int k = 0; k < 13; ++k)
+            for (int u = 0; u < 2; ++u) b.fuse[k][u] = orc_cdf_cost_raw(&MT_FILTER_INTRA_CDF[k], 2, u);
         for (int k = 0; k < 5; ++k) b.fmode[k] = orc_cdf_cost_raw(MT_FILTER_INTRA_MODE_CDF, 5, k);
         for (int ctx = 0; ctx < 4; ++ctx) {
-            for (int bl = 1; bl <= 3; ++bl) {
-                b.part[bl - 1][ctx][0] = orc_cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 0);
-                b.part[bl - 1][ctx][1] = orc_cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, 3);
-            }
-            b.part[3][ctx][0] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 0);
-            b.part[3][ctx][1] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, 3);
+            for (int bl = 1; bl <= 3; ++bl)
+                for (int sym = 0; sym < 4; ++sym)
+                    b.part[bl - 1][ctx][sym] = orc_cdf_cost_raw(MT_PART_SPLIT_CDF[bl - 1][ctx], 10, sym);
+            for (int sym = 0; sym < 4; ++sym) b.part[3][ctx][sym] = orc_cdf_cost_raw(MT_PART_BL8_CDF[ctx], 4, sym);
         }
         for (int y = 0; y < 13; ++y)
             for (int m = 0; m < 14; ++m) {
@@ -138,32 +188,34 @@ const SymBits& sym_bits() {
 
 const int kIntraModeCtx[13] = {0, 1, 2, 3, 4, 4, 4, 4, 3, 0, 1, 2, 0};  // src/intrapred.rs:1088
 const int kFilterDir[5] = {0, 1, 2, 6, 0};  // Filter_Intra_Mode_To_Intra_Dir
-const uint8_t kPartByte[5] = {0x10, 0x18, 0x1c, 0x1e, 0x1f};  // leaf of 64 .. 8; 4x4 split
+// partition context byte of a block dimension (part_ctx_bytes, src/coder/superblock.rs:1062):
+// the above context takes the width's, the left context the height's
+uint8_t part_byte(int d) { return d == 64 ? 0x10 : d == 32 ? 0x18 : d == 16 ? 0x1c : d == 8 ? 0x1e : 0x1f; }
+// the partition symbol of a node decision
+int part_sym(int split) { return split == kHorz ? 1 : split == kVert ? 2 : split == kSplit ? 3 : 0; }
 
-int sq_tx(int s) { return s == 4 ? 0 : s == 8 ? 1 : s == 16 ? 2 : s == 32 ? 3 : 4; }
 int bdi_of(int bd) { return bd == 8 ? 0 : bd == 10 ? 1 : 2; }
 int qctx_of(int q) { return q <= 20 ? 0 : q <= 60 ? 1 : q <= 120 ? 2 : 3; }
 
-// luma_intra_syntax_bits (src/coder/superblock.rs:192) of mode id `id` on an s x s block.
-float mode_extra_bits(int actx, int lctx, int id, int s, int flags) {
+// luma_intra_syntax_bits (src/coder/superblock.rs:192) of mode id `id` on a w x h block.
+float mode_extra_bits(int actx, int lctx, int id, int w, int h, int flags) {
     const SymBits& T = sym_bits();
     const int base = id_base(id);
-    const int L = s == 64 ? 0 : s == 32 ? 1 : s == 16 ? 2 : s == 8 ? 3 : 4;
     float bits = T.kf[actx][lctx][base];
-    if (base >= 1 && base <= 8 && s >= 8) {
+    if (base >= 1 && base <= 8 && w >= 8 && h >= 8) {
         bits = bits + T.angle[base - 1][id_delta(id) + 3];
-    } else if ((flags & kFilter) && base == 0 && s <= 32) {
+    } else if ((flags & kFilter) && base == 0 && w <= 32 && h <= 32) {
         const int use = id >= 61;
-        bits = bits + T.fuse[4 - L][use];
+        bits = bits + T.fuse[bsize_index(w, h)][use];
         if (use) bits = bits + T.fmode[id - 61];
     }
     return bits;
 }
 
 // chroma_tx_for_mode (src/coder/chroma_tx.rs:46): ADST_ADST 3, ADST_DCT 1, DCT_ADST 2 (the
-// AV1 TxType numbering), DCT_DCT otherwise and at 32x32.
-int uv_txtp(int m, int cs) {
-    if (cs >= 32) return 0;
+// AV1 TxType numbering), DCT_DCT otherwise and at 32 and up, where the intra set has no other.
+int uv_txtp(int m, int w, int h) {
+    if (w >= 32 || h >= 32) return 0;
     if (m == 12 || m == 9 || m == 4) return 3;
     if (m == 10 || m == 1 || m == 5 || m == 8) return 1;
     if (m == 11 || m == 2 || m == 6 || m == 7) return 2;
@@ -171,8 +223,8 @@ int uv_txtp(int m, int cs) {
 }
 
 // uv_mode_bits (src/coder/lossy_state.rs:1302): the uv_mode symbol (CfL-allowed or no-CfL
-// CDF as AV1 codes it), the delta-0 angle symbol where angle deltas are coded (>= 8x8), the
-// CfL joint sign and alpha magnitudes.
+// CDF as AV1 codes it), the delta-0 angle symbol where angle deltas are coded, the CfL joint
+// sign and alpha magnitudes.
 float uv_bits(int ym, int m, int au, int av, bool cfl_allowed, bool angle_ok) {
     const SymBits& T = sym_bits();
     if (m == 13) {
@@ -187,12 +239,18 @@ float uv_bits(int ym, int m, int au, int av, bool cfl_allowed, bool angle_ok) {
     return b;
 }
 
-// A chroma block: its node, chroma origin and size, transform blocks, luma mode context,
-// whether CfL is allowed / angle deltas are coded, the luma block of its availability.
+// A chroma block: the node it is coded at (and which half of a rectangular node), chroma
+// origin and size, transform blocks, luma mode context, whether CfL is allowed / angle
+// deltas are coded, and its availability.
 struct UvB {
-    int node, cx, cy, cs, nt, ym;
-    bool cfl_ok, angle_ok;
-    int lx, ly, ls;
+    int node, half, cx, cy, cw, ch, nt, ym;
+    bool cfl_ok, angle_ok, tr, bl;
+};
+
+// A luma leaf in coding order.
+struct Leaf {
+    Blk b;
+    int id;  // mode id
 };
 
 struct Frame {
@@ -201,7 +259,8 @@ struct Frame {
     float lambda, wsse;
     std::vector<uint16_t> rec;     // W x H
     std::vector<uint8_t> mode4;    // (W/4) x (H/4) luma modes
-    std::vector<uint8_t> part8;    // (W/8) x (H/8) partition context bytes
+    std::vector<uint8_t> parta8;   // (W/8) x (H/8) above partition context bytes
+    std::vector<uint8_t> partl8;   // (W/8) x (H/8) left partition context bytes
     orc_plane planes[6];  // source, reconstruction; with chroma U source / reconstruction, V
     int nplanes = 2;
     int minsz;
@@ -209,7 +268,79 @@ struct Frame {
     std::vector<uint16_t> rec_uv[2];  // Wc x Hc
     std::vector<uint8_t> uvmode4;     // (Wc/4) x (Hc/4) uv modes
 
+    int mode_at(int x, int y) const { return mode4[(size_t)(y >> 2) * (W >> 2) + (x >> 2)]; }
+    int pa_at(int x, int y) const { return parta8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
+    int pl_at(int x, int y) const { return partl8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
     int uv_at(int cx, int cy) const { return uvmode4[(size_t)(cy >> 2) * (Wc >> 2) + (cx >> 2)]; }
+
+    bool rect_on() const { return (flags & kRect) != 0; }
+
+    // the luma candidate of mode id `id` on block b of the SB at (x0, y0)
+    orc_cand cand(const Blk& b, int id, int x0, int y0, bool final_ctx) const {
+        orc_cand c;
+        std::memset(&c, 0, sizeof(c));
+        c.x = (uint16_t)b.x;
+        c.y = (uint16_t)b.y;
+        c.tx_size = (uint8_t)tx_of(b.w, b.h);
+        c.tx_type = 0;
+        c.mode = (uint8_t)id_mode(id);
+        c.angle_delta = (int8_t)id_delta(id);
+        // neighbours: final maps, except inside the SB at decision time (DC, not smooth)
+        const bool above_ok = b.y > 0 && (final_ctx || b.y == y0);
+        const bool left_ok = b.x > 0 && (final_ctx || b.x == x0);
+        const int am = above_ok ? mode_at(b.x, b.y - 1) : 0, lm = left_ok ? mode_at(b.x - 1, b.y) : 0;
+        const bool smooth = (am >= 9 && am <= 11) || (lm >= 9 && lm <= 11);
+        c.pflags = (uint8_t)((b.tr ? 1 : 0) | (b.bl ? 2 : 0) | (smooth ? 4 : 0));
+        c.planes = 0 | (1 << 2);
+        c.qidx = (uint8_t)qidx;
+        c.bd_qm = (uint8_t)(qm | (bdi_of(bd) << 4));
+        c.rplane = 0;
+        c.eob_bin = (uint8_t)eob_luma(std::min(b.w, 32) * std::min(b.h, 32));
+        c.txtp_set = (uint8_t)txtp_set_of(b.w, b.h);
+        c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
+        c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT in the intra sets
+        c.qctx = (uint8_t)qctx_of(qidx);
+        c.wsse_scale = wsse;
+        c.lambda = lambda;
+        c.extra_bits = mode_extra_bits(kIntraModeCtx[am], kIntraModeCtx[lm], id, b.w, b.h, flags);
+        return c;
+    }
+
+    // partition symbol `sym` (0 NONE, 1 HORZ, 2 VERT, 3 SPLIT) of the level-L node at (x, y)
+    float part_bits(int x, int y, int L, int sym, int x0, int y0, bool final_ctx) const {
+        const int bl = L + 1, sh = 4 - bl;
+        const int a = (y > 0 && (final_ctx || y == y0)) ? pa_at(x, y - 1) : 0;
+        const int l = (x > 0 && (final_ctx || x == x0)) ? pl_at(x - 1, y) : 0;
+        const int ctx = ((a >> sh) & 1) + (((l >> sh) & 1) << 1);
+        return sym_bits().part[bl - 1][ctx][sym];
+    }
+
+    // the leaves of the SB at (x0, y0) in coding order, from its node records
+    std::vector<Leaf> leaves(const orc_rdo_node* nd, int x0, int y0) const {
+        std::vector<Leaf> out;
+        int stack[64], sp = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const int id = stack[--sp];
+            const int d = nd[id].split;
+            if (d == kUncoded) continue;
+            const Geo g = node_geo(id);
+            if (d == kSplit) {
+                for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
+                continue;
+            }
+            const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+            const bool ntr = have_tr(x - x0, y - y0, g.s), nbl = have_bl(x - x0, y - y0, g.s);
+            if (d == kNone) {
+                out.push_back({{x, y, g.s, g.s, ntr, nbl}, nd[id].mode});
+            } else {
+                const int k0 = d == kHorz ? 0 : 2;
+                out.push_back({rect_blk(x, y, g.s, k0, ntr, nbl), nd[id].rmode0});
+                out.push_back({rect_blk(x, y, g.s, k0 + 1, ntr, nbl), nd[id].rmode1});
+            }
+        }
+        return out;
+    }
 
     // the chroma blocks of the SB at (x0, y0) in coding order
     std::vector<UvB> uv_blocks(const orc_rdo_node* nd, int x0, int y0) const {
@@ -218,27 +349,53 @@ struct Frame {
         stack[sp++] = 0;
         while (sp) {
             const int id = stack[--sp];
-            if (nd[id].split == 2) continue;
+            const int d = nd[id].split;
+            if (d == kUncoded) continue;
             const Geo g = node_geo(id);
-            const bool group = ss && g.s == 8 && nd[id].split == 1;
-            if (nd[id].split == 1 && !group) {
+            const bool group = ss && g.s == 8 && d != kNone;  // 4:2:0, an 8x8 coded as smaller blocks
+            if (d == kSplit && !group) {
                 for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
                 continue;
             }
-            UvB b;
-            b.node = id;
-            b.cx = (x0 + g.ix * g.s) >> ss;
-            b.cy = (y0 + g.iy * g.s) >> ss;
-            b.cs = ss ? (g.s == 8 ? 4 : g.s / 2) : g.s;
-            b.nt = b.cs == 64 ? 4 : 1;
-            b.ym = id_base(nd[group ? node_id(4, 2 * g.ix + 1, 2 * g.iy + 1) : id].mode);
-            const int lbs = group ? 4 : g.s;
-            b.cfl_ok = lbs <= 32;
-            b.angle_ok = lbs >= 8;
-            b.lx = g.ix * g.s;
-            b.ly = g.iy * g.s;
-            b.ls = g.s;
-            out.push_back(b);
+            const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
+            const bool ntr = have_tr(x - x0, y - y0, g.s), nbl = have_bl(x - x0, y - y0, g.s);
+            if (d == kNone || group) {
+                UvB b;
+                b.node = id;
+                b.half = 0;
+                b.cx = x >> ss;
+                b.cy = y >> ss;
+                b.cw = b.ch = group ? 4 : g.s >> ss;
+                b.nt = b.cw == 64 ? 4 : 1;
+                // the chroma of a split 8x8 is coded with (and under the mode of) its last block
+                b.ym = !group ? id_base(nd[id].mode)
+                       : d == kSplit ? id_base(nd[node_id(4, 2 * g.ix + 1, 2 * g.iy + 1)].mode)
+                                     : id_base(nd[id].rmode1);
+                b.cfl_ok = group || g.s <= 32;
+                b.angle_ok = !group;
+                b.tr = ntr;
+                b.bl = nbl;
+                out.push_back(b);
+                continue;
+            }
+            const int k0 = d == kHorz ? 0 : 2;
+            for (int h = 0; h < 2; ++h) {
+                const Blk r = rect_blk(x, y, g.s, k0 + h, ntr, nbl);
+                UvB b;
+                b.node = id;
+                b.half = h;
+                b.cx = r.x >> ss;
+                b.cy = r.y >> ss;
+                b.cw = r.w >> ss;
+                b.ch = r.h >> ss;
+                b.nt = 1;
+                b.ym = id_base(h ? nd[id].rmode1 : nd[id].rmode0);
+                b.cfl_ok = true;
+                b.angle_ok = r.w >= 8 && r.h >= 8;
+                b.tr = r.tr;
+                b.bl = r.bl;
+                out.push_back(b);
+            }
         }
         return out;
     }
@@ -250,25 +407,25 @@ struct Frame {
     orc_cand uv_cand(const UvB& b, int m, int p, int t, int x0, int y0, bool fin, int alpha) const {
         orc_cand c;
         std::memset(&c, 0, sizeof(c));
-        const int ts = b.nt == 4 ? 32 : b.cs;
-        const int tx = b.cx + (t & 1) * ts, ty = b.cy + (t >> 1) * ts;
+        const int tw = b.nt == 4 ? 32 : b.cw, th = b.nt == 4 ? 32 : b.ch;
+        const int tx = b.cx + (t & 1) * tw, ty = b.cy + (t >> 1) * th;
         c.x = (uint16_t)tx;
         c.y = (uint16_t)ty;
-        c.tx_size = (uint8_t)sq_tx(ts);
-        c.tx_type = (uint8_t)(m == 13 ? 0 : uv_txtp(m, ts));
+        c.tx_size = (uint8_t)tx_of(tw, th);
+        c.tx_type = (uint8_t)(m == 13 ? 0 : uv_txtp(m, tw, th));
         c.mode = (uint8_t)m;
         if (m == 13) {
             c.pflags = (uint8_t)((ss ? 8 | 16 : 0) | (fin ? 32 : 0));
             c.angle_delta = (int8_t)(fin ? alpha : 0);
             c.txtp_mode = 1;  // the luma reconstruction plane
         } else {
-            const int ax = b.nt == 4 ? (t & 1) * 32 : b.lx, ay = b.nt == 4 ? (t >> 1) * 32 : b.ly;
-            const int as = b.nt == 4 ? 32 : b.ls;
+            const bool tr = b.nt == 4 ? have_tr((t & 1) * 32, (t >> 1) * 32, 32) : b.tr;
+            const bool bl = b.nt == 4 ? have_bl((t & 1) * 32, (t >> 1) * 32, 32) : b.bl;
             const bool above_ok = ty > 0 && (fin || ty == (y0 >> ss));
             const bool left_ok = tx > 0 && (fin || tx == (x0 >> ss));
             const int am = above_ok ? uv_at(tx, ty - 1) : 0, lm = left_ok ? uv_at(tx - 1, ty) : 0;
             const bool smooth = (am >= 9 && am <= 11) || (lm >= 9 && lm <= 11);
-            c.pflags = (uint8_t)((have_tr(ax, ay, as) ? 1 : 0) | (have_bl(ax, ay, as) ? 2 : 0) | (smooth ? 4 : 0));
+            c.pflags = (uint8_t)((tr ? 1 : 0) | (bl ? 2 : 0) | (smooth ? 4 : 0));
         }
         const int sp = 2 + 2 * p, rp = sp + 1;
         c.planes = (uint8_t)((sp & 3) | ((rp & 3) << 2));
@@ -278,7 +435,7 @@ struct Frame {
         c.dc_delta = (int8_t)uv_dc;
         c.ac_delta = (int8_t)uv_ac;
         c.rplane = 1;
-        c.eob_bin = (uint8_t)(ts == 4 ? 1 : ts == 8 ? 6 : ts == 16 ? 12 : 16);  // chroma eob_bin CDFs
+        c.eob_bin = (uint8_t)eob_chroma(tw * th);  // the chroma eob_bin CDFs
         c.skip_ctx = 7;
         c.qctx = (uint8_t)qctx_of(qidx);
         c.wsse_scale = 0.0f;
@@ -286,60 +443,49 @@ struct Frame {
         return c;
     }
 
-    int mode_at(int x, int y) const { return mode4[(size_t)(y >> 2) * (W >> 2) + (x >> 2)]; }
-    int part_at(int x, int y) const { return part8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
-
-    orc_cand cand(int x, int y, int s, int id, int sb_x0, int sb_y0, bool final_ctx) const {
-        orc_cand c;
-        std::memset(&c, 0, sizeof(c));
-        c.x = (uint16_t)x;
-        c.y = (uint16_t)y;
-        c.tx_size = (uint8_t)sq_tx(s);
-        c.tx_type = 0;
-        c.mode = (uint8_t)id_mode(id);
-        c.angle_delta = (int8_t)id_delta(id);
-        // neighbours: final maps, except inside the SB at decision time (DC, not smooth)
-        const bool above_ok = y > 0 && (final_ctx || y == sb_y0);
-        const bool left_ok = x > 0 && (final_ctx || x == sb_x0);
-        const int am = above_ok ? mode_at(x, y - 1) : 0, lm = left_ok ? mode_at(x - 1, y) : 0;
-        const bool smooth = (am >= 9 && am <= 11) || (lm >= 9 && lm <= 11);
-        c.pflags = (uint8_t)((have_tr(x - sb_x0, y - sb_y0, s) ? 1 : 0) |
-                             (have_bl(x - sb_x0, y - sb_y0, s) ? 2 : 0) | (smooth ? 4 : 0));
-        c.planes = 0 | (1 << 2);
-        c.qidx = (uint8_t)qidx;
-        c.bd_qm = (uint8_t)(qm | (bdi_of(bd) << 4));
-        c.rplane = 0;
-        c.eob_bin = (uint8_t)(s == 4 ? 0 : s == 8 ? 5 : s == 16 ? 11 : 15);
-        c.txtp_set = (uint8_t)(s == 4 ? 1 : s == 8 ? 2 : s == 16 ? 3 : 0);
-        c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
-        c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT in the intra sets
-        c.qctx = (uint8_t)qctx_of(qidx);
-        c.wsse_scale = wsse;
-        c.lambda = lambda;
-        c.extra_bits = mode_extra_bits(kIntraModeCtx[am], kIntraModeCtx[lm], id, s, flags);
-        return c;
-    }
-
-    float part_bits(int x, int y, int L, int sym, int sb_x0, int sb_y0, bool final_ctx) const {
-        const int bl = L + 1, sh = 4 - bl;
-        const int a = (y > 0 && (final_ctx || y == sb_y0)) ? part_at(x, y - 1) : 0;
-        const int l = (x > 0 && (final_ctx || x == sb_x0)) ? part_at(x - 1, y) : 0;
-        const int ctx = ((a >> sh) & 1) + (((l >> sh) & 1) << 1);
-        return sym_bits().part[bl - 1][ctx][sym];
+    void put_leaf(const Leaf& lf) {
+        const Blk& b = lf.b;
+        const int base = id_base(lf.id);
+        for (int yy = b.y; yy < b.y + b.h; yy += 4)
+            for (int xx = b.x; xx < b.x + b.w; xx += 4) mode4[(size_t)(yy >> 2) * (W >> 2) + (xx >> 2)] = (uint8_t)base;
+        for (int yy = b.y; yy < b.y + b.h; yy += 4)
+            for (int xx = b.x; xx < b.x + b.w; xx += 4) {
+                const size_t u = (size_t)(yy >> 3) * (W >> 3) + (xx >> 3);
+                parta8[u] = part_byte(b.w);
+                partl8[u] = part_byte(b.h);
+            }
     }
 };
 
 struct SbOut {
     int64_t candidates = 0;
-    std::vector<int> leaves;  // node ids in coding order
+    int32_t leaves = 0;
 };
+
+// best candidate of block b over the mode ids allowed by mask m
+void best_of(const Frame& f, const Blk& b, uint16_t m, int x0, int y0, float& best, int& best_id, SbOut& out) {
+    best = __builtin_inff();
+    best_id = 0;
+    for (int mid = 0; mid < kModeIds; ++mid) {
+        if (!id_valid(mid, b.w, b.h, f.flags)) continue;
+        if (mid != 0 && !((m >> id_bit(mid)) & 1)) continue;
+        const orc_cand cd = f.cand(b, mid, x0, y0, false);
+        orc_candres res;
+        orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
+        ++out.candidates;
+        if (res.cost < best) {
+            best = res.cost;
+            best_id = mid;
+        }
+    }
+}
 
 // Decide and reconstruct one superblock (its neighbours of earlier steps are final).
 void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, SbOut& out) {
     const int x0 = c * 64, y0 = r * 64;
-    bool searched[kNodes], inside[kNodes], none_ok[kNodes], may_split[kNodes];
-    float best[kNodes];
-    int best_id[kNodes];
+    bool searched[kNodes], inside[kNodes], none_ok[kNodes], may_split[kNodes], rect_ok[kNodes];
+    float best[kNodes], rbest[kNodes][4];
+    int best_id[kNodes], rbest_id[kNodes][4];
     for (int id = 0; id < kNodes; ++id) {
         const Geo g = node_geo(id);
         const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
@@ -355,25 +501,23 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         inside[id] = x + g.s <= f.W && y + g.s <= f.H;
         none_ok[id] = sr && inside[id] && (g.s == f.minsz || !(m & kForceSplit) || (m & kNoSplit));
         may_split[id] = sr && g.s > f.minsz && (!inside[id] || !(m & kNoSplit));
+        // the halves of HORZ / VERT: 32x32 .. 8x8 nodes whose halves are not below the minimum
+        rect_ok[id] = f.rect_on() && sr && inside[id] && g.L >= 1 && g.L <= 3 && g.s / 2 >= f.minsz &&
+                      !(m & kForceSplit);
         best[id] = __builtin_inff();
         best_id[id] = 0;
-        if (!none_ok[id]) continue;
-        for (int mid = 0; mid < kModeIds; ++mid) {
-            if (!id_valid(mid, g.s, f.flags)) continue;
-            if (mid != 0 && !((m >> id_bit(mid)) & 1)) continue;
-            const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, false);
-            orc_candres res;
-            orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
-            ++out.candidates;
-            if (res.cost < best[id]) {
-                best[id] = res.cost;
-                best_id[id] = mid;
-            }
+        for (int k = 0; k < 4; ++k) {
+            rbest[id][k] = __builtin_inff();
+            rbest_id[id][k] = 0;
         }
+        const bool ntr = have_tr(x - x0, y - y0, g.s), nbl = have_bl(x - x0, y - y0, g.s);
+        if (none_ok[id]) best_of(f, {x, y, g.s, g.s, ntr, nbl}, m, x0, y0, best[id], best_id[id], out);
+        if (rect_ok[id])
+            for (int k = 0; k < 4; ++k)
+                best_of(f, rect_blk(x, y, g.s, k, ntr, nbl), m, x0, y0, rbest[id][k], rbest_id[id][k], out);
     }
     // bottom-up decision
     float cost[kNodes];
-    uint8_t split[kNodes];
     for (int id = kNodes - 1; id >= 0; --id) {
         const Geo g = node_geo(id);
         const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
@@ -381,60 +525,67 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         std::memset(&nd, 0, sizeof(nd));
         nd.mode = (uint8_t)best_id[id];
         nd.uv_mode = 255;
+        nd.uv_mode1 = 255;
         if (!searched[id]) {
             cost[id] = 0.0f;
-            split[id] = 2;
             nd.cost = __builtin_inff();
             nd.none_cost = __builtin_inff();
-            nd.split = 2;
+            nd.split = kUncoded;
             continue;
         }
-        float none = __builtin_inff(), spl = __builtin_inff();
-        if (none_ok[id] && best[id] < __builtin_inff()) {
+        const float inf = __builtin_inff();
+        float none = inf, horz = inf, vert = inf, spl = inf;
+        if (none_ok[id] && best[id] < inf) {
             none = best[id];
             if (g.s >= 8) none = none + f.lambda * f.part_bits(x, y, g.L, 0, x0, y0, false);
+        }
+        if (rect_ok[id]) {
+            if (rbest[id][0] < inf && rbest[id][1] < inf)
+                horz = (rbest[id][0] + rbest[id][1]) + f.lambda * f.part_bits(x, y, g.L, 1, x0, y0, false);
+            if (rbest[id][2] < inf && rbest[id][3] < inf)
+                vert = (rbest[id][2] + rbest[id][3]) + f.lambda * f.part_bits(x, y, g.L, 2, x0, y0, false);
         }
         if (may_split[id]) {
             float sum = 0.0f;
             for (int k = 0; k < 4; ++k) sum = sum + cost[node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1))];
-            spl = inside[id] ? sum + f.lambda * f.part_bits(x, y, g.L, 1, x0, y0, false) : sum;
+            spl = inside[id] ? sum + f.lambda * f.part_bits(x, y, g.L, 3, x0, y0, false) : sum;
         }
-        if (none <= spl) {
-            cost[id] = none;
-            split[id] = 0;
-        } else {
-            cost[id] = spl;
-            split[id] = 1;
+        float bc = none;
+        int d = kNone;
+        if (horz < bc) {
+            bc = horz;
+            d = kHorz;
         }
-        nd.cost = cost[id];
+        if (vert < bc) {
+            bc = vert;
+            d = kVert;
+        }
+        if (spl < bc) {
+            bc = spl;
+            d = kSplit;
+        }
+        cost[id] = bc;
+        nd.cost = bc;
         nd.none_cost = none;
-        nd.split = split[id];
-    }
-    // leaves in coding order; maps; reconstruction
-    int stack[64], sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-        const int id = stack[--sp];
-        if (split[id] == 2) continue;
-        const Geo g = node_geo(id);
-        if (split[id] == 1) {
-            for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
-            continue;
+        nd.split = (uint8_t)d;
+        if (d == kHorz || d == kVert) {
+            const int k0 = d == kHorz ? 0 : 2;
+            nd.rmode0 = (uint8_t)rbest_id[id][k0];
+            nd.rmode1 = (uint8_t)rbest_id[id][k0 + 1];
         }
-        out.leaves.push_back(id);
-        const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
-        const int mid = best_id[id];
-        const int base = id_base(mid);
-        for (int yy = y; yy < y + g.s; yy += 4)
-            for (int xx = x; xx < x + g.s; xx += 4) f.mode4[(size_t)(yy >> 2) * (f.W >> 2) + (xx >> 2)] = (uint8_t)base;
-        for (int yy = y; yy < y + g.s; yy += 8)
-            for (int xx = x; xx < x + g.s; xx += 8) f.part8[(size_t)(yy >> 3) * (f.W >> 3) + (xx >> 3)] = kPartByte[g.L];
-        const orc_cand cd = f.cand(x, y, g.s, mid, x0, y0, true);
+    }
+    // leaves in coding order: maps, reconstruction
+    const std::vector<Leaf> lv = f.leaves(nodes, x0, y0);
+    out.leaves = (int32_t)lv.size();
+    for (const Leaf& lf : lv) {
+        f.put_leaf(lf);
+        const orc_cand cd = f.cand(lf.b, lf.id, x0, y0, true);
         orc_candres res;
-        std::vector<int32_t> rc((size_t)g.s * g.s);
+        std::vector<int32_t> rc((size_t)lf.b.w * lf.b.h);
         orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
-        for (int yy = 0; yy < g.s; ++yy)
-            for (int xx = 0; xx < g.s; ++xx) f.rec[(size_t)(y + yy) * f.W + x + xx] = (uint16_t)rc[yy * g.s + xx];
+        for (int yy = 0; yy < lf.b.h; ++yy)
+            for (int xx = 0; xx < lf.b.w; ++xx)
+                f.rec[(size_t)(lf.b.y + yy) * f.W + lf.b.x + xx] = (uint16_t)rc[yy * lf.b.w + xx];
     }
     if (!f.chroma) return;
     // chroma decisions (decision-time contexts), the uv map, then the reconstruction
@@ -469,26 +620,33 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         um[i] = bm;
         ua[i] = ba;
         uav[i] = bv;
-        nodes[b.node].uv_mode = (uint8_t)bm;
-        nodes[b.node].cfl_alpha_u = (int8_t)ba;
-        nodes[b.node].cfl_alpha_v = (int8_t)bv;
-        for (int yy = 0; yy < b.cs; yy += 4)
-            for (int xx = 0; xx < b.cs; xx += 4)
+        orc_rdo_node& nd = nodes[b.node];
+        if (b.half) {
+            nd.uv_mode1 = (uint8_t)bm;
+            nd.cfl_alpha_u1 = (int8_t)ba;
+            nd.cfl_alpha_v1 = (int8_t)bv;
+        } else {
+            nd.uv_mode = (uint8_t)bm;
+            nd.cfl_alpha_u = (int8_t)ba;
+            nd.cfl_alpha_v = (int8_t)bv;
+        }
+        for (int yy = 0; yy < b.ch; yy += 4)
+            for (int xx = 0; xx < b.cw; xx += 4)
                 f.uvmode4[(size_t)((b.cy + yy) >> 2) * (f.Wc >> 2) + ((b.cx + xx) >> 2)] = (uint8_t)bm;
     }
     for (size_t i = 0; i < blocks.size(); ++i) {
         const UvB& b = blocks[i];
-        const int ts = b.nt == 4 ? 32 : b.cs;
+        const int tw = b.nt == 4 ? 32 : b.cw, th = b.nt == 4 ? 32 : b.ch;
         for (int pl = 0; pl < 2; ++pl)
             for (int t = 0; t < b.nt; ++t) {
                 const orc_cand cd = f.uv_cand(b, um[i], pl, t, x0, y0, true, pl ? uav[i] : ua[i]);
                 orc_candres res;
-                std::vector<int32_t> rc((size_t)ts * ts);
+                std::vector<int32_t> rc((size_t)tw * th);
                 orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
                 std::vector<uint16_t>& R = f.rec_uv[pl];
-                for (int yy = 0; yy < ts; ++yy)
-                    for (int xx = 0; xx < ts; ++xx)
-                        R[(size_t)(cd.y + yy) * f.Wc + cd.x + xx] = (uint16_t)rc[yy * ts + xx];
+                for (int yy = 0; yy < th; ++yy)
+                    for (int xx = 0; xx < tw; ++xx)
+                        R[(size_t)(cd.y + yy) * f.Wc + cd.x + xx] = (uint16_t)rc[yy * tw + xx];
             }
     }
 }
@@ -497,6 +655,10 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
 
 extern "C" int orc_rdo_have_tr(int xin, int yin, int s) { return have_tr(xin, yin, s) ? 1 : 0; }
 extern "C" int orc_rdo_have_bl(int xin, int yin, int s) { return have_bl(xin, yin, s) ? 1 : 0; }
+extern "C" int orc_rdo_rect_avail(int xin, int yin, int s, int k) {
+    const Blk b = rect_blk(xin, yin, s, k, have_tr(xin, yin, s), have_bl(xin, yin, s));
+    return (b.tr ? 1 : 0) | (b.bl ? 2 : 0);
+}
 
 extern "C" float orc_mode_lambda(int qidx, int bd, int sub) {
     // mode_lambda_q (src/cost.rs:285) on the luma DC step of Quant::new (src/quant.rs:350)
@@ -563,7 +725,8 @@ extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint
     f.rec.resize((size_t)f.W * f.H);
     for (int y = 0; y < f.H; ++y) std::memcpy(&f.rec[(size_t)y * f.W], src + (size_t)y * src_stride, f.W * 2);
     f.mode4.assign((size_t)(f.W / 4) * (f.H / 4), 0);
-    f.part8.assign((size_t)(f.W / 8) * (f.H / 8), 0);
+    f.parta8.assign((size_t)(f.W / 8) * (f.H / 8), 0);
+    f.partl8.assign((size_t)(f.W / 8) * (f.H / 8), 0);
     f.planes[0] = {src, src_stride, f.W, f.H, f.W, f.H, 0};
     f.planes[1] = {f.rec.data(), f.W, f.W, f.H, f.W, f.H, 0};
     f.chroma = p->chroma;
@@ -625,33 +788,49 @@ extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint
         int64_t e = 0;
         int stack[64], sp = 0;
         stack[sp++] = 0;
+        // per coded node in coding order: its partition symbol, then its leaf or halves
         while (sp) {
             const int id = stack[--sp];
-            if (nd[id].split == 2) continue;
+            const int d = nd[id].split;
+            if (d == kUncoded) continue;
             const Geo g = node_geo(id);
             const int x = x0 + g.ix * g.s, y = y0 + g.iy * g.s;
             const bool in = x + g.s <= f.W && y + g.s <= f.H;
-            if (g.s >= 8 && in) b += (double)f.part_bits(x, y, g.L, nd[id].split, x0, y0, true);
-            if (nd[id].split == 1) {
+            if (g.s >= 8 && in) b += (double)f.part_bits(x, y, g.L, part_sym(d), x0, y0, true);
+            if (d == kSplit) {
                 for (int k = 3; k >= 0; --k) stack[sp++] = node_id(g.L + 1, 2 * g.ix + (k & 1), 2 * g.iy + (k >> 1));
                 continue;
             }
-            const orc_cand cd = f.cand(x, y, g.s, nd[id].mode, x0, y0, true);
-            orc_candres res;
-            orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
-            b += (double)res.bits;
-            e += res.sse;
+            const bool ntr = have_tr(x - x0, y - y0, g.s), nbl = have_bl(x - x0, y - y0, g.s);
+            Leaf lf[2];
+            int nl = 1;
+            if (d == kNone) {
+                lf[0] = {{x, y, g.s, g.s, ntr, nbl}, nd[id].mode};
+            } else {
+                const int k0 = d == kHorz ? 0 : 2;
+                lf[0] = {rect_blk(x, y, g.s, k0, ntr, nbl), nd[id].rmode0};
+                lf[1] = {rect_blk(x, y, g.s, k0 + 1, ntr, nbl), nd[id].rmode1};
+                nl = 2;
+            }
+            for (int i = 0; i < nl; ++i) {
+                const orc_cand cd = f.cand(lf[i].b, lf[i].id, x0, y0, true);
+                orc_candres res;
+                orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
+                b += (double)res.bits;
+                e += res.sse;
+            }
         }
         double buv = 0.0;
         int64_t euv[2] = {0, 0};
         if (f.chroma) {
             for (const UvB& ub : f.uv_blocks(nd, x0, y0)) {
                 const orc_rdo_node& o = nd[ub.node];
+                const int um = ub.half ? o.uv_mode1 : o.uv_mode;
+                const int au = ub.half ? o.cfl_alpha_u1 : o.cfl_alpha_u, av = ub.half ? o.cfl_alpha_v1 : o.cfl_alpha_v;
                 for (int pl = 0; pl < 2; ++pl)
                     for (int t = 0; t < ub.nt; ++t) {
-                        orc_cand cd = f.uv_cand(ub, o.uv_mode, pl, t, x0, y0, true, pl ? o.cfl_alpha_v : o.cfl_alpha_u);
-                        if (pl == 0 && t == 0)
-                            cd.extra_bits = uv_bits(ub.ym, o.uv_mode, o.cfl_alpha_u, o.cfl_alpha_v, ub.cfl_ok, ub.angle_ok);
+                        orc_cand cd = f.uv_cand(ub, um, pl, t, x0, y0, true, pl ? av : au);
+                        if (pl == 0 && t == 0) cd.extra_bits = uv_bits(ub.ym, um, au, av, ub.cfl_ok, ub.angle_ok);
                         orc_candres res;
                         orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
                         b += (double)res.bits;
@@ -684,7 +863,7 @@ extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint
         sse_u += sb_sse_u[sb];
         sse_v += sb_sse_v[sb];
         cands += outs[sb].candidates;
-        leaves += (int32_t)outs[sb].leaves.size();
+        leaves += outs[sb].leaves;
     }
     std::memcpy(recon, f.rec.data(), f.rec.size() * 2);
     if (f.chroma) {

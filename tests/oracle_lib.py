@@ -312,8 +312,9 @@ def eval_batch(cands, planes, nthreads=8):
 
 # ---- whole-frame luma intra RDO (oracle/oracle_rdo.cpp) ---------------------------------
 RDO_NODE_DTYPE = np.dtype([("cost", "<f4"), ("none_cost", "<f4"), ("split", "u1"), ("mode", "u1"),
-                           ("uv_mode", "u1"), ("pad0", "u1"), ("cfl_alpha_u", "i1"), ("cfl_alpha_v", "i1"),
-                           ("pad1", "u1", (2,))])
+                           ("rmode0", "u1"), ("rmode1", "u1"), ("uv_mode", "u1"), ("uv_mode1", "u1"),
+                           ("cfl_alpha_u", "i1"), ("cfl_alpha_v", "i1"), ("cfl_alpha_u1", "i1"),
+                           ("cfl_alpha_v1", "i1"), ("pad", "u1", (2,))])
 
 
 class OrcRdoParams(ctypes.Structure):

@@ -20,8 +20,10 @@ import oracle_lib as O
 from maroontree_amd import workload as W
 from test_oracle_rdo import brute_avail, geo, nid, mode_of, base_of
 
-ANGLE, FILTER, MIN4, CFL = 1, 2, 4, 8
+ANGLE, FILTER, MIN4, CFL, RECT = 1, 2, 4, 8, 16
 TX = {4: 0, 8: 1, 16: 2, 32: 3}
+TXR = {(4, 4): 0, (8, 8): 1, (16, 16): 2, (32, 32): 3, (4, 8): 5, (8, 4): 6, (8, 16): 7, (16, 8): 8,
+       (16, 32): 9, (32, 16): 10}
 # chroma_tx_for_mode (src/coder/chroma_tx.rs:46) in the AV1 TxType numbering
 UV_TXTP = {0: 0, 1: 1, 2: 2, 3: 0, 4: 3, 5: 1, 6: 2, 7: 2, 8: 1, 9: 3, 10: 1, 11: 2, 12: 3}
 
@@ -55,35 +57,49 @@ def chroma_frame(seed, w, h, bd, fmt):
 
 
 def uv_blocks(nodes_sb, x0, y0, ss):
-    """Chroma blocks of one SB in coding order: (node, cx, cy, cs, nt, luma mode, cfl_ok,
-    angle_ok, lx, ly, ls)."""
+    """Chroma blocks of one SB in coding order: (node, half, cx, cy, cw, ch, nt, luma mode,
+    cfl_ok, angle_ok, (tr, bl))."""
+    from test_oracle_rdo_rect import halves, rect_avail
     out, stack = [], [0]
     while stack:
         n = stack.pop()
-        sp = nodes_sb[n]["split"]
+        sp = int(nodes_sb[n]["split"])
         if sp == 2:
             continue
         L, ix, iy, s = geo(n)
-        group = ss and s == 8 and sp == 1
+        group = ss and s == 8 and sp != 0
         if sp == 1 and not group:
             for q in (3, 2, 1, 0):
                 stack.append(nid(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1)))
             continue
-        cs = (4 if s == 8 else s // 2) if ss else s
-        ym = base_of(int(nodes_sb[nid(4, 2 * ix + 1, 2 * iy + 1) if group else n]["mode"]))
-        lbs = 4 if group else s
-        out.append((n, (x0 + ix * s) >> ss, (y0 + iy * s) >> ss, cs, 4 if cs == 64 else 1, ym, lbs <= 32,
-                    lbs >= 8, ix * s, iy * s, s))
+        x, y = ix * s, iy * s
+        if sp == 0 or group:
+            c = 4 if group else s >> ss
+            if not group:
+                ym = base_of(int(nodes_sb[n]["mode"]))
+            elif sp == 1:
+                ym = base_of(int(nodes_sb[nid(4, 2 * ix + 1, 2 * iy + 1)]["mode"]))
+            else:
+                ym = base_of(int(nodes_sb[n]["rmode1"]))
+            out.append((n, 0, (x0 + x) >> ss, (y0 + y) >> ss, c, c, 4 if c == 64 else 1, ym, group or s <= 32,
+                        not group, brute_avail(x, y, s)))
+            continue
+        k0 = 0 if sp == 3 else 2
+        for j in range(2):
+            bx, by, bw, bh = halves(x, y, s, k0 + j)
+            ym = base_of(int(nodes_sb[n]["rmode0" if j == 0 else "rmode1"]))
+            out.append((n, j, (x0 + bx) >> ss, (y0 + by) >> ss, bw >> ss, bh >> ss, 1, ym, True, bw >= 8 and bh >= 8,
+                        rect_avail(x, y, s, k0 + j)))
     return out
 
 
-def cfl_pred(dc, luma, lx, ly, cs, ss, alpha, bd):
+def cfl_pred(dc, luma, lx, ly, cw, ch, ss, alpha, bd):
     L = O.plib()
-    lum = np.ascontiguousarray(luma[ly:ly + (cs << ss), lx:lx + (cs << ss)], dtype=np.uint16)
-    ac = np.zeros(cs * cs, np.int32)
-    L.orc_cfl_ac_sub(lum.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), lum.shape[1], cs, cs, ss, ss,
+    lum = np.ascontiguousarray(luma[ly:ly + (ch << ss), lx:lx + (cw << ss)], dtype=np.uint16)
+    ac = np.zeros(cw * ch, np.int32)
+    L.orc_cfl_ac_sub(lum.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), lum.shape[1], cw, ch, ss, ss,
                      ac.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
-    return np.array([L.orc_cfl_pred_pixel(int(dc), int(a), int(alpha), bd) for a in ac], np.int32).reshape(cs, cs)
+    return np.array([L.orc_cfl_pred_pixel(int(dc), int(a), int(alpha), bd) for a in ac], np.int32).reshape(ch, cw)
 
 
 def replay_chroma(planes_src, luma_rec, nodes, W_, H_, ss, bd, qidx, uvq, min4):
@@ -99,42 +115,42 @@ def replay_chroma(planes_src, luma_rec, nodes, W_, H_, ss, bd, qidx, uvq, min4):
         r, c = divmod(sb, C)
         x0, y0 = 64 * c, 64 * r
         blocks = uv_blocks(nodes[sb], x0, y0, ss)
-        for (n, cx, cy, cs, nt, ym, cfl_ok, ang_ok, lx, ly, ls) in blocks:
-            m = int(nodes[sb][n]["uv_mode"])
+        for (n, half, cx, cy, cw, ch, nt, ym, cfl_ok, ang_ok, av) in blocks:
+            m = int(nodes[sb][n]["uv_mode1" if half else "uv_mode"])
             assert m <= 13 and (m < 13 or cfl_ok)
-            uvmap[cy // 4:(cy + cs) // 4, cx // 4:(cx + cs) // 4] = m
-        for (n, cx, cy, cs, nt, ym, cfl_ok, ang_ok, lx, ly, ls) in blocks:
-            m = int(nodes[sb][n]["uv_mode"])
-            alphas = (int(nodes[sb][n]["cfl_alpha_u"]), int(nodes[sb][n]["cfl_alpha_v"]))
-            ts = 32 if nt == 4 else cs
+            uvmap[cy // 4:(cy + ch) // 4, cx // 4:(cx + cw) // 4] = m
+        for (n, half, cx, cy, cw, ch, nt, ym, cfl_ok, ang_ok, av) in blocks:
+            nd = nodes[sb][n]
+            m = int(nd["uv_mode1" if half else "uv_mode"])
+            alphas = ((int(nd["cfl_alpha_u1"]), int(nd["cfl_alpha_v1"])) if half else
+                      (int(nd["cfl_alpha_u"]), int(nd["cfl_alpha_v"])))
+            tw, th = (32, 32) if nt == 4 else (cw, ch)
             for p in range(2):
                 rec, src = recs[p], planes_src[p]
                 for t in range(nt):
-                    tx, ty = cx + (t & 1) * ts, cy + (t >> 1) * ts
-                    if nt == 4:
-                        tr, bl = brute_avail((t & 1) * 32, (t >> 1) * 32, 32)
-                    else:
-                        tr, bl = brute_avail(lx, ly, ls)
+                    tx, ty = cx + (t & 1) * tw, cy + (t >> 1) * th
+                    tr, bl = brute_avail((t & 1) * 32, (t >> 1) * 32, 32) if nt == 4 else av
                     am = uvmap[(ty - 1) // 4, tx // 4] if ty > 0 else 0
                     lm = uvmap[ty // 4, (tx - 1) // 4] if tx > 0 else 0
                     smooth = am in (9, 10, 11) or lm in (9, 10, 11)
                     if m == 13:
-                        dc = O.intra_pred(0, 0, rec, tx, ty, ts, ts, False, False, Wc, Hc, False, bd)[0, 0]
-                        pred = cfl_pred(dc, luma_rec, tx << ss, ty << ss, ts, ss, alphas[p], bd)
+                        dc = O.intra_pred(0, 0, rec, tx, ty, tw, th, False, False, Wc, Hc, False, bd)[0, 0]
+                        pred = cfl_pred(dc, luma_rec, tx << ss, ty << ss, tw, th, ss, alphas[p], bd)
                         typ = 0
                     else:
-                        pred = O.intra_pred(m, 0, rec, tx, ty, ts, ts, tr, bl, Wc, Hc, smooth, bd)
-                        typ = UV_TXTP[m] if ts < 32 else 0
-                    resid = src[ty:ty + ts, tx:tx + ts].astype(np.int32) - pred
-                    lv, _ = O.fwd(TX[ts], typ, resid.ravel(), q)
-                    inv = O.inv(TX[ts], typ, lv, q).reshape(ts, ts)
-                    rec[ty:ty + ts, tx:tx + ts] = np.clip(pred + inv, 0, mx)
+                        pred = O.intra_pred(m, 0, rec, tx, ty, tw, th, tr, bl, Wc, Hc, smooth, bd)
+                        typ = UV_TXTP[m] if max(tw, th) < 32 else 0
+                    resid = src[ty:ty + th, tx:tx + tw].astype(np.int32) - pred
+                    lv, _ = O.fwd(TXR[(tw, th)], typ, resid.ravel(), q)
+                    inv = O.inv(TXR[(tw, th)], typ, lv, q).reshape(th, tw)
+                    rec[ty:ty + th, tx:tx + tw] = np.clip(pred + inv, 0, mx)
     return recs
 
 
 @pytest.mark.parametrize("fmt,bd,qidx,w,h,flags", [
     (1, 10, 91, 136, 72, ANGLE | FILTER | MIN4 | CFL), (1, 8, 180, 128, 64, ANGLE | CFL),
-    (2, 8, 53, 128, 72, MIN4 | CFL), (2, 12, 30, 72, 64, ANGLE)])
+    (2, 8, 53, 128, 72, MIN4 | CFL), (2, 12, 30, 72, 64, ANGLE),
+    (1, 10, 60, 136, 72, ANGLE | FILTER | MIN4 | CFL | RECT), (2, 8, 91, 128, 72, ANGLE | MIN4 | CFL | RECT)])
 def test_chroma_reconstruction_replays_from_the_decisions(fmt, bd, qidx, w, h, flags):
     y, u, v, ss = chroma_frame(31 + qidx, w, h, bd, fmt)
     qm = mt.qm_level_law(qidx, 2 if ss else 0)
@@ -142,6 +158,8 @@ def test_chroma_reconstruction_replays_from_the_decisions(fmt, bd, qidx, w, h, f
     uvq = O.chroma_laws(qidx, 2 if ss else 0)
     rec, ru, rv, nodes, st = O.intra_rdo_yuv(y, u, v, bd, qidx, qm, flags, lam, fmt, *uvq)
     rec_l, nodes_l, _, st_l = O.intra_rdo(y, bd, qidx, qm, flags & ~CFL, lam)
+    if flags & RECT:
+        assert np.isin(nodes["split"], (3, 4)).any()
     assert np.array_equal(rec, rec_l)  # chroma does not change the luma decisions
     assert np.array_equal(nodes["mode"], nodes_l["mode"]) and np.array_equal(nodes["split"], nodes_l["split"])
     pu, pv = replay_chroma([u, v], rec, nodes, w, h, ss, bd, qidx, uvq, flags & MIN4)
@@ -151,6 +169,8 @@ def test_chroma_reconstruction_replays_from_the_decisions(fmt, bd, qidx, w, h, f
     assert st["bits_uv"] > 0 and st["bits"] > st["bits_uv"]
     coded = nodes["uv_mode"] != 255
     assert coded.any()
+    if flags & RECT:
+        assert (nodes["uv_mode1"] != 255).any()  # chroma of second rectangular halves
     if flags & CFL:
         assert (nodes["uv_mode"] == 13).any()  # CfL wins somewhere on these images
     else:
@@ -198,11 +218,11 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
     uvmap = np.zeros((16, 16), np.int64)  # final uv modes of the other SBs
     for s2 in range(3):
         r, c = divmod(s2, 2)
-        for (n, cx, cy, cs, *_rest) in uv_blocks(nodes[s2], 64 * c, 64 * r, ss):
-            uvmap[cy // 4:(cy + cs) // 4, cx // 4:(cx + cs) // 4] = int(nodes[s2][n]["uv_mode"])
+        for (n, half, cx, cy, cw, ch, *_rest) in uv_blocks(nodes[s2], 64 * c, 64 * r, ss):
+            uvmap[cy // 4:(cy + ch) // 4, cx // 4:(cx + cw) // 4] = int(nodes[s2][n]["uv_mode1" if half else "uv_mode"])
     checked = 0
-    for (n, cx, cy, cs, nt, ym, cfl_ok, ang_ok, lx, ly, ls) in uv_blocks(nodes[sb], x0, y0, ss):
-        tr, bl = brute_avail(lx, ly, ls)
+    for (n, half, cx, cy, cw, ch, nt, ym, cfl_ok, ang_ok, (tr, bl)) in uv_blocks(nodes[sb], x0, y0, ss):
+        cs = cw
         am = uvmap[(cy - 1) // 4, cx // 4] if cy == 32 else 0
         lm = uvmap[cy // 4, (cx - 1) // 4] if cx == 32 else 0
         smooth = am in (9, 10, 11) or lm in (9, 10, 11)
