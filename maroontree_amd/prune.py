@@ -138,14 +138,24 @@ def labels_from_nodes(nodes, width, height):
         x = (sb % C) * 64 + ix * s
         y = (sb // C) * 64 + iy * s
         inside = (x + s <= width) & (y + s <= height)
-        coded = split[:, n] < 2
+        coded = split[:, n] != R.UNCODED
         m[:, n] = (inside & coded).astype(np.float32)
-        t[:, n] = (split[:, n] == 1).astype(np.float32)
-    leaf = split == 0
-    mode_id = nodes["mode"].astype(np.int64)
-    base = np.where(mode_id <= 12, mode_id, np.where(mode_id <= 60, 1 + (mode_id - 13) // 6, 0))
+        t[:, n] = (split[:, n] == R.SPLIT).astype(np.float32)
+
+    def base(mode_id):
+        mode_id = mode_id.astype(np.int64)
+        return np.where(mode_id <= 12, mode_id, np.where(mode_id <= 60, 1 + (mode_id - 13) // 6, 0))
+    leaf = split == R.NONE
+    b0 = base(nodes["mode"])
+    has_rect = "rmode0" in nodes.dtype.names  # records with the PARTITION_HORZ / VERT halves' modes
+    if has_rect:
+        rect = (split == R.HORZ) | (split == R.VERT)
+        r0, r1 = base(nodes["rmode0"]), base(nodes["rmode1"])
     for k in range(13):
-        t[:, SPLIT_NODES + k] = (leaf & (base == k)).any(axis=1).astype(np.float32)
+        used = leaf & (b0 == k)
+        if has_rect:
+            used |= rect & ((r0 == k) | (r1 == k))
+        t[:, SPLIT_NODES + k] = used.any(axis=1).astype(np.float32)
     m[:, SPLIT_NODES:] = 1.0
     return t, m
 

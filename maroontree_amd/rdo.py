@@ -14,6 +14,7 @@ from . import _lib
 from ._lib import MtPlane, _check, _dev_ptr, _stream_ptr, init, lib
 
 ANGLE_DELTAS, FILTER_INTRA, MIN4, CFL = 1, 2, 4, 8
+NONE, SPLIT, UNCODED, HORZ, VERT = 0, 1, 2, 3, 4  # node decisions (the split field; HORZ / VERT: oracle only)
 LUMA_ONLY, CHROMA_420, CHROMA_444 = 0, 1, 2
 NODES = 341
 ALL_MODES, FORCE_SPLIT, NO_SPLIT = 0x3FFF, 0x4000, 0x8000

@@ -53,7 +53,7 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     std::vector<float> targets(orc_tx_ncoef(s));
     const bool cfl = c.mode == 13;
     const int ssx = (c.pflags >> 3) & 1, ssy = (c.pflags >> 4) & 1;
-    if (cfl && (w != h || w > 32 || c.txtp_set != 0 || c.txtp_mode >= nplanes || c.angle_delta < -16 ||
+    if (cfl && (w > 32 || h > 32 || c.txtp_set != 0 || c.txtp_mode >= nplanes || c.angle_delta < -16 ||
                 c.angle_delta > 16 || ((c.x + w) << ssx) > planes[c.txtp_mode].width ||
                 ((c.y + h) << ssy) > planes[c.txtp_mode].height)) {
         r.status = -1;

@@ -372,7 +372,7 @@ struct Frame {
                        : d == kSplit ? id_base(nd[node_id(4, 2 * g.ix + 1, 2 * g.iy + 1)].mode)
                                      : id_base(nd[id].rmode1);
                 b.cfl_ok = group || g.s <= 32;
-                b.angle_ok = !group;
+                b.angle_ok = !group && g.s >= 8;
                 b.tr = ntr;
                 b.bl = nbl;
                 out.push_back(b);

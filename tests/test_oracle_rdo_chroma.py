@@ -82,7 +82,7 @@ def uv_blocks(nodes_sb, x0, y0, ss):
             else:
                 ym = base_of(int(nodes_sb[n]["rmode1"]))
             out.append((n, 0, (x0 + x) >> ss, (y0 + y) >> ss, c, c, 4 if c == 64 else 1, ym, group or s <= 32,
-                        not group, brute_avail(x, y, s)))
+                        not group and s >= 8, brute_avail(x, y, s)))
             continue
         k0 = 0 if sp == 3 else 2
         for j in range(2):
