@@ -412,8 +412,16 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
         const bool use_qm_type = !(is_idtx || is_1d);
 
         // Quantize one coefficient (fx, fy) of block b, record it, and leave its dequantized
-        // (and rect2-prescaled) value for the inverse in work[b][fy][fx].
-        auto emit_coef = [&](int b, int fx, int fy, int32_t lvl, float tgt) {
+        // (and rect2-prescaled) value for the inverse in work[b][fy][fx].  The block's hash
+        // and nonzero count gather in the caller's registers (hs, nz) and reach LDS once per
+        // line (flush): both are sums, so the order does not matter.
+        auto flush = [&](int b, uint32_t hs, uint32_t nz) {
+            if (MODE == MODE_CHAIN) {
+                atomicAdd(&s_hash[b], hs);
+                if (nz) atomicAdd(&s_nnz[b], nz);
+            }
+        };
+        auto emit_coef = [&](int b, int fx, int fy, int32_t lvl, float tgt, uint32_t& hs, uint32_t& nz) {
             const int rc = fx * H + fy;
             const BlkParam& p = prm[b];
             if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
@@ -422,8 +430,8 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                 if (MODE == MODE_FWD || a.targets) a.targets[o] = tgt;
             }
             if (MODE == MODE_CHAIN) {
-                atomicAdd(&s_hash[b], coef_hash(rc, lvl, tgt));
-                if (lvl) atomicAdd(&s_nnz[b], 1u);
+                hs += coef_hash(rc, lvl, tgt);
+                nz += lvl != 0;
             }
             if (MODE != MODE_FWD) {
                 int32_t step = rc == 0 ? p.dc : p.ac;
@@ -435,7 +443,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
             }
         };
         // Level and target of a 2-D-type coefficient (store_quant_target + apply_qmatrix).
-        auto quant2d = [&](int b, int fx, int fy, int32_t coeff) {
+        auto quant2d = [&](int b, int fx, int fy, int32_t coeff, uint32_t& hs, uint32_t& nz) {
             const BlkParam& p = prm[b];
             const int rc = fx * H + fy;
             int32_t m = rc == 0 ? p.qdc : p.qac;
@@ -443,7 +451,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
             int32_t lvl = quant_q16(coeff, m);
             float tgt = q16_target(coeff, m);
             if (p.qm_level < 15) lvl = qm_apply(tgt, g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc]);
-            emit_coef(b, fx, fy, lvl, tgt);
+            emit_coef(b, fx, fy, lvl, tgt, hs, nz);
         };
 
         if (MODE != MODE_INV) {
@@ -477,7 +485,9 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         const int32_t l = (am + qd / 2) / qd;
                         lvl = num < 0 ? -l : l;
                     }
-                    emit_coef(b, x, y, lvl, tgt);
+                    uint32_t hs = 0, nz = 0;
+                    emit_coef(b, x, y, lvl, tgt, hs, nz);
+                    flush(b, hs, nz);
                 }
             } else if (is_1d) {
                 // f1ddct*: DCT along one axis, identity along the other, per-length scale,
@@ -491,6 +501,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         for (int y = 0; y < H; y++) v[y] = work[b * BS + y * RS + x];
                         if constexpr (H <= 16) FDct<H>::run(v);
                         const BlkParam& p = prm[b];
+                        uint32_t hs = 0, nz = 0;
 #pragma unroll
                         for (int y = 0; y < H; y++) {
                             int32_t f;
@@ -499,8 +510,9 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                             else f = mulq16(v[y], kSqrt2Q16);
                             const int rc = y + x * H;
                             const int32_t m = rc == 0 ? p.qdc : p.qac;
-                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m));
+                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m), hs, nz);
                         }
+                        flush(b, hs, nz);
                     }
                 } else {
                     for (int line = tid; line < nb * H; line += NT) {
@@ -510,6 +522,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         for (int x = 0; x < W; x++) v[x] = work[b * BS + y * RS + x];
                         if constexpr (W <= 16) FDct<W>::run(v);
                         const BlkParam& p = prm[b];
+                        uint32_t hs = 0, nz = 0;
 #pragma unroll
                         for (int x = 0; x < W; x++) {
                             int32_t f;
@@ -518,8 +531,9 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                             else f = mulq16(v[x], kSqrt2Q16);
                             const int rc = y + x * H;
                             const int32_t m = rc == 0 ? p.qdc : p.qac;
-                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m));
+                            emit_coef(b, x, y, quant_full(f, m), q16_target(f, m), hs, nz);
                         }
+                        flush(b, hs, nz);
                     }
                 }
             } else if (!R::row_first) {
@@ -541,6 +555,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
 #pragma unroll
                     for (int x = 0; x < W; x++) v[x] = work[b * BS + fy * RS + x];
                     fwd_line<W>(row_adst, v);
+                    uint32_t hs = 0, nz = 0;
 #pragma unroll
                     for (int fx = 0; fx < W; fx++) {
                         int32_t c = v[fx];
@@ -548,8 +563,9 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         else if (R::post == 2) c = mulq16(c, 32768);
                         else if (R::post == 3) c = (c + 32) >> 6;
                         else if (R::post == 4) c = (int32_t)(((int64_t)c * 16384 + (1ll << 21)) >> 22);
-                        quant2d(b, fx, fy, c);
+                        quant2d(b, fx, fy, c, hs, nz);
                     }
+                    flush(b, hs, nz);
                 }
             } else {
                 // row pass (length W), then column pass (length H) + quantization
@@ -570,12 +586,14 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
 #pragma unroll
                     for (int y = 0; y < H; y++) v[y] = work[b * BS + y * RS + fx];
                     fwd_line<H>(col_adst, v);
+                    uint32_t hs = 0, nz = 0;
 #pragma unroll
                     for (int fy = 0; fy < H; fy++) {
                         int32_t c = v[fy];
                         if (R::post == 3) c = (c + 32) >> 6;
-                        quant2d(b, fx, fy, c);
+                        quant2d(b, fx, fy, c, hs, nz);
                     }
+                    flush(b, hs, nz);
                 }
             }
             __syncthreads();
