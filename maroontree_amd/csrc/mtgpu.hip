@@ -260,21 +260,16 @@ __device__ inline int32_t dequant(int32_t lvl, int32_t step, int dq_shift, int32
     mag = mag < lim ? mag : lim;
     return lvl < 0 ? -mag : mag;
 }
-__device__ inline uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x85ebca6bu;
-    x ^= x >> 13;
-    x *= 0xc2b2ae35u;
-    x ^= x >> 16;
-    return x;
-}
+// Order-independent block checksum: a sum over coefficients and samples of
+// fold(value * odd 24-bit weight of its position), all full-rate 24-bit products.
+__device__ inline uint32_t hw24(uint32_t p) { return (__umul24(p, 0x9E3779u) & 0xFFFFFFu) | 1u; }
+__device__ inline uint32_t fold13(uint32_t t) { return t ^ (t >> 13); }
 __device__ inline uint32_t coef_hash(int rc, int32_t lvl, float t) {
-    return mix32((uint32_t)rc * 0x9E3779B1u ^ (uint32_t)lvl) +
-           mix32(((uint32_t)rc * 0x85EBCA77u + 0x165667B1u) ^ __float_as_uint(t));
+    const uint32_t tb = __float_as_uint(t);
+    return fold13(__umul24((uint32_t)lvl, hw24((uint32_t)rc))) +
+           fold13(__umul24(tb, hw24((uint32_t)rc + 4096u)) ^ (tb >> 24));
 }
-__device__ inline uint32_t pix_hash(int j, int32_t r) {
-    return mix32(((uint32_t)j * 0xC2B2AE3Du + 0x27D4EB2Fu) ^ (uint32_t)r);
-}
+__device__ inline uint32_t pix_hash(int j, int32_t r) { return fold13(__umul24((uint32_t)r, hw24((uint32_t)j + 8192u))); }
 
 // One forward 1-D transform of length L on a register array.
 template <int L> __device__ inline void fwd_line(bool adst, int32_t* v) {

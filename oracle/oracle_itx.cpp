@@ -611,14 +611,11 @@ extern "C" int64_t orc_sse_recon(const int32_t* pred, const int32_t* resid, cons
 }
 
 namespace {
-inline uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x85ebca6bu;
-    x ^= x >> 13;
-    x *= 0xc2b2ae35u;
-    x ^= x >> 16;
-    return x;
-}
+// The library's block checksum: sum of fold(value * odd 24-bit position weight) with the
+// low 32 bits of 24 x 24-bit products.
+inline uint32_t u24mul(uint32_t a, uint32_t b) { return (uint32_t)((uint64_t)(a & 0xFFFFFFu) * (b & 0xFFFFFFu)); }
+inline uint32_t hw24(uint32_t p) { return (u24mul(p, 0x9E3779u) & 0xFFFFFFu) | 1u; }
+inline uint32_t fold13(uint32_t t) { return t ^ (t >> 13); }
 }  // namespace
 
 extern "C" uint32_t orc_block_hash(int ncoef, const int32_t* levels, const float* targets, int npx,
@@ -627,10 +624,10 @@ extern "C" uint32_t orc_block_hash(int ncoef, const int32_t* levels, const float
     for (int i = 0; i < ncoef; i++) {
         uint32_t tb;
         memcpy(&tb, &targets[i], 4);
-        h += mix32((uint32_t)i * 0x9E3779B1u ^ (uint32_t)levels[i]);
-        h += mix32(((uint32_t)i * 0x85EBCA77u + 0x165667B1u) ^ tb);
+        h += fold13(u24mul((uint32_t)levels[i], hw24((uint32_t)i)));
+        h += fold13(u24mul(tb, hw24((uint32_t)i + 4096u)) ^ (tb >> 24));
     }
-    for (int j = 0; j < npx; j++) h += mix32(((uint32_t)j * 0xC2B2AE3Du + 0x27D4EB2Fu) ^ (uint32_t)recon[j]);
+    for (int j = 0; j < npx; j++) h += fold13(u24mul((uint32_t)recon[j], hw24((uint32_t)j + 8192u)));
     return h;
 }
 
