@@ -243,8 +243,9 @@ __device__ inline float q16_target(int32_t c, int32_t m) {
 }
 // apply_qmatrix on one coefficient (src/avx/qmatrix.rs:34-66): scale the target by the
 // forward weight (1024 + iwt/2) / iwt / 32, then round half away from zero.
+__constant__ int32_t c_qmw[256];  // (1024 + iwt / 2) / iwt for every 8-bit iwt (0 at iwt 0)
 __device__ inline int32_t qm_apply(float& t, int iwt) {
-    const int w = (1024 + iwt / 2) / iwt;
+    const int w = c_qmw[iwt & 255];
     t = __fmul_rn(t, (float)w * (1.0f / 32.0f));
     const float tr = truncf(t);
     const float fr = fabsf(__fsub_rn(t, tr));
@@ -255,7 +256,8 @@ __device__ inline int32_t qm_apply(float& t, int iwt) {
 __device__ inline int32_t dequant(int32_t lvl, int32_t step, int dq_shift, int32_t cf_max) {
     if (lvl == 0) return 0;
     const uint32_t a = lvl < 0 ? 0u - (uint32_t)lvl : (uint32_t)lvl;
-    int32_t mag = (int32_t)(((a * (uint32_t)step) & 0xffffffu) >> dq_shift);
+    // the low 24 bits of a product depend only on the low 24 bits of its operands
+    int32_t mag = (int32_t)((__umul24(a, (uint32_t)step) & 0xffffffu) >> dq_shift);
     const int32_t lim = cf_max + (lvl < 0 ? 1 : 0);
     mag = mag < lim ? mag : lim;
     return lvl < 0 ? -mag : mag;
@@ -658,7 +660,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         const int32_t src = min(max(P + s, 0), maxv);
                         const int32_t rec = min(max(P + r, 0), maxv);
                         const int32_t d = src - rec;
-                        sse += (uint32_t)(d * d);
+                        sse += (uint32_t)__mul24(d, d);  // |d| < 2^12
                         hsh += pix_hash(j, r);
                     }
                 }
@@ -853,7 +855,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                             const int32_t src = min(max(P + (int32_t)a.pool16[(size_t)off + j], 0), maxv);
                             const int32_t rec = min(max(P + r, 0), maxv);
                             const int32_t dd = src - rec;
-                            sse += (uint32_t)(dd * dd);
+                            sse += (uint32_t)__mul24(dd, dd);  // |dd| < 2^12
                             hsh += pix_hash(j, r);
                         }
                     }
@@ -1118,6 +1120,11 @@ extern "C" int mt_init(int device) {
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst8), MT_FADST8_Q12, sizeof(MT_FADST8_Q12)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_fadst16), MT_FADST16_Q12, sizeof(MT_FADST16_Q12)));
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_iqm), MT_IQM, sizeof(MT_IQM)));
+    {
+        int32_t qmw[256];
+        for (int i = 0; i < 256; i++) qmw[i] = i ? (1024 + i / 2) / i : 0;
+        HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_qmw), qmw, sizeof(qmw)));
+    }
     HIP_TRY(upload_pred_tables());  // this unit's copy, for the RDO reconstruction
     // dct64_basis (src/dct.rs:4132): built on the host with the C library's cos, as the
     // reference builds it, so every f64 weight is the same.
