@@ -747,13 +747,16 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             double acc[32];
 #pragma unroll
             for (int j = 0; j < 32; j++) acc[j] = 0.0;
-#pragma unroll 2
+#pragma unroll 1
             for (int y = 0; y < 64; y++) {
                 const double c = MODE == MODE_CHAIN ? (double)in16[y * 64 + lane]
                                                     : (double)a.in32[(size_t)g * 4096 + y * 64 + lane];
+                // the row's 32 basis values in one batch of scalar loads (one wait per row)
+                double bv[32];
 #pragma unroll
-                for (int j = 0; j < 32; j++)
-                    acc[j] = __dadd_rn(acc[j], __dmul_rn(c_dct64_basis_t[y * 32 + j], c));
+                for (int j = 0; j < 32; j++) bv[j] = c_dct64_basis_t[y * 32 + j];
+#pragma unroll
+                for (int j = 0; j < 32; j++) acc[j] = __dadd_rn(acc[j], __dmul_rn(bv[j], c));
             }
             __syncthreads();  // the staged residual is dead
 #pragma unroll
