@@ -769,12 +769,21 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                     double hacc[8];
 #pragma unroll
                     for (int k = 0; k < 8; k++) hacc[k] = -0.0;
+                    // the lane's basis values stream one pair of x ahead of their use (the
+                    // last prefetch re-reads x = 62, 63 and goes unused)
+                    double bn0 = c_dct64_basis_t[u], bn1 = c_dct64_basis_t[32 + u];
 #pragma unroll 2
-                    for (int x = 0; x < 64; x++) {
-                        const double bu = c_dct64_basis_t[x * 32 + u];
+                    for (int x = 0; x < 64; x += 2) {
+                        const double b0 = bn0, b1 = bn1;
+                        const int xn = x + 2 < 64 ? x + 2 : 62;
+                        bn0 = c_dct64_basis_t[xn * 32 + u];
+                        bn1 = c_dct64_basis_t[(xn + 1) * 32 + u];
 #pragma unroll
                         for (int k = 0; k < 8; k++)
-                            hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
+                            hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], b0));
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x + 1], b1));
                     }
 #pragma unroll
                     for (int k = 0; k < 8; k++)
