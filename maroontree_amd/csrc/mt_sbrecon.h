@@ -105,11 +105,14 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
     double acc[32];
 #pragma unroll
     for (int j = 0; j < 32; j++) acc[j] = 0.0;
-#pragma unroll 2
+#pragma unroll 1
     for (int y = 0; y < 64; y++) {
         const double c = (double)res[y * 64 + lane];
+        double bv[32];  // the basis row in one batch of scalar loads (one wait per row)
 #pragma unroll
-        for (int j = 0; j < 32; j++) acc[j] = __dadd_rn(acc[j], __dmul_rn(c_dct64_basis_t[y * 32 + j], c));
+        for (int j = 0; j < 32; j++) bv[j] = c_dct64_basis_t[y * 32 + j];
+#pragma unroll
+        for (int j = 0; j < 32; j++) acc[j] = __dadd_rn(acc[j], __dmul_rn(bv[j], c));
     }
 #pragma unroll
     for (int hf = 0; hf < 2; hf++) {
@@ -121,11 +124,19 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
             double hacc[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) hacc[k] = -0.0;
+            // the lane's basis values stream one pair of x ahead of their use
+            double bn0 = c_dct64_basis_t[u], bn1 = c_dct64_basis_t[32 + u];
 #pragma unroll 2
-            for (int x = 0; x < 64; x++) {
-                const double bu = c_dct64_basis_t[x * 32 + u];
+            for (int x = 0; x < 64; x += 2) {
+                const double b0 = bn0, b1 = bn1;
+                const int xn = x + 2 < 64 ? x + 2 : 62;
+                bn0 = c_dct64_basis_t[xn * 32 + u];
+                bn1 = c_dct64_basis_t[(xn + 1) * 32 + u];
 #pragma unroll
-                for (int k = 0; k < 8; k++) hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], bu));
+                for (int k = 0; k < 8; k++) hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x], b0));
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    hacc[k] = __dadd_rn(hacc[k], __dmul_rn(vt[(2 * k + vp) * 64 + x + 1], b1));
             }
 #pragma unroll
             for (int k = 0; k < 8; k++)
