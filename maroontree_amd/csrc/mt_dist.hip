@@ -86,6 +86,7 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
     __shared__ int16_t s_e[64 * kES];  // error src - recon (|e| < 2^12), zero where invisible
     __shared__ float s_acc[32];
     __shared__ float s_sum[4];
+    __shared__ float s_t[16 * 16];  // column sums t[k][j] of one tile row: [tile][k * 4 + j]
     const int lane = threadIdx.x;
     const int n = (int)*count;
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
@@ -141,17 +142,37 @@ __global__ __launch_bounds__(64) void dist_kernel(const mt_distblk* __restrict__
             float total = 0.0f;
             if (pw > 0 && ph > 0) {
                 const int pairs = pw / 8 * 8;
-                if (lane < 32) {
-                    const int k = lane >> 3, half = (lane >> 2) & 1, l = lane & 3;
-                    float acc = 0.0f;
-                    const float wk = c_ww[k][l];
-                    for (int r0 = 0; r0 < ph; r0 += 4)
+                // Per row of tiles: all lanes first build the column sums t[k][j] of its
+                // paired tiles (wsse_coef's first stage, same operations), then lane a < 32
+                // finishes its coefficient (k, l) of tile c0 / 4 + half from them and runs its
+                // chain over the tiles in order, as wsse_coef would.
+                const int k = lane >> 3, half = (lane >> 2) & 1, l = lane & 3;
+                float acc = 0.0f;
+                const float wk = lane < 32 ? c_ww[k][l] : 0.0f;
+                for (int r0 = 0; r0 < ph; r0 += 4) {
+                    for (int e = lane; e < pairs * 4; e += 64) {  // pairs / 4 tiles x 16
+                        const int q = e >> 4, kk = (e >> 2) & 3, j = e & 3;
+                        const int16_t* col = s_e + r0 * kES + q * 4 + j;
+                        float v = __fmul_rn((float)col[0], c_wd[kk][0]);
+                        v = __fmaf_rn((float)col[kES], c_wd[kk][1], v);
+                        v = __fmaf_rn((float)col[2 * kES], c_wd[kk][2], v);
+                        v = __fmaf_rn((float)col[3 * kES], c_wd[kk][3], v);
+                        s_t[e] = v;
+                    }
+                    __syncthreads();
+                    if (lane < 32) {
                         for (int c0 = 0; c0 < pairs; c0 += 8) {
-                            const float c = wsse_coef(s_e, kES, r0, c0 + 4 * half, k, l);
+                            const float* t = s_t + ((c0 >> 2) + half) * 16 + k * 4;
+                            float c = __fmul_rn(c_wd[l][0], t[0]);
+                            c = __fmaf_rn(c_wd[l][1], t[1], c);
+                            c = __fmaf_rn(c_wd[l][2], t[2], c);
+                            c = __fmaf_rn(c_wd[l][3], t[3], c);
                             acc = __fmaf_rn(__fmul_rn(wk, c), c, acc);
                         }
-                    s_acc[lane] = acc;
+                    }
+                    __syncthreads();  // s_t is rewritten for the next row
                 }
+                if (lane < 32) s_acc[lane] = acc;
                 __syncthreads();
                 if (lane < 4) {
                     const int l = lane;
