@@ -859,7 +859,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                     for (int h2 = 0; h2 < 2; h2++) {
 #pragma unroll
                         for (int y = 0; y < 32; y++) rowbuf[y * 64 + x] = (v[h2 * 32 + y] + 8) >> 4;
-#pragma unroll 4
+#pragma unroll 8
                         for (int y = 0; y < 32; y++) {
                             const int j = (h2 * 32 + y) * 64 + x;
                             const int32_t r = rowbuf[y * 64 + x];
