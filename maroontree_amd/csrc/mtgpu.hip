@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             // quantize the lane's 16 coefficients, leave their dequantized values in place
             {
                 const int u = lane & 31, vp = lane >> 5;
-#pragma unroll 1
+#pragma unroll 4
                 for (int k = 0; k < 16; k++) {
                     const int v = 2 * k + vp;
                     const int32_t coeff = coef[v * 33 + u];
