@@ -461,7 +461,9 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restr
                 spl = (fl & kInside) ? __fadd_rn(sum, __fmul_rn(f.lambda, part_bits(f, x, y, L, 1, x0, y0, false)))
                                      : sum;
             }
-            const bool leaf = none <= spl;
+            // a 4x4 node, or one that may not split, is a leaf whatever its costs compare
+            // like (a NaN cost must not send a walk below level 4)
+            const bool leaf = L == 4 || !(fl & kMaySplit) || none <= spl;
             cost[id] = leaf ? none : spl;
             split[id] = leaf ? 0 : 1;
             nd.cost = cost[id];
@@ -479,11 +481,12 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restr
             const int id = stack[--sp];
             if (split[id] == 2) continue;
             const int L = node_level(id), kk = id - level_base(L), n = 1 << L;
-            if (split[id] == 1) {
+            if (split[id] == 1 && L < 4 && sp + 4 <= 32) {
                 for (int q = 3; q >= 0; --q)
                     stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
                 continue;
             }
+            if (split[id] == 1) continue;  // unreachable: level 4 never splits
             lv[nl++] = (uint16_t)id;
         }
         s_nl = nl;
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(64) void rdo_decide_kernel(const FrameArgs* __restr
         const int ux = (k % g) * cell, uy = (k / g) * cell;
         if (x0 + ux >= f.W || y0 + uy >= f.H) continue;
         int id = 0, L = 0;
-        while (split[id] == 1) {
+        while (split[id] == 1 && L < 4) {
             ++L;
             const int s = 64 >> L;
             id = node_id(L, ux / s, uy / s);
@@ -587,7 +590,8 @@ __device__ void uv_walk(const FrameArgs& f, const mt_rdo_node* nd, int x0, int y
         const int ix = kk % n, iy = kk / n;
         const bool group = f.ss && s == 8 && sp_ == 1;  // 4:2:0: 8x8 split into 4x4 leaves
         if (sp_ == 1 && !group) {
-            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1));
+            if (L < 4 && sp + 4 <= 32)
+                for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * ix + (q & 1), 2 * iy + (q >> 1));
             continue;
         }
         UvBlk b{};
@@ -833,8 +837,9 @@ __global__ void __launch_bounds__(64) rdo_final_gen_kernel(const FrameArgs* __re
         const int sp_ = nd[id].split;
         if (sp_ == 2) continue;
         const int L = node_level(id), kk = id - level_base(L), n = 1 << L, s = 64 >> L;
-        if (sp_ == 1) {
-            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+        if (sp_ == 1) {  // decided nodes split only above level 4
+            if (L < 4 && sp + 4 <= 32)
+                for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
             continue;
         }
         const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
@@ -887,8 +892,9 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
         const int x = x0 + (kk % n) * s, y = y0 + (kk / n) * s;
         if (s >= 8 && x + s <= f.W && y + s <= f.H)
             b = __dadd_rn(b, (double)part_bits(f, x, y, L, sp_, x0, y0, true));
-        if (sp_ == 1) {
-            for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
+        if (sp_ == 1) {  // decided nodes split only above level 4
+            if (L < 4 && sp + 4 <= 32)
+                for (int q = 3; q >= 0; --q) stack[sp++] = (uint16_t)node_id(L + 1, 2 * (kk % n) + (q & 1), 2 * (kk / n) + (q >> 1));
             continue;
         }
         b = __dadd_rn(b, (double)res[k].bits);

@@ -1,0 +1,186 @@
+"""The whole-frame RDO on the GPU against the oracle's RDO at the measured workloads' own
+sizes (not crops): the seeded 512x512 4:4:4 8-bit frame at quality 60 with CfL, the full
+3840x2160 10-bit 4:2:0 frame at quality 50, a 1920x1080 8-bit 4:2:0 frame under the masks
+of a network trained on the GPU, and an uneven batch holding 8192-sample-long frames with
+mixed bit depths and chroma formats (two plane-set groups in one call).  Every comparison
+covers the Y, U and V reconstructions, every node record (f32 costs as bit patterns, uv
+modes, CfL alphas) and the accounting (bits, chroma bits, per-plane SSE, candidates,
+leaves).
+
+Also: the trained network against an untrained one under the same pruning policy, on
+held-out frames at four qualities (Bjontegaard delta rate against the full search)."""
+import numpy as np
+import pytest
+import torch
+
+import maroontree_amd as mt
+import oracle_lib as O
+from maroontree_amd import prune as PR
+from maroontree_amd import rdo as R
+from maroontree_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+ALL = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
+SEED = 0x6D61726F6F6E
+NODE_FIELDS = ("split", "mode", "uv_mode", "cfl_alpha_u", "cfl_alpha_v")
+
+
+def _t(dev, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).to(dev)
+
+
+def gpu_frame(dev, y, u, v, bd, q, fmt, prune=None):
+    sub = 2 if fmt == R.CHROMA_420 else 0
+    fr = {"src": _t(dev, y), "src_u": _t(dev, u), "src_v": _t(dev, v), "bd": bd, "qidx": q,
+          "qm_level": mt.qm_level_law(q, sub), "lam": R.rdo_lambda(q, bd, sub), "chroma": fmt}
+    if prune is not None:
+        fr["prune"] = _t(dev, prune)
+    return fr
+
+
+def oracle(y, u, v, bd, q, fmt, flags, prune=None, nthreads=16):
+    sub = 2 if fmt == R.CHROMA_420 else 0
+    return O.intra_rdo_yuv(y, u, v, bd, q, mt.qm_level_law(q, sub), flags, R.rdo_lambda(q, bd, sub), fmt,
+                           *R.chroma_laws(q, sub), prune=prune, nthreads=nthreads)
+
+
+def assert_same(tag, got, want):
+    rec, nodes, st = got
+    o_rec, o_u, o_v, o_nodes, o_st = want
+    ru, rv = st["recon_uv"]
+    for name, a, b in (("y", rec, o_rec), ("u", ru, o_u), ("v", rv, o_v)):
+        d = np.argwhere(a.cpu().numpy().view(np.uint16) != b)
+        assert len(d) == 0, f"{tag} {name}: {len(d)} samples differ, first {d[:4].tolist()}"
+    g = R.nodes_to_np(nodes)
+    assert g.shape == o_nodes.shape, (tag, g.shape, o_nodes.shape)
+    for f in NODE_FIELDS:
+        bad = np.argwhere(g[f] != o_nodes[f])
+        assert len(bad) == 0, (tag, f, len(bad), bad[:4].tolist())
+    for f in ("cost", "none_cost"):
+        bad = np.argwhere(g[f].view(np.uint32) != o_nodes[f].view(np.uint32))
+        assert len(bad) == 0, (tag, f, len(bad), bad[:4].tolist())
+    for k in ("sse", "sse_u", "sse_v", "candidates", "leaves"):
+        assert st[k] == o_st[k], (tag, k, st[k], o_st[k])
+    assert st["bits"] == o_st["bits"] and st["bits_uv"] == o_st["bits_uv"], (tag, st, o_st)
+
+
+def test_plain_512_444_q60_matches_oracle(dev):
+    """The plain-encode configuration: seeded 512x512 RGB -> YCbCr 4:4:4 8-bit, quality 60
+    (qindex 70), luma + chroma + CfL."""
+    y, u, v = W.synth_yuv(SEED, 512, 512, 8, 0)
+    print("config 512x512 4:4:4 8-bit q70", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 8, 70, R.CHROMA_444)], ALL)[0]
+    torch.cuda.synchronize()
+    want = oracle(y, u, v, 8, 70, R.CHROMA_444, ALL)
+    assert_same("512", got, want)
+    nd = R.nodes_to_np(got[1])
+    assert (nd["uv_mode"] == 13).sum() > 0  # CfL was chosen somewhere
+
+
+def test_full_4k_10bit_420_q50_matches_oracle(dev):
+    """The exact full intra RDO configuration: the whole seeded 3840x2160 10-bit 4:2:0 frame
+    at quality 50 (qindex 91): 2040 superblocks, 126 wavefront steps."""
+    y, u, v = W.synth_yuv(SEED, 3840, 2160, 10, 2)
+    print("config 3840x2160 10-bit 4:2:0 q91", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 10, 91, R.CHROMA_420)], ALL)[0]
+    torch.cuda.synchronize()
+    print("  gpu done; oracle", flush=True)
+    want = oracle(y, u, v, 10, 91, R.CHROMA_420, ALL)
+    assert got[2]["steps"] == 126 and got[2]["sbs"] == 2040
+    assert_same("4k", got, want)
+
+
+def test_uneven_batch_with_8192_sides_matches_oracle(dev):
+    """An uneven batch: frames 8192 samples wide and 8192 tall (wavefronts of 129 and 255
+    steps), 8/10-bit, 4:2:0/4:4:4, four qualities, random prune masks on two frames; 12
+    chroma frames exceed one plane set (10 frames), so the call runs two groups."""
+    rng = np.random.default_rng(8192)
+    specs = [(8192, 72, 10, 2, 91), (72, 8192, 8, 0, 53), (8192, 64, 8, 0, 140)]
+    for i in range(9):
+        w, h = (int(np.exp(rng.uniform(np.log(64), np.log(400)))) // 8 * 8 for _ in range(2))
+        specs.append((w, h, int(rng.choice([8, 10])), int(rng.choice([0, 2])), int(rng.choice([140, 91, 53, 17]))))
+    frames, wants = [], []
+    for i, (w, h, bd, sub, q) in enumerate(specs):
+        y, u, v = W.synth_yuv(9000 + i, w, h, bd, sub)
+        fmt = R.CHROMA_420 if sub == 2 else R.CHROMA_444
+        prune = None
+        if i in (1, 5):
+            nsb = ((w + 63) // 64) * ((h + 63) // 64)
+            prune = rng.integers(0, 1 << 16, (nsb, R.NODES)).astype(np.uint16)
+        frames.append(gpu_frame(dev, y, u, v, bd, q, fmt, prune))
+        wants.append((f"{w}x{h} bd{bd} sub{sub} q{q}", (y, u, v, bd, q, fmt, prune)))
+    print("config uneven batch:", [s for s, _ in wants], flush=True)
+    got = R.intra_rdo_batch(frames, ALL)
+    torch.cuda.synchronize()
+    for g, (tag, (y, u, v, bd, q, fmt, prune)) in zip(got, wants):
+        assert_same(tag, g, oracle(y, u, v, bd, q, fmt, ALL, prune=prune))
+
+
+# ---- the learned pruning -----------------------------------------------------------------
+TOPK, LO, HI = [8, 10, 12, 13, 13], [0.0, 0.01, 0.005, 0.0], [1.0, 1.0, 1.0, 1.0]
+
+
+@pytest.fixture(scope="module")
+def trained(dev):
+    """A network trained on the GPU on exact-RDO labels of 24 seeded 1920x1080 8-bit frames
+    (luma decisions, qualities 30/50/70/90), and an untrained one of the same init."""
+    print("training the pruning network on 24 1080p frames", flush=True)
+    seeds = list(range(4000, 4024))
+    data = PR.make_dataset(dev, seeds, 1920, 1080, 8, [140, 91, 53, 17], R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4)
+    net = PR.PruneNet(PR.init_params(11), dev)
+    hist = PR.train(net, data, 8, epochs=30, batch=256, lr=2e-3, seed=1)
+    print(f"  loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
+    assert hist[-1] < 0.8 * hist[0]
+    return net, PR.PruneNet(PR.init_params(11), dev)
+
+
+def test_1080p_420_under_trained_masks_matches_oracle(dev, trained):
+    """A held-out 1920x1080 8-bit 4:2:0 frame at quality 50 pruned by the trained network's
+    masks (the learned form's policy): the GPU's pruned RDO equals the oracle's RDO under
+    the same masks, and prunes candidates."""
+    net, _ = trained
+    y, u, v = W.synth_yuv(4999, 1920, 1080, 8, 2)
+    src = _t(dev, y)
+    masks = PR.prune_masks(net, src, 8, 91, TOPK, LO, HI).cpu().numpy().view(np.uint16)
+    print("config 1920x1080 4:2:0 q91 pruned", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 8, 91, R.CHROMA_420, masks)], ALL)[0]
+    full = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 8, 91, R.CHROMA_420)], ALL)[0]
+    torch.cuda.synchronize()
+    assert_same("1080p pruned", got, oracle(y, u, v, 8, 91, R.CHROMA_420, ALL, prune=masks))
+    # the network prunes the luma search; the chroma candidates stay
+    assert got[2]["candidates"] < 0.8 * full[2]["candidates"]
+
+
+def test_trained_network_beats_untrained_under_same_policy(dev, trained):
+    """Held-out 1080p frames at qualities 30/50/70/90, luma RDO: the trained network's
+    pruned search loses less BD-rate (PSNR-Y vs bits) against the full search than the
+    untrained network under the same mode budgets and split thresholds, and stays within
+    1 %."""
+    net, raw = trained
+    imgs = W.synth_images([(5000 + i, 1920, 1080, 8) for i in range(3)])
+    srcs = [_t(dev, a) for a in imgs]
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4
+    pts = {"full": ([], []), "trained": ([], []), "untrained": ([], [])}
+    cands = {k: 0 for k in pts}
+    for q in (140, 91, 53, 17):
+        for name, model in (("full", None), ("trained", net), ("untrained", raw)):
+            frames = []
+            for s in srcs:
+                fr = {"src": s, "bd": 8, "qidx": q, "qm_level": mt.qm_level_law(q, 0)}
+                if model is not None:
+                    fr["prune"] = PR.prune_masks(model, s, 8, q, TOPK, LO, HI)
+                frames.append(fr)
+            out = R.intra_rdo_batch(frames, flags)
+            bits = sum(o[2]["bits"] for o in out)
+            sse = sum(o[2]["sse"] for o in out)
+            cands[name] += sum(o[2]["candidates"] for o in out)
+            pts[name][0].append(bits)
+            pts[name][1].append(R.psnr(sse, 3 * 1920 * 1080, 8))
+    bd_t = PR.bd_rate(*pts["full"], *pts["trained"])
+    bd_u = PR.bd_rate(*pts["full"], *pts["untrained"])
+    print(f"BD-rate trained {bd_t:+.3f} % ({cands['trained'] / cands['full']:.3f} of the candidates), "
+          f"untrained {bd_u:+.3f} % ({cands['untrained'] / cands['full']:.3f})", flush=True)
+    assert bd_t < bd_u
+    assert bd_t < 1.0
+    assert cands["trained"] < cands["full"]
