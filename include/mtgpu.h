@@ -120,6 +120,17 @@ int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* levels_d,
 int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                 int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream);
 
+/* mt_tx_chain without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as mt_tx_chain;
+ * its workspace is a fixed device array of the library (4 MiB per device, held until the
+ * process ends), so n is at most MT_NOALLOC_MAX_BLOCKS (larger n: MT_ERR_ARG, nothing
+ * launched).  Every call on one device shares that array: calls that overlap on one
+ * device (two streams, or two host threads) race and must be ordered by the caller. */
+#define MT_NOALLOC_MAX_BLOCKS 1048576
+int mt_tx_chain_noalloc(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
+                        int32_t* levels_d, float* targets_d, int32_t* recon_d, void* stream);
+
 /* ---- coefficient rate ------------------------------------------------------------- */
 /* One transform block to price: 32 bytes.  The rate is real_block_bits_bounded
  * (src/rate.rs:149) against the frame-initial coefficient CDFs Cdfs::new(qctx)

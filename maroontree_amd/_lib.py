@@ -109,6 +109,7 @@ def lib():
         L.mt_fwd_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp]
         L.mt_inv_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp]
         L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+        L.mt_tx_chain_noalloc.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
@@ -241,6 +242,19 @@ def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=No
     opt = lambda t: _dev_ptr(t) if t is not None else None  # noqa: E731
     _check(lib().mt_tx_chain(_dev_ptr(blocks), n, _dev_ptr(pool), _dev_ptr(res), opt(levels),
                              opt(targets), opt(recon), _stream_ptr(stream)))
+
+
+NOALLOC_MAX_BLOCKS = 1 << 20
+
+
+def tx_chain_noalloc(blocks, pool, res, levels=None, targets=None, recon=None, stream=None):
+    """mt_tx_chain_noalloc: tx_chain on the library's fixed device workspace (no allocation;
+    at most NOALLOC_MAX_BLOCKS blocks; calls overlapping on one device race)."""
+    n = blocks.numel() // BLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    opt = lambda t: _dev_ptr(t) if t is not None else None  # noqa: E731
+    _check(lib().mt_tx_chain_noalloc(_dev_ptr(blocks), n, _dev_ptr(pool), _dev_ptr(res), opt(levels),
+                                     opt(targets), opt(recon), _stream_ptr(stream)))
 
 
 PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",

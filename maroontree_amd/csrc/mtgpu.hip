@@ -899,6 +899,13 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
 // unsupported class are answered directly (sse = -1).
 constexpr int kClasses = 13 * 7;
 
+// Workspace of mt_tx_chain_noalloc, held by every device the library runs on until the
+// process ends: the class counters, starts and cursors (kNoAllocSlots words each) and the
+// block order for up to MT_NOALLOC_MAX_BLOCKS blocks (4 MiB + 1.5 KiB).  Host code never
+// uses this name as an address: it uses the per-device pointer hipGetSymbolAddress returns.
+constexpr int kNoAllocSlots = 128;
+static __device__ __align__(256) uint32_t scratch[3 * kNoAllocSlots + MT_NOALLOC_MAX_BLOCKS];
+
 __global__ void __launch_bounds__(256) bucket_count_kernel(const mt_txblk* blocks, int n,
                                                           uint32_t* counts, mt_txres* res) {
     __shared__ uint32_t h[kClasses];
@@ -979,6 +986,8 @@ struct DeviceState {
     uint32_t* cursor = nullptr;  // [kClasses]
     uint32_t* order = nullptr;   // [cap]
     size_t order_cap = 0;
+    // this device's address of `scratch` (mt_tx_chain_noalloc), from hipGetSymbolAddress
+    uint32_t* scratch_d = nullptr;
 };
 std::mutex g_mu;
 DeviceState g_dev[64];
@@ -1014,7 +1023,7 @@ int current_device(int* dev) {
 
 namespace mtlib {
 ProfScope::ProfScope(int slot_, hipStream_t s_) : slot(slot_), s(s_) {
-    if (!g_prof.on) return;
+    if (!g_prof.on || slot < 0) return;  // a negative slot records nothing
     a = prof_event();
     b = prof_event();
     if (a) (void)hipEventRecord(a, s);
@@ -1167,6 +1176,13 @@ extern "C" int mt_init(int device) {
     HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&st.cursor, kClasses * sizeof(uint32_t)));
+    // the allocation-free chain's workspace: this device's copy of `scratch`
+    {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(MT_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+        st.scratch_d = (uint32_t*)p;
+    }
     HIP_TRY(hipDeviceSynchronize());
     st.init = true;
     return MT_OK;
@@ -1330,6 +1346,52 @@ extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_
                                   (hipStream_t)stream);
 }
 
+namespace {
+// The chain's launches on a given workspace: counts / starts / cursor hold kClasses words,
+// order n words.  Shared by the library-workspace and the allocation-free entry points.
+int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t* order, int cus,
+                  const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
+                  int32_t* levels_d, float* targets_d, int32_t* recon_d, const int8_t* qdelta_d,
+                  hipStream_t s, bool profile) {
+    int rc;
+    {
+    ProfScope ps(profile ? 0 : -1, s);
+    // cleared by a kernel, not hipMemsetAsync, so the sequence stays graph-capturable
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, counts, kClasses);
+    HIP_TRY(hipGetLastError());
+    const int cgrid = std::min((n + 255) / 256, cus * 4);
+    hipLaunchKernelGGL(bucket_count_kernel, dim3(cgrid), dim3(256), 0, s, blocks_d, n, counts, res_d);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(64), 0, s, counts, starts, cursor);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n,
+                       cursor, order);
+    HIP_TRY(hipGetLastError());
+    }
+    KArgs a{};
+    a.blocks = blocks_d;
+    a.order = order;
+    a.cls_start = starts;
+    a.cls_count = counts;
+    a.pool16 = pool_d;
+    a.res = res_d;
+    a.levels = levels_d;
+    a.targets = targets_d;
+    a.recon = recon_d;
+    a.qdelta = qdelta_d;
+    // The class sizes live on the device; each launch is persistent over its own tiles, so
+    // the grid only needs an upper bound: n blocks can never need more tiles than this.
+    for (int slot = 0; slot < 13; slot++) {
+        const int sz = kSlotSize[slot];
+        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
+        ProfScope ps(profile ? 1 + slot : -1, s);
+        rc = launch_size<MODE_CHAIN>(sz, a, tiles, cus, s);
+        if (rc) return rc;
+    }
+    return MT_OK;
+}
+}  // namespace
+
 int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                            int32_t* levels_d, float* targets_d, int32_t* recon_d,
                            const int8_t* qdelta_d, hipStream_t stream) {
@@ -1345,42 +1407,31 @@ int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_
     DeviceState& st = g_dev[dev];
     rc = ensure_order(st, (size_t)n);
     if (rc) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    {
-    ProfScope ps(0, s);
-    // cleared by a kernel, not hipMemsetAsync, so the sequence stays graph-capturable
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, st.counts, kClasses);
-    HIP_TRY(hipGetLastError());
-    const int cgrid = std::min((n + 255) / 256, st.cus * 4);
-    hipLaunchKernelGGL(bucket_count_kernel, dim3(cgrid), dim3(256), 0, s, blocks_d, n, st.counts, res_d);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(64), 0, s, st.counts, st.starts, st.cursor);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(bucket_scatter_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n,
-                       st.cursor, st.order);
-    HIP_TRY(hipGetLastError());
-    }
-    KArgs a{};
-    a.blocks = blocks_d;
-    a.order = st.order;
-    a.cls_start = st.starts;
-    a.cls_count = st.counts;
-    a.pool16 = pool_d;
-    a.res = res_d;
-    a.levels = levels_d;
-    a.targets = targets_d;
-    a.recon = recon_d;
-    a.qdelta = qdelta_d;
-    // The class sizes live on the device; each launch is persistent over its own tiles, so
-    // the grid only needs an upper bound: n blocks can never need more tiles than this.
-    for (int slot = 0; slot < 13; slot++) {
-        const int sz = kSlotSize[slot];
-        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
-        ProfScope ps(1 + slot, s);
-        rc = launch_size<MODE_CHAIN>(sz, a, tiles, st.cus, s);
-        if (rc) return rc;
-    }
-    return MT_OK;
+    return enqueue_chain(st.counts, st.starts, st.cursor, st.order, st.cus, blocks_d, n, pool_d,
+                         res_d, levels_d, targets_d, recon_d, qdelta_d, stream, true);
+}
+
+extern "C" int mt_tx_chain_noalloc(const mt_txblk* blocks_d, int n, const int16_t* pool_d,
+                                   mt_txres* res_d, int32_t* levels_d, float* targets_d,
+                                   int32_t* recon_d, void* stream) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!blocks_d || !pool_d || !res_d)))
+        return set_err(MT_ERR_ARG, "mt_tx_chain_noalloc: bad pointer or count");
+    if (n > MT_NOALLOC_MAX_BLOCKS)
+        return set_err(MT_ERR_ARG, "mt_tx_chain_noalloc: more blocks than MT_NOALLOC_MAX_BLOCKS "
+                                   "(1048576); split the batch or use mt_tx_chain");
+    if (targets_d && !levels_d)
+        return set_err(MT_ERR_ARG, "mt_tx_chain_noalloc: targets need levels");
+    if (n == 0) return MT_OK;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState& st = g_dev[dev];
+    if (!st.scratch_d) return set_err(MT_ERR_NOT_INIT, "mt_tx_chain_noalloc: no scratch address");
+    uint32_t* base = st.scratch_d;
+    return enqueue_chain(base, base + kNoAllocSlots, base + 2 * kNoAllocSlots,
+                         base + 3 * kNoAllocSlots, st.cus, blocks_d, n, pool_d, res_d, levels_d,
+                         targets_d, recon_d, nullptr, (hipStream_t)stream, false);
 }
 
 extern "C" int mt_profile_begin(void) {

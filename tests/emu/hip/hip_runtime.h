@@ -26,6 +26,7 @@
 #define __constant__
 #define __launch_bounds__(...)
 #define HIP_SYMBOL(x) (&(x))
+#define __align__(n) alignas(n)
 
 struct dim3 {
     unsigned x, y, z;
@@ -58,6 +59,10 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
     return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = (void*)sym;
+    return hipSuccess;
+}
 inline hipError_t hipMemcpyToSymbol(const void* sym, const void* src, size_t n) {
     memcpy((void*)sym, src, n);
     return hipSuccess;
