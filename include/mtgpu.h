@@ -186,6 +186,36 @@ int mt_rate_default_cdfs(int qctx, mt_coef_cdfs* out);
  * device; synchronous, and no rate evaluation may be in flight on the device meanwhile. */
 int mt_rate_load_cdfs(int slot, const mt_coef_cdfs* cdfs);
 
+/* ---- trellis quantization ---------------------------------------------------------- */
+/* One transform block for mt_trellis: 24 bytes.  The quantizer is the chain's (qidx, bd_qm
+ * as in mt_txblk, the dc/ac qindex deltas with MT_TXBLK_CHROMA); its dc / ac steps are the
+ * trellis's dc_q / ac_q.  lambda0 is the reference's trellis lambda0 at the call site
+ * (trellis_lambda() times hbd_trellis_scale and the perceptual factor; mt_trellis_lambda0
+ * gives the first two); <= 0 leaves the block as it is.  The context variant prices against
+ * the CDFs of snapshot qctx with the eob_bin CDF, dc-sign context and plane as in
+ * mt_rateblk; band_tilt is Cdfs::band_tilt (0 off). */
+enum { MT_TRELLIS_CTX = 1 };
+typedef struct {
+    uint32_t offset;        /* the block's first level / target / workspace element (x-major) */
+    uint8_t tx_size, qidx, bd_qm, flags;
+    int8_t dc_delta, ac_delta;
+    uint8_t qctx, eob_bin, dcs_ctx, plane, pad0, pad1;
+    float lambda0, band_tilt;
+} mt_trellisblk;
+/* Trellis quantization of n blocks in place: levels_d (int32) from the f32 targets targets_d
+ * of the forward transform (both x-major at each block's offset, ncoef per block; 32x32 for
+ * 64x64).  Without MT_TRELLIS_CTX: trellis_optimize (src/trellis.rs:944) with the x86
+ * AVX2+FMA round-down (src/avx/trellis.rs:196); with it: trellis_optimize_ctx
+ * (src/trellis.rs:55) on the shipped switches.  work_d: a float workspace laid out like
+ * levels_d (overwritten).  status_d (optional, n int32): 0, or MT_ERR_ARG for an invalid
+ * descriptor (its levels untouched). */
+int mt_trellis(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
+               float* work_d, int32_t* status_d, void* stream);
+/* The reference's trellis lambda0 of a frame: trellis_lambda() (src/cost.rs:110) times
+ * hbd_trellis_scale (src/coder/lossy_state.rs:158, shipped tuning). sub: 0 4:4:4, 1 4:2:2,
+ * 2 4:2:0. */
+float mt_trellis_lambda0(int qidx, int bd, int sub);
+
 /* ---- planes and intra prediction -------------------------------------------------- */
 /* A plane of uint16 samples in device memory.  width/height: the frame size the intra
  * edge rules clip to (predict_nd_ad's fw/fh, src/intrapred.rs:359); every block must lie

@@ -110,6 +110,9 @@ def lib():
         L.mt_inv_txfm.argtypes = [i32, i32, i32, vp, vp, vp, vp]
         L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_tx_chain_noalloc.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+        L.mt_trellis.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.mt_trellis_lambda0.argtypes = [i32, i32, i32]
+        L.mt_trellis_lambda0.restype = ctypes.c_float
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
@@ -245,6 +248,24 @@ def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=No
 
 
 NOALLOC_MAX_BLOCKS = 1 << 20
+TRELLIS_CTX = 1
+TRELLISBLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("qidx", "u1"), ("bd_qm", "u1"),
+                             ("flags", "u1"), ("dc_delta", "i1"), ("ac_delta", "i1"), ("qctx", "u1"),
+                             ("eob_bin", "u1"), ("dcs_ctx", "u1"), ("plane", "u1"), ("pad0", "u1"),
+                             ("pad1", "u1"), ("lambda0", "<f4"), ("band_tilt", "<f4")])
+
+
+def trellis(blocks, levels, targets, work, status=None, stream=None):
+    """mt_trellis: blocks uint8 cuda tensor of TRELLISBLK_DTYPE records; levels int32 (rewritten
+    in place), targets float32 and work float32 cuda tensors laid out alike."""
+    n = blocks.numel() // TRELLISBLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    _check(lib().mt_trellis(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(targets), _dev_ptr(work),
+                            _dev_ptr(status) if status is not None else None, _stream_ptr(stream)))
+
+
+def trellis_lambda0(qidx, bd, sub):
+    return float(lib().mt_trellis_lambda0(qidx, bd, sub))
 
 
 def tx_chain_noalloc(blocks, pool, res, levels=None, targets=None, recon=None, stream=None):
@@ -259,7 +280,7 @@ def tx_chain_noalloc(blocks, pool, res, levels=None, targets=None, recon=None, s
 
 PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",
                  "16x32", "32x16", "32x32", "64x64", "rate", "intra_pred", "distortion", "eval",
-                 "rdo_decide", "rdo_recon"]
+                 "rdo_decide", "rdo_recon", "trellis"]
 
 
 def profile_begin():

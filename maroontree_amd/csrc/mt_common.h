@@ -81,7 +81,8 @@ enum {
     kProfEval = 17,
     kProfRdo = 18,     // the RDO's candidate generation, argmin and decision kernels
     kProfRecon = 19,   // the RDO's superblock reconstruction
-    kProfSlots = 20
+    kProfTrellis = 20,
+    kProfSlots = 21
 };
 // While profiling is on (mt_profile_begin), brackets a launch with HIP events on `s`.
 struct ProfScope {

@@ -141,6 +141,17 @@ float orc_skip_cost(int qctx, int cls, int ctx, int s);
 /* the scan table of an AV1 TxSize (64x64 uses the 32x32 scan); NULL if none */
 const uint32_t* orc_scan(int tx_size);
 
+/* ---- trellis quantization (src/trellis.rs) ---------------------------------------------
+ * cf (levels, x-major) is rewritten in place from the f32 targets tf.  orc_trellis is
+ * trellis_optimize (:944) with the x86 AVX2+FMA round-down; orc_trellis_ctx is
+ * trellis_optimize_ctx (:55) on the shipped switches, pricing against the CDFs of snapshot
+ * c->qctx with c->cls / plane / w / h / eob_bin / dcs_ctx (w, h: the coded block, 32 x 32
+ * for 64 x 64). */
+void orc_trellis(int32_t* cf, const float* tf, float dc_q, float ac_q, const uint32_t* scan, int n,
+                 float lambda0);
+void orc_trellis_ctx(int32_t* cf, const float* tf, float dc_q, float ac_q, const uint32_t* scan,
+                     float lambda0, const orc_rate_ctx* c, int qm_level, float band_tilt);
+
 
 /* ---- intra prediction (src/intrapred.rs) ------------------------------------------- */
 /* Edges are passed as arrays of 132 samples whose element 2 is logical index 0 (the

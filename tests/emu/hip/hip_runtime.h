@@ -63,8 +63,8 @@ inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
     *p = (void*)sym;
     return hipSuccess;
 }
-inline hipError_t hipMemcpyToSymbol(const void* sym, const void* src, size_t n) {
-    memcpy((void*)sym, src, n);
+inline hipError_t hipMemcpyToSymbol(const void* sym, const void* src, size_t n, size_t off = 0) {
+    memcpy((char*)sym + off, src, n);
     return hipSuccess;
 }
 
@@ -136,6 +136,11 @@ inline uint64_t __ballot(int pred) {
 }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+struct int4 { int x, y, z, w; };
+struct float2 { float x, y; };
+inline float2 make_float2(float x, float y) { return float2{x, y}; }
+struct float4 { float x, y, z, w; };
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fmul_rn(float a, float b) { return a * b; }
 inline double __dadd_rn(double a, double b) { return a + b; }

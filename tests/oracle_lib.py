@@ -157,6 +157,12 @@ def _rate_sigs(L):
     L.orc_skip_cost.restype = ctypes.c_float
     L.orc_scan.argtypes = [ctypes.c_int]
     L.orc_scan.restype = P(ctypes.c_uint32)
+    f32 = ctypes.c_float
+    L.orc_trellis.argtypes = [P(ctypes.c_int32), P(f32), f32, f32, P(ctypes.c_uint32), ctypes.c_int, f32]
+    L.orc_trellis.restype = None
+    L.orc_trellis_ctx.argtypes = [P(ctypes.c_int32), P(f32), f32, f32, P(ctypes.c_uint32), f32, P(OrcRateCtx),
+                                  ctypes.c_int, f32]
+    L.orc_trellis_ctx.restype = None
 
 
 def scan(tx_size):
@@ -184,6 +190,35 @@ def block_rate(cf, scan_tab, ctx, bound=float("inf")):
     cf = np.ascontiguousarray(cf, dtype=np.int32)
     sc = np.ascontiguousarray(scan_tab, dtype=np.uint32)
     return L.orc_block_rate(ptr(cf, ctypes.c_int32), ptr(sc, ctypes.c_uint32), ctypes.byref(ctx), bound)
+
+
+def trellis(cf, tf, dc_q, ac_q, scan_tab, lambda0):
+    """trellis_optimize on a copy of cf (x-major levels) with targets tf; returns the levels."""
+    L = lib()
+    if not getattr(L, "_rate_sigs", False):
+        _rate_sigs(L)
+        L._rate_sigs = True
+    out = np.array(cf, dtype=np.int32, copy=True)
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    sc = np.ascontiguousarray(scan_tab, dtype=np.uint32)
+    L.orc_trellis(ptr(out, ctypes.c_int32), ptr(tf, ctypes.c_float), float(dc_q), float(ac_q),
+                  ptr(sc, ctypes.c_uint32), len(sc), float(lambda0))
+    return out
+
+
+def trellis_ctx(cf, tf, dc_q, ac_q, scan_tab, lambda0, ctx, qm_level=15, band_tilt=0.0):
+    """trellis_optimize_ctx on a copy of cf; ctx an OrcRateCtx (qctx, cls, plane, w, h,
+    eob_bin, dcs_ctx used)."""
+    L = lib()
+    if not getattr(L, "_rate_sigs", False):
+        _rate_sigs(L)
+        L._rate_sigs = True
+    out = np.array(cf, dtype=np.int32, copy=True)
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    sc = np.ascontiguousarray(scan_tab, dtype=np.uint32)
+    L.orc_trellis_ctx(ptr(out, ctypes.c_int32), ptr(tf, ctypes.c_float), float(dc_q), float(ac_q),
+                      ptr(sc, ctypes.c_uint32), float(lambda0), ctypes.byref(ctx), int(qm_level), float(band_tilt))
+    return out
 
 
 def skip_cost(qctx, cls, ctx, s):

@@ -1,0 +1,63 @@
+"""mt_trellis on the GPU against the oracle's trellis (trellis_optimize and
+trellis_optimize_ctx of the reference's src/trellis.rs), level for level: random blocks of
+every reference size, transform type, bit depth, luma and chroma quantizers, QM, CDF
+snapshot, dc-sign context, lambda and band tilt, with the forward transform's own levels and
+targets; invalid descriptors; the reference's own random blocks at 16x16."""
+import numpy as np
+import pytest
+import torch
+
+import maroontree_amd as mt
+import oracle_lib as O
+import trellisgen as TG
+
+pytestmark = pytest.mark.gpu
+
+
+def run(dev, desc, levels, targets):
+    b = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+    lv = torch.from_numpy(levels.copy()).to(dev)
+    tg = torch.from_numpy(targets).to(dev)
+    wk = torch.full_like(tg, float("nan"))
+    st = torch.full((len(desc),), 7, dtype=torch.int32, device=dev)
+    mt.trellis(b, lv, tg, wk, st)
+    torch.cuda.synchronize()
+    return lv.cpu().numpy(), st.cpu().numpy()
+
+
+@pytest.mark.parametrize("seed,n", [(1, 4000), (2, 4000), (3, 20000)])
+def test_trellis_matches_oracle(dev, seed, n):
+    desc, levels, targets, want = TG.make(seed, n)
+    print(f"trellis seed {seed}: {n} blocks", flush=True)
+    got, st = run(dev, desc, levels, targets)
+    assert not st.any()
+    diff = np.nonzero(got != want)[0]
+    assert len(diff) == 0, f"{len(diff)} levels differ, first at {diff[:5]}"
+    assert (want != levels).sum() > 0
+
+
+def test_trellis_invalid_descriptors(dev):
+    desc, levels, targets, want = TG.make(9, 50)
+    desc[3]["tx_size"] = 12  # 64x32: no reference transform
+    desc[7]["flags"] = 0x81
+    desc[11]["bd_qm"] = 3 << 4  # no such bit depth
+    i = int(np.nonzero(desc["flags"] & mt.TRELLIS_CTX)[0][-1])
+    desc[i]["eob_bin"] = 16 if desc[i]["eob_bin"] != 16 else 0  # not the block's coefficient count
+    got, st = run(dev, desc, levels, targets)
+    bad = sorted({3, 7, 11, i})
+    assert list(np.nonzero(st)[0]) == bad and (st[bad] == -1).all()
+    for k in bad:
+        o = int(desc[k]["offset"])
+        e = int(desc[k + 1]["offset"]) if k + 1 < len(desc) else len(levels)
+        assert (got[o:e] == levels[o:e]).all()
+
+
+@pytest.mark.gpu
+def test_trellis_lambda0_law():
+    """trellis_lambda() * hbd_trellis_scale: 0.05 at 8 bits and at 4:4:4; above 8 bits at
+    4:2:0 the scale ramps from 1 at qindex 36 to 0.174 at 20 and below."""
+    assert mt.trellis_lambda0(10, 8, 2) == np.float32(0.05)
+    assert mt.trellis_lambda0(10, 10, 0) == np.float32(0.05)
+    assert mt.trellis_lambda0(40, 10, 2) == np.float32(0.05)
+    assert abs(mt.trellis_lambda0(10, 10, 2) - 0.05 * 0.174) < 1e-7
+    assert abs(mt.trellis_lambda0(28, 12, 2) - 0.05 * (1 + (0.174 - 1) * 0.5)) < 1e-7
