@@ -211,6 +211,16 @@ typedef struct {
  * descriptor (its levels untouched). */
 int mt_trellis(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
                float* work_d, int32_t* status_d, void* stream);
+/* The chain with the trellis between quantization and dequantization, as the reference's
+ * candidate loops run it at Speed::Slow (src/coder/partition_search.rs:509-513): forward +
+ * quantization, mt_trellis with trellis_d[i] (the same offset, size and quantizer as
+ * blocks_d[i]), then dequantization + inverse + SSE / nnz / checksum of the final levels as
+ * mt_tx_chain reports them.  levels_d, targets_d and recon_d are required (recon_d also
+ * serves as the trellis's workspace); blocks of an unsupported (size, type) get sse = -1
+ * and unspecified levels. */
+int mt_tx_chain_trellis(const mt_txblk* blocks_d, const mt_trellisblk* trellis_d, int n,
+                        const int16_t* pool_d, mt_txres* res_d, int32_t* levels_d, float* targets_d,
+                        int32_t* recon_d, void* stream);
 /* The reference's trellis lambda0 of a frame: trellis_lambda() (src/cost.rs:110) times
  * hbd_trellis_scale (src/coder/lossy_state.rs:158, shipped tuning). sub: 0 4:4:4, 1 4:2:2,
  * 2 4:2:0. */

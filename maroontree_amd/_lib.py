@@ -111,6 +111,7 @@ def lib():
         L.mt_tx_chain.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_tx_chain_noalloc.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_trellis.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.mt_tx_chain_trellis.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp]
         L.mt_trellis_lambda0.argtypes = [i32, i32, i32]
         L.mt_trellis_lambda0.restype = ctypes.c_float
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
@@ -262,6 +263,16 @@ def trellis(blocks, levels, targets, work, status=None, stream=None):
     init(blocks.device.index or 0)
     _check(lib().mt_trellis(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(targets), _dev_ptr(work),
                             _dev_ptr(status) if status is not None else None, _stream_ptr(stream)))
+
+
+def tx_chain_trellis(blocks, trellis_blocks, pool, res, levels, targets, recon, stream=None):
+    """mt_tx_chain_trellis: the chain with the trellis between quantization and
+    dequantization; trellis_blocks: uint8 cuda tensor of TRELLISBLK_DTYPE records (one per
+    block); levels / targets / recon required (laid out like the pool)."""
+    n = blocks.numel() // BLK_DTYPE.itemsize
+    init(blocks.device.index or 0)
+    _check(lib().mt_tx_chain_trellis(_dev_ptr(blocks), _dev_ptr(trellis_blocks), n, _dev_ptr(pool), _dev_ptr(res),
+                                     _dev_ptr(levels), _dev_ptr(targets), _dev_ptr(recon), _stream_ptr(stream)))
 
 
 def trellis_lambda0(qidx, bd, sub):

@@ -302,7 +302,17 @@ template <int L> __device__ inline void inv_line(int kind, int32_t* v, Clip c) {
 }
 
 // ------------------------------------------------------------------------- tile kernel
-enum { MODE_CHAIN = 0, MODE_FWD = 1, MODE_INV = 2 };
+// MODE_CHAIN: the whole chain of a bucketed batch; MODE_FWD / MODE_INV: one class of
+// contiguous blocks (the parity entry points); MODE_CFWD / MODE_CINV: the chain split around a
+// pass that rewrites the levels (the trellis): the forward half writes levels and targets at
+// each block's pool offset, the inverse half reads them back and measures like MODE_CHAIN.
+enum { MODE_CHAIN = 0, MODE_FWD = 1, MODE_INV = 2, MODE_CFWD = 3, MODE_CINV = 4 };
+template <int MODE> struct ModeTraits {
+    static constexpr bool bucketed = MODE == MODE_CHAIN || MODE == MODE_CFWD || MODE == MODE_CINV;
+    static constexpr bool fwd = MODE != MODE_INV && MODE != MODE_CINV;
+    static constexpr bool inv = MODE != MODE_FWD && MODE != MODE_CFWD;
+    static constexpr bool measure = MODE == MODE_CHAIN || MODE == MODE_CINV;
+};
 
 struct KArgs {
     // chain mode
@@ -326,11 +336,12 @@ struct KArgs {
 template <int W, int H, int MODE>
 __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx) {
     using R = TxRules<W, H>;
+    using M = ModeTraits<MODE>;
     constexpr int N = R::N, B = R::B, RS = R::RS, BS = R::BS;
     constexpr int NT = 256;
 
     __shared__ int32_t work[B * BS];
-    __shared__ int16_t in16[MODE == MODE_CHAIN ? B * N : 1];
+    __shared__ int16_t in16[M::measure ? B * N : 1];
     __shared__ BlkParam prm[B];
     __shared__ uint32_t gidx[B];      // global block index (chain) or block number (parity)
     __shared__ uint32_t goff[B];      // element offset of the block in the pool / arrays
@@ -342,13 +353,13 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
 
     // virtual tile space: chain mode walks the 7 type classes of this size
     __shared__ uint32_t s_cnt[7], s_st[7];
-    if (MODE == MODE_CHAIN && tid < 7) {
+    if (M::bucketed && tid < 7) {
         s_cnt[tid] = a.cls_count[size_slot_idx * 7 + tid];
         s_st[tid] = a.cls_start[size_slot_idx * 7 + tid];
     }
     __syncthreads();
     int total_tiles = 0;
-    if (MODE == MODE_CHAIN) {
+    if (M::bucketed) {
 #pragma unroll
         for (int t = 0; t < 7; t++) total_tiles += (int)((s_cnt[t] + B - 1) / B);
     } else {
@@ -358,7 +369,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
     for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
         // ---- tile setup: which blocks, their parameters
         if (tid == 0) {
-            if (MODE == MODE_CHAIN) {
+            if (M::bucketed) {
                 int rem = tile, t = 0;
                 for (; t < 7; t++) {
                     const int nt = (int)((s_cnt[t] + B - 1) / B);
@@ -382,7 +393,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
         __syncthreads();
         for (int b = tid; b < B; b += NT) {
             if (b < nb) {
-                if (MODE == MODE_CHAIN) {
+                if (M::bucketed) {
                     const uint32_t g = a.order[base + b];
                     const mt_txblk d = a.blocks[g];
                     gidx[b] = g;
@@ -421,16 +432,16 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
         auto emit_coef = [&](int b, int fx, int fy, int32_t lvl, float tgt, uint32_t& hs, uint32_t& nz) {
             const int rc = fx * H + fy;
             const BlkParam& p = prm[b];
-            if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
+            if (MODE == MODE_FWD || MODE == MODE_CFWD || (MODE == MODE_CHAIN && a.levels)) {
                 const size_t o = (MODE == MODE_FWD ? (size_t)gidx[b] * N : (size_t)goff[b]) + rc;
                 a.levels[o] = lvl;
-                if (MODE == MODE_FWD || a.targets) a.targets[o] = tgt;
+                if (MODE == MODE_FWD || MODE == MODE_CFWD || a.targets) a.targets[o] = tgt;
             }
             if (MODE == MODE_CHAIN) {
                 hs += coef_hash(rc, lvl, tgt);
                 nz += lvl != 0;
             }
-            if (MODE != MODE_FWD) {
+            if (M::inv) {
                 int32_t step = rc == 0 ? p.dc : p.ac;
                 if (use_qm_type && p.qm_level < 15)
                     step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
@@ -451,14 +462,14 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
             emit_coef(b, fx, fy, lvl, tgt, hs, nz);
         };
 
-        if (MODE != MODE_INV) {
+        if (M::fwd) {
             // ---- load the residual into LDS (row-major, padded rows)
             for (int i = tid; i < nb * N; i += NT) {
                 const int b = i / N, p = i % N, y = p / W, x = p % W;
                 int32_t v;
-                if (MODE == MODE_CHAIN) {
+                if (M::bucketed) {
                     const int16_t s = a.pool16[(size_t)goff[b] + p];
-                    in16[b * N + p] = s;
+                    if (M::measure) in16[b * N + p] = s;
                     v = s;
                 } else {
                     v = a.in32[(size_t)gidx[b] * N + p];
@@ -594,7 +605,7 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                 }
             }
             __syncthreads();
-        } else {
+        } else if (MODE == MODE_INV) {
             // ---- inverse-only: dequantize the given levels into LDS
             const bool use_qm = use_qm_type;
             for (int i = tid; i < nb * N; i += NT) {
@@ -609,9 +620,39 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                 work[b * BS + fy * RS + fx] = c;
             }
             __syncthreads();
+        } else {
+            // ---- the split chain's inverse half: the residual for the distortion, and the
+            // (rewritten) levels with their targets from the pool, one coefficient column
+            // (fx) per line so the checksum and nonzero count reach LDS once per line
+            for (int i = tid; i < nb * N; i += NT) {
+                const int b = i / N, p = i % N;
+                in16[b * N + p] = a.pool16[(size_t)goff[b] + p];
+            }
+            const bool use_qm = use_qm_type;
+            for (int line = tid; line < nb * W; line += NT) {
+                const int b = line / W, fx = line % W;
+                const BlkParam& p = prm[b];
+                uint32_t hs = 0, nz = 0;
+                for (int fy = 0; fy < H; fy++) {
+                    const int rc = fx * H + fy;
+                    const size_t o = (size_t)goff[b] + rc;
+                    const int32_t lvl = a.levels[o];
+                    hs += coef_hash(rc, lvl, a.targets[o]);
+                    nz += lvl != 0;
+                    int32_t step = rc == 0 ? p.dc : p.ac;
+                    if (use_qm && p.qm_level < 15)
+                        step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
+                    int32_t c = dequant(lvl, step, R::dq_shift, p.cf_max);
+                    if (R::rect2) c = (c * 181 + 128) >> 8;
+                    work[b * BS + fy * RS + fx] = c;
+                }
+                atomicAdd(&s_hash[b], hs);
+                if (nz) atomicAdd(&s_nnz[b], nz);
+            }
+            __syncthreads();
         }
 
-        if (MODE != MODE_FWD) {
+        if (M::inv) {
             // inverse kinds: ADST_DCT is vertical ADST (column pass)
             int rowk = KD, colk = KD;
             if (type == MT_TXTP_ADST_ADST) { rowk = KA; colk = KA; }
@@ -664,14 +705,14 @@ __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx
                         hsh += pix_hash(j, r);
                     }
                 }
-                if (MODE == MODE_CHAIN) {
+                if (M::measure) {
                     atomicAdd(&s_sse[b], (unsigned long long)sse);
                     atomicAdd(&s_hash[b], hsh);
                 }
             }
         }
         __syncthreads();
-        if (MODE == MODE_CHAIN) {
+        if (M::measure) {
             for (int b = tid; b < nb; b += NT) {
                 mt_txres r;
                 r.sse = (int64_t)s_sse[b];
@@ -705,15 +746,16 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
     __shared__ int32_t coef[32 * 33];       // dequantized coded coefficients [fy][fx]
     __shared__ unsigned long long s_sse;
     __shared__ uint32_t s_nnz, s_hash;
+    using M = ModeTraits<MODE>;
     int32_t* rowbuf = reinterpret_cast<int32_t*>(vt);  // 32 x 64 int32 after the forward
     const int lane = threadIdx.x;
-    const int nblocks = MODE == MODE_CHAIN ? (int)a.cls_count[12 * 7] : a.n;
-    const uint32_t chain_start = MODE == MODE_CHAIN ? a.cls_start[12 * 7] : 0;
+    const int nblocks = M::bucketed ? (int)a.cls_count[12 * 7] : a.n;
+    const uint32_t chain_start = M::bucketed ? a.cls_start[12 * 7] : 0;
 
     for (int blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
         BlkParam prm;
         uint32_t g, off;
-        if (MODE == MODE_CHAIN) {
+        if (M::bucketed) {
             g = a.order[chain_start + blk];
             const mt_txblk d = a.blocks[g];
             off = d.offset;
@@ -731,7 +773,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
         uint32_t nnz = 0, hsh = 0;
         const bool qm = prm.qm_level < 15;
         const uint8_t* qmrow = &g_iqm[qm ? prm.qm_level : 0][prm.qm_chroma][336];
-        if (MODE != MODE_INV) {
+        if (M::fwd) {
             // Vertical pass: lane x runs the 32 chains of column x over y ascending; the
             // residual is staged in LDS as int16 (the chain's input type; the parity entry
             // point reads its int32 input directly), the basis arrives wave-uniform through
@@ -739,7 +781,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             // followed by the horizontal pass of those rows (lane (u, parity) runs 8 chains
             // over x ascending on LDS broadcasts).
             int16_t* in16 = reinterpret_cast<int16_t*>(vt);
-            if (MODE == MODE_CHAIN) {
+            if (M::bucketed) {
 #pragma unroll 16
                 for (int y = 0; y < 64; y++) in16[y * 64 + lane] = a.pool16[(size_t)off + y * 64 + lane];
                 __syncthreads();
@@ -749,7 +791,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             for (int j = 0; j < 32; j++) acc[j] = 0.0;
 #pragma unroll 1
             for (int y = 0; y < 64; y++) {
-                const double c = MODE == MODE_CHAIN ? (double)in16[y * 64 + lane]
+                const double c = M::bucketed ? (double)in16[y * 64 + lane]
                                                     : (double)a.in32[(size_t)g * 4096 + y * 64 + lane];
                 // the row's 32 basis values in one batch of scalar loads (one wait per row)
                 double bv[32];
@@ -804,10 +846,10 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                     float tgt = q16_target(coeff, m);
                     const int iwt = qm ? qmrow[rc] : 32;
                     if (qm) lvl = qm_apply(tgt, iwt);
-                    if (MODE == MODE_FWD || (MODE == MODE_CHAIN && a.levels)) {
+                    if (MODE == MODE_FWD || MODE == MODE_CFWD || (MODE == MODE_CHAIN && a.levels)) {
                         const size_t o = (MODE == MODE_FWD ? (size_t)g * 1024 : (size_t)off) + rc;
                         a.levels[o] = lvl;
-                        if (MODE == MODE_FWD || a.targets) a.targets[o] = tgt;
+                        if (MODE == MODE_FWD || MODE == MODE_CFWD || a.targets) a.targets[o] = tgt;
                     }
                     if (MODE == MODE_CHAIN) {
                         int32_t step = rc == 0 ? prm.dc : prm.ac;
@@ -824,11 +866,19 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                 const int u = rc >> 5, v = rc & 31;
                 int32_t step = rc == 0 ? prm.dc : prm.ac;
                 if (qm) step = (step * (int32_t)qmrow[rc] + 16) >> 5;
-                coef[v * 33 + u] = dequant(a.in32[(size_t)g * 1024 + rc], step, 2, prm.cf_max);
+                int32_t lvl;
+                if (MODE == MODE_CINV) {  // the split chain's inverse half: the pool's levels
+                    lvl = a.levels[(size_t)off + rc];
+                    hsh += coef_hash(rc, lvl, a.targets[(size_t)off + rc]);
+                    nnz += lvl != 0;
+                } else {
+                    lvl = a.in32[(size_t)g * 1024 + rc];
+                }
+                coef[v * 33 + u] = dequant(lvl, step, 2, prm.cf_max);
             }
             __syncthreads();
         }
-        if (MODE != MODE_FWD) {
+        if (M::inv) {
             // row pass over the 32 coded rows, inter-pass (t + 2) >> 2 clip
             if (lane < 32) {
                 const int y = lane;
@@ -871,7 +921,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
                             hsh += pix_hash(j, r);
                         }
                     }
-                    if (MODE == MODE_CHAIN) {
+                    if (M::measure) {
                         atomicAdd(&s_sse, (unsigned long long)sse);
                         atomicAdd(&s_hash, hsh);
                         atomicAdd(&s_nnz, nnz);
@@ -880,7 +930,7 @@ __global__ void __launch_bounds__(64) tx64_kernel(KArgs a) {
             }
         }
         __syncthreads();
-        if (MODE == MODE_CHAIN && lane == 0) {
+        if (M::measure && lane == 0) {
             mt_txres r;
             r.sse = (int64_t)s_sse;
             r.nnz = s_nnz;
@@ -1349,10 +1399,12 @@ extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_
 namespace {
 // The chain's launches on a given workspace: counts / starts / cursor hold kClasses words,
 // order n words.  Shared by the library-workspace and the allocation-free entry points.
+// trellis_d (optional): the chain runs forward + quantization for every block, the trellis
+// over the levels (its workspace: recon_d), then dequantization + inverse + measurement.
 int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t* order, int cus,
                   const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                   int32_t* levels_d, float* targets_d, int32_t* recon_d, const int8_t* qdelta_d,
-                  hipStream_t s, bool profile) {
+                  hipStream_t s, bool profile, const mt_trellisblk* trellis_d = nullptr) {
     int rc;
     {
     ProfScope ps(profile ? 0 : -1, s);
@@ -1385,8 +1437,17 @@ int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t
         const int sz = kSlotSize[slot];
         const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
         ProfScope ps(profile ? 1 + slot : -1, s);
-        rc = launch_size<MODE_CHAIN>(sz, a, tiles, cus, s);
+        rc = trellis_d ? launch_size<MODE_CFWD>(sz, a, tiles, cus, s) : launch_size<MODE_CHAIN>(sz, a, tiles, cus, s);
         if (rc) return rc;
+    }
+    if (!trellis_d) return MT_OK;
+    if ((rc = mtlib::trellis_launch(trellis_d, n, levels_d, targets_d, reinterpret_cast<float*>(recon_d), nullptr, s)))
+        return rc;
+    for (int slot = 0; slot < 13; slot++) {
+        const int sz = kSlotSize[slot];
+        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
+        ProfScope ps(profile ? 1 + slot : -1, s);
+        if ((rc = launch_size<MODE_CINV>(sz, a, tiles, cus, s))) return rc;
     }
     return MT_OK;
 }
@@ -1394,7 +1455,7 @@ int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t
 
 int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                            int32_t* levels_d, float* targets_d, int32_t* recon_d,
-                           const int8_t* qdelta_d, hipStream_t stream) {
+                           const int8_t* qdelta_d, hipStream_t stream, const mt_trellisblk* trellis_d) {
     int dev;
     int rc = current_device(&dev);
     if (rc) return rc;
@@ -1402,13 +1463,23 @@ int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_
         return set_err(MT_ERR_ARG, "mt_tx_chain: bad pointer or count");
     if (targets_d && !levels_d)
         return set_err(MT_ERR_ARG, "mt_tx_chain: targets need levels");
+    if (trellis_d && (!levels_d || !targets_d || !recon_d))
+        return set_err(MT_ERR_ARG, "mt_tx_chain_trellis: levels, targets and recon are required");
     if (n == 0) return MT_OK;
     std::lock_guard<std::mutex> lk(g_mu);
     DeviceState& st = g_dev[dev];
     rc = ensure_order(st, (size_t)n);
     if (rc) return rc;
     return enqueue_chain(st.counts, st.starts, st.cursor, st.order, st.cus, blocks_d, n, pool_d,
-                         res_d, levels_d, targets_d, recon_d, qdelta_d, stream, true);
+                         res_d, levels_d, targets_d, recon_d, qdelta_d, stream, true, trellis_d);
+}
+
+extern "C" int mt_tx_chain_trellis(const mt_txblk* blocks_d, const mt_trellisblk* trellis_d, int n,
+                                   const int16_t* pool_d, mt_txres* res_d, int32_t* levels_d,
+                                   float* targets_d, int32_t* recon_d, void* stream) {
+    if (n > 0 && !trellis_d) return set_err(MT_ERR_ARG, "mt_tx_chain_trellis: no trellis descriptors");
+    return mtlib::tx_chain_launch(blocks_d, n, pool_d, res_d, levels_d, targets_d, recon_d, nullptr,
+                                  (hipStream_t)stream, trellis_d);
 }
 
 extern "C" int mt_tx_chain_noalloc(const mt_txblk* blocks_d, int n, const int16_t* pool_d,

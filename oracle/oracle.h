@@ -151,6 +151,21 @@ void orc_trellis(int32_t* cf, const float* tf, float dc_q, float ac_q, const uin
                  float lambda0);
 void orc_trellis_ctx(int32_t* cf, const float* tf, float dc_q, float ac_q, const uint32_t* scan,
                      float lambda0, const orc_rate_ctx* c, int qm_level, float band_tilt);
+/* Trellis parameters of one block, the library's mt_trellisblk layout. */
+typedef struct {
+    uint32_t offset;
+    uint8_t tx_size, qidx, bd_qm, flags;
+    int8_t dc_delta, ac_delta;
+    uint8_t qctx, eob_bin, dcs_ctx, plane, pad0, pad1;
+    float lambda0, band_tilt;
+} orc_trellisblk;
+/* The chain with the trellis between quantization and dequantization: per block i the
+ * chain of orc_chain_batch, with trellis tr[i] (flags bit 0: the context variant) applied
+ * to the levels before the inverse; the checksum covers the final levels. */
+int orc_chain_batch_trellis(int n, const uint8_t* tx_size, const uint8_t* tx_type, const uint8_t* qidx,
+                            const uint8_t* bd, const uint8_t* qm_level, const uint64_t* offset,
+                            const int16_t* pool, const orc_trellisblk* tr, int64_t* sse_out,
+                            uint32_t* hash_out, int nthreads);
 
 
 /* ---- intra prediction (src/intrapred.rs) ------------------------------------------- */

@@ -692,3 +692,70 @@ extern "C" int orc_chain_batch(int n, const uint8_t* tx_size, const uint8_t* tx_
     for (auto& x : th) x.join();
     return 0;
 }
+
+// The chain with the trellis (src/coder/partition_search.rs:509-513 order: forward +
+// quantization, trellis_optimize[_ctx], dequantization + inverse, SSE).
+extern "C" int orc_chain_batch_trellis(int n, const uint8_t* tx_size, const uint8_t* tx_type,
+                                       const uint8_t* qidx, const uint8_t* bd, const uint8_t* qm_level,
+                                       const uint64_t* offset, const int16_t* pool, const orc_trellisblk* tr,
+                                       int64_t* sse_out, uint32_t* hash_out, int nthreads) {
+    auto work = [&](int begin, int end) {
+        int32_t in[4096], lv[1024], rec[4096], pred[4096];
+        float tg[1024];
+        uint16_t src[4096];
+        for (int i = begin; i < end; i++) {
+            orc_quant q;
+            if (qm_level[i] & 0x40)
+                orc_quant_new_chroma(qidx[i], 0, 0, bd[i], qm_level[i] & 15, &q);
+            else
+                orc_quant_new(qidx[i], bd[i], qm_level[i] & 15, &q);
+            const int s = tx_size[i], t = tx_type[i];
+            if (!orc_tx_supported(s, t)) {
+                sse_out[i] = -1;
+                if (hash_out) hash_out[i] = 0;
+                continue;
+            }
+            const int W = orc_tx_w(s), H = orc_tx_h(s), N = W * H, NC = orc_tx_ncoef(s);
+            const int P = 1 << (q.bd - 1), maxv = (1 << q.bd) - 1;
+            const int16_t* r = pool + offset[i];
+            for (int k = 0; k < N; k++) {
+                in[k] = r[k];
+                pred[k] = P;
+                src[k] = (uint16_t)clip(P + r[k], 0, maxv);
+            }
+            orc_fwd_txfm(s, t, in, &q, lv, tg);
+            const orc_trellisblk& d = tr[i];
+            const int cw = W < 32 ? W : 32, ch = H < 32 ? H : 32;
+            if (d.flags & 1) {
+                orc_rate_ctx c{};
+                c.qctx = d.qctx;
+                c.cls = ((31 - __builtin_clz(W)) - 2 + (31 - __builtin_clz(H)) - 2 + 1) >> 1;
+                c.plane = d.plane;
+                c.w = cw;
+                c.h = ch;
+                c.eob_bin = d.eob_bin;
+                c.dcs_ctx = d.dcs_ctx;
+                orc_trellis_ctx(lv, tg, (float)q.dc, (float)q.ac, orc_scan(s), d.lambda0, &c, q.qm_level,
+                                d.band_tilt);
+            } else {
+                orc_trellis(lv, tg, (float)q.dc, (float)q.ac, orc_scan(s), NC, d.lambda0);
+            }
+            orc_inv_txfm(s, t, lv, &q, rec);
+            sse_out[i] = orc_sse_recon(pred, rec, src, W, W, H, maxv);
+            if (hash_out) hash_out[i] = orc_block_hash(NC, lv, tg, N, rec);
+        }
+    };
+    if (nthreads <= 1) {
+        work(0, n);
+        return 0;
+    }
+    std::vector<std::thread> th;
+    const int chunk = (n + nthreads - 1) / nthreads;
+    for (int k = 0; k < nthreads; k++) {
+        const int b = k * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back(work, b, e);
+    }
+    for (auto& x : th) x.join();
+    return 0;
+}

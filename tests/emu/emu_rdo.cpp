@@ -49,6 +49,9 @@ namespace mtlib {
 int rate_init_device(int) { return MT_OK; }
 int pred_init_device(int) { return MT_OK; }
 int dist_init_device(int) { return MT_OK; }
+int trellis_launch(const mt_trellisblk*, int, int32_t*, const float*, float*, int32_t*, hipStream_t) {
+    return MT_ERR_ARG;  // the trellis kernel is not part of this harness
+}
 float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s) { return orc_cdf_cost_raw(raw_cdf, nsym, s); }
 
 static_assert(sizeof(mt_cand) == sizeof(orc_cand), "cand");

@@ -58,6 +58,9 @@ def lib():
         L.orc_chain_batch.argtypes = [ctypes.c_int, u8p, u8p, u8p, u8p, u8p, P(ctypes.c_uint64),
                                       P(ctypes.c_int16), P(ctypes.c_int64), P(ctypes.c_uint32),
                                       ctypes.c_int]
+        L.orc_chain_batch_trellis.argtypes = [ctypes.c_int, u8p, u8p, u8p, u8p, u8p, P(ctypes.c_uint64),
+                                              P(ctypes.c_int16), ctypes.c_void_p, P(ctypes.c_int64),
+                                              P(ctypes.c_uint32), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -135,6 +138,24 @@ def chain_batch(sizes, types, qidx, bd, qm, offsets, pool, nthreads=8, with_hash
     lib().orc_chain_batch(n, *(ptr(x, u8) for x in a), ptr(offsets, ctypes.c_uint64),
                           ptr(pool, ctypes.c_int16), ptr(sse, ctypes.c_int64),
                           ptr(h, ctypes.c_uint32) if with_hash else None, nthreads)
+    return sse, h
+
+
+def chain_batch_trellis(sizes, types, qidx, bd, qm, offsets, pool, trellis, nthreads=8):
+    """orc_chain_batch_trellis: trellis is a structured array of the library's mt_trellisblk
+    layout, one per block.  Returns (sse, hash)."""
+    n = len(sizes)
+    sse = np.zeros(n, np.int64)
+    h = np.zeros(n, np.uint32)
+    a = [np.ascontiguousarray(x, dtype=np.uint8) for x in (sizes, types, qidx, bd, qm)]
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    pool = np.ascontiguousarray(pool, dtype=np.int16)
+    tr = np.ascontiguousarray(trellis)
+    assert tr.dtype.itemsize == 24
+    u8 = ctypes.c_uint8
+    lib().orc_chain_batch_trellis(n, *(ptr(x, u8) for x in a), ptr(offsets, ctypes.c_uint64),
+                                  ptr(pool, ctypes.c_int16), tr.ctypes.data_as(ctypes.c_void_p),
+                                  ptr(sse, ctypes.c_int64), ptr(h, ctypes.c_uint32), nthreads)
     return sse, h
 
 

@@ -35,15 +35,20 @@ def bins(tmp_path_factory):
     subprocess.run(["g++", *FLAGS, "-c", os.path.join(EMU, "emu_gpu.cpp"), "-o", gpu_o], check=True)
     chain = str(d / "emu_chain")
     subprocess.run(["g++", *FLAGS, "-o", chain, os.path.join(EMU, "emu_chain.cpp"), gpu_o, "-lpthread"], check=True)
+    rate_o = str(d / "emu_rate.o")
+    subprocess.run(["g++", *FLAGS, "-c", os.path.join(EMU, "emu_rate.cpp"), "-o", rate_o], check=True)
+    chain_tr = str(d / "emu_chain_trellis")
+    subprocess.run(["g++", *FLAGS, "-o", chain_tr, os.path.join(EMU, "emu_chain_trellis.cpp"), gpu_o, rate_o,
+                    "-lpthread"], check=True)
     rdo = str(d / "emu_rdo")
     odir = os.path.join(ROOT, "oracle")
     subprocess.run(["g++", *FLAGS, "-o", rdo, os.path.join(EMU, "emu_rdo.cpp"), gpu_o, "-L" + odir, "-loracle",
                     "-lpthread", "-Wl,-rpath," + odir], check=True)
-    return d, chain, rdo
+    return d, chain, rdo, chain_tr
 
 
 def test_emulated_chain_matches_oracle_on_extreme_inputs(bins):
-    d, chain, _ = bins
+    d, chain, _, _ = bins
     n = 1400
     blocks, pool = mt.gen_tx_blocks(77, n, all_sizes=True)
     pool = pool.copy()
@@ -84,7 +89,7 @@ def test_emulated_chain_matches_oracle_on_extreme_inputs(bins):
 
 @pytest.mark.parametrize("bd,qidx,w,h,flags", [(10, 77, 128, 72, 7), (8, 160, 72, 64, 3)])
 def test_emulated_rdo_matches_oracle(bins, bd, qidx, w, h, flags):
-    d, _, rdo = bins
+    d, _, rdo, _ = bins
     img = W.synth_image(100 + qidx, w, h, bd)
     path = d / f"img_{qidx}.raw"
     img.astype("<u2").tofile(path)
@@ -97,3 +102,27 @@ def test_emulated_rdo_matches_oracle(bins, bd, qidx, w, h, flags):
     orc = lines[1].split("bits ")[1]
     assert emu == orc, (emu, orc)
     assert lines[-1] == "recon diffs 0 node diffs 0", lines[-1]
+
+
+def test_emulated_trellis_chain_matches_oracle(bins):
+    """mt_tx_chain_trellis: bucketing, the forward halves of every chain kernel, the trellis,
+    the inverse halves with their measurement; SSE and checksum of every block equal the
+    oracle's chain with the trellis."""
+    import trellisgen as TG
+    d, _, _, chain_tr = bins
+    blocks, pool = mt.gen_tx_blocks(91, 900, all_sizes=True)
+    tr = TG.chain_descriptors(blocks, 5)
+    paths = [str(d / f) for f in ("tb", "tt", "tp", "to")]
+    blocks.tofile(paths[0])
+    tr.tofile(paths[1])
+    pool.tofile(paths[2])
+    subprocess.run([chain_tr] + paths, check=True, capture_output=True)
+    got = np.fromfile(paths[3], _lib.RES_DTYPE)
+    sse, h = TG.oracle_chain(blocks, pool, tr)
+    bad = np.nonzero((got["sse"] != sse) | (got["hash"] != h))[0]
+    assert len(bad) == 0, (len(bad), bad[:5], got[bad[:3]], sse[bad[:3]])
+    plain, ph = O.chain_batch(blocks["tx_size"], blocks["tx_type"], blocks["qidx"],
+                              np.where(((blocks["bd_qm"] >> 4) & 3) == 1, 10,
+                                       np.where(((blocks["bd_qm"] >> 4) & 3) == 2, 12, 8)),
+                              blocks["bd_qm"] & 0x4F, blocks["offset"].astype(np.uint64), pool)
+    assert (ph != h).sum() > 100  # the trellis changed many blocks

@@ -61,3 +61,25 @@ def test_trellis_lambda0_law():
     assert mt.trellis_lambda0(40, 10, 2) == np.float32(0.05)
     assert abs(mt.trellis_lambda0(10, 10, 2) - 0.05 * 0.174) < 1e-7
     assert abs(mt.trellis_lambda0(28, 12, 2) - 0.05 * (1 + (0.174 - 1) * 0.5)) < 1e-7
+
+
+@pytest.mark.parametrize("seed,n,all_sizes", [(0x6D61726F6F6E, 1 << 16, False), (77, 20000, True)])
+def test_chain_with_trellis_matches_oracle(dev, seed, n, all_sizes):
+    """mt_tx_chain_trellis (forward + quantization, trellis, dequantization + inverse, SSE,
+    checksum of the final levels) against the oracle's chain with the same trellis, block
+    for block; unsupported sizes get sse = -1."""
+    blocks, pool = mt.gen_tx_blocks(seed, n, all_sizes)
+    tr = TG.chain_descriptors(blocks, seed & 0xFFFF)
+    print(f"chain+trellis seed {seed:#x}: {n} blocks", flush=True)
+    t = lambda a: torch.from_numpy(a.view(np.uint8).copy()).to(dev)  # noqa: E731
+    pt = torch.from_numpy(pool).to(dev)
+    res = torch.zeros(n * 16, dtype=torch.uint8, device=dev)
+    lv = torch.zeros(len(pool), dtype=torch.int32, device=dev)
+    tg = torch.zeros(len(pool), dtype=torch.float32, device=dev)
+    rec = torch.zeros(len(pool), dtype=torch.int32, device=dev)
+    mt.tx_chain_trellis(t(blocks), t(tr), pt, res, lv, tg, rec)
+    torch.cuda.synchronize()
+    got = res.cpu().numpy().view(mt.RES_DTYPE)
+    sse, h = TG.oracle_chain(blocks, pool, tr)
+    bad = np.nonzero((got["sse"] != sse) | (got["hash"] != h))[0]
+    assert len(bad) == 0, (len(bad), bad[:5])

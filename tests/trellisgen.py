@@ -82,3 +82,36 @@ def make(seed, n):
         else:
             want[o:o + nc] = O.trellis(cf, tf, float(q.dc), float(q.ac), O.scan(s), float(d["lambda0"]))
     return desc, levels, targets, want
+
+
+def chain_descriptors(blocks, seed):
+    """Trellis descriptors for the chain's blocks (mt_txblk records): the same offset, size
+    and quantizer, random variant, context, lambda and tilt."""
+    rng = np.random.default_rng(seed)
+    n = len(blocks)
+    tr = np.zeros(n, mt.TRELLISBLK_DTYPE)
+    tr["offset"] = blocks["offset"]
+    tr["tx_size"] = blocks["tx_size"]
+    tr["qidx"] = blocks["qidx"]
+    tr["bd_qm"] = blocks["bd_qm"]
+    tr["flags"] = (rng.random(n) < 0.6).astype(np.uint8) * mt.TRELLIS_CTX
+    tr["qctx"] = rng.integers(0, 4, n)
+    tr["dcs_ctx"] = rng.integers(0, 3, n)
+    chroma = (blocks["bd_qm"] & mt.TXBLK_CHROMA) != 0
+    tr["plane"] = chroma.astype(np.uint8)
+    for i in range(n):
+        s = int(blocks["tx_size"][i])
+        if s not in SIZES:
+            continue
+        nc = min(O.tx_w(s), 32) * min(O.tx_h(s), 32)
+        tr["eob_bin"][i] = EOB_BIN[nc][1 if chroma[i] else 0]
+    tr["lambda0"] = rng.choice([0.05, 0.02, 0.2, 0.0], n).astype(np.float32)
+    tr["band_tilt"] = rng.choice([0.0, 0.0, 1.9], n).astype(np.float32)
+    return tr
+
+
+def oracle_chain(blocks, pool, tr, nthreads=8):
+    bdi = (blocks["bd_qm"] >> 4) & 3
+    bd = np.where(bdi == 1, 10, np.where(bdi == 2, 12, 8))
+    return O.chain_batch_trellis(blocks["tx_size"], blocks["tx_type"], blocks["qidx"], bd,
+                                 blocks["bd_qm"] & 0x4F, blocks["offset"].astype(np.uint64), pool, tr, nthreads)
