@@ -360,7 +360,11 @@ typedef struct {
                              bit 6 chroma: new_chroma_with_delta_qm with dc_delta / ac_delta */
     uint8_t rplane, eob_bin, skip_ctx, dcs_ctx;  /* rate context, as mt_rateblk */
     uint8_t txtp_set, txtp_mode, txtp, qctx;
-    uint8_t rflags;
+    uint8_t rflags;       /* bit 0: adaptive skip price (skip_one / skip_zero, as mt_rateblk);
+                             MT_CAND_TRELLIS: trellis_optimize_ctx in the rate context above
+                             between quantization and dequantization, lambda0 of
+                             mt_trellis_lambda0(qidx, bd, (rflags >> 2) & 3) (honoured by
+                             mt_eval_candidates_trellis) */
     uint8_t planes_hi;    /* bits 0-3: source plane index >> 2, bits 4-7: reconstruction >> 2
                              (0 with up to 4 planes) */
     int8_t dc_delta, ac_delta;  /* chroma qindex deltas (src/quant.rs:394) */
@@ -394,6 +398,13 @@ typedef struct {
  * device share them: issue them on one stream. */
 int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane* planes, int nplanes,
                        uint64_t pool_elems, mt_candres* res_d, void* stream);
+/* mt_eval_candidates with the trellis: candidates with MT_CAND_TRELLIS run the reference's
+ * Speed::Slow chain (forward + quant, trellis_optimize_ctx, dequant + inverse; src/coder/
+ * partition_search.rs:4839-4860), the others the plain chain; the levels the rate prices and
+ * nnz reports are the trellis's. */
+enum { MT_CAND_TRELLIS = 2 };
+int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* planes, int nplanes,
+                               uint64_t pool_elems, mt_candres* res_d, void* stream);
 
 /* ---- whole-frame luma intra RDO over the superblock wavefront ----------------------- */
 /* The candidate generators and argmin of the reference's decision code (A19:

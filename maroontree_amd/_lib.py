@@ -120,6 +120,7 @@ def lib():
         L.mt_distortion.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
         L.mt_cfl.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
         L.mt_eval_candidates.argtypes = [vp, i32, P(MtPlane), i32, u64, vp, vp]
+        L.mt_eval_candidates_trellis.argtypes = [vp, i32, P(MtPlane), i32, u64, vp, vp]
         L.mt_gen_pool_size.argtypes = [u64, i32, i32]
         L.mt_gen_pool_size.restype = u64
         L.mt_gen_tx_blocks.argtypes = [u64, i32, i32, vp, vp, u64]
@@ -250,6 +251,7 @@ def tx_chain(blocks, pool, res, levels=None, targets=None, recon=None, stream=No
 
 NOALLOC_MAX_BLOCKS = 1 << 20
 TRELLIS_CTX = 1
+CAND_TRELLIS = 2
 TRELLISBLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("qidx", "u1"), ("bd_qm", "u1"),
                              ("flags", "u1"), ("dc_delta", "i1"), ("ac_delta", "i1"), ("qctx", "u1"),
                              ("eob_bin", "u1"), ("dcs_ctx", "u1"), ("plane", "u1"), ("pad0", "u1"),
@@ -384,18 +386,19 @@ def distortion(blocks, planes, pred, resid=None, stream=None):
     return out
 
 
-def eval_candidates(cands, planes, pool_elems, stream=None, out=None):
+def eval_candidates(cands, planes, pool_elems, stream=None, out=None, trellis=False):
     """Evaluate n intra candidates.  cands: uint8 cuda tensor of n*48 bytes (CAND_DTYPE
     records); planes: list as plane_views(); returns a uint8 cuda tensor of n*48 bytes
-    (CANDRES_DTYPE)."""
+    (CANDRES_DTYPE).  trellis: mt_eval_candidates_trellis (candidates with MT_CAND_TRELLIS
+    in rflags run the trellis)."""
     import torch
     n = cands.numel() // CAND_DTYPE.itemsize
     init(cands.device.index or 0)
     if out is None:
         out = torch.empty(n * CANDRES_DTYPE.itemsize, dtype=torch.uint8, device=cands.device)
     pv = plane_views(planes)
-    _check(lib().mt_eval_candidates(_dev_ptr(cands), n, pv, len(planes), int(pool_elems),
-                                    _dev_ptr(out), _stream_ptr(stream)))
+    fn = lib().mt_eval_candidates_trellis if trellis else lib().mt_eval_candidates
+    _check(fn(_dev_ptr(cands), n, pv, len(planes), int(pool_elems), _dev_ptr(out), _stream_ptr(stream)))
     return out
 
 

@@ -47,7 +47,7 @@ int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, con
                       const int32_t* resid_d, mt_distres* out_d, hipStream_t s);
 // mt_eval_candidates on a prepared plane set.
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
-                mt_candres* res_d, hipStream_t s);
+                mt_candres* res_d, hipStream_t s, bool trellis = false);
 // One frame of the RDO's reconstruction: its source, reconstruction plane (rstride samples
 // per row, frame size of src) and quantizer (base_q_idx, bd_qm as in mt_txblk).
 struct ReconFrame {

@@ -34,6 +34,18 @@ __device__ __forceinline__ int tx_h(int s) {
     return s < 19 ? h[s] : 0;
 }
 
+// trellis_lambda() * hbd_trellis_scale (mt_trellis_lambda0): f32, the same operations.
+__device__ inline float trellis_lambda0(int qidx, int bd, int sub) {
+    float scale = 1.0f;
+    if (bd > 8 && (sub == 1 || sub == 2)) {
+        const float top = sub == 2 ? 0.174f : 1.0f, hi = sub == 2 ? 36.0f : 24.0f, lo = sub == 2 ? 20.0f : 10.0f;
+        float wgt = __fdiv_rn(__fsub_rn(hi, (float)qidx), fmaxf(__fsub_rn(hi, lo), 1.0f));
+        wgt = fminf(fmaxf(wgt, 0.0f), 1.0f);
+        scale = __fadd_rn(__fmul_rn(__fsub_rn(top, 1.0f), wgt), 1.0f);
+    }
+    return __fmul_rn(0.05f, scale);
+}
+
 // Splits each candidate into the stage descriptors; a candidate outside its planes or its
 // pool slot gets status MT_ERR_ARG and descriptors the stages reject.  CfL candidates
 // predict DC_PRED first and are listed for the CfL stage (cfl_list, *cfl_count).
@@ -44,6 +56,7 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
                                                          mt_distblk* __restrict__ db,
                                                          mt_rateblk* __restrict__ rb,
                                                          int8_t* __restrict__ qd,
+                                                         mt_trellisblk* __restrict__ trd,
                                                          int32_t* __restrict__ status,
                                                          uint32_t* __restrict__ cfl_list,
                                                          uint32_t* __restrict__ cfl_count) {
@@ -119,11 +132,27 @@ __global__ void __launch_bounds__(256) eval_setup_kernel(const mt_cand* __restri
     r.txtp_mode = cfl ? (uint8_t)0 : c.txtp_mode;  // CfL: txtp_mode names the luma plane
     r.txtp = c.txtp;
     r.qctx = c.qctx;
-    r.flags = c.rflags;
+    r.flags = c.rflags & 1;
     r.skip_one = c.skip_one;
     r.skip_zero = c.skip_zero;
     r.bound = __int_as_float(0x7f800000);  // +inf: price fully
     rb[i] = r;
+    // the trellis in the rate context above (lambda0 0: the block is left as quantized)
+    mt_trellisblk tr{};
+    tr.offset = c.offset;
+    tr.tx_size = ok ? c.tx_size : 0xff;
+    tr.qidx = c.qidx;
+    tr.bd_qm = c.bd_qm;
+    tr.flags = MT_TRELLIS_CTX;
+    tr.dc_delta = c.dc_delta;
+    tr.ac_delta = c.ac_delta;
+    tr.qctx = c.qctx;
+    tr.eob_bin = c.eob_bin;
+    tr.dcs_ctx = c.dcs_ctx;
+    tr.plane = c.rplane;
+    tr.lambda0 = (c.rflags & MT_CAND_TRELLIS) ? trellis_lambda0(c.qidx, bd, (c.rflags >> 2) & 3) : 0.0f;
+    tr.band_tilt = 0.0f;
+    trd[i] = tr;
 }
 
 __global__ void __launch_bounds__(256) eval_combine_kernel(const mt_cand* __restrict__ cands, int n,
@@ -168,6 +197,7 @@ struct EvalWork {
     mt_distblk* db = nullptr;
     mt_rateblk* rb = nullptr;
     int8_t* qd = nullptr;
+    mt_trellisblk* trd = nullptr;
     int32_t* status = nullptr;
     int32_t* pstatus = nullptr;
     uint32_t* cfl_list = nullptr;
@@ -180,6 +210,8 @@ struct EvalWork {
     int16_t* resid = nullptr;
     int32_t* levels = nullptr;
     int32_t* recon = nullptr;
+    float* targets = nullptr;  // the trellis chain only (allocated on its first use)
+    size_t cap_targets = 0;
 };
 std::mutex g_eval_mu;
 EvalWork g_work[64];
@@ -196,7 +228,8 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
     if (w.cap_n < n) {
         const size_t c = n < 4096 ? 4096 : n;
         if ((rc = grow(w.pb, c)) || (rc = grow(w.tb, c)) || (rc = grow(w.db, c)) ||
-            (rc = grow(w.rb, c)) || (rc = grow(w.qd, 2 * c)) || (rc = grow(w.status, c)) || (rc = grow(w.pstatus, c)) ||
+            (rc = grow(w.rb, c)) || (rc = grow(w.qd, 2 * c)) || (rc = grow(w.trd, c)) || (rc = grow(w.status, c)) ||
+            (rc = grow(w.pstatus, c)) ||
             (rc = grow(w.txr, c)) || (rc = grow(w.dr, c)) || (rc = grow(w.bits, c)) ||
             (rc = grow(w.cfl_list, c)) || (rc = grow(w.alpha, c)))
             return rc;
@@ -217,7 +250,7 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
 
 namespace mtlib {
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
-                mt_candres* res_d, hipStream_t s) {
+                mt_candres* res_d, hipStream_t s, bool trellis) {
     if (n == 0) return MT_OK;
     int dev = 0, cus = 0;
     int rc = device_ready(&dev, &cus);
@@ -226,17 +259,25 @@ int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool
     EvalWork& w = g_work[dev];
     rc = ensure(w, (size_t)n, (size_t)pool_elems);
     if (rc != MT_OK) return rc;
+    if (trellis && w.cap_targets < w.cap_pool) {
+        if ((rc = grow(w.targets, w.cap_pool))) return rc;
+        w.cap_targets = w.cap_pool;
+    }
     ProfScope prof(kProfEval, s);
     const int g256 = (n + 255) / 256;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w.cfl_count, 1);
     hipLaunchKernelGGL(eval_setup_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, ps, pool_elems,
-                       w.pb, w.tb, w.db, w.rb, w.qd, w.status, w.cfl_list, w.cfl_count);
+                       w.pb, w.tb, w.db, w.rb, w.qd, w.trd, w.status, w.cfl_list, w.cfl_count);
     MT_HIP_TRY(hipGetLastError());
     // prediction and residual_pred in one pass, then chroma-from-luma on the listed candidates
     if ((rc = intra_pred_launch(w.pb, n, ps, w.pred, w.pstatus, w.resid, s))) return rc;
     if ((rc = cfl_eval_launch(cands_d, w.cfl_list, w.cfl_count, n, ps, w.pred, w.resid, w.alpha, s))) return rc;
-    if ((rc = tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s)))
+    if (trellis) {  // forward + quant, the trellis, dequant + inverse
+        if ((rc = tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, w.targets, w.recon, w.qd, s, w.trd)))
+            return rc;
+    } else if ((rc = tx_chain_launch(w.tb, n, w.resid, w.txr, w.levels, nullptr, w.recon, w.qd, s))) {
         return rc;
+    }
     if ((rc = distortion_launch(w.db, n, ps, w.pred, w.recon, w.dr, s))) return rc;
     if ((rc = rate_launch_with_nnz(w.rb, n, w.levels, w.bits, w.txr, s))) return rc;
     hipLaunchKernelGGL(eval_combine_kernel, dim3(g256), dim3(256), 0, s, cands_d, n, w.status,
@@ -256,5 +297,18 @@ extern "C" int mt_eval_candidates(const mt_cand* cands_d, int n, const mt_plane*
     PlaneSet ps{};
     int rc = mtlib::planes_from_host(planes, nplanes, "mt_eval_candidates", &ps);
     if (rc != MT_OK) return rc;
-    return mtlib::eval_launch(cands_d, n, ps, pool_elems, res_d, (hipStream_t)stream);
+    return mtlib::eval_launch(cands_d, n, ps, pool_elems, res_d, (hipStream_t)stream, false);
+}
+
+extern "C" int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* planes,
+                                          int nplanes, uint64_t pool_elems, mt_candres* res_d,
+                                          void* stream) {
+    if (n < 0 || (n > 0 && (!cands_d || !res_d)))
+        return mtlib::set_error(MT_ERR_ARG, "mt_eval_candidates_trellis: bad pointer or count");
+    if (pool_elems > (uint64_t)UINT32_MAX + 1)
+        return mtlib::set_error(MT_ERR_ARG, "mt_eval_candidates_trellis: pool offsets are 32-bit");
+    PlaneSet ps{};
+    int rc = mtlib::planes_from_host(planes, nplanes, "mt_eval_candidates_trellis", &ps);
+    if (rc != MT_OK) return rc;
+    return mtlib::eval_launch(cands_d, n, ps, pool_elems, res_d, (hipStream_t)stream, true);
 }

@@ -151,6 +151,9 @@ void orc_trellis(int32_t* cf, const float* tf, float dc_q, float ac_q, const uin
                  float lambda0);
 void orc_trellis_ctx(int32_t* cf, const float* tf, float dc_q, float ac_q, const uint32_t* scan,
                      float lambda0, const orc_rate_ctx* c, int qm_level, float band_tilt);
+/* trellis_lambda() (src/cost.rs:110) times hbd_trellis_scale (src/coder/lossy_state.rs:158,
+ * shipped tuning); sub 0 4:4:4, 1 4:2:2, 2 4:2:0. */
+float orc_trellis_lambda0(int qidx, int bd, int sub);
 /* Trellis parameters of one block, the library's mt_trellisblk layout. */
 typedef struct {
     uint32_t offset;

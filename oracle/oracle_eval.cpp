@@ -85,6 +85,19 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
     else  // Quant::new_with_qm (src/quant.rs:354)
         orc_quant_new(c.qidx, bd, qm, &q);
     orc_fwd_txfm(s, t, resid.data(), &q, levels.data(), targets.data());
+    if (c.rflags & 2) {  // trellis_optimize_ctx in the candidate's rate context
+        orc_rate_ctx tc;
+        std::memset(&tc, 0, sizeof(tc));
+        tc.qctx = c.qctx;
+        tc.cls = (tx_dim_log2(w) - 2 + tx_dim_log2(h) - 2 + 1) >> 1;
+        tc.plane = c.rplane;
+        tc.w = std::min(w, 32);
+        tc.h = std::min(h, 32);
+        tc.eob_bin = c.eob_bin;
+        tc.dcs_ctx = c.dcs_ctx;
+        orc_trellis_ctx(levels.data(), targets.data(), (float)q.dc, (float)q.ac, orc_scan(s),
+                        orc_trellis_lambda0(c.qidx, bd, (c.rflags >> 2) & 3), &tc, q.qm_level, 0.0f);
+    }
     orc_inv_txfm(s, t, levels.data(), &q, inv.data());
     const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
     const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);

@@ -474,6 +474,20 @@ int last_nonzero(const int32_t* cf, const uint32_t* scan, int n) {
 
 }  // namespace
 
+// hbd_trellis_scale (src/coder/lossy_state.rs:158) on the shipped tuning (src/tuning.rs:954-959):
+// above 8 bits at 4:2:0 (top 0.174, hi 36, lo 20) and 4:2:2 (1.0, 24, 10) the factor ramps
+// from 1 down to top as qindex falls from hi to lo; f_fmlaf unfused.
+extern "C" float orc_trellis_lambda0(int qidx, int bd, int sub) {
+    float scale = 1.0f;
+    if (bd > 8 && (sub == 1 || sub == 2)) {
+        const float top = sub == 2 ? 0.174f : 1.0f, hi = sub == 2 ? 36.0f : 24.0f, lo = sub == 2 ? 20.0f : 10.0f;
+        float w = (hi - (float)qidx) / std::max(hi - lo, 1.0f);
+        w = std::min(std::max(w, 0.0f), 1.0f);
+        scale = (top - 1.0f) * w + 1.0f;
+    }
+    return 0.05f * scale;
+}
+
 // trellis_optimize (src/trellis.rs:944): round-down (Step A), then the EOB position against
 // the coef_rate_bits model with eob_sig(e) = bin * 0.9 + (extra + 2) (f_fmlaf, unfused), or
 // all-zero at 1 bit.

@@ -59,7 +59,7 @@ static_assert(sizeof(mt_candres) == sizeof(orc_candres), "candres");
 static_assert(sizeof(mt_plane) == sizeof(orc_plane), "plane");
 
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems, mt_candres* res_d,
-                hipStream_t) {
+                hipStream_t, bool) {
     (void)pool_elems;
     orc_eval_batch((const orc_cand*)cands_d, n, (const orc_plane*)ps.p, ps.n, (orc_candres*)res_d, 8);
     return MT_OK;

@@ -99,7 +99,12 @@ def oracle_candidate(r, planes_np, views):
         q = O.quant_chroma(int(r["qidx"]), int(r["dc_delta"]), int(r["ac_delta"]), bd, qm)
     else:
         q = O.quant(int(r["qidx"]), bd, qm)
-    levels, _ = O.fwd(s, t, resid, q)
+    levels, targets = O.fwd(s, t, resid, q)
+    if int(r["rflags"]) & 2:  # MT_CAND_TRELLIS: trellis_optimize_ctx in the rate context
+        tctx = O.rate_ctx(qctx=int(r["qctx"]), cls=cls_of(s), plane=int(r["rplane"]), w=min(w, 32),
+                          h=min(h, 32), eob_bin=int(r["eob_bin"]), dcs_ctx=int(r["dcs_ctx"]))
+        lam0 = O.trellis_lambda0(int(r["qidx"]), bd, (int(r["rflags"]) >> 2) & 3)
+        levels = O.trellis_ctx(levels, targets, float(q.dc), float(q.ac), O.scan(s), lam0, tctx, q.qm_level)
     inv = O.inv(s, t, levels, q)
     L = O.plib()
     vw, vh = min(max(svw - x, 0), w), min(max(svh - y, 0), h)

@@ -213,6 +213,13 @@ def block_rate(cf, scan_tab, ctx, bound=float("inf")):
     return L.orc_block_rate(ptr(cf, ctypes.c_int32), ptr(sc, ctypes.c_uint32), ctypes.byref(ctx), bound)
 
 
+def trellis_lambda0(qidx, bd, sub):
+    L = lib()
+    L.orc_trellis_lambda0.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_trellis_lambda0.restype = ctypes.c_float
+    return float(L.orc_trellis_lambda0(qidx, bd, sub))
+
+
 def trellis(cf, tf, dc_q, ac_q, scan_tab, lambda0):
     """trellis_optimize on a copy of cf (x-major levels) with targets tf; returns the levels."""
     L = lib()

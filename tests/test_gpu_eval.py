@@ -49,6 +49,29 @@ def test_eval_candidates_match_oracle(dev, bd):
     assert bad == 0, f"{bad} of {len(cands)} candidates differ"
 
 
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_eval_candidates_trellis_match_oracle(dev, bd):
+    """mt_eval_candidates_trellis: candidates with MT_CAND_TRELLIS (and a chroma
+    subsampling for the trellis lambda law) run trellis_optimize_ctx between quantization
+    and dequantization; every result field against the oracle's evaluation with the trellis,
+    bit for bit; the others as mt_eval_candidates."""
+    rng = np.random.default_rng(700 + bd)
+    src, recon = make_planes(rng, bd)
+    cands, pool = make_cands(rng, 3000, bd)
+    tr = rng.random(len(cands)) < 0.7
+    cands["rflags"] |= (tr * mt.CAND_TRELLIS + tr * (rng.integers(0, 3, len(cands)) << 2)).astype(np.uint8)
+    print(f"eval+trellis bd={bd} candidates={len(cands)}", flush=True)
+    t = [torch.from_numpy(a.view(np.int16)).to(dev) for a in (src, recon, src)]
+    views = [(t[0], PW, PH, PW, PH), (t[1], PW, PH, PW, PH), (t[2], PW, PH, 301, 233)]
+    out = mt.eval_candidates(torch.from_numpy(cands.view(np.uint8)).to(dev), views, pool, trellis=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(mt.CANDRES_DTYPE)
+    want = O.eval_batch(cands, [(src, PW, PH, PW, PH), (recon, PW, PH, PW, PH), (src, PW, PH, 301, 233)])
+    same = (got.view(np.uint8).reshape(len(cands), -1) == want.view(np.uint8).reshape(len(cands), -1)).all(axis=1)
+    assert same.all(), f"{int((~same).sum())} of {len(cands)} differ, first {np.nonzero(~same)[0][:5]}"
+    assert (got["status"] == 0).all()
+
+
 def test_eval_candidates_flags_invalid(dev):
     """A candidate outside its plane or pool slot, or of an unsupported (size, type), gets a
     status and cost +inf; its neighbours are unaffected."""

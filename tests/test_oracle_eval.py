@@ -1,7 +1,8 @@
 """The oracle's whole-candidate evaluation (oracle/oracle_eval.cpp, the reference's candidate
 loop body src/coder/partition_search.rs:471-520) equals the stage-by-stage composition of
 the individually pinned oracle kernels written out in Python, on every reference size,
-mode family, bit depth and clipped source planes."""
+mode family, bit depth and clipped source planes; with the trellis (MT_CAND_TRELLIS,
+trellis_optimize_ctx between quantization and the inverse) on half of the candidates."""
 import numpy as np
 
 import oracle_lib as O
@@ -13,6 +14,8 @@ def test_eval_batch_equals_stagewise_composition():
         rng = np.random.default_rng(40 + bd)
         src, recon = make_planes(rng, bd)
         cands, _ = make_cands(rng, 260, bd)
+        tr = rng.random(len(cands)) < 0.5
+        cands["rflags"] |= (tr * 2 + tr * (rng.integers(0, 3, len(cands)) << 2)).astype(np.uint8)
         planes = [(src, PW, PH, PW, PH), (recon, PW, PH, PW, PH), (src, PW, PH, 301, 233)]
         got = O.eval_batch(cands, planes, nthreads=4)
         for i in range(len(cands)):
