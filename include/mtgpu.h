@@ -463,7 +463,12 @@ int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* pl
  *    and counts neighbours inside it as not smooth; the chosen chroma blocks are then
  *    reconstructed in coding order, and the final accounting re-prices them with the final
  *    contexts (mt_rdo_stats bits_uv, sse_u, sse_v). */
-enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4, MT_RDO_CFL = 8 };
+/* MT_RDO_TRELLIS: the Speed::Slow chain in every candidate and in the reconstruction:
+ * trellis_optimize_ctx (src/trellis.rs:55) between quantization and dequantization in the
+ * candidate's rate context, lambda0 = mt_trellis_lambda0(qidx, bd, sub) with sub 0 for
+ * 4:4:4 frames and 2 for 4:2:0 and luma-only frames (their decision lambda's). */
+enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4, MT_RDO_CFL = 8,
+       MT_RDO_TRELLIS = 32 };
 enum { MT_RDO_LUMA_ONLY = 0, MT_RDO_CHROMA_420 = 1, MT_RDO_CHROMA_444 = 2 };
 #define MT_RDO_NODES 341          /* quadtree nodes of one SB: 1 + 4 + 16 + 64 + 256 */
 #define MT_RDO_ALL_MODES 0x3FFF   /* prune mask bits 0-12: intra modes; bit 13: filter intra */

@@ -59,12 +59,17 @@ struct ReconFrame {
     mt_plane src_uv[2];
     uint16_t* rec_uv[2];
     int32_t uv_stride, uv_bd_qm, uv_dc, uv_ac, ss, pad2;
+    // the trellis between quantization and dequantization (the RDO's MT_RDO_TRELLIS): on / off,
+    // and the chroma subsampling of its lambda law (0 4:4:4, 2 4:2:0)
+    int32_t trellis, tsub;
 };
 // The RDO's superblock reconstruction: one wave per superblock b walks the leaves
 // leaves_d[first_d[b] .. + count_d[b]) in coding order, predicting from and writing to the
 // reconstruction plane of frame frames_d[leaf.plane] (device array).
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
                     int nsb, const ReconFrame* frames_d, hipStream_t s);
+// qctx of a frame qindex: the coefficient CDF snapshot the RDO prices with (Cdfs::new(qctx))
+__host__ __device__ inline int rdo_qctx(int qidx) { return qidx <= 20 ? 0 : qidx <= 60 ? 1 : qidx <= 120 ? 2 : 3; }
 // The RDO's chroma reconstruction: one wave per superblock b walks the chroma transform
 // blocks desc_d[first_d[b] .. + count_d[b]) (descriptor fields as sb_recon_uv_kernel reads
 // them) of frame frames_d[desc.plane].

@@ -16,6 +16,7 @@
 
 #include "../../include/mtgpu.h"
 #include "mt_common.h"
+#include "mt_trellis.h"
 
 namespace {
 
@@ -32,18 +33,6 @@ __device__ __forceinline__ int tx_w(int s) {
 __device__ __forceinline__ int tx_h(int s) {
     constexpr int h[19] = {4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 4, 32, 8, 64, 16};
     return s < 19 ? h[s] : 0;
-}
-
-// trellis_lambda() * hbd_trellis_scale (mt_trellis_lambda0): f32, the same operations.
-__device__ inline float trellis_lambda0(int qidx, int bd, int sub) {
-    float scale = 1.0f;
-    if (bd > 8 && (sub == 1 || sub == 2)) {
-        const float top = sub == 2 ? 0.174f : 1.0f, hi = sub == 2 ? 36.0f : 24.0f, lo = sub == 2 ? 20.0f : 10.0f;
-        float wgt = __fdiv_rn(__fsub_rn(hi, (float)qidx), fmaxf(__fsub_rn(hi, lo), 1.0f));
-        wgt = fminf(fmaxf(wgt, 0.0f), 1.0f);
-        scale = __fadd_rn(__fmul_rn(__fsub_rn(top, 1.0f), wgt), 1.0f);
-    }
-    return __fmul_rn(0.05f, scale);
 }
 
 // Splits each candidate into the stage descriptors; a candidate outside its planes or its

@@ -202,6 +202,13 @@ __device__ float part_bits(const FrameArgs& f, int x, int y, int L, int sym, int
 __constant__ int kIntraModeCtx[13] = {0, 1, 2, 3, 4, 4, 4, 4, 3, 0, 1, 2, 0};
 __constant__ int kFilterDir[5] = {0, 1, 2, 6, 0};  // Filter_Intra_Mode_To_Intra_Dir
 
+// The candidates' trellis bits: MT_CAND_TRELLIS with the chroma subsampling of the trellis
+// lambda law (4:4:4 frames 0; 4:2:0 and luma-only frames 2, as their decision lambda).
+__device__ inline uint8_t trellis_rflags(const FrameArgs& f) {
+    if (!(f.flags & MT_RDO_TRELLIS)) return 0;
+    return (uint8_t)(MT_CAND_TRELLIS | ((f.chroma == MT_RDO_CHROMA_444 ? 0 : 2) << 2));
+}
+
 // The candidate of mode id `id` on the s x s block at (x, y) of the SB at (x0, y0).
 __device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, int x0, int y0, bool fin,
                              uint32_t offset) {
@@ -232,7 +239,8 @@ __device__ mt_cand make_cand(const FrameArgs& f, int x, int y, int s, int id, in
     c.txtp_set = (uint8_t)(s == 4 ? 1 : s == 8 ? 2 : s == 16 ? 3 : 0);
     c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
     c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT's index in the intra tx sets
-    c.qctx = (uint8_t)(f.qidx <= 20 ? 0 : f.qidx <= 60 ? 1 : f.qidx <= 120 ? 2 : 3);
+    c.qctx = (uint8_t)mtlib::rdo_qctx(f.qidx);
+    c.rflags = trellis_rflags(f);
     c.wsse_scale = f.wsse;
     c.lambda = f.lambda;
     // luma_intra_syntax_bits (src/coder/superblock.rs:192)
@@ -653,7 +661,8 @@ __device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p
     c.rplane = 1;
     c.eob_bin = (uint8_t)(ts == 4 ? 1 : ts == 8 ? 6 : ts == 16 ? 12 : 16);  // the chroma eob_bin CDFs
     c.skip_ctx = 7;
-    c.qctx = (uint8_t)(f.qidx <= 20 ? 0 : f.qidx <= 60 ? 1 : f.qidx <= 120 ? 2 : 3);
+    c.qctx = (uint8_t)mtlib::rdo_qctx(f.qidx);
+    c.rflags = trellis_rflags(f);
     c.wsse_scale = 0.0f;
     c.lambda = f.lambda;
     return c;
@@ -1035,6 +1044,8 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
         rf[i].rstride = fr.recon_stride;
         rf[i].qidx = fr.qidx;
         rf[i].bd_qm = fr.qm_level | (bdi << 4);
+        rf[i].trellis = (flags & MT_RDO_TRELLIS) ? 1 : 0;
+        rf[i].tsub = fr.chroma == MT_RDO_CHROMA_444 ? 0 : 2;
         f.chroma = fr.chroma;
         if (fr.chroma) {
             any_chroma = true;
@@ -1226,7 +1237,9 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
                                    (uint32_t)(k < max_steps ? step_pool[k] : 0), w.cands + n, w.uvblk, w.nuvblk);
             MT_HIP_TRY(hipGetLastError());
         }
-        if ((rc = mtlib::eval_launch(w.cands, (int)eval_n[k], ps, eval_pool[k], w.cres, s))) return rc;
+        if ((rc = mtlib::eval_launch(w.cands, (int)eval_n[k], ps, eval_pool[k], w.cres, s,
+                                     (flags & MT_RDO_TRELLIS) != 0)))
+            return rc;
         if (nk) {
             {
                 mtlib::ProfScope prof(mtlib::kProfRdo, s);
@@ -1281,7 +1294,7 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
                            w.leaf_base, w.pool_base, w.fcands);
         MT_HIP_TRY(hipGetLastError());
     }
-    if ((rc = mtlib::eval_launch(w.fcands, (int)tl, ps, tp, w.fres, s))) return rc;
+    if ((rc = mtlib::eval_launch(w.fcands, (int)tl, ps, tp, w.fres, s, (flags & MT_RDO_TRELLIS) != 0))) return rc;
     {
         mtlib::ProfScope prof(mtlib::kProfRdo, s);
         hipLaunchKernelGGL(rdo_final_sum_kernel, dim3((nall + 63) / 64), dim3(64), 0, s, w.fargs, w.all, (int)nall,
@@ -1387,7 +1400,7 @@ extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
 extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
                                   void* stream) {
     if (!frames || !stats || nframes < 0) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
-    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL))
+    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL | MT_RDO_TRELLIS))
         return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad flags");
     int rc = MT_OK;
     for (int i = 0; i < nframes; ++i)

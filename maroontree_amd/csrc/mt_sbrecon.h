@@ -11,16 +11,30 @@
 // equals the one the candidate evaluation measured for the same prediction.
 #pragma once
 #include "mt_pred_wave.h"
+#include "mt_trellis.h"
 
 namespace {
+
+// The trellis between quantization and dequantization of a reconstruction chain: the
+// block's descriptor (its offset unused), the tables, and LDS for the levels, the targets
+// and the trellis's workspace (one element per coefficient each).
+struct ReconTrellis {
+    const mt_trellisblk* d;  // nullptr: no trellis
+    const mtlib::TrellisTabs* tabs;
+    int32_t* lv;
+    float* tg;
+    float* suf;
+};
 
 // Square chain of one N x N block (N <= 32) by one wave: lane i runs line i of each pass.
 // type: DCT_DCT, or at N <= 16 ADST_DCT / DCT_ADST / ADST_ADST (vertical kind first, the
 // 2-D types the chain kernels run with the same per-size rules).  res: the residual (LDS,
 // row-major); out: the reconstructed residual (may alias res); work: LDS of N * (N + 1) ints.
+// With tr.d: the levels and targets go to LDS, one lane runs the trellis over them, and the
+// dequantization reads the trellis's levels.
 template <int N>
 __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p,
-                              int type = MT_TXTP_DCT_DCT) {
+                              int type = MT_TXTP_DCT_DCT, ReconTrellis tr = ReconTrellis{}) {
     using R = TxRules<N, N>;
     constexpr int RS = N + 1;
     const int lane = threadIdx.x & 63;
@@ -53,16 +67,36 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
             if (R::ratio) m = mulq16(m, R::ratio);
             int32_t lvl = quant_q16(c, m);
             int32_t step = rc == 0 ? p.dc : p.ac;
+            float tgt = q16_target(c, m);
             if (p.qm_level < 15) {
-                float tgt = q16_target(c, m);
                 const int iwt = g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc];
                 lvl = qm_apply(tgt, iwt);
                 step = (step * iwt + 16) >> 5;
             }
-            work[fy * RS + fx] = dequant(lvl, step, R::dq_shift, p.cf_max);
+            if (tr.d) {
+                tr.lv[rc] = lvl;
+                tr.tg[rc] = tgt;
+            } else {
+                work[fy * RS + fx] = dequant(lvl, step, R::dq_shift, p.cf_max);
+            }
         }
     }
     __syncthreads();
+    if (tr.d) {
+        if (lane == 0) (void)trellis_block(*tr.d, tr.lv, tr.tg, tr.suf, *tr.tabs);
+        __syncthreads();
+        if (lane < N) {
+            const int fy = lane;
+#pragma unroll
+            for (int fx = 0; fx < N; fx++) {
+                const int rc = fx * N + fy;
+                int32_t step = rc == 0 ? p.dc : p.ac;
+                if (p.qm_level < 15) step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
+                work[fy * RS + fx] = dequant(tr.lv[rc], step, R::dq_shift, p.cf_max);
+            }
+        }
+        __syncthreads();
+    }
     // inverse row pass (length N) with the row clip, inter-pass shift and column clip
     if (lane < N) {
         const int y = lane;
@@ -95,9 +129,11 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
 // TX_64X64 chain of one block by one wave: the f64 separable forward of dct64x64_coeffs
 // (src/dct.rs:4155) with every product and sum rounded on its own in the reference's order,
 // the 32x32-QM quantizer, and idct_dequant_64x64 (src/idct.rs:2677); the arithmetic of the
-// batched 64x64 kernel.  vt: LDS of 16 * 64 doubles; coef: LDS of 32 * 33 ints.
+// batched 64x64 kernel.  vt: LDS of 16 * 64 doubles; coef: LDS of 32 * 33 ints.  With tr.d
+// the trellis runs between quantization and dequantization (its levels and targets may live
+// in vt, which the forward no longer needs by then).
 __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32_t* coef,
-                             const BlkParam& p) {
+                             const BlkParam& p, ReconTrellis tr = ReconTrellis{}) {
     const int lane = threadIdx.x & 63;
     int32_t* rowbuf = reinterpret_cast<int32_t*>(vt);  // 32 x 64 int32 after the forward
     const bool qm = p.qm_level < 15;
@@ -158,10 +194,25 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
             if (qm) lvl = qm_apply(tgt, iwt);
             int32_t step = rc == 0 ? p.dc : p.ac;
             if (qm) step = (step * iwt + 16) >> 5;
-            coef[v * 33 + u] = dequant(lvl, step, 2, p.cf_max);
+            if (tr.d) {
+                tr.lv[rc] = lvl;
+                tr.tg[rc] = tgt;
+            } else {
+                coef[v * 33 + u] = dequant(lvl, step, 2, p.cf_max);
+            }
         }
     }
     __syncthreads();
+    if (tr.d) {
+        if (lane == 0) (void)trellis_block(*tr.d, tr.lv, tr.tg, tr.suf, *tr.tabs);
+        __syncthreads();
+        for (int rc = lane; rc < 1024; rc += 64) {
+            int32_t step = rc == 0 ? p.dc : p.ac;
+            if (qm) step = (step * (int32_t)qmrow[rc] + 16) >> 5;
+            coef[(rc & 31) * 33 + (rc >> 5)] = dequant(tr.lv[rc], step, 2, p.cf_max);
+        }
+        __syncthreads();
+    }
     if (lane < 32) {
         const int y = lane;
         int32_t v[64];
@@ -192,7 +243,8 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
 __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restrict__ leaves,
                                                       const uint32_t* __restrict__ first,
                                                       const uint32_t* __restrict__ count,
-                                                      const mtlib::ReconFrame* __restrict__ frames) {
+                                                      const mtlib::ReconFrame* __restrict__ frames,
+                                                      mtlib::TrellisTabs tabs) {
     __shared__ int32_t s_pred[4096];
     __shared__ int32_t s_res[4096];
     __shared__ int32_t s_a[kEdge], s_l[kEdge];
@@ -212,9 +264,21 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
     BlkParam prm;
     param_from_desc(q, prm, nullptr, 0);
     const int maxv = (1 << prm.bd) - 1;
+    // the trellis of the candidates (the levels in s_vt, its workspace in s_fi)
+    mt_trellisblk td{};
+    td.qidx = (uint8_t)fr.qidx;
+    td.bd_qm = (uint8_t)fr.bd_qm;
+    td.flags = MT_TRELLIS_CTX;
+    td.qctx = (uint8_t)mtlib::rdo_qctx(fr.qidx);
+    td.lambda0 = trellis_lambda0(fr.qidx, prm.bd, fr.tsub);
+    const ReconTrellis rt{fr.trellis ? &td : nullptr, &tabs, reinterpret_cast<int32_t*>(s_vt),
+                          reinterpret_cast<float*>(s_vt) + 1024, reinterpret_cast<float*>(s_fi)};
     for (uint32_t i = 0; i < n; i++) {
         const mt_predblk d = leaves[f0 + i];
         const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
+        td.tx_size = (uint8_t)(s == 4 ? MT_TX_4X4 : s == 8 ? MT_TX_8X8 : s == 16 ? MT_TX_16X16
+                               : s == 32 ? MT_TX_32X32 : MT_TX_64X64);
+        td.eob_bin = (uint8_t)(s == 4 ? 0 : s == 8 ? 5 : s == 16 ? 11 : 15);
         const int x0 = d.x, y0 = d.y;
         wave_predict(d, src.width, src.height,
                      [=](int x, int y) { return (int)rec[(size_t)y * rstride + x]; }, s_pred, s_a, s_l,
@@ -224,11 +288,11 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
             s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
         __syncthreads();
         switch (s) {
-            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm); break;
-            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm); break;
-            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm); break;
-            case 32: wave_chain_sq<32>(s_res, s_res, s_work, prm); break;
-            default: wave_chain64(s_res, s_res, s_vt, s_work, prm); break;
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
+            case 32: wave_chain_sq<32>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
+            default: wave_chain64(s_res, s_res, s_vt, s_work, prm, rt); break;
         }
         for (int p = lane; p < npx; p += 64) {
             const int v = s_pred[p] + s_res[p];
@@ -249,12 +313,14 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
 __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __restrict__ desc,
                                                          const uint32_t* __restrict__ first,
                                                          const uint32_t* __restrict__ count,
-                                                         const mtlib::ReconFrame* __restrict__ frames) {
+                                                         const mtlib::ReconFrame* __restrict__ frames,
+                                                         mtlib::TrellisTabs tabs) {
     __shared__ int32_t s_pred[1024];
     __shared__ int32_t s_res[1024];
     __shared__ int32_t s_a[kEdge], s_l[kEdge];
     __shared__ int32_t s_fi[33 * 33];
     __shared__ int32_t s_work[32 * 33];
+    __shared__ int32_t s_tr[2048];  // the trellis's levels and targets
     const int lane = threadIdx.x;
     const uint32_t f0 = first[blockIdx.x], n = count[blockIdx.x];
     if (n == 0) return;
@@ -267,6 +333,17 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
     param_from_desc(q, prm, qd, 0);
     const int maxv = (1 << prm.bd) - 1;
     const int ustride = fr.uv_stride;
+    mt_trellisblk td{};
+    td.qidx = (uint8_t)fr.qidx;
+    td.bd_qm = (uint8_t)fr.uv_bd_qm;
+    td.dc_delta = (int8_t)fr.uv_dc;
+    td.ac_delta = (int8_t)fr.uv_ac;
+    td.flags = MT_TRELLIS_CTX;
+    td.qctx = (uint8_t)mtlib::rdo_qctx(fr.qidx);
+    td.plane = 1;
+    td.lambda0 = trellis_lambda0(fr.qidx, prm.bd, fr.tsub);
+    const ReconTrellis rt{fr.trellis ? &td : nullptr, &tabs, s_tr, reinterpret_cast<float*>(s_tr) + 1024,
+                          reinterpret_cast<float*>(s_fi)};
     const mt_plane luma{fr.rec, fr.rstride, fr.src.width, fr.src.height, fr.src.width, fr.src.height, 0};
     for (uint32_t i = 0; i < n; i++) {
         mt_predblk d = desc[f0 + i];
@@ -276,6 +353,8 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
         uint16_t* __restrict__ rec = fr.rec_uv[pl];
         const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
         const int x0 = d.x, y0 = d.y;
+        td.tx_size = (uint8_t)(s == 4 ? MT_TX_4X4 : s == 8 ? MT_TX_8X8 : s == 16 ? MT_TX_16X16 : MT_TX_32X32);
+        td.eob_bin = (uint8_t)(s == 4 ? 1 : s == 8 ? 6 : s == 16 ? 12 : 16);
         const bool cfl = d.mode == MT_PRED_CFL;
         const int alpha = d.angle_delta;
         if (cfl) {
@@ -295,10 +374,10 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
             s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
         __syncthreads();
         switch (s) {
-            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, type); break;
-            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, type); break;
-            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, type); break;
-            default: wave_chain_sq<32>(s_res, s_res, s_work, prm, type); break;
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, type, rt); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, type, rt); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, type, rt); break;
+            default: wave_chain_sq<32>(s_res, s_res, s_work, prm, type, rt); break;
         }
         for (int p = lane; p < npx; p += 64) {
             const int v = s_pred[p] + s_res[p];
@@ -314,14 +393,20 @@ namespace mtlib {
 int sb_recon_uv_launch(const mt_predblk* desc_d, const uint32_t* first_d, const uint32_t* count_d, int nsb,
                        const ReconFrame* frames_d, hipStream_t s) {
     if (nsb <= 0) return MT_OK;
-    hipLaunchKernelGGL(sb_recon_uv_kernel, dim3(nsb, 2), dim3(64), 0, s, desc_d, first_d, count_d, frames_d);
+    int dev = 0;
+    MT_HIP_TRY(hipGetDevice(&dev));
+    hipLaunchKernelGGL(sb_recon_uv_kernel, dim3(nsb, 2), dim3(64), 0, s, desc_d, first_d, count_d, frames_d,
+                       trellis_tabs(dev));
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }
 int sb_recon_launch(const mt_predblk* leaves_d, const uint32_t* first_d, const uint32_t* count_d,
                     int nsb, const ReconFrame* frames_d, hipStream_t s) {
     if (nsb <= 0) return MT_OK;
-    hipLaunchKernelGGL(sb_recon_kernel, dim3(nsb), dim3(64), 0, s, leaves_d, first_d, count_d, frames_d);
+    int dev = 0;
+    MT_HIP_TRY(hipGetDevice(&dev));
+    hipLaunchKernelGGL(sb_recon_kernel, dim3(nsb), dim3(64), 0, s, leaves_d, first_d, count_d, frames_d,
+                       trellis_tabs(dev));
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

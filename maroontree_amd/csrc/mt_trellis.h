@@ -184,6 +184,18 @@ __device__ inline void round_down_chunk(int32_t* cf, const float* tf, const uint
     }
 }
 
+// trellis_lambda() * hbd_trellis_scale (mt_trellis_lambda0): f32, the same operations.
+__device__ inline float trellis_lambda0(int qidx, int bd, int sub) {
+    float scale = 1.0f;
+    if (bd > 8 && (sub == 1 || sub == 2)) {
+        const float top = sub == 2 ? 0.174f : 1.0f, hi = sub == 2 ? 36.0f : 24.0f, lo = sub == 2 ? 20.0f : 10.0f;
+        float wgt = __fdiv_rn(__fsub_rn(hi, (float)qidx), fmaxf(__fsub_rn(hi, lo), 1.0f));
+        wgt = fminf(fmaxf(wgt, 0.0f), 1.0f);
+        scale = __fadd_rn(__fmul_rn(__fsub_rn(top, 1.0f), wgt), 1.0f);
+    }
+    return __fmul_rn(0.05f, scale);
+}
+
 // The trellis of one block d over its levels cf and targets tf (x-major; generic pointers:
 // global or LDS), with suf a float workspace of the block's coefficient count.  Returns
 // MT_ERR_ARG (levels untouched) for an invalid descriptor, else MT_OK.

@@ -47,7 +47,7 @@ namespace {
 
 constexpr int kNodes = 341;
 constexpr int kLevelBase[5] = {0, 1, 5, 21, 85};
-enum { kAngle = 1, kFilter = 2, kMin4 = 4, kCfl = 8, kRect = 16 };
+enum { kAngle = 1, kFilter = 2, kMin4 = 4, kCfl = 8, kRect = 16, kTrellis = 32 };
 enum { kForceSplit = 0x4000, kNoSplit = 0x8000 };
 // node decisions (the split field of a node record)
 enum { kNone = 0, kSplit = 1, kUncoded = 2, kHorz = 3, kVert = 4 };
@@ -274,6 +274,9 @@ struct Frame {
     int uv_at(int cx, int cy) const { return uvmode4[(size_t)(cy >> 2) * (Wc >> 2) + (cx >> 2)]; }
 
     bool rect_on() const { return (flags & kRect) != 0; }
+    // the trellis bits of every candidate (rflags bit 1, the lambda law's subsampling in bits
+    // 2-3: 0 at 4:4:4, 2 at 4:2:0 and for luma-only frames)
+    uint8_t trellis_rflags() const { return (flags & kTrellis) ? (uint8_t)(2 | ((chroma == 2 ? 0 : 2) << 2)) : 0; }
 
     // the luma candidate of mode id `id` on block b of the SB at (x0, y0)
     orc_cand cand(const Blk& b, int id, int x0, int y0, bool final_ctx) const {
@@ -300,6 +303,7 @@ struct Frame {
         c.txtp_mode = (uint8_t)(id >= 61 ? kFilterDir[id - 61] : id_base(id));
         c.txtp = (uint8_t)(c.txtp_set ? 1 : 0);  // DCT_DCT in the intra sets
         c.qctx = (uint8_t)qctx_of(qidx);
+        c.rflags = trellis_rflags();
         c.wsse_scale = wsse;
         c.lambda = lambda;
         c.extra_bits = mode_extra_bits(kIntraModeCtx[am], kIntraModeCtx[lm], id, b.w, b.h, flags);
@@ -438,6 +442,7 @@ struct Frame {
         c.eob_bin = (uint8_t)eob_chroma(tw * th);  // the chroma eob_bin CDFs
         c.skip_ctx = 7;
         c.qctx = (uint8_t)qctx_of(qidx);
+        c.rflags = trellis_rflags();
         c.wsse_scale = 0.0f;
         c.lambda = lambda;
         return c;

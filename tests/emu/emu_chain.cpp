@@ -1,6 +1,6 @@
-// Test harness: mt_tx_chain (bucket and chain kernels of mtgpu.hip) run lane by lane on the
-// host through the CPU emulation of hip/hip_runtime.h, on blocks / pool read from files;
-// writes the per-block results.
+// Test harness: mt_tx_chain (bucket and chain kernels of mtgpu.hip, linked with the rate /
+// trellis unit) run lane by lane on the host through the CPU emulation of
+// hip/hip_runtime.h, on blocks / pool read from files; writes the per-block results.
 #include "hip/hip_runtime.h"
 #include "../../include/mtgpu.h"
 #include <vector>
@@ -40,13 +40,9 @@ void launch(const char* name, dim3 g, dim3 b, const std::function<void()>& body)
 }
 }  // namespace emu
 namespace mtlib {
-int rate_init_device(int) { return MT_OK; }
 int pred_init_device(int) { return MT_OK; }
 int dist_init_device(int) { return MT_OK; }
 int rdo_init_device(int) { return MT_OK; }
-int trellis_launch(const mt_trellisblk*, int, int32_t*, const float*, float*, int32_t*, hipStream_t) {
-    return MT_ERR_ARG;  // the trellis kernel is not part of this harness
-}
 }
 template <class T> std::vector<T> load(const char* p) {
     FILE* f = fopen(p, "rb");

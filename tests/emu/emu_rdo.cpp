@@ -1,7 +1,8 @@
-// Test harness: mt_intra_rdo with the RDO kernels (mt_rdo.hip) and the superblock
-// reconstruction (mtgpu.hip) run lane by lane on the host through the CPU emulation of
-// hip/hip_runtime.h; the batched candidate evaluation goes through the oracle's
-// orc_eval_batch (the evaluation kernels have their own GPU parity tests).  Prints the
+// Test harness: mt_intra_rdo with the RDO kernels (mt_rdo.hip), the superblock
+// reconstruction (mtgpu.hip) and the rate / trellis unit (mt_rate.hip, linked alongside) run
+// lane by lane on the host through the CPU emulation of hip/hip_runtime.h; the batched
+// candidate evaluation goes through the oracle's orc_eval_batch (the evaluation kernels have
+// their own GPU parity tests).  Prints the
 // stats of both and the number of differing samples and node records.
 #include "hip/hip_runtime.h"
 #include "../../maroontree_amd/csrc/mt_rdo.hip"
@@ -46,13 +47,8 @@ void launch(const char* name, dim3 g, dim3 b, const std::function<void()>& body)
 }  // namespace emu
 
 namespace mtlib {
-int rate_init_device(int) { return MT_OK; }
 int pred_init_device(int) { return MT_OK; }
 int dist_init_device(int) { return MT_OK; }
-int trellis_launch(const mt_trellisblk*, int, int32_t*, const float*, float*, int32_t*, hipStream_t) {
-    return MT_ERR_ARG;  // the trellis kernel is not part of this harness
-}
-float cdf_cost_raw(const uint16_t* raw_cdf, int nsym, int s) { return orc_cdf_cost_raw(raw_cdf, nsym, s); }
 
 static_assert(sizeof(mt_cand) == sizeof(orc_cand), "cand");
 static_assert(sizeof(mt_candres) == sizeof(orc_candres), "candres");

@@ -6,7 +6,10 @@ its 24-bit operand range), g++ compiles mtgpu.hip and mt_rdo.hip against it, and
     checksum of every block) without a 24-bit range violation;
   - mt_intra_rdo (plan, candidate generation, argmin, decisions, the superblock
     reconstruction kernel, final accounting; candidate evaluation through the oracle) equals
-    the oracle's RDO: reconstruction, every node record, the stats.
+    the oracle's RDO: reconstruction, every node record, the stats; also with the trellis
+    (MT_RDO_TRELLIS) in the candidates and in the reconstruction;
+  - mt_tx_chain_trellis (the chain kernels' split halves and the trellis) equals the
+    oracle's chain with the trellis.
 No GPU is involved; the same kernels have their GPU parity tests."""
 import os
 import shutil
@@ -33,17 +36,18 @@ def bins(tmp_path_factory):
     O.lib()  # the oracle library is built
     gpu_o = str(d / "emu_gpu.o")
     subprocess.run(["g++", *FLAGS, "-c", os.path.join(EMU, "emu_gpu.cpp"), "-o", gpu_o], check=True)
-    chain = str(d / "emu_chain")
-    subprocess.run(["g++", *FLAGS, "-o", chain, os.path.join(EMU, "emu_chain.cpp"), gpu_o, "-lpthread"], check=True)
     rate_o = str(d / "emu_rate.o")
     subprocess.run(["g++", *FLAGS, "-c", os.path.join(EMU, "emu_rate.cpp"), "-o", rate_o], check=True)
+    chain = str(d / "emu_chain")
+    subprocess.run(["g++", *FLAGS, "-o", chain, os.path.join(EMU, "emu_chain.cpp"), gpu_o, rate_o, "-lpthread"],
+                   check=True)
     chain_tr = str(d / "emu_chain_trellis")
     subprocess.run(["g++", *FLAGS, "-o", chain_tr, os.path.join(EMU, "emu_chain_trellis.cpp"), gpu_o, rate_o,
                     "-lpthread"], check=True)
     rdo = str(d / "emu_rdo")
     odir = os.path.join(ROOT, "oracle")
-    subprocess.run(["g++", *FLAGS, "-o", rdo, os.path.join(EMU, "emu_rdo.cpp"), gpu_o, "-L" + odir, "-loracle",
-                    "-lpthread", "-Wl,-rpath," + odir], check=True)
+    subprocess.run(["g++", *FLAGS, "-o", rdo, os.path.join(EMU, "emu_rdo.cpp"), gpu_o, rate_o, "-L" + odir,
+                    "-loracle", "-lpthread", "-Wl,-rpath," + odir], check=True)
     return d, chain, rdo, chain_tr
 
 
@@ -87,7 +91,8 @@ def test_emulated_chain_matches_oracle_on_extreme_inputs(bins):
     assert (blocks["tx_size"] == 4).any()  # the 64x64 kernel ran
 
 
-@pytest.mark.parametrize("bd,qidx,w,h,flags", [(10, 77, 128, 72, 7), (8, 160, 72, 64, 3)])
+@pytest.mark.parametrize("bd,qidx,w,h,flags", [(10, 77, 128, 72, 7), (8, 160, 72, 64, 3), (10, 30, 128, 72, 7 | 32),
+                                               (8, 120, 136, 64, 1 | 4 | 32)])
 def test_emulated_rdo_matches_oracle(bins, bd, qidx, w, h, flags):
     d, _, rdo, _ = bins
     img = W.synth_image(100 + qidx, w, h, bd)

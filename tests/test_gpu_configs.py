@@ -184,3 +184,30 @@ def test_trained_network_beats_untrained_under_same_policy(dev, trained):
     assert bd_t < bd_u
     assert bd_t < 1.0
     assert cands["trained"] < cands["full"]
+
+
+# ---- the Slow chain: trellis in every candidate and in the reconstruction ----------------
+@pytest.mark.parametrize("fmt,bd,q,w,h", [(R.CHROMA_420, 10, 91, 200, 136), (R.CHROMA_444, 8, 70, 136, 128),
+                                          (R.CHROMA_420, 12, 30, 128, 72)])
+def test_rdo_with_trellis_matches_oracle(dev, fmt, bd, q, w, h):
+    """MT_RDO_TRELLIS: trellis_optimize_ctx between quantization and dequantization in every
+    luma and chroma candidate and in the superblock reconstruction; the whole RDO against
+    the oracle's RDO with the same trellis, and fewer bits than without it."""
+    sub = 2 if fmt == R.CHROMA_420 else 0
+    y, u, v = W.synth_yuv(600 + q, w, h, bd, sub)
+    print(f"rdo+trellis {w}x{h} bd{bd} q{q} fmt{fmt}", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, bd, q, fmt)], ALL | R.TRELLIS)[0]
+    plain = R.intra_rdo_batch([gpu_frame(dev, y, u, v, bd, q, fmt)], ALL)[0]
+    torch.cuda.synchronize()
+    assert_same("trellis", got, oracle(y, u, v, bd, q, fmt, ALL | R.TRELLIS))
+    assert got[2]["bits"] < plain[2]["bits"]
+
+
+def test_full_4k_10bit_420_q50_with_trellis_matches_oracle(dev):
+    """The exact full intra RDO configuration with the Slow chain (the reference's default
+    speed): the whole 3840x2160 10-bit 4:2:0 frame at qindex 91 with the trellis."""
+    y, u, v = W.synth_yuv(SEED, 3840, 2160, 10, 2)
+    print("config 3840x2160 10-bit 4:2:0 q91 trellis", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 10, 91, R.CHROMA_420)], ALL | R.TRELLIS)[0]
+    torch.cuda.synchronize()
+    assert_same("4k trellis", got, oracle(y, u, v, 10, 91, R.CHROMA_420, ALL | R.TRELLIS))
