@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="wall-time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-slow", action="store_true",
+                    help="skip the chain-with-trellis leg and the Slow-mode RDO")
     ap.add_argument("--no-eval", action="store_true", help="skip the candidate-evaluation leg")
     ap.add_argument("--eval-steps", type=int, default=10)
     ap.add_argument("--eval-only", action="store_true",
@@ -272,6 +274,9 @@ def main():
                       f"after a warm-up ({best:.2f} s), std::thread over blocks",
             **cpu_info(threads),
         }
+    if not args.no_slow:
+        line["slow_chain"] = bench_slow_chain(args, dev, torch, mt, blocks, pool, bt, pt, res)
+    del bt, pt, res
     if not args.no_eval:
         line["eval"] = bench_eval(args, dev, torch, mt)
     if not args.no_rdo:
@@ -284,6 +289,89 @@ def main():
     if not args.no_stream:
         line["stream"] = bench_stream(args, dev, torch, mt, net)
     print(json.dumps(line), flush=True)
+
+
+def bench_slow_chain(args, dev, torch, mt, blocks, pool, bt, pt, res):
+    """The headline batch through mt_tx_chain_trellis: the chain with trellis_optimize_ctx
+    (src/trellis.rs:606) on every block between quantization and dequantization, the
+    reference's Slow speed (its default).  Descriptors: the frame-level context and lambda
+    of each block's qindex and bit depth at 4:2:0."""
+    from maroontree_amd import workload as W
+    n = len(blocks)
+    tr = W.chain_trellis_descriptors(blocks, sub=2)
+    trt = torch.from_numpy(tr.view(np.uint8).copy()).to(dev)
+    npx = len(pool)
+    lv = torch.empty(npx, dtype=torch.int32, device=dev)
+    tg = torch.empty(npx, dtype=torch.float32, device=dev)
+    rec = torch.empty(npx, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        mt.tx_chain_trellis(bt, trt, pt, res, lv, tg, rec, stream=stream)
+    torch.cuda.synchronize()
+    mt.profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mt.tx_chain_trellis(bt, trt, pt, res, lv, tg, rec, stream=stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    prof = mt.profile_end()
+    print(f"[bench] chain+trellis: {ms:.3f} ms/step, {npx / ms / 1e3:.0f} Mpx/s", file=sys.stderr, flush=True)
+    w = np.array([mt.tx_width(s) for s in range(19)], np.int64)
+    h = np.array([mt.tx_height(s) for s in range(19)], np.int64)
+    ncoef = int((np.minimum(w, 32) * np.minimum(h, 32))[blocks["tx_size"]].sum())
+    t_ms = prof["trellis"][0] / max(prof["trellis"][1], 1)
+    # algorithmic bytes of one trellis launch: levels read and written, targets read (4 B per
+    # coded coefficient each), one 24 B descriptor per block
+    tbytes = ncoef * 12 + n * tr.itemsize
+    tach = tbytes / (t_ms / 1000.0) / 1e9
+    r = {
+        "workload": "the headline batch (2^20 blocks, 13 TX sizes) through fwd -> quant -> trellis_optimize_ctx "
+                    "-> dequant -> inv -> SSE; per-block qindex context and Slow-mode lambda at 4:2:0",
+        "ms_per_step": round(ms, 4),
+        "mpx_per_s": round(npx / (ms / 1000.0) / 1e6, 2),
+        "candidates_per_s": round(n / (ms / 1000.0), 1),
+        "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
+        "trellis_roofline": {"kernel": "trellis_kernel", "bound": "hbm", "achieved": round(tach, 2),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tach / HBM_PEAK_GBS, 4),
+                             "traffic": None, "bytes_per_launch": tbytes,
+                             "note": "one thread per block walks its scan serially; latency bound"},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = host_cpu()[0]
+        bdi = (blocks["bd_qm"] >> 4) & 3
+        bd = np.where(bdi == 1, 10, np.where(bdi == 2, 12, 8)).astype(np.uint8)
+        px_of = w * h
+        off = blocks["offset"].astype(np.uint64)
+
+        def run(m):
+            O.chain_batch_trellis(blocks["tx_size"][:m], blocks["tx_type"][:m], blocks["qidx"][:m], bd[:m],
+                                  blocks["bd_qm"][:m] & 0x4F, off[:m], pool, tr[:m], threads)
+        m = 1 << 14
+        while m < n:
+            t0 = time.perf_counter()
+            run(m)
+            if time.perf_counter() - t0 > args.cpu_seconds / 10.0:
+                break
+            m = min(n, m * 2)
+        sample_px = int(px_of[blocks["tx_size"][:m]].sum())
+        run(m)
+        best = float("inf")
+        for _ in range(5):
+            t0 = time.perf_counter()
+            run(m)
+            best = min(best, time.perf_counter() - t0)
+        r["cpu_baseline"] = {
+            "value": round(sample_px / best / 1e6, 2),
+            "unit": "Mpx/s",
+            "kind": "port",
+            "sample": f"first {m} blocks ({sample_px / 1e6:.1f} Mpx) of the same batch; best of 5 passes "
+                      f"after a warm-up ({best:.2f} s), std::thread over blocks",
+            **cpu_info(threads),
+        }
+    return r
 
 
 def bench_eval(args, dev, torch, mt):
@@ -433,7 +521,7 @@ def bench_rdo(args, dev, torch, mt):
             "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
     }
-    if not args.no_cpu:
+    def cpu_crop(fl):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
         threads = host_cpu()[0]
@@ -443,10 +531,10 @@ def bench_rdo(args, dev, torch, mt):
         uvq = R.chroma_laws(qidx, 2)
         t0 = time.perf_counter()
         _, _, _, _, cst = O.intra_rdo_yuv(np.ascontiguousarray(y[:ch, :cw]), np.ascontiguousarray(u[:ch // 2, :cw // 2]),
-                                          np.ascontiguousarray(v[:ch // 2, :cw // 2]), bd, qidx, qm, flags, lam,
+                                          np.ascontiguousarray(v[:ch // 2, :cw // 2]), bd, qidx, qm, fl, lam,
                                           R.CHROMA_420, *uvq, nthreads=threads)
         cpu_s = time.perf_counter() - t0
-        r["cpu_baseline"] = {
+        return {
             "value": round(cw * ch / cpu_s / 1e6, 4),
             "unit": "Mpx/s",
             "candidates_per_s": round(cst["candidates"] / cpu_s, 1),
@@ -455,7 +543,31 @@ def bench_rdo(args, dev, torch, mt):
                       f"candidates), {cpu_s:.1f} s wall, std::thread over the superblocks of each wavefront step",
             **cpu_info(threads),
         }
+
+    if not args.no_cpu:
+        r["cpu_baseline"] = cpu_crop(flags)
         r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
+    if not args.no_slow:
+        # Slow speed (the reference's default): the trellis in every candidate's evaluation
+        # and in the reconstruction of the decisions
+        ms_s, st_s, prof_s = timed(frame, flags | R.TRELLIS, args.rdo_steps, prof=True)
+        print(f"[bench] rdo 4:2:0 slow: {ms_s:.2f} ms/frame", file=sys.stderr, flush=True)
+        r["slow"] = {
+            "flags": "ANGLE_DELTAS | FILTER_INTRA | MIN4 | CFL | TRELLIS",
+            "ms_per_frame": round(ms_s, 3),
+            "mpx_per_s": round(npx / (ms_s / 1000.0) / 1e6, 2),
+            "candidates": st_s["candidates"],
+            "candidates_per_s": round(st_s["candidates"] / (ms_s / 1000.0), 1),
+            "bits": round(st_s["bits"], 1),
+            "bits_uv": round(st_s["bits_uv"], 1),
+            "psnr_y": round(R.psnr(st_s["sse"], npx, bd), 4),
+            "psnr_u": round(R.psnr(st_s["sse_u"], npx // 4, bd), 4),
+            "psnr_v": round(R.psnr(st_s["sse_v"], npx // 4, bd), 4),
+            "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof_s.items() if v[1]},
+        }
+        if not args.no_cpu:
+            r["slow"]["cpu_baseline"] = cpu_crop(flags | R.TRELLIS)
+            r["slow"]["speedup_vs_cpu"] = round(r["slow"]["mpx_per_s"] / r["slow"]["cpu_baseline"]["value"], 1)
     return r
 
 

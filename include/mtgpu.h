@@ -206,17 +206,17 @@ typedef struct {
  * of the forward transform (both x-major at each block's offset, ncoef per block; 32x32 for
  * 64x64).  Without MT_TRELLIS_CTX: trellis_optimize (src/trellis.rs:944) with the x86
  * AVX2+FMA round-down (src/avx/trellis.rs:196); with it: trellis_optimize_ctx
- * (src/trellis.rs:55) on the shipped switches.  work_d: a float workspace laid out like
- * levels_d (overwritten).  status_d (optional, n int32): 0, or MT_ERR_ARG for an invalid
- * descriptor (its levels untouched). */
+ * (src/trellis.rs:55) on the shipped switches.  work_d: unused (may be null; the
+ * trellis works in on-chip memory).  status_d (optional, n int32): 0, or MT_ERR_ARG for an
+ * invalid descriptor (its levels untouched).  Uses a per-device sorting list that the
+ * library grows on demand: calls that overlap on one device race. */
 int mt_trellis(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
                float* work_d, int32_t* status_d, void* stream);
 /* The chain with the trellis between quantization and dequantization, as the reference's
  * candidate loops run it at Speed::Slow (src/coder/partition_search.rs:509-513): forward +
  * quantization, mt_trellis with trellis_d[i] (the same offset, size and quantizer as
  * blocks_d[i]), then dequantization + inverse + SSE / nnz / checksum of the final levels as
- * mt_tx_chain reports them.  levels_d, targets_d and recon_d are required (recon_d also
- * serves as the trellis's workspace); blocks of an unsupported (size, type) get sse = -1
+ * mt_tx_chain reports them.  levels_d, targets_d and recon_d are required; blocks of an unsupported (size, type) get sse = -1
  * and unspecified levels. */
 int mt_tx_chain_trellis(const mt_txblk* blocks_d, const mt_trellisblk* trellis_d, int n,
                         const int16_t* pool_d, mt_txres* res_d, int32_t* levels_d, float* targets_d,

@@ -258,12 +258,13 @@ TRELLISBLK_DTYPE = np.dtype([("offset", "<u4"), ("tx_size", "u1"), ("qidx", "u1"
                              ("pad1", "u1"), ("lambda0", "<f4"), ("band_tilt", "<f4")])
 
 
-def trellis(blocks, levels, targets, work, status=None, stream=None):
+def trellis(blocks, levels, targets, work=None, status=None, stream=None):
     """mt_trellis: blocks uint8 cuda tensor of TRELLISBLK_DTYPE records; levels int32 (rewritten
-    in place), targets float32 and work float32 cuda tensors laid out alike."""
+    in place) and targets float32 cuda tensors laid out alike; work is not used."""
     n = blocks.numel() // TRELLISBLK_DTYPE.itemsize
     init(blocks.device.index or 0)
-    _check(lib().mt_trellis(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(targets), _dev_ptr(work),
+    _check(lib().mt_trellis(_dev_ptr(blocks), n, _dev_ptr(levels), _dev_ptr(targets),
+                            _dev_ptr(work) if work is not None else None,
                             _dev_ptr(status) if status is not None else None, _stream_ptr(stream)))
 
 

@@ -28,7 +28,7 @@ int tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_t
                     hipStream_t stream, const mt_trellisblk* trellis_d = nullptr);
 // the trellis over levels in place (mt_rate.hip; mt_trellis without argument checks)
 int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
-                   float* work_d, int32_t* status_d, hipStream_t s);
+                   int32_t* status_d, hipStream_t s);
 // mt_intra_pred with an optional residual output (src - pred, int16, at each out_offset; the
 // source plane index in each descriptor's pad byte).
 int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int32_t* pred_d,

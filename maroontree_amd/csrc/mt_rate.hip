@@ -513,16 +513,115 @@ void build_tables(const mt_coef_cdfs& c, RateTables& t) {
     }
 }
 
-__global__ void __launch_bounds__(256) trellis_kernel(const mt_trellisblk* __restrict__ blocks, int n,
-                                                     int32_t* __restrict__ levels,
-                                                     const float* __restrict__ targets,
-                                                     float* __restrict__ work, int32_t* __restrict__ status,
-                                                     mtlib::TrellisTabs tabs) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const mt_trellisblk d = blocks[b];
-    const int rc = trellis_block(d, levels + d.offset, targets + d.offset, work + d.offset, tabs);
-    if (status) status[b] = rc;
+// The trellis's lane-group classes: the coded sizes by coefficient count and shorter side
+// (the longest anti-diagonal); a block runs on as many lanes as its shorter side.
+constexpr int kTrClasses = 8;
+
+__device__ inline int trellis_class(const TrellisGeom& g) {
+    switch (g.N) {
+        case 16: return 0;
+        case 32: return 1;
+        case 64: return min(g.w, g.h) == 4 ? 2 : 3;
+        case 128: return 4;
+        case 256: return 5;
+        case 512: return 6;
+        default: return 7;
+    }
+}
+
+// Sorts the blocks that need work into one list per class (class c's list at order + c * n,
+// its length in counts[c]); writes the status of the others: MT_ERR_ARG for an invalid
+// descriptor, MT_OK for lambda0 <= 0.  One reservation per class per workgroup.
+__global__ void __launch_bounds__(256) trellis_sort_kernel(const mt_trellisblk* __restrict__ blocks, int n,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ order,
+                                                          int32_t* __restrict__ status) {
+    __shared__ uint32_t h[kTrClasses], basepos[kTrClasses];
+    const int chunk_begin = blockIdx.x * 4096;
+    const int chunk_end = min(n, chunk_begin + 4096);
+    if (threadIdx.x < kTrClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
+        const mt_trellisblk d = blocks[i];
+        TrellisGeom g;
+        if (!trellis_valid(d, g)) {
+            if (status) status[i] = MT_ERR_ARG;
+        } else if (!(d.lambda0 > 0.0f)) {
+            if (status) status[i] = MT_OK;
+        } else {
+            atomicAdd(&h[trellis_class(g)], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kTrClasses) {
+        const int c = threadIdx.x;
+        basepos[c] = h[c] ? atomicAdd(&counts[c], h[c]) : 0;
+        h[c] = 0;
+    }
+    __syncthreads();
+    for (int i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
+        const mt_trellisblk d = blocks[i];
+        TrellisGeom g;
+        if (trellis_valid(d, g) && d.lambda0 > 0.0f) {
+            const int c = trellis_class(g);
+            order[(size_t)c * n + basepos[c] + atomicAdd(&h[c], 1u)] = (uint32_t)i;
+        }
+    }
+}
+
+// The blocks of one class (NE coefficients, G lanes each), 64 / G per one-wave workgroup,
+// persistent over the class's list: each group stages its block's levels and targets in LDS
+// (16-byte loads where aligned), runs trellis_group, and writes the levels back.
+template <int G, int NE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) trellis_group_kernel(const mt_trellisblk* __restrict__ blocks,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ count,
+                                                          int32_t* __restrict__ levels,
+                                                          const float* __restrict__ targets,
+                                                          int32_t* __restrict__ status, mtlib::TrellisTabs tabs) {
+    constexpr int NG = 64 / G;
+    __shared__ __align__(16) int32_t s_cf[NG * NE];
+    __shared__ __align__(16) float s_tf[NG * NE];
+    __shared__ __align__(16) float s_a[NG * NE];
+    __shared__ __align__(16) float s_b[NG * NE];
+    __shared__ uint8_t s_lb[NG * NE];
+    const int g = threadIdx.x / G, gl = threadIdx.x & (G - 1);
+    int32_t* cf = s_cf + g * NE;
+    float* tf = s_tf + g * NE;
+    const uint32_t cnt = *count;
+    for (uint32_t base = blockIdx.x * NG; base < cnt; base += gridDim.x * NG) {
+        const uint32_t j = base + g;
+        const bool have = j < cnt;
+        const uint32_t b = have ? list[j] : 0u;
+        mt_trellisblk d{};
+        if (have) d = blocks[b];
+        TrellisGeom gm{};
+        const int ne = have && trellis_geom(d.tx_size, gm) && gm.N == NE ? NE : 0;
+        int32_t* lv = levels + d.offset;
+        const float* tg = targets + d.offset;
+        const bool vec = ((reinterpret_cast<uintptr_t>(lv) | reinterpret_cast<uintptr_t>(tg)) & 15) == 0;
+        if (vec) {
+            for (int k = gl; k < ne / 4; k += G) {
+                reinterpret_cast<int4*>(cf)[k] = reinterpret_cast<const int4*>(lv)[k];
+                reinterpret_cast<float4*>(tf)[k] = reinterpret_cast<const float4*>(tg)[k];
+            }
+        } else {
+            for (int k = gl; k < ne; k += G) {
+                cf[k] = lv[k];
+                tf[k] = tg[k];
+            }
+        }
+        const int rc = trellis_group<G>(d, ne > 0, cf, tf, s_a + g * NE, s_b + g * NE, s_lb + g * NE, tabs);
+        if (rc == MT_OK) {
+            if (vec) {
+                for (int k = gl; k < ne / 4; k += G) reinterpret_cast<int4*>(lv)[k] = reinterpret_cast<const int4*>(cf)[k];
+            } else {
+                for (int k = gl; k < ne; k += G) lv[k] = cf[k];
+            }
+        }
+        if (have && gl == 0 && status) status[b] = ne > 0 ? rc : MT_ERR_ARG;
+        __syncthreads();  // the write-back has read the LDS the next block is staged into
+    }
 }
 
 struct ScanTabs {
@@ -686,16 +785,59 @@ extern "C" int mt_rate_load_cdfs(int slot, const mt_coef_cdfs* cdfs) {
     return MT_OK;
 }
 
+namespace {
+struct TrellisWork {
+    size_t cap = 0;
+    uint32_t* order = nullptr;   // kTrClasses lists of cap entries
+    uint32_t* counts = nullptr;  // kTrClasses
+};
+std::mutex g_trellis_mu;
+TrellisWork g_trellis_work[64];
+
+template <int G, int NE>
+void launch_trellis_class(const mt_trellisblk* blocks_d, const TrellisWork& w, int c, int n, int cus,
+                          int32_t* levels_d, const float* targets_d, int32_t* status_d, hipStream_t s, int dev) {
+    constexpr int NG = 64 / G;
+    const int grid = std::max(1, std::min((n + NG - 1) / NG, cus * 8));
+    hipLaunchKernelGGL((trellis_group_kernel<G, NE>), dim3(grid), dim3(64), 0, s, blocks_d, w.order + (size_t)c * n,
+                       w.counts + c, levels_d, targets_d, status_d, g_tabs[dev & 63]);
+}
+}  // namespace
+
 namespace mtlib {
+// Sorts the blocks into lane-group classes (per-device lists, grown on demand and shared by
+// the streams of the device: overlapping calls on one device race), then one persistent
+// launch per class.
 int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
-                   float* work_d, int32_t* status_d, hipStream_t s) {
+                   int32_t* status_d, hipStream_t s) {
     if (n == 0) return MT_OK;
     int dev = 0, cus = 0;
     int rc = device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_trellis_mu);
+    TrellisWork& w = g_trellis_work[dev & 63];
+    if (w.cap < (size_t)n) {
+        if (w.order) MT_HIP_TRY(hipFree(w.order));
+        w.order = nullptr;
+        w.cap = 0;
+        const size_t cap = n < 65536 ? 65536 : (size_t)n;
+        MT_HIP_TRY(hipMalloc(&w.order, kTrClasses * cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.counts) MT_HIP_TRY(hipMalloc(&w.counts, kTrClasses * sizeof(uint32_t)));
     ProfScope prof(kProfTrellis, s);
-    hipLaunchKernelGGL(trellis_kernel, dim3((n + 255) / 256), dim3(256), 0, s, blocks_d, n, levels_d, targets_d,
-                       work_d, status_d, g_tabs[dev & 63]);
+    hipLaunchKernelGGL(zero_counts_kernel<kTrClasses>, dim3(1), dim3(64), 0, s, w.counts);
+    hipLaunchKernelGGL(trellis_sort_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n, w.counts,
+                       w.order, status_d);
+    // the lists are n apart whatever the capacity, so their offsets match the sort's
+    launch_trellis_class<4, 16>(blocks_d, w, 0, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<4, 32>(blocks_d, w, 1, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<4, 64>(blocks_d, w, 2, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<8, 64>(blocks_d, w, 3, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<8, 128>(blocks_d, w, 4, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<16, 256>(blocks_d, w, 5, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<16, 512>(blocks_d, w, 6, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<32, 1024>(blocks_d, w, 7, n, cus, levels_d, targets_d, status_d, s, dev);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }
@@ -703,9 +845,10 @@ int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, cons
 
 extern "C" int mt_trellis(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
                           float* work_d, int32_t* status_d, void* stream) {
-    if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !targets_d || !work_d)))
+    if (n < 0 || (n > 0 && (!blocks_d || !levels_d || !targets_d)))
         return mtlib::set_error(MT_ERR_ARG, "mt_trellis: bad pointer or count");
-    return mtlib::trellis_launch(blocks_d, n, levels_d, targets_d, work_d, status_d, (hipStream_t)stream);
+    (void)work_d;  // the trellis works in LDS
+    return mtlib::trellis_launch(blocks_d, n, levels_d, targets_d, status_d, (hipStream_t)stream);
 }
 
 // trellis_lambda() * hbd_trellis_scale (src/cost.rs:110, src/coder/lossy_state.rs:158): above

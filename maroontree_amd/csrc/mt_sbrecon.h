@@ -23,14 +23,16 @@ struct ReconTrellis {
     const mtlib::TrellisTabs* tabs;
     int32_t* lv;
     float* tg;
-    float* suf;
+    float* suf;   // the trellis's workspaces (16-byte aligned): pre is the block's residual
+    float* pre;   // buffer and lb the chain's work buffer, both free from the forward's last
+    uint8_t* lb;  // read of them to the dequantization
 };
 
 // Square chain of one N x N block (N <= 32) by one wave: lane i runs line i of each pass.
 // type: DCT_DCT, or at N <= 16 ADST_DCT / DCT_ADST / ADST_ADST (vertical kind first, the
 // 2-D types the chain kernels run with the same per-size rules).  res: the residual (LDS,
 // row-major); out: the reconstructed residual (may alias res); work: LDS of N * (N + 1) ints.
-// With tr.d: the levels and targets go to LDS, one lane runs the trellis over them, and the
+// With tr.d: the levels and targets go to LDS, the wave runs the trellis over them, and the
 // dequantization reads the trellis's levels.
 template <int N>
 __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p,
@@ -83,7 +85,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
     }
     __syncthreads();
     if (tr.d) {
-        if (lane == 0) (void)trellis_block(*tr.d, tr.lv, tr.tg, tr.suf, *tr.tabs);
+        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb, *tr.tabs);
         __syncthreads();
         if (lane < N) {
             const int fy = lane;
@@ -204,7 +206,7 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
     }
     __syncthreads();
     if (tr.d) {
-        if (lane == 0) (void)trellis_block(*tr.d, tr.lv, tr.tg, tr.suf, *tr.tabs);
+        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb, *tr.tabs);
         __syncthreads();
         for (int rc = lane; rc < 1024; rc += 64) {
             int32_t step = rc == 0 ? p.dc : p.ac;
@@ -246,9 +248,9 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
                                                       const mtlib::ReconFrame* __restrict__ frames,
                                                       mtlib::TrellisTabs tabs) {
     __shared__ int32_t s_pred[4096];
-    __shared__ int32_t s_res[4096];
+    __shared__ __align__(16) int32_t s_res[4096];
     __shared__ int32_t s_a[kEdge], s_l[kEdge];
-    __shared__ int32_t s_fi[33 * 33];
+    __shared__ __align__(16) int32_t s_fi[33 * 33];
     __shared__ double s_vt[16 * 64];
     __shared__ int32_t s_work[32 * 33];
     const int lane = threadIdx.x;
@@ -272,7 +274,8 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
     td.qctx = (uint8_t)mtlib::rdo_qctx(fr.qidx);
     td.lambda0 = trellis_lambda0(fr.qidx, prm.bd, fr.tsub);
     const ReconTrellis rt{fr.trellis ? &td : nullptr, &tabs, reinterpret_cast<int32_t*>(s_vt),
-                          reinterpret_cast<float*>(s_vt) + 1024, reinterpret_cast<float*>(s_fi)};
+                          reinterpret_cast<float*>(s_vt) + 1024, reinterpret_cast<float*>(s_fi),
+                          reinterpret_cast<float*>(s_res), reinterpret_cast<uint8_t*>(s_work)};
     for (uint32_t i = 0; i < n; i++) {
         const mt_predblk d = leaves[f0 + i];
         const int s = d.bw, npx = s * s, lw = __builtin_ctz(s);
@@ -316,9 +319,9 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
                                                          const mtlib::ReconFrame* __restrict__ frames,
                                                          mtlib::TrellisTabs tabs) {
     __shared__ int32_t s_pred[1024];
-    __shared__ int32_t s_res[1024];
+    __shared__ __align__(16) int32_t s_res[1024];
     __shared__ int32_t s_a[kEdge], s_l[kEdge];
-    __shared__ int32_t s_fi[33 * 33];
+    __shared__ __align__(16) int32_t s_fi[33 * 33];
     __shared__ int32_t s_work[32 * 33];
     __shared__ int32_t s_tr[2048];  // the trellis's levels and targets
     const int lane = threadIdx.x;
@@ -343,7 +346,8 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
     td.plane = 1;
     td.lambda0 = trellis_lambda0(fr.qidx, prm.bd, fr.tsub);
     const ReconTrellis rt{fr.trellis ? &td : nullptr, &tabs, s_tr, reinterpret_cast<float*>(s_tr) + 1024,
-                          reinterpret_cast<float*>(s_fi)};
+                          reinterpret_cast<float*>(s_fi), reinterpret_cast<float*>(s_res),
+                          reinterpret_cast<uint8_t*>(s_work)};
     const mt_plane luma{fr.rec, fr.rstride, fr.src.width, fr.src.height, fr.src.width, fr.src.height, 0};
     for (uint32_t i = 0; i < n; i++) {
         mt_predblk d = desc[f0 + i];

@@ -196,91 +196,122 @@ __device__ inline float trellis_lambda0(int qidx, int bd, int sub) {
     return __fmul_rn(0.05f, scale);
 }
 
-// The trellis of one block d over its levels cf and targets tf (x-major; generic pointers:
-// global or LDS), with suf a float workspace of the block's coefficient count.  Returns
-// MT_ERR_ARG (levels untouched) for an invalid descriptor, else MT_OK.
-__device__ int trellis_block(const mt_trellisblk& d, int32_t* cf, const float* tf, float* suf,
-                             const mtlib::TrellisTabs& T) {
-    TrellisGeom G;
-    const bool ctxv = d.flags & MT_TRELLIS_CTX;
+// Whether descriptor d names a block the trellis can run on (its coded geometry in g).
+__device__ inline bool trellis_valid(const mt_trellisblk& d, TrellisGeom& g) {
     const int bdi = (d.bd_qm >> 4) & 3;
-    bool ok = trellis_geom(d.tx_size, G) && bdi < 3 && !(d.flags & ~MT_TRELLIS_CTX);
-    if (ok && ctxv) {
-        const int ebn = d.eob_bin < 3 ? 16 : d.eob_bin < 5 ? 32 : d.eob_bin < 8 ? 64 : d.eob_bin < 11 ? 128
-                      : d.eob_bin < 13 ? 256 : d.eob_bin < 15 ? 512 : 1024;
-        ok = d.eob_bin <= 16 && ebn == G.N && d.dcs_ctx <= 2 && d.plane <= 1 && d.qctx < MT_RATE_SLOTS;
+    if (!trellis_geom(d.tx_size, g) || bdi >= 3 || (d.flags & ~MT_TRELLIS_CTX)) return false;
+    if (!(d.flags & MT_TRELLIS_CTX)) return true;
+    const int ebn = d.eob_bin < 3 ? 16 : d.eob_bin < 5 ? 32 : d.eob_bin < 8 ? 64 : d.eob_bin < 11 ? 128
+                  : d.eob_bin < 13 ? 256 : d.eob_bin < 15 ? 512 : 1024;
+    return d.eob_bin <= 16 && ebn == g.N && d.dcs_ctx <= 2 && d.plane <= 1 && d.qctx < MT_RATE_SLOTS;
+}
+
+// Maximum / bitwise or over each aligned group of G lanes of the wave (G a power of two, at
+// most 64).
+template <int G>
+__device__ inline int group_max(int v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+template <int G>
+__device__ inline unsigned long long group_or(unsigned long long v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+    return v;
+}
+
+// First scan index of the anti-diagonal x + y == dd of a w x h block whose scan visits the
+// anti-diagonals one after another (every scan the trellis reads does).
+__device__ inline int diag_start(int dd, int w, int h) {
+    const int a = min(w, h), b = max(w, h);
+    if (dd <= a) return dd * (dd + 1) / 2;
+    if (dd <= b) return a * (a + 1) / 2 + (dd - a) * a;
+    const int k = dd - b;
+    return a * (a + 1) / 2 + (b - a) * a + k * (a - 1) - k * (k - 1) / 2;
+}
+
+// The trellis of one block per aligned group of G lanes (G >= 2, a power of two), the whole
+// workgroup being one wave that calls this together: d the group's descriptor (have false:
+// no block), cf / tf its x-major levels and targets in LDS, sa / sb LDS workspaces of one
+// float per coefficient (16-byte aligned), lb an LDS byte per coefficient.  Every f32
+// operation is the one of the reference's sequential walk, in its order; only independent
+// work is spread over the lanes:
+//   - trellis_optimize_ctx's Step A visits the scan backwards and prices each level against
+//     the final levels of the context neighbours (x+1, y), (x, y+1), (x+1, y+1), (x+2, y),
+//     (x, y+2) of the stride-h map, or, where y + 1 or y + 2 runs past the column, the next
+//     column's first entries.  The former lie on later anti-diagonals, the latter on earlier
+//     ones, and the scans run anti-diagonal by anti-diagonal, so the positions of one
+//     anti-diagonal read no level another position of it writes: the anti-diagonals run
+//     one after another from the last, their positions in parallel, and anti-diagonals
+//     without a nonzero level (where the walk changes nothing) are skipped.
+//   - the eob search's running sums (the suffix of the zero-level distortions, the prefix of
+//     the kept-levels cost) are sequential f32 sums: one lane of the group runs each, over
+//     per-position terms the group computed in parallel and laid out in the order of the
+//     sum; the candidate costs that read them are again per position, and the earliest
+//     strict minimum is a group reduction.
+// Returns -1 without a block, MT_ERR_ARG (levels untouched) for an invalid descriptor, else
+// MT_OK.  cf / tf must be visible to the group on entry; cf holds the result on return.
+template <int G>
+__device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, const float* tf, float* sa, float* sb,
+                             uint8_t* lb, const mtlib::TrellisTabs& T) {
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group of 2..64 lanes");
+    const int gl = threadIdx.x & (G - 1);
+    TrellisGeom g{};
+    const bool ok = have && trellis_valid(d, g);
+    const int status = !have ? -1 : ok ? MT_OK : MT_ERR_ARG;
+    bool act = ok && d.lambda0 > 0.0f;
+    const bool ctxv = d.flags & MT_TRELLIS_CTX;
+    const int N = act ? g.N : 0, h = act ? g.h : 1, log2h = act ? g.log2h : 0;
+    const uint16_t* scan = T.scan + (act ? g.so : 0);
+    // the quantizer steps (Quant::new_with_qm / new_chroma_with_delta_qm) and lambda
+    float dc_q = 1.0f, ac_q = 1.0f;
+    if (act) {
+        const int bdi = (d.bd_qm >> 4) & 3, qi = d.qidx;
+        const bool chroma_q = d.bd_qm & MT_TXBLK_CHROMA;
+        const int dci = chroma_q ? min(255, max(0, qi + d.dc_delta)) : max(0, qi - 8);
+        const int aci = chroma_q ? min(255, max(0, qi + d.ac_delta)) : qi;
+        dc_q = (float)T.dcq[bdi * 256 + dci];
+        ac_q = (float)T.acq[bdi * 256 + aci];
     }
-    if (!ok) return MT_ERR_ARG;
-    if (!(d.lambda0 > 0.0f)) return MT_OK;
-    // the quantizer steps (Quant::new_with_qm / new_chroma_with_delta_qm)
-    const bool chroma_q = d.bd_qm & MT_TXBLK_CHROMA;
-    const int qi = d.qidx;
-    const int dci = chroma_q ? min(255, max(0, qi + d.dc_delta)) : max(0, qi - 8);
-    const int aci = chroma_q ? min(255, max(0, qi + d.ac_delta)) : qi;
-    const float dc_q = (float)T.dcq[bdi * 256 + dci], ac_q = (float)T.acq[bdi * 256 + aci];
-    const int qm_level = d.bd_qm & 15;
-    const uint16_t* scan = T.scan + G.so;
-    const int N = G.N, h = G.h, log2h = G.log2h;
-    const float lam = trellis_lambda(dc_q, d.lambda0);
+    const float lam = trellis_lambda(dc_q, act ? d.lambda0 : 0.05f);
     const float q2dc = __fmul_rn(dc_q, dc_q), q2ac = __fmul_rn(ac_q, ac_q);
-    int eob = N - 1;
-    while (eob >= 0 && cf[scan[eob]] == 0) eob--;
-    if (eob < 0) return MT_OK;
-    const int m = eob + 1;
-
-    if (!ctxv) {
-        // trellis_optimize (src/trellis.rs:944)
-        int i = 0;
-        for (; i + 8 <= m; i += 8) round_down_chunk(cf, tf, scan, i, 8, q2dc, q2ac, lam);
-        for (; i + 4 <= m; i += 4) round_down_chunk(cf, tf, scan, i, 4, q2dc, q2ac, lam);
-        for (; i < m; i++) round_down_one(cf, tf, scan[i], scan[i] == 0 ? q2dc : q2ac, lam);
-        float acc = 0.0f;
-        for (int k = m - 1; k >= 0; k--) {
-            const uint32_t rc = scan[k];
-            const float at = fabsf(tf[rc]);
-            acc = __fadd_rn(acc, __fmul_rn(rc == 0 ? q2dc : q2ac, __fmul_rn(at, at)));
-            suf[k] = acc;
+    __syncthreads();
+    // the end of block, the level bytes of the context sums, the anti-diagonals holding a
+    // nonzero level
+    int e_loc = -1;
+    unsigned long long dm = 0;
+    for (int i = gl; i < N; i += G) {
+        const uint32_t rc = scan[i];
+        const int32_t c = cf[rc];
+        lb[rc] = (uint8_t)level_byte(uabs32(c));
+        if (c != 0) {
+            e_loc = i;
+            dm |= 1ull << ((rc >> log2h) + (rc & (h - 1)));
         }
-        int best_e = -1;
-        float best = INFINITY, pre = 0.0f;
-        for (int e = 0; e < m; e++) {
-            const uint32_t rc = scan[e];
-            const uint32_t l = uabs32(cf[rc]);
-            const float q2 = rc == 0 ? q2dc : q2ac;
-            const float er = __fsub_rn(fabsf(tf[rc]), (float)l);
-            const float dcur = __fmul_rn(q2, __fmul_rn(er, er));
-            if (l != 0) {
-                const int bin = e < 2 ? e : bitlen32((uint32_t)e);
-                const int extra = bin > 1 ? bin - 2 : 0;
-                const float sig = __fadd_rn(__fmul_rn((float)bin, 0.9f), __fadd_rn((float)extra, 2.0f));
-                const float c = __fadd_rn(__fadd_rn(__fadd_rn(pre, dcur), __fmul_rn(lam, sig)),
-                                          e + 1 < m ? suf[e + 1] : 0.0f);
-                if (c < best) {
-                    best = c;
-                    best_e = e;
-                }
-            }
-            pre = __fadd_rn(__fadd_rn(pre, dcur), __fmul_rn(lam, coef_rate_bits(l)));
-        }
-        const float skip = __fadd_rn(suf[0], __fmul_rn(lam, 1.0f));
-        const int keep = (best_e < 0 || skip < best) ? 0 : best_e + 1;
-        for (int k = keep; k < m; k++) cf[scan[k]] = 0;
-        return MT_OK;
     }
+    const int eob = group_max<G>(e_loc);
+    act = act && eob >= 0;
+    const int m = eob + 1;
+    dm = group_or<G>(dm);
+    if (!(act && ctxv && eob >= 2)) dm = 0ull;
 
-    // trellis_optimize_ctx (src/trellis.rs:55), shipped switches
-    const mtlib::RateTables& RT = T.rate[d.qctx];
-    const int cls = G.cls, cb = G.cb, pl = d.plane;
+    // context-variant tables and weights
+    const int qm_level = d.bd_qm & 15, pl = d.plane ? 1 : 0;
+    const bool qm_on = qm_level < 15, qm_full = qm_level >= 5;
+    const mtlib::RateTables& RT = T.rate[act && ctxv ? d.qctx : 0];
+    const int cls = act ? g.cls : 0, cb = act ? g.cb : 0;
     const float(*bt)[4] = RT.bt[cls][pl];
     const float(*br)[13] = RT.br[cb][pl];
     const float(*eob_base)[3] = RT.eob_base[cls][pl];
     const float(*eob_hi)[2] = RT.eob_hi[cls][pl];
-    const float* dc_sign = RT.dc_sign[pl][d.dcs_ctx];
-    const float* eob_bin = RT.eob_bin[d.eob_bin];
-    const bool qm_on = qm_level < 15, qm_full = qm_level >= 5;
-    const uint8_t* qmt = T.iqm + ((qm_on ? qm_level : 0) * 2 + (pl ? 1 : 0)) * MT_QM_USED_SIZE + trellis_qm_offset(G.w, h);
+    const float* dc_sign = RT.dc_sign[pl][act && ctxv ? d.dcs_ctx : 0];
+    const float* eob_bin = RT.eob_bin[act && ctxv ? d.eob_bin : 0];
+    const uint8_t* qmt = T.iqm + ((qm_on ? qm_level : 0) * 2 + pl) * MT_QM_USED_SIZE +
+                         (act ? trellis_qm_offset(g.w, h) : 0);
+    const uint8_t* lo_off = T.lo_off + g.oi * 25;
     const float tilt = d.band_tilt;
-    const float n_inv = __fdiv_rn(1.0f, (float)N);
+    const float n_inv = __fdiv_rn(1.0f, (float)(N > 0 ? N : 1));
     auto qm_at = [&](uint32_t rc) {
         const float wq = __fdiv_rn((float)qmt[rc], 32.0f);
         return qm_full ? __fmul_rn(wq, wq) : wq;
@@ -293,18 +324,36 @@ __device__ int trellis_block(const mt_trellisblk& d, int32_t* cf, const float* t
             dd = __fdiv_rn(dd, __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)idx, n_inv))));
         return dd;
     };
+    // the distortions (current level, zero) with QM and tilt weights
+    auto dists = [&](int i, uint32_t rc, float& dcur, float& dzero) {
+        const float q2 = rc == 0 ? q2dc : q2ac;
+        const float at = fabsf(tf[rc]);
+        const float e = __fsub_rn(at, (float)uabs32(cf[rc]));
+        dcur = __fmul_rn(q2, __fmul_rn(e, e));
+        dzero = __fmul_rn(q2, __fmul_rn(at, at));
+        if (qm_on) {
+            const float w2 = qm_at(rc);
+            dcur = __fmul_rn(dcur, w2);
+            dzero = __fmul_rn(dzero, w2);
+        }
+        if (tilt != 0.0f && at < 2.0f) {
+            const float wgt = __fdiv_rn(1.0f, __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)i, n_inv))));
+            dzero = __fmul_rn(dzero, wgt);
+            dcur = __fmul_rn(dcur, wgt);
+        }
+    };
     auto hi_cost = [&](uint32_t mm, int bc) {
         float v = br[bc][(mm - 3) < 12 ? (mm - 3) : 12];
         if (mm >= 15) v = __fadd_rn(v, golomb_bits(mm - 15));
         return v;
     };
-    auto lvl = [&](int k) -> uint32_t { return k < N ? level_byte(uabs32(cf[k])) : 0u; };
+    auto lvl = [&](int k) -> uint32_t { return k < N ? (uint32_t)lb[k] : 0u; };
     auto interior_ctx = [&](uint32_t rc, int& ctx, int& bc) {
         const int x = (int)(rc >> log2h), y = (int)(rc & (h - 1));
         const int base = x * h + y;
         const uint32_t hm = lvl(base + 1) + lvl(base + h) + lvl(base + h + 1);
         const uint32_t mag = hm + lvl(base + 2) + lvl(base + 2 * h);
-        ctx = T.lo_off[(G.oi * 5 + (y < 4 ? y : 4)) * 5 + (x < 4 ? x : 4)] + (mag > 512 ? 4 : (int)((mag + 64) >> 7));
+        ctx = lo_off[(y < 4 ? y : 4) * 5 + (x < 4 ? x : 4)] + (mag > 512 ? 4 : (int)((mag + 64) >> 7));
         const uint32_t mg = hm & 63;
         bc = ((y | x) > 1 ? 14 : 7) + (mg > 12 ? 6 : (int)((mg + 1) >> 1));
     };
@@ -319,32 +368,59 @@ __device__ int trellis_block(const mt_trellisblk& d, int32_t* cf, const float* t
         if (tok == 3) r = __fadd_rn(r, hi_cost(k, bc));
         return __fadd_rn(r, 1.0f);
     };
-    auto set_mag = [&](uint32_t rc, uint32_t k) { cf[rc] = cf[rc] < 0 ? -(int32_t)k : (int32_t)k; };
+    auto set_mag = [&](uint32_t rc, uint32_t k) {
+        cf[rc] = cf[rc] < 0 ? -(int32_t)k : (int32_t)k;
+        lb[rc] = (uint8_t)level_byte(k);
+    };
 
-    // Step A: interior positions eob-1 .. 1, then DC
-    for (int i = eob - 1; i >= 1; i--) {
-        const uint32_t rc = scan[i], l = uabs32(cf[rc]);
-        if (l == 0) continue;
-        int ctx, bc;
-        interior_ctx(rc, ctx, bc);
-        auto rate_k = [&](uint32_t k) {
-            return k == 0 ? bt[ctx][0] : k == 1 ? __fadd_rn(bt[ctx][1], 1.0f) : k == 2 ? __fadd_rn(bt[ctx][2], 1.0f)
-                 : __fadd_rn(__fadd_rn(bt[ctx][3], hi_cost(k, bc)), 1.0f);
-        };
-        uint32_t best_k = l;
-        float best_c = __fadd_rn(distw(i, rc, l), __fmul_rn(lam, rate_k(l)));
-        for (int k = (int)l - 1; k >= 0; k--) {
-            const float dk = distw(i, rc, (uint32_t)k);
-            if (dk >= best_c) break;
-            const float c = __fadd_rn(dk, __fmul_rn(lam, rate_k((uint32_t)k)));
-            if (c < best_c) {
-                best_c = c;
-                best_k = (uint32_t)k;
+    // trellis_optimize's round-down (src/trellis.rs:944): chunks of 8 scan positions, then one
+    // of 4, then the scalar tail; each chunk reads and writes only its own levels
+    if (act && !ctxv) {
+        const int m8 = m & ~7;
+        for (int c = gl; c * 8 < m8; c += G) round_down_chunk(cf, tf, scan, c * 8, 8, q2dc, q2ac, lam);
+        if (gl == 0) {
+            int i = m8;
+            if (i + 4 <= m) {
+                round_down_chunk(cf, tf, scan, i, 4, q2dc, q2ac, lam);
+                i += 4;
             }
+            for (; i < m; i++) round_down_one(cf, tf, scan[i], scan[i] == 0 ? q2dc : q2ac, lam);
         }
-        if (best_k != l) set_mag(rc, best_k);
     }
-    {
+    // trellis_optimize_ctx (src/trellis.rs:55) Step A: interior positions eob-1 .. 1 by
+    // anti-diagonal, then DC
+    for (unsigned long long wm = group_or<64>(dm) & ~1ull; wm != 0ull;) {
+        const int dd = 63 - __clzll((long long)wm);
+        wm &= ~(1ull << dd);
+        __syncthreads();
+        if (!((dm >> dd) & 1ull)) continue;
+        const int lo = max(1, diag_start(dd, g.w, h)), hi = min(eob - 1, diag_start(dd + 1, g.w, h) - 1);
+        for (int i = lo + gl; i <= hi; i += G) {
+            const uint32_t rc = scan[i], l = uabs32(cf[rc]);
+            if (l == 0) continue;
+            int ctx, bc;
+            interior_ctx(rc, ctx, bc);
+            const float b0 = bt[ctx][0], b1 = bt[ctx][1], b2 = bt[ctx][2], b3 = bt[ctx][3];
+            auto rate_k = [&](uint32_t k) {
+                return k == 0 ? b0 : k == 1 ? __fadd_rn(b1, 1.0f) : k == 2 ? __fadd_rn(b2, 1.0f)
+                     : __fadd_rn(__fadd_rn(b3, hi_cost(k, bc)), 1.0f);
+            };
+            uint32_t best_k = l;
+            float best_c = __fadd_rn(distw(i, rc, l), __fmul_rn(lam, rate_k(l)));
+            for (int k = (int)l - 1; k >= 0; k--) {
+                const float dk = distw(i, rc, (uint32_t)k);
+                if (dk >= best_c) break;
+                const float c = __fadd_rn(dk, __fmul_rn(lam, rate_k((uint32_t)k)));
+                if (c < best_c) {
+                    best_c = c;
+                    best_k = (uint32_t)k;
+                }
+            }
+            if (best_k != l) set_mag(rc, best_k);
+        }
+    }
+    __syncthreads();
+    if (act && ctxv && gl == 0) {
         const uint32_t rc = scan[0], l = uabs32(cf[rc]);
         if (l != 0) {
             const int bc = dc_brc();
@@ -370,84 +446,128 @@ __device__ int trellis_block(const mt_trellisblk& d, int32_t* cf, const float* t
             if (best_k != l) set_mag(rc, best_k);
         }
     }
+    __syncthreads();
 
-    // Step B: the distortions (current level, zero) with QM and tilt weights, their suffix
-    auto dists = [&](int i, float& dcur, float& dzero) {
-        const uint32_t rc = scan[i];
-        const float q2 = rc == 0 ? q2dc : q2ac;
-        const float at = fabsf(tf[rc]);
-        const float e = __fsub_rn(at, (float)uabs32(cf[rc]));
-        dcur = __fmul_rn(q2, __fmul_rn(e, e));
-        dzero = __fmul_rn(q2, __fmul_rn(at, at));
-        if (qm_on) {
-            const float w2 = qm_at(rc);
-            dcur = __fmul_rn(dcur, w2);
-            dzero = __fmul_rn(dzero, w2);
+    // the terms of the two running sums, in the order of the sums.  Context variant (i in
+    // 1..m-1): sa[m-1-i] the zero-level distortion of i, sb[i-1] lambda * rate of its kept
+    // level plus its distortion.  Plain variant (i in 0..m-1): sa[m-1-i] q2 * |t|^2,
+    // sb[i] the kept level's distortion.
+    if (act) {
+        for (int i = gl; i < m; i += G) {
+            const uint32_t rc = scan[i];
+            if (ctxv) {
+                if (i == 0) continue;
+                float dcur, dz;
+                dists(i, rc, dcur, dz);
+                int ctx, bc;
+                interior_ctx(rc, ctx, bc);
+                sa[m - 1 - i] = dz;
+                sb[i - 1] = __fadd_rn(__fmul_rn(lam, interior_rate(ctx, bc, uabs32(cf[rc]))), dcur);
+            } else {
+                const float q2 = rc == 0 ? q2dc : q2ac;
+                const float at = fabsf(tf[rc]);
+                const float er = __fsub_rn(at, (float)uabs32(cf[rc]));
+                sa[m - 1 - i] = __fmul_rn(q2, __fmul_rn(at, at));
+                sb[i] = __fmul_rn(q2, __fmul_rn(er, er));
+            }
         }
-        if (tilt != 0.0f && at < 2.0f) {
-            const float wgt = __fdiv_rn(1.0f, __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)i, n_inv))));
-            dzero = __fmul_rn(dzero, wgt);
-            dcur = __fmul_rn(dcur, wgt);
-        }
-    };
+    }
+    __syncthreads();
+    // the running sums, inclusive, in place: lane 0 of the group over sa (the suffix sums of
+    // the walk, sa[m-1-i] = the sum over i..m-1), lane 1 over sb (the prefix sums: the plain
+    // variant adds lambda * rate of each kept level after its distortion)
     {
-        float sacc = 0.0f;
-        for (int i = m - 1; i >= 1; i--) {
-            float dc_, dz;
-            dists(i, dc_, dz);
-            sacc = __fadd_rn(sacc, dz);
-            suf[i] = sacc;
+        const int cnt = act ? (ctxv ? m - 1 : m) : 0;
+        const bool run = act && gl < 2;
+        float* arr = gl == 0 ? sa : sb;
+        if (__ballot(act && !ctxv) == 0ull) {
+            // four terms per 16-byte LDS access, the next quad loaded ahead of the sums
+            const int nq = run ? (cnt + 3) >> 2 : 0;
+            const int kq = group_max<64>(nq);
+            float4* a4 = reinterpret_cast<float4*>(arr);
+            float4 nxt = nq > 0 ? a4[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float acc = 0.0f;
+            for (int q = 0; q < kq; q++) {
+                float4 v = nxt;
+                if (q + 1 < nq) nxt = a4[q + 1];
+                acc = __fadd_rn(acc, v.x);
+                v.x = acc;
+                acc = __fadd_rn(acc, v.y);
+                v.y = acc;
+                acc = __fadd_rn(acc, v.z);
+                v.z = acc;
+                acc = __fadd_rn(acc, v.w);
+                v.w = acc;
+                if (q < nq) a4[q] = v;
+            }
+        } else {
+            const int kmax = group_max<64>(run ? cnt : 0);
+            const bool lr = gl == 1 && !ctxv;
+            float acc = 0.0f;
+            for (int k = 0; k < kmax; k++) {
+                if (run && k < cnt) {
+                    const float v = arr[k];
+                    const float x = lr ? __fmul_rn(lam, coef_rate_bits(uabs32(cf[scan[k]]))) : 0.0f;
+                    acc = lr ? __fadd_rn(__fadd_rn(acc, v), x) : __fadd_rn(acc, v);
+                    arr[k] = acc;
+                }
+            }
         }
     }
-    auto suf_at = [&](int i) { return i < m ? suf[i] : 0.0f; };
-    const uint32_t dc_rc = scan[0], dc_m = uabs32(cf[dc_rc]);
-    float dcur0, dzero0;
-    dists(0, dcur0, dzero0);
-    auto eob_pt = [&](int e) {
-        const int bin = e < 2 ? e : bitlen32((uint32_t)e);
-        float c = eob_bin[bin];
-        if (bin > 1) {
-            const int nbits = bin - 2;
-            c = __fadd_rn(c, eob_hi[bin][(e >> nbits) & 1]);
-            c = __fadd_rn(c, (float)nbits);
-        }
-        return c;
-    };
-    auto eob_cand = [&](int e, uint32_t mm) {
-        const int ctx_e = 1 + (e > N / 8 ? 1 : 0) + (e > N / 4 ? 1 : 0);
-        const uint32_t tok = mm < 3 ? mm : 3;
-        float c = eob_base[ctx_e][tok - 1];
-        if (tok == 3) {
-            const uint32_t rc = scan[e];
-            const int ex = (int)(rc >> log2h), ey = (int)(rc & (h - 1));
-            c = __fadd_rn(c, hi_cost(mm, (ex | ey) > 1 ? 14 : 7));
-        }
-        return __fadd_rn(__fmul_rn(lam, __fadd_rn(c, 1.0f)), distw(e, scan[e], mm));
-    };
-    float dc_cost;
-    if (dc_m == 0) {
-        dc_cost = __fmul_rn(lam, bt[0][0]);
-    } else {
-        const uint32_t tok = dc_m < 3 ? dc_m : 3;
-        float r = bt[0][tok];
-        if (tok == 3) r = __fadd_rn(r, hi_cost(dc_m, dc_brc()));
-        r = __fadd_rn(r, dc_sign[cf[dc_rc] < 0 ? 1 : 0]);
-        dc_cost = __fmul_rn(lam, r);
-    }
-    dc_cost = __fadd_rn(dc_cost, dcur0);
-    int best_e = -1;
+    __syncthreads();
+
+    // the candidate ends of block: each lane keeps its earliest strict minimum, the group
+    // reduces them to the earliest strict minimum of the walk
+    float best = INFINITY;
+    int best_e = 0x7fffffff;
     uint32_t best_m = 0;
-    float best = INFINITY, pre = 0.0f;
-    for (int e = 1; e < m; e++) {
-        const uint32_t rc = scan[e], mm = uabs32(cf[rc]);
-        float dcur, dz;
-        dists(e, dcur, dz);
-        if (mm != 0) {
+    float dcur0 = 0.0f, dzero0 = 0.0f, dc_cost = 0.0f;
+    uint32_t dc_m = 0;
+    auto suf_at = [&](int i) { return i < m ? sa[m - 1 - i] : 0.0f; };  // sum of i..m-1
+    if (act && ctxv) {
+        const uint32_t dc_rc = scan[0];
+        dc_m = uabs32(cf[dc_rc]);
+        dists(0, dc_rc, dcur0, dzero0);
+        if (dc_m == 0) {
+            dc_cost = __fmul_rn(lam, bt[0][0]);
+        } else {
+            const uint32_t tok = dc_m < 3 ? dc_m : 3;
+            float r = bt[0][tok];
+            if (tok == 3) r = __fadd_rn(r, hi_cost(dc_m, dc_brc()));
+            r = __fadd_rn(r, dc_sign[cf[dc_rc] < 0 ? 1 : 0]);
+            dc_cost = __fmul_rn(lam, r);
+        }
+        dc_cost = __fadd_rn(dc_cost, dcur0);
+        auto eob_pt = [&](int e) {
+            const int bin = e < 2 ? e : bitlen32((uint32_t)e);
+            float c = eob_bin[bin];
+            if (bin > 1) {
+                const int nbits = bin - 2;
+                c = __fadd_rn(c, eob_hi[bin][(e >> nbits) & 1]);
+                c = __fadd_rn(c, (float)nbits);
+            }
+            return c;
+        };
+        auto eob_cand = [&](int e, uint32_t rc, uint32_t mm) {
+            const int ctx_e = 1 + (e > N / 8 ? 1 : 0) + (e > N / 4 ? 1 : 0);
+            const uint32_t tok = mm < 3 ? mm : 3;
+            float c = eob_base[ctx_e][tok - 1];
+            if (tok == 3) {
+                const int ex = (int)(rc >> log2h), ey = (int)(rc & (h - 1));
+                c = __fadd_rn(c, hi_cost(mm, (ex | ey) > 1 ? 14 : 7));
+            }
+            return __fadd_rn(__fmul_rn(lam, __fadd_rn(c, 1.0f)), distw(e, rc, mm));
+        };
+        for (int e = 1 + gl; e < m; e += G) {
+            const uint32_t rc = scan[e];
+            const uint32_t mm = uabs32(cf[rc]);
+            if (mm == 0) continue;
+            const float pre = e >= 2 ? sb[e - 2] : 0.0f;  // the kept-levels cost of 1..e-1
             const float base = __fadd_rn(__fadd_rn(__fadd_rn(dc_cost, pre), __fmul_rn(lam, eob_pt(e))), suf_at(e + 1));
             uint32_t mb = mm;
-            float c = __fadd_rn(base, eob_cand(e, mm));
+            float c = __fadd_rn(base, eob_cand(e, rc, mm));
             if (mm >= 2) {
-                const float lo = __fadd_rn(base, eob_cand(e, mm - 1));
+                const float lo = __fadd_rn(base, eob_cand(e, rc, mm - 1));
                 if (lo < c) {
                     c = lo;
                     mb = mm - 1;
@@ -459,34 +579,76 @@ __device__ int trellis_block(const mt_trellisblk& d, int32_t* cf, const float* t
                 best_m = mb;
             }
         }
-        int ctx, bc;
-        interior_ctx(rc, ctx, bc);
-        pre = __fadd_rn(pre, __fadd_rn(__fmul_rn(lam, interior_rate(ctx, bc, mm)), dcur));
-    }
-    if (dc_m != 0) {
-        const uint32_t tok = dc_m < 3 ? dc_m : 3;
-        float c0 = __fadd_rn(eob_bin[0], eob_base[0][tok - 1]);
-        if (tok == 3) c0 = __fadd_rn(c0, hi_cost(dc_m, 0));
-        c0 = __fadd_rn(c0, dc_sign[cf[dc_rc] < 0 ? 1 : 0]);
-        const float total0 = __fadd_rn(__fadd_rn(__fmul_rn(lam, c0), dcur0), suf_at(1));
-        if (total0 < best) {
-            best = total0;
-            best_e = 0;
-            best_m = dc_m;
+    } else if (act) {
+        for (int e = gl; e < m; e += G) {
+            const uint32_t rc = scan[e];
+            const uint32_t l = uabs32(cf[rc]);
+            if (l == 0) continue;
+            const float q2 = rc == 0 ? q2dc : q2ac;
+            const float er = __fsub_rn(fabsf(tf[rc]), (float)l);
+            const float dcur = __fmul_rn(q2, __fmul_rn(er, er));
+            const int bin = e < 2 ? e : bitlen32((uint32_t)e);
+            const int extra = bin > 1 ? bin - 2 : 0;
+            const float sig = __fadd_rn(__fmul_rn((float)bin, 0.9f), __fadd_rn((float)extra, 2.0f));
+            const float pre = e >= 1 ? sb[e - 1] : 0.0f;
+            const float c = __fadd_rn(__fadd_rn(__fadd_rn(pre, dcur), __fmul_rn(lam, sig)), suf_at(e + 1));
+            if (c < best) {
+                best = c;
+                best_e = e;
+            }
         }
     }
-    const float skip = __fadd_rn(__fadd_rn(suf_at(1), dzero0), __fmul_rn(lam, 1.0f));
-    if (best_e < 0 || skip < best) {
-        for (int k = 0; k < m; k++) cf[scan[k]] = 0;
-        return MT_OK;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o);
+        const int oe = __shfl_xor(best_e, o);
+        const uint32_t om = (uint32_t)__shfl_xor((int)best_m, o);
+        if (ob < best || (ob == best && oe < best_e)) {
+            best = ob;
+            best_e = oe;
+            best_m = om;
+        }
     }
-    const uint32_t rc = scan[best_e];
-    if (best_m != uabs32(cf[rc])) {
-        const bool neg = cf[rc] != 0 ? cf[rc] < 0 : tf[rc] < 0.0f;
-        cf[rc] = neg ? -(int32_t)best_m : (int32_t)best_m;
+    // the decision, the same on every lane of the group, then its writes
+    int keep = m;  // positions [keep, m) of the scan become zero
+    int set_e = -1;
+    if (act && ctxv) {
+        if (dc_m != 0) {
+            const uint32_t tok = dc_m < 3 ? dc_m : 3;
+            float c0 = __fadd_rn(eob_bin[0], eob_base[0][tok - 1]);
+            if (tok == 3) c0 = __fadd_rn(c0, hi_cost(dc_m, 0));
+            c0 = __fadd_rn(c0, dc_sign[cf[scan[0]] < 0 ? 1 : 0]);
+            const float total0 = __fadd_rn(__fadd_rn(__fmul_rn(lam, c0), dcur0), suf_at(1));
+            if (total0 < best) {
+                best = total0;
+                best_e = 0;
+                best_m = dc_m;
+            }
+        }
+        const float skip = __fadd_rn(__fadd_rn(suf_at(1), dzero0), __fmul_rn(lam, 1.0f));
+        if (best_e == 0x7fffffff || skip < best) {
+            keep = 0;
+        } else {
+            keep = best_e + 1;
+            set_e = best_e;
+        }
+    } else if (act) {
+        const float skip = __fadd_rn(suf_at(0), __fmul_rn(lam, 1.0f));
+        keep = (best_e == 0x7fffffff || skip < best) ? 0 : best_e + 1;
     }
-    for (int k = best_e + 1; k < m; k++) cf[scan[k]] = 0;
-    return MT_OK;
+    __syncthreads();  // every lane has read the levels the decision depends on
+    if (act) {
+        if (set_e >= 0 && gl == 0) {
+            const uint32_t rc = scan[set_e];
+            if (best_m != uabs32(cf[rc])) {
+                const bool neg = cf[rc] != 0 ? cf[rc] < 0 : tf[rc] < 0.0f;
+                cf[rc] = neg ? -(int32_t)best_m : (int32_t)best_m;
+            }
+        }
+        for (int k = keep + gl; k < m; k += G) cf[scan[k]] = 0;
+    }
+    __syncthreads();
+    return status;
 }
 
 }  // namespace

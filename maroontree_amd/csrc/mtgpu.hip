@@ -1441,7 +1441,7 @@ int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t
         if (rc) return rc;
     }
     if (!trellis_d) return MT_OK;
-    if ((rc = mtlib::trellis_launch(trellis_d, n, levels_d, targets_d, reinterpret_cast<float*>(recon_d), nullptr, s)))
+    if ((rc = mtlib::trellis_launch(trellis_d, n, levels_d, targets_d, nullptr, s)))
         return rc;
     for (int slot = 0; slot < 13; slot++) {
         const int sz = kSlotSize[slot];

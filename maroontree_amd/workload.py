@@ -192,3 +192,35 @@ def sb_candidates(width, height, bd=10, qidx=91, qm_level=None, lam=64.0, wsse_s
     c["lambda"] = lam
     assert off < (1 << 32)
     return c, off
+
+
+# rate-table slot of the end-of-block class of a coded coefficient count, (luma, chroma)
+EOB_BIN = {16: (0, 1), 32: (3, 4), 64: (5, 6), 128: (8, 9), 256: (11, 12), 512: (13, 14), 1024: (15, 16)}
+
+
+def chain_trellis_descriptors(blocks, sub=2):
+    """Slow-mode trellis descriptors (TRELLISBLK_DTYPE) for a chain batch of TXBLK_DTYPE
+    records: trellis_optimize_ctx on every block (src/trellis.rs:606), the frame qindex's
+    coefficient context (get_q_ctx), the luma/chroma end-of-block class of the coded size,
+    the DC-sign context of a block without coded neighbours, and the lambda the reference's
+    encoder derives per frame (trellis_lambda x hbd_trellis_scale at chroma subsampling
+    `sub`: 0 4:4:4, 1 4:2:2, 2 4:2:0)."""
+    n = len(blocks)
+    tr = np.zeros(n, _lib.TRELLISBLK_DTYPE)
+    for k in ("offset", "tx_size", "qidx", "bd_qm"):
+        tr[k] = blocks[k]
+    tr["flags"] = _lib.TRELLIS_CTX
+    q = blocks["qidx"].astype(np.int64)
+    tr["qctx"] = np.where(q <= 20, 0, np.where(q <= 60, 1, np.where(q <= 120, 2, 3)))
+    chroma = (blocks["bd_qm"] & _lib.TXBLK_CHROMA) != 0
+    tr["plane"] = chroma
+    w = np.array([_lib.tx_width(s) for s in range(19)])
+    h = np.array([_lib.tx_height(s) for s in range(19)])
+    nc = np.minimum(w, 32) * np.minimum(h, 32)
+    slot = np.array([EOB_BIN[int(c)] for c in nc], np.uint8)
+    tr["eob_bin"] = slot[blocks["tx_size"], chroma.astype(np.int64)]
+    bdi = (blocks["bd_qm"] >> 4) & 3
+    lam = np.array([[_lib.trellis_lambda0(qi, (8, 10, 12)[b], sub) for qi in range(256)] for b in range(3)],
+                   np.float32)
+    tr["lambda0"] = lam[np.minimum(bdi, 2), q]
+    return tr

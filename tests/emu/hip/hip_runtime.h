@@ -137,10 +137,12 @@ inline uint64_t __ballot(int pred) {
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
 struct int4 { int x, y, z, w; };
 struct float2 { float x, y; };
 inline float2 make_float2(float x, float y) { return float2{x, y}; }
 struct float4 { float x, y, z, w; };
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fmul_rn(float a, float b) { return a * b; }
 inline double __dadd_rn(double a, double b) { return a + b; }

@@ -25,10 +25,12 @@ def run(dev, desc, levels, targets):
     return lv.cpu().numpy(), st.cpu().numpy()
 
 
-@pytest.mark.parametrize("seed,n", [(1, 4000), (2, 4000), (3, 20000)])
-def test_trellis_matches_oracle(dev, seed, n):
-    desc, levels, targets, want = TG.make(seed, n)
-    print(f"trellis seed {seed}: {n} blocks", flush=True)
+@pytest.mark.parametrize("seed,n,ctx_frac", [(1, 4000, 0.6), (2, 4000, 0.6), (3, 20000, 0.6), (4, 8000, 1.0),
+                                              (5, 4000, 0.0)])
+def test_trellis_matches_oracle(dev, seed, n, ctx_frac):
+    """Variants mixed in each wave, the context variant alone, the plain variant alone."""
+    desc, levels, targets, want = TG.make(seed, n, ctx_frac)
+    print(f"trellis seed {seed}: {n} blocks, context variant {ctx_frac}", flush=True)
     got, st = run(dev, desc, levels, targets)
     assert not st.any()
     diff = np.nonzero(got != want)[0]
@@ -81,5 +83,29 @@ def test_chain_with_trellis_matches_oracle(dev, seed, n, all_sizes):
     torch.cuda.synchronize()
     got = res.cpu().numpy().view(mt.RES_DTYPE)
     sse, h = TG.oracle_chain(blocks, pool, tr)
+    bad = np.nonzero((got["sse"] != sse) | (got["hash"] != h))[0]
+    assert len(bad) == 0, (len(bad), bad[:5])
+
+
+def test_slow_chain_descriptors_match_oracle(dev):
+    """The Slow-mode descriptors of the chain benchmark (workload.chain_trellis_descriptors:
+    the context trellis on every block with its qindex context and frame lambda) at a slice
+    of the benchmark's seeded batch: block for block against the oracle."""
+    from maroontree_amd import workload as W
+    blocks, pool = mt.gen_tx_blocks(0x6D61726F6F6E, 1 << 16)
+    tr = W.chain_trellis_descriptors(blocks, sub=2)
+    print(f"slow chain descriptors: {len(blocks)} blocks", flush=True)
+    t = lambda a: torch.from_numpy(a.view(np.uint8).copy()).to(dev)  # noqa: E731
+    pt = torch.from_numpy(pool).to(dev)
+    n = len(blocks)
+    res = torch.zeros(n * 16, dtype=torch.uint8, device=dev)
+    lv = torch.zeros(len(pool), dtype=torch.int32, device=dev)
+    tg = torch.zeros(len(pool), dtype=torch.float32, device=dev)
+    rec = torch.zeros(len(pool), dtype=torch.int32, device=dev)
+    mt.tx_chain_trellis(t(blocks), t(tr), pt, res, lv, tg, rec)
+    torch.cuda.synchronize()
+    got = res.cpu().numpy().view(mt.RES_DTYPE)
+    sse, h = TG.oracle_chain(blocks, pool, tr)
+    assert (sse >= 0).all()
     bad = np.nonzero((got["sse"] != sse) | (got["hash"] != h))[0]
     assert len(bad) == 0, (len(bad), bad[:5])

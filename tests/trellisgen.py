@@ -16,8 +16,9 @@ def cls_of(s):
     return ((w.bit_length() - 3) + (h.bit_length() - 3) + 1) >> 1
 
 
-def make(seed, n):
-    """n random blocks: (descriptors, levels, targets, want) with want the oracle's levels."""
+def make(seed, n, ctx_frac=0.6):
+    """n random blocks: (descriptors, levels, targets, want) with want the oracle's levels;
+    a fraction ctx_frac of them with the context variant."""
     rng = np.random.default_rng(seed)
     desc = np.zeros(n, mt.TRELLISBLK_DTYPE)
     lv_parts, tg_parts = [], []
@@ -32,7 +33,7 @@ def make(seed, n):
         qidx = int(rng.integers(0, 256))
         qm = int(rng.choice([15, 15, 0, 4, 5, 9]))
         chroma = s != 4 and rng.random() < 0.3
-        ctx = rng.random() < 0.6
+        ctx = rng.random() < ctx_frac
         dcd, acd = (int(rng.integers(-12, 13)), int(rng.integers(-12, 13))) if chroma else (0, 0)
         q = O.quant_chroma(qidx, dcd, acd, bd, qm) if chroma else O.quant(qidx, bd, qm)
         scale = int(rng.choice([1, 2, 8, 32]))
