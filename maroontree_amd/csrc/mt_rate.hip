@@ -572,19 +572,21 @@ __global__ void __launch_bounds__(256) trellis_sort_kernel(const mt_trellisblk* 
 // The blocks of one class (NE coefficients, G lanes each), 64 / G per one-wave workgroup,
 // persistent over the class's list: each group stages its block's levels and targets in LDS
 // (16-byte loads where aligned), runs trellis_group, and writes the levels back.
-template <int G, int NE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) trellis_group_kernel(const mt_trellisblk* __restrict__ blocks,
-                                                          const uint32_t* __restrict__ list,
-                                                          const uint32_t* __restrict__ count,
-                                                          int32_t* __restrict__ levels,
-                                                          const float* __restrict__ targets,
-                                                          int32_t* __restrict__ status, mtlib::TrellisTabs tabs) {
+template <int G, int NE, int NS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+trellis_group_kernel(const mt_trellisblk* __restrict__ blocks, const uint32_t* __restrict__ list,
+                     const uint32_t* __restrict__ count, int32_t* __restrict__ levels,
+                     const float* __restrict__ targets, int32_t* __restrict__ status, mtlib::TrellisTabs tabs,
+                     int so0) {
     constexpr int NG = 64 / G;
+    __shared__ uint16_t s_scan[NS * NE];  // the class's scans (NS sizes, one after another)
+    for (int k = threadIdx.x; k < NS * NE; k += 64) s_scan[k] = tabs.scan[so0 + k];
     __shared__ __align__(16) int32_t s_cf[NG * NE];
     __shared__ __align__(16) float s_tf[NG * NE];
     __shared__ __align__(16) float s_a[NG * NE];
     __shared__ __align__(16) float s_b[NG * NE];
-    __shared__ uint8_t s_lb[NG * NE];
+    constexpr int LBN = NE + 2 * (NE / G) + 4;  // the level bytes and their zero pad
+    __shared__ uint8_t s_lb[NG * LBN];
     const int g = threadIdx.x / G, gl = threadIdx.x & (G - 1);
     int32_t* cf = s_cf + g * NE;
     float* tf = s_tf + g * NE;
@@ -611,7 +613,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) tr
                 tf[k] = tg[k];
             }
         }
-        const int rc = trellis_group<G>(d, ne > 0, cf, tf, s_a + g * NE, s_b + g * NE, s_lb + g * NE, tabs);
+        const int rc = trellis_group<G>(d, ne > 0, cf, tf, s_a + g * NE, s_b + g * NE, s_lb + g * LBN,
+                                        s_scan + (ne > 0 ? scan_offset(d.tx_size) - so0 : 0), tabs);
         if (rc == MT_OK) {
             if (vec) {
                 for (int k = gl; k < ne / 4; k += G) reinterpret_cast<int4*>(lv)[k] = reinterpret_cast<const int4*>(cf)[k];
@@ -794,13 +797,14 @@ struct TrellisWork {
 std::mutex g_trellis_mu;
 TrellisWork g_trellis_work[64];
 
-template <int G, int NE>
-void launch_trellis_class(const mt_trellisblk* blocks_d, const TrellisWork& w, int c, int n, int cus,
+// so0: the scan offset of the class's first size (its sizes' scans lie one after another)
+template <int G, int NE, int NS>
+void launch_trellis_class(const mt_trellisblk* blocks_d, const TrellisWork& w, int c, int so0, int n, int cus,
                           int32_t* levels_d, const float* targets_d, int32_t* status_d, hipStream_t s, int dev) {
     constexpr int NG = 64 / G;
     const int grid = std::max(1, std::min((n + NG - 1) / NG, cus * 8));
-    hipLaunchKernelGGL((trellis_group_kernel<G, NE>), dim3(grid), dim3(64), 0, s, blocks_d, w.order + (size_t)c * n,
-                       w.counts + c, levels_d, targets_d, status_d, g_tabs[dev & 63]);
+    hipLaunchKernelGGL((trellis_group_kernel<G, NE, NS>), dim3(grid), dim3(64), 0, s, blocks_d,
+                       w.order + (size_t)c * n, w.counts + c, levels_d, targets_d, status_d, g_tabs[dev & 63], so0);
 }
 }  // namespace
 
@@ -830,14 +834,19 @@ int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, cons
     hipLaunchKernelGGL(trellis_sort_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n, w.counts,
                        w.order, status_d);
     // the lists are n apart whatever the capacity, so their offsets match the sort's
-    launch_trellis_class<4, 16>(blocks_d, w, 0, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<4, 32>(blocks_d, w, 1, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<4, 64>(blocks_d, w, 2, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<8, 64>(blocks_d, w, 3, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<8, 128>(blocks_d, w, 4, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<16, 256>(blocks_d, w, 5, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<16, 512>(blocks_d, w, 6, n, cus, levels_d, targets_d, status_d, s, dev);
-    launch_trellis_class<32, 1024>(blocks_d, w, 7, n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<4, 16, 1>(blocks_d, w, 0, scan_offset(MT_TX_4X4), n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<4, 32, 2>(blocks_d, w, 1, scan_offset(MT_TX_4X8), n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<4, 64, 2>(blocks_d, w, 2, scan_offset(MT_TX_4X16), n, cus, levels_d, targets_d, status_d, s,
+                                   dev);
+    launch_trellis_class<8, 64, 1>(blocks_d, w, 3, scan_offset(MT_TX_8X8), n, cus, levels_d, targets_d, status_d, s, dev);
+    launch_trellis_class<8, 128, 2>(blocks_d, w, 4, scan_offset(MT_TX_8X16), n, cus, levels_d, targets_d, status_d, s,
+                                    dev);
+    launch_trellis_class<16, 256, 1>(blocks_d, w, 5, scan_offset(MT_TX_16X16), n, cus, levels_d, targets_d, status_d,
+                                     s, dev);
+    launch_trellis_class<16, 512, 2>(blocks_d, w, 6, scan_offset(MT_TX_16X32), n, cus, levels_d, targets_d, status_d,
+                                     s, dev);
+    launch_trellis_class<32, 1024, 1>(blocks_d, w, 7, scan_offset(MT_TX_32X32), n, cus, levels_d, targets_d, status_d,
+                                      s, dev);
     MT_HIP_TRY(hipGetLastError());
     return MT_OK;
 }

@@ -85,7 +85,8 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
     }
     __syncthreads();
     if (tr.d) {
-        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb, *tr.tabs);
+        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb,
+                                tr.tabs->scan + scan_offset(tr.d->tx_size), *tr.tabs);
         __syncthreads();
         if (lane < N) {
             const int fy = lane;
@@ -206,7 +207,8 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
     }
     __syncthreads();
     if (tr.d) {
-        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb, *tr.tabs);
+        (void)trellis_group<64>(*tr.d, true, tr.lv, tr.tg, tr.suf, tr.pre, tr.lb,
+                                tr.tabs->scan + scan_offset(tr.d->tx_size), *tr.tabs);
         __syncthreads();
         for (int rc = lane; rc < 1024; rc += 64) {
             int32_t step = rc == 0 ? p.dc : p.ac;

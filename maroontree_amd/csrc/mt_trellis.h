@@ -221,6 +221,16 @@ __device__ inline unsigned long long group_or(unsigned long long v) {
     return v;
 }
 
+// Orders the LDS accesses of the lanes of one wave around this point.  A wave's LDS
+// instructions execute in issue order, so lanes exchange data through LDS once the compiler
+// keeps each access on its side of the point; unlike a workgroup barrier this does not wait
+// for the wave's outstanding global loads.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // First scan index of the anti-diagonal x + y == dd of a w x h block whose scan visits the
 // anti-diagonals one after another (every scan the trellis reads does).
 __device__ inline int diag_start(int dd, int w, int h) {
@@ -234,7 +244,8 @@ __device__ inline int diag_start(int dd, int w, int h) {
 // The trellis of one block per aligned group of G lanes (G >= 2, a power of two), the whole
 // workgroup being one wave that calls this together: d the group's descriptor (have false:
 // no block), cf / tf its x-major levels and targets in LDS, sa / sb LDS workspaces of one
-// float per coefficient (16-byte aligned), lb an LDS byte per coefficient.  Every f32
+// float per coefficient (16-byte aligned), lb LDS bytes for the coefficients and 2 h + 1
+// more (h the coded height), scan the block's scan (LDS or global).  Every f32
 // operation is the one of the reference's sequential walk, in its order; only independent
 // work is spread over the lanes:
 //   - trellis_optimize_ctx's Step A visits the scan backwards and prices each level against
@@ -252,9 +263,9 @@ __device__ inline int diag_start(int dd, int w, int h) {
 //     strict minimum is a group reduction.
 // Returns -1 without a block, MT_ERR_ARG (levels untouched) for an invalid descriptor, else
 // MT_OK.  cf / tf must be visible to the group on entry; cf holds the result on return.
-template <int G>
-__device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, const float* tf, float* sa, float* sb,
-                             uint8_t* lb, const mtlib::TrellisTabs& T) {
+template <int G, bool TILT>
+__device__ int trellis_group_impl(const mt_trellisblk& d, bool have, int32_t* cf, const float* tf, float* sa,
+                                  float* sb, uint8_t* lb, const uint16_t* scan, const mtlib::TrellisTabs& T) {
     static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group of 2..64 lanes");
     const int gl = threadIdx.x & (G - 1);
     TrellisGeom g{};
@@ -263,7 +274,6 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
     bool act = ok && d.lambda0 > 0.0f;
     const bool ctxv = d.flags & MT_TRELLIS_CTX;
     const int N = act ? g.N : 0, h = act ? g.h : 1, log2h = act ? g.log2h : 0;
-    const uint16_t* scan = T.scan + (act ? g.so : 0);
     // the quantizer steps (Quant::new_with_qm / new_chroma_with_delta_qm) and lambda
     float dc_q = 1.0f, ac_q = 1.0f;
     if (act) {
@@ -276,20 +286,32 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
     }
     const float lam = trellis_lambda(dc_q, act ? d.lambda0 : 0.05f);
     const float q2dc = __fmul_rn(dc_q, dc_q), q2ac = __fmul_rn(ac_q, ac_q);
-    __syncthreads();
+    wave_lds_sync();
     // the end of block, the level bytes of the context sums, the anti-diagonals holding a
     // nonzero level
     int e_loc = -1;
     unsigned long long dm = 0;
-    for (int i = gl; i < N; i += G) {
-        const uint32_t rc = scan[i];
-        const int32_t c = cf[rc];
-        lb[rc] = (uint8_t)level_byte(uabs32(c));
-        if (c != 0) {
-            e_loc = i;
-            dm |= 1ull << ((rc >> log2h) + (rc & (h - 1)));
+    for (int i0 = gl; i0 < N; i0 += 4 * G) {  // four positions per lane and pass
+        uint32_t rc[4];
+        int32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * G;
+            rc[u] = i < N ? scan[i] : 0u;
+            c[u] = i < N ? cf[rc[u]] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * G;
+            if (i >= N) continue;
+            lb[rc[u]] = (uint8_t)level_byte(uabs32(c[u]));
+            if (c[u] != 0) {
+                e_loc = i;
+                dm |= 1ull << ((rc[u] >> log2h) + (rc[u] & (h - 1)));
+            }
         }
     }
+    for (int k = N + gl; k <= N + 2 * h; k += G) lb[k] = 0;  // the context reads past the block
     const int eob = group_max<G>(e_loc);
     act = act && eob >= 0;
     const int m = eob + 1;
@@ -320,7 +342,7 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
         const float e = __fsub_rn(fabsf(tf[rc]), (float)lev);
         float dd = __fmul_rn(rc == 0 ? q2dc : q2ac, __fmul_rn(e, e));
         if (qm_on) dd = __fmul_rn(dd, qm_at(rc));
-        if (tilt != 0.0f && fabsf(tf[rc]) < 2.0f)
+        if (TILT && tilt != 0.0f && fabsf(tf[rc]) < 2.0f)
             dd = __fdiv_rn(dd, __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)idx, n_inv))));
         return dd;
     };
@@ -336,7 +358,7 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
             dcur = __fmul_rn(dcur, w2);
             dzero = __fmul_rn(dzero, w2);
         }
-        if (tilt != 0.0f && at < 2.0f) {
+        if (TILT && tilt != 0.0f && at < 2.0f) {
             const float wgt = __fdiv_rn(1.0f, __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)i, n_inv))));
             dzero = __fmul_rn(dzero, wgt);
             dcur = __fmul_rn(dcur, wgt);
@@ -347,7 +369,7 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
         if (mm >= 15) v = __fadd_rn(v, golomb_bits(mm - 15));
         return v;
     };
-    auto lvl = [&](int k) -> uint32_t { return k < N ? (uint32_t)lb[k] : 0u; };
+    auto lvl = [&](int k) -> uint32_t { return lb[k]; };  // zero past the block
     auto interior_ctx = [&](uint32_t rc, int& ctx, int& bc) {
         const int x = (int)(rc >> log2h), y = (int)(rc & (h - 1));
         const int base = x * h + y;
@@ -388,38 +410,97 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
         }
     }
     // trellis_optimize_ctx (src/trellis.rs:55) Step A: interior positions eob-1 .. 1 by
-    // anti-diagonal, then DC
-    for (unsigned long long wm = group_or<64>(dm) & ~1ull; wm != 0ull;) {
-        const int dd = 63 - __clzll((long long)wm);
-        wm &= ~(1ull << dd);
-        __syncthreads();
-        if (!((dm >> dd) & 1ull)) continue;
+    // anti-diagonal, then DC.  What pricing a position reads besides its neighbours' levels
+    // (its scan position, level, target, weights, context offset) Step A does not change: a
+    // lane loads it for its position of the next anti-diagonal before the barrier.
+    struct StepPos {
+        uint32_t rc, l, base;
+        int i, lo_ctx, bcb;
+        float at, q2, w2, tdiv;
+        bool tilted;
+    };
+    auto fill = [&](int i) {
+        StepPos p;
+        const uint32_t rc = scan[i];
+        const int x = (int)(rc >> log2h), y = (int)(rc & (h - 1));
+        p.rc = rc;
+        p.l = uabs32(cf[rc]);
+        p.i = i;
+        p.base = (uint32_t)(x * h + y);
+        p.lo_ctx = lo_off[(y < 4 ? y : 4) * 5 + (x < 4 ? x : 4)];
+        p.bcb = (y | x) > 1 ? 14 : 7;
+        p.at = fabsf(tf[rc]);
+        p.q2 = rc == 0 ? q2dc : q2ac;
+        p.w2 = qm_on ? qm_at(rc) : 1.0f;
+        p.tilted = TILT && tilt != 0.0f && p.at < 2.0f;
+        p.tdiv = p.tilted ? __fadd_rn(1.0f, __fmul_rn(tilt, __fmul_rn((float)i, n_inv))) : 1.0f;
+        return p;
+    };
+    auto stage = [&](int dd) {
+        StepPos p{};
+        if (dd < 0 || !((dm >> dd) & 1ull)) return p;
         const int lo = max(1, diag_start(dd, g.w, h)), hi = min(eob - 1, diag_start(dd + 1, g.w, h) - 1);
-        for (int i = lo + gl; i <= hi; i += G) {
-            const uint32_t rc = scan[i], l = uabs32(cf[rc]);
-            if (l == 0) continue;
-            int ctx, bc;
-            interior_ctx(rc, ctx, bc);
-            const float b0 = bt[ctx][0], b1 = bt[ctx][1], b2 = bt[ctx][2], b3 = bt[ctx][3];
-            auto rate_k = [&](uint32_t k) {
-                return k == 0 ? b0 : k == 1 ? __fadd_rn(b1, 1.0f) : k == 2 ? __fadd_rn(b2, 1.0f)
-                     : __fadd_rn(__fadd_rn(b3, hi_cost(k, bc)), 1.0f);
-            };
-            uint32_t best_k = l;
-            float best_c = __fadd_rn(distw(i, rc, l), __fmul_rn(lam, rate_k(l)));
-            for (int k = (int)l - 1; k >= 0; k--) {
-                const float dk = distw(i, rc, (uint32_t)k);
-                if (dk >= best_c) break;
-                const float c = __fadd_rn(dk, __fmul_rn(lam, rate_k((uint32_t)k)));
-                if (c < best_c) {
-                    best_c = c;
-                    best_k = (uint32_t)k;
+        if (lo + gl <= hi) p = fill(lo + gl);
+        return p;
+    };
+    // the interior level decision of one position (distw and the rates of the walk)
+    auto price = [&](const StepPos& p) {
+        if (p.l == 0) return;
+        const int base = (int)p.base;
+        const uint32_t hm = lvl(base + 1) + lvl(base + h) + lvl(base + h + 1);
+        const uint32_t mag = hm + lvl(base + 2) + lvl(base + 2 * h);
+        const int ctx = p.lo_ctx + (mag > 512 ? 4 : (int)((mag + 64) >> 7));
+        const uint32_t mg = hm & 63;
+        const int bc = p.bcb + (mg > 12 ? 6 : (int)((mg + 1) >> 1));
+        const float b0 = bt[ctx][0], b1 = bt[ctx][1], b2 = bt[ctx][2], b3 = bt[ctx][3];
+        auto rate_k = [&](uint32_t k) {
+            return k == 0 ? b0 : k == 1 ? __fadd_rn(b1, 1.0f) : k == 2 ? __fadd_rn(b2, 1.0f)
+                 : __fadd_rn(__fadd_rn(b3, hi_cost(k, bc)), 1.0f);
+        };
+        auto dist = [&](uint32_t lev) {
+            const float e = __fsub_rn(p.at, (float)lev);
+            float dd = __fmul_rn(p.q2, __fmul_rn(e, e));
+            if (qm_on) dd = __fmul_rn(dd, p.w2);
+            if (TILT && p.tilted) dd = __fdiv_rn(dd, p.tdiv);
+            return dd;
+        };
+        uint32_t best_k = p.l;
+        float best_c = __fadd_rn(dist(p.l), __fmul_rn(lam, rate_k(p.l)));
+        for (int k = (int)p.l - 1; k >= 0; k--) {
+            const float dk = dist((uint32_t)k);
+            if (dk >= best_c) break;
+            const float c = __fadd_rn(dk, __fmul_rn(lam, rate_k((uint32_t)k)));
+            if (c < best_c) {
+                best_c = c;
+                best_k = (uint32_t)k;
+            }
+        }
+        if (best_k != p.l) set_mag(p.rc, best_k);
+    };
+    {
+        unsigned long long wm = group_or<64>(dm) & ~1ull;  // the same on every lane
+        wm = ((unsigned long long)__builtin_amdgcn_readfirstlane((int)(wm >> 32)) << 32) |
+             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)wm);
+        int dd = wm ? 63 - __clzll((long long)wm) : -1;
+        StepPos cur = stage(dd);
+        while (dd >= 0) {
+            wm &= ~(1ull << dd);
+            const int dn = wm ? 63 - __clzll((long long)wm) : -1;
+            const StepPos nxt = stage(dn);
+            wave_lds_sync();
+            price(cur);
+            if ((dm >> dd) & 1ull) {  // positions past the group's width (a group narrower than
+                                      // the anti-diagonal)
+                const int lo = max(1, diag_start(dd, g.w, h)), hi = min(eob - 1, diag_start(dd + 1, g.w, h) - 1);
+                for (int i = lo + gl + G; i <= hi; i += G) {
+                    price(fill(i));
                 }
             }
-            if (best_k != l) set_mag(rc, best_k);
+            cur = nxt;
+            dd = dn;
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (act && ctxv && gl == 0) {
         const uint32_t rc = scan[0], l = uabs32(cf[rc]);
         if (l != 0) {
@@ -446,33 +527,45 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
             if (best_k != l) set_mag(rc, best_k);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // the terms of the two running sums, in the order of the sums.  Context variant (i in
     // 1..m-1): sa[m-1-i] the zero-level distortion of i, sb[i-1] lambda * rate of its kept
     // level plus its distortion.  Plain variant (i in 0..m-1): sa[m-1-i] q2 * |t|^2,
     // sb[i] the kept level's distortion.
     if (act) {
-        for (int i = gl; i < m; i += G) {
+        auto term = [&](int i, float& ta, float& tb) {
             const uint32_t rc = scan[i];
             if (ctxv) {
-                if (i == 0) continue;
-                float dcur, dz;
-                dists(i, rc, dcur, dz);
+                float dcur;
+                dists(i, rc, dcur, ta);
                 int ctx, bc;
                 interior_ctx(rc, ctx, bc);
-                sa[m - 1 - i] = dz;
-                sb[i - 1] = __fadd_rn(__fmul_rn(lam, interior_rate(ctx, bc, uabs32(cf[rc]))), dcur);
+                tb = __fadd_rn(__fmul_rn(lam, interior_rate(ctx, bc, uabs32(cf[rc]))), dcur);
             } else {
                 const float q2 = rc == 0 ? q2dc : q2ac;
                 const float at = fabsf(tf[rc]);
                 const float er = __fsub_rn(at, (float)uabs32(cf[rc]));
-                sa[m - 1 - i] = __fmul_rn(q2, __fmul_rn(at, at));
-                sb[i] = __fmul_rn(q2, __fmul_rn(er, er));
+                ta = __fmul_rn(q2, __fmul_rn(at, at));
+                tb = __fmul_rn(q2, __fmul_rn(er, er));
+            }
+        };
+        const int first = ctxv ? 1 : 0;
+        // two positions per lane and pass, both priced before either is stored
+        for (int i0 = first + gl; i0 < m; i0 += 2 * G) {
+            const int i1 = i0 + G;
+            float a0, b0, a1 = 0.0f, b1 = 0.0f;
+            term(i0, a0, b0);
+            if (i1 < m) term(i1, a1, b1);
+            sa[m - 1 - i0] = a0;
+            sb[i0 - first] = b0;
+            if (i1 < m) {
+                sa[m - 1 - i1] = a1;
+                sb[i1 - first] = b1;
             }
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // the running sums, inclusive, in place: lane 0 of the group over sa (the suffix sums of
     // the walk, sa[m-1-i] = the sum over i..m-1), lane 1 over sb (the prefix sums: the plain
     // variant adds lambda * rate of each kept level after its distortion)
@@ -514,7 +607,7 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
             }
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // the candidate ends of block: each lane keeps its earliest strict minimum, the group
     // reduces them to the earliest strict minimum of the walk
@@ -636,7 +729,7 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
         const float skip = __fadd_rn(suf_at(0), __fmul_rn(lam, 1.0f));
         keep = (best_e == 0x7fffffff || skip < best) ? 0 : best_e + 1;
     }
-    __syncthreads();  // every lane has read the levels the decision depends on
+    wave_lds_sync();  // every lane has read the levels the decision depends on
     if (act) {
         if (set_e >= 0 && gl == 0) {
             const uint32_t rc = scan[set_e];
@@ -647,8 +740,18 @@ __device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, con
         }
         for (int k = keep + gl; k < m; k += G) cf[scan[k]] = 0;
     }
-    __syncthreads();
+    wave_lds_sync();
     return status;
+}
+
+// trellis_group_impl without the band-tilt division where no block of the wave tilts (the
+// reference's encoder passes a zero tilt): the division is then never evaluated.
+template <int G>
+__device__ int trellis_group(const mt_trellisblk& d, bool have, int32_t* cf, const float* tf, float* sa, float* sb,
+                             uint8_t* lb, const uint16_t* scan, const mtlib::TrellisTabs& T) {
+    if (__ballot(have && d.band_tilt != 0.0f) != 0ull)
+        return trellis_group_impl<G, true>(d, have, cf, tf, sa, sb, lb, scan, T);
+    return trellis_group_impl<G, false>(d, have, cf, tf, sa, sb, lb, scan, T);
 }
 
 }  // namespace

@@ -102,6 +102,9 @@ void launch(const char* name, dim3 g, dim3 b, const std::function<void()>& body)
 #define hipLaunchKernelGGL(k, g, b, shm, st, ...) emu::launch(#k, dim3(g), dim3(b), [&] { k(__VA_ARGS__); })
 
 inline void __syncthreads() { emu::g_bar->wait(); }
+// the emulated kernels that use wave-level ordering run one wave per block
+inline void __builtin_amdgcn_wave_barrier() { emu::g_bar->wait(); }
+#define __builtin_amdgcn_fence(...) ((void)0)
 
 template <class T> inline T emu_shfl(T v, int src_fn(int, int), int arg) {
     const int lane = threadIdx.x;
@@ -123,6 +126,7 @@ inline int emu_idx(int, int s) { return s; }
 template <class T> inline T __shfl_xor(T v, int o) { return emu_shfl(v, emu_xor, o); }
 template <class T> inline T __shfl_up(T v, unsigned o) { return emu_shfl(v, emu_up, (int)o); }
 template <class T> inline T __shfl(T v, int s) { return emu_shfl(v, emu_idx, s); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return emu_shfl(v, emu_idx, 0); }
 inline uint64_t __ballot(int pred) {
     const int lane = threadIdx.x;
     emu::g_xch[lane] = pred ? 1 : 0;
