@@ -228,10 +228,10 @@ __device__ __forceinline__ float wsse_col(const float (&t)[4][4], int k, int l) 
 // The same four measures as dist_kernel for one W x H block, by one thread.  Loops run over
 // the block's static tile grid (so the error stays in registers); the displayed extent only
 // predicates them.  The weighted SSE keeps weighted_sse_avx2's accumulators and order.
-template <int W, int H>
-__device__ mt_distres small_measures(const uint16_t* src, int sstride, const int32_t* pr,
-                                     const int32_t* rs, bool has_res, int vw, int vh, int maxv,
-                                     float scale) {
+// err(i, y, x, ev, epv) gives sample i = y * W + x's reconstruction error ev and prediction
+// error epv (both 0 outside the displayed extent).
+template <int W, int H, class Err>
+__device__ mt_distres small_measures(const Err& err, int vw, int vh, float scale) {
     int e[W * H];
     int64_t sse = 0;
     uint64_t satd = 0, psatd = 0, sad = 0;
@@ -245,13 +245,8 @@ __device__ mt_distres small_measures(const uint16_t* src, int sstride, const int
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int y = ty + r, x = tx + c, i = y * W + x;
-                    int ev = 0, epv = 0;
-                    if (x < vw && y < vh) {
-                        const int sv = src[(size_t)y * sstride + x];
-                        const int p = pr[i];
-                        ev = sv - clampi(p + (has_res ? rs[i] : 0), 0, maxv);
-                        epv = sv - p;
-                    }
+                    int ev, epv;
+                    err(i, y, x, ev, epv);
                     e[i] = ev;
                     ep[r * 4 + c] = epv;
                     sse += (int64_t)ev * ev;
@@ -322,24 +317,91 @@ __device__ mt_distres small_measures(const uint16_t* src, int sstride, const int
 
 using mtlib::kSmallShapes;
 
+// The error samples of one block (residual_pred then sse_recon's difference): the
+// reconstruction error and the prediction error, 0 outside the displayed extent.
+struct SmallBlk {
+    const uint16_t* src;
+    const int32_t* pr;
+    const int32_t* rs;  // null: no residual (prediction-only block)
+    int sstride, vw, vh, maxv;
+};
+__device__ __forceinline__ void small_err(const SmallBlk& B, int i, int y, int x, int& ev, int& epv) {
+    ev = 0;
+    epv = 0;
+    if (x < B.vw && y < B.vh) {
+        const int sv = B.src[(size_t)y * B.sstride + x];
+        const int p = B.pr[i];
+        ev = sv - clampi(p + (B.rs ? B.rs[i] : 0), 0, B.maxv);
+        epv = sv - p;
+    }
+}
+
+// One wave per 64 list entries, one block per lane.  The wave first stages every block's
+// error samples through LDS with loads spread over the lanes (a block's predictions and
+// residuals are contiguous, so each load instruction reads whole lines) as packed 16-bit
+// pairs; each lane then measures its block from LDS.  A wave whose errors do not fit 16 bits
+// (samples or predictions outside the bit depth) measures from memory instead.
 template <int W, int H>
-__global__ void __launch_bounds__(256) dist_small_kernel(const mt_distblk* __restrict__ blocks,
-                                                         const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ count,
-                                                         PlaneSet planes, const int32_t* __restrict__ pred,
-                                                         const int32_t* __restrict__ resid,
-                                                         mt_distres* __restrict__ out) {
+__global__ void __launch_bounds__(64) dist_small_kernel(const mt_distblk* __restrict__ blocks,
+                                                        const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ count,
+                                                        PlaneSet planes, const int32_t* __restrict__ pred,
+                                                        const int32_t* __restrict__ resid,
+                                                        mt_distres* __restrict__ out) {
+    constexpr int E = W * H, BPI = 64 / E, RS = E + 1;
+    __shared__ int32_t s_err[64 * RS];
+    __shared__ SmallBlk s_blk[64];
+    const int t = threadIdx.x;
     const int n = (int)*count;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x) {
-        const uint32_t b = list[li];
-        const mt_distblk d = blocks[b];
-        const mt_plane pl = planes.p[d.plane];
-        const int vw = min(max(pl.vis_w - (int)d.x, 0), W);
-        const int vh = min(max(pl.vis_h - (int)d.y, 0), H);
-        out[b] = small_measures<W, H>(pl.data + (size_t)d.y * pl.stride + d.x, pl.stride,
-                                      pred + d.pred_offset, resid + d.resid_offset,
-                                      !(d.flags & MT_DIST_NO_RESID), vw, vh, (1 << d.bd) - 1,
-                                      d.wsse_scale);
+    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        const int nb = min(64, n - base);
+        uint32_t b = 0;
+        SmallBlk mine{};
+        float scale = 0.0f;
+        if (t < nb) {
+            b = list[base + t];
+            const mt_distblk d = blocks[b];
+            const mt_plane pl = planes.p[d.plane];
+            mine.src = pl.data + (size_t)d.y * pl.stride + d.x;
+            mine.pr = pred + d.pred_offset;
+            mine.rs = (d.flags & MT_DIST_NO_RESID) ? nullptr : resid + d.resid_offset;
+            mine.sstride = pl.stride;
+            mine.vw = min(max(pl.vis_w - (int)d.x, 0), W);
+            mine.vh = min(max(pl.vis_h - (int)d.y, 0), H);
+            mine.maxv = (1 << d.bd) - 1;
+            scale = d.wsse_scale;
+            s_blk[t] = mine;
+        }
+        __syncthreads();
+        bool wide = false;
+        for (int j0 = 0; j0 < nb; j0 += BPI) {
+            const int j = j0 + t / E, i = t % E;
+            if (j < nb) {
+                int ev, epv;
+                small_err(s_blk[j], i, i / W, i % W, ev, epv);
+                wide |= ev != (int)(int16_t)ev || epv != (int)(int16_t)epv;
+                s_err[j * RS + i] = (int)(((uint32_t)ev & 0xffffu) | ((uint32_t)epv << 16));
+            }
+        }
+        const bool packed = __ballot(wide) == 0ull;
+        __syncthreads();
+        if (t < nb) {
+            if (packed) {
+                const int32_t* row = s_err + t * RS;
+                out[b] = small_measures<W, H>(
+                    [&](int i, int, int, int& ev, int& epv) {
+                        const int v = row[i];
+                        ev = (int)(int16_t)(v & 0xffff);
+                        epv = v >> 16;
+                    },
+                    mine.vw, mine.vh, scale);
+            } else {
+                out[b] = small_measures<W, H>(
+                    [&](int i, int y, int x, int& ev, int& epv) { small_err(mine, i, y, x, ev, epv); }, mine.vw,
+                    mine.vh, scale);
+            }
+        }
+        __syncthreads();  // the next entries reuse the LDS
     }
 }
 
@@ -418,15 +480,15 @@ int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, con
     hipLaunchKernelGGL(mtlib::zero_counts_kernel<kSmallShapes + 1>, dim3(1), dim3(64), 0, s, w.cnt);
     hipLaunchKernelGGL(dist_classify_kernel, dim3(std::max(1, std::min((n + 255) / 256, cus * 4))),
                        dim3(256), 0, s, blocks_d, n, ps, w.lists, w.cap, w.cnt, out_d);
-    const int gs = std::max(1, std::min((n + 255) / 256, cus * 8));
+    const int gs = std::max(1, std::min((n + 63) / 64, cus * 16));
     const uint32_t* L = w.lists;
     const size_t cap = w.cap;
-    hipLaunchKernelGGL((dist_small_kernel<4, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L, w.cnt, ps, pred_d, resid, out_d);
-    hipLaunchKernelGGL((dist_small_kernel<4, 8>), dim3(gs), dim3(256), 0, s, blocks_d, L + cap, w.cnt + 1, ps, pred_d, resid, out_d);
-    hipLaunchKernelGGL((dist_small_kernel<8, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L + 2 * cap, w.cnt + 2, ps, pred_d, resid, out_d);
-    hipLaunchKernelGGL((dist_small_kernel<8, 8>), dim3(gs), dim3(256), 0, s, blocks_d, L + 3 * cap, w.cnt + 3, ps, pred_d, resid, out_d);
-    hipLaunchKernelGGL((dist_small_kernel<4, 16>), dim3(gs), dim3(256), 0, s, blocks_d, L + 4 * cap, w.cnt + 4, ps, pred_d, resid, out_d);
-    hipLaunchKernelGGL((dist_small_kernel<16, 4>), dim3(gs), dim3(256), 0, s, blocks_d, L + 5 * cap, w.cnt + 5, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<4, 4>), dim3(gs), dim3(64), 0, s, blocks_d, L, w.cnt, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<4, 8>), dim3(gs), dim3(64), 0, s, blocks_d, L + cap, w.cnt + 1, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<8, 4>), dim3(gs), dim3(64), 0, s, blocks_d, L + 2 * cap, w.cnt + 2, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<8, 8>), dim3(gs), dim3(64), 0, s, blocks_d, L + 3 * cap, w.cnt + 3, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<4, 16>), dim3(gs), dim3(64), 0, s, blocks_d, L + 4 * cap, w.cnt + 4, ps, pred_d, resid, out_d);
+    hipLaunchKernelGGL((dist_small_kernel<16, 4>), dim3(gs), dim3(64), 0, s, blocks_d, L + 5 * cap, w.cnt + 5, ps, pred_d, resid, out_d);
     const int gw = n < cus * 32 ? n : cus * 32;
     hipLaunchKernelGGL(dist_kernel, dim3(gw), dim3(64), 0, s, blocks_d, L + 6 * cap, w.cnt + 6, ps,
                        pred_d, resid, out_d);

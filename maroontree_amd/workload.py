@@ -141,44 +141,46 @@ def synth_recon(src, seed, bd=10, amp=6):
 
 def sb_candidates(width, height, bd=10, qidx=91, qm_level=None, lam=64.0, wsse_scale=1.0):
     """Every (square block, luma mode) candidate of every full 64x64 superblock of a
-    width x height luma plane, as CAND_DTYPE records; returns (records, pool elements)."""
+    width x height luma plane, as CAND_DTYPE records; returns (records, pool elements).
+    The order is the encoder's: superblocks in raster order, within one the blocks by size
+    and then in raster order, and each block's modes one after another (the candidates of
+    one block share its edges and source samples); the pool follows the same order."""
     bdi = {8: 0, 10: 1, 12: 2}[bd]
     if qm_level is None:
         qm_level = _lib.qm_level_law(qidx, 0)
     sbw, sbh = width // 64, height // 64
-    per_sb = []
+    ox, oy, bsz, mode, pf = [], [], [], [], []
     for bs in (4, 8, 16, 32, 64):
         n = 64 // bs
-        by, bx = np.mgrid[0:n, 0:n]
-        for m in ND_LUMA_MODES:
-            per_sb.append((bx.ravel() * bs, by.ravel() * bs, bs, m))
-    cnt = sum(len(p[0]) for p in per_sb)
-    total = cnt * sbw * sbh
-    c = np.zeros(total, _lib.CAND_DTYPE)
+        for by in range(n):
+            for bx in range(n):
+                x, y = bx * bs, by * bs
+                # top-right / bottom-left availability in quadtree decoding order: the row
+                # above the superblock is decoded, and so is the right sibling's top edge for a
+                # left child, but not the next superblock's; bottom-left only along the decoded
+                # superblock to the left (the sample counts clip at the frame edge by themselves)
+                tr = (y == 0 or bx % 2 == 0) and not (x + bs == 64 and y > 0)
+                bl = x == 0 and y + bs < 64
+                for m in ND_LUMA_MODES:
+                    ox.append(x)
+                    oy.append(y)
+                    bsz.append(bs)
+                    mode.append(m)
+                    pf.append((1 if tr else 0) | (2 if bl else 0))
+    ox, oy, bsz = np.array(ox), np.array(oy), np.array(bsz)
     sy, sx = np.mgrid[0:sbh, 0:sbw]
     sx, sy = sx.ravel() * 64, sy.ravel() * 64
-    pos = 0
-    off = 0
-    for ox, oy, bs, m in per_sb:
-        k = len(ox)
-        for j in range(k):
-            sl = slice(pos, pos + sbw * sbh)
-            x = sx + ox[j]
-            y = sy + oy[j]
-            c["x"][sl] = x
-            c["y"][sl] = y
-            c["tx_size"][sl] = SQUARE_TX[bs]
-            c["mode"][sl] = m
-            # top-right / bottom-left availability in quadtree decoding order: the row above
-            # the superblock is decoded, and so is the right sibling's top edge for a left
-            # child, but not the next superblock's; bottom-left only along the decoded
-            # superblock to the left (the sample counts clip at the frame edge by themselves)
-            tr = (oy[j] == 0 or (ox[j] // bs) % 2 == 0) and not (ox[j] + bs == 64 and oy[j] > 0)
-            bl = ox[j] == 0 and oy[j] + bs < 64
-            c["pflags"][sl] = (1 if tr else 0) | (2 if bl else 0)
-            c["offset"][sl] = off + np.arange(sbw * sbh, dtype=np.uint64) * (bs * bs)
-            off += sbw * sbh * bs * bs
-            pos += sbw * sbh
+    total = len(sx) * len(ox)
+    c = np.zeros(total, _lib.CAND_DTYPE)
+    c["x"] = (sx[:, None] + ox[None, :]).ravel()
+    c["y"] = (sy[:, None] + oy[None, :]).ravel()
+    c["tx_size"] = np.tile(np.array([SQUARE_TX[int(b)] for b in bsz]), len(sx))
+    c["mode"] = np.tile(np.array(mode), len(sx))
+    c["pflags"] = np.tile(np.array(pf), len(sx))
+    px = np.tile(bsz.astype(np.uint64) ** 2, len(sx))
+    off = np.concatenate([[0], np.cumsum(px)[:-1]]).astype(np.uint64)
+    c["offset"] = off
+    off = int(px.sum()) if total else 0
     c["planes"] = 1 << 2
     c["qidx"] = qidx
     c["bd_qm"] = (qm_level & 15) | (bdi << 4)
@@ -200,7 +202,7 @@ EOB_BIN = {16: (0, 1), 32: (3, 4), 64: (5, 6), 128: (8, 9), 256: (11, 12), 512: 
 
 def chain_trellis_descriptors(blocks, sub=2):
     """Slow-mode trellis descriptors (TRELLISBLK_DTYPE) for a chain batch of TXBLK_DTYPE
-    records: trellis_optimize_ctx on every block (src/trellis.rs:606), the frame qindex's
+    records: trellis_optimize_ctx on every block (src/trellis.rs:55), the frame qindex's
     coefficient context (get_q_ctx), the luma/chroma end-of-block class of the coded size,
     the DC-sign context of a block without coded neighbours, and the lambda the reference's
     encoder derives per frame (trellis_lambda x hbd_trellis_scale at chroma subsampling
