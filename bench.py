@@ -406,6 +406,7 @@ def bench_eval(args, dev, torch, mt):
     ok = int((res["status"] == 0).sum())
     print(f"[bench] eval: {ms:.3f} ms/step, {n / ms / 1e3:.1f} M candidates/s", file=sys.stderr, flush=True)
     stages = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]}
+    roofline = eval_roofline(cands, stages, mt)
     r = {
         "workload": "every (square block 4x4..64x64, ND luma mode x 11) candidate of each 64x64 "
                     "superblock of a seeded 3840x2160 10-bit synthetic image (gradients, value "
@@ -420,6 +421,7 @@ def bench_eval(args, dev, torch, mt):
         "eval_mpx_per_s": round(pool / (ms / 1000.0) / 1e6, 2),
         "valid": ok,
         "stage_ms_per_launch": stages,
+        "roofline": roofline,
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -453,6 +455,54 @@ def bench_eval(args, dev, torch, mt):
             **cpu_info(threads),
         }
     return r
+
+
+# the kernels of each evaluation stage, by name
+STAGE_KERNELS = {
+    "distortion": ("dist_classify_kernel", "dist_small_kernel", "dist_kernel("),
+    "rate": ("rate_classify_kernel", "rate_kernel(", "rate_wave_kernel"),
+    "intra_pred": ("pred_classify_kernel", "pred_small_kernel", "intra_pred_kernel"),
+    "rdo_recon": ("sb_recon_kernel", "sb_recon_uv_kernel"),
+}
+
+
+def pmc_traffic(path, stage):
+    """HBM bytes per call of a stage's kernels from a committed counter profile, or None."""
+    full = os.path.join(ROOT, "profiles", path)
+    if not os.path.exists(full):
+        return None
+    pmc = json.load(open(full))
+    tot = 0.0
+    for name, k in pmc["kernels"].items():
+        if any(p in name for p in STAGE_KERNELS[stage]):
+            tot += k["traffic_bytes"]
+    return int(tot / pmc["calls_profiled"]) if tot else None
+
+
+def eval_roofline(cands, stages, mt):
+    """Roofline of the evaluation's dominant stage: the bytes each stage must move for the
+    candidate set (its inputs from and outputs to HBM) over its measured time per call."""
+    w = np.array([4, 8, 16, 32, 64])[cands["tx_size"]].astype(np.int64)
+    px = int((w * w).sum())
+    ncoef = int((np.minimum(w, 32) ** 2).sum())
+    edge = int((2 * (4 * w + 1)).sum())
+    n = len(cands)
+    alg = {
+        # source 2 B + prediction 4 B + reconstructed residual 4 B per sample, descriptor, result
+        "distortion": 10 * px + n * (24 + 32),
+        # levels 4 B per coded coefficient, descriptor, bits
+        "rate": 4 * ncoef + n * (32 + 4),
+        # edges (2 w + 2 h + 1 samples), source 2 B, prediction 4 B and residual 2 B per sample
+        "intra_pred": edge + 8 * px + n * 16,
+    }
+    stage = max(alg, key=lambda k: stages.get(k, 0.0))
+    ms = stages[stage]
+    ach = alg[stage] / (ms / 1000.0) / 1e9
+    return {"kernel": f"evaluation stage {stage}", "bound": "hbm", "achieved": round(ach, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic("eval_4k_pmc.json", stage), "bytes_per_launch": alg[stage],
+            "note": "per call over the stage's kernels; every stage is bound by VALU issue and "
+                    "latency, not HBM (profiles/eval_4k_pmc.json holds the instruction counts)"}
 
 
 def bench_rdo(args, dev, torch, mt):
@@ -521,6 +571,18 @@ def bench_rdo(args, dev, torch, mt):
             "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
     }
+    # the reconstruction walk: each leaf reads its source and writes its reconstruction, 2 B per
+    # sample each, in all three planes
+    rec_ms = prof["rdo_recon"][0] / args.rdo_steps
+    rec_bytes = 4 * (npx + 2 * (npx // 4))
+    rec_ach = rec_bytes / (rec_ms / 1000.0) / 1e9
+    r["roofline"] = {"kernel": "rdo_recon (sb_recon_kernel, sb_recon_uv_kernel)", "bound": "hbm",
+                     "achieved": round(rec_ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(rec_ach / HBM_PEAK_GBS, 6),
+                     "traffic": pmc_traffic("rdo_4k_420_pmc.json", "rdo_recon"), "bytes_per_launch": rec_bytes,
+                     "note": "per frame; one wave per superblock walks its leaves in coding order on "
+                             "the wavefront's critical path: latency bound"}
+
     def cpu_crop(fl):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
