@@ -544,6 +544,57 @@ float mt_mode_lambda(int qidx, int bd, int sub);
 int mt_chroma_q_deltas(int qidx, int sub, int* dc_delta, int* ac_delta);
 int mt_qm_chroma_level_law(int qidx, int sub);
 
+/* ---- the encoder's entry: an 8-bit RGB image to the RDO's decisions ---------------- */
+/* EncodeConfig (src/avif.rs:257-344) as far as this path goes; mt_encode_config_default
+ * gives its defaults.  chroma: ChromaFormat (src/avif.rs:75-86); speed: Speed (:91). */
+enum { MT_ENC_YUV420 = 0, MT_ENC_YUV422 = 1, MT_ENC_YUV444 = 2, MT_ENC_MONOCHROME = 3 };
+enum { MT_SPEED_EXTRA_SLOW = 0, MT_SPEED_SLOW = 1, MT_SPEED_MEDIUM = 2, MT_SPEED_FAST = 3 };
+typedef struct {
+    uint8_t quality;                /* 1..100 (80) -> base_q_idx by quality_to_q */
+    uint8_t chroma;                 /* MT_ENC_* (MT_ENC_YUV420) */
+    uint8_t speed;                  /* MT_SPEED_* (MT_SPEED_SLOW) */
+    uint8_t quantization_matrices;  /* 1: QM on (1) */
+    int16_t qmatrix_level;          /* 0..15 for every plane, or -1: the level laws (-1) */
+    uint8_t adaptive_quant, variance_boost, dark_aq, cdef, wiener, screen_content, intrabc,
+        updating_cdf;               /* 1, 1, 1, 0, 0, 1, 1, 1 */
+    uint8_t pad[2];
+} mt_encode_config;
+/* Config switches this library does not run, reported in mt_encode_out.not_applied: the
+ * per-superblock quantizer modulation (adaptive_quant / variance_boost / dark_aq), the
+ * frame filters (cdef, also implied by Speed::Slow; wiener), palette (screen_content),
+ * IntraBC, CDF adaptation (the rates price the frame-initial CDFs), and ExtraSlow's extra
+ * searches (it runs as Slow). */
+enum { MT_ENC_NA_AQ = 1, MT_ENC_NA_VARIANCE_BOOST = 2, MT_ENC_NA_DARK_AQ = 4, MT_ENC_NA_CDEF = 8,
+       MT_ENC_NA_WIENER = 16, MT_ENC_NA_PALETTE = 32, MT_ENC_NA_INTRABC = 64,
+       MT_ENC_NA_UPDATING_CDF = 128, MT_ENC_NA_EXTRA_SLOW = 256 };
+typedef struct {
+    /* out (device, caller-allocated): the planes after the colour conversion and their
+     * reconstructions, width x height for luma, the chroma size for chroma (width / 2 x
+     * height / 2 at 4:2:0, width x height at 4:4:4), rows packed */
+    uint16_t *y, *u, *v;
+    uint16_t *recon_y, *recon_u, *recon_v;
+    void* nodes;                    /* out (device): [nsb][MT_RDO_NODES] mt_rdo_node */
+    mt_rdo_stats stats;             /* out */
+    uint8_t qidx, qm_level, uv_qm_level, pad0;
+    int32_t rdo_flags;              /* the MT_RDO_* search run */
+    uint32_t not_applied;           /* MT_ENC_NA_* */
+    int32_t pad1;
+} mt_encode_out;
+void mt_encode_config_default(mt_encode_config* cfg);
+/* quality_to_q (src/avif.rs:637): quality 1..100 -> base_q_idx 215..1. */
+int mt_quality_to_qindex(int quality);
+/* encode_rgb8 (src/avif.rs:915) up to the bitstream: the colour conversion of
+ * encode_still_lossy_420_with_cdf / encode_still_lossy_with_cdf (src/encoder.rs:530-605,
+ * :398-410: BT.601 full range in Q13 integers, 4:2:0 chroma as the rounded 2 x 2 mean of the
+ * Q13 values) of the device planes g, b, r (PlanarImage's plane order, width x height bytes
+ * each, rows packed), then mt_intra_rdo at quality_to_q(quality) with the reference's decision
+ * lambda, chroma quantizer and QM laws (or the explicit level, or flat with
+ * quantization_matrices = 0).  Speed: Slow and ExtraSlow search with MT_RDO_TRELLIS, Medium
+ * and Fast without; every speed searches angle deltas, filter intra, 4x4 and CfL.
+ * width and height: multiples of 8.  MT_ERR_UNSUPPORTED for 4:2:2 and monochrome. */
+int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint8_t* r_d, int width, int height,
+                   const mt_encode_config* cfg, mt_encode_out* out, void* stream);
+
 /* ---- learned candidate pruning (bf16 CNN on MFMA) ---------------------------------- */
 /* A small network replaces / feeds the reference's hand pruning (proxy_mode_beam
  * src/coder/partition_search.rs:187, split_breakout_k :135-160, rank_luma_modes

@@ -181,3 +181,74 @@ def nodes_to_np(nodes):
 
 def psnr(sse, npx, bd):
     return 10.0 * np.log10(((1 << bd) - 1) ** 2 * npx / max(int(sse), 1))
+
+
+# ---- the encoder's entry (mt_encode_rgb8) ----------------------------------------------
+YUV420, YUV422, YUV444, MONOCHROME = 0, 1, 2, 3  # ChromaFormat (src/avif.rs:77)
+SPEED_EXTRA_SLOW, SPEED_SLOW, SPEED_MEDIUM, SPEED_FAST = 0, 1, 2, 3  # Speed (src/avif.rs:91)
+NOT_APPLIED = {1: "adaptive_quant", 2: "variance_boost", 4: "dark_aq", 8: "cdef", 16: "wiener",
+               32: "screen_content", 64: "intrabc", 128: "updating_cdf", 256: "extra_slow"}
+
+
+class MtEncodeConfig(ctypes.Structure):
+    _fields_ = [("quality", ctypes.c_uint8), ("chroma", ctypes.c_uint8), ("speed", ctypes.c_uint8),
+                ("quantization_matrices", ctypes.c_uint8), ("qmatrix_level", ctypes.c_int16),
+                ("adaptive_quant", ctypes.c_uint8), ("variance_boost", ctypes.c_uint8),
+                ("dark_aq", ctypes.c_uint8), ("cdef", ctypes.c_uint8), ("wiener", ctypes.c_uint8),
+                ("screen_content", ctypes.c_uint8), ("intrabc", ctypes.c_uint8),
+                ("updating_cdf", ctypes.c_uint8), ("pad", ctypes.c_uint8 * 2)]
+
+
+class MtEncodeOut(ctypes.Structure):
+    _fields_ = [("y", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("recon_y", ctypes.c_void_p), ("recon_u", ctypes.c_void_p), ("recon_v", ctypes.c_void_p),
+                ("nodes", ctypes.c_void_p), ("stats", MtRdoStats), ("qidx", ctypes.c_uint8),
+                ("qm_level", ctypes.c_uint8), ("uv_qm_level", ctypes.c_uint8), ("pad0", ctypes.c_uint8),
+                ("rdo_flags", ctypes.c_int32), ("not_applied", ctypes.c_uint32), ("pad1", ctypes.c_int32)]
+
+
+def encode_config(**overrides):
+    """EncodeConfig's defaults (mt_encode_config_default) with keyword overrides."""
+    cfg = MtEncodeConfig()
+    lib().mt_encode_config_default(ctypes.byref(cfg))
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def quality_to_qindex(quality):
+    return int(lib().mt_quality_to_qindex(int(quality)))
+
+
+def encode_rgb8(g, b, r, cfg=None, stream=None):
+    """mt_encode_rgb8: g, b, r uint8 cuda tensors (H, W) (PlanarImage's plane order), W and H
+    multiples of 8.  Returns a dict: y, u, v (the converted planes), recon_y, recon_u,
+    recon_v, nodes (uint8 (nsb, 341 * 16)), stats, qidx, qm_level, uv_qm_level, rdo_flags
+    and not_applied (the names of the config switches this library does not run)."""
+    import torch
+    cfg = cfg if cfg is not None else encode_config()
+    H, W = g.shape
+    dev = g.device
+    init(dev.index or 0)
+    c420 = cfg.chroma == YUV420
+    Hc, Wc = (H // 2, W // 2) if c420 else (H, W)
+    nsb = ((W + 63) // 64) * ((H + 63) // 64)
+    t = {k: torch.empty(s, dtype=torch.int16, device=dev)
+         for k, s in (("y", (H, W)), ("u", (Hc, Wc)), ("v", (Hc, Wc)), ("recon_y", (H, W)),
+                      ("recon_u", (Hc, Wc)), ("recon_v", (Hc, Wc)))}
+    nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    out = MtEncodeOut()
+    for k, v in t.items():
+        setattr(out, k, _dev_ptr(v).value)
+    out.nodes = _dev_ptr(nodes).value
+    _check(lib().mt_encode_rgb8(_dev_ptr(g), _dev_ptr(b), _dev_ptr(r), W, H, ctypes.byref(cfg), ctypes.byref(out),
+                                _stream_ptr(stream)))
+    st = out.stats
+    res = dict(t)
+    res.update(nodes=nodes, qidx=out.qidx, qm_level=out.qm_level, uv_qm_level=out.uv_qm_level,
+               rdo_flags=out.rdo_flags,
+               not_applied=[n for bit, n in NOT_APPLIED.items() if out.not_applied & bit],
+               stats={"bits": st.bits, "sse": st.sse, "candidates": st.candidates, "leaves": st.leaves,
+                      "sbs": st.sbs, "steps": st.steps, "bits_uv": st.bits_uv, "sse_u": st.sse_u,
+                      "sse_v": st.sse_v})
+    return res

@@ -303,6 +303,12 @@ int orc_rdo_have_bl(int xin, int yin, int s);
  * (xin, yin): bit 0 top-right, bit 1 bottom-left coded before it */
 int orc_rdo_rect_avail(int xin, int yin, int s, int k);
 
+/* the reference's RGB -> YCbCr conversion (c420: 4:2:0 chroma of the rounded-up half size,
+ * else 4:4:4) and quality_to_q (oracle_csc.cpp) */
+void orc_csc_rgb8(const uint8_t* r, const uint8_t* g, const uint8_t* b, int w, int h, int c420, uint16_t* y,
+                  uint16_t* cb, uint16_t* cr);
+int orc_quality_to_q(int quality);
+
 #ifdef __cplusplus
 }
 #endif

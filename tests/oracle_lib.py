@@ -538,3 +538,22 @@ def random_cdfs(rng):
     c["txtp8"] = rows((13,), 6)
     c["txtp16"] = rows((13,), 4)
     return c
+
+
+def csc_rgb8(r, g, b, c420):
+    """orc_csc_rgb8: the reference's RGB -> YCbCr of 8-bit planes (H, W); returns (y, cb, cr)
+    uint16, chroma of the rounded-up half size at 4:2:0."""
+    r, g, b = (np.ascontiguousarray(p, dtype=np.uint8) for p in (r, g, b))
+    H, W = r.shape
+    Hc, Wc = ((H + 1) // 2, (W + 1) // 2) if c420 else (H, W)
+    y = np.zeros((H, W), np.uint16)
+    cb = np.zeros((Hc, Wc), np.uint16)
+    cr = np.zeros((Hc, Wc), np.uint16)
+    lib().orc_csc_rgb8(r.ctypes.data_as(ctypes.c_void_p), g.ctypes.data_as(ctypes.c_void_p),
+                       b.ctypes.data_as(ctypes.c_void_p), W, H, 1 if c420 else 0, y.ctypes.data_as(ctypes.c_void_p),
+                       cb.ctypes.data_as(ctypes.c_void_p), cr.ctypes.data_as(ctypes.c_void_p))
+    return y, cb, cr
+
+
+def quality_to_q(quality):
+    return int(lib().orc_quality_to_q(int(quality)))

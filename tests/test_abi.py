@@ -119,3 +119,16 @@ def test_descriptor_layouts():
     assert [mt.PREDBLK_DTYPE.fields[f][1] for f in ("out_offset", "x", "y", "bw", "bh", "mode",
                                                      "angle_delta", "flags", "bd", "plane")] == \
         [0, 4, 6, 8, 9, 10, 11, 12, 13, 14]
+
+
+def test_encode_config_defaults_and_quality_map():
+    """mt_encode_config_default mirrors EncodeConfig::default (src/avif.rs:320-344);
+    mt_quality_to_qindex == the oracle's quality_to_q for every quality."""
+    from maroontree_amd import rdo as R
+    cfg = R.encode_config()
+    assert (cfg.quality, cfg.chroma, cfg.speed, cfg.quantization_matrices, cfg.qmatrix_level) == \
+        (80, R.YUV420, R.SPEED_SLOW, 1, -1)
+    assert (cfg.adaptive_quant, cfg.variance_boost, cfg.dark_aq, cfg.cdef, cfg.wiener, cfg.screen_content,
+            cfg.intrabc, cfg.updating_cdf) == (1, 1, 1, 0, 0, 1, 1, 1)
+    for q in range(-2, 104):
+        assert R.quality_to_qindex(q) == O.quality_to_q(q), q
