@@ -2,11 +2,18 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <map>
 #include <string>
+#include <utility>
 
 #include "../../include/mtgpu.h"
 
 namespace mtlib {
+// Library-owned workspaces are kept per (device, stream): calls on different streams never
+// share one, and the calls on one stream are ordered by it.
+using StreamKey = std::pair<int, uintptr_t>;
+inline StreamKey stream_key(int dev, hipStream_t s) { return {dev, reinterpret_cast<uintptr_t>(s)}; }
 // Records the calling thread's last error message and returns `code`.
 int set_error(int code, const std::string& what);
 // Per-device initialisation hooks of the other translation units, called by mt_init.

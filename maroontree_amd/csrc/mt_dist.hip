@@ -455,7 +455,7 @@ struct DistWork {
     uint32_t* cnt = nullptr;
 };
 std::mutex g_dist_mu;
-DistWork g_dist_work[64];
+std::map<mtlib::StreamKey, DistWork> g_dist_work;
 }  // namespace
 
 namespace mtlib {
@@ -466,7 +466,7 @@ int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, con
     int rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_dist_mu);
-    DistWork& w = g_dist_work[dev];
+    DistWork& w = g_dist_work[mtlib::stream_key(dev, s)];
     if (w.cap < (size_t)n) {
         if (w.lists) MT_HIP_TRY(hipFree(w.lists));
         w.lists = nullptr;

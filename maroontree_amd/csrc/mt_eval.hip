@@ -203,7 +203,7 @@ struct EvalWork {
     size_t cap_targets = 0;
 };
 std::mutex g_eval_mu;
-EvalWork g_work[64];
+std::map<mtlib::StreamKey, EvalWork> g_work;
 
 template <typename T> int grow(T*& p, size_t n) {
     if (p) MT_HIP_TRY(hipFree(p));
@@ -245,7 +245,7 @@ int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool
     int rc = device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_eval_mu);
-    EvalWork& w = g_work[dev];
+    EvalWork& w = g_work[mtlib::stream_key(dev, s)];
     rc = ensure(w, (size_t)n, (size_t)pool_elems);
     if (rc != MT_OK) return rc;
     if (trellis && w.cap_targets < w.cap_pool) {

@@ -408,7 +408,7 @@ struct PredWork {
     uint32_t* cnt = nullptr;
 };
 std::mutex g_pred_mu;
-PredWork g_pred_work[64];
+std::map<mtlib::StreamKey, PredWork> g_pred_work;
 }  // namespace
 
 namespace mtlib {
@@ -421,7 +421,7 @@ int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int
     int rc = device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_pred_mu);
-    PredWork& w = g_pred_work[dev];
+    PredWork& w = g_pred_work[mtlib::stream_key(dev, s)];
     constexpr int K = kSmallShapes + 1;
     if (w.cap < (size_t)n) {
         if (w.lists) MT_HIP_TRY(hipFree(w.lists));

@@ -727,7 +727,7 @@ struct RateWork {
     uint32_t* cnt = nullptr;
 };
 std::mutex g_rate_mu;
-RateWork g_rate_work[64];
+std::map<mtlib::StreamKey, RateWork> g_rate_work;
 }  // namespace
 
 namespace mtlib {
@@ -740,7 +740,7 @@ int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* level
     int rc = device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_rate_mu);
-    RateWork& w = g_rate_work[dev];
+    RateWork& w = g_rate_work[mtlib::stream_key(dev, s)];
     if (w.cap < (size_t)n) {
         if (w.list) MT_HIP_TRY(hipFree(w.list));
         w.list = nullptr;
@@ -795,7 +795,7 @@ struct TrellisWork {
     uint32_t* counts = nullptr;  // kTrClasses
 };
 std::mutex g_trellis_mu;
-TrellisWork g_trellis_work[64];
+std::map<mtlib::StreamKey, TrellisWork> g_trellis_work;
 
 // so0: the scan offset of the class's first size (its sizes' scans lie one after another)
 template <int G, int NE, int NS>
@@ -809,9 +809,8 @@ void launch_trellis_class(const mt_trellisblk* blocks_d, const TrellisWork& w, i
 }  // namespace
 
 namespace mtlib {
-// Sorts the blocks into lane-group classes (per-device lists, grown on demand and shared by
-// the streams of the device: overlapping calls on one device race), then one persistent
-// launch per class.
+// Sorts the blocks into lane-group classes (lists per device and stream, grown on demand),
+// then one persistent launch per class.
 int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
                    int32_t* status_d, hipStream_t s) {
     if (n == 0) return MT_OK;
@@ -819,7 +818,7 @@ int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, cons
     int rc = device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_trellis_mu);
-    TrellisWork& w = g_trellis_work[dev & 63];
+    TrellisWork& w = g_trellis_work[mtlib::stream_key(dev, s)];
     if (w.cap < (size_t)n) {
         if (w.order) MT_HIP_TRY(hipFree(w.order));
         w.order = nullptr;

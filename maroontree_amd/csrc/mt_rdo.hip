@@ -930,8 +930,10 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
     sb_sse_uv[2 * g + 1] = ev;
 }
 
-// Library-owned buffers of one device, grown on demand.
+// Library-owned buffers of one (device, stream), grown on demand; mu serialises the calls
+// that share them.
 struct RdoWork {
+    std::mutex mu;
     size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map = 0, cap_frames = 0;
     uint32_t *sb_uv_cands = nullptr, *sb_uv_pool = nullptr, *sb_uv_dec = nullptr;
     double* sb_bits_uv = nullptr;
@@ -959,8 +961,8 @@ struct RdoWork {
     FrameArgs* fargs = nullptr;
     mtlib::ReconFrame* rframes = nullptr;
 };
-std::mutex g_rdo_mu;
-RdoWork g_rdo[64];
+std::mutex g_rdo_mu;  // guards the map
+std::map<mtlib::StreamKey, RdoWork> g_rdo;
 
 template <typename T> int regrow(T*& p, size_t n) {
     if (p) MT_HIP_TRY(hipFree(p));
@@ -1409,8 +1411,13 @@ extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int f
     int dev = 0, cus = 0;
     rc = mtlib::device_ready(&dev, &cus);
     if (rc != MT_OK) return rc;
-    std::lock_guard<std::mutex> lk(g_rdo_mu);
-    RdoWork& w = g_rdo[dev];
+    RdoWork* wp;
+    {
+        std::lock_guard<std::mutex> lk(g_rdo_mu);
+        wp = &g_rdo[mtlib::stream_key(dev, (hipStream_t)stream)];
+    }
+    RdoWork& w = *wp;
+    std::lock_guard<std::mutex> lk(w.mu);
     for (int i0 = 0; i0 < nframes;) {
         int nf = 0, planes = 0;
         while (i0 + nf < nframes && nf < kBatchFrames && planes + frame_planes(frames[i0 + nf]) <= MT_MAX_PLANES)

@@ -1030,17 +1030,20 @@ namespace {
 struct DeviceState {
     bool init = false;
     int cus = 256;
-    // chain workspace
+    // this device's address of `scratch` (mt_tx_chain_noalloc), from hipGetSymbolAddress
+    uint32_t* scratch_d = nullptr;
+};
+// the chain's workspace of one (device, stream)
+struct ChainWork {
     uint32_t* counts = nullptr;  // [kClasses]
     uint32_t* starts = nullptr;  // [kClasses]
     uint32_t* cursor = nullptr;  // [kClasses]
     uint32_t* order = nullptr;   // [cap]
     size_t order_cap = 0;
-    // this device's address of `scratch` (mt_tx_chain_noalloc), from hipGetSymbolAddress
-    uint32_t* scratch_d = nullptr;
 };
 std::mutex g_mu;
 DeviceState g_dev[64];
+std::map<mtlib::StreamKey, ChainWork> g_chain;
 
 // Optional per-launch timing: while enabled, every launch of mt_tx_chain is bracketed by
 // a pair of HIP events on the caller's stream; mt_profile_end sums them per kernel slot.
@@ -1099,7 +1102,12 @@ int device_ready(int* dev, int* cus) {
 namespace {
 using mtlib::ProfScope;
 
-int ensure_order(DeviceState& st, size_t n) {
+int ensure_order(ChainWork& st, size_t n) {
+    if (!st.counts) {
+        HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&st.cursor, kClasses * sizeof(uint32_t)));
+    }
     if (st.order_cap >= n && st.order) return MT_OK;
     if (st.order) HIP_TRY(hipFree(st.order));
     st.order = nullptr;
@@ -1223,9 +1231,6 @@ extern "C" int mt_init(int device) {
         rc = mtlib::rdo_init_device(device);
         if (rc) return rc;
     }
-    HIP_TRY(hipMalloc(&st.counts, kClasses * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&st.starts, kClasses * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&st.cursor, kClasses * sizeof(uint32_t)));
     // the allocation-free chain's workspace: this device's copy of `scratch`
     {
         void* p = nullptr;
@@ -1467,10 +1472,10 @@ int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_
         return set_err(MT_ERR_ARG, "mt_tx_chain_trellis: levels, targets and recon are required");
     if (n == 0) return MT_OK;
     std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState& st = g_dev[dev];
-    rc = ensure_order(st, (size_t)n);
+    ChainWork& cw = g_chain[mtlib::stream_key(dev, stream)];
+    rc = ensure_order(cw, (size_t)n);
     if (rc) return rc;
-    return enqueue_chain(st.counts, st.starts, st.cursor, st.order, st.cus, blocks_d, n, pool_d,
+    return enqueue_chain(cw.counts, cw.starts, cw.cursor, cw.order, g_dev[dev].cus, blocks_d, n, pool_d,
                          res_d, levels_d, targets_d, recon_d, qdelta_d, stream, true, trellis_d);
 }
 
