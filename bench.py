@@ -77,6 +77,10 @@ def main():
                     help="time only the 4:2:0 RDO (not the luma-only RDO beside it)")
     ap.add_argument("--rdo-only", action="store_true",
                     help="profiling aid: run only the whole-frame RDO leg and print its object")
+    ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking-filter leg")
+    ap.add_argument("--deblock-only", action="store_true",
+                    help="run only the deblocking-filter leg and print its object as the JSON line")
+    ap.add_argument("--deblock-steps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruning leg")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-pruning leg and print its object")
@@ -137,6 +141,9 @@ def main():
         return
     if args.learned_only:
         print(json.dumps(bench_learned(args, dev, torch, mt)), flush=True)
+        return
+    if args.deblock_only:
+        print(json.dumps(bench_deblock(args, dev, torch, mt)), flush=True)
         return
     if args.stream_only:
         print(json.dumps(bench_stream(args, dev, torch, mt)), flush=True)
@@ -283,6 +290,8 @@ def main():
         line["rdo"] = bench_rdo(args, dev, torch, mt)
     if not args.no_rdo:
         line["plain_512"] = bench_plain512(args, dev, torch, mt)
+    if not args.no_deblock:
+        line["deblock"] = bench_deblock(args, dev, torch, mt)
     net = None
     if not args.no_learned:
         line["learned"], net = bench_learned(args, dev, torch, mt, return_net=True)
@@ -463,6 +472,7 @@ STAGE_KERNELS = {
     "rate": ("rate_classify_kernel", "rate_kernel(", "rate_wave_kernel"),
     "intra_pred": ("pred_classify_kernel", "pred_small_kernel", "intra_pred_kernel"),
     "rdo_recon": ("sb_recon_kernel", "sb_recon_uv_kernel"),
+    "deblock": ("deblock_kernel",),
 }
 
 
@@ -630,6 +640,94 @@ def bench_rdo(args, dev, torch, mt):
         if not args.no_cpu:
             r["slow"]["cpu_baseline"] = cpu_crop(flags | R.TRELLIS)
             r["slow"]["speedup_vs_cpu"] = round(r["slow"]["mpx_per_s"] / r["slow"]["cpu_baseline"]["value"], 1)
+    return r
+
+
+def bench_deblock(args, dev, torch, mt):
+    """The in-loop deblocking filter of the RDO leg's frame: the exact intra RDO (Fast flags)
+    of the seeded 3840x2160 10-bit 4:2:0 frame at qindex 91 runs once, untimed; its decisions
+    give the deblock maps (mt_rdo_deblock_maps) and the level law gives luma 11, chroma 5,
+    sharpness 7.  One step = one mt_deblock_frame call (luma, U, V) from the reconstruction to
+    output planes, timed with HIP events on its stream."""
+    from maroontree_amd import deblock as D
+    from maroontree_amd import rdo as R
+    from maroontree_amd import workload as W
+    width, height, bd, qidx = 3840, 2160, 10, 91
+    flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
+    y, u, v = W.synth_yuv(SEED, width, height, bd, 2)
+    t = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
+    nsb = ((width + 63) // 64) * ((height + 63) // 64)
+    frame = {"src": t(y), "src_u": t(u), "src_v": t(v), "bd": bd, "qidx": qidx, "qm_level": mt.qm_level_law(qidx, 2),
+             "lam": R.rdo_lambda(qidx, bd, 2), "chroma": R.CHROMA_420,
+             "recon": torch.empty((height, width), dtype=torch.int16, device=dev),
+             "recon_u": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
+             "recon_v": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
+             "nodes": torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)}
+    stream = torch.cuda.current_stream()
+    R.intra_rdo_batch([frame], flags, stream=stream)
+    maps = D.rdo_maps(frame["nodes"], width, height, stream=stream)
+    ly, luv, sharp = D.levels(qidx, 2)
+    src = (frame["recon"], frame["recon_u"], frame["recon_v"])
+    out = tuple(torch.empty_like(p) for p in src)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def step():
+        D.frame(src[0], src[1], src[2], maps, width, height, 1, sharp, bd, ly, luv, luv, out=out, status=status,
+                stream=stream)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.deblock_steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / args.deblock_steps
+    k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    npx = width * height
+    samples = npx + 2 * (npx // 4)
+    n4 = (width // 4) * (height // 4)
+    alg_bytes = 4 * samples + 6 * n4  # every sample read and written once (uint16), six 1-byte maps
+    ach = alg_bytes / (k_ms / 1000.0) / 1e9
+    print(f"[bench] deblock 4K 4:2:0: {k_ms * 1000:.1f} us/frame (wall {wall_ms * 1000:.1f} us), {ach:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    r = {
+        "workload": "in-loop deblocking (frame_deblock) of the reconstruction of the exact intra RDO (Fast flags) "
+                    "of a seeded 3840x2160 10-bit 4:2:0 synthetic frame at qindex 91: maps from its decisions, "
+                    f"levels y {ly} / uv {luv}, sharpness {sharp}; out of place, luma + U + V in one launch",
+        "us_per_frame": round(k_ms * 1000.0, 2),
+        "wall_us_per_frame": round(wall_ms * 1000.0, 2),
+        "mpx_per_s": round(npx / (k_ms / 1000.0) / 1e6, 1),
+        "status": int(status.item()),
+        "roofline": {"kernel": "deblock_kernel", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic("deblock_4k_420_pmc.json", "deblock"), "bytes_per_launch": alg_bytes,
+                     "note": "algorithmic bytes: 2 B read + 2 B written per sample of the three planes, plus the "
+                             "six 4x4-unit maps; duration from HIP events around each launch"},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        m = maps.cpu().numpy()
+        mp = {k: m[i] for i, k in enumerate(D.MAP_NAMES)}
+        planes = [p.cpu().numpy().view(np.uint16) for p in src]
+        best, got = float("inf"), None
+        for _ in range(3):
+            c = [p.copy() for p in planes]
+            t0 = time.perf_counter()
+            O.frame_deblock(c[0], c[1], c[2], width, height, mp, 1, 1, sharp, bd, ly, luv, luv)
+            best = min(best, time.perf_counter() - t0)
+            got = c
+        same = all((o.cpu().numpy().view(np.uint16) == g).all() for o, g in zip(out, got))
+        r["matches_oracle"] = bool(same)
+        r["cpu_baseline"] = {"value": round(npx / best / 1e6, 2), "unit": "Mpx/s", "kind": "port",
+                             "sample": f"the whole frame (luma + chroma), best of 3 ({best * 1000:.1f} ms), "
+                                       "the oracle's sequential per-superblock-row order, one thread",
+                             "cores": 1}
+        r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
     return r
 
 

@@ -562,11 +562,11 @@ typedef struct {
 /* Config switches this library does not run, reported in mt_encode_out.not_applied: the
  * per-superblock quantizer modulation (adaptive_quant / variance_boost / dark_aq), the
  * frame filters (cdef, also implied by Speed::Slow; wiener), palette (screen_content),
- * IntraBC, CDF adaptation (the rates price the frame-initial CDFs), and ExtraSlow's extra
- * searches (it runs as Slow). */
+ * IntraBC, CDF adaptation (the rates price the frame-initial CDFs), ExtraSlow's extra
+ * searches (it runs as Slow), and the deblocking filter when no dbk planes are given. */
 enum { MT_ENC_NA_AQ = 1, MT_ENC_NA_VARIANCE_BOOST = 2, MT_ENC_NA_DARK_AQ = 4, MT_ENC_NA_CDEF = 8,
        MT_ENC_NA_WIENER = 16, MT_ENC_NA_PALETTE = 32, MT_ENC_NA_INTRABC = 64,
-       MT_ENC_NA_UPDATING_CDF = 128, MT_ENC_NA_EXTRA_SLOW = 256 };
+       MT_ENC_NA_UPDATING_CDF = 128, MT_ENC_NA_EXTRA_SLOW = 256, MT_ENC_NA_DEBLOCK = 512 };
 typedef struct {
     /* out (device, caller-allocated): the planes after the colour conversion and their
      * reconstructions, width x height for luma, the chroma size for chroma (width / 2 x
@@ -579,6 +579,12 @@ typedef struct {
     int32_t rdo_flags;              /* the MT_RDO_* search run */
     uint32_t not_applied;           /* MT_ENC_NA_* */
     int32_t pad1;
+    /* out (device, caller-allocated, optional): the reconstruction after the in-loop
+     * deblocking filter (frame_deblock at loop_filter_levels, src/coder/frame_encode.rs:
+     * 303-346), the planes' sizes as above; NULL skips it (MT_ENC_NA_DEBLOCK) */
+    uint16_t *dbk_y, *dbk_u, *dbk_v;
+    uint8_t lf_level_y, lf_level_u, lf_level_v, lf_sharpness;  /* out: the levels applied */
+    uint32_t dbk_status;            /* out: the deblock's MT_DEBLOCK_* status (0) */
 } mt_encode_out;
 void mt_encode_config_default(mt_encode_config* cfg);
 /* quality_to_q (src/avif.rs:637): quality 1..100 -> base_q_idx 215..1. */
@@ -594,6 +600,65 @@ int mt_quality_to_qindex(int quality);
  * width and height: multiples of 8.  MT_ERR_UNSUPPORTED for 4:2:2 and monochrome. */
 int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint8_t* r_d, int width, int height,
                    const mt_encode_config* cfg, mt_encode_out* out, void* stream);
+
+/* ---- frame filters: the in-loop deblocking filter ------------------------------------- */
+/* Deblocking is out of place: dst planes must not overlap src planes.  Planes are device
+ * uint16, 8-byte aligned, with a stride (samples per row) that is a multiple of 4.  The
+ * library filters every vertical edge of a plane at once, then every horizontal edge: on
+ * the maps of a partition (edges at transform-block starts, each map holding the size of the
+ * transform it lies in) the result equals the reference's order (per 64-row superblock row,
+ * vertical then horizontal edges).  status_d (device, optional) receives the OR of
+ * MT_DEBLOCK_ORDER where two edges of one direction come within each other's reach, an edge
+ * reaches past the plane, or a horizontal edge reaches into the next superblock row (the
+ * result may then differ from the reference's), and MT_DEBLOCK_BAD_MAP where a size is 0
+ * under the alignment rule.  Asynchronous on `stream`. */
+enum { MT_DEBLOCK_ORDER = 1, MT_DEBLOCK_BAD_MAP = 2 };
+typedef struct {
+    const uint16_t* src;   /* device: the plane before filtering */
+    uint16_t* dst;         /* device: the filtered plane */
+    int32_t stride;        /* samples per row of src and dst */
+    int32_t w, h;          /* the plane's coded size, multiples of 4 */
+    int32_t vis_w, vis_h;  /* its visible size: edges at or past it are not filtered */
+    int32_t nc4;           /* row stride of the maps, >= w / 4 */
+    const uint8_t* bw4;    /* device, h / 4 rows: transform width in 4-sample units */
+    const uint8_t* bh4;    /* transform height in 4-sample units */
+    const uint8_t* vedge4; /* device or NULL: nonzero where a vertical edge starts; NULL: where
+                              the 4-unit column is a multiple of bw4 */
+    const uint8_t* hedge4; /* the same for horizontal edges, rows and bh4 */
+} mt_lf_plane;
+/* filter_plane_impl (src/loopfilter.rs:483) of one plane: level 1..63 (<= 0 copies the plane),
+ * sharpness 0..7, luma (widths 4 / 8 / 16) or chroma (4 / 6), sb_rows4 the superblock height
+ * in 4-units (16 for luma, 16 >> sub_y for chroma), bd 8 / 10 / 12. */
+int mt_deblock_plane(const mt_lf_plane* plane, int level, int sharp, int is_luma, int sb_rows4, int bd,
+                     uint32_t* status_d, void* stream);
+/* DeblockGeom (src/coder/frame_deblock.rs:38-57): the luma transform maps with their
+ * edge-start flags and the luma prediction maps the chroma geometry derives from
+ * (chroma_deblock_geom :91: the prediction size >> subsampling, 1..8 4-units, edges where the
+ * position is a multiple of it).  Maps: device, (h8 / 4) rows of w8 / 4. */
+typedef struct {
+    int32_t w8, h8;          /* coded luma size, multiples of 8 */
+    int32_t disp_w, disp_h;  /* visible size */
+    const uint8_t *blk4, *blk4h, *blk4v, *blk4t;
+    const uint8_t *pblk4, *pblk4h;
+    uint8_t sub_x, sub_y, mono, sharp;
+    uint8_t bd, pad[3];
+} mt_deblock_geom;
+/* frame_deblock (src/coder/frame_deblock.rs:206) in one launch: luma at lvl_y; U and V at
+ * lvl_u / lvl_v only when luma is filtered and one of them is > 0 (otherwise copied).  Chroma
+ * planes are (w8 >> sub_x) x (h8 >> sub_y); ignored when mono. */
+int mt_deblock_frame(const mt_deblock_geom* geom, const uint16_t* src_y, const uint16_t* src_u,
+                     const uint16_t* src_v, int32_t y_stride, int32_t uv_stride, uint16_t* dst_y,
+                     uint16_t* dst_u, uint16_t* dst_v, int lvl_y, int lvl_u, int lvl_v, uint32_t* status_d,
+                     void* stream);
+/* loop_filter_levels / loop_filter_sharpness (src/obu.rs:306-334) at the shipped tuning:
+ * lvl_y = round(qidx / 8) (<= 40); lvl_uv = max(round(qidx / 10 * s), 1) (<= 32) while lvl_y
+ * > 0, with s = 0.5 at 4:2:0 (sub 2) and 1 otherwise; sharpness 7. */
+int mt_deblock_levels(int qidx, int sub, int* lvl_y, int* lvl_uv, int* sharp);
+/* The deblock maps of an RDO's decisions (nodes as mt_rdo_frame.nodes of a width x height
+ * frame): record_blk (src/coder/superblock.rs:548) of every leaf, its transform being the
+ * block (the RDO's square partitions).  maps_d (device): six (height / 4) x (width / 4) planes
+ * in the order blk4, blk4h, blk4v, blk4t, pblk4, pblk4h. */
+int mt_rdo_deblock_maps(const void* nodes_d, int width, int height, uint8_t* maps_d, void* stream);
 
 /* ---- learned candidate pruning (bf16 CNN on MFMA) ---------------------------------- */
 /* A small network replaces / feeds the reference's hand pruning (proxy_mode_beam

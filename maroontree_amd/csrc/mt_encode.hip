@@ -1,5 +1,6 @@
 // The encoder's entry (mt_encode_rgb8): the reference's colour conversion of an 8-bit RGB
-// image on the GPU, then the whole-frame RDO at the configuration's quantizer.
+// image on the GPU, then the whole-frame RDO at the configuration's quantizer, then the
+// in-loop deblocking filter on the reconstruction.
 //
 // Colour conversion (src/encoder.rs:114-127, :398-410, :560-602): full-range BT.601 in Q13
 // integers, Y = clamp((Y_R r + Y_G g + Y_B b + 2^12) >> 13); at 4:4:4 Cb / Cr the same with
@@ -10,6 +11,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "../../include/mtgpu.h"
 #include "mt_common.h"
@@ -59,6 +62,30 @@ __global__ void __launch_bounds__(256) csc444_kernel(const uint8_t* __restrict__
     y[i] = (uint16_t)clamp8((kYR * ri + kYG * gi + kYB * bi + half) >> kQ, mx);
     u[i] = (uint16_t)clamp8((kCbR * ri + kCbG * gi + kCbB * bi + off_q + half) >> kQ, mx);
     v[i] = (uint16_t)clamp8((kCrR * ri + kCrG * gi + kCrB * bi + off_q + half) >> kQ, mx);
+}
+
+// Per (device, stream): the deblock maps of the RDO's decisions and the deblock status word.
+struct EncWork {
+    uint8_t* maps = nullptr;
+    size_t maps_bytes = 0;
+    uint32_t* status = nullptr;
+};
+std::mutex g_enc_mu;
+std::map<mtlib::StreamKey, EncWork> g_enc;
+
+int enc_work(int dev, hipStream_t s, size_t maps_bytes, EncWork** out) {
+    std::lock_guard<std::mutex> lk(g_enc_mu);
+    EncWork& w = g_enc[mtlib::stream_key(dev, s)];
+    if (!w.status) MT_HIP_TRY(hipMalloc(&w.status, sizeof(uint32_t)));
+    if (w.maps_bytes < maps_bytes) {
+        if (w.maps) MT_HIP_TRY(hipFree(w.maps));
+        w.maps = nullptr;
+        w.maps_bytes = 0;
+        MT_HIP_TRY(hipMalloc(&w.maps, maps_bytes));
+        w.maps_bytes = maps_bytes;
+    }
+    *out = &w;
+    return MT_OK;
 }
 
 }  // namespace
@@ -174,6 +201,50 @@ extern "C" int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint
     if (cfg->intrabc) na |= MT_ENC_NA_INTRABC;
     if (cfg->updating_cdf) na |= MT_ENC_NA_UPDATING_CDF;
     if (cfg->speed == MT_SPEED_EXTRA_SLOW) na |= MT_ENC_NA_EXTRA_SLOW;
+    out->lf_level_y = out->lf_level_u = out->lf_level_v = out->lf_sharpness = 0;
+    out->dbk_status = 0;
+    if (out->dbk_y && out->dbk_u && out->dbk_v) {
+        // frame_deblock on the reconstruction at loop_filter_levels (the frame has no IntraBC,
+        // so the reference filters it: src/coder/frame_encode.rs:303)
+        int ly = 0, luv = 0, sharp = 0;
+        rc = mt_deblock_levels(q, sub, &ly, &luv, &sharp);
+        if (rc != MT_OK) return rc;
+        int dev = 0;
+        MT_HIP_TRY(hipGetDevice(&dev));
+        const size_t n4 = (size_t)(width / 4) * (height / 4);
+        EncWork* wk = nullptr;
+        rc = enc_work(dev, s, 6 * n4, &wk);
+        if (rc != MT_OK) return rc;
+        MT_HIP_TRY(hipMemsetAsync(wk->status, 0, sizeof(uint32_t), s));
+        rc = mt_rdo_deblock_maps(out->nodes, width, height, wk->maps, stream);
+        if (rc != MT_OK) return rc;
+        mt_deblock_geom g{};
+        g.w8 = width;
+        g.h8 = height;
+        g.disp_w = width;
+        g.disp_h = height;
+        g.blk4 = wk->maps;
+        g.blk4h = wk->maps + n4;
+        g.blk4v = wk->maps + 2 * n4;
+        g.blk4t = wk->maps + 3 * n4;
+        g.pblk4 = wk->maps + 4 * n4;
+        g.pblk4h = wk->maps + 5 * n4;
+        g.sub_x = g.sub_y = c420 ? 1 : 0;
+        g.sharp = (uint8_t)sharp;
+        g.bd = 8;
+        rc = mt_deblock_frame(&g, out->recon_y, out->recon_u, out->recon_v, width, cw, out->dbk_y, out->dbk_u,
+                              out->dbk_v, ly, luv, luv, wk->status, stream);
+        if (rc != MT_OK) return rc;
+        uint32_t st = 0;
+        MT_HIP_TRY(hipMemcpyAsync(&st, wk->status, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        MT_HIP_TRY(hipStreamSynchronize(s));
+        out->lf_level_y = (uint8_t)ly;
+        out->lf_level_u = out->lf_level_v = (uint8_t)luv;
+        out->lf_sharpness = (uint8_t)sharp;
+        out->dbk_status = st;
+    } else {
+        na |= MT_ENC_NA_DEBLOCK;
+    }
     out->not_applied = na;
     return MT_OK;
 }

@@ -187,7 +187,7 @@ def psnr(sse, npx, bd):
 YUV420, YUV422, YUV444, MONOCHROME = 0, 1, 2, 3  # ChromaFormat (src/avif.rs:77)
 SPEED_EXTRA_SLOW, SPEED_SLOW, SPEED_MEDIUM, SPEED_FAST = 0, 1, 2, 3  # Speed (src/avif.rs:91)
 NOT_APPLIED = {1: "adaptive_quant", 2: "variance_boost", 4: "dark_aq", 8: "cdef", 16: "wiener",
-               32: "screen_content", 64: "intrabc", 128: "updating_cdf", 256: "extra_slow"}
+               32: "screen_content", 64: "intrabc", 128: "updating_cdf", 256: "extra_slow", 512: "deblock"}
 
 
 class MtEncodeConfig(ctypes.Structure):
@@ -204,7 +204,10 @@ class MtEncodeOut(ctypes.Structure):
                 ("recon_y", ctypes.c_void_p), ("recon_u", ctypes.c_void_p), ("recon_v", ctypes.c_void_p),
                 ("nodes", ctypes.c_void_p), ("stats", MtRdoStats), ("qidx", ctypes.c_uint8),
                 ("qm_level", ctypes.c_uint8), ("uv_qm_level", ctypes.c_uint8), ("pad0", ctypes.c_uint8),
-                ("rdo_flags", ctypes.c_int32), ("not_applied", ctypes.c_uint32), ("pad1", ctypes.c_int32)]
+                ("rdo_flags", ctypes.c_int32), ("not_applied", ctypes.c_uint32), ("pad1", ctypes.c_int32),
+                ("dbk_y", ctypes.c_void_p), ("dbk_u", ctypes.c_void_p), ("dbk_v", ctypes.c_void_p),
+                ("lf_level_y", ctypes.c_uint8), ("lf_level_u", ctypes.c_uint8), ("lf_level_v", ctypes.c_uint8),
+                ("lf_sharpness", ctypes.c_uint8), ("dbk_status", ctypes.c_uint32)]
 
 
 def encode_config(**overrides):
@@ -220,11 +223,13 @@ def quality_to_qindex(quality):
     return int(lib().mt_quality_to_qindex(int(quality)))
 
 
-def encode_rgb8(g, b, r, cfg=None, stream=None):
+def encode_rgb8(g, b, r, cfg=None, stream=None, deblock=True):
     """mt_encode_rgb8: g, b, r uint8 cuda tensors (H, W) (PlanarImage's plane order), W and H
     multiples of 8.  Returns a dict: y, u, v (the converted planes), recon_y, recon_u,
     recon_v, nodes (uint8 (nsb, 341 * 16)), stats, qidx, qm_level, uv_qm_level, rdo_flags
-    and not_applied (the names of the config switches this library does not run)."""
+    and not_applied (the names of the config switches this library does not run); with
+    deblock, also dbk_y, dbk_u, dbk_v (the reconstruction after the deblocking filter),
+    lf_levels (y, u, v, sharpness) and dbk_status."""
     import torch
     cfg = cfg if cfg is not None else encode_config()
     H, W = g.shape
@@ -235,7 +240,8 @@ def encode_rgb8(g, b, r, cfg=None, stream=None):
     nsb = ((W + 63) // 64) * ((H + 63) // 64)
     t = {k: torch.empty(s, dtype=torch.int16, device=dev)
          for k, s in (("y", (H, W)), ("u", (Hc, Wc)), ("v", (Hc, Wc)), ("recon_y", (H, W)),
-                      ("recon_u", (Hc, Wc)), ("recon_v", (Hc, Wc)))}
+                      ("recon_u", (Hc, Wc)), ("recon_v", (Hc, Wc)))
+                      + ((("dbk_y", (H, W)), ("dbk_u", (Hc, Wc)), ("dbk_v", (Hc, Wc))) if deblock else ())}
     nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     out = MtEncodeOut()
     for k, v in t.items():
@@ -246,7 +252,8 @@ def encode_rgb8(g, b, r, cfg=None, stream=None):
     st = out.stats
     res = dict(t)
     res.update(nodes=nodes, qidx=out.qidx, qm_level=out.qm_level, uv_qm_level=out.uv_qm_level,
-               rdo_flags=out.rdo_flags,
+               rdo_flags=out.rdo_flags, dbk_status=out.dbk_status,
+               lf_levels=(out.lf_level_y, out.lf_level_u, out.lf_level_v, out.lf_sharpness),
                not_applied=[n for bit, n in NOT_APPLIED.items() if out.not_applied & bit],
                stats={"bits": st.bits, "sse": st.sse, "candidates": st.candidates, "leaves": st.leaves,
                       "sbs": st.sbs, "steps": st.steps, "bits_uv": st.bits_uv, "sse_u": st.sse_u,

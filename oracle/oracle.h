@@ -309,6 +309,29 @@ void orc_csc_rgb8(const uint8_t* r, const uint8_t* g, const uint8_t* b, int w, i
                   uint16_t* cb, uint16_t* cr);
 int orc_quality_to_q(int quality);
 
+/* ---- in-loop deblocking (oracle_lf.cpp) ---- */
+/* limits (src/loopfilter.rs:33) */
+void orc_lf_limits(int level, int sharp, int* e, int* i, int* h);
+/* loop_filter_scalar (src/loopfilter.rs:106): four lines from px[base], stride_a between
+ * lines, stride_b across the edge */
+void orc_loop_filter(uint16_t* px, int64_t base, int e, int i, int h, int64_t stride_a, int64_t stride_b,
+                     int wd, int bd);
+/* filter_plane_impl (src/loopfilter.rs:483): px packed w x h; vedge4 / hedge4 may be NULL
+ * (edges where the position is a multiple of the size) */
+void orc_filter_plane(uint16_t* px, int w, int h, int vis_w, int vis_h, const uint8_t* bw4, const uint8_t* bh4,
+                      const uint8_t* vedge4, const uint8_t* hedge4, int nc4, int level, int sharp, int is_luma,
+                      int sb_rows4, int bd);
+/* chroma_deblock_geom (src/coder/frame_deblock.rs:91) */
+void orc_chroma_deblock_geom(const uint8_t* pblk4, const uint8_t* pblk4h, int nc4, int cw8, int ch8, int sub_x,
+                             int sub_y, uint8_t* cbw4, uint8_t* cbh4);
+/* frame_deblock (src/coder/frame_deblock.rs:206): planes packed, chroma (w8 >> sub_x) wide */
+void orc_frame_deblock(uint16_t* y, uint16_t* u, uint16_t* v, int w8, int h8, int disp_w, int disp_h,
+                       const uint8_t* blk4, const uint8_t* blk4h, const uint8_t* blk4v, const uint8_t* blk4t,
+                       const uint8_t* pblk4, const uint8_t* pblk4h, int sub_x, int sub_y, int mono, int sharp,
+                       int bd, int lvl_y, int lvl_u, int lvl_v);
+/* loop_filter_levels / loop_filter_sharpness (src/obu.rs:306-334), shipped tuning */
+void orc_lf_levels(int qidx, int sub, int* lvl_y, int* lvl_uv, int* sharp);
+
 #ifdef __cplusplus
 }
 #endif

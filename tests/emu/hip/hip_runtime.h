@@ -187,6 +187,22 @@ inline unsigned atomicAdd(unsigned* p, unsigned v) {
     *p = o + v;
     return o;
 }
+inline int atomicAdd(int* p, int v) {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    const int o = *p;
+    *p = o + v;
+    return o;
+}
+inline unsigned atomicOr(unsigned* p, unsigned v) {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    const unsigned o = *p;
+    *p = o | v;
+    return o;
+}
+struct uint2 { unsigned x, y; };
+inline uint2 make_uint2(unsigned x, unsigned y) { return uint2{x, y}; }
 inline float __fsub_rn(float a, float b) { return a - b; }
 inline float __fdiv_rn(float a, float b) { return a / b; }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {

@@ -557,3 +557,71 @@ def csc_rgb8(r, g, b, c420):
 
 def quality_to_q(quality):
     return int(lib().orc_quality_to_q(int(quality)))
+
+
+# ---- in-loop deblocking (oracle_lf.cpp) ----
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lf_limits(level, sharp):
+    e, i, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().orc_lf_limits(int(level), int(sharp), ctypes.byref(e), ctypes.byref(i), ctypes.byref(h))
+    return e.value, i.value, h.value
+
+
+def loop_filter(px, base, e, i, h, stride_a, stride_b, wd, bd):
+    """orc_loop_filter on a flat uint16 array (modified in place)."""
+    assert px.dtype == np.uint16 and px.flags.c_contiguous
+    L = lib()
+    L.orc_loop_filter.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_int] * 3 + \
+        [ctypes.c_int64] * 2 + [ctypes.c_int] * 2
+    L.orc_loop_filter(_vp(px), int(base), int(e), int(i), int(h), int(stride_a), int(stride_b), int(wd), int(bd))
+
+
+def filter_plane(px, vis_w, vis_h, bw4, bh4, vedge4, hedge4, level, sharp, is_luma, sb_rows4, bd):
+    """orc_filter_plane on a (h, w) uint16 plane (modified in place); maps (h/4, w/4) uint8."""
+    assert px.dtype == np.uint16 and px.flags.c_contiguous
+    h, w = px.shape
+    bw4, bh4 = np.ascontiguousarray(bw4, np.uint8), np.ascontiguousarray(bh4, np.uint8)
+    ve = None if vedge4 is None else np.ascontiguousarray(vedge4, np.uint8)
+    he = None if hedge4 is None else np.ascontiguousarray(hedge4, np.uint8)
+    L = lib()
+    L.orc_filter_plane.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4 + \
+        [ctypes.c_int] * 6
+    L.orc_filter_plane(_vp(px), w, h, int(vis_w), int(vis_h), _vp(bw4), _vp(bh4), _vp(ve), _vp(he),
+                       bw4.shape[1], int(level), int(sharp), int(is_luma), int(sb_rows4), int(bd))
+
+
+def chroma_deblock_geom(pblk4, pblk4h, sub_x, sub_y):
+    pblk4, pblk4h = np.ascontiguousarray(pblk4, np.uint8), np.ascontiguousarray(pblk4h, np.uint8)
+    nr4, nc4 = pblk4.shape
+    cw8, ch8 = (nc4 * 4) >> sub_x, (nr4 * 4) >> sub_y
+    cbw = np.zeros((ch8 // 4, cw8 // 4), np.uint8)
+    cbh = np.zeros_like(cbw)
+    L = lib()
+    L.orc_chroma_deblock_geom.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] * 5 + [ctypes.c_void_p] * 2
+    L.orc_chroma_deblock_geom(_vp(pblk4), _vp(pblk4h), nc4, cw8, ch8, sub_x, sub_y, _vp(cbw), _vp(cbh))
+    return cbw, cbh
+
+
+def frame_deblock(y, u, v, disp_w, disp_h, maps, sub_x, sub_y, sharp, bd, lvl_y, lvl_u, lvl_v):
+    """orc_frame_deblock on packed planes (modified in place).  maps: dict of (h8/4, w8/4)
+    uint8 arrays blk4, blk4h, blk4v, blk4t, pblk4, pblk4h.  u / v None: monochrome."""
+    h8, w8 = y.shape
+    m = {k: np.ascontiguousarray(maps[k], np.uint8) for k in ("blk4", "blk4h", "blk4v", "blk4t", "pblk4", "pblk4h")}
+    mono = u is None
+    L = lib()
+    L.orc_frame_deblock.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [ctypes.c_void_p] * 6 + \
+        [ctypes.c_int] * 8
+    L.orc_frame_deblock(_vp(y), _vp(u), _vp(v), w8, h8, int(disp_w), int(disp_h), _vp(m["blk4"]), _vp(m["blk4h"]),
+                        _vp(m["blk4v"]), _vp(m["blk4t"]), _vp(m["pblk4"]), _vp(m["pblk4h"]), int(sub_x), int(sub_y),
+                        1 if mono else 0, int(sharp), int(bd), int(lvl_y), int(lvl_u), int(lvl_v))
+
+
+def lf_levels(qidx, sub):
+    """orc_lf_levels -> (lvl_y, lvl_uv, sharpness)."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().orc_lf_levels(int(qidx), int(sub), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value

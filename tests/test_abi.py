@@ -132,3 +132,16 @@ def test_encode_config_defaults_and_quality_map():
             cfg.intrabc, cfg.updating_cdf) == (1, 1, 1, 0, 0, 1, 1, 1)
     for q in range(-2, 104):
         assert R.quality_to_qindex(q) == O.quality_to_q(q), q
+
+
+def test_deblock_levels_and_layouts():
+    """mt_deblock_levels (host code) == the oracle's loop_filter_levels for every base_q_idx and
+    subsampling; the deblock records match include/mtgpu.h (mt_lf_plane 72 bytes,
+    mt_deblock_geom 72, mt_encode_out 168 with dbk_y at 136)."""
+    from maroontree_amd import deblock as D
+    from maroontree_amd import rdo as R
+    for q in range(256):
+        for sub in (0, 1, 2):
+            assert D.levels(q, sub) == O.lf_levels(q, sub), (q, sub)
+    assert ctypes.sizeof(D.MtLfPlane) == 72 and ctypes.sizeof(D.MtDeblockGeom) == 72
+    assert ctypes.sizeof(R.MtEncodeOut) == 168 and R.MtEncodeOut.dbk_y.offset == 136
