@@ -9,8 +9,8 @@
 // horizontal edge: AV1's own order, with the same result.  Each tile checks those conditions
 // for the edges it owns and reports maps that break them in a status word.
 //
-// Kernel.  One 256-thread workgroup per 64 x 64 output tile.  The tile and a halo of 16
-// columns and 12 rows are loaded into LDS (88 x 96 samples); the vertical edges whose writes
+// Kernel.  One 512-thread workgroup per 128 x 64 output tile.  The tile and a halo of 16
+// columns and 12 rows are loaded into LDS (88 x 160 samples); the vertical edges whose writes
 // reach the tile are filtered over all 88 rows, then the horizontal edges of the tile's own
 // columns; the tile's interior is stored to the output plane.  The filter is out of place
 // (halos read the input while other tiles write the output).  Edges are listed per filter
@@ -30,15 +30,18 @@
 
 namespace {
 
-constexpr int TW = 64, TH = 64;  // output tile
+constexpr int TW = 128, TH = 64;  // output tile
 constexpr int HX = 16, HY = 12;  // halo columns / rows
-constexpr int LW = TW + 2 * HX;  // LDS columns (96)
+constexpr int LW = TW + 2 * HX;  // LDS columns (160)
 constexpr int LH = TH + 2 * HY;  // LDS rows (88)
 constexpr int NVR = LH / 4;      // 4-unit rows with vertical edges (22)
-constexpr int NVC = TW / 4 + 3;  // vertical-edge columns: the tile's 4-unit columns -1 .. +17 (19)
+constexpr int NVC = TW / 4 + 3;  // vertical-edge columns: the tile's 4-unit columns -1 .. +33 (35)
 constexpr int NHR = TH / 4 + 3;  // horizontal-edge rows: the tile's 4-unit rows -1 .. +17 (19)
-constexpr int NHC = TW / 4;      // horizontal-edge columns: the tile's own (16)
-constexpr int kThreads = 256;
+constexpr int NHC = TW / 4;      // horizontal-edge columns: the tile's own (32)
+constexpr int kThreads = 512;
+constexpr int kEnt = 6;          // edge-list entries: r << kEnt | c
+static_assert(HY == 12 && HX >= 12 && HX % 4 == 0, "halo: 7-sample reach of edges 4 past the tile");
+static_assert(NVC <= (1 << kEnt) && NHC <= (1 << kEnt) && NVR * (1 << kEnt) < 65536, "edge-list packing");
 
 struct LfPlane {
     const uint16_t* src;
@@ -76,13 +79,23 @@ __device__ __forceinline__ int size_h(const LfPlane& P, int r4, int c4) {
     return v < 1 ? 1 : v > 8 ? 8 : v;
 }
 
+// The tile's window of the maps in LDS: 4-unit rows ty0 / 4 - 4 .. + TH / 4 + 3 and columns
+// tx0 / 4 - 4 .. + TW / 4 + 3 (every edge a tile lists or checks, with its left / top
+// neighbour), entries outside the plane 0.
+constexpr int MWR = TH / 4 + 8, MWC = TW / 4 + 8;
+struct Win {
+    const uint8_t *w, *h, *ve, *he;  // sizes and edge flags, MWR x MWC
+    int r0, c0;                      // the window's first 4-unit row / column
+};
+
 // filter width of the vertical edge at (r4, c4), 0 where there is none
-__device__ int vwd(const LfPlane& P, int r4, int c4, uint32_t& st) {
+__device__ __forceinline__ int vwd(const LfPlane& P, const Win& W, int r4, int c4, uint32_t& st) {
     if (r4 < 0 || r4 >= P.h4 || c4 < 1 || c4 >= P.w4) return 0;
-    const int cur = size_w(P, r4, c4);
+    const int i = (r4 - W.r0) * MWC + (c4 - W.c0);
+    const int cur = W.w[i];
     bool present;
     if (P.ve) {
-        present = P.ve[(size_t)r4 * P.nc4 + c4] != 0;
+        present = W.ve[i] != 0;
     } else {
         if (cur == 0) {
             st |= MT_DEBLOCK_BAD_MAP;
@@ -90,14 +103,15 @@ __device__ int vwd(const LfPlane& P, int r4, int c4, uint32_t& st) {
         }
         present = c4 % cur == 0;
     }
-    return present ? edge_width(cur, size_w(P, r4, c4 - 1), P.luma) : 0;
+    return present ? edge_width(cur, W.w[i - 1], P.luma) : 0;
 }
-__device__ int hwd(const LfPlane& P, int r4, int c4, uint32_t& st) {
+__device__ __forceinline__ int hwd(const LfPlane& P, const Win& W, int r4, int c4, uint32_t& st) {
     if (r4 < 1 || r4 >= P.h4 || c4 < 0 || c4 >= P.w4) return 0;
-    const int cur = size_h(P, r4, c4);
+    const int i = (r4 - W.r0) * MWC + (c4 - W.c0);
+    const int cur = W.h[i];
     bool present;
     if (P.he) {
-        present = P.he[(size_t)r4 * P.nc4 + c4] != 0;
+        present = W.he[i] != 0;
     } else {
         if (cur == 0) {
             st |= MT_DEBLOCK_BAD_MAP;
@@ -105,29 +119,29 @@ __device__ int hwd(const LfPlane& P, int r4, int c4, uint32_t& st) {
         }
         present = r4 % cur == 0;
     }
-    return present ? edge_width(cur, size_h(P, r4 - 1, c4), P.luma) : 0;
+    return present ? edge_width(cur, W.h[i - MWC], P.luma) : 0;
 }
 
 // The conditions under which the parallel order equals the reference's, for one edge against
 // the edges before it (the pairs with the edges after it are checked by those).
-__device__ uint32_t check_v(const LfPlane& P, int r4, int c4, int wd) {
+__device__ uint32_t check_v(const LfPlane& P, const Win& W, int r4, int c4, int wd) {
     const int R = reach(wd);
     uint32_t st = 0;
     if (c4 * 4 - R < 0 || c4 * 4 + R > P.w) st |= MT_DEBLOCK_ORDER;
     for (int k = 1; k <= 3; ++k) {
-        const int w2 = vwd(P, r4, c4 - k, st);
+        const int w2 = vwd(P, W, r4, c4 - k, st);
         if (w2 && 4 * k < R + reach(w2)) st |= MT_DEBLOCK_ORDER;
     }
     return st;
 }
-__device__ uint32_t check_h(const LfPlane& P, int r4, int c4, int wd) {
+__device__ uint32_t check_h(const LfPlane& P, const Win& W, int r4, int c4, int wd) {
     const int R = reach(wd);
     uint32_t st = 0;
     if (r4 * 4 - R < 0 || r4 * 4 + R > P.h) st |= MT_DEBLOCK_ORDER;
     const int next = (r4 / P.sb_rows4 + 1) * P.sb_rows4;  // the next superblock row
     if (next < P.h4 && r4 * 4 + R > next * 4) st |= MT_DEBLOCK_ORDER;
     for (int k = 1; k <= 3; ++k) {
-        const int w2 = hwd(P, r4 - k, c4, st);
+        const int w2 = hwd(P, W, r4 - k, c4, st);
         if (w2 && 4 * k < R + reach(w2)) st |= MT_DEBLOCK_ORDER;
     }
     return st;
@@ -226,15 +240,15 @@ __device__ __forceinline__ void lf_line(uint16_t* q, int st, int e, int il, int 
 template <int WD>
 __device__ __forceinline__ void run_v(uint16_t* px, const uint16_t* list, int n, const LfPlane& P) {
     for (int k = threadIdx.x; k < 4 * n; k += kThreads) {
-        const int ent = list[k >> 2], r = ent >> 5, c = ent & 31;
-        lf_line<WD>(px + (4 * r + (k & 3)) * LW + 12 + 4 * c, 1, P.e, P.il, P.hth, P.bd);
+        const int ent = list[k >> 2], r = ent >> kEnt, c = ent & ((1 << kEnt) - 1);
+        lf_line<WD>(px + (4 * r + (k & 3)) * LW + HX - 4 + 4 * c, 1, P.e, P.il, P.hth, P.bd);
     }
 }
 template <int WD>
 __device__ __forceinline__ void run_h(uint16_t* px, const uint16_t* list, int n, const LfPlane& P) {
     for (int k = threadIdx.x; k < 4 * n; k += kThreads) {
-        const int ent = list[k >> 2], r = ent >> 5, c = ent & 31;
-        lf_line<WD>(px + (4 * r + 8) * LW + HX + 4 * c + (k & 3), LW, P.e, P.il, P.hth, P.bd);
+        const int ent = list[k >> 2], r = ent >> kEnt, c = ent & ((1 << kEnt) - 1);
+        lf_line<WD>(px + (4 * r + HY - 4) * LW + HX + 4 * c + (k & 3), LW, P.e, P.il, P.hth, P.bd);
     }
 }
 
@@ -250,44 +264,85 @@ __global__ void __launch_bounds__(kThreads) deblock_kernel(LfArgs a) {
     __shared__ __align__(16) uint16_t s_px[LH * LW];
     __shared__ uint16_t s_vl[4][NVR * NVC];
     __shared__ uint16_t s_hl[4][NHR * NHC];
+    __shared__ uint8_t s_map[4][MWR * MWC];
     __shared__ int s_cnt[8];
     const int tid = threadIdx.x;
+
+    // every global load of the tile (samples and maps) is issued before the first LDS store
+    constexpr int NPX = LH * (LW / 4), PIT = (NPX + kThreads - 1) / kThreads;
+    uint2 pv[PIT];
+#pragma unroll
+    for (int i = 0; i < PIT; ++i) {
+        const int k = tid + i * kThreads;
+        pv[i] = make_uint2(0u, 0u);
+        if (k < NPX) {
+            const int r = k / (LW / 4), c = k - r * (LW / 4);
+            const int gy = gy0 + r, gx = gx0 + 4 * c;
+            if (gy >= 0 && gy < P.h && gx >= 0 && gx < P.w)
+                pv[i] = *reinterpret_cast<const uint2*>(P.src + (size_t)gy * P.stride + gx);
+        }
+    }
+    constexpr int NM = MWR * MWC, MIT = (NM + kThreads - 1) / kThreads;
+    const int r0 = ty0 / 4 - 4, c0 = tx0 / 4 - 4;
+    uint32_t mv[MIT];
+#pragma unroll
+    for (int i = 0; i < MIT; ++i) {
+        const int k = tid + i * kThreads;
+        mv[i] = 0;
+        if (P.filter && k < NM) {
+            const int r4 = r0 + k / MWC, c4 = c0 + k % MWC;
+            if (r4 >= 0 && r4 < P.h / 4 && c4 >= 0 && c4 < P.w / 4) {
+                const size_t j = (size_t)r4 * P.nc4 + c4;
+                mv[i] = (uint32_t)size_w(P, r4, c4) | (uint32_t)size_h(P, r4, c4) << 8 |
+                        (P.ve ? (uint32_t)P.ve[j] << 16 : 0u) | (P.he ? (uint32_t)P.he[j] << 24 : 0u);
+            }
+        }
+    }
     if (tid < 8) s_cnt[tid] = 0;
+#pragma unroll
+    for (int i = 0; i < PIT; ++i) {
+        const int k = tid + i * kThreads;
+        if (k < NPX) {
+            const int r = k / (LW / 4), c = k - r * (LW / 4);
+            *reinterpret_cast<uint2*>(s_px + r * LW + 4 * c) = pv[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MIT; ++i) {
+        const int k = tid + i * kThreads;
+        if (k < NM) {
+            s_map[0][k] = (uint8_t)mv[i];
+            s_map[1][k] = (uint8_t)(mv[i] >> 8);
+            s_map[2][k] = (uint8_t)(mv[i] >> 16);
+            s_map[3][k] = (uint8_t)(mv[i] >> 24);
+        }
+    }
     __syncthreads();
     uint32_t st = 0;
     if (P.filter) {
+        const Win W{s_map[0], s_map[1], s_map[2], s_map[3], r0, c0};
         const int r40 = ty0 / 4 - 3, c40 = tx0 / 4 - 1;
         for (int k = tid; k < NVR * NVC; k += kThreads) {
             const int r = k / NVC, c = k - r * NVC, r4 = r40 + r, c4 = c40 + c;
-            const int wd = vwd(P, r4, c4, st);
+            const int wd = vwd(P, W, r4, c4, st);
             if (wd) {
                 const int cls = wd == 4 ? 0 : wd == 6 ? 1 : wd == 8 ? 2 : 3;
-                s_vl[cls][atomicAdd(&s_cnt[cls], 1)] = (uint16_t)(r * 32 + c);
+                s_vl[cls][atomicAdd(&s_cnt[cls], 1)] = (uint16_t)(r << kEnt | c);
                 if (r4 >= ty0 / 4 && r4 < ty0 / 4 + TH / 4 && c4 >= tx0 / 4 && c4 < tx0 / 4 + TW / 4)
-                    st |= check_v(P, r4, c4, wd);
+                    st |= check_v(P, W, r4, c4, wd);
             }
         }
         const int h40 = ty0 / 4 - 1;
         for (int k = tid; k < NHR * NHC; k += kThreads) {
             const int r = k / NHC, c = k - r * NHC, r4 = h40 + r, c4 = tx0 / 4 + c;
-            const int wd = hwd(P, r4, c4, st);
+            const int wd = hwd(P, W, r4, c4, st);
             if (wd) {
                 const int cls = wd == 4 ? 0 : wd == 6 ? 1 : wd == 8 ? 2 : 3;
-                s_hl[cls][atomicAdd(&s_cnt[4 + cls], 1)] = (uint16_t)(r * 32 + c);
-                if (r4 >= ty0 / 4 && r4 < ty0 / 4 + TH / 4) st |= check_h(P, r4, c4, wd);
+                s_hl[cls][atomicAdd(&s_cnt[4 + cls], 1)] = (uint16_t)(r << kEnt | c);
+                if (r4 >= ty0 / 4 && r4 < ty0 / 4 + TH / 4) st |= check_h(P, W, r4, c4, wd);
             }
         }
-    }
-    for (int k = tid; k < LH * (LW / 4); k += kThreads) {
-        const int r = k / (LW / 4), c = k - r * (LW / 4);
-        const int gy = gy0 + r, gx = gx0 + 4 * c;
-        uint2 v = make_uint2(0u, 0u);
-        if (gy >= 0 && gy < P.h && gx >= 0 && gx < P.w)
-            v = *reinterpret_cast<const uint2*>(P.src + (size_t)gy * P.stride + gx);
-        *reinterpret_cast<uint2*>(s_px + r * LW + 4 * c) = v;
-    }
-    __syncthreads();
-    if (P.filter) {
+        __syncthreads();
         run_v<16>(s_px, s_vl[3], s_cnt[3], P);
         run_v<8>(s_px, s_vl[2], s_cnt[2], P);
         run_v<6>(s_px, s_vl[1], s_cnt[1], P);
