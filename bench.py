@@ -103,6 +103,9 @@ def main():
                     help="NO_SPLIT below this split probability, per node level 64,32,16,8")
     ap.add_argument("--hi", default="1,1,1,1",
                     help="FORCE_SPLIT above this split probability, per node level 64,32,16,8")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the chain's saved (or default) launch schedules instead of "
+                         "autotuning them before the warm-up")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -160,6 +163,19 @@ def main():
     stream = torch.cuda.current_stream()
     print(f"[bench] {n} blocks, {npx / 1e6:.1f} Mpx generated in {gen_s:.1f}s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
+    # the per-size launch schedules: measured now (outside the timed region) by the
+    # autotuner, or the saved / default table
+    from maroontree_amd import autotune as AT
+    if args.no_autotune:
+        sched_src = "saved" if AT.apply() else "default"
+        sched = None
+    else:
+        sched = AT.tune(0, workload=(blocks, pt),
+                        log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+        sched_src = "autotuned"
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        AT.save(sched, os.path.join(ROOT, "gpurun_out", "tx_schedule.json"))
+    sched_now = {mt.TX_NAMES[s]: list(mt.tx_schedule_get(s)) for s in mt.TX_SIZES_REF}
 
     for _ in range(args.warmup):
         mt.tx_chain(bt, pt, res, stream=stream)
@@ -187,7 +203,8 @@ def main():
     nblk = int((blocks["tx_size"] == size_id).sum())
     alg_bytes = nblk * (2 * w * h + 8 + 16)  # int16 residual in, descriptor, result
     achieved = alg_bytes / (k_ms / 1000.0) / 1e9
-    kname = "void tx64_kernel<0>(KArgs)" if w == 64 else f"void tx_tile_kernel<{w}, {h}, 0>(KArgs, int)"
+    kname = ("void tx64_kernel<0>(KArgs)" if w == 64 else
+             f"void tx_tile_kernel<{w}, {h}, 0, {sched_now[slot][0]}>(KArgs, int)")
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", "tx_chain_2p20_pmc.json")
     if os.path.exists(pmc_path) and n == 1 << 20:
@@ -230,6 +247,8 @@ def main():
         },
         "candidates_per_s": round(cand_s, 1),
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
+        "schedule": {"source": sched_src, "tile_shift_wg_per_cu": sched_now,
+                     "tune_seconds": sched["seconds"] if sched else None},
         "roofline": {
             "kernel": f"tx chain {slot}",
             "bound": "hbm",

@@ -709,6 +709,17 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
 int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
                         const float* hi4, float mode_cover, uint16_t* prune_d, void* stream);
 
+/* Launch schedule of the chain's kernel for one transform size on one device (no
+ * counterpart in the reference, which runs one block per call on the CPU; the schedule
+ * autotuner, maroontree_amd/autotune.py, measures and installs these).  tile_shift scales
+ * the blocks a 256-thread workgroup holds in LDS: 1 doubles, 0 keeps, -1 halves the
+ * default max(256 / min(W, H), 4096 / (W * H)) (64x64: one wave per block, 0 only);
+ * wg_per_cu (1..64) caps the persistent grid at wg_per_cu workgroups per CU.  Applies to
+ * mt_tx_chain and mt_tx_chain_noalloc; results never depend on it.  Defaults: 0 / 8, and
+ * 0 / 16 at 64x64.  Sizes without a chain kernel give MT_ERR_UNSUPPORTED. */
+int mt_tx_schedule_set(int device, int tx_size, int tile_shift, int wg_per_cu);
+int mt_tx_schedule_get(int device, int tx_size, int* tile_shift, int* wg_per_cu);
+
 /* Per-kernel timing (no counterpart in the reference; measurement hook).  Between begin
  * and end every launch is bracketed by HIP events on its stream; end synchronizes them and
  * returns, per slot, the summed milliseconds and launch count.  Slot 0 = chain bucketing

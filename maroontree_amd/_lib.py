@@ -115,6 +115,8 @@ def lib():
         L.mt_trellis_lambda0.argtypes = [i32, i32, i32]
         L.mt_trellis_lambda0.restype = ctypes.c_float
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
+        L.mt_tx_schedule_set.argtypes = [i32, i32, i32, i32]
+        L.mt_tx_schedule_get.argtypes = [i32, i32, P(ctypes.c_int), P(ctypes.c_int)]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
         L.mt_intra_pred.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp]
         L.mt_distortion.argtypes = [vp, i32, P(MtPlane), i32, vp, vp, vp, vp]
@@ -290,6 +292,20 @@ def tx_chain_noalloc(blocks, pool, res, levels=None, targets=None, recon=None, s
     opt = lambda t: _dev_ptr(t) if t is not None else None  # noqa: E731
     _check(lib().mt_tx_chain_noalloc(_dev_ptr(blocks), n, _dev_ptr(pool), _dev_ptr(res), opt(levels),
                                      opt(targets), opt(recon), _stream_ptr(stream)))
+
+
+def tx_schedule_set(tx_size, tile_shift, wg_per_cu, device=0):
+    """Installs the chain's launch schedule of one transform size (mt_tx_schedule_set)."""
+    init(device)
+    _check(lib().mt_tx_schedule_set(device, tx_size, tile_shift, wg_per_cu))
+
+
+def tx_schedule_get(tx_size, device=0):
+    """(tile_shift, wg_per_cu) of one transform size on `device`."""
+    init(device)
+    ts, wg = ctypes.c_int(), ctypes.c_int()
+    _check(lib().mt_tx_schedule_get(device, tx_size, ctypes.byref(ts), ctypes.byref(wg)))
+    return ts.value, wg.value
 
 
 PROFILE_SLOTS = ["bucket", "4x4", "4x8", "8x4", "4x16", "16x4", "8x8", "8x16", "16x8", "16x16",

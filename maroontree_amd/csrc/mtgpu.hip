@@ -333,11 +333,15 @@ struct KArgs {
     int type;   // parity modes: tx type
 };
 
-template <int W, int H, int MODE>
+// TS scales the tile: B << 1 blocks per workgroup (TS = 1), B (0) or B >> 1 (-1).  Every
+// line loop strides over the tile's blocks with the 256 threads, so any B gives the same
+// results; the launch schedule of each size picks it.
+template <int W, int H, int MODE, int TS = 0>
 __global__ void __launch_bounds__(256) tx_tile_kernel(KArgs a, int size_slot_idx) {
     using R = TxRules<W, H>;
     using M = ModeTraits<MODE>;
-    constexpr int N = R::N, B = R::B, RS = R::RS, BS = R::BS;
+    constexpr int N = R::N, RS = R::RS, BS = R::BS;
+    constexpr int B = TS > 0 ? R::B << 1 : TS < 0 ? R::B >> 1 : R::B;
     constexpr int NT = 256;
 
     __shared__ int32_t work[B * BS];
@@ -1027,9 +1031,23 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const mt_txblk* blo
 // ------------------------------------------------------------------------- host side
 namespace {
 
+// Launch schedule of one transform size in the chain: tile scale (tx_tile_kernel's TS) and the
+// persistent grid's workgroups per CU (64x64: waves per CU).  The defaults are the hand-picked
+// shapes; mt_tx_schedule_set installs the ones an autotuner measured.
+struct TxSched {
+    int8_t tile_shift = 0;
+    uint8_t wg_per_cu = 8;
+};
+inline TxSched default_sched(int tx_size) {
+    TxSched sc;
+    if (tx_size == MT_TX_64X64) sc.wg_per_cu = 16;
+    return sc;
+}
+
 struct DeviceState {
     bool init = false;
     int cus = 256;
+    TxSched sched[19];  // the chain's launch schedule per transform size
     // this device's address of `scratch` (mt_tx_chain_noalloc), from hipGetSymbolAddress
     uint32_t* scratch_d = nullptr;
 };
@@ -1117,49 +1135,52 @@ int ensure_order(ChainWork& st, size_t n) {
     return MT_OK;
 }
 
-int grid_for(int tiles, int cus) {
-    const int cap = cus * 8;
-    return std::max(1, std::min(tiles, cap));
-}
 
-template <int W, int H, int MODE> int launch_tiles(const KArgs& a, int slot, int tiles_hint,
-                                                  int cus, hipStream_t s) {
-    const int grid = grid_for(tiles_hint, cus);
-    hipLaunchKernelGGL((tx_tile_kernel<W, H, MODE>), dim3(grid), dim3(256), 0, s, a, slot);
+template <int W, int H, int MODE, int TS> int launch_tiles_ts(const KArgs& a, int slot, int n,
+                                                              int wg_per_cu, int cus, hipStream_t s) {
+    using R = TxRules<W, H>;
+    constexpr int B = TS > 0 ? R::B << 1 : TS < 0 ? R::B >> 1 : R::B;
+    const int tiles = (n + B - 1) / B;
+    const int grid = std::max(1, std::min(tiles, cus * wg_per_cu));
+    hipLaunchKernelGGL((tx_tile_kernel<W, H, MODE, TS>), dim3(grid), dim3(256), 0, s, a, slot);
     HIP_TRY(hipGetLastError());
     return MT_OK;
 }
 
-template <int MODE> int launch_size(int tx_size, const KArgs& a, int tiles_hint, int cus,
+// n: the batch's block count (an upper bound of the class's); the chain's whole-batch mode
+// takes the schedule's tile scale, the other modes the default tile.
+template <int W, int H, int MODE> int launch_tiles(const KArgs& a, int slot, int n, TxSched sc,
+                                                  int cus, hipStream_t s) {
+    if constexpr (MODE == MODE_CHAIN) {
+        if (sc.tile_shift > 0) return launch_tiles_ts<W, H, MODE, 1>(a, slot, n, sc.wg_per_cu, cus, s);
+        if (sc.tile_shift < 0) return launch_tiles_ts<W, H, MODE, -1>(a, slot, n, sc.wg_per_cu, cus, s);
+    }
+    return launch_tiles_ts<W, H, MODE, 0>(a, slot, n, sc.wg_per_cu, cus, s);
+}
+
+template <int MODE> int launch_size(int tx_size, const KArgs& a, int n, TxSched sc, int cus,
                                     hipStream_t s) {
     switch (tx_size) {
-        case MT_TX_4X4: return launch_tiles<4, 4, MODE>(a, 0, tiles_hint, cus, s);
-        case MT_TX_4X8: return launch_tiles<4, 8, MODE>(a, 1, tiles_hint, cus, s);
-        case MT_TX_8X4: return launch_tiles<8, 4, MODE>(a, 2, tiles_hint, cus, s);
-        case MT_TX_4X16: return launch_tiles<4, 16, MODE>(a, 3, tiles_hint, cus, s);
-        case MT_TX_16X4: return launch_tiles<16, 4, MODE>(a, 4, tiles_hint, cus, s);
-        case MT_TX_8X8: return launch_tiles<8, 8, MODE>(a, 5, tiles_hint, cus, s);
-        case MT_TX_8X16: return launch_tiles<8, 16, MODE>(a, 6, tiles_hint, cus, s);
-        case MT_TX_16X8: return launch_tiles<16, 8, MODE>(a, 7, tiles_hint, cus, s);
-        case MT_TX_16X16: return launch_tiles<16, 16, MODE>(a, 8, tiles_hint, cus, s);
-        case MT_TX_16X32: return launch_tiles<16, 32, MODE>(a, 9, tiles_hint, cus, s);
-        case MT_TX_32X16: return launch_tiles<32, 16, MODE>(a, 10, tiles_hint, cus, s);
-        case MT_TX_32X32: return launch_tiles<32, 32, MODE>(a, 11, tiles_hint, cus, s);
+        case MT_TX_4X4: return launch_tiles<4, 4, MODE>(a, 0, n, sc, cus, s);
+        case MT_TX_4X8: return launch_tiles<4, 8, MODE>(a, 1, n, sc, cus, s);
+        case MT_TX_8X4: return launch_tiles<8, 4, MODE>(a, 2, n, sc, cus, s);
+        case MT_TX_4X16: return launch_tiles<4, 16, MODE>(a, 3, n, sc, cus, s);
+        case MT_TX_16X4: return launch_tiles<16, 4, MODE>(a, 4, n, sc, cus, s);
+        case MT_TX_8X8: return launch_tiles<8, 8, MODE>(a, 5, n, sc, cus, s);
+        case MT_TX_8X16: return launch_tiles<8, 16, MODE>(a, 6, n, sc, cus, s);
+        case MT_TX_16X8: return launch_tiles<16, 8, MODE>(a, 7, n, sc, cus, s);
+        case MT_TX_16X16: return launch_tiles<16, 16, MODE>(a, 8, n, sc, cus, s);
+        case MT_TX_16X32: return launch_tiles<16, 32, MODE>(a, 9, n, sc, cus, s);
+        case MT_TX_32X16: return launch_tiles<32, 16, MODE>(a, 10, n, sc, cus, s);
+        case MT_TX_32X32: return launch_tiles<32, 32, MODE>(a, 11, n, sc, cus, s);
         case MT_TX_64X64: {
-            const int grid = std::max(1, std::min(tiles_hint, cus * 16));
+            const int grid = std::max(1, std::min(n, cus * (int)sc.wg_per_cu));
             hipLaunchKernelGGL((tx64_kernel<MODE>), dim3(grid), dim3(64), 0, s, a);
             HIP_TRY(hipGetLastError());
             return MT_OK;
         }
         default: return set_err(MT_ERR_UNSUPPORTED, "unsupported transform size");
     }
-}
-
-int blocks_per_tile(int tx_size) {
-    const int w = kTxW[tx_size], h = kTxH[tx_size];
-    if (w == 64) return 1;
-    const int n = w * h, mn = std::min(w, h);
-    return std::max(256 / mn, 4096 / n);
 }
 
 // round(65536 / step) as the reference computes it (f64, round half away from zero).
@@ -1221,6 +1242,7 @@ extern "C" int mt_init(int device) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     st.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    for (int t = 0; t < 19; t++) st.sched[t] = default_sched(t);
     {
         int rc = mtlib::rate_init_device(device);
         if (rc) return rc;
@@ -1374,10 +1396,10 @@ static int parity_launch(int mode, int tx_size, int tx_type, int n, const int32_
     a.recon = out;
     a.n = n;
     a.type = tx_type;
-    const int tiles = (n + blocks_per_tile(tx_size) - 1) / blocks_per_tile(tx_size);
     hipStream_t s = (hipStream_t)stream;
-    if (mode == MODE_FWD) return launch_size<MODE_FWD>(tx_size, a, tiles, g_dev[dev].cus, s);
-    return launch_size<MODE_INV>(tx_size, a, tiles, g_dev[dev].cus, s);
+    const TxSched sc = default_sched(tx_size);
+    if (mode == MODE_FWD) return launch_size<MODE_FWD>(tx_size, a, n, sc, g_dev[dev].cus, s);
+    return launch_size<MODE_INV>(tx_size, a, n, sc, g_dev[dev].cus, s);
 }
 
 extern "C" int mt_fwd_txfm(int tx_size, int tx_type, int n, const int32_t* resid_d,
@@ -1407,6 +1429,7 @@ namespace {
 // trellis_d (optional): the chain runs forward + quantization for every block, the trellis
 // over the levels (its workspace: recon_d), then dequantization + inverse + measurement.
 int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t* order, int cus,
+                  const TxSched* sched,
                   const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,
                   int32_t* levels_d, float* targets_d, int32_t* recon_d, const int8_t* qdelta_d,
                   hipStream_t s, bool profile, const mt_trellisblk* trellis_d = nullptr) {
@@ -1440,9 +1463,9 @@ int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t
     // the grid only needs an upper bound: n blocks can never need more tiles than this.
     for (int slot = 0; slot < 13; slot++) {
         const int sz = kSlotSize[slot];
-        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
         ProfScope ps(profile ? 1 + slot : -1, s);
-        rc = trellis_d ? launch_size<MODE_CFWD>(sz, a, tiles, cus, s) : launch_size<MODE_CHAIN>(sz, a, tiles, cus, s);
+        rc = trellis_d ? launch_size<MODE_CFWD>(sz, a, n, sched[sz], cus, s)
+                       : launch_size<MODE_CHAIN>(sz, a, n, sched[sz], cus, s);
         if (rc) return rc;
     }
     if (!trellis_d) return MT_OK;
@@ -1450,9 +1473,8 @@ int enqueue_chain(uint32_t* counts, uint32_t* starts, uint32_t* cursor, uint32_t
         return rc;
     for (int slot = 0; slot < 13; slot++) {
         const int sz = kSlotSize[slot];
-        const int tiles = (n + blocks_per_tile(sz) - 1) / blocks_per_tile(sz);
         ProfScope ps(profile ? 1 + slot : -1, s);
-        if ((rc = launch_size<MODE_CINV>(sz, a, tiles, cus, s))) return rc;
+        if ((rc = launch_size<MODE_CINV>(sz, a, n, sched[sz], cus, s))) return rc;
     }
     return MT_OK;
 }
@@ -1475,7 +1497,8 @@ int mtlib::tx_chain_launch(const mt_txblk* blocks_d, int n, const int16_t* pool_
     ChainWork& cw = g_chain[mtlib::stream_key(dev, stream)];
     rc = ensure_order(cw, (size_t)n);
     if (rc) return rc;
-    return enqueue_chain(cw.counts, cw.starts, cw.cursor, cw.order, g_dev[dev].cus, blocks_d, n, pool_d,
+    return enqueue_chain(cw.counts, cw.starts, cw.cursor, cw.order, g_dev[dev].cus, g_dev[dev].sched,
+                         blocks_d, n, pool_d,
                          res_d, levels_d, targets_d, recon_d, qdelta_d, stream, true, trellis_d);
 }
 
@@ -1506,8 +1529,33 @@ extern "C" int mt_tx_chain_noalloc(const mt_txblk* blocks_d, int n, const int16_
     if (!st.scratch_d) return set_err(MT_ERR_NOT_INIT, "mt_tx_chain_noalloc: no scratch address");
     uint32_t* base = st.scratch_d;
     return enqueue_chain(base, base + kNoAllocSlots, base + 2 * kNoAllocSlots,
-                         base + 3 * kNoAllocSlots, st.cus, blocks_d, n, pool_d, res_d, levels_d,
+                         base + 3 * kNoAllocSlots, st.cus, st.sched, blocks_d, n, pool_d, res_d, levels_d,
                          targets_d, recon_d, nullptr, (hipStream_t)stream, false);
+}
+
+extern "C" int mt_tx_schedule_set(int device, int tx_size, int tile_shift, int wg_per_cu) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (device < 0 || device >= 64 || !g_dev[device].init)
+        return set_err(MT_ERR_NOT_INIT, "mt_tx_schedule_set: mt_init was not called for this device");
+    if (tx_size < 0 || tx_size >= 19 || !tx_supported(tx_size, MT_TXTP_DCT_DCT))
+        return set_err(MT_ERR_UNSUPPORTED, "mt_tx_schedule_set: no chain kernel for this size");
+    if (tile_shift < -1 || tile_shift > 1 || (tx_size == MT_TX_64X64 && tile_shift != 0))
+        return set_err(MT_ERR_ARG, "mt_tx_schedule_set: tile_shift must be -1, 0 or 1 (0 at 64x64)");
+    if (wg_per_cu < 1 || wg_per_cu > 64)
+        return set_err(MT_ERR_ARG, "mt_tx_schedule_set: wg_per_cu must be 1..64");
+    g_dev[device].sched[tx_size].tile_shift = (int8_t)tile_shift;
+    g_dev[device].sched[tx_size].wg_per_cu = (uint8_t)wg_per_cu;
+    return MT_OK;
+}
+
+extern "C" int mt_tx_schedule_get(int device, int tx_size, int* tile_shift, int* wg_per_cu) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (device < 0 || device >= 64 || !g_dev[device].init)
+        return set_err(MT_ERR_NOT_INIT, "mt_tx_schedule_get: mt_init was not called for this device");
+    if (tx_size < 0 || tx_size >= 19) return set_err(MT_ERR_ARG, "mt_tx_schedule_get: bad size");
+    if (tile_shift) *tile_shift = g_dev[device].sched[tx_size].tile_shift;
+    if (wg_per_cu) *wg_per_cu = g_dev[device].sched[tx_size].wg_per_cu;
+    return MT_OK;
 }
 
 extern "C" int mt_profile_begin(void) {
