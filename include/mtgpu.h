@@ -709,6 +709,39 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
 int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
                         const float* hi4, float mode_cover, uint16_t* prune_d, void* stream);
 
+/* ---- decision pricing ---------------------------------------------------------------- */
+/* luma_satd_scale (src/partition_rd.rs:30), k_satd (:54) and proxy_wsse
+ * (src/coder/lossy_state.rs:1829) at the shipped tuning (host functions). */
+float mt_luma_satd_scale(int base_q_idx, int subsampled);
+float mt_k_satd(int base_q_idx, int subsampled);
+float mt_proxy_wsse(float d);
+/* luma_partition_distortion (src/coder/lossy_state.rs:2385): the SATD partition price of n
+ * blocks from their luma_satd (mt_distortion's / mt_eval_candidates' satd), in the proxy
+ * unit (the shipped proxy_luma_bridge): satd * max(qstep, 1) / 4 * luma_satd_scale *
+ * K_PART_LAMBDA / k_satd * proxy_luma_scale, f32 in that order.  qstep_d (n floats, the
+ * ac_q the caller passes) may be NULL: then every block uses `qstep`.  subsampled: 4:2:0,
+ * 4:2:2 or monochrome. */
+int mt_luma_partition_distortion(const uint64_t* satd_d, const float* qstep_d, float qstep, int n,
+                                 int base_q_idx, int subsampled, float* price_d, void* stream);
+/* sb_activity (src/coder/aq.rs:140) of every 64x64 superblock of a luma plane (act_d:
+ * ceil(W/64) * ceil(H/64) floats, raster order) and, when ref_act_d is not NULL,
+ * tile_ref_activity (:163): their f32 mean in raster order (one float on the device). */
+int mt_sb_activity(mt_plane src, int bd, float* act_d, float* ref_act_d, void* stream);
+/* One block of mt_perceptual_rd_scale: 8 bytes. */
+typedef struct {
+    uint16_t x, y;     /* luma position */
+    uint16_t dim;      /* the block size the caller prices (4..64) */
+    uint16_t pad;
+} mt_prdoblk;
+/* perceptual_rd_scale (src/coder/lossy_state.rs:4189): the per-block lambda scale of n blocks
+ * of the tile whose luma source is `src` (its width x height is the tile), at the shipped
+ * tuning.  format: MT_ENC_*; mono_large: LossyTile::with_mono_large; ref_act_d: the AQ
+ * state's tile activity (one float on the device, e.g. mt_sb_activity's ref_act_d), or NULL
+ * for AQ off (0); sb_act_d: mt_sb_activity of the same plane. */
+int mt_perceptual_rd_scale(const mt_prdoblk* blocks_d, int n, mt_plane src, int bd, int base_q_idx,
+                           int format, int mono_large, const float* ref_act_d, const float* sb_act_d,
+                           float* scale_d, void* stream);
+
 /* Launch schedule of the chain's kernel for one transform size on one device (no
  * counterpart in the reference, which runs one block per call on the CPU; the schedule
  * autotuner, maroontree_amd/autotune.py, measures and installs these).  tile_shift scales

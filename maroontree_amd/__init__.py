@@ -9,6 +9,8 @@ from ._lib import (  # noqa: F401
     qm_level_law, tx_width, tx_height, tx_ncoef, tx_supported, fwd_txfm, inv_txfm, tx_chain,
     tx_chain_noalloc, NOALLOC_MAX_BLOCKS, trellis, trellis_lambda0, TRELLISBLK_DTYPE, TRELLIS_CTX,
     tx_chain_trellis, CAND_TRELLIS, tx_schedule_set, tx_schedule_get,
+    PRDOBLK_DTYPE, luma_satd_scale, k_satd, proxy_wsse, luma_partition_distortion, sb_activity,
+    perceptual_rd_scale,
     gen_tx_blocks, block_rate, RATEBLK_DTYPE, BLK_DTYPE, RES_DTYPE, PREDBLK_DTYPE, MtPlane,
     plane_views, intra_pred, distortion, DISTBLK_DTYPE, DISTRES_DTYPE, DIST_NO_RESID,
     PRED_HAVE_TR, PRED_HAVE_BL, PRED_SMOOTH_NB, PRED_FILTER0, PRED_CFL, CAND_CFL_SS_HOR, CAND_CFL_SS_VER,

@@ -115,6 +115,13 @@ def lib():
         L.mt_trellis_lambda0.argtypes = [i32, i32, i32]
         L.mt_trellis_lambda0.restype = ctypes.c_float
         L.mt_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int), i32]
+        f32 = ctypes.c_float
+        L.mt_luma_satd_scale.argtypes = L.mt_k_satd.argtypes = [i32, i32]
+        L.mt_luma_satd_scale.restype = L.mt_k_satd.restype = L.mt_proxy_wsse.restype = f32
+        L.mt_proxy_wsse.argtypes = [f32]
+        L.mt_luma_partition_distortion.argtypes = [vp, vp, f32, i32, i32, i32, vp, vp]
+        L.mt_sb_activity.argtypes = [MtPlane, i32, vp, vp, vp]
+        L.mt_perceptual_rd_scale.argtypes = [vp, i32, MtPlane, i32, i32, i32, i32, vp, vp, vp, vp]
         L.mt_tx_schedule_set.argtypes = [i32, i32, i32, i32]
         L.mt_tx_schedule_get.argtypes = [i32, i32, P(ctypes.c_int), P(ctypes.c_int)]
         L.mt_block_rate.argtypes = [vp, i32, vp, vp, vp]
@@ -292,6 +299,68 @@ def tx_chain_noalloc(blocks, pool, res, levels=None, targets=None, recon=None, s
     opt = lambda t: _dev_ptr(t) if t is not None else None  # noqa: E731
     _check(lib().mt_tx_chain_noalloc(_dev_ptr(blocks), n, _dev_ptr(pool), _dev_ptr(res), opt(levels),
                                      opt(targets), opt(recon), _stream_ptr(stream)))
+
+
+PRDOBLK_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("dim", "<u2"), ("pad", "<u2")])
+
+
+def luma_satd_scale(base_q_idx, subsampled):
+    return lib().mt_luma_satd_scale(base_q_idx, int(subsampled))
+
+
+def k_satd(base_q_idx, subsampled):
+    return lib().mt_k_satd(base_q_idx, int(subsampled))
+
+
+def proxy_wsse(d):
+    return lib().mt_proxy_wsse(d)
+
+
+def luma_partition_distortion(satd, qstep, base_q_idx, subsampled, stream=None):
+    """SATD partition prices (float32 cuda tensor) of an int64/uint64 cuda tensor of SATD
+    values; qstep is a float or a float32 cuda tensor of the same length."""
+    import torch
+    n = satd.numel()
+    init(satd.device.index or 0)
+    out = torch.empty(n, dtype=torch.float32, device=satd.device)
+    qd = _dev_ptr(qstep) if isinstance(qstep, torch.Tensor) else None
+    qs = 0.0 if isinstance(qstep, torch.Tensor) else float(qstep)
+    _check(lib().mt_luma_partition_distortion(_dev_ptr(satd), qd, qs, n, base_q_idx, int(subsampled),
+                                              _dev_ptr(out), _stream_ptr(stream)))
+    return out
+
+
+def _one_plane(t, width=None, height=None):
+    h, stride = t.shape
+    w = width or stride
+    hh = height or h
+    return MtPlane(_dev_ptr(t), stride, w, hh, w, hh, 0)
+
+
+def sb_activity(plane, bd, width=None, height=None, stream=None):
+    """(sb_activity per superblock, tile_ref_activity) of a 2-D uint16 cuda plane, both
+    float32 cuda tensors."""
+    import torch
+    init(plane.device.index or 0)
+    pl = _one_plane(plane, width, height)
+    nsb = ((pl.width + 63) // 64) * ((pl.height + 63) // 64)
+    act = torch.empty(nsb, dtype=torch.float32, device=plane.device)
+    ref = torch.empty(1, dtype=torch.float32, device=plane.device)
+    _check(lib().mt_sb_activity(pl, bd, _dev_ptr(act), _dev_ptr(ref), _stream_ptr(stream)))
+    return act, ref
+
+
+def perceptual_rd_scale(blocks, plane, bd, base_q_idx, fmt, sb_act, ref_act=None, mono_large=False,
+                        width=None, height=None, stream=None):
+    """perceptual_rd_scale of PRDOBLK_DTYPE blocks (uint8 cuda tensor) as float32 cuda tensor."""
+    import torch
+    n = blocks.numel() // PRDOBLK_DTYPE.itemsize
+    init(plane.device.index or 0)
+    out = torch.empty(n, dtype=torch.float32, device=plane.device)
+    _check(lib().mt_perceptual_rd_scale(_dev_ptr(blocks), n, _one_plane(plane, width, height), bd, base_q_idx,
+                                        fmt, int(mono_large), _dev_ptr(ref_act) if ref_act is not None else None,
+                                        _dev_ptr(sb_act), _dev_ptr(out), _stream_ptr(stream)))
+    return out
 
 
 def tx_schedule_set(tx_size, tile_shift, wg_per_cu, device=0):

@@ -332,6 +332,21 @@ void orc_frame_deblock(uint16_t* y, uint16_t* u, uint16_t* v, int w8, int h8, in
 /* loop_filter_levels / loop_filter_sharpness (src/obu.rs:306-334), shipped tuning */
 void orc_lf_levels(int qidx, int sub, int* lvl_y, int* lvl_uv, int* sharp);
 
+/* ---- decision pricing helpers (oracle_price.cpp) ---- */
+float orc_dirty_log1pf(float d);              /* src/aq_common.rs:90 */
+float orc_dirty_exp2f(float d);               /* src/coder/lossy_state.rs:4284 */
+float orc_exp2_table(int i);                  /* its table, 2^((i - 32) / 64) */
+void orc_block_moments(const uint16_t* plane, int stride, int px, int py, int w, int h,
+                       int64_t* sum, int64_t* sum_sq);
+float orc_sb_activity(const uint16_t* yp, int pw, int sb_y, int sb_x, int width, int height, int bd);
+float orc_tile_ref_activity(const uint16_t* yp, int pw, int w, int h, int bd);
+float orc_luma_satd_scale(int base_q_idx, int subsampled);   /* src/partition_rd.rs:30 */
+float orc_k_satd(int base_q_idx, int subsampled);            /* src/partition_rd.rs:54 */
+float orc_luma_partition_distortion(uint64_t satd, float qstep, int base_q_idx, int subsampled);
+float orc_proxy_wsse(float d);
+float orc_perceptual_rd_scale(const uint16_t* src, int pw, int w, int h, int bd, int base_q_idx,
+                              int fmt, int mono_large, float ref_act, int px, int py, int dim);
+
 #ifdef __cplusplus
 }
 #endif

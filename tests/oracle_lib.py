@@ -625,3 +625,80 @@ def lf_levels(qidx, sub):
     a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib().orc_lf_levels(int(qidx), int(sub), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
     return a.value, b.value, c.value
+
+
+# ---- decision pricing helpers (oracle_price.cpp) ----
+def _price_lib():
+    L = lib()
+    if not getattr(L, "_price_bound", False):
+        f, i, u16p = ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_uint16)
+        for name in ("orc_dirty_log1pf", "orc_dirty_exp2f", "orc_proxy_wsse"):
+            getattr(L, name).argtypes = [f]
+            getattr(L, name).restype = f
+        L.orc_exp2_table.argtypes = [i]
+        L.orc_exp2_table.restype = f
+        L.orc_luma_satd_scale.argtypes = L.orc_k_satd.argtypes = [i, i]
+        L.orc_luma_satd_scale.restype = L.orc_k_satd.restype = f
+        L.orc_luma_partition_distortion.argtypes = [ctypes.c_uint64, f, i, i]
+        L.orc_luma_partition_distortion.restype = f
+        L.orc_sb_activity.argtypes = [u16p, i, i, i, i, i, i]
+        L.orc_sb_activity.restype = f
+        L.orc_tile_ref_activity.argtypes = [u16p, i, i, i, i]
+        L.orc_tile_ref_activity.restype = f
+        L.orc_perceptual_rd_scale.argtypes = [u16p, i, i, i, i, i, i, i, f, i, i, i]
+        L.orc_perceptual_rd_scale.restype = f
+        L._price_bound = True
+    return L
+
+
+def dirty_log1pf(d):
+    return _price_lib().orc_dirty_log1pf(float(d))
+
+
+def dirty_exp2f(d):
+    return _price_lib().orc_dirty_exp2f(float(d))
+
+
+def exp2_table():
+    return np.array([_price_lib().orc_exp2_table(i) for i in range(64)], np.float32)
+
+
+def luma_satd_scale(q, sub):
+    return _price_lib().orc_luma_satd_scale(int(q), int(sub))
+
+
+def k_satd(q, sub):
+    return _price_lib().orc_k_satd(int(q), int(sub))
+
+
+def luma_partition_distortion(satd, qstep, q, sub):
+    return _price_lib().orc_luma_partition_distortion(int(satd), float(qstep), int(q), int(sub))
+
+
+def proxy_wsse(d):
+    return _price_lib().orc_proxy_wsse(float(d))
+
+
+def _u16(plane):
+    plane = np.ascontiguousarray(plane, np.uint16)
+    return plane, plane.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+
+
+def sb_activity(plane, sb_y, sb_x, bd, w=None, h=None):
+    plane, p = _u16(plane)
+    return _price_lib().orc_sb_activity(p, plane.shape[1], sb_y, sb_x, plane.shape[1] if w is None else w,
+                                        plane.shape[0] if h is None else h, bd)
+
+
+def tile_ref_activity(plane, bd, w=None, h=None):
+    plane, p = _u16(plane)
+    return _price_lib().orc_tile_ref_activity(p, plane.shape[1], plane.shape[1] if w is None else w,
+                                              plane.shape[0] if h is None else h, bd)
+
+
+def perceptual_rd_scale(plane, bd, q, fmt, ref_act, px, py, dim, mono_large=0, w=None, h=None):
+    plane, p = _u16(plane)
+    return _price_lib().orc_perceptual_rd_scale(p, plane.shape[1], plane.shape[1] if w is None else w,
+                                                plane.shape[0] if h is None else h, bd,
+                                                int(q), int(fmt), int(mono_large), float(ref_act), int(px),
+                                                int(py), int(dim))
