@@ -497,6 +497,16 @@ typedef struct {
     int32_t recon_uv_stride;
     int8_t uv_dc_delta, uv_ac_delta;
     uint8_t uv_qm_level, pad1;
+    /* optional outputs (device; NULL = none): the quantized levels of the reconstruction's
+     * transform blocks (what the bitstream codes; after the trellis with MT_RDO_TRELLIS):
+     * the level of frequency (fx, fy) of the block at (x, y) is stored at sample
+     * (x + fx, y + fy) of an int32 plane of the frame size (coef_stride per row; chroma:
+     * the chroma size, coef_uv_stride); 64x64 blocks fill their low 32x32 frequencies and
+     * leave the rest of their area untouched. */
+    int32_t* coef;
+    int32_t* coef_u;
+    int32_t* coef_v;
+    int32_t coef_stride, coef_uv_stride;
 } mt_rdo_frame;
 
 typedef struct {
@@ -708,6 +718,56 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
  * mode_cover (0..1; 0 = never), the node keeps all 13 modes. */
 int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,
                         const float* hi4, float mode_cover, uint16_t* prune_d, void* stream);
+
+/* ---- the AV1 bitstream of the RDO's decisions (host) -------------------------------- */
+/* One frame as the RDO left it, in host memory.  The coder writes the syntax of AV1's intra
+ * frame (specification 5.9-5.11) with one tile, static frame-initial CDFs
+ * (disable_cdf_update = 1, the CDFs the RDO prices against), TX_MODE_LARGEST (every block's
+ * transform is its size; 4:4:4 64x64 chroma in four 32x32), DCT_DCT luma and the decoder-
+ * derived chroma types, no CDEF / restoration / segmentation, and the reference's luma DC
+ * offset DeltaQYDc = -8 (src/obu.rs:443-451).  The reference writes the same syntax with its
+ * own decisions (src/obu.rs:396-528, src/coder/tile_encode.rs). */
+typedef struct {
+    int32_t width, height;      /* frame size, multiples of 8; one tile: at most 4096 wide and
+                                   4096 * 2304 samples (else MT_ERR_UNSUPPORTED) */
+    uint8_t bd;                 /* 8, 10, 12 */
+    uint8_t chroma;             /* MT_RDO_LUMA_ONLY (monochrome), MT_RDO_CHROMA_420, _444 */
+    uint8_t qidx;               /* base_q_idx (1..255) */
+    uint8_t qm_level;           /* luma QM level (15 flat) */
+    uint8_t uv_qm_level;        /* chroma QM level */
+    int8_t uv_dc_delta, uv_ac_delta;
+    uint8_t filter_intra;       /* 1: the RDO ran with MT_RDO_FILTER_INTRA (enable_filter_intra) */
+    uint8_t lf_level[4];        /* loop_filter_level[0..3] (0..63; luma 0, 0 = no filtering) */
+    uint8_t lf_sharpness;       /* 0..7 */
+    uint8_t cicp[3];            /* colour primaries, transfer, matrix (2, 2, 2: unspecified);
+                                   the identity matrix needs 4:4:4 */
+    uint8_t full_range;
+    uint8_t pad[3];
+    const mt_rdo_node* nodes;   /* [nsb][MT_RDO_NODES], superblocks in raster order */
+    const int32_t* coef_y;      /* mt_rdo_frame's level planes, copied to the host */
+    const int32_t* coef_u;
+    const int32_t* coef_v;
+    int32_t coef_stride, coef_uv_stride;
+} mt_av1_frame;
+
+/* The frame's sequence header OBU and frame OBU (an AV1 temporal unit without its temporal
+ * delimiter).  out may be NULL to ask for the size (*len); a smaller cap is MT_ERR_ARG with
+ * *len set. */
+int mt_av1_encode(const mt_av1_frame* f, uint8_t* out, size_t cap, size_t* len);
+/* The same OBUs as an AVIF still image (ftyp, meta: hdlr, pitm, iloc, iinf, iprp with ispe,
+ * av1C, pixi, colr; mdat), the container of src/isobmff.rs. */
+int mt_avif_encode(const mt_av1_frame* f, uint8_t* out, size_t cap, size_t* len);
+/* The frame's entropy tokens in coding order (the wire format between the decisions and the
+ * range coder, the role of the reference's EntropyTokens, src/odec.rs:96-141): one uint32 per
+ * coded symbol.  Bit 31 clear: a symbol of a stored CDF, offset << 8 | (nsym - 1) << 4 |
+ * symbol (mt_av1_cdf returns the CDF); bit 31 set: a bool, cdf0 << 1 | value with cdf0 =
+ * 32768 * P(0) (literal bits: cdf0 = 16384).  tokens may be NULL to ask for the count. */
+int mt_av1_tokens(const mt_av1_frame* f, uint32_t* tokens, size_t cap, size_t* n);
+/* The range coder alone: the tile data of a token stream (specification 8.2.6 inverted,
+ * with the exit process's trailing one bit). */
+int mt_av1_range_code(const uint32_t* tokens, size_t n, uint8_t* out, size_t cap, size_t* len);
+/* The nsym - 1 cumulative values (AV1 form, 32768 * P(symbol <= k)) of a token's CDF. */
+int mt_av1_cdf(uint32_t offset, int nsym, uint16_t* cdf);
 
 /* ---- decision pricing ---------------------------------------------------------------- */
 /* luma_satd_scale (src/partition_rd.rs:30), k_satd (:54) and proxy_wsse

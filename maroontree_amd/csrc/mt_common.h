@@ -69,6 +69,10 @@ struct ReconFrame {
     // the trellis between quantization and dequantization (the RDO's MT_RDO_TRELLIS): on / off,
     // and the chroma subsampling of its lambda law (0 4:4:4, 2 4:2:0)
     int32_t trellis, tsub;
+    // optional level planes (frequency (fx, fy) of the block at (x, y) at sample (x + fx, y + fy))
+    int32_t* coef;
+    int32_t* coef_uv[2];
+    int32_t coef_stride, coef_uv_stride;
 };
 // The RDO's superblock reconstruction: one wave per superblock b walks the leaves
 // leaves_d[first_d[b] .. + count_d[b]) in coding order, predicting from and writing to the

@@ -993,7 +993,11 @@ int check_frame(const mt_rdo_frame& fr) {
         if (!fr.recon_u || !fr.recon_v || fr.recon_uv_stride < Wc || fr.uv_qm_level > 15 || fr.uv_dc_delta < -64 ||
             fr.uv_dc_delta > 63 || fr.uv_ac_delta < -64 || fr.uv_ac_delta > 63)
             return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad chroma parameters");
+        if ((fr.coef_u || fr.coef_v) && (!fr.coef_u || !fr.coef_v || fr.coef_uv_stride < Wc))
+            return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: chroma level planes need both planes and a stride");
     }
+    if (fr.coef && fr.coef_stride < W)
+        return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: level plane stride below the width");
     return MT_OK;
 }
 
@@ -1047,6 +1051,11 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
         rf[i].qidx = fr.qidx;
         rf[i].bd_qm = fr.qm_level | (bdi << 4);
         rf[i].trellis = (flags & MT_RDO_TRELLIS) ? 1 : 0;
+        rf[i].coef = fr.coef;
+        rf[i].coef_stride = fr.coef_stride;
+        rf[i].coef_uv[0] = fr.coef_u;
+        rf[i].coef_uv[1] = fr.coef_v;
+        rf[i].coef_uv_stride = fr.coef_uv_stride;
         rf[i].tsub = fr.chroma == MT_RDO_CHROMA_444 ? 0 : 2;
         f.chroma = fr.chroma;
         if (fr.chroma) {

@@ -34,9 +34,12 @@ struct ReconTrellis {
 // row-major); out: the reconstructed residual (may alias res); work: LDS of N * (N + 1) ints.
 // With tr.d: the levels and targets go to LDS, the wave runs the trellis over them, and the
 // dequantization reads the trellis's levels.
+// lv_out (optional, global): the levels the dequantization reads, frequency (fx, fy) at
+// lv_out[fy * lv_stride + fx].
 template <int N>
 __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, const BlkParam& p,
-                              int type = MT_TXTP_DCT_DCT, ReconTrellis tr = ReconTrellis{}) {
+                              int type = MT_TXTP_DCT_DCT, ReconTrellis tr = ReconTrellis{},
+                              int32_t* lv_out = nullptr, int lv_stride = 0) {
     using R = TxRules<N, N>;
     constexpr int RS = N + 1;
     const int lane = threadIdx.x & 63;
@@ -80,6 +83,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
                 tr.tg[rc] = tgt;
             } else {
                 work[fy * RS + fx] = dequant(lvl, step, R::dq_shift, p.cf_max);
+                if (lv_out) lv_out[fy * lv_stride + fx] = lvl;
             }
         }
     }
@@ -96,6 +100,7 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
                 int32_t step = rc == 0 ? p.dc : p.ac;
                 if (p.qm_level < 15) step = (step * (int32_t)g_iqm[p.qm_level][p.qm_chroma][R::qmoff + rc] + 16) >> 5;
                 work[fy * RS + fx] = dequant(tr.lv[rc], step, R::dq_shift, p.cf_max);
+                if (lv_out) lv_out[fy * lv_stride + fx] = tr.lv[rc];
             }
         }
         __syncthreads();
@@ -136,7 +141,8 @@ __device__ void wave_chain_sq(const int32_t* res, int32_t* out, int32_t* work, c
 // the trellis runs between quantization and dequantization (its levels and targets may live
 // in vt, which the forward no longer needs by then).
 __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32_t* coef,
-                             const BlkParam& p, ReconTrellis tr = ReconTrellis{}) {
+                             const BlkParam& p, ReconTrellis tr = ReconTrellis{}, int32_t* lv_out = nullptr,
+                             int lv_stride = 0) {
     const int lane = threadIdx.x & 63;
     int32_t* rowbuf = reinterpret_cast<int32_t*>(vt);  // 32 x 64 int32 after the forward
     const bool qm = p.qm_level < 15;
@@ -202,6 +208,7 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
                 tr.tg[rc] = tgt;
             } else {
                 coef[v * 33 + u] = dequant(lvl, step, 2, p.cf_max);
+                if (lv_out) lv_out[v * lv_stride + u] = lvl;
             }
         }
     }
@@ -214,6 +221,7 @@ __device__ void wave_chain64(const int32_t* res, int32_t* out, double* vt, int32
             int32_t step = rc == 0 ? p.dc : p.ac;
             if (qm) step = (step * (int32_t)qmrow[rc] + 16) >> 5;
             coef[(rc & 31) * 33 + (rc >> 5)] = dequant(tr.lv[rc], step, 2, p.cf_max);
+            if (lv_out) lv_out[(rc & 31) * lv_stride + (rc >> 5)] = tr.lv[rc];
         }
         __syncthreads();
     }
@@ -292,12 +300,14 @@ __global__ __launch_bounds__(64) void sb_recon_kernel(const mt_predblk* __restri
         for (int p = lane; p < npx; p += 64)
             s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
         __syncthreads();
+        int32_t* lvo = fr.coef ? fr.coef + (size_t)y0 * fr.coef_stride + x0 : nullptr;
+        const int ls = fr.coef_stride;
         switch (s) {
-            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
-            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
-            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
-            case 32: wave_chain_sq<32>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt); break;
-            default: wave_chain64(s_res, s_res, s_vt, s_work, prm, rt); break;
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt, lvo, ls); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt, lvo, ls); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt, lvo, ls); break;
+            case 32: wave_chain_sq<32>(s_res, s_res, s_work, prm, MT_TXTP_DCT_DCT, rt, lvo, ls); break;
+            default: wave_chain64(s_res, s_res, s_vt, s_work, prm, rt, lvo, ls); break;
         }
         for (int p = lane; p < npx; p += 64) {
             const int v = s_pred[p] + s_res[p];
@@ -379,11 +389,13 @@ __global__ __launch_bounds__(64) void sb_recon_uv_kernel(const mt_predblk* __res
         for (int p = lane; p < npx; p += 64)
             s_res[p] = (int)src.data[(size_t)(y0 + (p >> lw)) * src.stride + x0 + (p & (s - 1))] - s_pred[p];
         __syncthreads();
+        int32_t* lvo = fr.coef_uv[pl] ? fr.coef_uv[pl] + (size_t)y0 * fr.coef_uv_stride + x0 : nullptr;
+        const int ls = fr.coef_uv_stride;
         switch (s) {
-            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, type, rt); break;
-            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, type, rt); break;
-            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, type, rt); break;
-            default: wave_chain_sq<32>(s_res, s_res, s_work, prm, type, rt); break;
+            case 4: wave_chain_sq<4>(s_res, s_res, s_work, prm, type, rt, lvo, ls); break;
+            case 8: wave_chain_sq<8>(s_res, s_res, s_work, prm, type, rt, lvo, ls); break;
+            case 16: wave_chain_sq<16>(s_res, s_res, s_work, prm, type, rt, lvo, ls); break;
+            default: wave_chain_sq<32>(s_res, s_res, s_work, prm, type, rt, lvo, ls); break;
         }
         for (int p = lane; p < npx; p += 64) {
             const int v = s_pred[p] + s_res[p];

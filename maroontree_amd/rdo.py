@@ -32,7 +32,9 @@ class MtRdoFrame(ctypes.Structure):
                 ("src_u", MtPlane), ("src_v", MtPlane), ("recon_u", ctypes.c_void_p),
                 ("recon_v", ctypes.c_void_p), ("recon_uv_stride", ctypes.c_int32),
                 ("uv_dc_delta", ctypes.c_int8), ("uv_ac_delta", ctypes.c_int8),
-                ("uv_qm_level", ctypes.c_uint8), ("pad1", ctypes.c_uint8)]
+                ("uv_qm_level", ctypes.c_uint8), ("pad1", ctypes.c_uint8),
+                ("coef", ctypes.c_void_p), ("coef_u", ctypes.c_void_p), ("coef_v", ctypes.c_void_p),
+                ("coef_stride", ctypes.c_int32), ("coef_uv_stride", ctypes.c_int32)]
 
 
 class MtRdoStats(ctypes.Structure):
@@ -98,9 +100,12 @@ def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale
     src (16-bit cuda tensor (H, W)), bd, qidx and optionally qm_level (15), lam (the
     reference's decision lambda for 4:2:0), prune, recon, nodes; with chroma: chroma
     (CHROMA_420 or CHROMA_444), src_u and src_v (16-bit cuda tensors of the chroma size),
-    optionally uv_dc, uv_ac, uv_qm (default: the reference's laws), recon_u, recon_v.
+    optionally uv_dc, uv_ac, uv_qm (default: the reference's laws), recon_u, recon_v;
+    levels=True adds the reconstruction's level planes (int32 cuda tensors of the plane sizes,
+    zero where no level is stored).
     Returns a list of (recon, nodes, stats) in the same order; with chroma, stats holds
-    recon_uv = (recon_u, recon_v), bits_uv, sse_u and sse_v."""
+    recon_uv = (recon_u, recon_v), bits_uv, sse_u and sse_v; with levels, stats["levels"] =
+    (luma, u, v) (u and v None without chroma)."""
     import torch
     n = len(frames)
     arr = (MtRdoFrame * max(n, 1))()
@@ -160,7 +165,21 @@ def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale
             f.uv_ac_delta = fr.get("uv_ac", ac)
             f.uv_qm_level = fr.get("uv_qm", qm)
             uv = (ru, rv)
-        outs.append((rec, nodes, uv))
+        lv = None
+        if fr.get("levels"):
+            ly = torch.zeros((H, W), dtype=torch.int32, device=dev)
+            f.coef = _dev_ptr(ly).value
+            f.coef_stride = W
+            lu = lvv = None
+            if ch:
+                Hc, Wc = fr["src_u"].shape
+                lu = torch.zeros((Hc, Wc), dtype=torch.int32, device=dev)
+                lvv = torch.zeros((Hc, Wc), dtype=torch.int32, device=dev)
+                f.coef_u = _dev_ptr(lu).value
+                f.coef_v = _dev_ptr(lvv).value
+                f.coef_uv_stride = Wc
+            lv = (ly, lu, lvv)
+        outs.append((rec, nodes, uv, lv))
     st = (MtRdoStats * max(n, 1))()
     _check(lib().mt_intra_rdo_batch(arr, n, flags, st, _stream_ptr(stream)))
     res = []
@@ -169,6 +188,8 @@ def intra_rdo_batch(frames, flags=ANGLE_DELTAS | FILTER_INTRA | MIN4, wsse_scale
              "sbs": st[i].sbs, "steps": st[i].steps}
         if outs[i][2] is not None:
             d.update(recon_uv=outs[i][2], bits_uv=st[i].bits_uv, sse_u=st[i].sse_u, sse_v=st[i].sse_v)
+        if outs[i][3] is not None:
+            d["levels"] = outs[i][3]
         res.append((outs[i][0], outs[i][1], d))
     return res
 

@@ -64,7 +64,7 @@ CDF_TABLES_TABLES = [
     "LO_CTX_OFF", "LO_CTX_OFF_WLH", "LO_CTX_OFF_WGH",
     "KF_Y_MODE_CDF", "UV_MODE_NOCFL_CDF", "UV_MODE_CFL_CDF", "TXTP_INTRA1_TX8", "TXTP_INTRA2_TX16",
     "TXTP_INTRA1_TX4", "ANGLE_DELTA_CDF", "INTRA_FILTER_TAPS", "FILTER_INTRA_CDF",
-    "FILTER_INTRA_MODE_CDF", "PART_SPLIT_CDF", "PART_BL8_CDF",
+    "FILTER_INTRA_MODE_CDF", "PART_SPLIT_CDF", "PART_BL8_CDF", "SKIP_CDF",
 ]
 
 

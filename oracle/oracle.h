@@ -249,6 +249,8 @@ typedef struct {
     int32_t stride, width, height, vis_w, vis_h, pad;
 } orc_plane;
 void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, int nplanes, orc_candres* r);
+void orc_eval_candidate_recon_levels(const orc_cand* c, const orc_plane* planes, int nplanes, orc_candres* r,
+                                     int32_t* recon, int32_t* levels);
 void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,
                     orc_candres* r, int nthreads);
 /* orc_eval_candidate that also returns the reconstruction clamp(pred + inverse), w*h */
@@ -294,6 +296,13 @@ int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint16_t* src_u
                       orc_rdo_stats* stats, int nthreads);
 /* the decision lambda of the reference (src/coder/lossy_state.rs:1769-1790, src/cost.rs:132,
  * :285): mode_lambda_q(dc_q, bd) * mode_lambda_weight(qidx, sub) / K_PART_LAMBDA */
+/* orc_intra_rdo_yuv with the reconstruction's levels (optional, NULL = none): the level of
+ * frequency (fx, fy) of the transform block at (x, y) stored at sample (x + fx, y + fy) of a
+ * W x H int32 plane (chroma: the chroma size); 64x64 blocks fill their low 32x32. */
+int orc_intra_rdo_yuv_coef(const uint16_t* src, int src_stride, const uint16_t* src_u, const uint16_t* src_v,
+                           int uv_stride, const orc_rdo_params* p, const uint16_t* prune, uint16_t* recon,
+                           uint16_t* recon_u, uint16_t* recon_v, orc_rdo_node* nodes, uint8_t* modes,
+                           orc_rdo_stats* stats, int nthreads, int32_t* coef, int32_t* coef_u, int32_t* coef_v);
 float orc_rdo_lambda(int qidx, int bd, int sub);
 float orc_mode_lambda(int qidx, int bd, int sub);
 /* the RDO's top-right / bottom-left availability of an s x s block at (xin, yin) of its SB */

@@ -18,9 +18,10 @@ namespace {
 
 int tx_dim_log2(int v) { return __builtin_ctz((unsigned)v); }
 
-// recon (optional): clamp(pred + inverse residual), w*h row-major.
+// recon (optional): clamp(pred + inverse residual), w*h row-major; levels_out (optional):
+// the levels the inverse reads (after the trellis), ncoef x-major.
 void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candres& r,
-              int32_t* recon = nullptr) {
+              int32_t* recon = nullptr, int32_t* levels_out = nullptr) {
     std::memset(&r, 0, sizeof(r));
     r.sse = -1;
     r.dist = -1;
@@ -99,6 +100,7 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
                         orc_trellis_lambda0(c.qidx, bd, (c.rflags >> 2) & 3), &tc, q.qm_level, 0.0f);
     }
     orc_inv_txfm(s, t, levels.data(), &q, inv.data());
+    if (levels_out) std::memcpy(levels_out, levels.data(), levels.size() * sizeof(int32_t));
     const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
     const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);
     r.sse = orc_sse_recon_vis(pred.data(), inv.data(), src, sp.stride, w, vw, vh, bd);
@@ -153,6 +155,11 @@ extern "C" void orc_eval_candidate(const orc_cand* c, const orc_plane* planes, i
 extern "C" void orc_eval_candidate_recon(const orc_cand* c, const orc_plane* planes, int nplanes,
                                          orc_candres* r, int32_t* recon) {
     eval_one(*c, planes, nplanes, *r, recon);
+}
+
+extern "C" void orc_eval_candidate_recon_levels(const orc_cand* c, const orc_plane* planes, int nplanes,
+                                                orc_candres* r, int32_t* recon, int32_t* levels) {
+    eval_one(*c, planes, nplanes, *r, recon, levels);
 }
 
 extern "C" void orc_eval_batch(const orc_cand* c, int n, const orc_plane* planes, int nplanes,

@@ -267,6 +267,16 @@ struct Frame {
     int chroma = 0, ss = 0, Wc = 0, Hc = 0, uv_dc = 0, uv_ac = 0, uv_qm = 15;
     std::vector<uint16_t> rec_uv[2];  // Wc x Hc
     std::vector<uint8_t> uvmode4;     // (Wc/4) x (Hc/4) uv modes
+    // optional: the reconstruction's levels, frequency (fx, fy) of the transform block at
+    // (x, y) at sample (x + fx, y + fy) (W x H, chroma Wc x Hc)
+    int32_t* coef = nullptr;
+    int32_t* coef_uv[2] = {nullptr, nullptr};
+    void put_levels(int32_t* plane, int pw, int x, int y, int n, const int32_t* lv) const {
+        if (!plane) return;
+        const int nc = n > 32 ? 32 : n;  // 64x64 codes its low 32x32 (x-major, 32 high)
+        for (int fx = 0; fx < nc; ++fx)
+            for (int fy = 0; fy < nc; ++fy) plane[(size_t)(y + fy) * pw + x + fx] = lv[fx * nc + fy];
+    }
 
     int mode_at(int x, int y) const { return mode4[(size_t)(y >> 2) * (W >> 2) + (x >> 2)]; }
     int pa_at(int x, int y) const { return parta8[(size_t)(y >> 3) * (W >> 3) + (x >> 3)]; }
@@ -586,8 +596,9 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         f.put_leaf(lf);
         const orc_cand cd = f.cand(lf.b, lf.id, x0, y0, true);
         orc_candres res;
-        std::vector<int32_t> rc((size_t)lf.b.w * lf.b.h);
-        orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
+        std::vector<int32_t> rc((size_t)lf.b.w * lf.b.h), lvl(1024);
+        orc_eval_candidate_recon_levels(&cd, f.planes, f.nplanes, &res, rc.data(), lvl.data());
+        if (lf.b.w == lf.b.h) f.put_levels(f.coef, f.W, lf.b.x, lf.b.y, lf.b.w, lvl.data());
         for (int yy = 0; yy < lf.b.h; ++yy)
             for (int xx = 0; xx < lf.b.w; ++xx)
                 f.rec[(size_t)(lf.b.y + yy) * f.W + lf.b.x + xx] = (uint16_t)rc[yy * lf.b.w + xx];
@@ -646,8 +657,9 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
             for (int t = 0; t < b.nt; ++t) {
                 const orc_cand cd = f.uv_cand(b, um[i], pl, t, x0, y0, true, pl ? uav[i] : ua[i]);
                 orc_candres res;
-                std::vector<int32_t> rc((size_t)tw * th);
-                orc_eval_candidate_recon(&cd, f.planes, f.nplanes, &res, rc.data());
+                std::vector<int32_t> rc((size_t)tw * th), lvl(1024);
+                orc_eval_candidate_recon_levels(&cd, f.planes, f.nplanes, &res, rc.data(), lvl.data());
+                if (tw == th) f.put_levels(f.coef_uv[pl], f.Wc, cd.x, cd.y, tw, lvl.data());
                 std::vector<uint16_t>& R = f.rec_uv[pl];
                 for (int yy = 0; yy < th; ++yy)
                     for (int xx = 0; xx < tw; ++xx)
@@ -706,6 +718,15 @@ extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint
                                  int uv_stride, const orc_rdo_params* p, const uint16_t* prune, uint16_t* recon,
                                  uint16_t* recon_u, uint16_t* recon_v, orc_rdo_node* nodes, uint8_t* modes,
                                  orc_rdo_stats* stats, int nthreads) {
+    return orc_intra_rdo_yuv_coef(src, src_stride, src_u, src_v, uv_stride, p, prune, recon, recon_u, recon_v, nodes,
+                                  modes, stats, nthreads, nullptr, nullptr, nullptr);
+}
+
+extern "C" int orc_intra_rdo_yuv_coef(const uint16_t* src, int src_stride, const uint16_t* src_u,
+                                      const uint16_t* src_v, int uv_stride, const orc_rdo_params* p,
+                                      const uint16_t* prune, uint16_t* recon, uint16_t* recon_u, uint16_t* recon_v,
+                                      orc_rdo_node* nodes, uint8_t* modes, orc_rdo_stats* stats, int nthreads,
+                                      int32_t* coef, int32_t* coef_u, int32_t* coef_v) {
     if (!p || !src || !recon || !nodes || !stats) return -1;
     if (p->chroma < 0 || p->chroma > 2) return -1;
     if (p->chroma && (!src_u || !src_v || !recon_u || !recon_v || p->uv_qm_level < 0 || p->uv_qm_level > 15))
@@ -718,6 +739,9 @@ extern "C" int orc_intra_rdo_yuv(const uint16_t* src, int src_stride, const uint
     Frame f;
     f.src = src;
     f.sstride = src_stride;
+    f.coef = coef;
+    f.coef_uv[0] = coef_u;
+    f.coef_uv[1] = coef_v;
     f.W = p->width;
     f.H = p->height;
     f.bd = p->bd;

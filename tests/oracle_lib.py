@@ -402,6 +402,8 @@ def _rdo_sigs(L):
     vp = ctypes.c_void_p
     L.orc_intra_rdo_yuv.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.POINTER(OrcRdoParams), vp, vp,
                                     vp, vp, vp, vp, ctypes.POINTER(OrcRdoStats), ctypes.c_int]
+    L.orc_intra_rdo_yuv_coef.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.POINTER(OrcRdoParams), vp,
+                                         vp, vp, vp, vp, vp, ctypes.POINTER(OrcRdoStats), ctypes.c_int, vp, vp, vp]
     for f in ("orc_qm_chroma_level_law", "orc_chroma_dc_delta", "orc_chroma_ac_delta"):
         getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int]
     L.orc_rdo_lambda.argtypes = [ctypes.c_int] * 3
@@ -702,3 +704,39 @@ def perceptual_rd_scale(plane, bd, q, fmt, ref_act, px, py, dim, mono_large=0, w
                                                 plane.shape[0] if h is None else h, bd,
                                                 int(q), int(fmt), int(mono_large), float(ref_act), int(px),
                                                 int(py), int(dim))
+
+
+def intra_rdo_levels(src, bd, qidx, qm_level, flags, lam, chroma=0, src_u=None, src_v=None, uv_dc=0, uv_ac=0,
+                     uv_qm=15, wsse_scale=1.0, nthreads=8):
+    """orc_intra_rdo_yuv_coef: the RDO (luma only, or with chroma) with the reconstruction's level
+    planes.  Returns (recon, recon_u, recon_v, nodes (nsb, 341), stats, (coef_y, coef_u, coef_v));
+    the chroma entries are None for luma only."""
+    L = lib()
+    _rdo_sigs(L)
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    H, W = src.shape
+    nsb = ((W + 63) // 64) * ((H + 63) // 64)
+    p = OrcRdoParams(W, H, bd, qidx, qm_level, flags, lam, wsse_scale, chroma, uv_dc, uv_ac, uv_qm)
+    recon = np.zeros((H, W), np.uint16)
+    cy = np.zeros((H, W), np.int32)
+    nodes = np.zeros((nsb, 341), RDO_NODE_DTYPE)
+    st = OrcRdoStats()
+    ru = rv = cu = cv = su = sv = None
+    Wc = 0
+    if chroma:
+        su = np.ascontiguousarray(src_u, dtype=np.uint16)
+        sv = np.ascontiguousarray(src_v, dtype=np.uint16)
+        Hc, Wc = su.shape
+        ru = np.zeros((Hc, Wc), np.uint16)
+        rv = np.zeros((Hc, Wc), np.uint16)
+        cu = np.zeros((Hc, Wc), np.int32)
+        cv = np.zeros((Hc, Wc), np.int32)
+    d = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    rc = L.orc_intra_rdo_yuv_coef(src.ctypes.data, W, d(su), d(sv), Wc, ctypes.byref(p), None, recon.ctypes.data,
+                                  d(ru), d(rv), nodes.ctypes.data, None, ctypes.byref(st), nthreads,
+                                  cy.ctypes.data, d(cu), d(cv))
+    if rc != 0:
+        raise ValueError("orc_intra_rdo_yuv_coef: invalid parameters")
+    stats = {"bits": st.bits, "sse": st.sse, "candidates": st.candidates, "leaves": st.leaves, "sbs": st.sbs,
+             "bits_uv": st.bits_uv, "sse_u": st.sse_u, "sse_v": st.sse_v}
+    return recon, ru, rv, nodes, stats, (cy, cu, cv)
