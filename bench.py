@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--learned-train", type=int, default=200, help="training frames (1920x1080)")
     ap.add_argument("--learned-eval", type=int, default=50, help="held-out frames per quality")
     ap.add_argument("--learned-epochs", type=int, default=40)
+    ap.add_argument("--learned-model-bits", action="store_true",
+                    help="skip coding the held-out frames into AV1 bytes (BD-rate on the RDO's rate "
+                         "model only)")
     ap.add_argument("--learned-luma-only", action="store_true",
                     help="run the learned-pruning comparison on the luma RDO alone")
     ap.add_argument("--no-stream", action="store_true", help="skip the uneven-stream leg")
@@ -600,6 +603,27 @@ def bench_rdo(args, dev, torch, mt):
             "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
     }
+    # the bitstream: one more call with the level planes, copied to the host and coded there
+    from maroontree_amd import av1
+    fl = dict(frame, levels=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, nodes_l, st_lv = R.intra_rdo_batch([fl], flags, stream=stream)[0]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lv = [a.cpu().numpy() for a in st_lv["levels"]]
+    uvq = R.chroma_laws(qidx, 2)
+    f_av1, keep = av1.frame(R.nodes_to_np(nodes_l), lv, width, height, bd, qidx, chroma=R.CHROMA_420, qm_level=qm,
+                            uv_qm_level=uvq[2], uv_dc=uvq[0], uv_ac=uvq[1], filter_intra=True, cicp=av1.BT601)
+    data = av1.encode_avif(f_av1)
+    t2 = time.perf_counter()
+    print(f"[bench] rdo 4:2:0 AV1 file: {len(data)} bytes, host coder {1e3 * (t2 - t1):.1f} ms",
+          file=sys.stderr, flush=True)
+    r["av1"] = {"output_bytes": len(data), "bits_per_px": round(8 * len(data) / npx, 4),
+                "rdo_with_levels_ms": round(1e3 * (t1 - t0), 2),
+                "host_coder_ms": round(1e3 * (t2 - t1), 2),
+                "note": "level planes copied to the host, tokens per superblock on host threads, range "
+                        "coder, OBUs and AVIF container; not pipelined with the next frame's RDO"}
     # the reconstruction walk: each leaf reads its source and writes its reconstruction, 2 B per
     # sample each, in all three planes
     rec_ms = prof["rdo_recon"][0] / args.rdo_steps
@@ -760,7 +784,8 @@ def bench_learned(args, dev, torch, mt, return_net=False):
     identical results), and the batched + learned form (one network forward over all
     superblocks, pruned luma search); every form codes the chroma too (4:2:0, CfL) unless
     --learned-luma-only.  Speedups, BD-rate (PSNR-Y against the RDO's final bits, luma and
-    chroma) of the pruned search, and the same policy with the untrained network."""
+    chroma, and against the AV1 bytes the frames code to) of the pruned search, and the same
+    policy with the untrained network."""
     from maroontree_amd import prune as PR
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
@@ -824,7 +849,31 @@ def bench_learned(args, dev, torch, mt, return_net=False):
         return (dt, sum(o[2]["bits"] for o in out), R.psnr(sum(o[2]["sse"] for o in out), npx, bd),
                 sum(o[2]["candidates"] for o in out))
 
+    from maroontree_amd import av1
+
+    def coded_bytes(q, model):
+        """Untimed: the AV1 OBU bytes of every frame of the form (levels to the host, host coder)."""
+        frames = frames_at(q)
+        for fr in frames:
+            fr["levels"] = True
+        if model is not None:
+            for fr, m in zip(frames, PR.prune_masks_batch(model, srcs, bd, q, args.topk, args.lo, args.hi,
+                                                          args.mode_cover)):
+                fr["prune"] = m
+        uvq = R.chroma_laws(q, 2)
+        total = 0
+        for k in range(0, len(frames), 10):
+            for _, nodes, st in R.intra_rdo_batch(frames[k:k + 10], flags):
+                lv = [None if a is None else a.cpu().numpy() for a in st["levels"]]
+                f, keep = av1.frame(R.nodes_to_np(nodes), lv, width, height, bd, q,
+                                    chroma=R.CHROMA_420 if chroma else R.LUMA_ONLY,
+                                    qm_level=mt.qm_level_law(q, 2), uv_qm_level=uvq[2], uv_dc=uvq[0],
+                                    uv_ac=uvq[1], filter_intra=True, cicp=av1.BT601)
+                total += len(av1.encode_obus(f))
+        return total
+
     res = {"plain": [], "batched": [], "learned": [], "untrained": []}
+    coded = {"batched": [], "learned": []}
     for q in qs:
         batched(q, None)  # warm-up of this batch shape
         res["plain"].append(summary(*plain(q)))
@@ -833,6 +882,10 @@ def bench_learned(args, dev, torch, mt, return_net=False):
         res["untrained"].append(summary(*batched(q, untrained)))
         log(f"q {q}: plain {res['plain'][-1][0]:.3f} s, batched {res['batched'][-1][0]:.3f} s, "
             f"batched+learned {res['learned'][-1][0]:.3f} s")
+        if not args.learned_model_bits:
+            coded["batched"].append(coded_bytes(q, None))
+            coded["learned"].append(coded_bytes(q, net))
+            log(f"q {q}: AV1 bytes full {coded['batched'][-1]}, learned {coded['learned'][-1]}")
     T = {k: sum(r[0] for r in v) for k, v in res.items()}
     full_pts = res["batched"]
     out = {
@@ -865,6 +918,12 @@ def bench_learned(args, dev, torch, mt, return_net=False):
             "candidate_fraction": round(sum(r[3] for r in res[name]) / sum(r[3] for r in full_pts), 4),
             "points": [[round(r[1], 1), round(r[2], 4)] for r in res[name]],
         }
+    if coded["batched"]:
+        out["learned"]["coded_bytes"] = coded["learned"]
+        out["full_coded_bytes"] = coded["batched"]
+        out["learned"]["bd_rate_pct_coded"] = round(PR.bd_rate(
+            [8.0 * b for b in coded["batched"]], [r[2] for r in full_pts],
+            [8.0 * b for b in coded["learned"]], [r[2] for r in res["learned"]]), 3)
     return (out, net) if return_net else out
 
 
@@ -872,7 +931,7 @@ def bench_plain512(args, dev, torch, mt):
     """The plain-encode configuration: a seeded 512x512 RGB image as full-range BT.601 YCbCr
     4:4:4, quality 60 (qindex 70), the reference's 4:4:4 lambda, chroma quantizer laws and
     QMs, coded by the exact RDO (luma, chroma, CfL) on the GPU and by the oracle's RDO on the
-    host (no entropy coder: the rate is the RDO's final bits)."""
+    host, both entropy coded by the library's host coder into AVIF files."""
     from maroontree_amd import rdo as R
     from maroontree_amd import workload as W
     q, bd = 70, 8
@@ -882,30 +941,56 @@ def bench_plain512(args, dev, torch, mt):
     flags = R.ANGLE_DELTAS | R.FILTER_INTRA | R.MIN4 | R.CFL
     t = lambda a: torch.from_numpy(a.view(np.int16)).to(dev)  # noqa: E731
     fr = {"src": t(y), "src_u": t(u), "src_v": t(v), "bd": bd, "qidx": q, "qm_level": qm, "lam": lam,
-          "chroma": R.CHROMA_444}
+          "chroma": R.CHROMA_444, "levels": True}
     R.intra_rdo_batch([fr], flags)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, _, st = R.intra_rdo_batch([fr], flags)[0]
+    rec, nodes, st = R.intra_rdo_batch([fr], flags)[0]
     torch.cuda.synchronize()
     gpu_ms = (time.perf_counter() - t0) * 1e3
+    # the AV1 bytes: level planes to the host, the host coder (tokens, range coder, OBUs, AVIF)
+    from maroontree_amd import av1
+    t0 = time.perf_counter()
+    lv = [a.cpu().numpy() for a in st["levels"]]
+    nd = R.nodes_to_np(nodes)
+    uvq = R.chroma_laws(q, 0)
+    kw = dict(chroma=R.CHROMA_444, qm_level=qm, uv_qm_level=uvq[2], uv_dc=uvq[0], uv_ac=uvq[1],
+              filter_intra=True, cicp=av1.BT601)
+    f, keep = av1.frame(nd, lv, 512, 512, bd, q, **kw)
+    data = av1.encode_avif(f)
+    coder_ms = (time.perf_counter() - t0) * 1e3
     npx = 512 * 512
     r = {"workload": "seeded 512x512 RGB (gradients, noise, rectangles) -> BT.601 YCbCr 4:4:4, quality 60 "
-                     "(qindex 70), 4:4:4 lambda and chroma laws, exact full RDO (luma, chroma, CfL)",
+                     "(qindex 70), 4:4:4 lambda and chroma laws, exact full RDO (luma, chroma, CfL), AV1 "
+                     "coded with static CDFs into an AVIF file",
+         "output_bytes": len(data), "bits_per_px": round(8 * len(data) / npx, 4),
          "bits_rate_model": round(st["bits"], 1),
          "psnr_y": round(R.psnr(st["sse"], npx, bd), 4), "psnr_u": round(R.psnr(st["sse_u"], npx, bd), 4),
-         "psnr_v": round(R.psnr(st["sse_v"], npx, bd), 4), "gpu_ms": round(gpu_ms, 2)}
+         "psnr_v": round(R.psnr(st["sse_v"], npx, bd), 4), "gpu_ms": round(gpu_ms, 2),
+         "host_coder_ms": round(coder_ms, 2)}
+    try:  # decodability by the AVIF decoder installed with Pillow
+        import io
+        from PIL import Image
+        im = Image.open(io.BytesIO(data))
+        im.load()
+        r["decodes"] = im.size == (512, 512)
+    except Exception as e:  # noqa: BLE001 -- reported, not fatal to the bench
+        r["decodes"] = f"no: {e}"
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
         threads = host_cpu()[0]
-        uvq = R.chroma_laws(q, 0)
         t0 = time.perf_counter()
-        *_, cst = O.intra_rdo_yuv(y, u, v, bd, q, qm, flags, lam, R.CHROMA_444, *uvq, nthreads=threads)
+        _, _, _, cnodes, cst, clv = O.intra_rdo_levels(y, bd, q, qm, flags, lam, chroma=R.CHROMA_444, src_u=u,
+                                                       src_v=v, uv_dc=uvq[0], uv_ac=uvq[1], uv_qm=uvq[2],
+                                                       nthreads=threads)
+        cf, ckeep = av1.frame(cnodes, clv, 512, 512, bd, q, **kw)
+        cdata = av1.encode_avif(cf)
         r["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         r["cpu_cores"] = threads
         r["identical_to_cpu"] = bool(cst["bits"] == st["bits"] and cst["sse"] == st["sse"] and
                                      cst["sse_u"] == st["sse_u"] and cst["sse_v"] == st["sse_v"])
+        r["bytes_identical_to_cpu"] = cdata == data
     return r
 
 
