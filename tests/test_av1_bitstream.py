@@ -198,3 +198,37 @@ def test_refusals():
     f.width = 4160  # 65 superblock columns need two tiles
     with pytest.raises(av1._lib.MtError):
         av1.encode_obus(f)
+
+
+@pytest.mark.parametrize("fmt,w,h,q", [(0, 200, 136, 91), (0, 136, 72, 160), (2, 200, 136, 120), (2, 264, 136, 60)])
+def test_deblocked_reconstruction_equals_the_decoders_loop_filter(fmt, w, h, q):
+    """The frame header signals the reference's loop-filter levels (loop_filter_levels at the
+    shipped tuning); the decoder's in-loop deblocking of the decoded frame equals the oracle's
+    frame_deblock (src/coder/frame_deblock.rs:206) of the oracle's reconstruction over the
+    maps of its decisions: the AV1 loop filter of the specification's 7.14, run by an
+    independent decoder, pins the deblocking restatement end to end."""
+    import lfgen
+    flags = ANG | FI | MIN4 | (CFL if fmt else 0)
+    sub = 0 if fmt == 2 else 2
+    ly, luv, sharp = O.lf_levels(q, sub)
+    assert ly > 0
+    if fmt == 0:
+        y = W.synth_image(13, w, h, 8)
+        rec, ru, rv, nodes, st, lv = O.intra_rdo_levels(y, 8, q, 15, flags, O.rdo_lambda(q, 8, 2))
+        f, keep = av1.frame(nodes, lv, w, h, 8, q, filter_intra=True, lf_levels=(ly, ly, 0, 0),
+                            lf_sharpness=sharp, cicp=av1.BT601)
+    else:
+        y, u, v = W.synth_yuv(13, w, h, 8, 0)
+        dc, ac, qm = O.chroma_laws(q, 0)
+        rec, ru, rv, nodes, st, lv = O.intra_rdo_levels(y, 8, q, 15, flags, O.rdo_lambda(q, 8, 0), chroma=2,
+                                                        src_u=u, src_v=v, uv_dc=dc, uv_ac=ac, uv_qm=qm)
+        f, keep = av1.frame(nodes, lv, w, h, 8, q, chroma=2, uv_qm_level=qm, uv_dc=dc, uv_ac=ac, filter_intra=True,
+                            lf_levels=(ly, ly, luv, luv), lf_sharpness=sharp, cicp=av1.SRGB_IDENTITY)
+    m = lfgen.maps_from_leaves(lfgen.leaves_from_nodes(nodes, w, h), w, h)
+    want = [rec.copy(), None if ru is None else ru.copy(), None if rv is None else rv.copy()]
+    O.frame_deblock(want[0], want[1], want[2], w, h, m, 0, 0, sharp, 8, ly, luv, luv)
+    assert (want[0] != rec).any()  # the filter changed something
+    dec = _decode(av1.encode_avif(f))
+    assert (dec[..., 1] == want[0]).all(), f"Y: {int((dec[..., 1] != want[0]).sum())} samples differ"
+    if fmt:
+        assert (dec[..., 2] == want[1]).all() and (dec[..., 0] == want[2]).all()
