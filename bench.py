@@ -574,6 +574,9 @@ def bench_rdo(args, dev, torch, mt):
 
     ms, st, prof = timed(frame, flags, args.rdo_steps, prof=True)
     nd = R.nodes_to_np(frame["nodes"])
+    # the same RDO with its wavefront steps captured into one HIP graph per call
+    ms_g, st_g, _ = timed(frame, flags | R.GRAPH, args.rdo_steps)
+    print(f"[bench] rdo 4:2:0 as one graph: {ms_g:.2f} ms/frame", file=sys.stderr, flush=True)
     cfl_blocks, chroma_blocks = int((nd["uv_mode"] == 13).sum()), int((nd["uv_mode"] != 255).sum())
     ms_l, st_l, _ = timed(luma, flags & ~R.CFL, args.rdo_steps) if not args.rdo_skip_luma else (None, None, None)
     npx = width * height
@@ -602,6 +605,11 @@ def bench_rdo(args, dev, torch, mt):
             "ms_per_frame": round(ms_l, 3), "mpx_per_s": round(npx / (ms_l / 1000.0) / 1e6, 2),
             "candidates_per_s": round(st_l["candidates"] / (ms_l / 1000.0), 1)},
         "stage_ms_per_frame": {k: round(v[0] / args.rdo_steps, 3) for k, v in prof.items() if v[1]},
+        "graph": {"ms_per_frame": round(ms_g, 3), "mpx_per_s": round(npx / (ms_g / 1000.0) / 1e6, 2),
+                  "identical": bool(st_g["bits"] == st["bits"] and st_g["sse"] == st["sse"] and
+                                    st_g["sse_u"] == st["sse_u"] and st_g["sse_v"] == st["sse_v"]),
+                  "note": "MT_RDO_GRAPH: the steps' launches captured into one HIP graph per call, "
+                          "capture and instantiation included"},
     }
     # the bitstream: one more call with the level planes, copied to the host and coded there
     from maroontree_amd import av1

@@ -468,7 +468,11 @@ int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* pl
  * candidate's rate context, lambda0 = mt_trellis_lambda0(qidx, bd, sub) with sub 0 for
  * 4:4:4 frames and 2 for 4:2:0 and luma-only frames (their decision lambda's). */
 enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4, MT_RDO_CFL = 8,
-       MT_RDO_TRELLIS = 32 };
+       MT_RDO_TRELLIS = 32, MT_RDO_GRAPH = 64 };
+/* MT_RDO_GRAPH: the wavefront steps' launches (candidate generation, evaluation, argmin,
+ * decisions, reconstruction of every step) are captured from the stream into one HIP graph,
+ * instantiated and launched as a whole; the results are the same.  No counterpart in the
+ * reference (its wavefront runs host threads). */
 enum { MT_RDO_LUMA_ONLY = 0, MT_RDO_CHROMA_420 = 1, MT_RDO_CHROMA_444 = 2 };
 #define MT_RDO_NODES 341          /* quadtree nodes of one SB: 1 + 4 + 16 + 64 + 256 */
 #define MT_RDO_ALL_MODES 0x3FFF   /* prune mask bits 0-12: intra modes; bit 13: filter intra */

@@ -52,6 +52,17 @@ int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* level
 // mt_distortion on a prepared plane set.
 int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, const int32_t* pred_d,
                       const int32_t* resid_d, mt_distres* out_d, hipStream_t s);
+// Grow the (device, stream) workspaces of the candidate evaluation and of every stage it
+// launches (prediction, distortion and rate class lists, the chain's order, the trellis's
+// classes) to n candidates and pool_elems pool elements, so that launches up to that size
+// allocate and free nothing (what a stream capture needs: a buffer freed while being
+// captured would be left in the graph).
+int eval_reserve(size_t n, uint64_t pool_elems, hipStream_t s, bool trellis);
+int pred_reserve(size_t n, hipStream_t s);
+int dist_reserve(size_t n, hipStream_t s);
+int rate_reserve(size_t n, hipStream_t s);
+int trellis_reserve(size_t n, hipStream_t s);
+int chain_reserve(size_t n, hipStream_t s);
 // mt_eval_candidates on a prepared plane set.
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
                 mt_candres* res_d, hipStream_t s, bool trellis = false);

@@ -459,6 +459,25 @@ std::map<mtlib::StreamKey, DistWork> g_dist_work;
 }  // namespace
 
 namespace mtlib {
+int dist_grow(DistWork& w, size_t n) {
+    if (w.cap < n) {
+        if (w.lists) MT_HIP_TRY(hipFree(w.lists));
+        w.lists = nullptr;
+        const size_t cap = n < 65536 ? 65536 : n;
+        MT_HIP_TRY(hipMalloc(&w.lists, (kSmallShapes + 1) * cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, (kSmallShapes + 1) * sizeof(uint32_t)));
+    return MT_OK;
+}
+int dist_reserve(size_t n, hipStream_t s) {
+    int dev = 0, cus = 0;
+    int rc = mtlib::device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_dist_mu);
+    return dist_grow(g_dist_work[mtlib::stream_key(dev, s)], n);
+}
+
 int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, const int32_t* pred_d,
                       const int32_t* resid_d, mt_distres* out_d, hipStream_t s) {
     if (n == 0) return MT_OK;
@@ -467,14 +486,7 @@ int distortion_launch(const mt_distblk* blocks_d, int n, const PlaneSet& ps, con
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_dist_mu);
     DistWork& w = g_dist_work[mtlib::stream_key(dev, s)];
-    if (w.cap < (size_t)n) {
-        if (w.lists) MT_HIP_TRY(hipFree(w.lists));
-        w.lists = nullptr;
-        const size_t cap = n < 65536 ? 65536 : (size_t)n;
-        MT_HIP_TRY(hipMalloc(&w.lists, (kSmallShapes + 1) * cap * sizeof(uint32_t)));
-        w.cap = cap;
-    }
-    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, (kSmallShapes + 1) * sizeof(uint32_t)));
+    if ((rc = dist_grow(w, (size_t)n))) return rc;
     const int32_t* resid = resid_d ? resid_d : pred_d;
     mtlib::ProfScope prof(mtlib::kProfDist, s);
     hipLaunchKernelGGL(mtlib::zero_counts_kernel<kSmallShapes + 1>, dim3(1), dim3(64), 0, s, w.cnt);

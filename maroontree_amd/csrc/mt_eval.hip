@@ -238,6 +238,26 @@ int ensure(EvalWork& w, size_t n, size_t pool) {
 }  // namespace
 
 namespace mtlib {
+int eval_reserve(size_t n, uint64_t pool_elems, hipStream_t s, bool trellis) {
+    if (n == 0) return MT_OK;
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_eval_mu);
+        EvalWork& w = g_work[mtlib::stream_key(dev, s)];
+        if ((rc = ensure(w, n, (size_t)pool_elems))) return rc;
+        if (trellis && w.cap_targets < w.cap_pool) {
+            if ((rc = grow(w.targets, w.cap_pool))) return rc;
+            w.cap_targets = w.cap_pool;
+        }
+    }
+    if ((rc = pred_reserve(n, s)) || (rc = dist_reserve(n, s)) || (rc = rate_reserve(n, s)) ||
+        (rc = chain_reserve(n, s)))
+        return rc;
+    return trellis ? trellis_reserve(n, s) : MT_OK;
+}
+
 int eval_launch(const mt_cand* cands_d, int n, const PlaneSet& ps, uint64_t pool_elems,
                 mt_candres* res_d, hipStream_t s, bool trellis) {
     if (n == 0) return MT_OK;

@@ -412,6 +412,27 @@ std::map<mtlib::StreamKey, PredWork> g_pred_work;
 }  // namespace
 
 namespace mtlib {
+// The class lists of (device, stream) for n blocks (grown, never shrunk).
+int pred_grow(PredWork& w, size_t n) {
+    constexpr int K = kSmallShapes + 1;
+    if (w.cap < n) {
+        if (w.lists) MT_HIP_TRY(hipFree(w.lists));
+        w.lists = nullptr;
+        const size_t cap = n < 65536 ? 65536 : n;
+        MT_HIP_TRY(hipMalloc(&w.lists, K * cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, K * sizeof(uint32_t)));
+    return MT_OK;
+}
+int pred_reserve(size_t n, hipStream_t s) {
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_pred_mu);
+    return pred_grow(g_pred_work[mtlib::stream_key(dev, s)], n);
+}
+
 // mt_intra_pred, optionally also writing src - pred (int16) into resid_d at each block's
 // out_offset, the source plane index taken from each descriptor's pad byte.
 int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int32_t* pred_d,
@@ -423,14 +444,7 @@ int intra_pred_launch(const mt_predblk* blocks_d, int n, const PlaneSet& ps, int
     std::lock_guard<std::mutex> lk(g_pred_mu);
     PredWork& w = g_pred_work[mtlib::stream_key(dev, s)];
     constexpr int K = kSmallShapes + 1;
-    if (w.cap < (size_t)n) {
-        if (w.lists) MT_HIP_TRY(hipFree(w.lists));
-        w.lists = nullptr;
-        const size_t cap = n < 65536 ? 65536 : (size_t)n;
-        MT_HIP_TRY(hipMalloc(&w.lists, K * cap * sizeof(uint32_t)));
-        w.cap = cap;
-    }
-    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, K * sizeof(uint32_t)));
+    if ((rc = pred_grow(w, (size_t)n))) return rc;
     ProfScope prof(kProfPred, s);
     hipLaunchKernelGGL(zero_counts_kernel<K>, dim3(1), dim3(64), 0, s, w.cnt);
     hipLaunchKernelGGL(pred_classify_kernel, dim3(std::max(1, std::min((n + 255) / 256, cus * 4))),

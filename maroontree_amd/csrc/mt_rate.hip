@@ -731,6 +731,25 @@ std::map<mtlib::StreamKey, RateWork> g_rate_work;
 }  // namespace
 
 namespace mtlib {
+int rate_grow(RateWork& w, size_t n) {
+    if (w.cap < n) {
+        if (w.list) MT_HIP_TRY(hipFree(w.list));
+        w.list = nullptr;
+        const size_t cap = n < 65536 ? 65536 : n;
+        MT_HIP_TRY(hipMalloc(&w.list, cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, sizeof(uint32_t)));
+    return MT_OK;
+}
+int rate_reserve(size_t n, hipStream_t s) {
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_rate_mu);
+    return rate_grow(g_rate_work[mtlib::stream_key(dev, s)], n);
+}
+
 // mt_block_rate for the candidate pipeline: the small blocks one lane each with the chain's
 // nonzero counts steering the eob search, the large blocks one wave each.
 int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* levels_d, float* bits_d,
@@ -741,14 +760,7 @@ int rate_launch_with_nnz(const mt_rateblk* blocks_d, int n, const int32_t* level
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_rate_mu);
     RateWork& w = g_rate_work[mtlib::stream_key(dev, s)];
-    if (w.cap < (size_t)n) {
-        if (w.list) MT_HIP_TRY(hipFree(w.list));
-        w.list = nullptr;
-        const size_t cap = n < 65536 ? 65536 : (size_t)n;
-        MT_HIP_TRY(hipMalloc(&w.list, cap * sizeof(uint32_t)));
-        w.cap = cap;
-    }
-    if (!w.cnt) MT_HIP_TRY(hipMalloc(&w.cnt, sizeof(uint32_t)));
+    if ((rc = rate_grow(w, (size_t)n))) return rc;
     ProfScope prof(kProfRate, s);
     hipLaunchKernelGGL(zero_counts_kernel<1>, dim3(1), dim3(64), 0, s, w.cnt);
     hipLaunchKernelGGL(rate_classify_kernel, dim3(std::max(1, std::min((n + 255) / 256, cus * 4))),
@@ -809,6 +821,26 @@ void launch_trellis_class(const mt_trellisblk* blocks_d, const TrellisWork& w, i
 }  // namespace
 
 namespace mtlib {
+int trellis_grow(TrellisWork& w, size_t n) {
+    if (w.cap < n) {
+        if (w.order) MT_HIP_TRY(hipFree(w.order));
+        w.order = nullptr;
+        w.cap = 0;
+        const size_t cap = n < 65536 ? 65536 : n;
+        MT_HIP_TRY(hipMalloc(&w.order, kTrClasses * cap * sizeof(uint32_t)));
+        w.cap = cap;
+    }
+    if (!w.counts) MT_HIP_TRY(hipMalloc(&w.counts, kTrClasses * sizeof(uint32_t)));
+    return MT_OK;
+}
+int trellis_reserve(size_t n, hipStream_t s) {
+    int dev = 0, cus = 0;
+    int rc = device_ready(&dev, &cus);
+    if (rc != MT_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_trellis_mu);
+    return trellis_grow(g_trellis_work[mtlib::stream_key(dev, s)], n);
+}
+
 // Sorts the blocks into lane-group classes (lists per device and stream, grown on demand),
 // then one persistent launch per class.
 int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, const float* targets_d,
@@ -819,15 +851,7 @@ int trellis_launch(const mt_trellisblk* blocks_d, int n, int32_t* levels_d, cons
     if (rc != MT_OK) return rc;
     std::lock_guard<std::mutex> lk(g_trellis_mu);
     TrellisWork& w = g_trellis_work[mtlib::stream_key(dev, s)];
-    if (w.cap < (size_t)n) {
-        if (w.order) MT_HIP_TRY(hipFree(w.order));
-        w.order = nullptr;
-        w.cap = 0;
-        const size_t cap = n < 65536 ? 65536 : (size_t)n;
-        MT_HIP_TRY(hipMalloc(&w.order, kTrClasses * cap * sizeof(uint32_t)));
-        w.cap = cap;
-    }
-    if (!w.counts) MT_HIP_TRY(hipMalloc(&w.counts, kTrClasses * sizeof(uint32_t)));
+    if ((rc = trellis_grow(w, (size_t)n))) return rc;
     ProfScope prof(kProfTrellis, s);
     hipLaunchKernelGGL(zero_counts_kernel<kTrClasses>, dim3(1), dim3(64), 0, s, w.counts);
     hipLaunchKernelGGL(trellis_sort_kernel, dim3((n + 4095) / 4096), dim3(256), 0, s, blocks_d, n, w.counts,

@@ -934,6 +934,7 @@ __global__ void __launch_bounds__(64) rdo_final_sum_kernel(const FrameArgs* __re
 // that share them.
 struct RdoWork {
     std::mutex mu;
+    hipStream_t capture = nullptr;  // MT_RDO_GRAPH: the stream the steps are captured from
     size_t cap_sb = 0, cap_step_sb = 0, cap_cand = 0, cap_final = 0, cap_map = 0, cap_frames = 0;
     uint32_t *sb_uv_cands = nullptr, *sb_uv_pool = nullptr, *sb_uv_dec = nullptr;
     double* sb_bits_uv = nullptr;
@@ -1228,6 +1229,30 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
     }
     MT_HIP_TRY(hipMemcpyAsync(w.tasks, tasks.data(), tasks.size() * sizeof(SbTask), hipMemcpyHostToDevice, s));
 
+    // The steps' launches: every size is known now, so with MT_RDO_GRAPH they are captured
+    // into one graph (relaxed mode: a workspace that grows allocates outside the graph) and
+    // launched at once.  Fills are kernels, never memset nodes.
+    // The capture runs on a stream of the workspace (the caller's may be the null stream,
+    // which cannot be captured); it only records, and the graph runs on the caller's stream.
+    const bool as_graph = (flags & MT_RDO_GRAPH) != 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const hipStream_t caller = s;
+    if (as_graph) {
+        if (!w.capture) MT_HIP_TRY(hipStreamCreateWithFlags(&w.capture, hipStreamNonBlocking));
+        // every workspace the steps use, at its largest step, before the capture: a buffer
+        // that grew during it would free memory the graph's earlier nodes still point at
+        uint64_t nmax = 0, pmax = 0;
+        for (int k = 0; k <= max_steps; ++k) {
+            nmax = std::max(nmax, eval_n[k]);
+            pmax = std::max(pmax, eval_pool[k]);
+        }
+        if ((rc = mtlib::eval_reserve((size_t)nmax, pmax, w.capture, (flags & MT_RDO_TRELLIS) != 0))) return rc;
+        MT_HIP_TRY(hipStreamBeginCapture(w.capture, hipStreamCaptureModeRelaxed));
+    }
+    rc = [&]() -> int {
+    const hipStream_t s = as_graph ? w.capture : caller;
+    int rc = MT_OK;
     for (int k = 0; k <= max_steps; ++k) {
         const int nk = step_nk(k), pk = any_chroma ? step_nk(k - 1) : 0;
         if (nk == 0 && pk == 0) continue;
@@ -1276,6 +1301,22 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             if ((rc = mtlib::sb_recon_uv_launch(w.uvdesc, w.uvfirst, w.uvcount, pk, w.rframes, s))) return rc;
         }
     }
+    return rc;
+    }();
+    if (as_graph) {
+        const hipError_t e = hipStreamEndCapture(w.capture, &graph);
+        if (rc == MT_OK && e != hipSuccess)
+            rc = mtlib::set_error(MT_ERR_DEVICE, std::string("mt_intra_rdo: graph capture: ") + hipGetErrorString(e));
+        if (rc == MT_OK && hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = mtlib::set_error(MT_ERR_DEVICE, "mt_intra_rdo: graph instantiation failed");
+        if (rc == MT_OK && hipGraphLaunch(gexec, s) != hipSuccess)
+            rc = mtlib::set_error(MT_ERR_DEVICE, "mt_intra_rdo: graph launch failed");
+        if (rc == MT_OK && hipStreamSynchronize(s) != hipSuccess)
+            rc = mtlib::set_error(MT_ERR_DEVICE, "mt_intra_rdo: graph execution failed");
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    if (rc) return rc;
 
     // final accounting with the final contexts
     std::vector<uint32_t> nleaf(nall), lpool(nall), lbase(nall), pbase(nall), uvc(nall), uvp(nall), uvd(nall);
@@ -1411,7 +1452,7 @@ extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
 extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
                                   void* stream) {
     if (!frames || !stats || nframes < 0) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
-    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL | MT_RDO_TRELLIS))
+    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL | MT_RDO_TRELLIS | MT_RDO_GRAPH))
         return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad flags");
     int rc = MT_OK;
     for (int i = 0; i < nframes; ++i)

@@ -1095,6 +1095,8 @@ int current_device(int* dev) {
 namespace mtlib {
 ProfScope::ProfScope(int slot_, hipStream_t s_) : slot(slot_), s(s_) {
     if (!g_prof.on || slot < 0) return;  // a negative slot records nothing
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;  // in a graph
     a = prof_event();
     b = prof_event();
     if (a) (void)hipEventRecord(a, s);
@@ -1415,6 +1417,14 @@ extern "C" int mt_inv_txfm(int tx_size, int tx_type, int n, const int32_t* level
     if (n > 0 && !out_d) return set_err(MT_ERR_ARG, "null output");
     return parity_launch(MODE_INV, tx_size, tx_type, n, levels_d, quants_d, nullptr, nullptr,
                          out_d, stream);
+}
+
+int mtlib::chain_reserve(size_t n, hipStream_t stream) {
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return ensure_order(g_chain[mtlib::stream_key(dev, stream)], n);
 }
 
 extern "C" int mt_tx_chain(const mt_txblk* blocks_d, int n, const int16_t* pool_d, mt_txres* res_d,

@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 from ._lib import MtPlane, _check, _dev_ptr, _stream_ptr, init, lib
 
-ANGLE_DELTAS, FILTER_INTRA, MIN4, CFL, TRELLIS = 1, 2, 4, 8, 32
+ANGLE_DELTAS, FILTER_INTRA, MIN4, CFL, TRELLIS, GRAPH = 1, 2, 4, 8, 32, 64
 NONE, SPLIT, UNCODED, HORZ, VERT = 0, 1, 2, 3, 4  # node decisions (the split field; HORZ / VERT: oracle only)
 LUMA_ONLY, CHROMA_420, CHROMA_444 = 0, 1, 2
 NODES = 341
