@@ -99,10 +99,10 @@ def main():
     ap.add_argument("--stream-max-side", type=int, default=8192)
     ap.add_argument("--topk", default="8,10,12,13,13",
                     help="intra modes kept per superblock, per node level 64,32,16,8,4")
-    ap.add_argument("--mode-cover", type=float, default=0.0,
+    ap.add_argument("--mode-cover", type=float, default=0.3,
                     help="keep all modes where the kept ones cover the SB's modes with a lower "
                          "estimated probability")
-    ap.add_argument("--lo", default="0,0.01,0.005,0",
+    ap.add_argument("--lo", default="0,0.03,0.02,0",
                     help="NO_SPLIT below this split probability, per node level 64,32,16,8")
     ap.add_argument("--hi", default="1,1,1,1",
                     help="FORCE_SPLIT above this split probability, per node level 64,32,16,8")
@@ -902,7 +902,8 @@ def bench_learned(args, dev, torch, mt, return_net=False):
                     "luma RDO (partitions 64..4, 13 modes + angle deltas + filter intra); qindex 140/91/53/17; "
                     + ("every form codes the chroma (13 uv modes + CfL), the network prunes the luma search"
                        if chroma else "luma only"),
-        "network": "conv3x3s2 1-16-32-64, pool, fc 257-64-98; 46178 params; bf16 MFMA, fp32 accumulation",
+        "network": "conv3x3s2 1-16-32-64, pool, fc 257-32-863 (85 nodes x 10 partition-type logits + 13 mode "
+                   "scores); 60031 params; bf16 MFMA, fp32 accumulation",
         "train_superblocks": n_train_sb,
         "label_seconds": round(t_data, 2),
         "train_seconds": round(t_train, 2),

@@ -679,13 +679,18 @@ int mt_rdo_deblock_maps(const void* nodes_d, int width, int height, uint8_t* map
  * src/coder/partition_search.rs:187, split_breakout_k :135-160, rank_luma_modes
  * src/coder/superblock.rs:102): 64x64 luma superblock + qindex -> conv 3x3 s2 (1->16) ->
  * conv 3x3 s2 (16->32) -> conv 3x3 s2 (32->64), ReLU each -> 4x4 average pool ++ qindex/255
- * -> fc 257->64 ReLU -> fc 64->98: outputs 0..84 the split logits of quadtree nodes 0..84
- * (64x64 .. 8x8), 85..97 the logits of intra modes 0..12 being used by the superblock's
- * leaves.  Parameters: one flat fp32 array (weights [K][N] row-major then biases, layer by
- * layer: conv1 9x16 + 16, conv2 144x32 + 32 (k = tap * 16 + channel), conv3 288x64 + 64,
- * fc1 257x64 + 64, fc2 64x98 + 98); every product runs in bf16 with fp32 accumulation. */
-#define MT_PNET_PARAMS 46178
-#define MT_PNET_OUTPUTS 98
+ * -> fc 257->32 ReLU -> fc 32->863: outputs 10 n .. 10 n + 9 the logits of the 10 AV1
+ * partition types (NONE, HORZ, VERT, SPLIT, HORZ_A, HORZ_B, VERT_A, VERT_B, HORZ_4, VERT_4)
+ * of quadtree node n = 0..84 (64x64 .. 8x8), outputs 850..862 the scores of intra modes 0..12
+ * being used by the superblock's leaves (the masks keep the top-k).  Parameters: one flat
+ * fp32 array (weights [K][N] row-major then biases, layer by layer: conv1 9x16 + 16, conv2
+ * 144x32 + 32 (k = tap * 16 + channel), conv3 288x64 + 64, fc1 257x32 + 32, fc2 32x863 + 863);
+ * every product runs in bf16 with fp32 accumulation.  Labels and loss terms are [batch][98]:
+ * column n < 85 the partition type of node n (its softmax cross-entropy), 85 + j mode j
+ * (binary cross-entropy). */
+#define MT_PNET_PARAMS 60031
+#define MT_PNET_OUTPUTS 863
+#define MT_PNET_LABELS 98
 
 /* C[m][n] = sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn], A and B bf16 (device), C fp32 [M][N]:
  * the MFMA GEMM every layer uses. */
@@ -697,15 +702,16 @@ size_t mt_pnet_workspace_bytes(int batch);
  * replicated past the frame). */
 int mt_pnet_extract(const mt_plane* plane, uint16_t* pix_d, void* stream);
 /* Forward of `batch` superblocks: sample b = pix_d[idx_d[b]] (idx_d NULL: b) with qindex
- * qidx_d[idx_d[b]]; logits_d (optional): [batch][98].  Keeps the activations in work_d
+ * qidx_d[idx_d[b]]; logits_d (optional): [batch][863].  Keeps the activations in work_d
  * for mt_pnet_backward. */
 int mt_pnet_forward(const float* params_d, const uint16_t* pix_d, const int32_t* idx_d,
                     const uint8_t* qidx_d, int bd, int batch, void* work_d, float* logits_d,
                     void* stream);
-/* Backward of the last forward for masked binary cross-entropy with logits against
- * targets_d / mask_d [batch][98]; the split terms (0..84) are weighted by norm_split, the
- * mode terms by norm_mode.  grads_d: MT_PNET_PARAMS fp32 (overwritten); loss_terms_d
- * (optional): the [batch][98] weighted loss terms. */
+/* Backward of the last forward against targets_d / mask_d [batch][98]: per node n < 85 the
+ * masked softmax cross-entropy of its 10 partition logits with the partition type targets[n]
+ * (0..9), weighted by norm_split; per mode j the masked binary cross-entropy of its score with
+ * targets[85 + j], weighted by norm_mode.  grads_d: MT_PNET_PARAMS fp32 (overwritten);
+ * loss_terms_d (optional): the [batch][98] weighted loss terms. */
 int mt_pnet_backward(const float* params_d, const float* targets_d, const float* mask_d, int batch,
                      float norm_split, float norm_mode, void* work_d, float* grads_d,
                      float* loss_terms_d, void* stream);
@@ -714,10 +720,11 @@ int mt_pnet_adam(float* params_d, const float* grads_d, float* m_d, float* v_d, 
                  float beta1, float beta2, float eps, void* stream);
 /* mt_intra_rdo prune masks [batch][MT_RDO_NODES] from the logits: on a node of level L
  * (0: 64x64 .. 4: 4x4) the superblock's topk5[L] best-scored intra modes (DC_PRED and
- * filter intra always kept), and for nodes 0..84 MT_RDO_NO_SPLIT when sigmoid(split logit)
- * < lo4[L], MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays).  Between the thresholds a node
+ * filter intra always kept), and for nodes 0..84 MT_RDO_NO_SPLIT when the probability that
+ * the node is partitioned, 1 - softmax(partition logits)[NONE], is < lo4[L],
+ * MT_RDO_FORCE_SPLIT when > hi4[L] (host arrays).  Between the thresholds a node
  * keeps both choices: the full search there.  Modes fall back the same way: when the
- * product over the dropped modes j of (1 - sigmoid(mode logit j)) -- the estimated
+ * product over the dropped modes j of (1 - sigmoid(mode score j)) -- the estimated
  * probability that the kept modes cover every mode the superblock uses -- is below
  * mode_cover (0..1; 0 = never), the node keeps all 13 modes. */
 int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, const float* lo4,

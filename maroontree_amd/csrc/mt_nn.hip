@@ -5,10 +5,11 @@
 // hand (proxy_mode_beam src/coder/partition_search.rs:187, split_breakout_k :135-160,
 // rank_luma_modes src/coder/superblock.rs:102).
 //
-// Network (46,178 parameters, fp32 master copy, bf16 in every product):
+// Network (60,031 parameters, fp32 master copy, bf16 in every product):
 //   x[64x64] -> conv 3x3 s2 (1->16) ReLU -> conv 3x3 s2 (16->32) ReLU -> conv 3x3 s2 (32->64)
-//   ReLU -> 4x4 average pool [2x2x64] ++ qindex/255 -> fc 257->64 ReLU -> fc 64->98
-//   (outputs 0..84: split logits of nodes 0..84; 85..97: intra mode logits 0..12).
+//   ReLU -> 4x4 average pool [2x2x64] ++ qindex/255 -> fc 257->32 ReLU -> fc 32->863
+//   (outputs 10 n + k: logit of AV1 partition type k of node n = 0..84; 850 + j: the score of
+//   intra mode j = 0..12).
 // Every convolution and fully connected layer, forward and backward, is one call of the
 // MFMA GEMM below (im2col / col2im around the convolutions); activations are NHWC.
 #include <hip/hip_runtime.h>
@@ -87,7 +88,8 @@ __global__ void __launch_bounds__(256) splitk_sum_kernel(const float* __restrict
 }
 
 // ---------------------------------------------------------------------------- layers
-constexpr int kC1 = 16, kC2 = 32, kC3 = 64, kF = 257, kH = 64, kOut = MT_PNET_OUTPUTS;
+constexpr int kC1 = 16, kC2 = 32, kC3 = 64, kF = 257, kH = 32, kOut = MT_PNET_OUTPUTS;
+constexpr int kNodes = 85, kParts = 10, kModeOff = kNodes * kParts, kLabels = MT_PNET_LABELS;
 // parameter offsets in the flat fp32 array (weights [K][N], then biases)
 constexpr int kW1 = 0, kB1 = kW1 + 9 * kC1;
 constexpr int kW2 = kB1 + kC1, kB2 = kW2 + 9 * kC1 * kC2;
@@ -210,25 +212,43 @@ __global__ void __launch_bounds__(256) relu_back_kernel(const float* __restrict_
     }
 }
 
-// Masked binary cross-entropy with logits: dlogit = mask * (sigmoid(z) - t) / norm; the loss
-// terms go to loss_terms[i] (summed on the host).  norm: the batch's mask sum per head
-// (split head 0..84 and mode head 85..97 normalised separately).
-__global__ void __launch_bounds__(256) bce_kernel(const float* __restrict__ z, const float* __restrict__ t,
-                                                  const float* __restrict__ m, int B, float norm_split, float norm_mode,
-                                                  uint16_t* __restrict__ dz_bf, float* __restrict__ dz,
-                                                  float* __restrict__ loss_terms) {
-    const long n = (long)B * kOut;
+// The loss of one label column, one thread each: column n < 85 the masked softmax
+// cross-entropy of node n's 10 partition logits with its type t (dlogit_k = w (p_k - [k = t])),
+// column 85 + j the masked binary cross-entropy of mode score j (dlogit = w (sigmoid(z) - t)).
+// w = mask * norm of the column's head; the weighted loss terms go to loss_terms (summed on
+// the host).
+__global__ void __launch_bounds__(256) pnet_loss_kernel(const float* __restrict__ z, const float* __restrict__ t,
+                                                        const float* __restrict__ m, int B, float norm_split,
+                                                        float norm_mode, uint16_t* __restrict__ dz_bf,
+                                                        float* __restrict__ dz, float* __restrict__ loss_terms) {
+    const long n = (long)B * kLabels;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int o = (int)(i % kOut);
-        const float w = m[i] * (o < 85 ? norm_split : norm_mode);
-        const float zz = z[i];
-        const float p = 1.0f / (1.0f + expf(-zz));
-        const float g = w * (p - t[i]);
-        dz[i] = g;
-        dz_bf[i] = f2bf(g);
-        // log(1 + e^z) - t z, stable
-        const float l = fmaxf(zz, 0.0f) - zz * t[i] + log1pf(expf(-fabsf(zz)));
-        loss_terms[i] = w * l;
+        const long b = i / kLabels;
+        const int c = (int)(i % kLabels);
+        if (c < kNodes) {
+            const float w = m[i] * norm_split;
+            const long o = b * kOut + c * kParts;
+            const int tc = (int)t[i];
+            float mx = z[o];
+            for (int k = 1; k < kParts; ++k) mx = fmaxf(mx, z[o + k]);
+            float se = 0.0f;
+            for (int k = 0; k < kParts; ++k) se += expf(z[o + k] - mx);
+            for (int k = 0; k < kParts; ++k) {
+                const float g = w * (expf(z[o + k] - mx) / se - (k == tc ? 1.0f : 0.0f));
+                dz[o + k] = g;
+                dz_bf[o + k] = f2bf(g);
+            }
+            loss_terms[i] = w * (logf(se) + mx - z[o + (tc >= 0 && tc < kParts ? tc : 0)]);
+        } else {
+            const float w = m[i] * norm_mode;
+            const long o = b * kOut + kModeOff + (c - kNodes);
+            const float zz = z[o], tt = t[i];
+            const float g = w * (1.0f / (1.0f + expf(-zz)) - tt);
+            dz[o] = g;
+            dz_bf[o] = f2bf(g);
+            // log(1 + e^z) - t z, stable
+            loss_terms[i] = w * (fmaxf(zz, 0.0f) - zz * tt + log1pf(expf(-fabsf(zz))));
+        }
     }
 }
 
@@ -300,20 +320,27 @@ __global__ void __launch_bounds__(256) prune_mask_kernel(const float* __restrict
         const long b = i / MT_RDO_NODES;
         const int node = (int)(i % MT_RDO_NODES);
         const float* z = logits + b * kOut;
+        const float* zm = z + kModeOff;
         const int lv = node >= 85 ? 4 : node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
         const int topk = th.topk[lv];
         uint16_t m = (1u << 0) | (1u << 13);
         float covered = 1.0f;
         for (int j = 0; j < 13; ++j) {
             int rank = 0;
-            for (int q = 0; q < 13; ++q) rank += (z[85 + q] > z[85 + j] || (z[85 + q] == z[85 + j] && q < j)) ? 1 : 0;
+            for (int q = 0; q < 13; ++q) rank += (zm[q] > zm[j] || (zm[q] == zm[j] && q < j)) ? 1 : 0;
             if (rank < topk) m |= (uint16_t)(1u << j);
-            else if (j != 0) covered *= 1.0f - 1.0f / (1.0f + expf(-z[85 + j]));
+            else if (j != 0) covered *= 1.0f - 1.0f / (1.0f + expf(-zm[j]));
         }
         if (covered < th.cover) m |= 0x1fffu;
         if (node < 85) {
             const int L = node >= 21 ? 3 : node >= 5 ? 2 : node >= 1 ? 1 : 0;
-            const float ps = 1.0f / (1.0f + expf(-z[node]));
+            // the probability that the node is partitioned at all: 1 - softmax(...)[NONE]
+            const float* zp = z + node * kParts;
+            float mx = zp[0];
+            for (int k = 1; k < kParts; ++k) mx = fmaxf(mx, zp[k]);
+            float se = 0.0f;
+            for (int k = 0; k < kParts; ++k) se += expf(zp[k] - mx);
+            const float ps = 1.0f - expf(zp[0] - mx) / se;
             if (ps < th.lo[L]) m |= MT_RDO_NO_SPLIT;
             else if (ps > th.hi[L]) m |= MT_RDO_FORCE_SPLIT;
         }
@@ -500,7 +527,7 @@ extern "C" int mt_pnet_backward(const float* params_d, const float* targets_d, c
     float* split = f32(w.split);
     float* g = grads_d;
     const long M1 = (long)B * 1024, M2 = (long)B * 256, M3 = (long)B * 64;
-    hipLaunchKernelGGL(bce_kernel, dim3(grid_for((long)B * kOut, cus)), dim3(256), 0, s, f32(w.z5), targets_d, mask_d, B,
+    hipLaunchKernelGGL(pnet_loss_kernel, dim3(grid_for((long)B * kLabels, cus)), dim3(256), 0, s, f32(w.z5), targets_d, mask_d, B,
                        norm_split, norm_mode, bf(w.dz5b), f32(w.dz5f), loss_terms_d ? loss_terms_d : f32(w.loss));
     // fc2: dW5 = a4^T dz5, db5, da4 = dz5 W5^T
     if ((rc = gemm(kH, kOut, B, bf(w.a4), 1, kH, bf(w.dz5b), kOut, 1, g + kW5, split, w.split_elems, s))) return rc;

@@ -1,11 +1,14 @@
 """Learned candidate pruning: the bf16 CNN of mt_nn.hip (forward, backward and Adam on the
 GPU), its training data from the exact RDO's decisions, and the pruned RDO.
 
-Labels come from mt_intra_rdo's full search: for quadtree nodes 0..84 (64x64 .. 8x8) whether
-the search split the node (nodes below a leaf, and nodes the frame edge forces to split,
-are masked out), and for the 13 intra modes whether any leaf of the superblock uses it.
+The network scores, per quadtree node 0..84 (64x64 .. 8x8), the 10 AV1 partition types, and
+per superblock the 13 intra modes.  Labels come from the exact RDO's decisions: the partition
+type each node got (NONE or SPLIT from mt_intra_rdo's search, HORZ / VERT where records of the
+oracle's rectangular search carry them; nodes below a leaf, and nodes the frame edge forces to
+split, are masked out), and for each intra mode whether any leaf of the superblock uses it.
 The masks the trained network produces go to mt_intra_rdo's prune argument; a node whose
-split probability lies between the thresholds keeps both choices (full search there).
+probability of being partitioned lies between the thresholds keeps both choices (full search
+there).
 """
 import ctypes
 
@@ -15,9 +18,14 @@ from . import _lib
 from . import rdo as R
 from ._lib import MtPlane, _check, _dev_ptr, _stream_ptr, init, lib
 
-PARAMS = 46178
-OUTPUTS = 98
+PARAMS = 60031
+OUTPUTS = 863          # 85 nodes x 10 partition-type logits, then 13 mode scores
+LABELS = 98            # label / mask columns: 85 node partition types, then 13 modes
 SPLIT_NODES = 85
+PART_TYPES = 10
+MODE_OFFSET = SPLIT_NODES * PART_TYPES
+# AV1 partition types
+P_NONE, P_HORZ, P_VERT, P_SPLIT = 0, 1, 2, 3
 
 
 def _sigs():
@@ -83,7 +91,7 @@ class PruneNet:
 
     def forward(self, pix, qidx, bd, idx=None, stream=None):
         """pix: uint16 cuda (N, 4096); qidx: uint8 cuda (N,); idx: optional int32 cuda (B,)
-        selecting the batch.  Returns logits float32 cuda (B, 98)."""
+        selecting the batch.  Returns logits float32 cuda (B, 863)."""
         import torch
         B = pix.shape[0] if idx is None else idx.shape[0]
         logits = torch.empty((B, OUTPUTS), dtype=torch.float32, device=self.dev)
@@ -92,14 +100,15 @@ class PruneNet:
         return logits
 
     def backward(self, targets, mask, stream=None):
-        """Gradients of the masked BCE loss of the last forward (same batch size); returns
-        the loss as a cuda scalar."""
+        """Gradients of the masked loss of the last forward (same batch size): softmax
+        cross-entropy of each node's partition type, binary cross-entropy of each mode;
+        targets / mask (B, 98).  Returns the loss as a cuda scalar."""
         import torch
         B = targets.shape[0]
         ns = float(mask[:, :SPLIT_NODES].sum().item())
         norm_split = 1.0 / max(ns, 1.0)
-        norm_mode = 1.0 / (B * (OUTPUTS - SPLIT_NODES))
-        terms = torch.empty((B, OUTPUTS), dtype=torch.float32, device=self.dev)
+        norm_mode = 1.0 / (B * (LABELS - SPLIT_NODES))
+        terms = torch.empty((B, LABELS), dtype=torch.float32, device=self.dev)
         _check(_sigs().mt_pnet_backward(_p(self.params), _p(targets), _p(mask), B, norm_split, norm_mode,
                                         _p(self.workspace(B)), _p(self.grads), _p(terms), _stream_ptr(stream)))
         return terms.sum()
@@ -126,11 +135,12 @@ class PruneNet:
 
 def labels_from_nodes(nodes, width, height):
     """(targets, mask) float32 (nsb, 98) from one frame's RDO node records (numpy,
-    NODE_DTYPE (nsb, 341))."""
+    NODE_DTYPE (nsb, 341)): column n < 85 the AV1 partition type of node n, 85 + j whether
+    mode j is used."""
     nsb = nodes.shape[0]
     C = (width + 63) // 64
-    t = np.zeros((nsb, OUTPUTS), np.float32)
-    m = np.zeros((nsb, OUTPUTS), np.float32)
+    t = np.zeros((nsb, LABELS), np.float32)
+    m = np.zeros((nsb, LABELS), np.float32)
     split = nodes["split"]
     for n in range(SPLIT_NODES):
         L, ix, iy, s = R.node_geometry(n)
@@ -140,7 +150,9 @@ def labels_from_nodes(nodes, width, height):
         inside = (x + s <= width) & (y + s <= height)
         coded = split[:, n] != R.UNCODED
         m[:, n] = (inside & coded).astype(np.float32)
-        t[:, n] = (split[:, n] == R.SPLIT).astype(np.float32)
+        sp = split[:, n]
+        t[:, n] = np.where(sp == R.SPLIT, P_SPLIT, np.where(sp == R.HORZ, P_HORZ,
+                                                              np.where(sp == R.VERT, P_VERT, P_NONE)))
 
     def base(mode_id):
         mode_id = mode_id.astype(np.int64)
@@ -253,7 +265,7 @@ def init_params(seed):
     """He-normal weights and zero biases of the network (fp32, flat, in the layer order of
     the parameter layout)."""
     rng = np.random.default_rng(seed)
-    shapes = [(9, 16), (16,), (144, 32), (32,), (288, 64), (64,), (257, 64), (64,), (64, 98), (98,)]
+    shapes = [(9, 16), (16,), (144, 32), (32,), (288, 64), (64,), (257, 32), (32,), (32, OUTPUTS), (OUTPUTS,)]
     parts = []
     for s in shapes:
         if len(s) == 2:

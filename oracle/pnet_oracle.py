@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY: tests/ and bench.py use it as the checker of the HIP f
 backward and Adam kernels; the library never calls it.
 
 It restates the network in numpy: conv 3x3 s2 (1->16) -> conv 3x3 s2 (16->32) -> conv 3x3 s2
-(32->64), ReLU each, 4x4 average pool ++ qindex/255 -> fc 257->64 ReLU -> fc 64->98, with the
+(32->64), ReLU each, 4x4 average pool ++ qindex/255 -> fc 257->32 ReLU -> fc 32->863 (10
+partition-type logits per quadtree node 0..84, then 13 mode scores), with the
 GPU's numerics when `bf16=True`: every GEMM operand rounded to bf16 (round to nearest even),
 products accumulated in fp32, biases / ReLU / pooling / losses in fp32 with their outputs
 rounded to bf16 where the GPU stores bf16.  With `bf16=False` and float64 arrays it is the
@@ -15,7 +16,10 @@ pruning heuristics, src/coder/partition_search.rs:135-289); parity is with the H
 """
 import numpy as np
 
-C1, C2, C3, F, H, OUT = 16, 32, 64, 257, 64, 98
+C1, C2, C3, F, H = 16, 32, 64, 257, 32
+NODES, PARTS, MODES = 85, 10, 13
+OUT = NODES * PARTS + MODES   # logits
+LABELS = NODES + MODES        # label columns: each node's partition type, then the modes
 SHAPES = [("w1", (9, C1)), ("b1", (C1,)), ("w2", (9 * C1, C2)), ("b2", (C2,)),
           ("w3", (9 * C2, C3)), ("b3", (C3,)), ("w4", (F, H)), ("b4", (H,)),
           ("w5", (H, OUT)), ("b5", (OUT,))]
@@ -81,7 +85,7 @@ class Net:
         return pad[:, 1:-1, 1:-1]
 
     def forward(self, params, pix, qidx, bd):
-        """pix [B, 64, 64] uint16, qidx [B] -> logits [B, 98]; keeps the cache."""
+        """pix [B, 64, 64] uint16, qidx [B] -> logits [B, 863]; keeps the cache."""
         q, dt = self.q, self.dt
         P = {k: v.astype(dt) for k, v in unpack(np.asarray(params, dtype=dt)).items()}
         Wq = {k: q(v) for k, v in P.items() if k.startswith("w")}
@@ -106,15 +110,30 @@ class Net:
         return z5
 
     def backward(self, targets, mask, norm_split, norm_mode):
-        """Gradients (flat, like the parameters) of the masked BCE-with-logits loss, and the
-        loss value."""
+        """Gradients (flat, like the parameters) of the masked loss, and the loss value:
+        targets / mask [B, 98]; per node n < 85 the softmax cross-entropy of its 10 logits
+        with the partition type targets[n], per mode j the binary cross-entropy with logits
+        of its score with targets[85 + j]."""
         q, dt, c = self.q, self.dt, self.c
         B, Wq = c["B"], c["Wq"]
         z = c["z5"]
-        w = mask * np.where(np.arange(OUT) < 85, dt(norm_split), dt(norm_mode))
-        p = 1.0 / (1.0 + np.exp(-z))
-        dz5 = (w * (p - targets)).astype(dt)
-        loss = float((w * (np.maximum(z, 0) - z * targets + np.log1p(np.exp(-np.abs(z))))).sum())
+        zp = z[:, :NODES * PARTS].reshape(B, NODES, PARTS)
+        zm = z[:, NODES * PARTS:]
+        tp = targets[:, :NODES].astype(np.int64)
+        tm = targets[:, NODES:]
+        wp = mask[:, :NODES] * dt(norm_split)
+        wm = mask[:, NODES:] * dt(norm_mode)
+        mx = zp.max(axis=2, keepdims=True)
+        e = np.exp(zp - mx)
+        se = e.sum(axis=2, keepdims=True)
+        onehot = (np.arange(PARTS)[None, None, :] == tp[:, :, None]).astype(dt)
+        dzp = wp[:, :, None] * (e / se - onehot)
+        pick = np.take_along_axis(zp, tp[:, :, None], axis=2)[:, :, 0]
+        loss = float((wp * (np.log(se[:, :, 0]) + mx[:, :, 0] - pick)).sum())
+        p = 1.0 / (1.0 + np.exp(-zm))
+        dzm = wm * (p - tm)
+        loss += float((wm * (np.maximum(zm, 0) - zm * tm + np.log1p(np.exp(-np.abs(zm))))).sum())
+        dz5 = np.concatenate([dzp.reshape(B, NODES * PARTS), dzm], axis=1).astype(dt)
         g = {}
         dz5q = q(dz5)
         g["w5"] = q(c["a4"]).T @ dz5q

@@ -118,7 +118,10 @@ def test_uneven_batch_with_8192_sides_matches_oracle(dev):
 
 
 # ---- the learned pruning -----------------------------------------------------------------
-TOPK, LO, HI = [8, 10, 12, 13, 13], [0.0, 0.01, 0.005, 0.0], [1.0, 1.0, 1.0, 1.0]
+TOPK, LO, HI, COVER = [8, 10, 12, 13, 13], [0.0, 0.03, 0.02, 0.0], [1.0, 1.0, 1.0, 1.0], 0.3
+# fixed budgets for comparing networks: no confidence fallback (an untrained network's flat
+# mode scores would fall back to the full mode search) and lower split thresholds
+LO_FIXED = [0.0, 0.01, 0.005, 0.0]
 
 
 @pytest.fixture(scope="module")
@@ -142,21 +145,22 @@ def test_1080p_420_under_trained_masks_matches_oracle(dev, trained):
     net, _ = trained
     y, u, v = W.synth_yuv(4999, 1920, 1080, 8, 2)
     src = _t(dev, y)
-    masks = PR.prune_masks(net, src, 8, 91, TOPK, LO, HI).cpu().numpy().view(np.uint16)
+    masks = PR.prune_masks(net, src, 8, 91, TOPK, LO, HI, COVER).cpu().numpy().view(np.uint16)
     print("config 1920x1080 4:2:0 q91 pruned", flush=True)
     got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 8, 91, R.CHROMA_420, masks)], ALL)[0]
     full = R.intra_rdo_batch([gpu_frame(dev, y, u, v, 8, 91, R.CHROMA_420)], ALL)[0]
     torch.cuda.synchronize()
     assert_same("1080p pruned", got, oracle(y, u, v, 8, 91, R.CHROMA_420, ALL, prune=masks))
     # the network prunes the luma search; the chroma candidates stay
-    assert got[2]["candidates"] < 0.8 * full[2]["candidates"]
+    print(f"  candidates {got[2]['candidates']} of {full[2]['candidates']}", flush=True)
+    assert got[2]["candidates"] < 0.9 * full[2]["candidates"]
 
 
 def test_trained_network_beats_untrained_under_same_policy(dev, trained):
     """Held-out 1080p frames at qualities 30/50/70/90, luma RDO: the trained network's
     pruned search loses less BD-rate (PSNR-Y vs bits) against the full search than the
-    untrained network under the same mode budgets and split thresholds, and stays within
-    1 %."""
+    untrained network under the same fixed mode budgets and split thresholds (no confidence
+    fallback), and stays within 1 %."""
     net, raw = trained
     imgs = W.synth_images([(5000 + i, 1920, 1080, 8) for i in range(3)])
     srcs = [_t(dev, a) for a in imgs]
@@ -169,7 +173,7 @@ def test_trained_network_beats_untrained_under_same_policy(dev, trained):
             for s in srcs:
                 fr = {"src": s, "bd": 8, "qidx": q, "qm_level": mt.qm_level_law(q, 0)}
                 if model is not None:
-                    fr["prune"] = PR.prune_masks(model, s, 8, q, TOPK, LO, HI)
+                    fr["prune"] = PR.prune_masks(model, s, 8, q, TOPK, LO_FIXED, HI)
                 frames.append(fr)
             out = R.intra_rdo_batch(frames, flags)
             bits = sum(o[2]["bits"] for o in out)

@@ -16,7 +16,8 @@ from maroontree_amd import rdo as R  # noqa: E402
 
 
 def test_parameter_count_matches_library():
-    assert PO.NPARAMS == PR.PARAMS == 46178
+    assert PO.NPARAMS == PR.PARAMS == 60031
+    assert PO.OUT == PR.OUTPUTS == 85 * 10 + 13 and PO.LABELS == PR.LABELS == 98
 
 
 def test_bf16_rounding_known_answers():
@@ -51,6 +52,7 @@ def test_gradients_match_finite_differences():
     pix = rng.integers(0, 1024, (B, 64, 64)).astype(np.uint16)
     qidx = rng.integers(0, 256, B).astype(np.uint8)
     t = (rng.random((B, 98)) < 0.4).astype(np.float64)
+    t[:, :85] = rng.integers(0, 10, (B, 85))  # partition types of the nodes
     m = (rng.random((B, 98)) < 0.8).astype(np.float64)
     net.forward(p, pix, qidx, 10)
     g, loss = net.backward(t, m, 0.1, 0.2)
@@ -60,7 +62,7 @@ def test_gradients_match_finite_differences():
         net2.forward(pp, pix, qidx, 10)
         return net2.backward(t, m, 0.1, 0.2)[1]
     picks = np.concatenate([rng.choice(np.arange(a, b), 6, replace=False) for a, b in (
-        (0, 144), (160, 4768), (4800, 23232), (23296, 39744), (39808, 46080), (46080, 46178))])
+        (0, 144), (160, 4768), (4800, 23232), (23296, 31520), (31552, 59168), (59168, 60031))])
     h = 1e-6
     for i in picks:
         e = np.zeros_like(p)
@@ -78,7 +80,7 @@ def test_labels_from_rdo_nodes():
     nodes["mode"][0, 1:5] = [9, 13 + 6 * 2 + 1, 0, 0]  # SMOOTH, D45 delta -2
     t, m = PR.labels_from_nodes(nodes, 64, 40)
     assert m[0, 0] == 0  # the root crosses the frame edge: forced
-    assert list(m[0, 1:5]) == [1, 1, 0, 0] and list(t[0, 1:5]) == [0, 0, 1, 1]
+    assert list(m[0, 1:5]) == [1, 1, 0, 0] and list(t[0, 1:5]) == [PR.P_NONE, PR.P_NONE, PR.P_SPLIT, PR.P_SPLIT]
     assert t[0, 85 + 9] == 1 and t[0, 85 + 3] == 1 and t[0, 85:].sum() == 2
 
 
