@@ -339,11 +339,15 @@ int mt_distortion(const mt_distblk* blocks_d, int n, const mt_plane* planes, int
  * cfl_pred_pixel(dc_pred, alpha, luma AC) (src/intrapred.rs:1194) with the DC prediction
  * from the reconstruction plane and the AC of cfl_ac_sub / cfl_ac_444 (:1112, :1133) from
  * the luma reconstruction plane at (x << ss_x, y << ss_y); the alpha is cfl_best_alpha's
- * (:1218, against the source) unless MT_CAND_CFL_ALPHA_GIVEN; mt_candres.alpha reports it.
+ * (:1218, against the source) unless MT_CAND_CFL_ALPHA_GIVEN; with MT_CAND_CFL_ALPHA_STEP it
+ * is the alpha angle_delta steps from cfl_best_alpha's along its sign (< 0: toward zero;
+ * clipped to -16..16 and never past zero), the alphas cfl_alpha_rd_refine's walk visits
+ * (src/cost.rs:332); mt_candres.alpha reports the alpha used.
  * For it, pflags holds MT_CAND_CFL_* (not the MT_PRED_* flags), angle_delta the given
  * alpha, and txtp_mode the luma reconstruction plane's index (txtp_set must be 0: chroma
  * codes no transform type). */
-enum { MT_CAND_CFL_SS_HOR = 8, MT_CAND_CFL_SS_VER = 16, MT_CAND_CFL_ALPHA_GIVEN = 32 };
+enum { MT_CAND_CFL_SS_HOR = 8, MT_CAND_CFL_SS_VER = 16, MT_CAND_CFL_ALPHA_GIVEN = 32,
+       MT_CAND_CFL_ALPHA_STEP = 64 };
 typedef struct {
     uint32_t offset;      /* element offset of the candidate's W*H slots in the work pools */
     uint16_t x, y;        /* block origin (the source and reconstruction planes share it) */
@@ -446,8 +450,13 @@ int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* pl
  *    order (the largest chroma transform), each predicted in turn;
  *  - per chroma block, the uv modes 0..12 (angle delta 0) and, with MT_RDO_CFL where CfL is
  *    allowed (luma block <= 32x32), CfL (13): DC prediction + alpha * the AC of the final
- *    luma reconstruction, each plane's alpha from cfl_best_alpha; CfL with both alphas 0 is
- *    DC_PRED and is not a CfL candidate.  Each (mode, plane, transform block) is one
+ *    luma reconstruction, each plane's alpha from cfl_best_alpha; with S =
+ *    MT_RDO_CFL_STEPS_OF(flags) > 0 also the 2S candidates per plane of the alphas S steps
+ *    toward and away from zero, and cfl_alpha_rd_refine's walk (src/cost.rs:332; the
+ *    reference's cfl_alpha_rd_steps, shipped 0, src/tuning.rs:830) picks each plane's alpha,
+ *    U first against V's energy optimum, then V against U's result, scoring an alpha by
+ *    dist + lambda * (coefficient bits + uv_mode_bits of the pair) (src/cost.rs:324); CfL
+ *    with both alphas 0 is DC_PRED and is not a CfL candidate.  Each (mode, plane, transform block) is one
  *    candidate: the transform type the decoder derives from the uv mode
  *    (chroma_tx_for_mode, src/coder/chroma_tx.rs:46; DCT_DCT at 32x32), the chroma
  *    quantizer (qidx with uv_dc_delta / uv_ac_delta, uv_qm_level), plain SSE, the plane-1
@@ -469,6 +478,9 @@ int mt_eval_candidates_trellis(const mt_cand* cands_d, int n, const mt_plane* pl
  * 4:4:4 frames and 2 for 4:2:0 and luma-only frames (their decision lambda's). */
 enum { MT_RDO_ANGLE_DELTAS = 1, MT_RDO_FILTER_INTRA = 2, MT_RDO_MIN4 = 4, MT_RDO_CFL = 8,
        MT_RDO_TRELLIS = 32, MT_RDO_GRAPH = 64 };
+/* CfL alpha walk steps (0..15) in flag bits 8-11, with MT_RDO_CFL. */
+#define MT_RDO_CFL_STEPS(s) (((s) & 15) << 8)
+#define MT_RDO_CFL_STEPS_OF(flags) (((flags) >> 8) & 15)
 /* MT_RDO_GRAPH: the wavefront steps' launches (candidate generation, evaluation, argmin,
  * decisions, reconstruction of every step) are captured from the stream into one HIP graph,
  * instantiated and launched as a whole; the results are the same.  No counterpart in the

@@ -371,7 +371,8 @@ __global__ __launch_bounds__(64) void cfl_eval_kernel(const mt_cand* __restrict_
         const int alpha = cfl_block(planes.p[c.txtp_mode], (int)c.x << (sh ? 1 : 0), (int)c.y << (sv ? 1 : 0), sh,
                                     sv, planes.p[si], c.x, c.y, w, w, bd, dc,
                                     (c.pflags & MT_CAND_CFL_ALPHA_GIVEN) != 0, c.angle_delta, s_ac,
-                                    pred + c.offset, nullptr, resid + c.offset);
+                                    pred + c.offset, nullptr, resid + c.offset,
+                                    (c.pflags & MT_CAND_CFL_ALPHA_STEP) ? c.angle_delta : 0);
         if ((threadIdx.x & 63) == 0) alpha_out[i] = alpha;
     }
 }

@@ -347,12 +347,14 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 // One CfL block by one wave: cfl_ac_sub's subsampled, scaled luma minus its rounded mean
 // (cfl_ac_444 without subsampling) from the luma plane at (lx, ly); unless alpha_given,
 // cfl_best_alpha_strided's analytic alpha and its +/-3 energy search over the displayed part
-// of the source block at (x, y); cfl_pred_pixel over the whole block into dst (and the AC
+// of the source block at (x, y), then alpha_step steps from that alpha along its sign
+// (< 0 toward zero; clipped to -16..16 and never past zero, as cfl_alpha_rd_refine's walk
+// visits them, src/cost.rs:332); cfl_pred_pixel over the whole block into dst (and the AC
 // into ac_dst, src - prediction into resid_dst, when given).  s_ac: LDS of cw * ch ints.
 // Returns the alpha.  The caller has checked that both blocks lie inside their planes.
 __device__ int cfl_block(const mt_plane& lp, int lx, int ly, bool sh, bool sv, const mt_plane& sp, int x, int y,
                          int cw, int ch, int bd, int dc, bool alpha_given, int alpha_in, int32_t* s_ac,
-                         int32_t* dst, int32_t* ac_dst, int16_t* resid_dst) {
+                         int32_t* dst, int32_t* ac_dst, int16_t* resid_dst, int alpha_step = 0) {
     const int lane = threadIdx.x & 63;
     const int lcw = __builtin_ctz(cw), npx = cw * ch;
     const int shift = 1 + (sv ? 0 : 1) + (sh ? 0 : 1);
@@ -421,6 +423,12 @@ __device__ int cfl_block(const mt_plane& lp, int lx, int ly, bool sh, bool sv, c
                     alpha = c;
                 }
             }
+        }
+        if (alpha_step != 0) {
+            const int dir = alpha > 0 ? 1 : (alpha < 0 ? -1 : 0);
+            int a = min(max(alpha + dir * alpha_step, -16), 16);
+            if (a * dir < 0) a = 0;
+            alpha = a;
         }
     }
     for (int p = lane; p < npx; p += 64) {

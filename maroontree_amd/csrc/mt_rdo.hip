@@ -565,11 +565,12 @@ __host__ __device__ inline int uv_tx_type(int m, int cs) {
 // uv_mode_bits (src/coder/lossy_state.rs:1302) of uv mode m (13 CfL with alphas au, av)
 // under luma mode ym: the uv_mode symbol from the CfL-allowed or the no-CfL CDF, the delta-0
 // angle symbol of a directional mode where angle deltas are coded, the CfL joint sign and
-// the alpha magnitudes.
+// the alpha magnitudes (CfL with both alphas zero is priced as DC_PRED, :1333).
 __device__ float uv_mode_bits(int ym, int m, int au, int av, int flags) {
     const int allowed = (flags & kUvCflAllowed) ? 1 : 0;
     if (m == MT_PRED_CFL) {
         const int su = au == 0 ? 0 : au < 0 ? 1 : 2, sv = av == 0 ? 0 : av < 0 ? 1 : 2;
+        if (su == 0 && sv == 0) return c_sym.uv[1][ym][MT_PRED_DC];  // both zero: DC_PRED
         float b = __fadd_rn(c_sym.uv[1][ym][MT_PRED_CFL], c_sym.cfl_sign[su * 3 + sv - 1]);
         if (su) b = __fadd_rn(b, c_sym.cfl_alpha[(su == 2) * 3 + sv][abs(au) - 1]);
         if (sv) b = __fadd_rn(b, c_sym.cfl_alpha[(sv == 2) * 3 + su][abs(av) - 1]);
@@ -614,13 +615,21 @@ __device__ void uv_walk(const FrameArgs& f, const mt_rdo_node* nd, int x0, int y
         b.ymode = (uint8_t)id_base(nd[group ? node_id(4, 2 * ix + 1, 2 * iy + 1) : id].mode);
         const int luma_bs = group ? 4 : s;  // the block the chroma is coded with
         b.flags = (uint8_t)((luma_bs <= 32 ? kUvCflAllowed : 0) | (luma_bs >= 8 ? kUvAngle : 0));
-        b.nm = (uint8_t)(13 + ((cfl_search && (b.flags & kUvCflAllowed)) ? 1 : 0));
+        // uv modes 0..12, CfL (13), then CfL's walk candidates (14 + j: alpha step -(j + 1)
+        // toward zero for j < S, j - S + 1 away from zero after that)
+        b.nm = (uint8_t)(13 + ((cfl_search && (b.flags & kUvCflAllowed)) ? 1 + 2 * MT_RDO_CFL_STEPS_OF(f.flags) : 0));
         visit(b);
     }
 }
 
-// Candidate of uv mode m (13 CfL), plane p (0 U, 1 V), transform block t of chroma block b;
-// alpha >= -16 with the final contexts gives CfL its alpha.
+// The alpha step of CfL walk candidate m >= 14 under S walk steps.
+__device__ inline int cfl_walk_step(int m, int S) {
+    const int j = m - 14;
+    return j < S ? -(j + 1) : j - S + 1;
+}
+
+// Candidate of uv mode m (13 CfL, 14.. CfL walk steps), plane p (0 U, 1 V), transform block t
+// of chroma block b; with the final contexts, alpha gives CfL its alpha.
 __device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p, int t, int x0, int y0, bool fin,
                                 int alpha, uint32_t offset) {
     mt_cand c{};
@@ -630,12 +639,12 @@ __device__ mt_cand make_uv_cand(const FrameArgs& f, const UvBlk& b, int m, int p
     c.x = (uint16_t)tx;
     c.y = (uint16_t)ty;
     c.tx_size = (uint8_t)(ts == 4 ? MT_TX_4X4 : ts == 8 ? MT_TX_8X8 : ts == 16 ? MT_TX_16X16 : MT_TX_32X32);
-    c.tx_type = (uint8_t)(m == MT_PRED_CFL ? MT_TXTP_DCT_DCT : uv_tx_type(m, ts));
-    c.mode = (uint8_t)m;
-    if (m == MT_PRED_CFL) {
+    c.tx_type = (uint8_t)(m >= MT_PRED_CFL ? MT_TXTP_DCT_DCT : uv_tx_type(m, ts));
+    c.mode = (uint8_t)(m > MT_PRED_CFL ? MT_PRED_CFL : m);
+    if (m >= MT_PRED_CFL) {
         c.pflags = (uint8_t)(f.ss ? (MT_CAND_CFL_SS_HOR | MT_CAND_CFL_SS_VER) : 0) |
-                   (uint8_t)(fin ? MT_CAND_CFL_ALPHA_GIVEN : 0);
-        c.angle_delta = (int8_t)(fin ? alpha : 0);
+                   (uint8_t)(fin ? MT_CAND_CFL_ALPHA_GIVEN : 0) | (uint8_t)(m > MT_PRED_CFL ? MT_CAND_CFL_ALPHA_STEP : 0);
+        c.angle_delta = (int8_t)(fin ? alpha : m > MT_PRED_CFL ? cfl_walk_step(m, MT_RDO_CFL_STEPS_OF(f.flags)) : 0);
         c.txtp_mode = (uint8_t)(f.pbase + 1);
     } else {
         // availability: the luma block's (4:4:4 64x64: each 32x32 transform block's)
@@ -721,6 +730,52 @@ __global__ __launch_bounds__(64) void rdo_uv_gen_kernel(const FrameArgs* __restr
     if (lane == 0) nblks[local] = (uint32_t)nb;
 }
 
+// CfL of chroma block b from its candidates q (q[0] U, q[1] V at cfl_best_alpha's alphas,
+// then the 2S walk candidates j of plane p at q[2 + 2j + p]): with S > 0, cfl_alpha_rd_refine's
+// walk (src/cost.rs:332) per plane, U first against V's energy optimum, then V against U's
+// result, an alpha scored as dist + lambda * (coefficient bits + uv_mode_bits of the pair)
+// (cfl_alpha_walk_cost, :324).  Returns the summed rd_cost of the chosen candidates.
+__device__ float cfl_decide(const FrameArgs& f, const UvBlk& b, const mt_candres* q, int& au, int& av) {
+    const int S = MT_RDO_CFL_STEPS_OF(f.flags);
+    int a[2] = {q[0].alpha, q[1].alpha};
+    float body[2] = {q[0].cost, q[1].cost};
+    for (int pl = 0; pl < 2 && S > 0; ++pl) {
+        const mt_candres* st = q + 2 + pl;  // walk candidate j at st[2 j]
+        auto walk_cost = [&](const mt_candres& r) {
+            const int pu = pl ? a[0] : r.alpha, pv = pl ? r.alpha : a[1];
+            return __fadd_rn((float)r.dist,
+                             __fmul_rn(f.lambda, __fadd_rn(r.bits, uv_mode_bits(b.ymode, MT_PRED_CFL, pu, pv, b.flags))));
+        };
+        const int a0 = a[pl];
+        if (a0 == 0) continue;
+        const int dir = a0 > 0 ? 1 : -1;
+        int best = -1;  // the walk candidate chosen; -1: the energy optimum
+        float best_c = walk_cost(q[pl]);
+        for (int s = 1; s <= S; ++s) {  // toward zero
+            if ((a0 - dir * s) * dir < 0) break;
+            const float c = walk_cost(st[2 * (s - 1)]);
+            if (!(c < best_c)) break;
+            best = s - 1;
+            best_c = c;
+        }
+        if (best < 0)
+            for (int s = 1; s <= S; ++s) {  // away from zero
+                if (abs(a0 + dir * s) > 16) break;
+                const float c = walk_cost(st[2 * (S + s - 1)]);
+                if (!(c < best_c)) break;
+                best = S + s - 1;
+                best_c = c;
+            }
+        if (best >= 0) {
+            a[pl] = st[2 * best].alpha;
+            body[pl] = st[2 * best].cost;
+        }
+    }
+    au = a[0];
+    av = a[1];
+    return __fadd_rn(__fadd_rn(0.0f, body[0]), body[1]);
+}
+
 // One wave per SB: per chroma block (lanes over blocks) the cheapest uv mode, the uv map and
 // the node record; then (final contexts) the reconstruction descriptors in coding order:
 // per block U then V, each over its transform blocks.  Descriptor fields: x, y, bw = bh,
@@ -753,15 +808,15 @@ __global__ __launch_bounds__(64) void rdo_uv_decide_kernel(const FrameArgs* __re
         const int per = 2 * b.nt;
         float best = inf;
         int bm = 0, ba = 0, bv = 0;
-        for (int m = 0; m < b.nm; ++m) {
+        for (int m = 0; m < min((int)b.nm, MT_PRED_CFL + 1); ++m) {
             const mt_candres* q = rs + b.cand + m * per;
             float sum = 0.0f;
-            for (int k = 0; k < per; ++k) sum = __fadd_rn(sum, q[k].cost);
             int au = 0, av = 0;
-            if (m == MT_PRED_CFL) {
-                au = q[0].alpha;
-                av = q[b.nt].alpha;
+            if (m == MT_PRED_CFL) {  // one transform block per plane
+                sum = cfl_decide(f, b, q, au, av);
                 if (au == 0 && av == 0) continue;  // CfL with both alphas 0 is DC_PRED
+            } else {
+                for (int k = 0; k < per; ++k) sum = __fadd_rn(sum, q[k].cost);
             }
             if (!(sum < inf)) continue;
             const float cost = __fadd_rn(sum, __fmul_rn(f.lambda, uv_mode_bits(b.ymode, m, au, av, b.flags)));
@@ -1081,9 +1136,11 @@ int run_batch(RdoWork& w, const mt_rdo_frame* frames, int nf, int flags, mt_rdo_
             rf[i].uv_dc = fr.uv_dc_delta;
             rf[i].uv_ac = fr.uv_ac_delta;
             rf[i].ss = f.ss;
-            // 4:2:0: <= 64 chroma blocks of <= 14 modes x 2 planes; 4:4:4: <= 256 (4x4 leaves)
-            const uint32_t c1 = (f.ss || f.minsz == 8) ? 64u * 28u : 256u * 28u;
-            const uint32_t p1 = f.ss ? 28u * 1024u : 28u * 4096u;
+            // 4:2:0: <= 64 chroma blocks of <= 14 + 2S modes x 2 planes; 4:4:4: <= 256 (4x4
+            // leaves)
+            const uint32_t per = 2u * (14u + 2u * (uint32_t)MT_RDO_CFL_STEPS_OF(flags));
+            const uint32_t c1 = (f.ss || f.minsz == 8) ? 64u * per : 256u * per;
+            const uint32_t p1 = f.ss ? per * 1024u : per * 4096u;
             maxc = std::max(maxc, c1);
             maxp = std::max(maxp, p1);
         }
@@ -1452,7 +1509,8 @@ extern "C" float mt_rdo_lambda(int qidx, int bd, int sub) {
 extern "C" int mt_intra_rdo_batch(const mt_rdo_frame* frames, int nframes, int flags, mt_rdo_stats* stats,
                                   void* stream) {
     if (!frames || !stats || nframes < 0) return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: null argument");
-    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL | MT_RDO_TRELLIS | MT_RDO_GRAPH))
+    if (flags & ~(MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL | MT_RDO_TRELLIS | MT_RDO_GRAPH |
+                  MT_RDO_CFL_STEPS(15)))
         return mtlib::set_error(MT_ERR_ARG, "mt_intra_rdo: bad flags");
     int rc = MT_OK;
     for (int i = 0; i < nframes; ++i)

@@ -14,6 +14,13 @@ from . import _lib
 from ._lib import MtPlane, _check, _dev_ptr, _stream_ptr, init, lib
 
 ANGLE_DELTAS, FILTER_INTRA, MIN4, CFL, TRELLIS, GRAPH = 1, 2, 4, 8, 32, 64
+
+
+def cfl_steps(s):
+    """Flag bits of cfl_alpha_rd_refine's walk with s (0..15) steps each way (with CFL)."""
+    if not 0 <= s <= 15:
+        raise ValueError("CfL walk steps are 0..15")
+    return s << 8
 NONE, SPLIT, UNCODED, HORZ, VERT = 0, 1, 2, 3, 4  # node decisions (the split field; HORZ / VERT: oracle only)
 LUMA_ONLY, CHROMA_420, CHROMA_444 = 0, 1, 2
 NODES = 341

@@ -201,6 +201,10 @@ void orc_cfl_ac_sub(const uint16_t* luma, int lstride, int cw, int ch, int ss_ho
 int32_t orc_cfl_pred_pixel(int32_t dc, int32_t ac, int32_t alpha, int bd);
 int32_t orc_cfl_best_alpha(const int32_t* ac, int ac_stride, const uint16_t* src, int src_stride,
                            int w, int h, int32_t dc, int bd);
+/* cfl_alpha_rd_refine's walk (src/cost.rs:332) over walk costs cost[a + 16], and the alpha
+ * s steps from a0 along a0's sign (s < 0 toward zero; clipped to -16..16, never past 0). */
+int32_t orc_cfl_alpha_step(int32_t a0, int32_t s);
+int32_t orc_cfl_alpha_rd_refine(int32_t a0, float c0, int steps, const float* cost);
 
 /* ---- residual and distortion (src/rd_sse.rs, src/partition_rd.rs) ----------------- */
 void orc_residual_pred(const uint16_t* src, int stride, const int32_t* pred, int w, int h,

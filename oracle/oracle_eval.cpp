@@ -75,7 +75,8 @@ void eval_one(const orc_cand& c, const orc_plane* planes, int nplanes, orc_candr
         const int dc = pred[0];
         const int vw = std::min(std::max(sp.vis_w - (int)c.x, 0), w);
         const int vh = std::min(std::max(sp.vis_h - (int)c.y, 0), h);
-        const int alpha = (c.pflags & 32) ? c.angle_delta : orc_cfl_best_alpha(ac.data(), w, src, sp.stride, vw, vh, dc, bd);
+        int alpha = (c.pflags & 32) ? c.angle_delta : orc_cfl_best_alpha(ac.data(), w, src, sp.stride, vw, vh, dc, bd);
+        if (!(c.pflags & 32) && (c.pflags & 64)) alpha = orc_cfl_alpha_step(alpha, c.angle_delta);
         for (int i = 0; i < w * h; ++i) pred[i] = orc_cfl_pred_pixel(dc, ac[i], alpha, bd);
         r.alpha = alpha;
     }

@@ -439,6 +439,45 @@ int32_t orc_cfl_best_alpha(const int32_t* ac, int ac_stride, const uint16_t* src
     return best_a;
 }
 
+// The alpha s steps from a0 in the direction of a0's sign (s < 0: toward zero), as
+// cfl_alpha_rd_refine's walk visits them (src/cost.rs:332-372): clipped to -16..16 and never
+// past zero; a0 = 0 stays 0.
+int32_t orc_cfl_alpha_step(int32_t a0, int32_t s) {
+    const int32_t dir = a0 > 0 ? 1 : (a0 < 0 ? -1 : 0);
+    int32_t a = clampi(a0 + dir * s, -16, 16);
+    if (a * dir < 0) a = 0;
+    return a;
+}
+
+// cfl_alpha_rd_refine (src/cost.rs:332): from the energy optimum a0 of walk cost c0, steps
+// toward zero while the cost keeps falling; if the first such step does not lower it, steps
+// away from zero the same way; at most `steps` each way.  cost[a + 16] is the walk cost of
+// alpha a (read only for the alphas the walk visits).
+int32_t orc_cfl_alpha_rd_refine(int32_t a0, float c0, int steps, const float* cost) {
+    if (steps <= 0 || a0 == 0) return a0;
+    const int32_t dir = a0 > 0 ? 1 : -1;
+    int32_t best_a = a0;
+    float best_c = c0;
+    for (int s = 1; s <= steps; ++s) {
+        const int32_t a = a0 - dir * s;
+        if (a * dir < 0) break;
+        const float c = cost[a + 16];
+        if (!(c < best_c)) break;
+        best_a = a;
+        best_c = c;
+    }
+    if (best_a == a0)
+        for (int s = 1; s <= steps; ++s) {
+            const int32_t a = a0 + dir * s;
+            if (std::abs(a) > 16) break;
+            const float c = cost[a + 16];
+            if (!(c < best_c)) break;
+            best_a = a;
+            best_c = c;
+        }
+    return best_a;
+}
+
 // ---- residual and distortion (src/rd_sse.rs, src/partition_rd.rs) ---------------------
 
 // residual_pred_scalar (src/rd_sse.rs:1179)

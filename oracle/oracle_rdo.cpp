@@ -229,6 +229,7 @@ float uv_bits(int ym, int m, int au, int av, bool cfl_allowed, bool angle_ok) {
     const SymBits& T = sym_bits();
     if (m == 13) {
         const int su = au == 0 ? 0 : au < 0 ? 1 : 2, sv = av == 0 ? 0 : av < 0 ? 1 : 2;
+        if (su == 0 && sv == 0) return T.uv[1][ym][0];  // both zero: DC_PRED (:1333)
         float b = T.uv[1][ym][13] + T.cfl_sign[su * 3 + sv - 1];
         if (su) b = b + T.cfl_alpha[(su == 2) * 3 + sv][std::abs(au) - 1];
         if (sv) b = b + T.cfl_alpha[(sv == 2) * 3 + su][std::abs(av) - 1];
@@ -414,7 +415,15 @@ struct Frame {
         return out;
     }
 
-    int uv_modes(const UvB& b) const { return 13 + ((flags & kCfl) && b.cfl_ok ? 1 : 0); }
+    // cfl_alpha_rd_steps (src/tuning.rs:343): flag bits 8-11
+    int cfl_steps() const { return (flags >> 8) & 15; }
+    // uv modes 0..12, CfL (13), then CfL's walk candidates (14 + j: alpha step -(j + 1) toward
+    // zero for j < S, j - S + 1 away from zero after that)
+    int uv_modes(const UvB& b) const { return 13 + ((flags & kCfl) && b.cfl_ok ? 1 + 2 * cfl_steps() : 0); }
+    int cfl_step(int m) const {
+        const int j = m - 14, S = cfl_steps();
+        return j < S ? -(j + 1) : j - S + 1;
+    }
 
     // candidate of uv mode m, plane p (0 U, 1 V), transform block t; fin: final contexts and
     // the given CfL alpha
@@ -426,11 +435,11 @@ struct Frame {
         c.x = (uint16_t)tx;
         c.y = (uint16_t)ty;
         c.tx_size = (uint8_t)tx_of(tw, th);
-        c.tx_type = (uint8_t)(m == 13 ? 0 : uv_txtp(m, tw, th));
-        c.mode = (uint8_t)m;
-        if (m == 13) {
-            c.pflags = (uint8_t)((ss ? 8 | 16 : 0) | (fin ? 32 : 0));
-            c.angle_delta = (int8_t)(fin ? alpha : 0);
+        c.tx_type = (uint8_t)(m >= 13 ? 0 : uv_txtp(m, tw, th));
+        c.mode = (uint8_t)(m > 13 ? 13 : m);
+        if (m >= 13) {
+            c.pflags = (uint8_t)((ss ? 8 | 16 : 0) | (fin ? 32 : 0) | (m > 13 ? 64 : 0));
+            c.angle_delta = (int8_t)(fin ? alpha : m > 13 ? cfl_step(m) : 0);
             c.txtp_mode = 1;  // the luma reconstruction plane
         } else {
             const bool tr = b.nt == 4 ? have_tr((t & 1) * 32, (t >> 1) * 32, 32) : b.tr;
@@ -456,6 +465,53 @@ struct Frame {
         c.wsse_scale = 0.0f;
         c.lambda = lambda;
         return c;
+    }
+
+    // CfL of chroma block b (one transform block per plane): each plane's cfl_best_alpha
+    // candidate, then with S = cfl_steps() > 0 the walk of cfl_alpha_rd_refine
+    // (src/coder/block32.rs:2350-2391, src/cost.rs:332) over the 2S step candidates, U first
+    // against V's energy optimum, then V against U's result, each alpha scored by
+    // cfl_alpha_walk_cost's default (src/cost.rs:324): dist + lambda * (coefficient bits +
+    // uv_mode_bits of the alpha pair).  Returns the summed body cost (rd_cost of each plane's
+    // chosen candidate) and the alphas.
+    float cfl_decide(const UvB& b, int x0, int y0, int& au, int& av, int64_t& ncand) const {
+        orc_candres base[2];
+        for (int pl = 0; pl < 2; ++pl) {
+            const orc_cand cd = uv_cand(b, 13, pl, 0, x0, y0, false, 0);
+            orc_eval_candidate(&cd, planes, nplanes, &base[pl]);
+            ++ncand;
+        }
+        int a[2] = {base[0].alpha, base[1].alpha};
+        float body[2] = {base[0].cost, base[1].cost};
+        const int S = cfl_steps();
+        if (S > 0) {
+            std::vector<orc_candres> st(4 * (size_t)S);
+            for (int pl = 0; pl < 2; ++pl)
+                for (int j = 0; j < 2 * S; ++j) {
+                    const orc_cand cd = uv_cand(b, 14 + j, pl, 0, x0, y0, false, 0);
+                    orc_eval_candidate(&cd, planes, nplanes, &st[(size_t)pl * 2 * S + j]);
+                    ++ncand;
+                }
+            for (int pl = 0; pl < 2; ++pl) {
+                auto walk_cost = [&](const orc_candres& r, int al) {
+                    const int pu = pl ? a[0] : al, pv = pl ? al : a[1];
+                    return (float)r.dist + lambda * (r.bits + uv_bits(b.ym, 13, pu, pv, true, b.angle_ok));
+                };
+                float cost[33], bc[33];
+                for (int k = 0; k < 33; ++k) cost[k] = bc[k] = __builtin_inff();
+                for (int j = 0; j < 2 * S; ++j) {
+                    const orc_candres& r = st[(size_t)pl * 2 * S + j];
+                    cost[r.alpha + 16] = walk_cost(r, r.alpha);
+                    bc[r.alpha + 16] = r.cost;
+                }
+                const int a1 = orc_cfl_alpha_rd_refine(a[pl], walk_cost(base[pl], a[pl]), S, cost);
+                if (a1 != a[pl]) body[pl] = bc[a1 + 16];
+                a[pl] = a1;
+            }
+        }
+        au = a[0];
+        av = a[1];
+        return (0.0f + body[0]) + body[1];
     }
 
     void put_leaf(const Leaf& lf) {
@@ -611,18 +667,21 @@ void do_sb(Frame& f, int r, int c, const uint16_t* mask, orc_rdo_node* nodes, Sb
         const UvB& b = blocks[i];
         float bestc = __builtin_inff();
         int bm = 0, ba = 0, bv = 0;
-        for (int m = 0; m < f.uv_modes(b); ++m) {
+        for (int m = 0; m < std::min(f.uv_modes(b), 14); ++m) {
             float sum = 0.0f;
             int au = 0, av = 0;
-            for (int pl = 0; pl < 2; ++pl)
-                for (int t = 0; t < b.nt; ++t) {
-                    const orc_cand cd = f.uv_cand(b, m, pl, t, x0, y0, false, 0);
-                    orc_candres res;
-                    orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
-                    ++out.candidates;
-                    sum = sum + res.cost;
-                    if (m == 13 && t == 0) (pl ? av : au) = res.alpha;
-                }
+            if (m == 13) {
+                sum = f.cfl_decide(b, x0, y0, au, av, out.candidates);
+            } else {
+                for (int pl = 0; pl < 2; ++pl)
+                    for (int t = 0; t < b.nt; ++t) {
+                        const orc_cand cd = f.uv_cand(b, m, pl, t, x0, y0, false, 0);
+                        orc_candres res;
+                        orc_eval_candidate(&cd, f.planes, f.nplanes, &res);
+                        ++out.candidates;
+                        sum = sum + res.cost;
+                    }
+            }
             if (m == 13 && au == 0 && av == 0) continue;
             if (!(sum < __builtin_inff())) continue;
             const float cost = sum + f.lambda * uv_bits(b.ym, m, au, av, b.cfl_ok, b.angle_ok);

@@ -59,6 +59,26 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
     return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// Graph capture is not emulated: every graph call fails, so a run with MT_RDO_GRAPH reports an
+// error instead of skipping its work.
+typedef void* hipGraph_t;
+typedef void* hipGraphExec_t;
+enum { hipStreamNonBlocking = 1 };
+enum hipStreamCaptureMode { hipStreamCaptureModeRelaxed = 2 };
+constexpr hipError_t kEmuNoGraphs = 801;
+inline hipError_t hipStreamCreateWithFlags(hipStream_t*, unsigned) { return kEmuNoGraphs; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return kEmuNoGraphs; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return kEmuNoGraphs; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, char*, size_t) { return kEmuNoGraphs; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return kEmuNoGraphs; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return kEmuNoGraphs; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return kEmuNoGraphs; }
+// host execution never captures a stream
+enum hipStreamCaptureStatus { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive = 1 };
+inline hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus* st) {
+    *st = hipStreamCaptureStatusNone;
+    return hipSuccess;
+}
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
     *p = (void*)sym;
     return hipSuccess;

@@ -740,3 +740,20 @@ def intra_rdo_levels(src, bd, qidx, qm_level, flags, lam, chroma=0, src_u=None, 
     stats = {"bits": st.bits, "sse": st.sse, "candidates": st.candidates, "leaves": st.leaves, "sbs": st.sbs,
              "bits_uv": st.bits_uv, "sse_u": st.sse_u, "sse_v": st.sse_v}
     return recon, ru, rv, nodes, stats, (cy, cu, cv)
+
+
+def cfl_alpha_step(a0, s):
+    L = lib()
+    L.orc_cfl_alpha_step.restype = ctypes.c_int32
+    L.orc_cfl_alpha_step.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    return int(L.orc_cfl_alpha_step(int(a0), int(s)))
+
+
+def cfl_alpha_rd_refine(a0, c0, steps, cost_by_alpha):
+    """The oracle's walk; cost_by_alpha: 33 walk costs, index alpha + 16."""
+    L = lib()
+    L.orc_cfl_alpha_rd_refine.restype = ctypes.c_int32
+    L.orc_cfl_alpha_rd_refine.argtypes = [ctypes.c_int32, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
+    c = np.ascontiguousarray(cost_by_alpha, np.float32)
+    assert c.shape == (33,)
+    return int(L.orc_cfl_alpha_rd_refine(int(a0), float(c0), int(steps), c.ctypes.data))

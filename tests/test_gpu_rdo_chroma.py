@@ -74,6 +74,24 @@ def test_rdo_chroma_matches_oracle(dev, fmt, bd, q, w, h, flags):
         assert (R.nodes_to_np(got[1])["uv_mode"] == 13).any()
 
 
+@pytest.mark.parametrize("fmt,bd,q,w,h,flags", [
+    (R.CHROMA_420, 10, 91, 200, 136, ALL | R.cfl_steps(3)),
+    (R.CHROMA_444, 8, 53, 136, 72, ALL | R.cfl_steps(15)),
+    (R.CHROMA_420, 12, 30, 136, 72, ALL | R.TRELLIS | R.cfl_steps(2)),
+])
+def test_rdo_cfl_walk_matches_oracle(dev, fmt, bd, q, w, h, flags):
+    """cfl_alpha_rd_refine's walk (the 2S step candidates per plane and the walk over them):
+    every output equals the oracle's, and some alpha differs from the run without the walk."""
+    y, u, v = frame(300 + q, w, h, bd, fmt)
+    print(f"rdo cfl walk fmt={fmt} {w}x{h} bd={bd} q={q} flags={flags}", flush=True)
+    got = R.intra_rdo_batch([gpu_frame(dev, y, u, v, bd, q, fmt)], flags)[0]
+    plain = R.intra_rdo_batch([gpu_frame(dev, y, u, v, bd, q, fmt)], flags & 0xFF)[0]
+    torch.cuda.synchronize()
+    assert_same(got, oracle(y, u, v, bd, q, fmt, flags))
+    g, p = R.nodes_to_np(got[1]), R.nodes_to_np(plain[1])
+    assert ((g["cfl_alpha_u"] != p["cfl_alpha_u"]) | (g["cfl_alpha_v"] != p["cfl_alpha_v"])).any()
+
+
 def test_rdo_mixed_formats_batch_matches_oracle(dev):
     """Luma-only, 4:2:0 and 4:4:4 frames of different sizes and depths in one batch (more
     frames than one step group holds with chroma): each equals its oracle result."""

@@ -177,11 +177,42 @@ def test_chroma_reconstruction_replays_from_the_decisions(fmt, bd, qidx, w, h, f
         assert not (nodes["uv_mode"] == 13).any()
 
 
-def test_chroma_decisions_are_the_argmin_over_candidates():
+def ref_walk(a0, c0, steps, eval_cost):
+    """cfl_alpha_rd_refine (src/cost.rs:332) as the reference writes it, over a cost callback."""
+    if steps == 0 or a0 == 0:
+        return a0
+    best_a, best_c = a0, c0
+    d = 1 if a0 > 0 else -1
+    for s in range(1, steps + 1):
+        a = a0 - d * s
+        if a * d < 0:
+            break
+        c = eval_cost(a)
+        if c < best_c:
+            best_a, best_c = a, c
+        else:
+            break
+    if best_a == a0:
+        for s in range(1, steps + 1):
+            a = a0 + d * s
+            if abs(a) > 16:
+                break
+            c = eval_cost(a)
+            if c < best_c:
+                best_a, best_c = a, c
+            else:
+                break
+    return best_a
+
+
+@pytest.mark.parametrize("steps", [0, 3])
+def test_chroma_decisions_are_the_argmin_over_candidates(steps):
     """Rebuild every chroma candidate of the last SB of a 2 x 2 SB frame (4:2:0, CfL) on the
     decision-time planes (final reconstruction outside the SB, source inside) and check each
-    block's uv mode and CfL alphas."""
-    bd, qidx, flags, fmt = 10, 91, ANGLE | MIN4 | CFL, 1
+    block's uv mode and CfL alphas; with CfL walk steps, each plane's alpha walked from its
+    energy optimum over candidates rebuilt at given alphas (U against V's optimum, then V
+    against U's result; walk cost dist + lambda * (bits + uv_mode_bits of the pair))."""
+    bd, qidx, flags, fmt = 10, 91, ANGLE | MIN4 | CFL | (steps << 8), 1
     y, u, v, ss = chroma_frame(77, 128, 128, bd, fmt)
     qm = mt.qm_level_law(qidx, 2)
     lam = np.float32(O.rdo_lambda(qidx, bd, 2))
@@ -198,6 +229,8 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
         if m == 13:
             su = 0 if au == 0 else 1 if au < 0 else 2
             sv = 0 if av == 0 else 1 if av < 0 else 2
+            if su == 0 and sv == 0:  # both zero: priced as DC_PRED (src/coder/lossy_state.rs:1333)
+                return cost(tabs["UV_MODE_CFL_CDF"][ym], 14, 0)
             b = np.float32(cost(tabs["UV_MODE_CFL_CDF"][ym], 14, 13) + cost(tabs["CFL_SIGN_CDF"], 8, su * 3 + sv - 1))
             if su:
                 b = np.float32(b + cost(tabs["CFL_ALPHA_CDF"][(su == 2) * 3 + sv], 16, abs(au) - 1))
@@ -220,21 +253,23 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
         r, c = divmod(s2, 2)
         for (n, half, cx, cy, cw, ch, *_rest) in uv_blocks(nodes[s2], 64 * c, 64 * r, ss):
             uvmap[cy // 4:(cy + ch) // 4, cx // 4:(cx + cw) // 4] = int(nodes[s2][n]["uv_mode1" if half else "uv_mode"])
-    checked = 0
+    checked = moved = 0
     for (n, half, cx, cy, cw, ch, nt, ym, cfl_ok, ang_ok, (tr, bl)) in uv_blocks(nodes[sb], x0, y0, ss):
         cs = cw
         am = uvmap[(cy - 1) // 4, cx // 4] if cy == 32 else 0
         lm = uvmap[cy // 4, (cx - 1) // 4] if cx == 32 else 0
         smooth = am in (9, 10, 11) or lm in (9, 10, 11)
         best, bm, ba = np.float32(np.inf), None, (0, 0)
-        for m in range(14 if cfl_ok else 13):
+
+        def cands(m, given=None):
             cd = np.zeros(2, mt.CAND_DTYPE)
             for p in range(2):
                 c = cd[p]
                 c["x"], c["y"], c["tx_size"] = cx, cy, TX[cs]
                 c["mode"] = m
                 if m == 13:
-                    c["pflags"] = 8 | 16
+                    c["pflags"] = 8 | 16 | (32 if given is not None else 0)
+                    c["angle_delta"] = 0 if given is None else given[p]
                     c["txtp_mode"] = 1
                 else:
                     c["tx_type"] = UV_TXTP[m] if cs < 32 else 0
@@ -250,10 +285,28 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
                 c["skip_ctx"] = 7
                 c["qctx"] = 2
                 c["lambda"] = lam
-            res = O.eval_batch(cd, planes)
-            s = np.float32(np.float32(0) + res["cost"][0])
-            s = np.float32(s + res["cost"][1])
+            return O.eval_batch(cd, planes)
+
+        for m in range(14 if cfl_ok else 13):
+            res = cands(m)
+            body = [res["cost"][0], res["cost"][1]]
             au, av = (int(res["alpha"][0]), int(res["alpha"][1])) if m == 13 else (0, 0)
+            if m == 13 and steps:
+                al = [au, av]
+                for p in range(2):
+                    def walk_cost(a, p=p):
+                        r = cands(13, (a, a))[p]
+                        pair = (a, al[1]) if p == 0 else (al[0], a)
+                        return np.float32(np.float32(r["dist"]) +
+                                          np.float32(lam * np.float32(r["bits"] + uv_bits(ym, 13, *pair, True, ang_ok))))
+                    a1 = ref_walk(al[p], walk_cost(al[p]), steps, walk_cost)
+                    if a1 != al[p]:
+                        body[p] = cands(13, (a1, a1))["cost"][p]
+                        moved += 1
+                    al[p] = a1
+                au, av = al
+            s = np.float32(np.float32(0) + body[0])
+            s = np.float32(s + body[1])
             if m == 13 and au == 0 and av == 0:
                 continue
             tot = np.float32(s + np.float32(lam * uv_bits(ym, m, au, av, cfl_ok, ang_ok)))
@@ -264,3 +317,5 @@ def test_chroma_decisions_are_the_argmin_over_candidates():
         assert (int(nd["cfl_alpha_u"]), int(nd["cfl_alpha_v"])) == ba
         checked += 1
     assert checked >= 4
+    print(f"{checked} blocks, {moved} walked alphas")
+    assert moved > 0 or not steps
