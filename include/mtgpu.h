@@ -676,6 +676,43 @@ int mt_deblock_frame(const mt_deblock_geom* geom, const uint16_t* src_y, const u
                      const uint16_t* src_v, int32_t y_stride, int32_t uv_stride, uint16_t* dst_y,
                      uint16_t* dst_u, uint16_t* dst_v, int lvl_y, int lvl_u, int lvl_v, uint32_t* status_d,
                      void* stream);
+/* ---- CDEF (src/cdef.rs, src/coder/frame_cdef.rs) ------------------------------------ */
+typedef struct {
+    int32_t on;        /* 1: the frame is filtered (enable_cdef with these entries) */
+    int32_t bits;      /* cdef_bits (0..3) */
+    int32_t damping;   /* the signalled damping, 3 + min(qidx / 64, 3) */
+    int32_t n;         /* 1 << bits strength entries */
+    uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];  /* signalled strengths per entry */
+} mt_cdef_params;
+typedef struct {
+    int32_t width, height;  /* luma size, multiples of 8 (the coded area is the frame) */
+    uint8_t bd, chroma;     /* chroma: MT_RDO_LUMA_ONLY, MT_RDO_CHROMA_420, MT_RDO_CHROMA_444 */
+    uint8_t slow;           /* 1: Speed::Slow's two-stage candidate pairs, 0: the full list */
+    uint8_t pad0;
+    int32_t qidx;           /* base_q_idx: the damping */
+    float wsse_scale;       /* > 0: the decision distortion is the scaled weighted SSE */
+    float lambda;           /* rate multiplier of the header and cdef_idx bits (the frame's
+                               lambda_d, mt_rdo_lambda) */
+    const uint16_t *rec_y, *rec_u, *rec_v;  /* device: the deblocked reconstruction */
+    int32_t y_stride, uv_stride;            /* of rec_* and dst_* */
+    const uint16_t *src_y, *src_u, *src_v;  /* device: the source */
+    int32_t src_y_stride, src_uv_stride;
+    const uint8_t* skip8;   /* device: mt_av1_skip8's map */
+    uint16_t *dst_y, *dst_u, *dst_v;        /* device: the filtered planes (not the inputs) */
+} mt_cdef_frame_args;
+/* frame_cdef (src/coder/frame_cdef.rs:49) on the GPU: per luma 8x8 the direction and
+ * variance (src/cdef.rs:103-179), the distortion of all twelve signalled strength pairs
+ * (primary 0, 1, 2, 4 x secondary 0, 1, 2) per 64x64 unit for luma and for U + V, the
+ * frame decision on those tables (the directional-variance gate, Slow / Fast candidate
+ * pairs, greedy strength set, mildest entry clearing a 40 per-mille margin, cdef_bits on
+ * rate; shipped tuning, src/tuning.rs:772-776), and the filter of every non-skip 8x8 from
+ * the pre-CDEF planes into dst (copied where nothing is filtered).  prm: the header values;
+ * grid (host, optional): each 64x64 unit's entry, raster order; tables (host, optional):
+ * [2][units][12] int64, luma then U + V.  Synchronizes the stream (the decision runs on
+ * the host between the table and filter launches). */
+int mt_cdef_frame(const mt_cdef_frame_args* args, mt_cdef_params* prm, uint8_t* grid, int64_t* tables,
+                  void* stream);
+
 /* loop_filter_levels / loop_filter_sharpness (src/obu.rs:306-334) at the shipped tuning:
  * lvl_y = round(qidx / 8) (<= 40); lvl_uv = max(round(qidx / 10 * s), 1) (<= 32) while lvl_y
  * > 0, with s = 0.5 at 4:2:0 (sub 2) and 1 otherwise; sharpness 7. */
@@ -747,7 +784,7 @@ int mt_pnet_prune_masks(const float* logits_d, int batch, const int* topk5, cons
  * frame (specification 5.9-5.11) with one tile, static frame-initial CDFs
  * (disable_cdf_update = 1, the CDFs the RDO prices against), TX_MODE_LARGEST (every block's
  * transform is its size; 4:4:4 64x64 chroma in four 32x32), DCT_DCT luma and the decoder-
- * derived chroma types, no CDEF / restoration / segmentation, and the reference's luma DC
+ * derived chroma types, CDEF as given, no restoration / segmentation, the reference's luma DC
  * offset DeltaQYDc = -8 (src/obu.rs:443-451).  The reference writes the same syntax with its
  * own decisions (src/obu.rs:396-528, src/coder/tile_encode.rs). */
 typedef struct {
@@ -771,7 +808,20 @@ typedef struct {
     const int32_t* coef_u;
     const int32_t* coef_v;
     int32_t coef_stride, coef_uv_stride;
+    /* CDEF (5.9.19 cdef_params, 5.11.56 read_cdef), as mt_cdef_frame decided it: cdef_on 0
+     * codes enable_cdef = 0; else the damping (3..6), cdef_bits (0..3), the 1 << cdef_bits
+     * strength entries (primary 0..15, secondary 0, 1, 2 or 4; uv ignored for monochrome)
+     * and, with cdef_bits > 0, each superblock's entry (raster order) coded at its first
+     * non-skip block. */
+    uint8_t cdef_on, cdef_damping, cdef_bits, cdef_pad;
+    uint8_t cdef_y_pri[8], cdef_y_sec[8], cdef_uv_pri[8], cdef_uv_sec[8];
+    const uint8_t* cdef_grid;   /* [sb_rows][sb_cols] entry indices; NULL with cdef_bits 0 */
 } mt_av1_frame;
+
+/* The luma 8x8 skip map of a frame's decisions (host, [height / 8][width / 8]): 1 where
+ * every block covering the 8x8 codes no coefficient in any plane (the AV1 skip flag; CDEF
+ * leaves such 8x8s alone, specification 7.15). */
+int mt_av1_skip8(const mt_av1_frame* f, uint8_t* skip8);
 
 /* The frame's sequence header OBU and frame OBU (an AV1 temporal unit without its temporal
  * delimiter).  out may be NULL to ask for the size (*len); a smaller cap is MT_ERR_ARG with

@@ -19,7 +19,11 @@ class MtAv1Frame(ctypes.Structure):
                 ("filter_intra", ctypes.c_uint8), ("lf_level", ctypes.c_uint8 * 4), ("lf_sharpness", ctypes.c_uint8),
                 ("cicp", ctypes.c_uint8 * 3), ("full_range", ctypes.c_uint8), ("pad", ctypes.c_uint8 * 3),
                 ("nodes", ctypes.c_void_p), ("coef_y", ctypes.c_void_p), ("coef_u", ctypes.c_void_p),
-                ("coef_v", ctypes.c_void_p), ("coef_stride", ctypes.c_int32), ("coef_uv_stride", ctypes.c_int32)]
+                ("coef_v", ctypes.c_void_p), ("coef_stride", ctypes.c_int32), ("coef_uv_stride", ctypes.c_int32),
+                ("cdef_on", ctypes.c_uint8), ("cdef_damping", ctypes.c_uint8), ("cdef_bits", ctypes.c_uint8),
+                ("cdef_pad", ctypes.c_uint8), ("cdef_y_pri", ctypes.c_uint8 * 8), ("cdef_y_sec", ctypes.c_uint8 * 8),
+                ("cdef_uv_pri", ctypes.c_uint8 * 8), ("cdef_uv_sec", ctypes.c_uint8 * 8),
+                ("cdef_grid", ctypes.c_void_p)]
 
 
 SRGB_IDENTITY = (1, 13, 0)   # BT.709 primaries, sRGB transfer, identity matrix (4:4:4 only)
@@ -33,6 +37,7 @@ def _sigs():
         vp, sz = ctypes.c_void_p, ctypes.c_size_t
         for f in ("mt_av1_encode", "mt_avif_encode", "mt_av1_tokens"):
             getattr(L, f).argtypes = [ctypes.POINTER(MtAv1Frame), vp, sz, ctypes.POINTER(sz)]
+        L.mt_av1_skip8.argtypes = [ctypes.POINTER(MtAv1Frame), vp]
         L.mt_av1_range_code.argtypes = [vp, sz, vp, sz, ctypes.POINTER(sz)]
         L.mt_av1_cdf.argtypes = [ctypes.c_uint32, ctypes.c_int, vp]
         L._av1_sigs = True
@@ -40,9 +45,12 @@ def _sigs():
 
 
 def frame(nodes, levels, width, height, bd, qidx, chroma=LUMA_ONLY, qm_level=15, uv_qm_level=15, uv_dc=0,
-          uv_ac=0, filter_intra=True, lf_levels=(0, 0, 0, 0), lf_sharpness=0, cicp=UNSPECIFIED, full_range=True):
+          uv_ac=0, filter_intra=True, lf_levels=(0, 0, 0, 0), lf_sharpness=0, cicp=UNSPECIFIED, full_range=True,
+          cdef=None, cdef_grid=None):
     """An mt_av1_frame over host arrays: nodes (NODE_DTYPE records, (nsb, 341) or flat), levels
-    = (luma, u, v) int32 planes (u, v None for luma only).  Returns (struct, keep-alive list)."""
+    = (luma, u, v) int32 planes (u, v None for luma only); cdef: a CDEF parameter dict (the
+    params of cdef.frame or the oracle's) with its per-superblock grid.  Returns (struct,
+    keep-alive list)."""
     nodes = np.asarray(nodes)
     if nodes.dtype.names and nodes.dtype != NODE_DTYPE:  # another record layout with the same fields
         conv = np.zeros(nodes.size, NODE_DTYPE)
@@ -70,6 +78,15 @@ def frame(nodes, levels, width, height, bd, qidx, chroma=LUMA_ONLY, qm_level=15,
         f.coef_u = keep[2].ctypes.data
         f.coef_v = keep[3].ctypes.data
         f.coef_uv_stride = keep[2].shape[1]
+    if cdef is not None and cdef.get("on", 1):
+        f.cdef_on, f.cdef_damping, f.cdef_bits = 1, cdef["damping"], cdef["bits"]
+        for i in range(1 << cdef["bits"]):
+            f.cdef_y_pri[i], f.cdef_y_sec[i] = cdef["y_pri"][i], cdef["y_sec"][i]
+            f.cdef_uv_pri[i], f.cdef_uv_sec[i] = cdef["uv_pri"][i], cdef["uv_sec"][i]
+        if cdef["bits"]:
+            g = np.ascontiguousarray(cdef_grid, np.uint8)
+            keep.append(g)
+            f.cdef_grid = g.ctypes.data
     return f, keep
 
 
@@ -119,3 +136,10 @@ def token_cdf(tok):
     out = np.zeros(nsym - 1, np.uint16)
     _check(L.mt_av1_cdf(int(tok) >> 8, nsym, out.ctypes.data))
     return out, nsym
+
+
+def skip8(f):
+    """mt_av1_skip8: the frame's luma 8x8 skip map (uint8, (height / 8, width / 8))."""
+    out = np.zeros((f.height // 8, f.width // 8), np.uint8)
+    _check(_sigs().mt_av1_skip8(ctypes.byref(f), out.ctypes.data))
+    return out

@@ -16,7 +16,7 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = [os.path.join(PKG, "csrc", f) for f in ("mtgpu.hip", "mt_rate.hip", "mt_pred.hip", "mt_dist.hip", "mt_eval.hip", "mt_rdo.hip", "mt_nn.hip", "mt_encode.hip", "mt_deblock.hip", "mt_price.hip", "mt_av1enc.cpp")]
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("mtgpu.hip", "mt_rate.hip", "mt_pred.hip", "mt_dist.hip", "mt_eval.hip", "mt_rdo.hip", "mt_nn.hip", "mt_encode.hip", "mt_deblock.hip", "mt_price.hip", "mt_cdef.hip", "mt_av1enc.cpp")]
 
 
 def _compile(src):

@@ -339,6 +339,7 @@ struct SbCoder {
     const Tabs& T;
     std::vector<uint32_t>& tok;
     int sb, r0, c0;  // superblock index and origin (mi)
+    bool cdef_done = false;  // the SB's cdef_idx is coded
 
     void sym(uint32_t off, int nsym, int s) { tok.push_back(tok_sym(off, nsym, s)); }
     void lit(int v) { tok.push_back(tok_bool(16384, v)); }
@@ -407,6 +408,12 @@ struct SbCoder {
         const int sctx = (r > 0 ? F.mi_skip[(size_t)(r - 1) * F.mi_cols + c] : 0) +
                          (c > 0 ? F.mi_skip[(size_t)r * F.mi_cols + c - 1] : 0);
         sym(T.skip + sctx, 2, skip);
+        // read_cdef: the SB's strength entry at its first non-skip block
+        if (F.f->cdef_on && !skip && !cdef_done) {
+            const int e = F.f->cdef_bits ? F.f->cdef_grid[sb] : 0;
+            for (int b = F.f->cdef_bits - 1; b >= 0; --b) lit((e >> b) & 1);
+            cdef_done = true;
+        }
         // y mode, angle, chroma, filter intra
         const int ym = id_ymode(nd.mode), fi = id_filter(nd.mode);
         const int am = r > 0 ? F.mi_ymode[(size_t)(r - 1) * F.mi_cols + c] : 0;
@@ -673,6 +680,23 @@ void build_maps(Frame& F) {
         }
 }
 
+// the luma 8x8 skip map: an 8x8 is skip when every 4x4 (mi) in it is
+int skip8_map(const mt_av1_frame* f, uint8_t* out) {
+    Frame F;
+    int rc = build_frame(f, F);
+    if (rc) return rc;
+    build_maps(F);
+    const int nbx = f->width / 8, nby = f->height / 8;
+    for (int by = 0; by < nby; ++by)
+        for (int bx = 0; bx < nbx; ++bx) {
+            int all = 1;
+            for (int r = 2 * by; r < 2 * by + 2; ++r)
+                for (int c = 2 * bx; c < 2 * bx + 2; ++c) all &= F.mi_skip[(size_t)r * F.mi_cols + c];
+            out[(size_t)by * nbx + bx] = (uint8_t)all;
+        }
+    return MT_OK;
+}
+
 int gen_tokens(const mt_av1_frame* f, Frame& F, std::vector<uint32_t>& all, int nthreads) {
     int rc = build_frame(f, F);
     if (rc) return rc;
@@ -742,7 +766,7 @@ std::vector<uint8_t> sequence_header(const mt_av1_frame* f) {
     w.flag(f->filter_intra != 0);  // enable_filter_intra
     w.flag(true);                  // enable_intra_edge_filter
     w.flag(false);                 // enable_superres
-    w.flag(false);                 // enable_cdef
+    w.flag(f->cdef_on != 0);       // enable_cdef
     w.flag(false);                 // enable_restoration
     // color_config
     const bool hbd = f->bd > 8;
@@ -813,6 +837,19 @@ void frame_header(const mt_av1_frame* f, int sb_cols, int sb_rows, BitWriter& w)
     }
     w.put(f->lf_sharpness, 3);
     w.flag(false);  // loop_filter_delta_enabled
+    if (f->cdef_on) {  // cdef_params
+        w.put((uint32_t)f->cdef_damping - 3, 2);
+        w.put(f->cdef_bits, 2);
+        auto sec_code = [](int s) { return s == 4 ? 3u : (uint32_t)s; };
+        for (int i = 0; i < (1 << f->cdef_bits); ++i) {
+            w.put(f->cdef_y_pri[i], 4);
+            w.put(sec_code(f->cdef_y_sec[i]), 2);
+            if (!mono) {
+                w.put(f->cdef_uv_pri[i], 4);
+                w.put(sec_code(f->cdef_uv_sec[i]), 2);
+            }
+        }
+    }
     w.flag(false);  // tx_mode_select: TX_MODE_LARGEST
     w.flag(false);  // reduced_tx_set
 }
@@ -824,6 +861,14 @@ int check_frame_args(const mt_av1_frame* f) {
         return mtlib::set_error(MT_ERR_ARG, "mt_av1: bad quantizer or filter parameters");
     for (int i = 0; i < 4; ++i)
         if (f->lf_level[i] > 63) return mtlib::set_error(MT_ERR_ARG, "mt_av1: loop filter level above 63");
+    if (f->cdef_on) {
+        if (f->cdef_damping < 3 || f->cdef_damping > 6 || f->cdef_bits > 3 || (f->cdef_bits && !f->cdef_grid))
+            return mtlib::set_error(MT_ERR_ARG, "mt_av1: bad CDEF parameters");
+        for (int i = 0; i < (1 << f->cdef_bits); ++i)
+            if (f->cdef_y_pri[i] > 15 || f->cdef_uv_pri[i] > 15 || f->cdef_y_sec[i] == 3 || f->cdef_y_sec[i] > 4 ||
+                f->cdef_uv_sec[i] == 3 || f->cdef_uv_sec[i] > 4)
+                return mtlib::set_error(MT_ERR_ARG, "mt_av1: bad CDEF strength");
+    }
     if (f->cicp[2] == 0 && f->chroma != MT_RDO_CHROMA_444)
         return mtlib::set_error(MT_ERR_ARG, "mt_av1: the identity matrix needs 4:4:4");
     return MT_OK;
@@ -993,6 +1038,13 @@ extern "C" int mt_av1_tokens(const mt_av1_frame* f, uint32_t* tokens, size_t cap
     if (cap < toks.size()) return mtlib::set_error(MT_ERR_ARG, "mt_av1_tokens: token buffer too small");
     std::memcpy(tokens, toks.data(), toks.size() * sizeof(uint32_t));
     return MT_OK;
+}
+
+extern "C" int mt_av1_skip8(const mt_av1_frame* f, uint8_t* skip8) {
+    int rc = check_frame_args(f);
+    if (rc) return rc;
+    if (!skip8) return mtlib::set_error(MT_ERR_ARG, "mt_av1_skip8: no output");
+    return skip8_map(f, skip8);
 }
 
 extern "C" int mt_av1_range_code(const uint32_t* tokens, size_t n, uint8_t* out, size_t cap, size_t* len) {

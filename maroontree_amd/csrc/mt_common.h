@@ -13,6 +13,8 @@ namespace mtlib {
 // Library-owned workspaces are kept per (device, stream): calls on different streams never
 // share one, and the calls on one stream are ordered by it.
 using StreamKey = std::pair<int, uintptr_t>;
+// WSSE_DCT4 and the normalised WSSE_WEIGHTS (src/rd_sse.rs:1402-1420) as f32 host tables
+void wsse_tables(float wd[4][4], float ww[4][4]);
 inline StreamKey stream_key(int dev, hipStream_t s) { return {dev, reinterpret_cast<uintptr_t>(s)}; }
 // Records the calling thread's last error message and returns `code`.
 int set_error(int code, const std::string& what);

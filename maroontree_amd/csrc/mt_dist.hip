@@ -432,15 +432,25 @@ __global__ void __launch_bounds__(256) dist_classify_kernel(const mt_distblk* __
 }  // namespace
 
 namespace mtlib {
-int dist_init_device(int /*device*/) {
+void wsse_tables(float wd[4][4], float ww[4][4]) {
     // WSSE_DCT4: the f32 literals of the reference; WSSE_WEIGHTS: the raw table divided by
     // its mean (0.4625) in f32, as the reference's const evaluation rounds it.
     const float a = 0.5f, bb = 0.6532815f, c = 0.27059805f;
-    const float wd[4][4] = {{a, a, a, a}, {bb, c, -c, -bb}, {a, -a, -a, a}, {c, -bb, bb, -c}};
+    const float d[4][4] = {{a, a, a, a}, {bb, c, -c, -bb}, {a, -a, -a, a}, {c, -bb, bb, -c}};
     volatile float mean = 0.4625f;
     const float dc = 1.8f / mean, diag = 0.8f / mean, axis = 0.4f / mean, hf = 0.2f / mean,
                 ac1 = 1.0f / mean;
-    const float ww[4][4] = {{dc, ac1, axis, hf}, {ac1, diag, hf, hf}, {axis, hf, hf, hf}, {hf, hf, hf, hf}};
+    const float w[4][4] = {{dc, ac1, axis, hf}, {ac1, diag, hf, hf}, {axis, hf, hf, hf}, {hf, hf, hf, hf}};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            wd[i][j] = d[i][j];
+            ww[i][j] = w[i][j];
+        }
+}
+
+int dist_init_device(int /*device*/) {
+    float wd[4][4], ww[4][4];
+    wsse_tables(wd, ww);
     MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_wd), wd, sizeof(wd)));
     MT_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_ww), ww, sizeof(ww)));
     return MT_OK;

@@ -206,6 +206,26 @@ int32_t orc_cfl_best_alpha(const int32_t* ac, int ac_stride, const uint16_t* src
 int32_t orc_cfl_alpha_step(int32_t a0, int32_t s);
 int32_t orc_cfl_alpha_rd_refine(int32_t a0, float c0, int steps, const float* cost);
 
+/* ---- CDEF (src/cdef.rs, src/coder/frame_cdef.rs) ------------------------------------ */
+typedef struct {
+    int32_t bits, damping, n;  /* cdef_bits, the signalled damping (3..6), 1 << bits entries */
+    uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];
+} orc_cdef_params;
+int orc_cdef_dir(const uint16_t* p, int stride, int w, int h, int x, int y, int bd, int32_t* var);
+int32_t orc_cdef_adjust_pri(int32_t pri, int32_t var);
+void orc_cdef_filter(const uint16_t* p, int stride, int w, int h, int x, int y, int bw, int bh, int pri, int sec,
+                     int dir, int damping, int bd, uint16_t* dst, int dstride);
+void orc_cdef_candidates(const uint16_t* p, int stride, int w, int h, int x, int y, int bw, int bh, int dir,
+                         int32_t var, int damping, int bd, int32_t* out);
+/* frame_cdef: decision and apply in place (planes w x h, chroma (w >> sub_x) x (h >> sub_y),
+ * w and h multiples of 8).  skip8: [h/8][w/8] luma 8x8 skip flags.  Returns 1 when CDEF is
+ * applied, 0 when off, -1 on bad arguments.  Optional outputs: grid [units], dirs / vars
+ * [h/8][w/8], tables [2][units][12] (luma, then U + V). */
+int orc_cdef_frame(uint16_t* rec_y, uint16_t* rec_u, uint16_t* rec_v, const uint16_t* src_y, const uint16_t* src_u,
+                   const uint16_t* src_v, int w, int h, int sub_x, int sub_y, int mono, const uint8_t* skip8,
+                   int qidx, int bd, int slow, float wsse_scale, float lambda, orc_cdef_params* prm,
+                   uint8_t* grid_out, int32_t* dirs_out, int32_t* vars_out, int64_t* tables_out);
+
 /* ---- residual and distortion (src/rd_sse.rs, src/partition_rd.rs) ----------------- */
 void orc_residual_pred(const uint16_t* src, int stride, const int32_t* pred, int w, int h,
                        int32_t* dst);

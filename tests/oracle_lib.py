@@ -757,3 +757,85 @@ def cfl_alpha_rd_refine(a0, c0, steps, cost_by_alpha):
     c = np.ascontiguousarray(cost_by_alpha, np.float32)
     assert c.shape == (33,)
     return int(L.orc_cfl_alpha_rd_refine(int(a0), float(c0), int(steps), c.ctypes.data))
+
+
+class OrcCdefParams(ctypes.Structure):
+    _fields_ = [("bits", ctypes.c_int32), ("damping", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("y_pri", ctypes.c_uint8 * 8), ("y_sec", ctypes.c_uint8 * 8), ("uv_pri", ctypes.c_uint8 * 8),
+                ("uv_sec", ctypes.c_uint8 * 8)]
+
+
+def _cdef_sigs(L):
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    L.orc_cdef_dir.restype = i32
+    L.orc_cdef_dir.argtypes = [vp, i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int32)]
+    L.orc_cdef_adjust_pri.restype = ctypes.c_int32
+    L.orc_cdef_adjust_pri.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    L.orc_cdef_filter.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, vp, i32]
+    L.orc_cdef_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_int32, i32, i32, vp]
+    L.orc_cdef_frame.restype = i32
+    L.orc_cdef_frame.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, i32, i32, i32, ctypes.c_float,
+                                 ctypes.c_float, ctypes.POINTER(OrcCdefParams), vp, vp, vp, vp]
+
+
+def cdef_dir(plane, x, y, bd):
+    L = lib()
+    _cdef_sigs(L)
+    p = np.ascontiguousarray(plane, np.uint16)
+    v = ctypes.c_int32()
+    d = L.orc_cdef_dir(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], x, y, bd, ctypes.byref(v))
+    return d, v.value
+
+
+def cdef_filter(plane, x, y, bw, bh, pri, sec, direction, damping, bd):
+    """The block filtered at effective strengths into a copy of the plane."""
+    L = lib()
+    _cdef_sigs(L)
+    p = np.ascontiguousarray(plane, np.uint16)
+    out = p.copy()
+    L.orc_cdef_filter(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], x, y, bw, bh, pri, sec, direction, damping,
+                      bd, out.ctypes.data, p.shape[1])
+    return out
+
+
+def cdef_candidates(plane, x, y, bw, bh, direction, var, damping, bd):
+    L = lib()
+    _cdef_sigs(L)
+    p = np.ascontiguousarray(plane, np.uint16)
+    out = np.zeros((12, bh, bw), np.int32)
+    L.orc_cdef_candidates(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], x, y, bw, bh, direction, var, damping,
+                          bd, out.ctypes.data)
+    return out
+
+
+def cdef_frame(rec, src, skip8, chroma, bd, qidx, lam, wsse_scale=0.0, slow=True):
+    """orc_cdef_frame on copies of rec = (y, u, v) (u, v None for luma only).  Returns
+    (planes, params dict (with 'on'), grid, dirs, vars, tables (2, units, 12))."""
+    L = lib()
+    _cdef_sigs(L)
+    mono = chroma == 0
+    ry = np.ascontiguousarray(rec[0], np.uint16).copy()
+    h, w = ry.shape
+    ru = None if mono else np.ascontiguousarray(rec[1], np.uint16).copy()
+    rv = None if mono else np.ascontiguousarray(rec[2], np.uint16).copy()
+    sy = np.ascontiguousarray(src[0], np.uint16)
+    su = None if mono else np.ascontiguousarray(src[1], np.uint16)
+    sv = None if mono else np.ascontiguousarray(src[2], np.uint16)
+    sk = np.ascontiguousarray(skip8, np.uint8)
+    nu = ((w + 63) // 64) * ((h + 63) // 64)
+    grid = np.zeros(nu, np.uint8)
+    dirs = np.zeros((h // 8, w // 8), np.int32)
+    vars_ = np.zeros((h // 8, w // 8), np.int32)
+    tabs = np.zeros((2, nu, 12), np.int64)
+    prm = OrcCdefParams()
+    sub = 1 if chroma == 1 else 0
+    d = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    on = L.orc_cdef_frame(ry.ctypes.data, d(ru), d(rv), sy.ctypes.data, d(su), d(sv), w, h, sub, sub, 1 if mono else 0,
+                          sk.ctypes.data, qidx, bd, 1 if slow else 0, wsse_scale, lam, ctypes.byref(prm),
+                          grid.ctypes.data, dirs.ctypes.data, vars_.ctypes.data, tabs.ctypes.data)
+    if on < 0:
+        raise ValueError("orc_cdef_frame: bad arguments")
+    n = prm.n
+    p = {"on": on, "bits": prm.bits, "damping": prm.damping, "n": n, "y_pri": list(prm.y_pri[:n]),
+         "y_sec": list(prm.y_sec[:n]), "uv_pri": list(prm.uv_pri[:n]), "uv_sec": list(prm.uv_sec[:n])}
+    return (ry, ru, rv), p, grid, dirs, vars_, tabs
