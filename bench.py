@@ -77,7 +77,7 @@ def main():
                     help="time only the 4:2:0 RDO (not the luma-only RDO beside it)")
     ap.add_argument("--rdo-only", action="store_true",
                     help="profiling aid: run only the whole-frame RDO leg and print its object")
-    ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking-filter leg")
+    ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking-filter and CDEF legs")
     ap.add_argument("--deblock-only", action="store_true",
                     help="run only the deblocking-filter leg and print its object as the JSON line")
     ap.add_argument("--deblock-steps", type=int, default=50)
@@ -149,7 +149,10 @@ def main():
         print(json.dumps(bench_learned(args, dev, torch, mt)), flush=True)
         return
     if args.deblock_only:
-        print(json.dumps(bench_deblock(args, dev, torch, mt)), flush=True)
+        prep = {}
+        r = bench_deblock(args, dev, torch, mt, prep)
+        r["cdef"] = bench_cdef(args, dev, torch, mt, prep)
+        print(json.dumps(r), flush=True)
         return
     if args.stream_only:
         print(json.dumps(bench_stream(args, dev, torch, mt)), flush=True)
@@ -313,7 +316,9 @@ def main():
     if not args.no_rdo:
         line["plain_512"] = bench_plain512(args, dev, torch, mt)
     if not args.no_deblock:
-        line["deblock"] = bench_deblock(args, dev, torch, mt)
+        prep = {}
+        line["deblock"] = bench_deblock(args, dev, torch, mt, prep)
+        line["cdef"] = bench_cdef(args, dev, torch, mt, prep)
     net = None
     if not args.no_learned:
         line["learned"], net = bench_learned(args, dev, torch, mt, return_net=True)
@@ -694,7 +699,7 @@ def bench_rdo(args, dev, torch, mt):
     return r
 
 
-def bench_deblock(args, dev, torch, mt):
+def bench_deblock(args, dev, torch, mt, prep=None):
     """The in-loop deblocking filter of the RDO leg's frame: the exact intra RDO (Fast flags)
     of the seeded 3840x2160 10-bit 4:2:0 frame at qindex 91 runs once, untimed; its decisions
     give the deblock maps (mt_rdo_deblock_maps) and the level law gives luma 11, chroma 5,
@@ -713,9 +718,10 @@ def bench_deblock(args, dev, torch, mt):
              "recon": torch.empty((height, width), dtype=torch.int16, device=dev),
              "recon_u": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
              "recon_v": torch.empty((height // 2, width // 2), dtype=torch.int16, device=dev),
-             "nodes": torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)}
+             "nodes": torch.empty(nsb * R.NODES * R.NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev),
+             "levels": True}
     stream = torch.cuda.current_stream()
-    R.intra_rdo_batch([frame], flags, stream=stream)
+    rdo_out = R.intra_rdo_batch([frame], flags, stream=stream)[0]
     maps = D.rdo_maps(frame["nodes"], width, height, stream=stream)
     ly, luv, sharp = D.levels(qidx, 2)
     src = (frame["recon"], frame["recon_u"], frame["recon_v"])
@@ -779,6 +785,96 @@ def bench_deblock(args, dev, torch, mt):
         r["cpu_baseline"] = {"value": round(npx / best / 1e6, 2), "unit": "Mpx/s", "kind": "port",
                              "sample": f"the whole frame (luma + chroma), best of 3 ({best * 1000:.1f} ms), "
                                        "the oracle's sequential per-superblock-row order, one thread",
+                             "cores": 1}
+        r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
+    if prep is not None:
+        prep.update(src=(frame["src"], frame["src_u"], frame["src_v"]), deblocked=out, nodes=rdo_out[1],
+                    levels=rdo_out[2]["levels"], lf=(ly, luv, sharp), stream=stream, width=width, height=height,
+                    bd=bd, qidx=qidx, flags=flags)
+    return r
+
+
+def bench_cdef(args, dev, torch, mt, prep):
+    """CDEF of the deblock leg's frame (the deblocked reconstruction of the exact intra RDO of
+    the seeded 3840x2160 10-bit 4:2:0 frame at qindex 91): the luma 8x8 skip map of its
+    decisions (mt_av1_skip8, on the host, untimed), then one step = one mt_cdef_frame call
+    (directions, the twelve strength pairs' unit tables for luma and U + V, the decision on
+    the host, the filter of the three planes), Speed::Slow candidates, plain SSE (the RDO's
+    frames price plain SSE), lambda_d of the frame; timed with HIP events around the call."""
+    from maroontree_amd import av1
+    from maroontree_amd import cdef as C
+    from maroontree_amd import rdo as R
+    width, height, bd, qidx = prep["width"], prep["height"], prep["bd"], prep["qidx"]
+    stream = prep["stream"]
+    ly, luv, sharp = prep["lf"]
+    lv = [None if a is None else a.cpu().numpy() for a in prep["levels"]]
+    dc, ac, uqm = R.chroma_laws(qidx, 2)
+    f, keep = av1.frame(R.nodes_to_np(prep["nodes"]), lv, width, height, bd, qidx, chroma=R.CHROMA_420,
+                        qm_level=mt.qm_level_law(qidx, 2), uv_qm_level=uqm, uv_dc=dc, uv_ac=ac,
+                        lf_levels=(ly, ly, luv, luv), lf_sharpness=sharp)
+    skip = av1.skip8(f)
+    skip_d = torch.from_numpy(skip).to(dev)
+    lam = R.rdo_lambda(qidx, bd, 2)
+    rec, src = prep["deblocked"], prep["src"]
+    out = tuple(torch.empty_like(p) for p in rec)
+
+    def step():
+        return C.frame(rec, src, skip_d, R.CHROMA_420, bd, qidx, lam, wsse_scale=0.0, slow=True, out=out,
+                       stream=stream)
+
+    for _ in range(max(args.warmup, 1)):
+        _, prm, grid = step()
+    torch.cuda.synchronize()
+    n = max(1, args.deblock_steps // 5)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / n
+    k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    npx = width * height
+    samples = npx + 2 * (npx // 4)
+    nb = npx // 64
+    # the table pass reads the reconstruction and the source, the filter pass reads the
+    # reconstruction and writes the output (uint16), plus the skip map twice
+    alg_bytes = 8 * samples + 2 * nb
+    ach = alg_bytes / (k_ms / 1000.0) / 1e9
+    print(f"[bench] cdef 4K 4:2:0: {k_ms * 1000:.1f} us/frame (wall {wall_ms * 1000:.1f} us), {ach:.0f} GB/s, "
+          f"{prm}", file=sys.stderr, flush=True)
+    r = {
+        "workload": "CDEF (frame_cdef: direction search, the twelve strength pairs' per-unit distortion tables, "
+                    "the Slow decision, the filter) of the deblocked reconstruction of the exact intra RDO of a "
+                    "seeded 3840x2160 10-bit 4:2:0 synthetic frame at qindex 91; skip map of its decisions; out of "
+                    "place",
+        "us_per_frame": round(k_ms * 1000.0, 2),
+        "wall_us_per_frame": round(wall_ms * 1000.0, 2),
+        "mpx_per_s": round(npx / (k_ms / 1000.0) / 1e6, 1),
+        "skip_8x8_fraction": round(float(skip.mean()), 4),
+        "decision": {k: prm[k] for k in ("on", "bits", "damping", "y_pri", "y_sec", "uv_pri", "uv_sec")},
+        "roofline": {"kernel": "mt_cdef_frame (all kernels + host decision)", "bound": "hbm", "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": alg_bytes,
+                     "note": "algorithmic bytes: the table pass reads 2 B of reconstruction and 2 B of source per "
+                             "sample, the filter pass reads 2 B and writes 2 B; duration from HIP events around the "
+                             "whole call, which includes the host decision between the table and filter launches"},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        planes = [p.cpu().numpy().view(np.uint16) for p in rec]
+        sp = [p.cpu().numpy().view(np.uint16) for p in src]
+        t0 = time.perf_counter()
+        o_planes, o_prm, o_grid, _, _, _ = O.cdef_frame(planes, sp, skip, R.CHROMA_420, bd, qidx, lam, wsse_scale=0.0,
+                                                        slow=True)
+        cpu_s = time.perf_counter() - t0
+        same = all((o.cpu().numpy().view(np.uint16) == g).all() for o, g in zip(out, o_planes)) and \
+            (o_grid == grid).all()
+        r["matches_oracle"] = bool(same)
+        r["cpu_baseline"] = {"value": round(npx / cpu_s / 1e6, 2), "unit": "Mpx/s", "kind": "port",
+                             "sample": f"the whole frame (luma + chroma), one run ({cpu_s * 1000:.0f} ms), one thread",
                              "cores": 1}
         r["speedup_vs_cpu"] = round(r["mpx_per_s"] / r["cpu_baseline"]["value"], 1)
     return r

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -329,6 +330,42 @@ __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
         const int bx = b % a.nbx, by = b / a.nbx, x = bx * 8, y = by * 8;
         const int u = (by / 8) * a.uc + bx / 8, dir = a.dirs[b];
         int o[12];
+        if (a.scale <= 0.0f) {  // plain SSE: integer sums, reduced across the wave in any order
+            unsigned sq[12], csq[12];
+            {
+                const int px = x + (lane & 7), py = y + (lane >> 3);
+                px12(a.rec[0], px, py, dir, a.vars[b], a.damping, a.bd, o);
+                const int s = a.src[0].p[(size_t)py * a.src[0].stride + px];
+#pragma unroll
+                for (int c = 0; c < 12; ++c) {
+                    const int e = o[c] - s;
+                    sq[c] = (unsigned)(e * e);
+                    csq[c] = 0u;
+                }
+            }
+            if (!a.mono)
+                for (int pl = 1; pl < 3; ++pl)
+                    if (lane < cn) {
+                        const int px = (x >> a.sub_x) + lane % cbw, py = (y >> a.sub_y) + lane / cbw;
+                        px12(a.rec[pl], px, py, dir, -1, a.damping - 1, a.bd, o);
+                        const int s = a.src[pl].p[(size_t)py * a.src[pl].stride + px];
+#pragma unroll
+                        for (int c = 0; c < 12; ++c) csq[c] += (unsigned)((o[c] - s) * (o[c] - s));
+                    }
+            // 64 x 4095^2 < 2^32 per plane; U + V summed in 64-bit below
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                unsigned long long l = sq[c], cc = csq[c];
+                for (int off = 32; off > 0; off >>= 1) {
+                    l += __shfl_xor(l, off);
+                    cc += __shfl_xor(cc, off);
+                }
+                sq[c] = 0u;
+                if (lane == c) atomicAdd((unsigned long long*)&a.ltab[(size_t)u * 12 + c], l);
+                if (!a.mono && lane == 16 + c) atomicAdd((unsigned long long*)&a.ctab[(size_t)u * 12 + c], cc);
+            }
+            continue;
+        }
         {
             const int px = x + (lane & 7), py = y + (lane >> 3);
             px12(a.rec[0], px, py, dir, a.vars[b], a.damping, a.bd, o);
@@ -380,6 +417,23 @@ __global__ void __launch_bounds__(256) cdef_apply_kernel(ApplyArgs a) {
         v = filter_px(a.in, x, y, pe, sec << cs, pri ? a.dirs[b] : 0, a.damping, a.bd);
     }
     a.out[(size_t)y * a.ostride + x] = (uint16_t)v;
+}
+
+// Per 64x64 unit: signalled (some non-skip 8x8) and trusted by the directional-variance gate
+// (mean variance of its non-skip 8x8s >= 15000, src/coder/frame_cdef.rs:163-177).
+__global__ void cdef_units_kernel(const uint8_t* __restrict__ skip8, const int* __restrict__ vars, int nbx, int nby,
+                                  int uc, int nu, uint8_t* __restrict__ flags) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const int ux = (u % uc) * 8, uy = (u / uc) * 8;
+    long long s = 0, n = 0;
+    for (int by = uy; by < min(uy + 8, nby); ++by)
+        for (int bx = ux; bx < min(ux + 8, nbx); ++bx)
+            if (!skip8[by * nbx + bx]) {
+                s += vars[by * nbx + bx];
+                ++n;
+            }
+    flags[u] = (uint8_t)((n > 0 ? 1 : 0) | ((s / (n > 0 ? n : 1) >= 15000) ? 2 : 0));
 }
 
 __global__ void zero_i64_kernel(long long* p, int n) {
@@ -475,46 +529,63 @@ bool decide(const std::vector<long long>& lt, const std::vector<long long>& ct, 
     const int ny = (int)cands.size(), nc = mono ? 1 : ny, npairs = ny * nc;
     std::vector<int> ci(ny);
     for (int i = 0; i < ny; ++i) ci[i] = cand_index(cands[i].first, cands[i].second);
-    auto dpair = [&](int p, int u) {
-        return lt[(size_t)u * 12 + ci[p / nc]] + (mono ? 0 : ct[(size_t)u * 12 + ci[p % nc]]);
-    };
+    // every pair's distortion per unit, pair-major: D[p * nu + u]
+    std::vector<long long> D((size_t)npairs * nu);
+    for (int p = 0; p < npairs; ++p)
+        for (int u = 0; u < nu; ++u)
+            D[(size_t)p * nu + u] = lt[(size_t)u * 12 + ci[p / nc]] + (mono ? 0 : ct[(size_t)u * 12 + ci[p % nc]]);
+    auto dpair = [&](int p, int u) { return D[(size_t)p * nu + u]; };
     std::vector<long long> doff(nu), thr(nu);
-    long long total_off = 0;
+    long long total_off = 0, off_unsig = 0;
+    std::vector<int> su;  // the signalled units
     for (int u = 0; u < nu; ++u) {
         doff[u] = dpair(0, u);
         thr[u] = doff[u] - doff[u] * margin / 1000;
         total_off += doff[u];
+        if (sig[u]) su.push_back(u);
+        else off_unsig += doff[u];
     }
-    long long prefix0 = INT64_MAX;
-    int global_entry = 0;
-    for (int p = 0; p < npairs; ++p) {
-        long long t = 0;
-        for (int u = 0; u < nu; ++u) t += sig[u] ? dpair(p, u) : doff[u];
-        if (t < prefix0) {
-            prefix0 = t;
-            global_entry = p;
-        }
-    }
+    // each pair's total over the signalled units (a few hundred thousand additions per
+    // round: cheaper on one host thread than the cost of starting more); argmins take the
+    // first minimum in pair order
+    auto for_pairs = [&](const std::function<long long(int)>& total, std::vector<long long>& out) {
+        out.assign(npairs, 0);
+        for (int p = 0; p < npairs; ++p) out[p] = total(p);
+    };
+    auto argmin = [](const std::vector<long long>& v, long long* best) {
+        int bi = 0;
+        long long b = INT64_MAX;
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i] < b) {
+                b = v[i];
+                bi = (int)i;
+            }
+        *best = b;
+        return bi;
+    };
+    std::vector<long long> tot;
+    for_pairs([&](int p) {
+        long long t = off_unsig;
+        const long long* d = &D[(size_t)p * nu];
+        for (int u : su) t += d[u];
+        return t;
+    }, tot);
+    long long prefix0;
+    const int global_entry = argmin(tot, &prefix0);
     std::vector<long long> cur = doff;
     std::vector<int> entries{0};
     while (entries.size() < 8) {
-        int best_p = 0;
-        long long best_t = INT64_MAX;
-        for (int p = 0; p < npairs; ++p) {
-            long long t = 0;
-            for (int u = 0; u < nu; ++u) {
-                if (!sig[u]) {
-                    t += doff[u];
-                    continue;
-                }
-                const long long b = std::min(cur[u], dpair(p, u));
+        for_pairs([&](int p) {
+            long long t = off_unsig;
+            const long long* d = &D[(size_t)p * nu];
+            for (int u : su) {
+                const long long b = std::min(cur[u], d[u]);
                 t += b >= thr[u] ? doff[u] : b;
             }
-            if (t < best_t) {
-                best_t = t;
-                best_p = p;
-            }
-        }
+            return t;
+        }, tot);
+        long long best_t;
+        const int best_p = argmin(tot, &best_t);
         entries.push_back(best_p);
         for (int u = 0; u < nu; ++u) cur[u] = std::min(cur[u], dpair(best_p, u));
     }
@@ -664,25 +735,21 @@ extern "C" int mt_cdef_frame(const mt_cdef_frame_args* fa, mt_cdef_params* prm, 
     ta.ctab = wk->tab + (size_t)nu * 12;
     hipLaunchKernelGGL(cdef_tables_kernel, dim3(std::min(nb, cus * 16)), dim3(64), 0, s, ta);
     MT_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(cdef_units_kernel, dim3((nu + 255) / 256), dim3(256), 0, s, fa->skip8, wk->vars, nbx, nby, uc,
+                       nu, wk->ustr);
+    MT_HIP_TRY(hipGetLastError());
     std::vector<long long> tab((size_t)nu * 24);
-    std::vector<int> vars(nb);
-    std::vector<uint8_t> skip(nb);
+    std::vector<uint8_t> uflags(nu);
     MT_HIP_TRY(hipMemcpyAsync(tab.data(), wk->tab, tab.size() * sizeof(long long), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipMemcpyAsync(vars.data(), wk->vars, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-    MT_HIP_TRY(hipMemcpyAsync(skip.data(), fa->skip8, (size_t)nb, hipMemcpyDeviceToHost, s));
+    MT_HIP_TRY(hipMemcpyAsync(uflags.data(), wk->ustr, (size_t)nu, hipMemcpyDeviceToHost, s));
     MT_HIP_TRY(hipStreamSynchronize(s));
     if (tables) std::copy(tab.begin(), tab.end(), tables);
     // signalled units and the directional-variance gate (src/coder/frame_cdef.rs:120-177)
     std::vector<char> sig(nu, 0), trusted(nu, 0);
-    std::vector<long long> dvs(nu, 0), dvc(nu, 0);
-    for (int b = 0; b < nb; ++b)
-        if (!skip[b]) {
-            const int u = (b / nbx / 8) * uc + (b % nbx) / 8;
-            sig[u] = 1;
-            dvs[u] += vars[b];
-            dvc[u] += 1;
-        }
-    for (int u = 0; u < nu; ++u) trusted[u] = dvs[u] / std::max<long long>(dvc[u], 1) >= 15000 ? 1 : 0;
+    for (int u = 0; u < nu; ++u) {
+        sig[u] = (uflags[u] & 1) ? 1 : 0;
+        trusted[u] = (uflags[u] & 2) ? 1 : 0;
+    }
     std::vector<long long> lt(tab.begin(), tab.begin() + (size_t)nu * 12), ct(tab.begin() + (size_t)nu * 12, tab.end());
     *prm = mt_cdef_params{};
     prm->damping = sig_damp;
