@@ -195,7 +195,8 @@ def mutant_libs(tmp_path_factory):
         so = str(path) + ".so"
         subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fwrapv", "-ffp-contract=off", "-shared",
                         "-I", os.path.join(ROOT, "oracle"), "-o", so, str(path),
-                        os.path.join(ROOT, "oracle", "oracle_tx.cpp")], check=True)
+                        os.path.join(ROOT, "oracle", "oracle_tx.cpp"),
+                        os.path.join(ROOT, "oracle", "oracle_rate.cpp")], check=True)  # scans, trellis
         L = ctypes.CDLL(so)
         i32p = ctypes.POINTER(ctypes.c_int32)
         L.orc_inv_txfm.argtypes = [ctypes.c_int, ctypes.c_int, i32p, ctypes.POINTER(O.OrcQuant), i32p]
