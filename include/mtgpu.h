@@ -585,6 +585,14 @@ typedef struct {
         updating_cdf;               /* 1, 1, 1, 0, 0, 1, 1, 1 */
     uint8_t pad[2];
 } mt_encode_config;
+/* The CDEF decision of a frame (mt_cdef_frame, the encoder entry). */
+typedef struct {
+    int32_t on;        /* 1: the frame is filtered (enable_cdef with these entries) */
+    int32_t bits;      /* cdef_bits (0..3) */
+    int32_t damping;   /* the signalled damping, 3 + min(qidx / 64, 3) */
+    int32_t n;         /* 1 << bits strength entries */
+    uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];  /* signalled strengths per entry */
+} mt_cdef_params;
 /* Config switches this library does not run, reported in mt_encode_out.not_applied: the
  * per-superblock quantizer modulation (adaptive_quant / variance_boost / dark_aq), the
  * frame filters (cdef, also implied by Speed::Slow; wiener), palette (screen_content),
@@ -611,6 +619,14 @@ typedef struct {
     uint16_t *dbk_y, *dbk_u, *dbk_v;
     uint8_t lf_level_y, lf_level_u, lf_level_v, lf_sharpness;  /* out: the levels applied */
     uint32_t dbk_status;            /* out: the deblock's MT_DEBLOCK_* status (0) */
+    /* out (device, caller-allocated, optional, with the dbk planes): the reconstruction after
+     * CDEF (frame_cdef, src/coder/frame_cdef.rs:49), run when cdef is set or the speed is
+     * Slow / ExtraSlow (src/avif.rs:449-451), with Slow's two-stage candidate pairs at those
+     * speeds and the full list otherwise, over the skip flags of the frame's coded levels;
+     * NULL skips it (MT_ENC_NA_CDEF) */
+    uint16_t *cdef_y, *cdef_u, *cdef_v;
+    mt_cdef_params cdef;            /* out: the CDEF header values (cdef.on 0: not filtered) */
+    uint8_t* cdef_grid;             /* out (host, optional): each 64x64 unit's entry, raster */
 } mt_encode_out;
 void mt_encode_config_default(mt_encode_config* cfg);
 /* quality_to_q (src/avif.rs:637): quality 1..100 -> base_q_idx 215..1. */
@@ -677,13 +693,6 @@ int mt_deblock_frame(const mt_deblock_geom* geom, const uint16_t* src_y, const u
                      uint16_t* dst_u, uint16_t* dst_v, int lvl_y, int lvl_u, int lvl_v, uint32_t* status_d,
                      void* stream);
 /* ---- CDEF (src/cdef.rs, src/coder/frame_cdef.rs) ------------------------------------ */
-typedef struct {
-    int32_t on;        /* 1: the frame is filtered (enable_cdef with these entries) */
-    int32_t bits;      /* cdef_bits (0..3) */
-    int32_t damping;   /* the signalled damping, 3 + min(qidx / 64, 3) */
-    int32_t n;         /* 1 << bits strength entries */
-    uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];  /* signalled strengths per entry */
-} mt_cdef_params;
 typedef struct {
     int32_t width, height;  /* luma size, multiples of 8 (the coded area is the frame) */
     uint8_t bd, chroma;     /* chroma: MT_RDO_LUMA_ONLY, MT_RDO_CHROMA_420, MT_RDO_CHROMA_444 */

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "../../include/mtgpu.h"
 #include "mt_common.h"
@@ -69,7 +70,21 @@ struct EncWork {
     uint8_t* maps = nullptr;
     size_t maps_bytes = 0;
     uint32_t* status = nullptr;
+    int32_t* coef = nullptr;   // CDEF's skip flags: the RDO's level planes (luma, U, V)
+    size_t coef_n = 0;
+    uint8_t* skip8 = nullptr;
+    size_t skip_n = 0;
 };
+
+template <typename T> int grow_buf(T*& p, size_t& have, size_t n) {
+    if (have >= n) return MT_OK;
+    if (p) MT_HIP_TRY(hipFree(p));
+    p = nullptr;
+    have = 0;
+    MT_HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    have = n;
+    return MT_OK;
+}
 std::mutex g_enc_mu;
 std::map<mtlib::StreamKey, EncWork> g_enc;
 
@@ -185,6 +200,24 @@ extern "C" int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint
     f.uv_qm_level = (uint8_t)uvqm;
     int flags = MT_RDO_ANGLE_DELTAS | MT_RDO_FILTER_INTRA | MT_RDO_MIN4 | MT_RDO_CFL;
     if (cfg->speed <= MT_SPEED_SLOW) flags |= MT_RDO_TRELLIS;
+    // CDEF (on with cdef or at Slow / ExtraSlow, src/avif.rs:449-451) needs the deblocked
+    // planes, its own outputs and the skip flags of the coded levels
+    const bool cdef_asked = cfg->cdef || cfg->speed <= MT_SPEED_SLOW;
+    const bool cdef_run = cdef_asked && out->cdef_y && out->cdef_u && out->cdef_v && out->dbk_y && out->dbk_u &&
+                          out->dbk_v;
+    EncWork* cw_wk = nullptr;
+    const size_t nl = (size_t)width * height, nc = (size_t)cw * chh;
+    if (cdef_run) {
+        int dev = 0;
+        MT_HIP_TRY(hipGetDevice(&dev));
+        if ((rc = enc_work(dev, s, 0, &cw_wk)) != MT_OK) return rc;
+        if ((rc = grow_buf(cw_wk->coef, cw_wk->coef_n, nl + 2 * nc)) != MT_OK) return rc;
+        f.coef = cw_wk->coef;
+        f.coef_u = cw_wk->coef + nl;
+        f.coef_v = cw_wk->coef + nl + nc;
+        f.coef_stride = width;
+        f.coef_uv_stride = cw;
+    }
     rc = mt_intra_rdo(&f, flags, &out->stats, stream);
     if (rc != MT_OK) return rc;
     out->qidx = (uint8_t)q;
@@ -195,7 +228,7 @@ extern "C" int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint
     if (cfg->adaptive_quant) na |= MT_ENC_NA_AQ;
     if (cfg->variance_boost) na |= MT_ENC_NA_VARIANCE_BOOST;
     if (cfg->dark_aq) na |= MT_ENC_NA_DARK_AQ;
-    if (cfg->cdef || cfg->speed <= MT_SPEED_SLOW) na |= MT_ENC_NA_CDEF;
+    if (cdef_asked && !cdef_run) na |= MT_ENC_NA_CDEF;
     if (cfg->wiener) na |= MT_ENC_NA_WIENER;
     if (cfg->screen_content) na |= MT_ENC_NA_PALETTE;
     if (cfg->intrabc) na |= MT_ENC_NA_INTRABC;
@@ -244,6 +277,64 @@ extern "C" int mt_encode_rgb8(const uint8_t* g_d, const uint8_t* b_d, const uint
         out->dbk_status = st;
     } else {
         na |= MT_ENC_NA_DEBLOCK;
+    }
+    out->cdef = mt_cdef_params{};
+    if (cdef_run) {
+        // the skip flags of the coded frame (mt_av1_skip8 over the decisions and levels on the
+        // host), then frame_cdef on the deblocked planes
+        const int nsb = ((width + 63) / 64) * ((height + 63) / 64);
+        std::vector<mt_rdo_node> nodes((size_t)nsb * MT_RDO_NODES);
+        std::vector<int32_t> lv(nl + 2 * nc);
+        MT_HIP_TRY(hipMemcpyAsync(nodes.data(), out->nodes, nodes.size() * sizeof(mt_rdo_node), hipMemcpyDeviceToHost, s));
+        MT_HIP_TRY(hipMemcpyAsync(lv.data(), cw_wk->coef, lv.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        MT_HIP_TRY(hipStreamSynchronize(s));
+        mt_av1_frame af{};
+        af.width = width;
+        af.height = height;
+        af.bd = 8;
+        af.chroma = (uint8_t)f.chroma;
+        af.qidx = (uint8_t)q;
+        af.qm_level = (uint8_t)qm;
+        af.uv_qm_level = (uint8_t)uvqm;
+        af.uv_dc_delta = (int8_t)dc;
+        af.uv_ac_delta = (int8_t)ac;
+        af.filter_intra = 1;
+        af.cicp[0] = af.cicp[1] = af.cicp[2] = 2;
+        af.nodes = nodes.data();
+        af.coef_y = lv.data();
+        af.coef_u = lv.data() + nl;
+        af.coef_v = lv.data() + nl + nc;
+        af.coef_stride = width;
+        af.coef_uv_stride = cw;
+        const size_t nb = (size_t)(width / 8) * (height / 8);
+        std::vector<uint8_t> skip(nb);
+        if ((rc = mt_av1_skip8(&af, skip.data())) != MT_OK) return rc;
+        if ((rc = grow_buf(cw_wk->skip8, cw_wk->skip_n, nb)) != MT_OK) return rc;
+        MT_HIP_TRY(hipMemcpyAsync(cw_wk->skip8, skip.data(), nb, hipMemcpyHostToDevice, s));
+        mt_cdef_frame_args ca{};
+        ca.width = width;
+        ca.height = height;
+        ca.bd = 8;
+        ca.chroma = (uint8_t)f.chroma;
+        ca.slow = cfg->speed <= MT_SPEED_SLOW ? 1 : 0;
+        ca.qidx = q;
+        ca.wsse_scale = f.wsse_scale;
+        ca.lambda = f.lambda;
+        ca.rec_y = out->dbk_y;
+        ca.rec_u = out->dbk_u;
+        ca.rec_v = out->dbk_v;
+        ca.y_stride = width;
+        ca.uv_stride = cw;
+        ca.src_y = out->y;
+        ca.src_u = out->u;
+        ca.src_v = out->v;
+        ca.src_y_stride = width;
+        ca.src_uv_stride = cw;
+        ca.skip8 = cw_wk->skip8;
+        ca.dst_y = out->cdef_y;
+        ca.dst_u = out->cdef_u;
+        ca.dst_v = out->cdef_v;
+        if ((rc = mt_cdef_frame(&ca, &out->cdef, out->cdef_grid, nullptr, stream)) != MT_OK) return rc;
     }
     out->not_applied = na;
     return MT_OK;

@@ -227,6 +227,11 @@ class MtEncodeConfig(ctypes.Structure):
                 ("updating_cdf", ctypes.c_uint8), ("pad", ctypes.c_uint8 * 2)]
 
 
+def _cdef_params_type():
+    from .cdef import MtCdefParams
+    return MtCdefParams
+
+
 class MtEncodeOut(ctypes.Structure):
     _fields_ = [("y", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
                 ("recon_y", ctypes.c_void_p), ("recon_u", ctypes.c_void_p), ("recon_v", ctypes.c_void_p),
@@ -235,7 +240,9 @@ class MtEncodeOut(ctypes.Structure):
                 ("rdo_flags", ctypes.c_int32), ("not_applied", ctypes.c_uint32), ("pad1", ctypes.c_int32),
                 ("dbk_y", ctypes.c_void_p), ("dbk_u", ctypes.c_void_p), ("dbk_v", ctypes.c_void_p),
                 ("lf_level_y", ctypes.c_uint8), ("lf_level_u", ctypes.c_uint8), ("lf_level_v", ctypes.c_uint8),
-                ("lf_sharpness", ctypes.c_uint8), ("dbk_status", ctypes.c_uint32)]
+                ("lf_sharpness", ctypes.c_uint8), ("dbk_status", ctypes.c_uint32),
+                ("cdef_y", ctypes.c_void_p), ("cdef_u", ctypes.c_void_p), ("cdef_v", ctypes.c_void_p),
+                ("cdef", _cdef_params_type()), ("cdef_grid", ctypes.c_void_p)]
 
 
 def encode_config(**overrides):
@@ -251,13 +258,15 @@ def quality_to_qindex(quality):
     return int(lib().mt_quality_to_qindex(int(quality)))
 
 
-def encode_rgb8(g, b, r, cfg=None, stream=None, deblock=True):
+def encode_rgb8(g, b, r, cfg=None, stream=None, deblock=True, cdef=True):
     """mt_encode_rgb8: g, b, r uint8 cuda tensors (H, W) (PlanarImage's plane order), W and H
     multiples of 8.  Returns a dict: y, u, v (the converted planes), recon_y, recon_u,
     recon_v, nodes (uint8 (nsb, 341 * 16)), stats, qidx, qm_level, uv_qm_level, rdo_flags
     and not_applied (the names of the config switches this library does not run); with
     deblock, also dbk_y, dbk_u, dbk_v (the reconstruction after the deblocking filter),
-    lf_levels (y, u, v, sharpness) and dbk_status."""
+    lf_levels (y, u, v, sharpness) and dbk_status; with deblock and cdef, also cdef_y,
+    cdef_u, cdef_v (after CDEF, where the config runs it), cdef (the header values) and
+    cdef_grid."""
     import torch
     cfg = cfg if cfg is not None else encode_config()
     H, W = g.shape
@@ -269,18 +278,22 @@ def encode_rgb8(g, b, r, cfg=None, stream=None, deblock=True):
     t = {k: torch.empty(s, dtype=torch.int16, device=dev)
          for k, s in (("y", (H, W)), ("u", (Hc, Wc)), ("v", (Hc, Wc)), ("recon_y", (H, W)),
                       ("recon_u", (Hc, Wc)), ("recon_v", (Hc, Wc)))
-                      + ((("dbk_y", (H, W)), ("dbk_u", (Hc, Wc)), ("dbk_v", (Hc, Wc))) if deblock else ())}
+                      + ((("dbk_y", (H, W)), ("dbk_u", (Hc, Wc)), ("dbk_v", (Hc, Wc))) if deblock else ())
+                      + ((("cdef_y", (H, W)), ("cdef_u", (Hc, Wc)), ("cdef_v", (Hc, Wc))) if deblock and cdef
+                         else ())}
     nodes = torch.empty(nsb * NODES * NODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     out = MtEncodeOut()
     for k, v in t.items():
         setattr(out, k, _dev_ptr(v).value)
     out.nodes = _dev_ptr(nodes).value
+    grid = np.zeros(nsb, np.uint8)
+    out.cdef_grid = grid.ctypes.data
     _check(lib().mt_encode_rgb8(_dev_ptr(g), _dev_ptr(b), _dev_ptr(r), W, H, ctypes.byref(cfg), ctypes.byref(out),
                                 _stream_ptr(stream)))
     st = out.stats
     res = dict(t)
     res.update(nodes=nodes, qidx=out.qidx, qm_level=out.qm_level, uv_qm_level=out.uv_qm_level,
-               rdo_flags=out.rdo_flags, dbk_status=out.dbk_status,
+               rdo_flags=out.rdo_flags, dbk_status=out.dbk_status, cdef=out.cdef.as_dict(), cdef_grid=grid,
                lf_levels=(out.lf_level_y, out.lf_level_u, out.lf_level_v, out.lf_sharpness),
                not_applied=[n for bit, n in NOT_APPLIED.items() if out.not_applied & bit],
                stats={"bits": st.bits, "sse": st.sse, "candidates": st.candidates, "leaves": st.leaves,

@@ -144,4 +144,7 @@ def test_deblock_levels_and_layouts():
         for sub in (0, 1, 2):
             assert D.levels(q, sub) == O.lf_levels(q, sub), (q, sub)
     assert ctypes.sizeof(D.MtLfPlane) == 72 and ctypes.sizeof(D.MtDeblockGeom) == 72
-    assert ctypes.sizeof(R.MtEncodeOut) == 168 and R.MtEncodeOut.dbk_y.offset == 136
+    assert ctypes.sizeof(R.MtEncodeOut) == 248 and R.MtEncodeOut.dbk_y.offset == 136
+    assert R.MtEncodeOut.cdef_y.offset == 168 and R.MtEncodeOut.cdef.offset == 192
+    from maroontree_amd import av1, cdef
+    assert ctypes.sizeof(av1.MtAv1Frame) == 120 and ctypes.sizeof(cdef.MtCdefParams) == 48
