@@ -6,7 +6,8 @@ import ctypes
 import numpy as np
 
 from ._lib import _check, _dev_ptr, _stream_ptr, init, lib
-from .rdo import LUMA_ONLY, CHROMA_420
+
+LUMA_ONLY, CHROMA_420, CHROMA_444 = 0, 1, 2  # MT_RDO_LUMA_ONLY, MT_RDO_CHROMA_420, _444 (as rdo's)
 
 N_UNITS_MAX_ENTRIES = 8
 

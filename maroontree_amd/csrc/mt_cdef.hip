@@ -112,8 +112,9 @@ __device__ int filter_px(const PlaneV& P, int x, int y, int pri, int sec, int di
 // The twelve signalled pairs of one sample (cdef_block_candidates, src/cdef.rs:651):
 // out[si * 4 + pi] = (CAND_PRI[pi] = 0, 1, 2, 4; CAND_SEC[si] = 0, 1, 2); var < 0: chroma.
 // A zero signalled primary filters the secondary taps along direction 0.
-__device__ void px12(const PlaneV& P, int x, int y, int dir, int var, int damping, int bd, int* out) {
-    const int c = P.p[(size_t)y * P.stride + x];
+// tapf(dx, dy): the sample at that offset from the centre, kVeryLarge outside the plane.
+template <class Tap> __device__ void px12(Tap tapf, int dir, int var, int damping, int bd, int* out) {
+    const int c = tapf(0, 0);
     const int cs = bd - 8, damp = max(damping, 1), maxv = (1 << bd) - 1;
     int pe[3], psh[3], pw0[3], pw1[3];
 #pragma unroll
@@ -133,7 +134,7 @@ __device__ void px12(const PlaneV& P, int x, int y, int dir, int var, int dampin
     for (int k = 0; k < 2; ++k) {
 #pragma unroll
         for (int sg = 1; sg >= -1; sg -= 2) {
-            const int v = tap(P, x + sg * c_dir[dir][k][1], y + sg * c_dir[dir][k][0]);
+            const int v = tapf(sg * c_dir[dir][k][1], sg * c_dir[dir][k][0]);
             if (v != kVeryLarge) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
@@ -151,7 +152,7 @@ __device__ void px12(const PlaneV& P, int x, int y, int dir, int var, int dampin
                 const int sd = (d0 + (o ? 6 : 2)) & 7;
 #pragma unroll
                 for (int sg = 1; sg >= -1; sg -= 2) {
-                    const int v = tap(P, x + sg * c_dir[sd][k][1], y + sg * c_dir[sd][k][0]);
+                    const int v = tapf(sg * c_dir[sd][k][1], sg * c_dir[sd][k][0]);
                     if (v == kVeryLarge) continue;
 #pragma unroll
                     for (int q = 0; q < 2; ++q) (pass ? ssum0 : ssum)[q] += (k ? 1 : 2) * constrain(v - c, se[q], ssh[q]);
@@ -322,6 +323,7 @@ struct TabArgs {
 // (plane, pair) measures and adds to the unit's table.
 __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
     __shared__ int s_err[3][12][64];
+    __shared__ int s_tile[3][12 * 12];  // each plane's block with a 2-sample border
     const int lane = threadIdx.x;
     const int nb = a.nbx * a.nby;
     const int cbw = 8 >> a.sub_x, cbh = 8 >> a.sub_y, cn = cbw * cbh;
@@ -329,12 +331,20 @@ __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
         if (a.skip8[b]) continue;
         const int bx = b % a.nbx, by = b / a.nbx, x = bx * 8, y = by * 8;
         const int u = (by / 8) * a.uc + bx / 8, dir = a.dirs[b];
+        __syncthreads();  // the previous block's tiles are read
+        for (int pl = 0; pl < (a.mono ? 1 : 3); ++pl) {
+            const int tw = (pl ? cbw : 8) + 4, th = (pl ? cbh : 8) + 4;
+            const int x0 = (pl ? x >> a.sub_x : x) - 2, y0 = (pl ? y >> a.sub_y : y) - 2;
+            for (int i = lane; i < tw * th; i += 64) s_tile[pl][i] = tap(a.rec[pl], x0 + i % tw, y0 + i / tw);
+        }
+        __syncthreads();
         int o[12];
         if (a.scale <= 0.0f) {  // plain SSE: integer sums, reduced across the wave in any order
             unsigned sq[12], csq[12];
             {
                 const int px = x + (lane & 7), py = y + (lane >> 3);
-                px12(a.rec[0], px, py, dir, a.vars[b], a.damping, a.bd, o);
+                const int* T0 = s_tile[0] + ((lane >> 3) + 2) * 12 + (lane & 7) + 2;
+                px12([&](int dx, int dy) { return T0[dy * 12 + dx]; }, dir, a.vars[b], a.damping, a.bd, o);
                 const int s = a.src[0].p[(size_t)py * a.src[0].stride + px];
 #pragma unroll
                 for (int c = 0; c < 12; ++c) {
@@ -347,7 +357,9 @@ __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
                 for (int pl = 1; pl < 3; ++pl)
                     if (lane < cn) {
                         const int px = (x >> a.sub_x) + lane % cbw, py = (y >> a.sub_y) + lane / cbw;
-                        px12(a.rec[pl], px, py, dir, -1, a.damping - 1, a.bd, o);
+                        const int tw = cbw + 4;
+                        const int* Tc = s_tile[pl] + (lane / cbw + 2) * tw + lane % cbw + 2;
+                        px12([&](int dx, int dy) { return Tc[dy * tw + dx]; }, dir, -1, a.damping - 1, a.bd, o);
                         const int s = a.src[pl].p[(size_t)py * a.src[pl].stride + px];
 #pragma unroll
                         for (int c = 0; c < 12; ++c) csq[c] += (unsigned)((o[c] - s) * (o[c] - s));
@@ -368,7 +380,8 @@ __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
         }
         {
             const int px = x + (lane & 7), py = y + (lane >> 3);
-            px12(a.rec[0], px, py, dir, a.vars[b], a.damping, a.bd, o);
+            const int* T0 = s_tile[0] + ((lane >> 3) + 2) * 12 + (lane & 7) + 2;
+            px12([&](int dx, int dy) { return T0[dy * 12 + dx]; }, dir, a.vars[b], a.damping, a.bd, o);
             const int s = a.src[0].p[(size_t)py * a.src[0].stride + px];
 #pragma unroll
             for (int c = 0; c < 12; ++c) s_err[0][c][lane] = o[c] - s;
@@ -377,7 +390,9 @@ __global__ void __launch_bounds__(64) cdef_tables_kernel(TabArgs a) {
             for (int pl = 1; pl < 3; ++pl)
                 if (lane < cn) {
                     const int px = (x >> a.sub_x) + lane % cbw, py = (y >> a.sub_y) + lane / cbw;
-                    px12(a.rec[pl], px, py, dir, -1, a.damping - 1, a.bd, o);
+                    const int tw = cbw + 4;
+                    const int* Tc = s_tile[pl] + (lane / cbw + 2) * tw + lane % cbw + 2;
+                    px12([&](int dx, int dy) { return Tc[dy * tw + dx]; }, dir, -1, a.damping - 1, a.bd, o);
                     const int s = a.src[pl].p[(size_t)py * a.src[pl].stride + px];
 #pragma unroll
                     for (int c = 0; c < 12; ++c) s_err[pl][c][lane] = o[c] - s;

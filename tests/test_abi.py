@@ -148,3 +148,13 @@ def test_deblock_levels_and_layouts():
     assert R.MtEncodeOut.cdef_y.offset == 168 and R.MtEncodeOut.cdef.offset == 192
     from maroontree_amd import av1, cdef
     assert ctypes.sizeof(av1.MtAv1Frame) == 120 and ctypes.sizeof(cdef.MtCdefParams) == 48
+
+
+def test_modules_import_in_any_order():
+    """Each Python module imports first in a fresh interpreter (no import cycle)."""
+    import subprocess
+    import sys
+    for m in ("cdef", "rdo", "av1", "deblock", "prune"):
+        r = subprocess.run([sys.executable, "-c", f"import maroontree_amd.{m}"], capture_output=True, text=True,
+                           cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        assert r.returncode == 0, (m, r.stderr[-500:])
